@@ -1,0 +1,193 @@
+"""FDTD3D throughput on one MI355X: Gcell-updates/s and HBM-roofline fraction.
+
+``python bench.py [--steps K] [--warmup W] [--workload c3_512|c2_128|c1_64]``
+
+A step is one full Yee time step (H half-step, then E half-step with CPML
+psi, conductivity and the three dipoles) over the whole synthetic grid; the
+default workload is the 512^3 Gaussian-random-field grid with a 16-cell CPML
+(the roofline size).  Fields, coefficients and psi are resident in HBM before
+the timed region.  The K timed steps run back to back on one stream; HIP
+events around every H and E launch give each kernel's average duration.
+Prints one JSON line.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def build_workload(name: str, device):
+    from jaxphys_amd import workloads as W
+    if name == "c3_512":
+        g, steps = W.c3_grf(512, device=device)
+    elif name == "c3_256":
+        g, steps = W.c3_grf(256, device=device)
+    elif name == "c2_128":
+        g, steps = W.c2_boxes(128)
+    elif name == "c1_64":
+        g, steps = W.c1_sphere(64)
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    return g, steps
+
+
+def cpu_baseline(name: str, budget_s: float):
+    """The float32 C/OpenMP restatement of the same scheme (oracle), run on
+    this host's cores on a bounded number of steps of the same workload."""
+    from oracle import c_oracle
+    return c_oracle.time_workload(name, budget_s)
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="c3_512")
+    ap.add_argument("--cpu-budget", type=float, default=20.0,
+                    help="seconds of CPU work for the cpu_baseline sample (0 = skip)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run only a plain copy of the field bytes (counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
+              "(the z-slab decomposition is described in DESIGN.md)", file=sys.stderr)
+        raise SystemExit(2)
+
+    import numpy as np
+    import torch
+
+    from jaxphys_amd import _native
+    from jaxphys_amd import workloads as W
+
+    dev = _native.require_device()
+    torch.cuda.set_device(dev)
+
+    if args.copy_probe:
+        n = 512**3
+        a = torch.randn(n, device=dev)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            _native.copy_probe(a, b)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        reps = 20
+        e0.record()
+        for _ in range(reps):
+            _native.copy_probe(a, b)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        print(json.dumps({"copy_probe_bytes": 8 * n, "ms": ms, "GBps": 8 * n / ms / 1e6}))
+        return
+
+    t_setup = time.time()
+    grid, default_steps = build_workload(args.workload, dev)
+    K = args.steps if args.steps is not None else default_steps
+    Wm = args.warmup
+    solver = grid.build_solver(W.dt_for(1.0), K + Wm)
+    torch.cuda.synchronize()
+    setup_s = time.time() - t_setup
+    nx, ny, nz = grid.size
+    cells = nx * ny * nz
+    lossy = solver.ca is not None
+    npml = solver.pml_layers if solver.cpml is not None else 0
+
+    # warm-up (untimed)
+    solver.run(0, Wm)
+    torch.cuda.synchronize()
+
+    stream = torch.cuda.current_stream()
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(K)]
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record(stream)
+    for q in range(K):
+        ev[q][0].record(stream)
+        solver.step_h(stream)
+        ev[q][1].record(stream)
+        solver.step_e(Wm + q, stream)
+        ev[q][2].record(stream)
+    stop.record(stream)
+    torch.cuda.synchronize()
+    total_ms = start.elapsed_time(stop)
+    h_ms = sum(ev[q][0].elapsed_time(ev[q][1]) for q in range(K)) / K
+    e_ms = sum(ev[q][1].elapsed_time(ev[q][2]) for q in range(K)) / K
+    ms_step = total_ms / K
+    gcells = cells * K / (total_ms / 1e3) / 1e9
+
+    # algorithmic bytes per launch
+    psi_half = W.psi_bytes_per_step(grid.size, npml) // 2 if npml else 0
+    h_bytes = 36 * cells + psi_half
+    e_bytes = (44 if lossy else 40) * cells + psi_half
+    dom_name, dom_bytes, dom_ms = ("yee_e_kernel", e_bytes, e_ms) if e_ms >= h_ms else \
+        ("yee_h_kernel", h_bytes, h_ms)
+    achieved = dom_bytes / (dom_ms / 1e3) / 1e9
+    step_bytes = h_bytes + e_bytes
+    finite = bool(all(torch.isfinite(f).all().item() for f in solver.fields))
+
+    cpu = None
+    if args.cpu_budget > 0:
+        try:
+            cpu = cpu_baseline(args.workload, args.cpu_budget)
+        except Exception as exc:  # reported, never silently replaced
+            cpu = {"value": None, "error": f"{type(exc).__name__}: {exc}"}
+
+    out = {
+        "metric": "FDTD3D Gcell-upd/s & %HBM roofline",
+        "value": round(gcells, 3),
+        "unit": "Gcell-upd/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": Wm,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": args.workload,
+            "grid": [nx, ny, nz],
+            "boundary": solver.boundary_name,
+            "npml": npml,
+            "lossy": lossy,
+            "sources": solver.n_sources,
+            "courant": W.COURANT,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom_name,
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "bytes_per_launch": dom_bytes,
+            "ms_per_launch": round(dom_ms, 4),
+        },
+        "step": {
+            "h_ms": round(h_ms, 4), "e_ms": round(e_ms, 4),
+            "bytes": step_bytes,
+            "GBps": round(step_bytes / (ms_step / 1e3) / 1e9, 1),
+            "frac": round(step_bytes / (ms_step / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+        },
+        "cpu_baseline": cpu,
+        "fields_finite": finite,
+        "setup_s": round(setup_s, 2),
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,132 @@
+/*
+ * jaxphys_amd -- C ABI of the MI355X (gfx950) 3D Yee FDTD library
+ * (libyee_hip.so).
+ *
+ * The reference (sushaan-k/jaxphys, package neurosim) has no FFI: its hot
+ * path is the Python closure `step` inside `EMGrid3D.simulate`
+ * (src/neurosim/em/fdtd3d.py:280-379) driven by `jax.lax.scan`
+ * (fdtd3d.py:393).  This header is the seam that replaces that scan: the
+ * Python `EMGrid3D` (same construction/run API as fdtd3d.py:71-417) builds
+ * the coefficient arrays once and calls these entry points through ctypes.
+ *
+ * Storage: every per-cell array is float32, x fastest,
+ *     idx(i, j, k) = (k * ny + j) * nx + i,
+ * where (i, j, k) are the reference's (x, y, z) indices F[i, j, k]
+ * (fdtd3d.py:212-217, 240).  All pointers are device pointers owned by the
+ * caller (torch tensors); the library borrows them for the duration of a
+ * call.  All launches go on the caller's HIP stream; no entry point blocks
+ * the host except yee_energy_fp64 (which returns host scalars).
+ *
+ * Return codes: 0 ok, YEE_EINVAL bad argument, YEE_EHIP HIP error,
+ * YEE_EUNSUPPORTED unsupported configuration; yee_last_error() gives a
+ * thread-local message.
+ */
+#ifndef JAXPHYS_AMD_YEE_H
+#define JAXPHYS_AMD_YEE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define YEE_OK 0
+#define YEE_EINVAL (-1)
+#define YEE_EHIP (-2)
+#define YEE_EUNSUPPORTED (-3)
+
+/* Boundary treatment (fdtd3d.py:247-366).
+ *  WALLS    : the reference's non-periodic sliced update with no damping
+ *             (any boundary string other than "absorbing"/"periodic").
+ *  SPONGE   : "absorbing": WALLS + E *= exp(-sigma_avg*dt/eps0) on every
+ *             cell, sigma_avg from three 1-D cubic profiles (fdtd3d.py:139-167,
+ *             221, 361-366).
+ *  PERIODIC : "periodic": rolled curls on every cell, no damping
+ *             (fdtd3d.py:247-301).
+ *  CPML     : extension: WALLS + convolutional PML psi variables stored on
+ *             the 2*npml-cell slabs of each axis only.                      */
+#define YEE_BOUNDARY_WALLS 0
+#define YEE_BOUNDARY_SPONGE 1
+#define YEE_BOUNDARY_PERIODIC 2
+#define YEE_BOUNDARY_CPML 3
+
+/* Field component order used by every entry point. */
+#define YEE_EX 0
+#define YEE_EY 1
+#define YEE_EZ 2
+#define YEE_HX 3
+#define YEE_HY 4
+#define YEE_HZ 5
+
+typedef struct YeeGridDesc {
+  int32_t nx, ny, nz;   /* cells per axis, each >= 2                        */
+  int32_t boundary;     /* YEE_BOUNDARY_*                                   */
+  float ch;             /* dt / (mu0 dx)         fdtd3d.py:223               */
+  const float* cb;      /* per cell: ce / eps_r  fdtd3d.py:224, 336-354      */
+  const float* ca;      /* per cell E self-coefficient, NULL = 1 (lossless)  */
+  /* SPONGE: sigma_x[nx], sigma_y[ny], sigma_z[nz] concatenated, plus the
+   * float32-rounded dt and eps0 of fdtd3d.py:363.                         */
+  const float* sponge_sigma;
+  float sponge_dt;
+  float sponge_eps0;
+  /* CPML: slab thickness (same on all six faces, 2*npml <= n per axis) and
+   * the coefficient tables: for axis a in x,y,z, for node kind E (integer
+   * position) then H (half-integer position): b[n_a], c[n_a], 1/kappa[n_a];
+   * i.e. 6*n_a floats per axis, axes concatenated x, y, z.                 */
+  int32_t npml;
+  const float* cpml;
+} YeeGridDesc;
+
+typedef struct YeePlan YeePlan;
+
+/* Number of float32 values of the CPML psi buffer for this grid (0 unless
+ * CPML): 12 arrays, each 2*npml cells thick along its derivative axis.  */
+int64_t yee_psi_floats(const YeeGridDesc* desc);
+
+/* Validate `desc` and build a plan (host object; holds no device memory
+ * except the source table uploaded by yee_set_sources when it has more
+ * than YEE_FUSED_SOURCES sources).  Replaces the prologue of
+ * fdtd3d.py:185-231 below the Python layer.                              */
+int yee_plan_create(const YeeGridDesc* desc, YeePlan** out);
+void yee_plan_destroy(YeePlan* plan);
+
+#define YEE_FUSED_SOURCES 8
+
+/* Point sources (fdtd3d.py:233-245, 368-369): source s adds table[n*nsrc+s]
+ * to component comp[s] (YEE_EX/EY/EZ) at cell pos[3s..3s+2] = (i,j,k) after
+ * the E update (and after sponge damping) of step n, in insertion order.
+ * comp, pos and table are HOST arrays; the plan copies them.             */
+int yee_set_sources(YeePlan* plan, int32_t nsrc, const int32_t* comp,
+                    const int32_t* pos, const float* table, int32_t nsteps);
+
+/* Run steps n0 .. n0+nsteps-1 (fdtd3d.py:280-393: H half-step with the old
+ * E, then E half-step with the new H, damping, sources).  fields[6] in the
+ * YEE_EX..YEE_HZ order; psi may be NULL unless CPML.                     */
+int yee_run(YeePlan* plan, float* const* fields, float* psi, int32_t n0,
+            int32_t nsteps, void* stream);
+
+/* The two half-steps of step n separately (same semantics as yee_run). */
+int yee_step_h(YeePlan* plan, float* const* fields, float* psi, void* stream);
+int yee_step_e(YeePlan* plan, float* const* fields, float* psi, int32_t n,
+               void* stream);
+
+/* Float64 accumulators for the checks (no reference counterpart):
+ * work[0] = sum_cells eps_r |E|^2 (eps_r from an explicit per-cell array,
+ * NULL = 1), work[1] = sum_cells |H|^2.  `work` is a caller-owned device
+ * buffer of YEE_ENERGY_WORK doubles; the summation order is fixed, so the
+ * result is deterministic.  Asynchronous on the stream.                  */
+#define YEE_ENERGY_WORK (2 + 2 * 1024)
+int yee_energy_fp64(int32_t nx, int32_t ny, int32_t nz, float* const* fields,
+                    const float* eps_r, double* work, void* stream);
+
+/* Plain device copy of n floats, used to calibrate HBM byte counters
+ * (no reference counterpart).                                            */
+int yee_copy_probe(const float* src, float* dst, int64_t n, void* stream);
+
+const char* yee_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* JAXPHYS_AMD_YEE_H */

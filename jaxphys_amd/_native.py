@@ -1,0 +1,178 @@
+"""ctypes binding of libyee_hip.so (C ABI in include/yee.h).
+
+The library is loaded after ``import torch`` so that it binds to the HIP
+runtime torch already loaded (both name it libamdhip64.so.7): device
+pointers of torch tensors and torch's streams are then valid inside it.
+There is no fallback: if the library is missing or no HIP device is present
+every call raises ``NativeLibraryError``.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+import torch  # noqa: F401  (must precede the library load, see above)
+
+from jaxphys_amd.exceptions import ConfigurationError, NativeLibraryError, SimulationError
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+YEE_OK, YEE_EINVAL, YEE_EHIP, YEE_EUNSUPPORTED = 0, -1, -2, -3
+BOUNDARY_WALLS, BOUNDARY_SPONGE, BOUNDARY_PERIODIC, BOUNDARY_CPML = 0, 1, 2, 3
+FUSED_SOURCES = 8
+ENERGY_WORK = 2 + 2 * 1024
+
+_c_float_p = ctypes.c_void_p
+
+
+class YeeGridDesc(ctypes.Structure):
+    """Mirror of ``YeeGridDesc`` in include/yee.h."""
+
+    _fields_ = [
+        ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+        ("boundary", ctypes.c_int32),
+        ("ch", ctypes.c_float),
+        ("cb", ctypes.c_void_p), ("ca", ctypes.c_void_p),
+        ("sponge_sigma", ctypes.c_void_p),
+        ("sponge_dt", ctypes.c_float), ("sponge_eps0", ctypes.c_float),
+        ("npml", ctypes.c_int32),
+        ("cpml", ctypes.c_void_p),
+    ]
+
+
+_LIB = None
+
+EXPORTS = {
+    "yee_psi_floats": (ctypes.c_int64, [ctypes.POINTER(YeeGridDesc)]),
+    "yee_plan_create": (ctypes.c_int, [ctypes.POINTER(YeeGridDesc), ctypes.POINTER(ctypes.c_void_p)]),
+    "yee_plan_destroy": (None, [ctypes.c_void_p]),
+    "yee_set_sources": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
+                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32]),
+    "yee_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                               ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p]),
+    "yee_step_h": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "yee_step_e": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                  ctypes.c_int32, ctypes.c_void_p]),
+    "yee_energy_fp64": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.c_void_p]),
+    "yee_copy_probe": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                      ctypes.c_void_p]),
+    "yee_last_error": (ctypes.c_char_p, []),
+}
+
+
+def library_path(name: str = "libyee_hip.so") -> str:
+    return os.path.join(HERE, name)
+
+
+def lib():
+    """Load (once) and return the HIP library; raise if it is not built."""
+    global _LIB
+    if _LIB is None:
+        path = library_path()
+        if not os.path.exists(path):
+            raise NativeLibraryError(
+                f"{path} is not built; run `python -m jaxphys_amd.build` "
+                "(hipcc, gfx950) first")
+        handle = ctypes.CDLL(path)
+        for name, (res, args) in EXPORTS.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = handle
+    return _LIB
+
+
+def require_device() -> torch.device:
+    if not torch.cuda.is_available():
+        raise NativeLibraryError(
+            "jaxphys_amd runs its FDTD step on an AMD GPU (HIP); no device is visible")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def check(rc: int, what: str) -> None:
+    if rc == YEE_OK:
+        return
+    msg = lib().yee_last_error().decode(errors="replace")
+    if rc in (YEE_EINVAL, YEE_EUNSUPPORTED):
+        raise ConfigurationError(f"{what}: {msg}")
+    raise SimulationError(f"{what}: {msg}")
+
+
+def ptr(t) -> int:
+    return 0 if t is None else int(t.data_ptr())
+
+
+def field_array(fields) -> ctypes.Array:
+    return (ctypes.c_void_p * 6)(*[ptr(f) for f in fields])
+
+
+def stream_handle(stream=None) -> int:
+    s = torch.cuda.current_stream() if stream is None else stream
+    return int(s.cuda_stream)
+
+
+class Plan:
+    """Owns a ``YeePlan*``; the tensors named in the descriptor must outlive it."""
+
+    def __init__(self, desc: YeeGridDesc, keep=()):
+        self.desc = desc
+        self._keep = list(keep)  # tensors whose pointers the descriptor holds
+        h = ctypes.c_void_p()
+        check(lib().yee_plan_create(ctypes.byref(desc), ctypes.byref(h)), "yee_plan_create")
+        self.handle = h
+
+    def psi_floats(self) -> int:
+        return int(lib().yee_psi_floats(ctypes.byref(self.desc)))
+
+    def set_sources(self, comp, pos, table, nsteps: int) -> None:
+        import numpy as np
+        comp = np.ascontiguousarray(comp, dtype=np.int32)
+        pos = np.ascontiguousarray(pos, dtype=np.int32).reshape(-1)
+        table = np.ascontiguousarray(table, dtype=np.float32).reshape(-1)
+        nsrc = int(comp.shape[0])
+        check(lib().yee_set_sources(self.handle, nsrc, comp.ctypes.data, pos.ctypes.data,
+                                    table.ctypes.data if table.size else None, int(nsteps)),
+              "yee_set_sources")
+
+    def run(self, fields, psi, n0: int, nsteps: int, stream=None) -> None:
+        check(lib().yee_run(self.handle, field_array(fields), ptr(psi) or None, int(n0),
+                            int(nsteps), stream_handle(stream)), "yee_run")
+
+    def step_h(self, fields, psi, stream=None) -> None:
+        check(lib().yee_step_h(self.handle, field_array(fields), ptr(psi) or None,
+                               stream_handle(stream)), "yee_step_h")
+
+    def step_e(self, fields, psi, n: int, stream=None) -> None:
+        check(lib().yee_step_e(self.handle, field_array(fields), ptr(psi) or None, int(n),
+                               stream_handle(stream)), "yee_step_e")
+
+    def close(self) -> None:
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            lib().yee_plan_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def energy_fp64(shape_xyz, fields, eps_r=None, work=None, stream=None):
+    """float64 (sum eps_r |E|^2, sum |H|^2) over the grid; returns a device
+    tensor of 2 doubles (asynchronous)."""
+    nx, ny, nz = shape_xyz
+    dev = fields[0].device
+    if work is None:
+        work = torch.empty(ENERGY_WORK, dtype=torch.float64, device=dev)
+    check(lib().yee_energy_fp64(nx, ny, nz, field_array(fields), ptr(eps_r) or None,
+                                ptr(work), stream_handle(stream)), "yee_energy_fp64")
+    return work[:2]
+
+
+def copy_probe(src: torch.Tensor, dst: torch.Tensor, stream=None) -> None:
+    check(lib().yee_copy_probe(ptr(src), ptr(dst), int(src.numel()), stream_handle(stream)),
+          "yee_copy_probe")

@@ -1,0 +1,653 @@
+// jaxphys_amd: MI355X (gfx950) kernels of the 3D Yee FDTD step and the
+// C ABI declared in include/yee.h.
+//
+// Reference path: src/neurosim/em/fdtd3d.py:280-379 (`step`), scanned by
+// jax.lax.scan at fdtd3d.py:393.  One time step is two passes over HBM:
+//   * the H half-step (fdtd3d.py:303-330; periodic :290-295) reads E and H,
+//     writes H;
+//   * the E half-step (fdtd3d.py:332-359; periodic :297-301) reads H, E and
+//     the per-cell coefficient(s), writes E, and applies in the same pass the
+//     sponge damping (fdtd3d.py:361-366) or the CPML psi recursion, then the
+//     point sources (fdtd3d.py:368-369, insertion order).
+//
+// Mapping to CDNA4: a 256-thread block is one 64-wide x row segment (one
+// wavefront) by 4 y rows; each thread owns a column (i, j) and marches a
+// chunk of z planes, carrying the z neighbour of the previous plane in
+// registers, so every field is streamed from HBM once per pass with
+// coalesced x-fastest loads.  x and y neighbours are re-read from L1/L2.
+// Arithmetic is float32 with contraction disabled (-ffp-contract=off) so that
+// the operation order, and therefore the rounding, is that of the reference's
+// float32 expressions.
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/yee.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define YEE_HIP_CHECK(expr)                                                   \
+  do {                                                                        \
+    hipError_t e_ = (expr);                                                   \
+    if (e_ != hipSuccess)                                                     \
+      return fail(YEE_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+constexpr int kBX = 64;   // x extent of a block: one wavefront
+constexpr int kBY = 4;    // y rows per block
+constexpr int kThreads = kBX * kBY;
+
+enum Mode { kWalls = 0, kSponge = 1, kPeriodic = 2, kCpml = 3 };
+
+// psi array slots (CPML).  y-slab arrays are (nz, 2p, nx), z-slab arrays
+// (2p, ny, nx), x-slab arrays (nz, ny, 2p), all x fastest.
+enum Psi {
+  kExy = 0, kEzy, kHxy, kHzy,   // derivative along y
+  kExz, kEyz, kHxz, kHyz,       // along z
+  kEyx, kEzx, kHyx, kHzx,       // along x
+  kNumPsi
+};
+
+struct GridArgs {
+  int nx, ny, nz, p;
+  float ch;
+  float* f[6];
+  const float* cb;
+  const float* ca;
+  const float* ssx;
+  const float* ssy;
+  const float* ssz;
+  float sdt, seps0;
+  // CPML tables per axis: bE cE kE bH cH kH, each n_a long
+  const float* cpx;
+  const float* cpy;
+  const float* cpz;
+  float* psi[kNumPsi];
+};
+
+struct SrcArgs {
+  int nsrc;  // sources handled inside the E kernel (<= YEE_FUSED_SOURCES)
+  int comp[YEE_FUSED_SOURCES];
+  long long idx[YEE_FUSED_SOURCES];
+  float val[YEE_FUSED_SOURCES];
+};
+
+__device__ __forceinline__ int slab_index(int a, int n, int p) {
+  return a < p ? a : a - (n - 2 * p);
+}
+
+// One psi recursion + curl term:  psi = b psi + c d;  return d/kappa + psi.
+__device__ __forceinline__ float cpml_term(float* __restrict__ ps, const float* __restrict__ tab,
+                                           int n, int a, int kind, float d) {
+  const float b = tab[(3 * kind + 0) * n + a];
+  const float c = tab[(3 * kind + 1) * n + a];
+  const float kinv = tab[(3 * kind + 2) * n + a];
+  const float v = b * (*ps) + c * d;
+  *ps = v;
+  return d * kinv + v;
+}
+
+// ---------------------------------------------------------------------------
+// H half-step: H -= ch * curl(E) (forward differences).
+// ---------------------------------------------------------------------------
+template <int MODE>
+__global__ __launch_bounds__(kThreads) void yee_h_kernel(GridArgs g, int kz) {
+  const int i = blockIdx.x * kBX + (threadIdx.x % kBX);
+  const int j = blockIdx.y * kBY + (threadIdx.x / kBX);
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  if (i >= nx || j >= ny) return;
+  const int k0 = blockIdx.z * kz;
+  const int k1 = min(k0 + kz, nz);
+  const long long P = (long long)nx * ny;
+  constexpr bool per = MODE == kPeriodic;
+  const bool xin = per || i < nx - 1;
+  const bool yin = per || j < ny - 1;
+  const long long di = (per && i == nx - 1) ? -(long long)(nx - 1) : 1;
+  const long long dj = (per && j == ny - 1) ? -(long long)(ny - 1) * nx : nx;
+  const float* __restrict__ ex = g.f[0];
+  const float* __restrict__ ey = g.f[1];
+  const float* __restrict__ ez = g.f[2];
+  float* __restrict__ hx = g.f[3];
+  float* __restrict__ hy = g.f[4];
+  float* __restrict__ hz = g.f[5];
+  const float ch = g.ch;
+  long long idx = (long long)k0 * P + (long long)j * nx + i;
+  float ex0 = ex[idx], ey0 = ey[idx];
+  const int p = g.p;
+  const bool xs = MODE == kCpml && (i < p || i >= nx - p);
+  const bool ys = MODE == kCpml && (j < p || j >= ny - p);
+  const int sx = xs ? slab_index(i, nx, p) : 0;
+  const int sy = ys ? slab_index(j, ny, p) : 0;
+  for (int k = k0; k < k1; ++k, idx += P) {
+    const bool zin = per || k < nz - 1;
+    const long long dk = (per && k == nz - 1) ? -(long long)(nz - 1) * P : P;
+    const float ez0 = ez[idx];
+    float ex1 = 0.f, ey1 = 0.f, ezj = 0.f, exj = 0.f, ezi = 0.f, eyi = 0.f;
+    if (zin) { ex1 = ex[idx + dk]; ey1 = ey[idx + dk]; }
+    if (yin) { ezj = ez[idx + dj]; exj = ex[idx + dj]; }
+    if (xin) { ezi = ez[idx + di]; eyi = ey[idx + di]; }
+    const bool zs = MODE == kCpml && (k < p || k >= nz - p);
+    const int sz = zs ? slab_index(k, nz, p) : 0;
+    if (yin && zin) {  // fdtd3d.py:305-312
+      float a = ezj - ez0, b = ey1 - ey0;
+      if (MODE == kCpml) {
+        if (ys) a = cpml_term(g.psi[kHxy] + ((long long)(k * 2 * p + sy) * nx + i), g.cpy, ny, j, 1, a);
+        if (zs) b = cpml_term(g.psi[kHxz] + ((long long)(sz * ny + j) * nx + i), g.cpz, nz, k, 1, b);
+      }
+      hx[idx] = hx[idx] - ch * (a - b);
+    }
+    if (xin && zin) {  // fdtd3d.py:314-321
+      float a = ex1 - ex0, b = ezi - ez0;
+      if (MODE == kCpml) {
+        if (zs) a = cpml_term(g.psi[kHyz] + ((long long)(sz * ny + j) * nx + i), g.cpz, nz, k, 1, a);
+        if (xs) b = cpml_term(g.psi[kHyx] + ((long long)k * ny + j) * 2 * p + sx, g.cpx, nx, i, 1, b);
+      }
+      hy[idx] = hy[idx] - ch * (a - b);
+    }
+    if (xin && yin) {  // fdtd3d.py:323-330
+      float a = eyi - ey0, b = exj - ex0;
+      if (MODE == kCpml) {
+        if (xs) a = cpml_term(g.psi[kHzx] + ((long long)k * ny + j) * 2 * p + sx, g.cpx, nx, i, 1, a);
+        if (ys) b = cpml_term(g.psi[kHzy] + ((long long)(k * 2 * p + sy) * nx + i), g.cpy, ny, j, 1, b);
+      }
+      hz[idx] = hz[idx] - ch * (a - b);
+    }
+    ex0 = ex1;
+    ey0 = ey1;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// E half-step: E = [Ca] E + Cb * curl(H) (backward differences), then sponge
+// damping or nothing, then the fused point sources.
+// ---------------------------------------------------------------------------
+template <int MODE, bool LOSSY>
+__global__ __launch_bounds__(kThreads) void yee_e_kernel(GridArgs g, SrcArgs s, int kz) {
+  const int i = blockIdx.x * kBX + (threadIdx.x % kBX);
+  const int j = blockIdx.y * kBY + (threadIdx.x / kBX);
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  if (i >= nx || j >= ny) return;
+  const int k0 = blockIdx.z * kz;
+  const int k1 = min(k0 + kz, nz);
+  const long long P = (long long)nx * ny;
+  constexpr bool per = MODE == kPeriodic;
+  const bool xin = per || i > 0;
+  const bool yin = per || j > 0;
+  const long long di = (per && i == 0) ? (long long)(nx - 1) : -1;
+  const long long dj = (per && j == 0) ? (long long)(ny - 1) * nx : -(long long)nx;
+  float* __restrict__ ex = g.f[0];
+  float* __restrict__ ey = g.f[1];
+  float* __restrict__ ez = g.f[2];
+  const float* __restrict__ hx = g.f[3];
+  const float* __restrict__ hy = g.f[4];
+  const float* __restrict__ hz = g.f[5];
+  long long idx = (long long)k0 * P + (long long)j * nx + i;
+  float hxm = 0.f, hym = 0.f;  // H at plane k-1
+  if (k0 > 0) {
+    hxm = hx[idx - P];
+    hym = hy[idx - P];
+  } else if (per) {
+    hxm = hx[idx + (long long)(nz - 1) * P];
+    hym = hy[idx + (long long)(nz - 1) * P];
+  }
+  const int p = g.p;
+  const bool xs = MODE == kCpml && (i < p || i >= nx - p);
+  const bool ys = MODE == kCpml && (j < p || j >= ny - p);
+  const int sx = xs ? slab_index(i, nx, p) : 0;
+  const int sy = ys ? slab_index(j, ny, p) : 0;
+  float sxy = 0.f;
+  if (MODE == kSponge) sxy = g.ssx[i] + g.ssy[j];
+  for (int k = k0; k < k1; ++k, idx += P) {
+    const bool zin = per || k > 0;
+    const float hx0 = hx[idx], hy0 = hy[idx], hz0 = hz[idx];
+    float hzj = 0.f, hxj = 0.f, hzi = 0.f, hyi = 0.f;
+    if (yin) { hzj = hz[idx + dj]; hxj = hx[idx + dj]; }
+    if (xin) { hzi = hz[idx + di]; hyi = hy[idx + di]; }
+    const float cb = g.cb[idx];
+    const float ca = LOSSY ? g.ca[idx] : 1.f;
+    float vx = ex[idx], vy = ey[idx], vz = ez[idx];
+    const bool zs = MODE == kCpml && (k < p || k >= nz - p);
+    const int sz = zs ? slab_index(k, nz, p) : 0;
+    if (yin && zin) {  // fdtd3d.py:334-341
+      float a = hz0 - hzj, b = hy0 - hym;
+      if (MODE == kCpml) {
+        if (ys) a = cpml_term(g.psi[kExy] + ((long long)(k * 2 * p + sy) * nx + i), g.cpy, ny, j, 0, a);
+        if (zs) b = cpml_term(g.psi[kExz] + ((long long)(sz * ny + j) * nx + i), g.cpz, nz, k, 0, b);
+      }
+      vx = LOSSY ? ca * vx + cb * (a - b) : vx + cb * (a - b);
+    }
+    if (xin && zin) {  // fdtd3d.py:343-350
+      float a = hx0 - hxm, b = hz0 - hzi;
+      if (MODE == kCpml) {
+        if (zs) a = cpml_term(g.psi[kEyz] + ((long long)(sz * ny + j) * nx + i), g.cpz, nz, k, 0, a);
+        if (xs) b = cpml_term(g.psi[kEyx] + ((long long)k * ny + j) * 2 * p + sx, g.cpx, nx, i, 0, b);
+      }
+      vy = LOSSY ? ca * vy + cb * (a - b) : vy + cb * (a - b);
+    }
+    if (xin && yin) {  // fdtd3d.py:352-359
+      float a = hy0 - hyi, b = hx0 - hxj;
+      if (MODE == kCpml) {
+        if (xs) a = cpml_term(g.psi[kEzx] + ((long long)k * ny + j) * 2 * p + sx, g.cpx, nx, i, 0, a);
+        if (ys) b = cpml_term(g.psi[kEzy] + ((long long)(k * 2 * p + sy) * nx + i), g.cpy, ny, j, 0, b);
+      }
+      vz = LOSSY ? ca * vz + cb * (a - b) : vz + cb * (a - b);
+    }
+    if (MODE == kSponge) {  // fdtd3d.py:361-366, sigma_avg of :221
+      const float sig = (sxy + g.ssz[k]) / 3.0f;
+      if (sig != 0.f) {  // exp(-0) == 1 exactly: the multiply is a no-op
+        const float d = expf(((-sig) * g.sdt) / g.seps0);
+        vx *= d;
+        vy *= d;
+        vz *= d;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < YEE_FUSED_SOURCES; ++q) {  // fdtd3d.py:236-244
+      if (q < s.nsrc && s.idx[q] == idx) {
+        if (s.comp[q] == 0) vx += s.val[q];
+        else if (s.comp[q] == 1) vy += s.val[q];
+        else vz += s.val[q];
+      }
+    }
+    ex[idx] = vx;
+    ey[idx] = vy;
+    ez[idx] = vz;
+    hxm = hx0;
+    hym = hy0;
+  }
+}
+
+// Sources beyond the fused ones: one thread, insertion order.
+__global__ void yee_inject_kernel(float* ex, float* ey, float* ez, const int* comp,
+                                  const long long* idx, const float* table, int nsrc,
+                                  int first, int n) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  for (int q = first; q < nsrc; ++q) {
+    const float v = table[(long long)n * nsrc + q];
+    float* f = comp[q] == 0 ? ex : (comp[q] == 1 ? ey : ez);
+    f[idx[q]] += v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// float64 energy sums (deterministic two-stage reduction)
+// ---------------------------------------------------------------------------
+constexpr int kEnergyBlocks = 1024;
+
+__device__ double block_sum(double v, double* sh) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+  if (lane == 0) sh[wave] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < (int)(blockDim.x / 64); ++w) r += sh[w];
+  __syncthreads();
+  return r;
+}
+
+__global__ __launch_bounds__(256) void energy_partial_kernel(GridArgs g, const float* eps,
+                                                            long long ncell, double* part) {
+  __shared__ double sh[4];
+  double we = 0.0, wh = 0.0;
+  for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < ncell;
+       c += (long long)gridDim.x * blockDim.x) {
+    const double e = (double)g.f[0][c] * g.f[0][c] + (double)g.f[1][c] * g.f[1][c] +
+                     (double)g.f[2][c] * g.f[2][c];
+    we += (eps ? (double)eps[c] : 1.0) * e;
+    wh += (double)g.f[3][c] * g.f[3][c] + (double)g.f[4][c] * g.f[4][c] +
+          (double)g.f[5][c] * g.f[5][c];
+  }
+  we = block_sum(we, sh);
+  wh = block_sum(wh, sh);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = we;
+    part[2 * blockIdx.x + 1] = wh;
+  }
+}
+
+__global__ __launch_bounds__(256) void energy_final_kernel(const double* part, int nb, double* out) {
+  __shared__ double sh[4];
+  double we = 0.0, wh = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    we += part[2 * b];
+    wh += part[2 * b + 1];
+  }
+  we = block_sum(we, sh);
+  wh = block_sum(wh, sh);
+  if (threadIdx.x == 0) {
+    out[0] = we;
+    out[1] = wh;
+  }
+}
+
+__global__ __launch_bounds__(256) void copy_probe_kernel(const float4* __restrict__ src,
+                                                         float4* __restrict__ dst, long long n4) {
+  for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n4;
+       c += (long long)gridDim.x * blockDim.x)
+    dst[c] = src[c];
+}
+
+__global__ void copy_tail_kernel(const float* src, float* dst, long long from, long long n) {
+  const long long c = from + (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < n) dst[c] = src[c];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// plan
+// ---------------------------------------------------------------------------
+struct YeePlan {
+  YeeGridDesc d;
+  int kz;
+  dim3 grid;
+  // sources
+  int nsrc = 0;
+  int nsteps_table = 0;
+  std::vector<int> comp;
+  std::vector<long long> idx;
+  std::vector<float> table;  // host copy, step-major
+  int* d_comp = nullptr;     // device copies for the unfused tail
+  long long* d_idx = nullptr;
+  float* d_table = nullptr;
+};
+
+namespace {
+
+void free_sources(YeePlan* p) {
+  if (p->d_comp) (void)hipFree(p->d_comp);
+  if (p->d_idx) (void)hipFree(p->d_idx);
+  if (p->d_table) (void)hipFree(p->d_table);
+  p->d_comp = nullptr;
+  p->d_idx = nullptr;
+  p->d_table = nullptr;
+}
+
+long long psi_slab_floats(const YeeGridDesc& d, int which /*0 y,1 z,2 x*/) {
+  const long long tp = 2LL * d.npml;
+  if (which == 0) return (long long)d.nz * tp * d.nx;
+  if (which == 1) return tp * d.ny * d.nx;
+  return (long long)d.nz * d.ny * tp;
+}
+
+int validate(const YeeGridDesc* d) {
+  if (!d) return fail(YEE_EINVAL, "null grid descriptor");
+  if (d->nx < 2 || d->ny < 2 || d->nz < 2)
+    return fail(YEE_EINVAL, "grid must be at least 2 cells per axis");
+  if (d->boundary < YEE_BOUNDARY_WALLS || d->boundary > YEE_BOUNDARY_CPML)
+    return fail(YEE_EINVAL, "unknown boundary code");
+  if (!d->cb) return fail(YEE_EINVAL, "cb (ce/eps_r) array is required");
+  if (d->boundary == YEE_BOUNDARY_SPONGE && !d->sponge_sigma)
+    return fail(YEE_EINVAL, "sponge boundary needs sponge_sigma");
+  if (d->boundary == YEE_BOUNDARY_CPML) {
+    if (d->npml < 1) return fail(YEE_EINVAL, "cpml needs npml >= 1");
+    if (2 * d->npml > d->nx || 2 * d->npml > d->ny || 2 * d->npml > d->nz)
+      return fail(YEE_EINVAL, "cpml slabs overlap: need 2*npml <= n on every axis");
+    if (!d->cpml) return fail(YEE_EINVAL, "cpml needs coefficient tables");
+  }
+  return YEE_OK;
+}
+
+GridArgs make_args(const YeePlan* p, float* const* fields, float* psi) {
+  const YeeGridDesc& d = p->d;
+  GridArgs g;
+  std::memset(&g, 0, sizeof(g));
+  g.nx = d.nx;
+  g.ny = d.ny;
+  g.nz = d.nz;
+  g.p = d.boundary == YEE_BOUNDARY_CPML ? d.npml : 0;
+  g.ch = d.ch;
+  for (int c = 0; c < 6; ++c) g.f[c] = fields[c];
+  g.cb = d.cb;
+  g.ca = d.ca;
+  if (d.boundary == YEE_BOUNDARY_SPONGE) {
+    g.ssx = d.sponge_sigma;
+    g.ssy = d.sponge_sigma + d.nx;
+    g.ssz = d.sponge_sigma + d.nx + d.ny;
+    g.sdt = d.sponge_dt;
+    g.seps0 = d.sponge_eps0;
+  }
+  if (d.boundary == YEE_BOUNDARY_CPML) {
+    g.cpx = d.cpml;
+    g.cpy = d.cpml + 6LL * d.nx;
+    g.cpz = d.cpml + 6LL * (d.nx + d.ny);
+    const long long ny_ = psi_slab_floats(d, 0), nz_ = psi_slab_floats(d, 1),
+                    nx_ = psi_slab_floats(d, 2);
+    float* q = psi;
+    for (int a = kExy; a <= kHzy; ++a) { g.psi[a] = q; q += ny_; }
+    for (int a = kExz; a <= kHyz; ++a) { g.psi[a] = q; q += nz_; }
+    for (int a = kEyx; a <= kHzx; ++a) { g.psi[a] = q; q += nx_; }
+  }
+  return g;
+}
+
+int launch_h(const YeePlan* p, const GridArgs& g, hipStream_t st) {
+  switch (p->d.boundary) {
+    case YEE_BOUNDARY_PERIODIC:
+      yee_h_kernel<kPeriodic><<<p->grid, kThreads, 0, st>>>(g, p->kz);
+      break;
+    case YEE_BOUNDARY_CPML:
+      yee_h_kernel<kCpml><<<p->grid, kThreads, 0, st>>>(g, p->kz);
+      break;
+    default:  // walls and sponge share the H update (sponge damps E only)
+      yee_h_kernel<kWalls><<<p->grid, kThreads, 0, st>>>(g, p->kz);
+      break;
+  }
+  YEE_HIP_CHECK(hipGetLastError());
+  return YEE_OK;
+}
+
+template <bool LOSSY>
+void launch_e_mode(const YeePlan* p, const GridArgs& g, const SrcArgs& s, hipStream_t st) {
+  switch (p->d.boundary) {
+    case YEE_BOUNDARY_PERIODIC:
+      yee_e_kernel<kPeriodic, LOSSY><<<p->grid, kThreads, 0, st>>>(g, s, p->kz);
+      break;
+    case YEE_BOUNDARY_CPML:
+      yee_e_kernel<kCpml, LOSSY><<<p->grid, kThreads, 0, st>>>(g, s, p->kz);
+      break;
+    case YEE_BOUNDARY_SPONGE:
+      yee_e_kernel<kSponge, LOSSY><<<p->grid, kThreads, 0, st>>>(g, s, p->kz);
+      break;
+    default:
+      yee_e_kernel<kWalls, LOSSY><<<p->grid, kThreads, 0, st>>>(g, s, p->kz);
+      break;
+  }
+}
+
+int launch_e(const YeePlan* p, const GridArgs& g, int n, hipStream_t st) {
+  SrcArgs s;
+  std::memset(&s, 0, sizeof(s));
+  if (p->nsrc > 0 && (n < 0 || n >= p->nsteps_table))
+    return fail(YEE_EINVAL, "step " + std::to_string(n) + " outside the source table (" +
+                                std::to_string(p->nsteps_table) + " steps)");
+  s.nsrc = p->nsrc < YEE_FUSED_SOURCES ? p->nsrc : YEE_FUSED_SOURCES;
+  for (int q = 0; q < s.nsrc; ++q) {
+    s.comp[q] = p->comp[q];
+    s.idx[q] = p->idx[q];
+    s.val[q] = p->table[(size_t)n * p->nsrc + q];
+  }
+  if (p->d.ca)
+    launch_e_mode<true>(p, g, s, st);
+  else
+    launch_e_mode<false>(p, g, s, st);
+  YEE_HIP_CHECK(hipGetLastError());
+  if (p->nsrc > YEE_FUSED_SOURCES) {
+    yee_inject_kernel<<<1, 64, 0, st>>>(g.f[0], g.f[1], g.f[2], p->d_comp, p->d_idx, p->d_table,
+                                        p->nsrc, YEE_FUSED_SOURCES, n);
+    YEE_HIP_CHECK(hipGetLastError());
+  }
+  return YEE_OK;
+}
+
+int check_fields(const YeePlan* p, float* const* fields, float* psi) {
+  if (!p) return fail(YEE_EINVAL, "null plan");
+  if (!fields) return fail(YEE_EINVAL, "null field array");
+  for (int c = 0; c < 6; ++c)
+    if (!fields[c]) return fail(YEE_EINVAL, "null field pointer " + std::to_string(c));
+  if (p->d.boundary == YEE_BOUNDARY_CPML && !psi)
+    return fail(YEE_EINVAL, "cpml needs the psi buffer");
+  return YEE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* yee_last_error(void) { return g_err.c_str(); }
+
+int64_t yee_psi_floats(const YeeGridDesc* d) {
+  if (!d || d->boundary != YEE_BOUNDARY_CPML) return 0;
+  return 4 * (psi_slab_floats(*d, 0) + psi_slab_floats(*d, 1) + psi_slab_floats(*d, 2));
+}
+
+int yee_plan_create(const YeeGridDesc* desc, YeePlan** out) {
+  if (!out) return fail(YEE_EINVAL, "null output pointer");
+  *out = nullptr;
+  int rc = validate(desc);
+  if (rc) return rc;
+  YeePlan* p = new (std::nothrow) YeePlan();
+  if (!p) return fail(YEE_EINVAL, "out of host memory");
+  p->d = *desc;
+  const int gx = (desc->nx + kBX - 1) / kBX;
+  const int gy = (desc->ny + kBY - 1) / kBY;
+  // z chunking: enough blocks to fill 256 CUs several times over, while
+  // keeping each column march long enough to amortise the carried plane.
+  int kz = 0;
+  if (const char* e = std::getenv("JAXPHYS_AMD_KZ")) kz = std::atoi(e);
+  if (kz <= 0) {
+    const long long want = 8192;
+    long long chunks = (want + (long long)gx * gy - 1) / ((long long)gx * gy);
+    if (chunks < 1) chunks = 1;
+    if (chunks > desc->nz) chunks = desc->nz;
+    kz = (int)((desc->nz + chunks - 1) / chunks);
+  }
+  if (kz > desc->nz) kz = desc->nz;
+  p->kz = kz;
+  p->grid = dim3(gx, gy, (desc->nz + kz - 1) / kz);
+  *out = p;
+  return YEE_OK;
+}
+
+void yee_plan_destroy(YeePlan* p) {
+  if (!p) return;
+  free_sources(p);
+  delete p;
+}
+
+int yee_set_sources(YeePlan* p, int32_t nsrc, const int32_t* comp, const int32_t* pos,
+                    const float* table, int32_t nsteps) {
+  if (!p) return fail(YEE_EINVAL, "null plan");
+  if (nsrc < 0 || nsteps < 0) return fail(YEE_EINVAL, "negative source or step count");
+  if (nsrc > 0 && (!comp || !pos || (nsteps > 0 && !table)))
+    return fail(YEE_EINVAL, "null source arrays");
+  free_sources(p);
+  p->comp.assign(nsrc, 0);
+  p->idx.assign(nsrc, 0);
+  for (int q = 0; q < nsrc; ++q) {
+    const int i = pos[3 * q], j = pos[3 * q + 1], k = pos[3 * q + 2];
+    if (i < 0 || i >= p->d.nx || j < 0 || j >= p->d.ny || k < 0 || k >= p->d.nz)
+      return fail(YEE_EINVAL, "source position out of grid bounds");
+    if (comp[q] < YEE_EX || comp[q] > YEE_EZ)
+      return fail(YEE_EINVAL, "source component must be EX, EY or EZ");
+    p->comp[q] = comp[q];
+    p->idx[q] = ((long long)k * p->d.ny + j) * p->d.nx + i;
+  }
+  p->table.assign(table ? table : (const float*)nullptr,
+                  table ? table + (size_t)nsrc * nsteps : (const float*)nullptr);
+  p->nsrc = nsrc;
+  p->nsteps_table = nsteps;
+  if (nsrc > YEE_FUSED_SOURCES) {
+    YEE_HIP_CHECK(hipMalloc(&p->d_comp, sizeof(int) * nsrc));
+    YEE_HIP_CHECK(hipMalloc(&p->d_idx, sizeof(long long) * nsrc));
+    YEE_HIP_CHECK(hipMalloc(&p->d_table, sizeof(float) * (size_t)nsrc * (nsteps > 0 ? nsteps : 1)));
+    YEE_HIP_CHECK(hipMemcpy(p->d_comp, p->comp.data(), sizeof(int) * nsrc, hipMemcpyHostToDevice));
+    YEE_HIP_CHECK(hipMemcpy(p->d_idx, p->idx.data(), sizeof(long long) * nsrc, hipMemcpyHostToDevice));
+    if (nsteps > 0)
+      YEE_HIP_CHECK(hipMemcpy(p->d_table, p->table.data(), sizeof(float) * (size_t)nsrc * nsteps,
+                              hipMemcpyHostToDevice));
+  }
+  return YEE_OK;
+}
+
+int yee_step_h(YeePlan* p, float* const* fields, float* psi, void* stream) {
+  int rc = check_fields(p, fields, psi);
+  if (rc) return rc;
+  return launch_h(p, make_args(p, fields, psi), (hipStream_t)stream);
+}
+
+int yee_step_e(YeePlan* p, float* const* fields, float* psi, int32_t n, void* stream) {
+  int rc = check_fields(p, fields, psi);
+  if (rc) return rc;
+  return launch_e(p, make_args(p, fields, psi), n, (hipStream_t)stream);
+}
+
+int yee_run(YeePlan* p, float* const* fields, float* psi, int32_t n0, int32_t nsteps,
+            void* stream) {
+  int rc = check_fields(p, fields, psi);
+  if (rc) return rc;
+  if (nsteps < 0) return fail(YEE_EINVAL, "negative step count");
+  const GridArgs g = make_args(p, fields, psi);
+  hipStream_t st = (hipStream_t)stream;
+  for (int n = n0; n < n0 + nsteps; ++n) {
+    rc = launch_h(p, g, st);
+    if (rc) return rc;
+    rc = launch_e(p, g, n, st);
+    if (rc) return rc;
+  }
+  return YEE_OK;
+}
+
+int yee_energy_fp64(int32_t nx, int32_t ny, int32_t nz, float* const* fields, const float* eps_r,
+                    double* work, void* stream) {
+  if (nx < 1 || ny < 1 || nz < 1) return fail(YEE_EINVAL, "bad grid size");
+  if (!fields || !work) return fail(YEE_EINVAL, "null pointer");
+  GridArgs g;
+  std::memset(&g, 0, sizeof(g));
+  for (int c = 0; c < 6; ++c) {
+    if (!fields[c]) return fail(YEE_EINVAL, "null field pointer");
+    g.f[c] = fields[c];
+  }
+  const long long ncell = (long long)nx * ny * nz;
+  hipStream_t st = (hipStream_t)stream;
+  energy_partial_kernel<<<kEnergyBlocks, 256, 0, st>>>(g, eps_r, ncell, work + 2);
+  YEE_HIP_CHECK(hipGetLastError());
+  energy_final_kernel<<<1, 256, 0, st>>>(work + 2, kEnergyBlocks, work);
+  YEE_HIP_CHECK(hipGetLastError());
+  return YEE_OK;
+}
+
+int yee_copy_probe(const float* src, float* dst, int64_t n, void* stream) {
+  if (!src || !dst || n < 0) return fail(YEE_EINVAL, "bad copy arguments");
+  hipStream_t st = (hipStream_t)stream;
+  const long long n4 = n / 4;
+  if (((uintptr_t)src | (uintptr_t)dst) % 16 != 0) return fail(YEE_EINVAL, "copy needs 16-byte alignment");
+  if (n4 > 0) {
+    long long blocks = (n4 + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    copy_probe_kernel<<<(unsigned)blocks, 256, 0, st>>>((const float4*)src, (float4*)dst, n4);
+    YEE_HIP_CHECK(hipGetLastError());
+  }
+  if (n > n4 * 4) {
+    copy_tail_kernel<<<1, 64, 0, st>>>(src, dst, n4 * 4, n);
+    YEE_HIP_CHECK(hipGetLastError());
+  }
+  return YEE_OK;
+}
+
+}  // extern "C"

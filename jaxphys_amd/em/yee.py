@@ -1,0 +1,197 @@
+"""Device state of a 3D Yee run on the GPU and its calls into libyee_hip.so.
+
+``YeeSolver`` owns, as torch tensors on one HIP device, the six field
+arrays, the per-cell E coefficients, the boundary tables, the CPML psi
+buffer and the plan; it advances them with the fused HIP kernels.  Every
+per-cell array is stored x fastest, torch shape ``(nz, ny, nx)``; the
+reference's ``F[i, j, k]`` is ``F_internal[k, j, i]``
+(src/neurosim/em/fdtd3d.py:212-217, 240).
+
+Coefficients (reference file:line):
+* ``ch = dt/(MU0*dx)``, ``ce = dt/(EPS0*dx)``            fdtd3d.py:223-224
+* ``cb = ce/eps_r`` as a float32 division                  fdtd3d.py:336-354
+* sponge: three 1-D cubic profiles, ``sigma_max=3.2/dx``   fdtd3d.py:139-167
+* CPML and lossy Ca/Cb are extensions (Roden & Gedney 2000; Taflove &
+  Hagness 3rd ed. ch. 7 and eq. 3.31), see ``cpml_tables``.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+
+from jaxphys_amd import _native
+from jaxphys_amd.em.constants import EPS0, ETA0, MU0
+
+FIELD_NAMES = ("ex", "ey", "ez", "hx", "hy", "hz")
+
+
+def sponge_profile_1d(n: int, n_pml: int, dx: float) -> np.ndarray:
+    """fdtd3d.py:149-157: ``s[i] = s[n-1-i] = sigma_max*((n_pml-i)/n_pml)**3``
+    written in increasing ``i`` (later writes win when the slabs overlap),
+    stored as float32."""
+    s = np.zeros(n, dtype=np.float32)
+    if n_pml == 0:
+        return s
+    sigma_max = 0.8 * 4.0 / (dx * 1.0)
+    for i in range(n_pml):
+        val = sigma_max * ((n_pml - i) / n_pml) ** 3
+        s[i] = val
+        s[n - 1 - i] = val
+    return s
+
+
+def cpml_tables(n: int, npml: int, dx: float, dt: float, grading: float = 3.0,
+                R0: float = 1e-6, kappa_max: float = 1.0, alpha_max: float = 0.0) -> np.ndarray:
+    """CPML recursive-convolution coefficients along one axis.
+
+    Returns float32 ``[bE, cE, kinvE, bH, cH, kinvH]`` (each length ``n``).
+    E-type derivatives sit at integer positions ``i``, H-type at ``i+1/2``.
+    Depth ``d`` is measured from the inner interface (``npml`` on the low
+    side, ``n-1-npml`` on the high side), normalised by ``npml`` and clipped
+    to [0, 1]; only indices in the slabs ``[0, npml)`` and ``[n-npml, n)``
+    are absorbing.  ``sigma = sigma_max d^m``, ``kappa = 1+(kappa_max-1)d^m``,
+    ``alpha = alpha_max (1-d)``, ``sigma_max = -(m+1) ln R0 / (2 eta0 npml dx)``,
+    ``b = exp(-(sigma/kappa+alpha) dt/eps0)``,
+    ``c = sigma (b-1) / (sigma kappa + kappa^2 alpha)``.
+    """
+    out = np.zeros((6, n), dtype=np.float64)
+    sigma_max = -(grading + 1.0) * math.log(R0) / (2.0 * ETA0 * npml * dx)
+    idx = np.arange(n)
+    slab = (idx < npml) | (idx >= n - npml)
+    for kind, half in ((0, 0.0), (1, 0.5)):
+        x = idx + half
+        d = np.where(idx < npml, (npml - x) / npml,
+                     np.where(idx >= n - npml, (x - (n - 1 - npml)) / npml, 0.0))
+        d = np.clip(d, 0.0, 1.0)
+        sig = sigma_max * d**grading
+        kap = 1.0 + (kappa_max - 1.0) * d**grading
+        alp = np.where(slab, alpha_max * (1.0 - d), 0.0)
+        b = np.exp(-(sig / kap + alp) * dt / EPS0)
+        den = sig * kap + kap * kap * alp
+        c = np.where(sig > 0, sig * (b - 1.0) / np.where(den > 0, den, 1.0), 0.0)
+        out[3 * kind + 0] = b
+        out[3 * kind + 1] = c
+        out[3 * kind + 2] = 1.0 / kap
+    return out.astype(np.float32)
+
+
+class YeeSolver:
+    """GPU state and stepping of one Yee grid (internal layout (nz, ny, nx))."""
+
+    def __init__(self, size, dx: float, dt: float, boundary: str, pml_layers: int,
+                 eps_r: torch.Tensor, sigma: torch.Tensor | None = None,
+                 cpml: dict | None = None, device=None):
+        self.device = torch.device(device) if device is not None else _native.require_device()
+        _native.lib()
+        self.nx, self.ny, self.nz = (int(s) for s in size)
+        self.dx = float(dx)
+        self.dt = float(dt)
+        self.boundary_name = boundary
+        self.pml_layers = int(pml_layers)
+        nx, ny, nz = self.nx, self.ny, self.nz
+        shape = (nz, ny, nx)
+        dev = self.device
+        eps = eps_r.to(device=dev, dtype=torch.float32).contiguous()
+        assert tuple(eps.shape) == shape
+        self.eps_r = eps
+        ch = np.float32(self.dt / (float(MU0) * self.dx))
+        ce = self.dt / (float(EPS0) * self.dx)
+        # fdtd3d.py:336: (dt_eps_dx / eps_r) -- a float32 division per cell
+        self.cb = torch.div(torch.full(shape, float(np.float32(ce)), dtype=torch.float32,
+                                       device=dev), eps)
+        self.ca = None
+        if sigma is not None:
+            sig = sigma.to(device=dev, dtype=torch.float64)
+            if bool((sig != 0).any()):
+                e64 = eps.to(torch.float64)
+                x = sig * self.dt / (2.0 * EPS0 * e64)
+                lossy = sig != 0
+                self.ca = torch.where(lossy, (1.0 - x) / (1.0 + x),
+                                      torch.ones_like(x)).to(torch.float32).contiguous()
+                self.cb = torch.where(lossy, ((ce / e64) / (1.0 + x)).to(torch.float32),
+                                      self.cb).contiguous()
+        desc = _native.YeeGridDesc()
+        desc.nx, desc.ny, desc.nz = nx, ny, nz
+        desc.ch = float(ch)
+        desc.cb = _native.ptr(self.cb)
+        desc.ca = _native.ptr(self.ca) if self.ca is not None else None
+        self.sponge = None
+        self.cpml = None
+        if boundary == "periodic":
+            desc.boundary = _native.BOUNDARY_PERIODIC
+        elif boundary == "absorbing":
+            desc.boundary = _native.BOUNDARY_SPONGE
+            prof = np.concatenate([sponge_profile_1d(n, self.pml_layers, self.dx)
+                                   for n in (nx, ny, nz)])
+            self.sponge = torch.from_numpy(prof).to(dev)
+            desc.sponge_sigma = _native.ptr(self.sponge)
+            desc.sponge_dt = float(np.float32(self.dt))
+            desc.sponge_eps0 = float(np.float32(EPS0))
+        elif boundary == "cpml":
+            desc.boundary = _native.BOUNDARY_CPML
+            kw = dict(cpml or {})
+            tabs = np.concatenate([cpml_tables(n, self.pml_layers, self.dx, self.dt, **kw).ravel()
+                                   for n in (nx, ny, nz)])
+            self.cpml = torch.from_numpy(tabs).to(dev)
+            desc.npml = self.pml_layers
+            desc.cpml = _native.ptr(self.cpml)
+        else:  # any other string: the reference's non-periodic, undamped update
+            desc.boundary = _native.BOUNDARY_WALLS
+        self.plan = _native.Plan(desc, keep=[self.cb, self.ca, self.sponge, self.cpml])
+        self.fields = [torch.zeros(shape, dtype=torch.float32, device=dev) for _ in range(6)]
+        npsi = self.plan.psi_floats()
+        self.psi = torch.zeros(max(npsi, 1), dtype=torch.float32, device=dev) if npsi else None
+        self.n_sources = 0
+        self.table_steps = 0
+
+    # -- sources --------------------------------------------------------
+    def set_sources(self, comps, positions, table: np.ndarray) -> None:
+        """``comps[s]`` in 0..2 (Ex, Ey, Ez), ``positions[s] = (i, j, k)``,
+        ``table`` float32 of shape ``(n_steps, n_sources)``."""
+        table = np.ascontiguousarray(table, dtype=np.float32)
+        if table.ndim != 2 or table.shape[1] != len(comps):
+            raise ValueError("source table must be (n_steps, n_sources)")
+        self.plan.set_sources(np.asarray(comps, dtype=np.int32),
+                              np.asarray(positions, dtype=np.int32).reshape(-1, 3),
+                              table, table.shape[0])
+        self.n_sources = len(comps)
+        self.table_steps = table.shape[0]
+
+    # -- stepping -------------------------------------------------------
+    def run(self, n0: int, nsteps: int, stream=None) -> None:
+        if nsteps > 0:
+            self.plan.run(self.fields, self.psi, n0, nsteps, stream)
+
+    def step_h(self, stream=None) -> None:
+        self.plan.step_h(self.fields, self.psi, stream)
+
+    def step_e(self, n: int, stream=None) -> None:
+        self.plan.step_e(self.fields, self.psi, n, stream)
+
+    def reset(self) -> None:
+        for f in self.fields:
+            f.zero_()
+        if self.psi is not None:
+            self.psi.zero_()
+
+    def energy_fp64(self):
+        """float64 (sum eps_r |E|^2, sum |H|^2) as Python floats."""
+        w = _native.energy_fp64((self.nx, self.ny, self.nz), self.fields, self.eps_r)
+        a, b = w.cpu().tolist()
+        return a, b
+
+    def em_energy_joules(self) -> float:
+        """Discrete field energy 0.5 eps0 sum eps_r|E|^2 dx^3 + 0.5 mu0 sum|H|^2 dx^3."""
+        we, wh = self.energy_fp64()
+        v = self.dx**3
+        return 0.5 * EPS0 * we * v + 0.5 * MU0 * wh * v
+
+    def reference_view(self, t: torch.Tensor) -> torch.Tensor:
+        """(nz, ny, nx) storage as the reference's (nx, ny, nz) index order."""
+        return t.permute(2, 1, 0)
+
+    def close(self) -> None:
+        self.plan.close()

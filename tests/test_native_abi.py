@@ -1,0 +1,78 @@
+"""The C ABI library loads and exports every entry point include/*.h
+declares; host-side validation works without a GPU (no compute calls)."""
+
+import ctypes
+import os
+import re
+
+import pytest
+
+from jaxphys_amd import _native
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return set(re.findall(r"\b([a-z][a-z0-9_]*)\s*\(", text)) - {"if", "sizeof"}
+
+
+def test_yee_library_exports_all_declared_symbols():
+    lib = _native.lib()
+    names = _declared("yee.h")
+    assert "yee_run" in names and "yee_plan_create" in names
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(_native.EXPORTS) == names
+
+
+def _desc(**kw):
+    d = _native.YeeGridDesc()
+    d.nx, d.ny, d.nz = kw.get("n", (16, 12, 10))
+    d.boundary = kw.get("boundary", _native.BOUNDARY_WALLS)
+    d.ch = 0.1
+    d.cb = kw.get("cb", 0x1000)  # never dereferenced by plan creation
+    d.npml = kw.get("npml", 0)
+    d.cpml = kw.get("cpml", None)
+    d.sponge_sigma = kw.get("sponge", None)
+    return d
+
+
+def test_psi_floats_formula():
+    d = _desc(n=(20, 18, 16), boundary=_native.BOUNDARY_CPML, npml=4, cpml=0x2000)
+    n = _native.lib().yee_psi_floats(ctypes.byref(d))
+    tp = 8
+    assert n == 4 * (16 * tp * 20 + tp * 18 * 20 + 16 * 18 * tp)
+    assert _native.lib().yee_psi_floats(ctypes.byref(_desc())) == 0
+
+
+@pytest.mark.parametrize("kw,msg", [
+    (dict(n=(1, 12, 10)), "at least 2"),
+    (dict(boundary=7), "unknown boundary"),
+    (dict(cb=None), "cb"),
+    (dict(boundary=_native.BOUNDARY_SPONGE), "sponge"),
+    (dict(boundary=_native.BOUNDARY_CPML, npml=6, cpml=0x2000), "overlap"),
+    (dict(boundary=_native.BOUNDARY_CPML, npml=2), "tables"),
+])
+def test_plan_create_rejects_bad_descriptors(kw, msg):
+    d = _desc(**kw)
+    h = ctypes.c_void_p()
+    rc = _native.lib().yee_plan_create(ctypes.byref(d), ctypes.byref(h))
+    assert rc == _native.YEE_EINVAL
+    assert msg in _native.lib().yee_last_error().decode()
+    with pytest.raises(Exception):
+        _native.check(rc, "yee_plan_create")
+
+
+def test_plan_and_source_validation_host_only():
+    from jaxphys_amd import ConfigurationError
+
+    plan = _native.Plan(_desc())
+    import numpy as np
+    with pytest.raises(ConfigurationError, match="out of grid"):
+        plan.set_sources([2], [(16, 0, 0)], np.zeros((3, 1), np.float32), 3)
+    with pytest.raises(ConfigurationError, match="component"):
+        plan.set_sources([4], [(1, 1, 1)], np.zeros((3, 1), np.float32), 3)
+    plan.set_sources([2, 0], [(1, 1, 1), (2, 2, 2)], np.zeros((3, 2), np.float32), 3)
+    plan.close()
