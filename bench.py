@@ -41,11 +41,40 @@ def build_workload(name: str, device):
     return g, steps
 
 
-def cpu_baseline(name: str, budget_s: float):
-    """The float32 C/OpenMP restatement of the same scheme (oracle), run on
-    this host's cores on a bounded number of steps of the same workload."""
+def cpu_baseline(solver, budget_s: float):
+    """The float32 C/OpenMP restatement of the same scheme (the oracle), timed
+    on this host's cores for about ``budget_s`` seconds of steps of the same
+    workload (same grid, coefficients, CPML tables and sources)."""
+    import numpy as np
+
     from oracle import c_oracle
-    return c_oracle.time_workload(name, budget_s)
+
+    cb = solver.cb.cpu().numpy()
+    ca = solver.ca.cpu().numpy() if solver.ca is not None else None
+    cp = solver.cpml.cpu().numpy() if solver.cpml is not None else None
+    tab = solver.source_table
+    comps = solver.source_comps
+    nx, ny, _ = solver.nx, solver.ny, solver.nz
+    sidx = [(k * ny + j) * nx + i for i, j, k in solver.source_positions]
+    ch = float(np.float32(solver.dt / (solver.mu0 * solver.dx)))
+    cores = c_oracle.host_cores()
+    steps, secs = c_oracle.time_arrays((solver.nx, solver.ny, solver.nz), cb, ca, cp,
+                                       solver.pml_layers, ch, tab, comps, sidx, budget_s,
+                                       threads=cores)
+    cells = solver.nx * solver.ny * solver.nz
+    cpu_model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    cpu_model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": round(cells * steps / secs / 1e9, 4), "unit": "Gcell-upd/s",
+            "cores": cores, "kind": "port",
+            "sample": f"{steps} float32 steps of the same {solver.nx}x{solver.ny}x{solver.nz} "
+                      f"workload in {secs:.1f} s (C/OpenMP oracle, {cpu_model})"}
 
 
 def main() -> None:
@@ -139,7 +168,7 @@ def main() -> None:
     cpu = None
     if args.cpu_budget > 0:
         try:
-            cpu = cpu_baseline(args.workload, args.cpu_budget)
+            cpu = cpu_baseline(solver, args.cpu_budget)
         except Exception as exc:  # reported, never silently replaced
             cpu = {"value": None, "error": f"{type(exc).__name__}: {exc}"}
 

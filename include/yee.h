@@ -110,6 +110,69 @@ int yee_step_h(YeePlan* plan, float* const* fields, float* psi, void* stream);
 int yee_step_e(YeePlan* plan, float* const* fields, float* psi, int32_t n,
                void* stream);
 
+/* ---- adjoint (no reference counterpart) --------------------------------
+ * The reference's gradient path is generic jax.grad through the scan
+ * (src/neurosim/optimize.py:84; fdtd3d.py:393), which stores every step.
+ * yee_adjoint_step applies the exact transpose of one forward step
+ * (yee_step_h + yee_step_e, including sponge damping, CPML psi and lossy
+ * Ca) to the adjoint state: lam[6] holds dJ/d(E^{n+1}, H^{n+1/2}) on entry
+ * and dJ/d(E^n, H^{n-1/2}) on exit; lam_psi (same size as psi) holds the
+ * psi adjoints, updated with one step of deferral for the H part (the
+ * caller runs steps n = N-1 .. 0 in order and does nothing else with it).
+ * Objective terms are added between steps with yee_probe_scatter_add.
+ * Optional design box: cells where dJ/d(eps_r) is wanted, all with damping
+ * 1 (outside sponge/CPML slabs) and every E component updated; the caller
+ * passes the forward E^{n+1} and E^n on the box (comp-major, x fastest)
+ * and the kernel accumulates acc_cb += sum_c g_c (E^{n+1}_c - Ca E^n_c),
+ * acc_ca += sum_c g_c E^n_c, with g = lam(E^{n+1}); then dJ/dCb =
+ * acc_cb / Cb and dJ/dCa = acc_ca.                                        */
+typedef struct YeeDesignBox {
+  int32_t x0, y0, z0, nx, ny, nz;
+  const float* e_next;
+  const float* e_cur;
+  float* acc_cb;
+  float* acc_ca; /* may be NULL when lossless */
+} YeeDesignBox;
+
+int yee_adjoint_step(YeePlan* plan, float* const* lam, float* lam_psi,
+                     const YeeDesignBox* box, void* stream);
+
+/* Probe helpers for objectives (no reference counterpart):
+ * gather  out[q] = fields[comp[q]][idx[q]]
+ * scatter fields[comp[q]][idx[q]] += scale * w[q] * vals[q]  (w may be NULL)
+ * comp (0..5, int32) and idx (cell index, int64) are device arrays; the
+ * scatter indices must be distinct.                                      */
+int yee_probe_gather(float* const* fields, const int32_t* comp, const int64_t* idx,
+                     int32_t n, float* out, void* stream);
+int yee_probe_scatter_add(float* const* fields, const int32_t* comp, const int64_t* idx,
+                          const float* w, const float* vals, float scale, int32_t n,
+                          void* stream);
+/* Copy Ex, Ey, Ez on box = {x0, y0, z0, nx, ny, nz} (host array) into out
+ * (comp-major, x fastest, 3*nx*ny*nz floats).                            */
+int yee_box_gather(float* const* fields, int32_t nx, int32_t ny, int32_t nz,
+                   const int32_t* box, float* out, void* stream);
+
+/* yee_run plus recording after every step s = 0..nsteps-1 (step n0+s):
+ * probe_out[s*nprobe + q] = field value of probe q (as yee_probe_gather)
+ * and box_out[s*3*boxcells ...] = E on the box (as yee_box_gather).  Either
+ * part may be disabled (nprobe == 0 / box == NULL).                      */
+int yee_run_record(YeePlan* plan, float* const* fields, float* psi, int32_t n0,
+                   int32_t nsteps, int32_t nprobe, const int32_t* probe_comp,
+                   const int64_t* probe_idx, float* probe_out, const int32_t* box,
+                   float* box_out, void* stream);
+
+/* Adjoint sweep over steps n_hi-1 down to n_lo (yee_adjoint_step each),
+ * adding after step n (for n >= 1) the objective term
+ * scale * w[q] * probe_hist[(n-1)*nprobe + q] to the adjoint of probe q
+ * (the objective sum_n sum_q w_q F_q(after step n)^2 gives scale = 2).
+ * box history: box_hist[m] = E on the box after step n_lo-1+m, m = 0 ..
+ * n_hi-n_lo (so box_hist[0] is E^{n_lo}).                               */
+int yee_adjoint_run(YeePlan* plan, float* const* lam, float* lam_psi, int32_t n_lo,
+                    int32_t n_hi, int32_t nprobe, const int32_t* probe_comp,
+                    const int64_t* probe_idx, const float* w, const float* probe_hist,
+                    float scale, const int32_t* box, const float* box_hist,
+                    float* acc_cb, float* acc_ca, void* stream);
+
 /* Float64 accumulators for the checks (no reference counterpart):
  * work[0] = sum_cells eps_r |E|^2 (eps_r from an explicit per-cell array,
  * NULL = 1), work[1] = sum_cells |H|^2.  `work` is a caller-owned device

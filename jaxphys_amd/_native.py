@@ -60,7 +60,36 @@ EXPORTS = {
     "yee_copy_probe": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
                                       ctypes.c_void_p]),
     "yee_last_error": (ctypes.c_char_p, []),
+    "yee_adjoint_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.c_void_p]),
+    "yee_probe_gather": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p]),
+    "yee_probe_scatter_add": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float,
+                                             ctypes.c_int32, ctypes.c_void_p]),
+    "yee_box_gather": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
+                                      ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
+                                      ctypes.c_void_p]),
+    "yee_run_record": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                      ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "yee_adjoint_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.c_void_p, ctypes.c_float, ctypes.c_void_p,
+                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.c_void_p]),
 }
+
+
+class YeeDesignBox(ctypes.Structure):
+    """Mirror of ``YeeDesignBox`` in include/yee.h."""
+
+    _fields_ = [("x0", ctypes.c_int32), ("y0", ctypes.c_int32), ("z0", ctypes.c_int32),
+                ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+                ("e_next", ctypes.c_void_p), ("e_cur", ctypes.c_void_p),
+                ("acc_cb", ctypes.c_void_p), ("acc_ca", ctypes.c_void_p)]
 
 
 def library_path(name: str = "libyee_hip.so") -> str:
@@ -159,6 +188,55 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+def box_array(box):
+    return None if box is None else (ctypes.c_int32 * 6)(*[int(v) for v in box])
+
+
+def run_record(plan: "Plan", fields, psi, n0, nsteps, probe_comp=None, probe_idx=None,
+               probe_out=None, box=None, box_out=None, stream=None) -> None:
+    """``yee_run_record``: steps with per-step probe / design-box recording."""
+    nprobe = 0 if probe_comp is None else int(probe_comp.numel())
+    check(lib().yee_run_record(plan.handle, field_array(fields), ptr(psi) or None, int(n0),
+                               int(nsteps), nprobe, ptr(probe_comp) or None,
+                               ptr(probe_idx) or None, ptr(probe_out) or None,
+                               box_array(box), ptr(box_out) or None, stream_handle(stream)),
+          "yee_run_record")
+
+
+def adjoint_run(plan: "Plan", lam, lam_psi, n_lo, n_hi, probe_comp, probe_idx, w, probe_hist,
+                scale, box=None, box_hist=None, acc_cb=None, acc_ca=None, stream=None) -> None:
+    nprobe = 0 if probe_comp is None else int(probe_comp.numel())
+    check(lib().yee_adjoint_run(plan.handle, field_array(lam), ptr(lam_psi) or None, int(n_lo),
+                                int(n_hi), nprobe, ptr(probe_comp) or None,
+                                ptr(probe_idx) or None, ptr(w) or None, ptr(probe_hist) or None,
+                                float(scale), box_array(box), ptr(box_hist) or None,
+                                ptr(acc_cb) or None, ptr(acc_ca) or None, stream_handle(stream)),
+          "yee_adjoint_run")
+
+
+def probe_gather(fields, comp, idx, out, stream=None) -> None:
+    check(lib().yee_probe_gather(field_array(fields), ptr(comp), ptr(idx), int(comp.numel()),
+                                 ptr(out), stream_handle(stream)), "yee_probe_gather")
+
+
+def probe_scatter_add(fields, comp, idx, w, vals, scale, stream=None) -> None:
+    check(lib().yee_probe_scatter_add(field_array(fields), ptr(comp), ptr(idx), ptr(w) or None,
+                                      ptr(vals), float(scale), int(comp.numel()),
+                                      stream_handle(stream)), "yee_probe_scatter_add")
+
+
+def box_gather(fields, shape_xyz, box, out, stream=None) -> None:
+    nx, ny, nz = shape_xyz
+    check(lib().yee_box_gather(field_array(fields), nx, ny, nz, box_array(box), ptr(out),
+                               stream_handle(stream)), "yee_box_gather")
+
+
+def adjoint_step(plan: "Plan", lam, lam_psi, box=None, stream=None) -> None:
+    check(lib().yee_adjoint_step(plan.handle, field_array(lam), ptr(lam_psi) or None,
+                                 ctypes.byref(box) if box is not None else None,
+                                 stream_handle(stream)), "yee_adjoint_step")
 
 
 def energy_fp64(shape_xyz, fields, eps_r=None, work=None, stream=None):

@@ -1,0 +1,192 @@
+"""d(objective)/d(eps_r) on the GPU with the hand-written adjoint.
+
+No reference counterpart: the reference would differentiate
+``EMGrid3D.simulate`` with generic ``jax.grad`` (src/neurosim/optimize.py:84,
+reverse mode through ``lax.scan`` at src/neurosim/em/fdtd3d.py:393), which
+keeps residuals of every step (O(n_steps * N^3) memory).  Here:
+
+1. forward run on the GPU, saving the full state (6 fields + CPML psi) every
+   ``checkpoint_every`` steps and the probe values after every step;
+2. for each checkpoint segment, last first: restore the checkpoint, recompute
+   the segment forward while recording E on the design box after every step,
+   then run the adjoint steps of the segment in reverse
+   (``yee_adjoint_run``), which accumulate ``sum_n g (E^{n+1} - Ca E^n)`` and
+   ``sum_n g E^n`` per design cell;
+3. chain rule to eps_r: ``dJ/deps = acc_cb/Cb * dCb/deps + acc_ca * dCa/deps``.
+
+Objective: ``J = sum_n sum_q w_q F_q(after step n)^2`` over probes q
+(component, cell), e.g. the time-integrated |Ez|^2 on a monitor patch.
+``FDTDObjective`` wraps it as a ``torch.autograd.Function`` of the design
+permittivities.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from jaxphys_amd import _native
+from jaxphys_amd.exceptions import ConfigurationError
+
+_COMP = {"ex": 0, "ey": 1, "ez": 2, "hx": 3, "hy": 4, "hz": 5}
+
+
+class AdjointProblem:
+    """Objective and gradient w.r.t. eps_r on a design box of an EMGrid3D.
+
+    Args:
+        grid: the EMGrid3D (materials give the background eps_r; sources are
+            fixed).  Must not be periodic.
+        n_steps, dt: the run.
+        design_box: ``(x0, y0, z0, nx, ny, nz)`` in the reference's (i, j, k)
+            index order.  Must lie outside absorbing slabs, at i, j, k >= 1,
+            and contain no source.
+        probes: ``[(component, (i, j, k)), ...]`` with component in
+            ex..hz, and optional ``weights``.
+        checkpoint_every: forward checkpoint interval K.
+    """
+
+    def __init__(self, grid, n_steps: int, dt: float, design_box, probes, weights=None,
+                 checkpoint_every: int = 100):
+        self.grid = grid
+        self.n_steps = int(n_steps)
+        self.K = max(1, int(checkpoint_every))
+        self.dt = float(dt)
+        sol = grid.build_solver(self.dt, self.n_steps)
+        self.solver = sol
+        dev = sol.device
+        nx, ny, nz = sol.nx, sol.ny, sol.nz
+        x0, y0, z0, bx, by, bz = (int(v) for v in design_box)
+        self.box_ref = (x0, y0, z0, bx, by, bz)
+        if bx <= 0 or by <= 0 or bz <= 0 or x0 < 1 or y0 < 1 or z0 < 1 or \
+                x0 + bx > nx or y0 + by > ny or z0 + bz > nz:
+            raise ConfigurationError(f"design box {design_box} must lie inside the grid at "
+                                     "indices >= 1")
+        if sol.boundary_name == "periodic":
+            raise ConfigurationError("the adjoint is provided for non-periodic grids")
+        p = sol.pml_layers
+        if sol.boundary_name == "cpml":
+            if x0 < p or y0 < p or z0 < p or x0 + bx > nx - p or y0 + by > ny - p or \
+                    z0 + bz > nz - p:
+                raise ConfigurationError("design box overlaps the CPML slabs")
+        if sol.boundary_name == "absorbing" and sol.sponge is not None:
+            prof = sol.sponge.cpu().numpy()
+            sx, sy, sz = prof[:nx], prof[nx:nx + ny], prof[nx + ny:]
+            if (sx[x0:x0 + bx].any() or sy[y0:y0 + by].any() or sz[z0:z0 + bz].any()):
+                raise ConfigurationError("design box overlaps the sponge layers")
+        for (i, j, k) in sol.source_positions:
+            if x0 <= i < x0 + bx and y0 <= j < y0 + by and z0 <= k < z0 + bz:
+                raise ConfigurationError("design box contains a source")
+        self.region = (slice(z0, z0 + bz), slice(y0, y0 + by), slice(x0, x0 + bx))
+        self.box_int = (x0, y0, z0, bx, by, bz)
+        self.bcells = bx * by * bz
+        comps, idx = [], []
+        for comp, (i, j, k) in probes:
+            c = _COMP[comp] if isinstance(comp, str) else int(comp)
+            if c not in (0, 1, 2):
+                # the psi_H adjoint update is deferred by one step and reads
+                # lamH unmodified; objective terms are therefore E-only
+                raise ConfigurationError("adjoint objectives are sums over E components")
+            if not (0 <= i < nx and 0 <= j < ny and 0 <= k < nz):
+                raise ConfigurationError(f"probe {(i, j, k)} outside the grid")
+            comps.append(c)
+            idx.append((k * ny + j) * nx + i)
+        self.nprobe = len(comps)
+        self.probe_comp = torch.tensor(comps, dtype=torch.int32, device=dev)
+        self.probe_idx = torch.tensor(idx, dtype=torch.int64, device=dev)
+        w = np.ones(self.nprobe) if weights is None else np.asarray(weights, dtype=np.float64)
+        self.w = torch.tensor(w, dtype=torch.float32, device=dev)
+        self.background = sol.eps_r.clone()
+        self.probe_hist = torch.zeros((self.n_steps, self.nprobe), dtype=torch.float32, device=dev)
+        self.checkpoints = []
+        self.box_hist = None
+
+    # ------------------------------------------------------------------
+    def set_design(self, eps_box: torch.Tensor) -> None:
+        """``eps_box``: (bx, by, bz) in the reference index order."""
+        x0, y0, z0, bx, by, bz = self.box_ref
+        if tuple(eps_box.shape) != (bx, by, bz):
+            raise ConfigurationError(f"design eps shape {tuple(eps_box.shape)} != {(bx, by, bz)}")
+        eps = self.background.clone()
+        eps[self.region] = eps_box.detach().to(eps.device, torch.float32).permute(2, 1, 0)
+        self.solver.set_eps(eps)
+
+    def forward(self, eps_box: torch.Tensor | None = None) -> float:
+        """Forward run with checkpoints; returns J (float64 on the host)."""
+        sol = self.solver
+        if eps_box is not None:
+            self.set_design(eps_box)
+        sol.reset()
+        self.checkpoints = []
+        N, K = self.n_steps, self.K
+        for s0 in range(0, N, K):
+            self.checkpoints.append(([f.clone() for f in sol.fields],
+                                     sol.psi.clone() if sol.psi is not None else None))
+            ns = min(K, N - s0)
+            _native.run_record(sol.plan, sol.fields, sol.psi, s0, ns, self.probe_comp,
+                               self.probe_idx, self.probe_hist[s0:s0 + ns])
+        ph = self.probe_hist.to(torch.float64)
+        return float((self.w.to(torch.float64) * ph * ph).sum())
+
+    def backward(self) -> torch.Tensor:
+        """dJ/d eps_r on the design box (float64, reference index order)."""
+        sol = self.solver
+        dev = sol.device
+        N, K = self.n_steps, self.K
+        if not self.checkpoints:
+            raise RuntimeError("backward() needs a forward() first")
+        lam = [torch.zeros_like(f) for f in sol.fields]
+        lam_psi = torch.zeros_like(sol.psi) if sol.psi is not None else None
+        acc_cb = torch.zeros(self.bcells, dtype=torch.float32, device=dev)
+        acc_ca = torch.zeros(self.bcells, dtype=torch.float32, device=dev) \
+            if sol.ca is not None else None
+        if self.box_hist is None or self.box_hist.shape[0] < K + 1:
+            self.box_hist = torch.empty((K + 1, 3 * self.bcells), dtype=torch.float32, device=dev)
+        # adjoint of the final E: 2 w F(after step N-1)
+        _native.probe_scatter_add(lam, self.probe_comp, self.probe_idx, self.w,
+                                  self.probe_hist[N - 1], 2.0)
+        for seg in range(len(self.checkpoints) - 1, -1, -1):
+            s0 = seg * K
+            s1 = min(N, s0 + K)
+            fields, psi = self.checkpoints[seg]
+            for f, c in zip(sol.fields, fields):
+                f.copy_(c)
+            if psi is not None:
+                sol.psi.copy_(psi)
+            _native.box_gather(sol.fields, (sol.nx, sol.ny, sol.nz), self.box_int,
+                               self.box_hist[0])
+            _native.run_record(sol.plan, sol.fields, sol.psi, s0, s1 - s0, box=self.box_int,
+                               box_out=self.box_hist[1:])
+            _native.adjoint_run(sol.plan, lam, lam_psi, s0, s1, self.probe_comp, self.probe_idx,
+                                self.w, self.probe_hist, 2.0, box=self.box_int,
+                                box_hist=self.box_hist, acc_cb=acc_cb, acc_ca=acc_ca)
+        x0, y0, z0, bx, by, bz = self.box_ref
+        shape = (bz, by, bx)
+        dcb, dca = sol.d_coeffs_d_eps(self.region)
+        cb = sol.cb[self.region].to(torch.float64)
+        g = acc_cb.view(shape).to(torch.float64) / cb * dcb
+        if acc_ca is not None:
+            g = g + acc_ca.view(shape).to(torch.float64) * dca
+        return g.permute(2, 1, 0).contiguous()
+
+    def value_and_grad(self, eps_box: torch.Tensor):
+        J = self.forward(eps_box)
+        return J, self.backward()
+
+
+class FDTDObjective(torch.autograd.Function):
+    """``J = FDTDObjective.apply(eps_box, problem)``; ``J.backward()`` runs
+    the checkpointed adjoint and returns dJ/d eps_box."""
+
+    @staticmethod
+    def forward(ctx, eps_box, problem):
+        J = problem.forward(eps_box)
+        ctx.problem = problem
+        ctx.dtype = eps_box.dtype
+        ctx.device = eps_box.device
+        return torch.tensor(J, dtype=torch.float64, device=eps_box.device)
+
+    @staticmethod
+    def backward(ctx, gJ):
+        g = ctx.problem.backward().to(device=ctx.device)
+        return (gJ.to(g.dtype) * g).to(ctx.dtype), None

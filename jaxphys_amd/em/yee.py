@@ -91,28 +91,23 @@ class YeeSolver:
         self.dt = float(dt)
         self.boundary_name = boundary
         self.pml_layers = int(pml_layers)
+        self.mu0 = float(MU0)
         nx, ny, nz = self.nx, self.ny, self.nz
         shape = (nz, ny, nx)
         dev = self.device
         eps = eps_r.to(device=dev, dtype=torch.float32).contiguous()
         assert tuple(eps.shape) == shape
-        self.eps_r = eps
         ch = np.float32(self.dt / (float(MU0) * self.dx))
-        ce = self.dt / (float(EPS0) * self.dx)
-        # fdtd3d.py:336: (dt_eps_dx / eps_r) -- a float32 division per cell
-        self.cb = torch.div(torch.full(shape, float(np.float32(ce)), dtype=torch.float32,
-                                       device=dev), eps)
-        self.ca = None
+        self.ce = self.dt / (float(EPS0) * self.dx)
+        self.sigma = None
         if sigma is not None:
-            sig = sigma.to(device=dev, dtype=torch.float64)
+            sig = sigma.to(device=dev, dtype=torch.float64).contiguous()
             if bool((sig != 0).any()):
-                e64 = eps.to(torch.float64)
-                x = sig * self.dt / (2.0 * EPS0 * e64)
-                lossy = sig != 0
-                self.ca = torch.where(lossy, (1.0 - x) / (1.0 + x),
-                                      torch.ones_like(x)).to(torch.float32).contiguous()
-                self.cb = torch.where(lossy, ((ce / e64) / (1.0 + x)).to(torch.float32),
-                                      self.cb).contiguous()
+                self.sigma = sig
+        self.eps_r = torch.empty_like(eps)
+        self.cb = torch.empty_like(eps)
+        self.ca = torch.empty_like(eps) if self.sigma is not None else None
+        self.set_eps(eps)
         desc = _native.YeeGridDesc()
         desc.nx, desc.ny, desc.nz = nx, ny, nz
         desc.ch = float(ch)
@@ -147,6 +142,38 @@ class YeeSolver:
         self.n_sources = 0
         self.table_steps = 0
 
+    def set_eps(self, eps: torch.Tensor) -> None:
+        """Recompute the E coefficients in place (the plan keeps pointing at
+        the same tensors).  Lossless cells: ``cb = f32(ce) / eps_r``, a
+        float32 division per cell (fdtd3d.py:336); lossy cells (extension,
+        Taflove & Hagness eq. 3.31): ``x = sigma dt / (2 eps0 eps_r)``,
+        ``ca = (1-x)/(1+x)``, ``cb = (ce/eps_r)/(1+x)`` in float64, rounded."""
+        eps = eps.to(device=self.device, dtype=torch.float32)
+        self.eps_r.copy_(eps)
+        ce = self.ce
+        torch.div(torch.full_like(eps, float(np.float32(ce))), eps, out=self.cb)
+        if self.sigma is not None:
+            e64 = eps.to(torch.float64)
+            x = self.sigma * self.dt / (2.0 * EPS0 * e64)
+            lossy = self.sigma != 0
+            self.ca.copy_(torch.where(lossy, (1.0 - x) / (1.0 + x), torch.ones_like(x)))
+            self.cb.copy_(torch.where(lossy, ((ce / e64) / (1.0 + x)).to(torch.float32), self.cb))
+
+    def d_coeffs_d_eps(self, region=None):
+        """(dCb/deps_r, dCa/deps_r) per cell (float64) for the adjoint chain
+        rule; ``region`` is an index tuple into the internal layout."""
+        eps = self.eps_r.to(torch.float64)
+        cb = self.cb.to(torch.float64)
+        if region is not None:
+            eps, cb = eps[region], cb[region]
+        if self.sigma is None:
+            return -cb / eps, None
+        sig = self.sigma if region is None else self.sigma[region]
+        x = sig * self.dt / (2.0 * EPS0 * eps)
+        dcb = -cb / (eps * (1.0 + x))
+        dca = 2.0 * x / (eps * (1.0 + x) ** 2)
+        return dcb, dca
+
     # -- sources --------------------------------------------------------
     def set_sources(self, comps, positions, table: np.ndarray) -> None:
         """``comps[s]`` in 0..2 (Ex, Ey, Ez), ``positions[s] = (i, j, k)``,
@@ -159,6 +186,9 @@ class YeeSolver:
                               table, table.shape[0])
         self.n_sources = len(comps)
         self.table_steps = table.shape[0]
+        self.source_table = table
+        self.source_comps = list(comps)
+        self.source_positions = [tuple(p) for p in positions]
 
     # -- stepping -------------------------------------------------------
     def run(self, n0: int, nsteps: int, stream=None) -> None:

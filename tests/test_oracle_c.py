@@ -1,0 +1,117 @@
+"""Pins of the C/OpenMP oracle against the NumPy restatements (no GPU).
+
+The C oracle is the checker at sizes NumPy is slow for and the CPU timing
+baseline; it must reproduce ``ref_fdtd`` (reference modes) and ``ext_fdtd``
+(CPML, lossy) in float32 bit for bit -- same operation order, contraction off
+-- and in float64 to rounding."""
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import c_oracle as C
+from oracle import ext_fdtd as X
+from oracle import ref_fdtd as R
+
+
+def _ref_final(size, dx, boundary, pml, srcs, mats, n, dt, dtype):
+    _, s = R.simulate(size, dx, boundary, pml, srcs, mats, dt=dt, save_every=n - 1 if n > 2 else 1,
+                      n_steps=n, dtype=dtype)
+    return [s[c][-1] for c in ("ex", "ey", "ez", "hx", "hy", "hz")]
+
+
+@pytest.mark.parametrize("boundary,pml", [("absorbing", 4), ("periodic", 0), ("walls", 0)])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_c_oracle_matches_numpy_reference_modes(boundary, pml, dtype):
+    size = (19, 14, 11)
+    rng = np.random.default_rng(2)
+    mats = [R.Mat(rng.random(size) > 0.5, 4.0)]
+    srcs = [R.Src(5e9, (9, 7, 5)), R.Src(3e9, (3, 4, 8), 0.5, "x"), R.Src(3e9, (0, 0, 0), 1.0, "y")]
+    dt = R.default_dt(0.01, dtype)
+    n = 25
+    ref = _ref_final(size, 0.01, boundary, pml, srcs, mats, n, dt, dtype)
+    eps = R.build_eps_r(size, mats, np.float64)
+    o = C.COracle(size, 0.01, dt, boundary, eps, npml=pml, dtype=dtype)
+    tabs = []
+    for s in srcs:
+        col = [R.source_values([s], q, 0.0, dt, dtype)[0][0] for q in range(n)]
+        comp = 2 if s.polarization == "z" else (0 if s.polarization == "x" else 1)
+        tabs.append((comp, s.position, np.asarray(col, dtype=np.float64)))
+    o.set_sources(tabs)
+    o.run(0, n, threads=4)
+    for a, b in zip(o.fields_ref(), ref):
+        if dtype == np.float32:
+            np.testing.assert_array_equal(a, b)
+        else:
+            np.testing.assert_allclose(a, b, rtol=1e-13, atol=1e-13 * np.abs(b).max())
+
+
+@pytest.mark.parametrize("lossy", [False, True])
+def test_c_oracle_matches_ext_cpml(lossy):
+    size = (20, 18, 16)
+    dx = 1.0
+    dt = 0.5 / math.sqrt(3) / R.C0
+    rng = np.random.default_rng(4)
+    eps = (1 + 5 * rng.random(size)).astype(np.float32)
+    sigma = None
+    if lossy:
+        sigma = np.zeros(size)
+        sigma[:, 5:9, :] = 0.05 * R.EPS0 * R.C0
+    n = 50
+    tab = X.waveform_table("gaussian_derivative", n, t0=15, spread=5)
+    srcs = [(2, (10, 9, 8), tab), (0, (5, 12, 6), 0.5 * tab)]
+    st = X.run(size, dx, dt, "cpml", eps, srcs, n, sigma=sigma, npml=4, dtype=np.float32)
+    o = C.COracle(size, dx, dt, "cpml", eps, sigma=sigma, npml=4, dtype=np.float32)
+    o.set_sources(srcs)
+    o.run(0, n, threads=3)
+    for a, b in zip(o.fields_ref(), st.f):
+        np.testing.assert_array_equal(a, b)
+    assert np.abs(st.f[2]).max() > 0
+
+
+def test_cpml_absorbs_better_than_sponge():
+    """Physics pin of the CPML restatement: a pulse radiated in a 40^3 box
+    leaves far less energy behind with the 8-cell CPML than with the
+    reference's 8-cell sponge, and decays monotonically once the source is
+    off (no instability)."""
+    size = (40, 40, 40)
+    dx = 1.0
+    dt = 0.5 / math.sqrt(3) / R.C0
+    n = 260
+    tab = X.waveform_table("gaussian_derivative", n, t0=30, spread=8)
+    eps = np.ones(size)
+    res = {}
+    for b in ("cpml", "absorbing"):
+        o = C.COracle(size, dx, dt, b, eps, npml=8, dtype=np.float64)
+        o.set_sources([(2, (20, 20, 20), tab)])
+        energies = []
+        for q in range(0, n, 20):
+            o.run(q, 20, threads=8)
+            energies.append(sum(float((a * a).sum()) for a in o.f[:3]))
+        res[b] = energies
+    peak = max(res["cpml"])
+    assert res["cpml"][-1] < 1e-3 * peak
+    assert res["cpml"][-1] < 0.2 * res["absorbing"][-1]
+    tail = res["cpml"][5:]
+    assert all(b <= a * 1.05 for a, b in zip(tail, tail[1:]))
+
+
+def test_cpml_coefficients_closed_form():
+    n, p, dx = 30, 6, 1.0
+    dt = 0.5 / math.sqrt(3) / R.C0
+    t = X.cpml_coeffs(n, p, dx, dt, m=3.0, R0=1e-6)
+    b, c, kinv = t["e"]
+    smax = -(4.0) * math.log(1e-6) / (2 * X.ETA0 * p * dx)
+    # outermost E node (depth 1) on both sides
+    assert b[0] == pytest.approx(math.exp(-smax * dt / R.EPS0), rel=1e-12)
+    assert b[n - 1] == pytest.approx(b[0], rel=1e-12)
+    assert c[0] == pytest.approx(b[0] - 1.0, rel=1e-12)  # kappa=1, alpha=0: c = b-1
+    assert np.all(kinv == 1.0)
+    assert np.all(b[p:n - p] == 1.0) and np.all(c[p:n - p] == 0.0)
+    bh, ch, _ = t["h"]
+    d = (p - 0.5) / p  # low-side H node at i=0 sits at depth (p-1/2)/p
+    assert bh[0] == pytest.approx(math.exp(-smax * d**3 * dt / R.EPS0), rel=1e-12)
+    # in normalised units sigma_max dt/eps0 = -(m+1) ln(R0) S / (2 npml)
+    S = R.C0 * dt / dx
+    assert -math.log(b[0]) == pytest.approx(-(4.0) * math.log(1e-6) * S / (2 * p), rel=1e-12)
