@@ -85,6 +85,8 @@ def main() -> None:
     ap.add_argument("--workload", default="c3_512")
     ap.add_argument("--cpu-budget", type=float, default=20.0,
                     help="seconds of CPU work for the cpu_baseline sample (0 = skip)")
+    ap.add_argument("--mode", default="fused", choices=["fused", "two_pass"],
+                    help="one fused launch per step (default) or the H and E half-step kernels")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain copy of the field bytes (counter calibration)")
     args = ap.parse_args()
@@ -125,6 +127,8 @@ def main() -> None:
     K = args.steps if args.steps is not None else default_steps
     Wm = args.warmup
     solver = grid.build_solver(W.dt_for(1.0), K + Wm)
+    if args.mode == "two_pass":
+        solver.fused = False
     torch.cuda.synchronize()
     setup_s = time.time() - t_setup
     nx, ny, nz = grid.size
@@ -137,32 +141,49 @@ def main() -> None:
     torch.cuda.synchronize()
 
     stream = torch.cuda.current_stream()
-    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(K)]
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    start.record(stream)
-    for q in range(K):
-        ev[q][0].record(stream)
-        solver.step_h(stream)
-        ev[q][1].record(stream)
-        solver.step_e(Wm + q, stream)
-        ev[q][2].record(stream)
-    stop.record(stream)
-    torch.cuda.synchronize()
-    total_ms = start.elapsed_time(stop)
-    h_ms = sum(ev[q][0].elapsed_time(ev[q][1]) for q in range(K)) / K
-    e_ms = sum(ev[q][1].elapsed_time(ev[q][2]) for q in range(K)) / K
+    psi_bytes = W.psi_bytes_per_step(grid.size, npml) if npml else 0
+    if args.mode == "fused":
+        # one launch per step: HIP events around every launch on this stream
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(K + 1)]
+        torch.cuda.synchronize()
+        start.record(stream)
+        ev[0].record(stream)
+        for q in range(K):
+            solver.step_fused(Wm + q, stream)
+            ev[q + 1].record(stream)
+        stop.record(stream)
+        torch.cuda.synchronize()
+        total_ms = start.elapsed_time(stop)
+        launch_ms = sum(ev[q].elapsed_time(ev[q + 1]) for q in range(K)) / K
+        dom_name = "yee_fused_kernel"
+        dom_bytes = (48 + 4 + (4 if lossy else 0)) * cells + psi_bytes
+        dom_ms = launch_ms
+        step_bytes = dom_bytes
+        h_ms = e_ms = None
+    else:
+        ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(K)]
+        torch.cuda.synchronize()
+        start.record(stream)
+        for q in range(K):
+            ev[q][0].record(stream)
+            solver.step_h(stream)
+            ev[q][1].record(stream)
+            solver.step_e(Wm + q, stream)
+            ev[q][2].record(stream)
+        stop.record(stream)
+        torch.cuda.synchronize()
+        total_ms = start.elapsed_time(stop)
+        h_ms = sum(ev[q][0].elapsed_time(ev[q][1]) for q in range(K)) / K
+        e_ms = sum(ev[q][1].elapsed_time(ev[q][2]) for q in range(K)) / K
+        h_bytes = 36 * cells + psi_bytes // 2
+        e_bytes = (44 if lossy else 40) * cells + psi_bytes // 2
+        dom_name, dom_bytes, dom_ms = ("yee_e_kernel", e_bytes, e_ms) if e_ms >= h_ms else \
+            ("yee_h_kernel", h_bytes, h_ms)
+        step_bytes = h_bytes + e_bytes
     ms_step = total_ms / K
     gcells = cells * K / (total_ms / 1e3) / 1e9
-
-    # algorithmic bytes per launch
-    psi_half = W.psi_bytes_per_step(grid.size, npml) // 2 if npml else 0
-    h_bytes = 36 * cells + psi_half
-    e_bytes = (44 if lossy else 40) * cells + psi_half
-    dom_name, dom_bytes, dom_ms = ("yee_e_kernel", e_bytes, e_ms) if e_ms >= h_ms else \
-        ("yee_h_kernel", h_bytes, h_ms)
     achieved = dom_bytes / (dom_ms / 1e3) / 1e9
-    step_bytes = h_bytes + e_bytes
     finite = bool(all(torch.isfinite(f).all().item() for f in solver.fields))
 
     cpu = None
@@ -206,7 +227,9 @@ def main() -> None:
             "ms_per_launch": round(dom_ms, 4),
         },
         "step": {
-            "h_ms": round(h_ms, 4), "e_ms": round(e_ms, 4),
+            "mode": args.mode,
+            "h_ms": None if h_ms is None else round(h_ms, 4),
+            "e_ms": None if e_ms is None else round(e_ms, 4),
             "bytes": step_bytes,
             "GBps": round(step_bytes / (ms_step / 1e3) / 1e9, 1),
             "frac": round(step_bytes / (ms_step / 1e3) / 1e9 / HBM_PEAK_GBS, 4),

@@ -105,6 +105,17 @@ int yee_set_sources(YeePlan* plan, int32_t nsrc, const int32_t* comp,
 int yee_run(YeePlan* plan, float* const* fields, float* psi, int32_t n0,
             int32_t nsteps, void* stream);
 
+/* Fused single-pass step n (same arithmetic, bit-for-bit the same result as
+ * yee_step_h + yee_step_e): reads fields_in / psi_in, writes fields_out /
+ * psi_out (two distinct buffer sets).  One launch per step; 52 B of field
+ * traffic per cell-update (56 B with Ca) instead of 76 / 80 B.           */
+int yee_step_fused(YeePlan* plan, float* const* fields_in, float* const* fields_out,
+                   float* psi_in, float* psi_out, int32_t n, void* stream);
+/* Steps n0 .. n0+nsteps-1 alternating A->B, B->A, ...; the state ends in A
+ * when nsteps is even, in B when odd.                                     */
+int yee_run_fused(YeePlan* plan, float* const* fields_a, float* const* fields_b,
+                  float* psi_a, float* psi_b, int32_t n0, int32_t nsteps, void* stream);
+
 /* The two half-steps of step n separately (same semantics as yee_run). */
 int yee_step_h(YeePlan* plan, float* const* fields, float* psi, void* stream);
 int yee_step_e(YeePlan* plan, float* const* fields, float* psi, int32_t n,

@@ -60,6 +60,12 @@ EXPORTS = {
     "yee_copy_probe": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
                                       ctypes.c_void_p]),
     "yee_last_error": (ctypes.c_char_p, []),
+    "yee_step_fused": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                      ctypes.c_void_p]),
+    "yee_run_fused": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                     ctypes.c_int32, ctypes.c_void_p]),
     "yee_adjoint_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                         ctypes.c_void_p, ctypes.c_void_p]),
     "yee_probe_gather": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
@@ -188,6 +194,18 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+def run_fused(plan: "Plan", fa, fb, psi_a, psi_b, n0, nsteps, stream=None) -> None:
+    check(lib().yee_run_fused(plan.handle, field_array(fa), field_array(fb), ptr(psi_a) or None,
+                              ptr(psi_b) or None, int(n0), int(nsteps), stream_handle(stream)),
+          "yee_run_fused")
+
+
+def step_fused(plan: "Plan", fin, fout, psi_in, psi_out, n, stream=None) -> None:
+    check(lib().yee_step_fused(plan.handle, field_array(fin), field_array(fout),
+                               ptr(psi_in) or None, ptr(psi_out) or None, int(n),
+                               stream_handle(stream)), "yee_step_fused")
 
 
 def box_array(box):

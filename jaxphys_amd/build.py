@@ -28,7 +28,7 @@ HIPFLAGS = [
 ]
 
 LIBS = {
-    "libyee_hip.so": ["csrc/yee.hip", "csrc/yee_adjoint.hip"],
+    "libyee_hip.so": ["csrc/yee.hip", "csrc/yee_fused.hip", "csrc/yee_adjoint.hip"],
 }
 
 

@@ -345,21 +345,17 @@ void launch_e_mode(const YeePlan* p, const GridArgs& g, const SrcArgs& s, hipStr
 
 int launch_e(const YeePlan* p, const GridArgs& g, int n, hipStream_t st) {
   SrcArgs s;
-  std::memset(&s, 0, sizeof(s));
-  if (p->nsrc > 0 && (n < 0 || n >= p->nsteps_table))
-    return fail(YEE_EINVAL, "step " + std::to_string(n) + " outside the source table (" +
-                                std::to_string(p->nsteps_table) + " steps)");
-  s.nsrc = p->nsrc < YEE_FUSED_SOURCES ? p->nsrc : YEE_FUSED_SOURCES;
-  for (int q = 0; q < s.nsrc; ++q) {
-    s.comp[q] = p->comp[q];
-    s.idx[q] = p->idx[q];
-    s.val[q] = p->table[(size_t)n * p->nsrc + q];
-  }
+  int rc = make_src_args(p, n, s);
+  if (rc) return rc;
   if (p->d.ca)
     launch_e_mode<true>(p, g, s, st);
   else
     launch_e_mode<false>(p, g, s, st);
   YEE_HIP_CHECK(hipGetLastError());
+  return launch_inject(p, g, n, st);
+}
+
+int launch_inject(const YeePlan* p, const GridArgs& g, int n, hipStream_t st) {
   if (p->nsrc > YEE_FUSED_SOURCES) {
     yee_inject_kernel<<<1, 64, 0, st>>>(g.f[0], g.f[1], g.f[2], p->d_comp, p->d_idx, p->d_table,
                                         p->nsrc, YEE_FUSED_SOURCES, n);

@@ -147,7 +147,23 @@ inline int check_fields(const YeePlan* p, float* const* fields, float* psi) {
   return YEE_OK;
 }
 
+// sources of step n fused into an E update (the first YEE_FUSED_SOURCES)
+inline int make_src_args(const YeePlan* p, int n, SrcArgs& s) {
+  std::memset(&s, 0, sizeof(s));
+  if (p->nsrc > 0 && (n < 0 || n >= p->nsteps_table))
+    return fail(YEE_EINVAL, "step " + std::to_string(n) + " outside the source table (" +
+                                std::to_string(p->nsteps_table) + " steps)");
+  s.nsrc = p->nsrc < YEE_FUSED_SOURCES ? p->nsrc : YEE_FUSED_SOURCES;
+  for (int q = 0; q < s.nsrc; ++q) {
+    s.comp[q] = p->comp[q];
+    s.idx[q] = p->idx[q];
+    s.val[q] = p->table[(size_t)n * p->nsrc + q];
+  }
+  return YEE_OK;
+}
+
 // forward launches (yee.hip)
+int launch_inject(const YeePlan* p, const GridArgs& g, int n, hipStream_t st);
 int launch_h(const YeePlan* p, const GridArgs& g, hipStream_t st);
 int launch_e(const YeePlan* p, const GridArgs& g, int n, hipStream_t st);
 

@@ -83,7 +83,7 @@ class YeeSolver:
 
     def __init__(self, size, dx: float, dt: float, boundary: str, pml_layers: int,
                  eps_r: torch.Tensor, sigma: torch.Tensor | None = None,
-                 cpml: dict | None = None, device=None):
+                 cpml: dict | None = None, device=None, fused: bool = True):
         self.device = torch.device(device) if device is not None else _native.require_device()
         _native.lib()
         self.nx, self.ny, self.nz = (int(s) for s in size)
@@ -136,9 +136,15 @@ class YeeSolver:
         else:  # any other string: the reference's non-periodic, undamped update
             desc.boundary = _native.BOUNDARY_WALLS
         self.plan = _native.Plan(desc, keep=[self.cb, self.ca, self.sponge, self.cpml])
-        self.fields = [torch.zeros(shape, dtype=torch.float32, device=dev) for _ in range(6)]
         npsi = self.plan.psi_floats()
-        self.psi = torch.zeros(max(npsi, 1), dtype=torch.float32, device=dev) if npsi else None
+        # two state buffers: the fused kernel reads one and writes the other
+        self.fused = bool(fused)
+        nbuf = 2 if self.fused else 1
+        self._bufs = [[torch.zeros(shape, dtype=torch.float32, device=dev) for _ in range(6)]
+                      for _ in range(nbuf)]
+        self._psis = [torch.zeros(npsi, dtype=torch.float32, device=dev) if npsi else None
+                      for _ in range(nbuf)]
+        self._cur = 0
         self.n_sources = 0
         self.table_steps = 0
 
@@ -191,9 +197,39 @@ class YeeSolver:
         self.source_positions = [tuple(p) for p in positions]
 
     # -- stepping -------------------------------------------------------
+    @property
+    def fields(self):
+        """The six current field tensors (Ex, Ey, Ez, Hx, Hy, Hz)."""
+        return self._bufs[self._cur]
+
+    @property
+    def psi(self):
+        """The current CPML psi buffer (None unless CPML)."""
+        return self._psis[self._cur]
+
     def run(self, n0: int, nsteps: int, stream=None) -> None:
+        """Steps n0 .. n0+nsteps-1: one fused launch per step (default) or the
+        two half-step launches."""
+        if nsteps <= 0:
+            return
+        if self.fused:
+            a, b = self._cur, 1 - self._cur
+            _native.run_fused(self.plan, self._bufs[a], self._bufs[b], self._psis[a],
+                              self._psis[b], n0, nsteps, stream)
+            self._cur = a if nsteps % 2 == 0 else b
+        else:
+            self.plan.run(self.fields, self.psi, n0, nsteps, stream)
+
+    def run_two_pass(self, n0: int, nsteps: int, stream=None) -> None:
+        """Steps with the separate H and E kernels (same results bit for bit)."""
         if nsteps > 0:
             self.plan.run(self.fields, self.psi, n0, nsteps, stream)
+
+    def step_fused(self, n: int, stream=None) -> None:
+        a, b = self._cur, 1 - self._cur
+        _native.step_fused(self.plan, self._bufs[a], self._bufs[b], self._psis[a],
+                           self._psis[b], n, stream)
+        self._cur = b
 
     def step_h(self, stream=None) -> None:
         self.plan.step_h(self.fields, self.psi, stream)
@@ -202,10 +238,12 @@ class YeeSolver:
         self.plan.step_e(self.fields, self.psi, n, stream)
 
     def reset(self) -> None:
-        for f in self.fields:
-            f.zero_()
-        if self.psi is not None:
-            self.psi.zero_()
+        for buf, ps in zip(self._bufs, self._psis):
+            for f in buf:
+                f.zero_()
+            if ps is not None:
+                ps.zero_()
+        self._cur = 0
 
     def energy_fp64(self):
         """float64 (sum eps_r |E|^2, sum |H|^2) as Python floats."""
