@@ -77,6 +77,108 @@ def cpu_baseline(solver, budget_s: float):
                       f"workload in {secs:.1f} s (C/OpenMP oracle, {cpu_model})"}
 
 
+def adjoint_bench(args, dev):
+    """160^3, 12-cell CPML, 800 steps; 96^3 design region eps_r U[1,4] in
+    vacuum; z dipole 20 cells below it; J = time-integrated Ez^2 on a 32x32
+    patch 20 cells above it; checkpoints every 100 steps.  Reports the cost
+    of value+gradient relative to one plain forward run and central finite
+    differences (h=1e-3) on 16 seeded design cells."""
+    import numpy as np
+    import torch
+
+    from jaxphys_amd import workloads as W
+    from jaxphys_amd.adjoint import AdjointProblem
+
+    n, steps = 160, args.steps or 800
+    g, box, probes, eps_box = W.c4_design(n, dev)
+    prob = AdjointProblem(g, steps, W.dt_for(1.0), box, probes,
+                          checkpoint_every=args.checkpoint_every)
+    # plain forward (the run being differentiated), timed alone
+    plain = g.build_solver(W.dt_for(1.0), steps)
+    eps_full = prob.background.clone()
+    eps_full[prob.region] = eps_box.permute(2, 1, 0)
+    plain.set_eps(eps_full)
+    plain.run(0, 5)
+    plain.reset()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plain.run(0, steps)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    del plain
+    prob.value_and_grad(eps_box)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    J = prob.forward(eps_box)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    grad = prob.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    rng = np.random.default_rng(42)
+    h = 1e-3
+    fd_err = []
+    for _ in range(args.fd_cells):
+        c = tuple(int(rng.integers(0, s)) for s in eps_box.shape)
+        ep = eps_box.clone(); ep[c] += h  # noqa: E702
+        em = eps_box.clone(); em[c] -= h  # noqa: E702
+        fd = (prob.forward(ep) - prob.forward(em)) / (2 * h)
+        gv = float(grad[c])
+        fd_err.append({"cell": c, "adjoint": gv, "fd": fd})
+    scale = float(grad.abs().max())
+    worst = max((abs(e["adjoint"] - e["fd"]) / scale for e in fd_err), default=None)
+    return {
+        "metric": "adjoint dJ/deps_r cost (x one forward run)",
+        "value": round((t2 - t0) / t_fwd, 3), "unit": "x forward", "higher_is_better": False,
+        "n_gpus": 1, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "adjoint_c4", "grid": [n, n, n], "steps": steps, "npml": 12,
+                   "design_box": list(box), "checkpoint_every": args.checkpoint_every,
+                   "probes": len(probes)},
+        "forward_s": round(t_fwd, 4), "value_and_grad_s": round(t2 - t0, 4),
+        "checkpointed_forward_s": round(t1 - t0, 4), "adjoint_s": round(t2 - t1, 4),
+        "J": J, "fd_check": {"h": h, "cells": len(fd_err),
+                              "max_abs_err_over_max_grad": worst, "samples": fd_err[:4]},
+    }
+
+
+def closure_bench(args, dev):
+    """Learned closure: train on fine-run/coarse pairs, evaluate on held-out
+    seeds; reports the corrected coarse error and the wall-time ratio."""
+    import numpy as np
+
+    from jaxphys_amd import closure as CL
+
+    nf = args.closure_n
+    train_seeds = list(range(args.closure_train))
+    test_seeds = list(range(1000, 1000 + args.closure_test))
+    t0 = time.time()
+    net, hist = CL.train_closure(dev, train_seeds, n_fine=nf, n_coarse=nf * 400 // 256,
+                                 epochs=args.closure_epochs, crop=min(64, nf // 2 - 16),
+                                 lr=args.closure_lr,
+                                 log=lambda m: print("[closure]", m, file=sys.stderr, flush=True))
+    t_train = time.time() - t0
+    res = [CL.evaluate_closure(net, s, n_fine=nf, n_coarse=nf * 400 // 256) for s in test_seeds]
+    err_c = [r["err_closure"] for r in res]
+    err_0 = [r["err_coarse"] for r in res]
+    sp = [r["speedup"] for r in res]
+    return {
+        "metric": "closure: coarse+net field error vs fine FDTD",
+        "value": round(float(np.mean(err_c)), 5), "unit": "rel-L2 (E, interior)",
+        "higher_is_better": False, "n_gpus": 1, "dtype": "bf16 net / f32 FDTD",
+        "data": "synthetic",
+        "config": {"workload": "closure_c5", "fine": [nf] * 3, "coarse": [nf // 2] * 3,
+                   "train_seeds": len(train_seeds), "test_seeds": len(test_seeds),
+                   "epochs": args.closure_epochs},
+        "err_coarse_no_closure": round(float(np.mean(err_0)), 5),
+        "err_closure_max": round(float(np.max(err_c)), 5),
+        "speedup_vs_fine": round(float(np.mean(sp)), 3),
+        "t_fine_s": round(float(np.mean([r["t_fine_s"] for r in res])), 4),
+        "t_closure_s": round(float(np.mean([r["t_closure_s"] for r in res])), 4),
+        "t_coarse_s": round(float(np.mean([r["t_coarse_s"] for r in res])), 4),
+        "train_s": round(t_train, 1), "final_loss": float(np.mean(hist[-40:])) if hist else None,
+    }
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,6 +189,13 @@ def main() -> None:
                     help="seconds of CPU work for the cpu_baseline sample (0 = skip)")
     ap.add_argument("--mode", default="fused", choices=["fused", "two_pass"],
                     help="one fused launch per step (default) or the H and E half-step kernels")
+    ap.add_argument("--checkpoint-every", type=int, default=100)
+    ap.add_argument("--fd-cells", type=int, default=16)
+    ap.add_argument("--closure-n", type=int, default=256)
+    ap.add_argument("--closure-train", type=int, default=64)
+    ap.add_argument("--closure-test", type=int, default=16)
+    ap.add_argument("--closure-epochs", type=int, default=2)
+    ap.add_argument("--closure-lr", type=float, default=1e-3)
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain copy of the field bytes (counter calibration)")
     args = ap.parse_args()
@@ -120,6 +229,13 @@ def main() -> None:
         torch.cuda.synchronize()
         ms = e0.elapsed_time(e1) / reps
         print(json.dumps({"copy_probe_bytes": 8 * n, "ms": ms, "GBps": 8 * n / ms / 1e6}))
+        return
+
+    if args.workload == "adjoint_c4":
+        print(json.dumps(adjoint_bench(args, dev)), flush=True)
+        return
+    if args.workload == "closure_c5":
+        print(json.dumps(closure_bench(args, dev)), flush=True)
         return
 
     t_setup = time.time()

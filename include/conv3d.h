@@ -1,0 +1,81 @@
+/*
+ * jaxphys_amd -- C ABI of the BF16-MFMA 3D convolution library
+ * (libconv3d_hip.so) used by the learned dispersion closure.
+ *
+ * No reference counterpart: the reference (sushaan-k/jaxphys) has no neural
+ * network or closure code.  These entry points implement the 3x3x3, stride
+ * 1, zero-padded ("same") convolution of a small channels-last network as
+ * implicit GEMMs on gfx950 matrix cores (v_mfma_f32_16x16x32_bf16) with
+ * float32 accumulation, for inference and for training (input and weight
+ * gradients).
+ *
+ * Layouts (all device pointers, caller-owned):
+ *  activations: bf16, channels last, [nz][ny][nx][C] with C = the padded
+ *               channel count (a multiple of 8);
+ *  weights:     bf16, [cout_pad][kpad] with k = tap*cin_pad + c,
+ *               tap = (dz+1)*9 + (dy+1)*3 + (dx+1), kpad = 27*cin_pad
+ *               rounded up to a multiple of 32 (the tail is zero);
+ *               cout_pad a multiple of 16;
+ *  bias:        float32 [cout] (may be NULL);
+ *  weight grad: float32 [cout_pad][kpad].
+ * Return codes as in yee.h: 0 ok, -1 bad argument, -2 HIP error.
+ */
+#ifndef JAXPHYS_AMD_CONV3D_H
+#define JAXPHYS_AMD_CONV3D_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* epilogue of conv3d_fwd */
+#define CONV_EPI_GELU 0      /* z = acc + bias (bf16 to z_out if given); y = gelu(z) bf16 */
+#define CONV_EPI_LINEAR_F32 1 /* y = acc + bias as float32 [..][cout_real] */
+#define CONV_EPI_GELU_BWD 2  /* y = acc * gelu'(aux) bf16 (aux: bf16 pre-activation) */
+#define CONV_EPI_LINEAR_BF16 3 /* y = acc + bias as bf16 */
+
+typedef struct ConvDesc {
+  int32_t nx, ny, nz;
+  int32_t cin_pad;    /* channels of x (multiple of 8) */
+  int32_t cout;       /* real output channels */
+  int32_t cout_pad;   /* multiple of 16 */
+  int32_t y_cpad;     /* channel stride of the output tensor (>= cout) */
+} ConvDesc;
+
+/* y = epilogue(conv(x, w) + bias) */
+int conv3d_fwd(const ConvDesc* d, const void* x, const void* w, const float* bias, int32_t epi,
+               void* z_out, const void* aux, void* y, void* stream);
+
+/* dw[co][k] = sum_p dy[p][co] * x[p + off(tap(k))][c(k)]; db[co] = sum_p dy[p][co].
+ * dy: bf16 channels last with channel stride dy_cpad (>= cout).  `work` is a
+ * float32 device buffer of conv3d_wgrad_work_floats(d) floats (deterministic
+ * two-stage reduction); dw, db are overwritten.                          */
+int64_t conv3d_wgrad_work_floats(const ConvDesc* d);
+int conv3d_wgrad(const ConvDesc* d, const void* x, const void* dy, int32_t dy_cpad, float* work,
+                 float* dw, float* db, void* stream);
+
+/* Weight transform for the input gradient: wt[ci][tap'*cout_pad_in + co] =
+ * w[co][(26-tap')*cin_pad + ci]  (spatially flipped, channels swapped), so
+ * that dx = conv3d_fwd(dy, wt).  wt is [cin_pad rounded to 16][kpad_t] with
+ * kpad_t = 27*cout_pad_in rounded up to 32, cout_pad_in = dy channel stride
+ * rounded up to 8.                                                        */
+int conv3d_flip_weights(const ConvDesc* d, const void* w, int32_t dy_cpad, void* wt, void* stream);
+
+/* Pack the six Yee fields and eps_r (float32, x fastest, (nz,ny,nx)) into a
+ * bf16 channels-last tensor with 8 channels: c = field / scale[c] for
+ * c < 6, eps channel (eps - eps_shift) / eps_scale, channel 7 = 0.       */
+int conv3d_pack_fields(int32_t nx, int32_t ny, int32_t nz, float* const* fields, const float* eps,
+                       const float* scale, float eps_shift, float eps_scale, void* out, void* stream);
+/* fields[c] += gain * scale[c] * y[..][c], y float32 channels last [..][ystride] */
+int conv3d_apply_correction(int32_t nx, int32_t ny, int32_t nz, float* const* fields,
+                            const float* y, int32_t ystride, const float* scale, float gain,
+                            void* stream);
+
+const char* conv3d_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* JAXPHYS_AMD_CONV3D_H */

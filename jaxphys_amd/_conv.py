@@ -1,0 +1,111 @@
+"""ctypes binding of libconv3d_hip.so (C ABI in include/conv3d.h)."""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+import torch  # noqa: F401  (load order: torch's HIP runtime first)
+
+from jaxphys_amd._native import field_array, ptr, stream_handle
+from jaxphys_amd.exceptions import ConfigurationError, NativeLibraryError, SimulationError
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+EPI_GELU, EPI_LINEAR_F32, EPI_GELU_BWD, EPI_LINEAR_BF16 = 0, 1, 2, 3
+
+
+class ConvDesc(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+                ("cin_pad", ctypes.c_int32), ("cout", ctypes.c_int32),
+                ("cout_pad", ctypes.c_int32), ("y_cpad", ctypes.c_int32)]
+
+
+EXPORTS = {
+    "conv3d_fwd": (ctypes.c_int, [ctypes.POINTER(ConvDesc), ctypes.c_void_p, ctypes.c_void_p,
+                                  ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
+                                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "conv3d_wgrad_work_floats": (ctypes.c_int64, [ctypes.POINTER(ConvDesc)]),
+    "conv3d_wgrad": (ctypes.c_int, [ctypes.POINTER(ConvDesc), ctypes.c_void_p, ctypes.c_void_p,
+                                    ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
+                                    ctypes.c_void_p, ctypes.c_void_p]),
+    "conv3d_flip_weights": (ctypes.c_int, [ctypes.POINTER(ConvDesc), ctypes.c_void_p,
+                                           ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p]),
+    "conv3d_pack_fields": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                          ctypes.c_float, ctypes.c_float, ctypes.c_void_p,
+                                          ctypes.c_void_p]),
+    "conv3d_apply_correction": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                               ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                               ctypes.c_void_p, ctypes.c_float, ctypes.c_void_p]),
+    "conv3d_last_error": (ctypes.c_char_p, []),
+}
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(HERE, "libconv3d_hip.so")
+        if not os.path.exists(path):
+            raise NativeLibraryError(f"{path} is not built; run `python -m jaxphys_amd.build`")
+        h = ctypes.CDLL(path)
+        for name, (res, args) in EXPORTS.items():
+            fn = getattr(h, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = h
+    return _LIB
+
+
+def check(rc: int, what: str) -> None:
+    if rc == 0:
+        return
+    msg = lib().conv3d_last_error().decode(errors="replace")
+    if rc == -1:
+        raise ConfigurationError(f"{what}: {msg}")
+    raise SimulationError(f"{what}: {msg}")
+
+
+def kpad_of(cin_pad: int) -> int:
+    return ((27 * cin_pad + 31) // 32) * 32
+
+
+def desc(shape_xyz, cin_pad, cout, cout_pad, y_cpad) -> ConvDesc:
+    nx, ny, nz = shape_xyz
+    return ConvDesc(nx, ny, nz, cin_pad, cout, cout_pad, y_cpad)
+
+
+def fwd(d: ConvDesc, x, w, bias, epi, y, z_out=None, aux=None, stream=None) -> None:
+    check(lib().conv3d_fwd(ctypes.byref(d), ptr(x), ptr(w), ptr(bias) or None, int(epi),
+                           ptr(z_out) or None, ptr(aux) or None, ptr(y), stream_handle(stream)),
+          "conv3d_fwd")
+
+
+def wgrad_work_floats(d: ConvDesc) -> int:
+    return int(lib().conv3d_wgrad_work_floats(ctypes.byref(d)))
+
+
+def wgrad(d: ConvDesc, x, dy, dy_cpad, work, dw, db=None, stream=None) -> None:
+    check(lib().conv3d_wgrad(ctypes.byref(d), ptr(x), ptr(dy), int(dy_cpad), ptr(work), ptr(dw),
+                             ptr(db) or None, stream_handle(stream)), "conv3d_wgrad")
+
+
+def flip_weights(d: ConvDesc, w, dy_cpad, wt, stream=None) -> None:
+    check(lib().conv3d_flip_weights(ctypes.byref(d), ptr(w), int(dy_cpad), ptr(wt),
+                                    stream_handle(stream)), "conv3d_flip_weights")
+
+
+def pack_fields(shape_xyz, fields, eps, scale, eps_shift, eps_scale, out, stream=None) -> None:
+    nx, ny, nz = shape_xyz
+    check(lib().conv3d_pack_fields(nx, ny, nz, field_array(fields), ptr(eps), ptr(scale),
+                                   float(eps_shift), float(eps_scale), ptr(out),
+                                   stream_handle(stream)), "conv3d_pack_fields")
+
+
+def apply_correction(shape_xyz, fields, y, ystride, scale, gain=1.0, stream=None) -> None:
+    nx, ny, nz = shape_xyz
+    check(lib().conv3d_apply_correction(nx, ny, nz, field_array(fields), ptr(y), int(ystride),
+                                        ptr(scale), float(gain), stream_handle(stream)),
+          "conv3d_apply_correction")

@@ -1,0 +1,529 @@
+// jaxphys_amd: BF16 matrix-core (MFMA) 3x3x3 convolutions for the learned
+// dispersion closure, gfx950.  C ABI in include/conv3d.h.  No reference
+// counterpart (the reference has no neural network code).
+//
+// Forward (and input gradient, via spatially flipped / channel-swapped
+// weights) is an implicit GEMM  D[co][p] = sum_k W[co][k] X[p + off(k)][c(k)]
+// with M = output channels (16-row tiles), N = output positions (16
+// consecutive x per tile), K = 27 taps x input channels (32 per MFMA).
+// One v_mfma_f32_16x16x32_bf16 per (M tile, N tile, K step); float32
+// accumulation.  The layer's weights are staged once per block in LDS; each
+// wave owns one y row of 16*NT x positions and marches a chunk of z planes;
+// input fragments are 16-byte channel-contiguous loads (channels last).
+// Epilogues fuse bias, exact-erf GELU, the GELU derivative of the backward
+// pass and the float32 output of the last layer.
+//
+// Weight gradient: dW[co][k] = sum_p dY[p][co] X[p + off(k)][c(k)], an MFMA
+// GEMM with K = positions.  Each block reduces a set of 32-position row
+// segments; dY and the 3x3 neighbourhood rows of X are staged transposed in
+// LDS so that fragments are 8 consecutive positions; per-block partial sums
+// go to a work buffer reduced by a second kernel in fixed order.
+//
+// MFMA register layout used (16x16x32 bf16): lane t holds A[t%16][8(t/16)+v]
+// and B[8(t/16)+v][t%16] (v = 0..7) and D[4(t/16)+v][t%16] (v = 0..3).
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <string>
+
+#include "../../include/conv3d.h"
+
+namespace conv {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& m) {
+  g_err = m;
+  return code;
+}
+
+#define CONV_CHECK(expr)                                                      \
+  do {                                                                        \
+    hipError_t e_ = (expr);                                                   \
+    if (e_ != hipSuccess) return fail(-2, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short u16;
+
+__device__ __forceinline__ float bf2f(u16 h) { return __uint_as_float((unsigned)h << 16); }
+__device__ __forceinline__ u16 f2bf(float f) {  // round to nearest even
+  unsigned u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (u16)(u >> 16);  // inf / nan
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (u16)(u >> 16);
+}
+
+__device__ __forceinline__ bf16x8 load8(const u16* p) {
+  uint4 v = *reinterpret_cast<const uint4*>(p);
+  bf16x8 r;
+  __builtin_memcpy(&r, &v, 16);
+  return r;
+}
+
+__device__ __forceinline__ float gelu(float z) {
+  return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f));
+}
+__device__ __forceinline__ float gelu_grad(float z) {
+  return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) +
+         z * 0.39894228040143268f * expf(-0.5f * z * z);
+}
+
+struct ConvArgs {
+  int nx, ny, nz, cin, cout, cout_pad, ycp, kpad, zchunk;
+  const u16* x;
+  const u16* w;
+  const float* bias;
+  u16* zout;
+  const u16* aux;
+  void* y;
+};
+
+template <int EPI, int NT, int MT>
+__global__ __launch_bounds__(256) void conv_fwd_kernel(ConvArgs a) {
+  extern __shared__ u16 wsh[];  // [cout_pad][kpad + 8]
+  const int ldw = a.kpad + 8;
+  // stage weights
+  {
+    const int n16 = a.cout_pad * (a.kpad / 8);
+    for (int q = threadIdx.x; q < n16; q += 256) {
+      const int row = q / (a.kpad / 8), col = (q % (a.kpad / 8)) * 8;
+      *reinterpret_cast<uint4*>(wsh + row * ldw + col) =
+          *reinterpret_cast<const uint4*>(a.w + (long long)row * a.kpad + col);
+    }
+  }
+  __syncthreads();
+  const int t = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const int g = t >> 4;       // k group
+  const int r = t & 15;       // row / column within the tile
+  const int x0 = blockIdx.x * 16 * NT;
+  const int yy = blockIdx.y * 4 + wv;
+  const int z0 = blockIdx.z * a.zchunk;
+  const int z1 = min(z0 + a.zchunk, a.nz);
+  const int nx = a.nx, ny = a.ny, nz = a.nz, cin = a.cin;
+  const bool rowok = yy < ny;
+  for (int zz = z0; zz < z1; ++zz) {
+    f32x4 acc[MT][NT];
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+      for (int n = 0; n < NT; ++n) acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+    if (rowok) {
+      for (int k0 = 0; k0 < a.kpad; k0 += 32) {
+        const int k = k0 + 8 * g;
+        const int tap = k / cin;
+        const int c0 = k - tap * cin;
+        bf16x8 af[MT];
+#pragma unroll
+        for (int m = 0; m < MT; ++m) af[m] = load8(wsh + (m * 16 + r) * ldw + k);
+        const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+        const int sz = zz + dz, sy = yy + dy;
+        const bool pl = tap < 27 && sz >= 0 && sz < nz && sy >= 0 && sy < ny;
+        const u16* rowp = a.x + ((long long)(pl ? sz : 0) * ny + (pl ? sy : 0)) * nx * cin;
+#pragma unroll
+        for (int n = 0; n < NT; ++n) {
+          const int sx = x0 + n * 16 + r + dx;
+          bf16x8 bf;
+          if (pl && sx >= 0 && sx < nx) {
+            bf = load8(rowp + (long long)sx * cin + c0);
+          } else {
+            bf = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
+          }
+#pragma unroll
+          for (int m = 0; m < MT; ++m)
+            acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[m], bf, acc[m][n], 0, 0, 0);
+        }
+      }
+    }
+    if (!rowok) continue;
+    // epilogue: lane holds D[co = m*16 + 4g + v][pos = x0 + n*16 + r]
+#pragma unroll
+    for (int n = 0; n < NT; ++n) {
+      const int px = x0 + n * 16 + r;
+      if (px >= nx) continue;
+      const long long pos = ((long long)zz * ny + yy) * nx + px;
+#pragma unroll
+      for (int m = 0; m < MT; ++m) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int co = m * 16 + 4 * g + v;
+          if (co >= a.ycp) continue;
+          const bool real = co < a.cout;
+          float val = acc[m][n][v];
+          if (EPI == CONV_EPI_GELU) {
+            const float zv = real ? val + (a.bias ? a.bias[co] : 0.f) : 0.f;
+            const u16 zb = f2bf(zv);
+            if (a.zout) a.zout[pos * a.ycp + co] = zb;
+            // gelu of the bf16-rounded pre-activation (what backward sees)
+            reinterpret_cast<u16*>(a.y)[pos * a.ycp + co] = real ? f2bf(gelu(bf2f(zb))) : (u16)0;
+          } else if (EPI == CONV_EPI_LINEAR_F32) {
+            if (real) reinterpret_cast<float*>(a.y)[pos * a.ycp + co] =
+                val + (a.bias ? a.bias[co] : 0.f);
+          } else if (EPI == CONV_EPI_LINEAR_BF16) {
+            reinterpret_cast<u16*>(a.y)[pos * a.ycp + co] =
+                real ? f2bf(val + (a.bias ? a.bias[co] : 0.f)) : (u16)0;
+          } else {  // CONV_EPI_GELU_BWD
+            const float zv = bf2f(a.aux[pos * a.ycp + co]);
+            reinterpret_cast<u16*>(a.y)[pos * a.ycp + co] = real ? f2bf(val * gelu_grad(zv)) : (u16)0;
+          }
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// weight gradient
+// ---------------------------------------------------------------------------
+struct WgradArgs {
+  int nx, ny, nz, cin, cout_pad, dycp, kpad, nseg_x, segs_per_block, total_segs;
+  const u16* x;
+  const u16* dy;
+  float* part;  // [gridDim.x][cout_pad][kpad]
+};
+
+constexpr int kSeg = 32;  // positions per reduction step (one MFMA K)
+
+// Each block: 4 waves; wave w owns N tiles (k columns) n = w, w+4, ...;
+// all waves share the M tiles (output channels).  LDS: dY^T [cout_pad][32]
+// and X^T for the 9 neighbour rows [9][cin][kSeg + 2] of one segment.
+template <int MT>
+__global__ __launch_bounds__(256) void conv_wgrad_kernel(WgradArgs a) {
+  extern __shared__ u16 sh[];
+  u16* dyT = sh;                                   // [cout_pad][kSeg]
+  u16* xT = sh + a.cout_pad * kSeg;                // [9][cin][kSeg + 2]
+  const int xs = kSeg + 2;
+  const int t = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const int g = t >> 4, r = t & 15;
+  const int ntile = a.kpad / 16;
+  constexpr int kMaxTiles = 16;  // N tiles per wave (kpad <= 4*16*16 = 1024)
+  f32x4 acc[MT][kMaxTiles];
+#pragma unroll
+  for (int m = 0; m < MT; ++m)
+#pragma unroll
+    for (int q = 0; q < kMaxTiles; ++q) acc[m][q] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int cin = a.cin, nx = a.nx, ny = a.ny, nz = a.nz;
+  const int s_begin = blockIdx.x * a.segs_per_block;
+  const int s_end = min(s_begin + a.segs_per_block, a.total_segs);
+  for (int s = s_begin; s < s_end; ++s) {
+    const int sx0 = (s % a.nseg_x) * kSeg;
+    const int row = s / a.nseg_x;
+    const int yy = row % ny, zz = row / ny;
+    __syncthreads();
+    // stage dY^T: dyT[co][p] = dY[pos(sx0 + p)][co]
+    for (int q = threadIdx.x; q < a.cout_pad * kSeg; q += 256) {
+      const int co = q / kSeg, p = q % kSeg;
+      const int px = sx0 + p;
+      u16 v = 0;
+      if (px < nx && co < a.dycp) v = a.dy[(((long long)zz * ny + yy) * nx + px) * a.dycp + co];
+      dyT[co * kSeg + p] = v;
+    }
+    // stage X^T of the 9 neighbour rows, x from sx0-1 .. sx0+kSeg
+    for (int q = threadIdx.x; q < 9 * cin * xs; q += 256) {
+      const int rr = q / (cin * xs);
+      const int rem = q % (cin * xs);
+      const int c = rem / xs, p = rem % xs;
+      const int sz = zz + rr / 3 - 1, sy = yy + rr % 3 - 1, px = sx0 + p - 1;
+      u16 v = 0;
+      if (sz >= 0 && sz < nz && sy >= 0 && sy < ny && px >= 0 && px < nx)
+        v = a.x[(((long long)sz * ny + sy) * nx + px) * cin + c];
+      xT[(rr * cin + c) * xs + p] = v;
+    }
+    __syncthreads();
+    bf16x8 af[MT];
+#pragma unroll
+    for (int m = 0; m < MT; ++m) af[m] = load8(dyT + (m * 16 + r) * kSeg + 8 * g);
+#pragma unroll
+    for (int q = 0; q < kMaxTiles; ++q) {
+      const int nt = wv + 4 * q;
+      if (nt >= ntile) break;
+      const int k = nt * 16 + r;  // this lane's B column
+      const int tap = k / cin, c = k - tap * cin;
+      bf16x8 bf;
+      if (tap < 27) {
+        const int rr = (tap / 9) * 3 + (tap / 3) % 3;
+        const int dx = tap % 3 - 1;
+        const u16* src = xT + (rr * cin + c) * xs + 8 * g + dx + 1;
+        u16 tmp[8];
+#pragma unroll
+        for (int v = 0; v < 8; ++v) tmp[v] = src[v];
+        __builtin_memcpy(&bf, tmp, 16);
+      } else {
+        bf = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
+      }
+#pragma unroll
+      for (int m = 0; m < MT; ++m)
+        acc[m][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[m], bf, acc[m][q], 0, 0, 0);
+    }
+  }
+  // partial sums: D[co = m*16 + 4g + v][k = nt*16 + r]
+  float* out = a.part + (long long)blockIdx.x * a.cout_pad * a.kpad;
+#pragma unroll
+  for (int q = 0; q < kMaxTiles; ++q) {
+    const int nt = wv + 4 * q;
+    if (nt >= ntile) break;
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        out[(long long)(m * 16 + 4 * g + v) * a.kpad + nt * 16 + r] = acc[m][q][v];
+  }
+}
+
+__global__ void wgrad_reduce_kernel(const float* part, int nparts, long long n, float* dw) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float s = 0.f;
+  for (int b = 0; b < nparts; ++b) s += part[(long long)b * n + i];
+  dw[i] = s;
+}
+
+// db[co] = sum_p dY[p][co] (fixed order: per-block partials then a sum)
+__global__ __launch_bounds__(256) void bias_grad_partial(const u16* dy, int dycp, int cout,
+                                                        long long npos, float* part) {
+  __shared__ float sh[256];
+  for (int co = 0; co < cout; ++co) {
+    float s = 0.f;
+    for (long long p = (long long)blockIdx.x * 256 + threadIdx.x; p < npos;
+         p += (long long)gridDim.x * 256)
+      s += bf2f(dy[p * dycp + co]);
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+      if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x * cout + co] = sh[0];
+    __syncthreads();
+  }
+}
+
+__global__ void bias_grad_final(const float* part, int nb, int cout, float* db) {
+  const int co = threadIdx.x;
+  if (co >= cout) return;
+  float s = 0.f;
+  for (int b = 0; b < nb; ++b) s += part[b * cout + co];
+  db[co] = s;
+}
+
+__global__ void flip_weights_kernel(const u16* w, int cout, int cout_pad, int cin, int kpad,
+                                    int co_t, int kpad_t, int rows_t, u16* wt) {
+  // wt[ci][tap' * co_t + co] = w[co][(26 - tap') * cin + ci]
+  const long long n = (long long)rows_t * kpad_t;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int ci = (int)(i / kpad_t), kk = (int)(i % kpad_t);
+  const int tp = kk / co_t, co = kk % co_t;
+  u16 v = 0;
+  if (tp < 27 && co < cout && ci < cin) v = w[(long long)co * kpad + (26 - tp) * cin + ci];
+  wt[i] = v;
+}
+
+__global__ void pack_fields_kernel(int nx, int ny, int nz, const float* f0, const float* f1,
+                                   const float* f2, const float* f3, const float* f4,
+                                   const float* f5, const float* eps, const float* scale,
+                                   float eshift, float escale, u16* out) {
+  const long long n = (long long)nx * ny * nz;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float* f[6] = {f0, f1, f2, f3, f4, f5};
+  u16 v[8];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) v[c] = f2bf(f[c][i] / scale[c]);
+  v[6] = f2bf((eps[i] - eshift) / escale);
+  v[7] = 0;
+  uint4 pk;
+  __builtin_memcpy(&pk, v, 16);
+  *reinterpret_cast<uint4*>(out + i * 8) = pk;
+}
+
+__global__ void apply_correction_kernel(int nx, int ny, int nz, float* f0, float* f1, float* f2,
+                                        float* f3, float* f4, float* f5, const float* y, int ys,
+                                        const float* scale, float gain) {
+  const long long n = (long long)nx * ny * nz;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float* f[6] = {f0, f1, f2, f3, f4, f5};
+#pragma unroll
+  for (int c = 0; c < 6; ++c) f[c][i] += gain * scale[c] * y[i * ys + c];
+}
+
+int kpad_of(int cin) { return ((27 * cin + 31) / 32) * 32; }
+
+}  // namespace conv
+
+using namespace conv;
+
+extern "C" {
+
+const char* conv3d_last_error(void) { return g_err.c_str(); }
+
+int conv3d_fwd(const ConvDesc* d, const void* x, const void* w, const float* bias, int32_t epi,
+               void* z_out, const void* aux, void* y, void* stream) {
+  if (!d || !x || !w || !y) return fail(-1, "null argument");
+  if (d->cin_pad <= 0 || d->cin_pad % 8 || d->cout_pad % 16 || d->cout_pad <= 0 ||
+      d->cout_pad > 32 || d->cout > d->cout_pad || d->y_cpad < d->cout || d->y_cpad > d->cout_pad)
+    return fail(-1, "unsupported channel configuration");
+  if (d->nx < 1 || d->ny < 1 || d->nz < 1) return fail(-1, "bad grid");
+  if (epi == CONV_EPI_GELU_BWD && !aux) return fail(-1, "GELU backward needs the pre-activation");
+  if ((epi == CONV_EPI_GELU || epi == CONV_EPI_GELU_BWD || epi == CONV_EPI_LINEAR_BF16) &&
+      d->y_cpad % 8)
+    return fail(-1, "bf16 outputs need y_cpad multiple of 8");
+  ConvArgs a;
+  a.nx = d->nx;
+  a.ny = d->ny;
+  a.nz = d->nz;
+  a.cin = d->cin_pad;
+  a.cout = d->cout;
+  a.cout_pad = d->cout_pad;
+  a.ycp = d->y_cpad;
+  a.kpad = kpad_of(d->cin_pad);
+  a.x = (const u16*)x;
+  a.w = (const u16*)w;
+  a.bias = bias;
+  a.zout = (u16*)z_out;
+  a.aux = (const u16*)aux;
+  a.y = y;
+  constexpr int NT = 4;
+  const int gx = (d->nx + 16 * NT - 1) / (16 * NT);
+  const int gy = (d->ny + 3) / 4;
+  long long blocks_xy = (long long)gx * gy;
+  int zchunk = (int)((blocks_xy * d->nz + 2047) / 2048);  // ~2048 blocks
+  zchunk = zchunk < 1 ? 1 : (zchunk > 8 ? 8 : zchunk);
+  a.zchunk = zchunk;
+  const dim3 grid(gx, gy, (d->nz + zchunk - 1) / zchunk);
+  const size_t shm = (size_t)d->cout_pad * (a.kpad + 8) * sizeof(u16);
+  if (shm > 160 * 1024) return fail(-1, "weights exceed LDS");
+  hipStream_t st = (hipStream_t)stream;
+#define CONV_LAUNCH(E, MT)                                                             \
+  hipLaunchKernelGGL((conv_fwd_kernel<E, NT, MT>), grid, dim3(256), shm, st, a)
+  const int mt = d->cout_pad / 16;
+  switch (epi) {
+    case CONV_EPI_GELU:
+      if (mt == 2) CONV_LAUNCH(CONV_EPI_GELU, 2); else CONV_LAUNCH(CONV_EPI_GELU, 1);
+      break;
+    case CONV_EPI_LINEAR_F32:
+      if (mt == 2) CONV_LAUNCH(CONV_EPI_LINEAR_F32, 2); else CONV_LAUNCH(CONV_EPI_LINEAR_F32, 1);
+      break;
+    case CONV_EPI_GELU_BWD:
+      if (mt == 2) CONV_LAUNCH(CONV_EPI_GELU_BWD, 2); else CONV_LAUNCH(CONV_EPI_GELU_BWD, 1);
+      break;
+    case CONV_EPI_LINEAR_BF16:
+      if (mt == 2) CONV_LAUNCH(CONV_EPI_LINEAR_BF16, 2); else CONV_LAUNCH(CONV_EPI_LINEAR_BF16, 1);
+      break;
+    default: return fail(-1, "unknown epilogue");
+  }
+#undef CONV_LAUNCH
+  CONV_CHECK(hipGetLastError());
+  return 0;
+}
+
+static void wgrad_shape(const ConvDesc* d, int& nseg_x, int& total, int& spb, int& nblocks) {
+  nseg_x = (d->nx + kSeg - 1) / kSeg;
+  total = nseg_x * d->ny * d->nz;
+  nblocks = total < 1024 ? total : 1024;
+  spb = (total + nblocks - 1) / nblocks;
+  nblocks = (total + spb - 1) / spb;
+}
+
+int64_t conv3d_wgrad_work_floats(const ConvDesc* d) {
+  if (!d) return 0;
+  int nseg_x, total, spb, nb;
+  wgrad_shape(d, nseg_x, total, spb, nb);
+  const long long kp = kpad_of(d->cin_pad);
+  return (long long)nb * d->cout_pad * kp + 2048LL * 32;
+}
+
+int conv3d_wgrad(const ConvDesc* d, const void* x, const void* dy, int32_t dy_cpad, float* work,
+                 float* dw, float* db, void* stream) {
+  if (!d || !x || !dy || !work || !dw) return fail(-1, "null argument");
+  if (d->cin_pad % 8 || d->cout_pad % 16 || d->cout_pad > 32 || dy_cpad > d->cout_pad ||
+      dy_cpad < d->cout)
+    return fail(-1, "unsupported channel configuration");
+  const int kpad = kpad_of(d->cin_pad);
+  if (kpad / 16 > 4 * 16) return fail(-1, "too many input channels for the weight gradient");
+  int nseg_x, total, spb, nb;
+  wgrad_shape(d, nseg_x, total, spb, nb);
+  WgradArgs a;
+  a.nx = d->nx;
+  a.ny = d->ny;
+  a.nz = d->nz;
+  a.cin = d->cin_pad;
+  a.cout_pad = d->cout_pad;
+  a.dycp = dy_cpad;
+  a.kpad = kpad;
+  a.nseg_x = nseg_x;
+  a.segs_per_block = spb;
+  a.total_segs = total;
+  a.x = (const u16*)x;
+  a.dy = (const u16*)dy;
+  a.part = work;
+  hipStream_t st = (hipStream_t)stream;
+  const size_t shm = ((size_t)d->cout_pad * kSeg + 9 * (size_t)d->cin_pad * (kSeg + 2)) * sizeof(u16);
+  if (d->cout_pad == 32)
+    hipLaunchKernelGGL((conv_wgrad_kernel<2>), dim3(nb), dim3(256), shm, st, a);
+  else
+    hipLaunchKernelGGL((conv_wgrad_kernel<1>), dim3(nb), dim3(256), shm, st, a);
+  CONV_CHECK(hipGetLastError());
+  const long long n = (long long)d->cout_pad * kpad;
+  hipLaunchKernelGGL(wgrad_reduce_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                     (const float*)work, nb, n, dw);
+  CONV_CHECK(hipGetLastError());
+  if (db) {
+    float* bpart = work + (long long)nb * n;
+    const long long npos = (long long)d->nx * d->ny * d->nz;
+    const int bb = 2048;
+    hipLaunchKernelGGL(bias_grad_partial, dim3(bb), dim3(256), 0, st, (const u16*)dy, dy_cpad,
+                       d->cout, npos, bpart);
+    CONV_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(bias_grad_final, dim3(1), dim3(64), 0, st, (const float*)bpart, bb, d->cout,
+                       db);
+    CONV_CHECK(hipGetLastError());
+  }
+  return 0;
+}
+
+int conv3d_flip_weights(const ConvDesc* d, const void* w, int32_t dy_cpad, void* wt, void* stream) {
+  if (!d || !w || !wt) return fail(-1, "null argument");
+  const int kpad = kpad_of(d->cin_pad);
+  const int co_t = ((dy_cpad + 7) / 8) * 8;
+  const int kpad_t = kpad_of(co_t);
+  const int rows_t = ((d->cin_pad + 15) / 16) * 16;
+  const long long n = (long long)rows_t * kpad_t;
+  hipLaunchKernelGGL(flip_weights_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, (const u16*)w, d->cout, d->cout_pad, d->cin_pad, kpad,
+                     co_t, kpad_t, rows_t, (u16*)wt);
+  CONV_CHECK(hipGetLastError());
+  return 0;
+}
+
+int conv3d_pack_fields(int32_t nx, int32_t ny, int32_t nz, float* const* fields, const float* eps,
+                       const float* scale, float eps_shift, float eps_scale, void* out,
+                       void* stream) {
+  if (!fields || !eps || !scale || !out) return fail(-1, "null argument");
+  const long long n = (long long)nx * ny * nz;
+  hipLaunchKernelGGL(pack_fields_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, nx, ny, nz, fields[0], fields[1], fields[2], fields[3],
+                     fields[4], fields[5], eps, scale, eps_shift, eps_scale, (u16*)out);
+  CONV_CHECK(hipGetLastError());
+  return 0;
+}
+
+int conv3d_apply_correction(int32_t nx, int32_t ny, int32_t nz, float* const* fields,
+                            const float* y, int32_t ystride, const float* scale, float gain,
+                            void* stream) {
+  if (!fields || !y || !scale || ystride < 6) return fail(-1, "bad argument");
+  const long long n = (long long)nx * ny * nz;
+  hipLaunchKernelGGL(apply_correction_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, nx, ny, nz, fields[0], fields[1], fields[2], fields[3],
+                     fields[4], fields[5], y, ystride, scale, gain);
+  CONV_CHECK(hipGetLastError());
+  return 0;
+}
+
+}  // extern "C"

@@ -130,3 +130,28 @@ def psi_bytes_per_step(size, npml: int) -> int:
     nx, ny, nz = size
     tp = 2 * npml
     return 8 * 4 * (nz * tp * nx + tp * ny * nx + nz * ny * tp)
+
+
+def c4_design(n: int = 160, device=None, seed: int = SEEDS["c4_160"]):
+    """Adjoint workload: n^3 (160), 12-cell CPML, vacuum background, central
+    design region [n/5, 4n/5)^3 (96^3 at 160) with eps_r U[1,4] (seeded), a
+    z-dipole 20 cells below the region (Gaussian pulse), and the objective's
+    probes: Ez on a 32x32 patch on the plane 20 cells above the region.
+    Returns (grid, design box (x0,y0,z0,nx,ny,nz), probes, eps_box)."""
+    size = (n, n, n)
+    g = EMGrid3D(size, 1.0, "cpml", max(4, 12 * n // 160), device=device)
+    lo, hi = n // 5, 4 * n // 5
+    box = (lo, lo, lo, hi - lo, hi - lo, hi - lo)
+    c = n // 2
+    npml = max(4, 12 * n // 160)
+    zsrc = lo - 20 if lo - 20 >= npml else (lo + npml) // 2
+    g.add_source(PointSource3D(0.0, (c, c, zsrc),
+                               waveform="gaussian_sine", wavelength=20.0, t0=90.0, spread=30.0))
+    zmon = min(hi + 19, n - 14)
+    half = min(16, (hi - lo) // 6)
+    probes = [("ez", (i, j, zmon)) for i in range(c - half, c + half)
+              for j in range(c - half, c + half)]
+    rng = np.random.default_rng(seed)
+    eps_box = torch.tensor(rng.uniform(1.0, 4.0, (hi - lo,) * 3), dtype=torch.float32,
+                           device=device)
+    return g, box, probes, eps_box

@@ -1,0 +1,45 @@
+"""Host-side pins of the closure: staggered restriction operators (exact on
+linear fields at the right staggered positions) and the source scaling."""
+
+import numpy as np
+import torch
+
+from jaxphys_amd.closure import ClosureProblem, restrict_e, restrict_h
+
+
+def _lin(shape, axis_pos):
+    """f = 1 + 0.5 x + 0.25 y - 0.125 z at the given (x, y, z) offsets of a
+    staggered component; arrays are (nz, ny, nx)."""
+    nz, ny, nx = shape
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    ox, oy, oz = axis_pos
+    return torch.tensor(1 + 0.5 * (x + ox) + 0.25 * (y + oy) - 0.125 * (z + oz))
+
+
+def test_restriction_exact_on_linear_fields():
+    shape = (8, 10, 12)
+    # fine Yee offsets (x, y, z) in fine cells
+    e_off = [(0.5, 0, 0), (0, 0.5, 0), (0, 0, 0.5)]
+    h_off = [(0, 0.5, 0.5), (0.5, 0, 0.5), (0.5, 0.5, 0)]
+    fe = [_lin(shape, o) for o in e_off]
+    fh = [_lin(shape, o) for o in h_off]
+    ce = restrict_e(fe)
+    ch = restrict_h(fh)
+    cshape = (4, 5, 6)
+    # coarse offsets in coarse cells are the same; positions in fine units = 2*(I + off)
+    for got, o in zip(ce + ch, e_off + h_off):
+        nz, ny, nx = cshape
+        z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+        want = 1 + 0.5 * 2 * (x + o[0]) + 0.25 * 2 * (y + o[1]) - 0.125 * 2 * (z + o[2])
+        assert got.shape == cshape
+        np.testing.assert_allclose(got.numpy(), want, rtol=0, atol=1e-12)
+
+
+def test_problem_media_agree_between_grids():
+    p = ClosureProblem(seed=3, n_fine=64, npml_fine=8)
+    assert p.eps_f.shape == (64, 64, 64) and p.eps_c.shape == (32, 32, 32)
+    np.testing.assert_array_equal(p.eps_f[::2, ::2, ::2], p.eps_c)
+    np.testing.assert_array_equal(p.eps_f[1::2, 1::2, 1::2], p.eps_c)
+    assert 1.0 <= p.eps_c.min() and p.eps_c.max() <= 6.0
+    i, j, k = p.src_c
+    assert all(8 <= v < 24 for v in (i, j, k))

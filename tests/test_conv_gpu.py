@@ -1,0 +1,182 @@
+"""BF16-MFMA conv3d kernels vs torch float64 conv3d on the CPU (same
+bf16-rounded inputs and weights), and the closure net's own backward pass vs
+torch autograd of a float64 restatement."""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+import torch.nn.functional as F
+
+from jaxphys_amd import _conv
+from jaxphys_amd.closure import CIN, COUT, HID, ClosureNet
+
+pytestmark = pytest.mark.gpu
+
+
+def _rand_bf16(shape, gen, scale=1.0):
+    return (torch.randn(shape, generator=gen) * scale).to(torch.bfloat16)
+
+
+def _ref_conv(x_cl, w, b, cin_real):
+    """x_cl: (nz, ny, nx, Cpad) bf16 -> float64 torch conv3d (zero pad 1).
+    w: (cout, 27, cin_pad) bf16."""
+    x = x_cl.double().permute(3, 0, 1, 2)[None, :cin_real]
+    cout = w.shape[0]
+    wt = w.double()[:, :, :cin_real].reshape(cout, 3, 3, 3, cin_real).permute(0, 4, 1, 2, 3)
+    y = F.conv3d(x, wt, None if b is None else b.double(), padding=1)
+    return y[0].permute(1, 2, 3, 0)  # (nz, ny, nx, cout)
+
+
+def _gelu(z):
+    return 0.5 * z * (1.0 + torch.erf(z / math.sqrt(2.0)))
+
+
+@pytest.mark.parametrize("cin_pad,cin_real,cout,cout_pad,ycp,epi", [
+    (8, 7, 32, 32, 32, _conv.EPI_GELU),
+    (32, 32, 32, 32, 32, _conv.EPI_GELU),
+    (32, 32, 6, 16, 8, _conv.EPI_LINEAR_F32),
+    (32, 32, 32, 32, 32, _conv.EPI_LINEAR_BF16),
+])
+def test_conv_fwd_matches_torch(gpu, cin_pad, cin_real, cout, cout_pad, ycp, epi):
+    g = torch.Generator().manual_seed(cin_pad * 100 + cout)
+    shape = (37, 19, 11)  # (nx, ny, nz), not tile multiples
+    nx, ny, nz = shape
+    x = _rand_bf16((nz, ny, nx, cin_pad), g)
+    x[..., cin_real:] = 0
+    w = _rand_bf16((cout, 27, cin_pad), g, 1.0 / math.sqrt(27 * cin_real))
+    w[:, :, cin_real:] = 0
+    b = torch.randn(cout, generator=g) * 0.1
+    kpad = _conv.kpad_of(cin_pad)
+    wp = torch.zeros(cout_pad, kpad, dtype=torch.bfloat16)
+    wp[:cout, :27 * cin_pad] = w.reshape(cout, -1)
+    d = _conv.desc(shape, cin_pad, cout, cout_pad, ycp)
+    xd, wd, bd = x.to(gpu).contiguous(), wp.to(gpu), b.to(gpu)
+    if epi == _conv.EPI_LINEAR_F32:
+        y = torch.zeros(nz * ny * nx, ycp, device=gpu)
+    else:
+        y = torch.zeros(nz * ny * nx, ycp, dtype=torch.bfloat16, device=gpu)
+    z = torch.zeros(nz * ny * nx, ycp, dtype=torch.bfloat16, device=gpu)
+    _conv.fwd(d, xd, wd, bd, epi, y, z_out=z if epi == _conv.EPI_GELU else None)
+    ref = _ref_conv(x, w, b, cin_real).reshape(-1, cout)
+    got = y.double().cpu()[:, :cout]
+    if epi == _conv.EPI_LINEAR_F32:
+        rel = float((got - ref).norm() / ref.norm())
+        assert rel < 1e-5, rel
+    elif epi == _conv.EPI_LINEAR_BF16:
+        rel = float((got - ref).norm() / ref.norm())
+        assert rel < 5e-3, rel
+    else:
+        zr = ref.to(torch.bfloat16).double()
+        rel_z = float((z.double().cpu()[:, :cout] - ref).norm() / ref.norm())
+        assert rel_z < 5e-3, rel_z
+        rel = float((got - _gelu(zr)).norm() / _gelu(zr).norm())
+        assert rel < 5e-3, rel
+    if ycp > cout:
+        assert float(y.double().cpu()[:, cout:].abs().max()) == 0.0
+
+
+def test_conv_wgrad_and_dgrad_match_autograd(gpu):
+    g = torch.Generator().manual_seed(5)
+    shape = (40, 13, 9)
+    nx, ny, nz = shape
+    cin, cout = 32, 32
+    x = _rand_bf16((nz, ny, nx, cin), g)
+    w = _rand_bf16((cout, 27, cin), g, 1.0 / math.sqrt(27 * cin))
+    dy = _rand_bf16((nz, ny, nx, cout), g)
+    zaux = _rand_bf16((nz, ny, nx, cin), g)
+    kpad = _conv.kpad_of(cin)
+    wp = torch.zeros(32, kpad, dtype=torch.bfloat16)
+    wp[:, :27 * cin] = w.reshape(cout, -1)
+    d = _conv.desc(shape, cin, cout, 32, 32)
+    work = torch.empty(_conv.wgrad_work_floats(d), device=gpu)
+    dw = torch.empty(32, kpad, device=gpu)
+    db = torch.empty(cout, device=gpu)
+    _conv.wgrad(d, x.to(gpu), dy.to(gpu), 32, work, dw, db)
+    wt = torch.empty(32, _conv.kpad_of(32), dtype=torch.bfloat16, device=gpu)
+    _conv.flip_weights(d, wp.to(gpu), 32, wt)
+    dx = torch.empty(nz * ny * nx, 32, dtype=torch.bfloat16, device=gpu)
+    _conv.fwd(_conv.desc(shape, 32, cin, 32, 32), dy.to(gpu), wt, None, _conv.EPI_GELU_BWD, dx,
+              aux=zaux.to(gpu))
+    # torch float64 reference
+    xr = x.double().permute(3, 0, 1, 2)[None].requires_grad_(True)
+    wr = w.double().reshape(cout, 3, 3, 3, cin).permute(0, 4, 1, 2, 3).contiguous()
+    wr.requires_grad_(True)
+    br = torch.zeros(cout, dtype=torch.float64, requires_grad=True)
+    y = F.conv3d(xr, wr, br, padding=1)
+    y.backward(dy.double().permute(3, 0, 1, 2)[None])
+    dw_ref = wr.grad.permute(0, 2, 3, 4, 1).reshape(cout, 27 * cin)
+    rel_w = float((dw.double().cpu()[:, :27 * cin] - dw_ref).norm() / dw_ref.norm())
+    assert rel_w < 1e-4, rel_w
+    rel_b = float((db.double().cpu() - br.grad).norm() / br.grad.norm())
+    assert rel_b < 1e-4, rel_b
+    zd = zaux.double()
+    gp = 0.5 * (1 + torch.erf(zd / math.sqrt(2))) + zd * torch.exp(-0.5 * zd * zd) / math.sqrt(
+        2 * math.pi)
+    dx_ref = (xr.grad[0].permute(1, 2, 3, 0) * gp).reshape(-1, cin)
+    rel_x = float((dx.double().cpu() - dx_ref).norm() / dx_ref.norm())
+    assert rel_x < 8e-3, rel_x
+
+
+def test_closure_net_backward_matches_autograd(gpu):
+    shape = (24, 20, 12)
+    nx, ny, nz = shape
+    net = ClosureNet(gpu, seed=3, out_scale=1.0)
+    g = torch.Generator().manual_seed(9)
+    x = _rand_bf16((nz * ny * nx, CIN), g)
+    x[:, 7] = 0
+    target = torch.randn(nz * ny * nx, 8, generator=g) * 0.1
+    y = net.forward(x.to(gpu), shape, train=True)
+    dy = torch.zeros_like(y)
+    dy[:, :COUT] = 2 * (y[:, :COUT] - target[:, :COUT].to(gpu)) / y[:, :COUT].numel()
+    grads = net.backward(dy)
+    # float64 restatement on the same bf16-rounded weights
+    P = {k: v.detach().cpu().double() for k, v in net.params.items()}
+    for k in ("w1", "w2", "w3"):
+        P[k] = P[k].to(torch.bfloat16).double()
+    for v in P.values():
+        v.requires_grad_(True)
+
+    def conv(inp, w, b, cin):
+        wt = w[:, :, :cin].reshape(w.shape[0], 3, 3, 3, cin).permute(0, 4, 1, 2, 3)
+        return F.conv3d(inp, wt, b, padding=1)
+
+    xi = x.double().reshape(nz, ny, nx, CIN).permute(3, 0, 1, 2)[None][:, :7]
+    a1 = _gelu(conv(xi, P["w1"], P["b1"], 7))
+    a2 = _gelu(conv(a1, P["w2"], P["b2"], HID))
+    out = conv(a2, P["w3"], P["b3"], HID)[0].permute(1, 2, 3, 0).reshape(-1, COUT)
+    loss = ((out - target[:, :COUT].double()) ** 2).mean()
+    loss.backward()
+    rel_y = float((y[:, :COUT].double().cpu() - out.detach()).norm() / out.detach().norm())
+    assert rel_y < 2e-2, rel_y
+    for k in ("w1", "b1", "w2", "b2", "w3", "b3"):
+        ref = P[k].grad
+        if k == "w1":
+            ref = ref.clone()
+            ref[:, :, 7:] = 0
+        rel = float((grads[k].double().cpu() - ref).norm() / ref.norm())
+        print(f"[conv] grad {k} rel {rel:.2e}", flush=True)
+        assert rel < 5e-2, (k, rel)
+
+
+def test_closure_training_reduces_loss(gpu):
+    """Fit a fixed random linear 3x3x3 map with the net's own kernels."""
+    shape = (32, 32, 16)
+    nx, ny, nz = shape
+    net = ClosureNet(gpu, seed=1, out_scale=1.0)
+    g = torch.Generator().manual_seed(2)
+    k = torch.randn(6, 7, 3, 3, 3, generator=g, dtype=torch.float64) * 0.05
+    losses = []
+    for it in range(60):
+        x = _rand_bf16((nz * ny * nx, CIN), g)
+        x[:, 7] = 0
+        xi = x.double().reshape(nz, ny, nx, CIN).permute(3, 0, 1, 2)[None][:, :7]
+        tgt = F.conv3d(xi, k, padding=1)[0].permute(1, 2, 3, 0).reshape(-1, 6).float().to(gpu)
+        y = net.forward(x.to(gpu), shape, train=True)
+        err = y[:, :6] - tgt
+        losses.append(float((err ** 2).mean()))
+        dy = torch.zeros_like(y)
+        dy[:, :6] = 2 * err / err.numel()
+        net.adam_step(net.backward(dy), lr=3e-3)
+    assert losses[-1] < 0.3 * losses[0], losses[::10]
