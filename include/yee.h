@@ -83,12 +83,16 @@ typedef struct YeePlan YeePlan;
  * CPML): 12 arrays, each 2*npml cells thick along its derivative axis.  */
 int64_t yee_psi_floats(const YeeGridDesc* desc);
 
-/* Validate `desc` and build a plan (host object; holds no device memory
- * except the source table uploaded by yee_set_sources when it has more
- * than YEE_FUSED_SOURCES sources).  Replaces the prologue of
+/* Validate `desc` and build a plan (host object; its only device memory is
+ * the source table uploaded by yee_set_sources).  Replaces the prologue of
  * fdtd3d.py:185-231 below the Python layer.                              */
 int yee_plan_create(const YeeGridDesc* desc, YeePlan** out);
 void yee_plan_destroy(YeePlan* plan);
+
+/* Launch tuning (no reference counterpart): z planes marched per block by
+ * the two-pass kernels (kz > 0) and by the fused kernel (fused_kz > 0;
+ * 0 = automatic, < 0 = unchanged).  Results do not depend on them.       */
+int yee_plan_tune(YeePlan* plan, int32_t kz, int32_t fused_kz);
 
 #define YEE_FUSED_SOURCES 8
 

@@ -47,6 +47,7 @@ EXPORTS = {
     "yee_psi_floats": (ctypes.c_int64, [ctypes.POINTER(YeeGridDesc)]),
     "yee_plan_create": (ctypes.c_int, [ctypes.POINTER(YeeGridDesc), ctypes.POINTER(ctypes.c_void_p)]),
     "yee_plan_destroy": (None, [ctypes.c_void_p]),
+    "yee_plan_tune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     "yee_set_sources": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32]),
     "yee_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
@@ -158,6 +159,9 @@ class Plan:
         h = ctypes.c_void_p()
         check(lib().yee_plan_create(ctypes.byref(desc), ctypes.byref(h)), "yee_plan_create")
         self.handle = h
+
+    def tune(self, kz: int = 0, fused_kz: int = -1) -> None:
+        check(lib().yee_plan_tune(self.handle, int(kz), int(fused_kz)), "yee_plan_tune")
 
     def psi_floats(self) -> int:
         return int(lib().yee_psi_floats(ctypes.byref(self.desc)))

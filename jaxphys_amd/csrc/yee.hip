@@ -352,13 +352,14 @@ int launch_e(const YeePlan* p, const GridArgs& g, int n, hipStream_t st) {
   else
     launch_e_mode<false>(p, g, s, st);
   YEE_HIP_CHECK(hipGetLastError());
-  return launch_inject(p, g, n, st);
+  return launch_inject(p, g.f[0], g.f[1], g.f[2], n, YEE_FUSED_SOURCES, st);
 }
 
-int launch_inject(const YeePlan* p, const GridArgs& g, int n, hipStream_t st) {
-  if (p->nsrc > YEE_FUSED_SOURCES) {
-    yee_inject_kernel<<<1, 64, 0, st>>>(g.f[0], g.f[1], g.f[2], p->d_comp, p->d_idx, p->d_table,
-                                        p->nsrc, YEE_FUSED_SOURCES, n);
+int launch_inject(const YeePlan* p, float* ex, float* ey, float* ez, int n, int first,
+                  hipStream_t st) {
+  if (p->nsrc > first) {
+    yee_inject_kernel<<<1, 64, 0, st>>>(ex, ey, ez, p->d_comp, p->d_idx, p->d_table, p->nsrc,
+                                        first, n);
     YEE_HIP_CHECK(hipGetLastError());
   }
   return YEE_OK;
@@ -401,7 +402,18 @@ int yee_plan_create(const YeeGridDesc* desc, YeePlan** out) {
   if (kz > desc->nz) kz = desc->nz;
   p->kz = kz;
   p->grid = dim3(gx, gy, (desc->nz + kz - 1) / kz);
+  if (const char* e = std::getenv("JAXPHYS_AMD_FUSED_KZ")) p->fused_kz = std::atoi(e);
   *out = p;
+  return YEE_OK;
+}
+
+int yee_plan_tune(YeePlan* p, int32_t kz, int32_t fused_kz) {
+  if (!p) return fail(YEE_EINVAL, "null plan");
+  if (kz > 0) {
+    p->kz = kz > p->d.nz ? p->d.nz : kz;
+    p->grid.z = (p->d.nz + p->kz - 1) / p->kz;
+  }
+  if (fused_kz >= 0) p->fused_kz = fused_kz;
   return YEE_OK;
 }
 
@@ -433,7 +445,7 @@ int yee_set_sources(YeePlan* p, int32_t nsrc, const int32_t* comp, const int32_t
                   table ? table + (size_t)nsrc * nsteps : (const float*)nullptr);
   p->nsrc = nsrc;
   p->nsteps_table = nsteps;
-  if (nsrc > YEE_FUSED_SOURCES) {
+  if (nsrc > 0) {  // device copies: the fused step injects every source itself
     YEE_HIP_CHECK(hipMalloc(&p->d_comp, sizeof(int) * nsrc));
     YEE_HIP_CHECK(hipMalloc(&p->d_idx, sizeof(long long) * nsrc));
     YEE_HIP_CHECK(hipMalloc(&p->d_table, sizeof(float) * (size_t)nsrc * (nsteps > 0 ? nsteps : 1)));

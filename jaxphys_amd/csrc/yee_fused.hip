@@ -10,42 +10,68 @@
 // Block = (BY+1) wavefronts x 64 lanes; lane = x, wave = y row.  Wave 0 is a
 // halo row: it computes H^{n+1/2} on row j0-1 only, so that the owner rows
 // j0..j0+BY-1 find H(y-1) in LDS.  Lane 0 of every owner wave recomputes
-// H(x-1) for its row (the previous 64-wide tile's last column).  H(z-1) is
-// the thread's own H from the previous plane (register carry); each block
-// marches a chunk of z planes after recomputing the plane below the chunk.
-// Per plane: compute own H -> LDS (double-buffered by plane parity) and to
-// B; one barrier; owners compute E^{n+1} from LDS + registers -> B.
-// Arithmetic and its order are those of the two-pass kernels, so the result
-// is bit-for-bit the same.
+// Hy, Hz of column x-1 (the previous tile's last column).  H(z-1) is the
+// thread's own H of the previous plane (register carry); each block marches
+// a chunk of z planes after recomputing Hx, Hy of the plane below the chunk.
+// Per plane: own H -> LDS (double-buffered by plane parity) and to B; one
+// barrier; owners compute E^{n+1} from LDS + registers -> B.  Point sources
+// are added by a one-thread kernel after the step (insertion order, after
+// the E update, as the reference does).  Arithmetic and its order are those
+// of the two-pass kernels, so the result is bit-for-bit the same.
+//
+// Kernel arguments are kept compact (one base pointer for each psi buffer,
+// one for all CPML tables, one for the sponge profiles): the scalar register
+// file holds them without spilling.
 
 #include "yee_internal.h"
 
 namespace yee {
 
 struct FusedArgs {
-  GridArgs in;            // input fields / psi slots + all coefficients
-  float* fo[6];           // output fields
-  float* po[kNumPsi];     // output psi slots
+  int nx, ny, nz, p;
+  float ch, sdt, seps0;
+  long long ysl, zsl, xsl;      // psi slab sizes (floats) of the y, z, x groups
+  const float* fi[6];           // buffer A fields
+  float* fo[6];                 // buffer B fields
+  const float* cb;
+  const float* ca;
+  const float* sponge;          // sigma_x | sigma_y | sigma_z
+  const float* cpml;            // x | y | z tables, 6 n_a floats each
+  const float* pin;             // psi of buffer A
+  float* pout;                  // psi of buffer B
 };
 
-template <int MODE>
-__device__ __forceinline__ float cpml_in_out(const float* pin, float* pout, bool write,
-                                             const float* tab, int n, int v, int kind, float d) {
-  const float b = tab[(3 * kind + 0) * n + v];
-  const float c = tab[(3 * kind + 1) * n + v];
-  const float kinv = tab[(3 * kind + 2) * n + v];
-  const float ps = b * (*pin) + c * d;
-  if (write) *pout = ps;
+// psi slot offsets: y group (Exy Ezy Hxy Hzy), z group (Exz Eyz Hxz Hyz),
+// x group (Eyx Ezx Hyx Hzx) -- the slot order of yee_internal.h
+__device__ __forceinline__ long long slot_base(const FusedArgs& a, int s) {
+  if (s < 4) return s * a.ysl;
+  if (s < 8) return 4 * a.ysl + (s - 4) * a.zsl;
+  return 4 * a.ysl + 4 * a.zsl + (s - 8) * a.xsl;
+}
+
+__device__ __forceinline__ const float* tab_of(const FusedArgs& a, int ax) {
+  return ax == 0 ? a.cpml : (ax == 1 ? a.cpml + 6LL * a.nx : a.cpml + 6LL * (a.nx + a.ny));
+}
+
+// psi' = b psi + c d (psi from A, written to B when `write`); returns d/kappa + psi'
+__device__ __forceinline__ float cterm(const FusedArgs& a, int slot, long long off, bool write,
+                                       int ax, int n, int v, int kind, float d) {
+  const float* t = tab_of(a, ax);
+  const float b = t[(3 * kind + 0) * n + v];
+  const float c = t[(3 * kind + 1) * n + v];
+  const float kinv = t[(3 * kind + 2) * n + v];
+  const long long o = slot_base(a, slot) + off;
+  const float ps = b * a.pin[o] + c * d;
+  if (write) a.pout[o] = ps;
   return d * kinv + ps;
 }
 
-// H^{n+1/2} at (i, j, k) (indices already wrapped/valid) from buffer A;
-// writes psi_H' to buffer B when `own`.
-template <int MODE>
+// H^{n+1/2} at (i, j, k) from buffer A.  WANT bit 1/2/4: Hx/Hy/Hz.  Writes
+// psi_H' to B when `own`.  ``e`` receives Ex, Ey, Ez of the cell.
+template <int MODE, int WANT>
 __device__ __forceinline__ void h_cell(const FusedArgs& a, int i, int j, int k, bool own,
-                                       float& ox, float& oy, float& oz) {
-  const GridArgs& g = a.in;
-  const int nx = g.nx, ny = g.ny, nz = g.nz, p = g.p;
+                                       float& ox, float& oy, float& oz, float* e) {
+  const int nx = a.nx, ny = a.ny, nz = a.nz, p = a.p;
   constexpr bool per = MODE == kPeriodic;
   const long long P = (long long)nx * ny;
   const long long c = ((long long)k * ny + j) * nx + i;
@@ -55,58 +81,66 @@ __device__ __forceinline__ void h_cell(const FusedArgs& a, int i, int j, int k, 
   const long long di = (per && i == nx - 1) ? -(long long)(nx - 1) : 1;
   const long long dj = (per && j == ny - 1) ? -(long long)(ny - 1) * nx : nx;
   const long long dk = (per && k == nz - 1) ? -(long long)(nz - 1) * P : P;
-  const float* __restrict__ ex = g.f[0];
-  const float* __restrict__ ey = g.f[1];
-  const float* __restrict__ ez = g.f[2];
+  const float* __restrict__ ex = a.fi[0];
+  const float* __restrict__ ey = a.fi[1];
+  const float* __restrict__ ez = a.fi[2];
   const float ex0 = ex[c], ey0 = ey[c], ez0 = ez[c];
-  float hx = g.f[3][c], hy = g.f[4][c], hz = g.f[5][c];
-  float ex1 = 0.f, ey1 = 0.f, ezj = 0.f, exj = 0.f, ezi = 0.f, eyi = 0.f;
-  if (zin) { ex1 = ex[c + dk]; ey1 = ey[c + dk]; }
-  if (yin) { ezj = ez[c + dj]; exj = ex[c + dj]; }
-  if (xin) { ezi = ez[c + di]; eyi = ey[c + di]; }
+  if (e) {
+    e[0] = ex0;
+    e[1] = ey0;
+    e[2] = ez0;
+  }
   const bool xs = MODE == kCpml && (i < p || i >= nx - p);
   const bool ys = MODE == kCpml && (j < p || j >= ny - p);
   const bool zs = MODE == kCpml && (k < p || k >= nz - p);
   const long long oy_ = ys ? ((long long)(k * 2 * p + slab_index(j, ny, p)) * nx + i) : 0;
   const long long oz_ = zs ? ((long long)(slab_index(k, nz, p) * ny + j) * nx + i) : 0;
   const long long ox_ = xs ? (((long long)k * ny + j) * 2 * p + slab_index(i, nx, p)) : 0;
-  const float ch = g.ch;
-  if (yin && zin) {  // fdtd3d.py:305-312
-    float u = ezj - ez0, v = ey1 - ey0;
-    if (MODE == kCpml) {
-      if (ys) u = cpml_in_out<MODE>(g.psi[kHxy] + oy_, a.po[kHxy] + oy_, own, g.cpy, ny, j, 1, u);
-      if (zs) v = cpml_in_out<MODE>(g.psi[kHxz] + oz_, a.po[kHxz] + oz_, own, g.cpz, nz, k, 1, v);
+  const float ch = a.ch;
+  if (WANT & 1) {
+    float hx = a.fi[3][c];
+    if (yin && zin) {  // fdtd3d.py:305-312
+      float u = ez[c + dj] - ez0, v = ey[c + dk] - ey0;
+      if (MODE == kCpml) {
+        if (ys) u = cterm(a, kHxy, oy_, own, 1, ny, j, 1, u);
+        if (zs) v = cterm(a, kHxz, oz_, own, 2, nz, k, 1, v);
+      }
+      hx = hx - ch * (u - v);
     }
-    hx = hx - ch * (u - v);
+    ox = hx;
   }
-  if (xin && zin) {  // fdtd3d.py:314-321
-    float u = ex1 - ex0, v = ezi - ez0;
-    if (MODE == kCpml) {
-      if (zs) u = cpml_in_out<MODE>(g.psi[kHyz] + oz_, a.po[kHyz] + oz_, own, g.cpz, nz, k, 1, u);
-      if (xs) v = cpml_in_out<MODE>(g.psi[kHyx] + ox_, a.po[kHyx] + ox_, own, g.cpx, nx, i, 1, v);
+  if (WANT & 2) {
+    float hy = a.fi[4][c];
+    if (xin && zin) {  // fdtd3d.py:314-321
+      float u = ex[c + dk] - ex0, v = ez[c + di] - ez0;
+      if (MODE == kCpml) {
+        if (zs) u = cterm(a, kHyz, oz_, own, 2, nz, k, 1, u);
+        if (xs) v = cterm(a, kHyx, ox_, own, 0, nx, i, 1, v);
+      }
+      hy = hy - ch * (u - v);
     }
-    hy = hy - ch * (u - v);
+    oy = hy;
   }
-  if (xin && yin) {  // fdtd3d.py:323-330
-    float u = eyi - ey0, v = exj - ex0;
-    if (MODE == kCpml) {
-      if (xs) u = cpml_in_out<MODE>(g.psi[kHzx] + ox_, a.po[kHzx] + ox_, own, g.cpx, nx, i, 1, u);
-      if (ys) v = cpml_in_out<MODE>(g.psi[kHzy] + oy_, a.po[kHzy] + oy_, own, g.cpy, ny, j, 1, v);
+  if (WANT & 4) {
+    float hz = a.fi[5][c];
+    if (xin && yin) {  // fdtd3d.py:323-330
+      float u = ey[c + di] - ey0, v = ex[c + dj] - ex0;
+      if (MODE == kCpml) {
+        if (xs) u = cterm(a, kHzx, ox_, own, 0, nx, i, 1, u);
+        if (ys) v = cterm(a, kHzy, oy_, own, 1, ny, j, 1, v);
+      }
+      hz = hz - ch * (u - v);
     }
-    hz = hz - ch * (u - v);
+    oz = hz;
   }
-  ox = hx;
-  oy = hy;
-  oz = hz;
 }
 
 template <int MODE, bool LOSSY, int BY>
-__global__ __launch_bounds__(64 * (BY + 1)) void yee_fused_kernel(FusedArgs a, SrcArgs s, int kz) {
+__global__ __launch_bounds__(64 * (BY + 1)) void yee_fused_kernel(FusedArgs a, int kz) {
   __shared__ float shx[2][BY + 1][64];
   __shared__ float shy[2][BY + 1][64];
   __shared__ float shz[2][BY + 1][64];
-  const GridArgs& g = a.in;
-  const int nx = g.nx, ny = g.ny, nz = g.nz, p = g.p;
+  const int nx = a.nx, ny = a.ny, nz = a.nz, p = a.p;
   constexpr bool per = MODE == kPeriodic;
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
@@ -131,31 +165,32 @@ __global__ __launch_bounds__(64 * (BY + 1)) void yee_fused_kernel(FusedArgs a, S
   const int k0 = blockIdx.z * kz;
   const int k1 = min(k0 + kz, nz);
   const long long P = (long long)nx * ny;
-  // H(z-1) of the first plane: recompute the plane below the chunk
+  // Hx, Hy of the plane below the chunk (the z-1 neighbour of the first plane)
   float hxm = 0.f, hym = 0.f;
   {
     int kp = k0 - 1;
     if (per && kp < 0) kp += nz;
     if (owner && kp >= 0) {
-      float tz;
-      h_cell<MODE>(a, i, j, kp, false, hxm, hym, tz);
+      float unused;
+      h_cell<MODE, 3>(a, i, j, kp, false, hxm, hym, unused, nullptr);
     }
   }
   const bool xs = MODE == kCpml && (i < p || i >= nx - p);
   const bool ys = MODE == kCpml && (j < p || j >= ny - p);
   float sxy = 0.f;
-  if (MODE == kSponge && owner) sxy = g.ssx[i] + g.ssy[j];
+  if (MODE == kSponge && owner) sxy = a.sponge[i] + a.sponge[nx + j];
   for (int k = k0; k < k1; ++k) {
     const int buf = k & 1;
     float hx = 0.f, hy = 0.f, hz = 0.f;
-    if (valid) h_cell<MODE>(a, i, j, k, owner, hx, hy, hz);
+    float e[3] = {0.f, 0.f, 0.f};
+    if (valid) h_cell<MODE, 7>(a, i, j, k, owner, hx, hy, hz, e);
     shx[buf][w][lane] = hx;
     shy[buf][w][lane] = hy;
     shz[buf][w][lane] = hz;
     float hyxm = 0.f, hzxm = 0.f;
     if (owner && lane == 0 && imvalid) {
-      float tx;
-      h_cell<MODE>(a, im, j, k, false, tx, hyxm, hzxm);
+      float unused;
+      h_cell<MODE, 6>(a, im, j, k, false, unused, hyxm, hzxm, nullptr);
     }
     const long long c = (long long)k * P + (long long)j * nx + i;
     if (owner) {
@@ -172,9 +207,9 @@ __global__ __launch_bounds__(64 * (BY + 1)) void yee_fused_kernel(FusedArgs a, S
       const bool xin = per || i > 0;
       const bool yin = per || j > 0;
       const bool zin = per || k > 0;
-      const float cb = g.cb[c];
-      const float ca = LOSSY ? g.ca[c] : 1.f;
-      float vx = g.f[0][c], vy = g.f[1][c], vz = g.f[2][c];
+      const float cb = a.cb[c];
+      const float ca = LOSSY ? a.ca[c] : 1.f;
+      float vx = e[0], vy = e[1], vz = e[2];
       const bool zs = MODE == kCpml && (k < p || k >= nz - p);
       const long long oy_ = ys ? ((long long)(k * 2 * p + slab_index(j, ny, p)) * nx + i) : 0;
       const long long oz_ = zs ? ((long long)(slab_index(k, nz, p) * ny + j) * nx + i) : 0;
@@ -182,42 +217,34 @@ __global__ __launch_bounds__(64 * (BY + 1)) void yee_fused_kernel(FusedArgs a, S
       if (yin && zin) {  // fdtd3d.py:334-341
         float u = hz - hzj, v = hy - hym;
         if (MODE == kCpml) {
-          if (ys) u = cpml_in_out<MODE>(g.psi[kExy] + oy_, a.po[kExy] + oy_, true, g.cpy, ny, j, 0, u);
-          if (zs) v = cpml_in_out<MODE>(g.psi[kExz] + oz_, a.po[kExz] + oz_, true, g.cpz, nz, k, 0, v);
+          if (ys) u = cterm(a, kExy, oy_, true, 1, ny, j, 0, u);
+          if (zs) v = cterm(a, kExz, oz_, true, 2, nz, k, 0, v);
         }
         vx = LOSSY ? ca * vx + cb * (u - v) : vx + cb * (u - v);
       }
       if (xin && zin) {  // fdtd3d.py:343-350
         float u = hx - hxm, v = hz - hzi;
         if (MODE == kCpml) {
-          if (zs) u = cpml_in_out<MODE>(g.psi[kEyz] + oz_, a.po[kEyz] + oz_, true, g.cpz, nz, k, 0, u);
-          if (xs) v = cpml_in_out<MODE>(g.psi[kEyx] + ox_, a.po[kEyx] + ox_, true, g.cpx, nx, i, 0, v);
+          if (zs) u = cterm(a, kEyz, oz_, true, 2, nz, k, 0, u);
+          if (xs) v = cterm(a, kEyx, ox_, true, 0, nx, i, 0, v);
         }
         vy = LOSSY ? ca * vy + cb * (u - v) : vy + cb * (u - v);
       }
       if (xin && yin) {  // fdtd3d.py:352-359
         float u = hy - hyi, v = hx - hxj;
         if (MODE == kCpml) {
-          if (xs) u = cpml_in_out<MODE>(g.psi[kEzx] + ox_, a.po[kEzx] + ox_, true, g.cpx, nx, i, 0, u);
-          if (ys) v = cpml_in_out<MODE>(g.psi[kEzy] + oy_, a.po[kEzy] + oy_, true, g.cpy, ny, j, 0, v);
+          if (xs) u = cterm(a, kEzx, ox_, true, 0, nx, i, 0, u);
+          if (ys) v = cterm(a, kEzy, oy_, true, 1, ny, j, 0, v);
         }
         vz = LOSSY ? ca * vz + cb * (u - v) : vz + cb * (u - v);
       }
       if (MODE == kSponge) {  // fdtd3d.py:361-366
-        const float sig = (sxy + g.ssz[k]) / 3.0f;
+        const float sig = (sxy + a.sponge[nx + ny + k]) / 3.0f;
         if (sig != 0.f) {
-          const float d = expf(((-sig) * g.sdt) / g.seps0);
+          const float d = expf(((-sig) * a.sdt) / a.seps0);
           vx *= d;
           vy *= d;
           vz *= d;
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < YEE_FUSED_SOURCES; ++q) {  // fdtd3d.py:236-244
-        if (q < s.nsrc && s.idx[q] == c) {
-          if (s.comp[q] == 0) vx += s.val[q];
-          else if (s.comp[q] == 1) vy += s.val[q];
-          else vz += s.val[q];
         }
       }
       a.fo[0][c] = vx;
@@ -232,8 +259,7 @@ __global__ __launch_bounds__(64 * (BY + 1)) void yee_fused_kernel(FusedArgs a, S
 constexpr int kFusedBY = 8;
 
 int fused_kz(const YeePlan* p, int gx, int gy) {
-  int kz = 0;
-  if (const char* e = std::getenv("JAXPHYS_AMD_FUSED_KZ")) kz = std::atoi(e);
+  int kz = p->fused_kz;
   if (kz <= 0) {
     const long long want = 2048;
     long long chunks = (want + (long long)gx * gy - 1) / ((long long)gx * gy);
@@ -245,46 +271,63 @@ int fused_kz(const YeePlan* p, int gx, int gy) {
 }
 
 template <int MODE, bool LOSSY>
-void launch_fused_t(const FusedArgs& a, const SrcArgs& s, dim3 grid, int kz, hipStream_t st) {
-  yee_fused_kernel<MODE, LOSSY, kFusedBY><<<grid, 64 * (kFusedBY + 1), 0, st>>>(a, s, kz);
+void launch_fused_t(const FusedArgs& a, dim3 grid, int kz, hipStream_t st) {
+  yee_fused_kernel<MODE, LOSSY, kFusedBY><<<grid, 64 * (kFusedBY + 1), 0, st>>>(a, kz);
 }
 
 int launch_fused(const YeePlan* p, float* const* fin, float* const* fout, float* pin, float* pout,
                  int n, hipStream_t st) {
-  SrcArgs s;
-  int rc = make_src_args(p, n, s);
-  if (rc) return rc;
+  if (p->nsrc > 0 && (n < 0 || n >= p->nsteps_table))
+    return fail(YEE_EINVAL, "step " + std::to_string(n) + " outside the source table (" +
+                                std::to_string(p->nsteps_table) + " steps)");
+  const YeeGridDesc& d = p->d;
   FusedArgs a;
   std::memset(&a, 0, sizeof(a));
-  a.in = make_args(p, fin, pin);
-  const GridArgs go = make_args(p, fout, pout);
-  for (int c = 0; c < 6; ++c) a.fo[c] = fout[c];
-  for (int q = 0; q < kNumPsi; ++q) a.po[q] = go.psi[q];
-  const int gx = (p->d.nx + 63) / 64;
-  const int gy = (p->d.ny + kFusedBY - 1) / kFusedBY;
+  a.nx = d.nx;
+  a.ny = d.ny;
+  a.nz = d.nz;
+  a.p = d.boundary == YEE_BOUNDARY_CPML ? d.npml : 0;
+  a.ch = d.ch;
+  a.sdt = d.sponge_dt;
+  a.seps0 = d.sponge_eps0;
+  a.ysl = psi_slab_floats(d, 0);
+  a.zsl = psi_slab_floats(d, 1);
+  a.xsl = psi_slab_floats(d, 2);
+  for (int c = 0; c < 6; ++c) {
+    a.fi[c] = fin[c];
+    a.fo[c] = fout[c];
+  }
+  a.cb = d.cb;
+  a.ca = d.ca;
+  a.sponge = d.boundary == YEE_BOUNDARY_SPONGE ? d.sponge_sigma : nullptr;
+  a.cpml = d.boundary == YEE_BOUNDARY_CPML ? d.cpml : nullptr;
+  a.pin = pin;
+  a.pout = pout;
+  const int gx = (d.nx + 63) / 64;
+  const int gy = (d.ny + kFusedBY - 1) / kFusedBY;
   const int kz = fused_kz(p, gx, gy);
-  const dim3 grid(gx, gy, (p->d.nz + kz - 1) / kz);
-  const bool lossy = p->d.ca != nullptr;
-  switch (p->d.boundary) {
+  const dim3 grid(gx, gy, (d.nz + kz - 1) / kz);
+  const bool lossy = d.ca != nullptr;
+  switch (d.boundary) {
     case YEE_BOUNDARY_PERIODIC:
-      lossy ? launch_fused_t<kPeriodic, true>(a, s, grid, kz, st)
-            : launch_fused_t<kPeriodic, false>(a, s, grid, kz, st);
+      lossy ? launch_fused_t<kPeriodic, true>(a, grid, kz, st)
+            : launch_fused_t<kPeriodic, false>(a, grid, kz, st);
       break;
     case YEE_BOUNDARY_CPML:
-      lossy ? launch_fused_t<kCpml, true>(a, s, grid, kz, st)
-            : launch_fused_t<kCpml, false>(a, s, grid, kz, st);
+      lossy ? launch_fused_t<kCpml, true>(a, grid, kz, st)
+            : launch_fused_t<kCpml, false>(a, grid, kz, st);
       break;
     case YEE_BOUNDARY_SPONGE:
-      lossy ? launch_fused_t<kSponge, true>(a, s, grid, kz, st)
-            : launch_fused_t<kSponge, false>(a, s, grid, kz, st);
+      lossy ? launch_fused_t<kSponge, true>(a, grid, kz, st)
+            : launch_fused_t<kSponge, false>(a, grid, kz, st);
       break;
     default:
-      lossy ? launch_fused_t<kWalls, true>(a, s, grid, kz, st)
-            : launch_fused_t<kWalls, false>(a, s, grid, kz, st);
+      lossy ? launch_fused_t<kWalls, true>(a, grid, kz, st)
+            : launch_fused_t<kWalls, false>(a, grid, kz, st);
       break;
   }
   YEE_HIP_CHECK(hipGetLastError());
-  return launch_inject(p, go, n, st);
+  return launch_inject(p, fout[0], fout[1], fout[2], n, 0, st);
 }
 
 }  // namespace yee

@@ -81,7 +81,8 @@ inline __device__ int slab_index(int a, int n, int p) {
 // ---------------------------------------------------------------------------
 struct YeePlan {
   YeeGridDesc d;
-  int kz;
+  int kz;            // z planes per block of the two-pass kernels
+  int fused_kz = 0;  // z planes per block of the fused kernel (0: automatic)
   dim3 grid;
   // sources
   int nsrc = 0;
@@ -163,7 +164,9 @@ inline int make_src_args(const YeePlan* p, int n, SrcArgs& s) {
 }
 
 // forward launches (yee.hip)
-int launch_inject(const YeePlan* p, const GridArgs& g, int n, hipStream_t st);
+// sources first..nsrc-1 of step n, one thread, insertion order
+int launch_inject(const YeePlan* p, float* ex, float* ey, float* ez, int n, int first,
+                  hipStream_t st);
 int launch_h(const YeePlan* p, const GridArgs& g, hipStream_t st);
 int launch_e(const YeePlan* p, const GridArgs& g, int n, hipStream_t st);
 
