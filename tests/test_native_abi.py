@@ -74,5 +74,5 @@ def test_plan_and_source_validation_host_only():
         plan.set_sources([2], [(16, 0, 0)], np.zeros((3, 1), np.float32), 3)
     with pytest.raises(ConfigurationError, match="component"):
         plan.set_sources([4], [(1, 1, 1)], np.zeros((3, 1), np.float32), 3)
-    plan.set_sources([2, 0], [(1, 1, 1), (2, 2, 2)], np.zeros((3, 2), np.float32), 3)
+    # (a valid call uploads the table to the device: exercised by the GPU tests)
     plan.close()
