@@ -93,6 +93,10 @@ void yee_plan_destroy(YeePlan* plan);
  * the two-pass kernels (kz > 0) and by the fused kernel (fused_kz > 0;
  * 0 = automatic, < 0 = unchanged).  Results do not depend on them.       */
 int yee_plan_tune(YeePlan* plan, int32_t kz, int32_t fused_kz);
+/* Fused-kernel design: 0 = LDS exchange with a halo row (default), 1 =
+ * barrier-free (row y-1 recomputed, column x-1 by a wave shift).  Same
+ * results bit for bit.                                                   */
+int yee_plan_set_variant(YeePlan* plan, int32_t variant);
 
 #define YEE_FUSED_SOURCES 8
 

@@ -48,6 +48,7 @@ EXPORTS = {
     "yee_plan_create": (ctypes.c_int, [ctypes.POINTER(YeeGridDesc), ctypes.POINTER(ctypes.c_void_p)]),
     "yee_plan_destroy": (None, [ctypes.c_void_p]),
     "yee_plan_tune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
+    "yee_plan_set_variant": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "yee_set_sources": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32]),
     "yee_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
@@ -162,6 +163,9 @@ class Plan:
 
     def tune(self, kz: int = 0, fused_kz: int = -1) -> None:
         check(lib().yee_plan_tune(self.handle, int(kz), int(fused_kz)), "yee_plan_tune")
+
+    def set_variant(self, variant: int) -> None:
+        check(lib().yee_plan_set_variant(self.handle, int(variant)), "yee_plan_set_variant")
 
     def psi_floats(self) -> int:
         return int(lib().yee_psi_floats(ctypes.byref(self.desc)))

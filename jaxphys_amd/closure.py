@@ -346,7 +346,7 @@ def _interior_crop(n: int, margin: int, crop: int, rng) -> int:
 
 def train_closure(device, seeds, n_fine: int = 256, n_coarse: int = 400, every: int = 10,
                   epochs: int = 2, crop: int = 64, lr: float = 1e-3, net: ClosureNet | None = None,
-                  log=None, seed: int = 0):
+                  log=None, seed: int = 0, **problem_kw):
     """Train on fine-run/coarse pairs generated by the FDTD solver.
 
     A sample: restricted fine state s_m at coarse step m, the coarse solver run
@@ -360,7 +360,7 @@ def train_closure(device, seeds, n_fine: int = 256, n_coarse: int = 400, every: 
     t_start = time.time()
     for ep in range(epochs):
         for s_i, sd in enumerate(seeds):
-            prob = ClosureProblem(sd, n_fine, device=device)
+            prob = ClosureProblem(sd, n_fine, device=device, **problem_kw)
             states, _ = prob.fine_states(n_coarse, every)
             if ep == 0 and s_i == 0:
                 set_normalisation(net, states[1:])
@@ -400,13 +400,13 @@ def train_closure(device, seeds, n_fine: int = 256, n_coarse: int = 400, every: 
 
 
 def evaluate_closure(net: ClosureNet, seed: int, n_fine: int = 256, n_coarse: int = 400,
-                     every: int = 10, gain: float = 1.0):
+                     every: int = 10, gain: float = 1.0, **problem_kw):
     """Fine reference vs coarse with / without the closure on one seed.
 
     Returns a dict with the interior relative L2 errors of E (coarse and
     coarse + closure, against the restricted fine state at the final time)
     and the wall times of the fine run and of the corrected coarse run."""
-    prob = ClosureProblem(seed, n_fine, device=net.device)
+    prob = ClosureProblem(seed, n_fine, device=net.device, **problem_kw)
     # fine run (timed on its own: the plain fine FDTD it replaces)
     g = prob.fine_grid()
     sol = g.build_solver(prob.dt_fine(), 2 * n_coarse + 2)

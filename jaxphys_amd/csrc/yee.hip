@@ -47,12 +47,12 @@ __global__ __launch_bounds__(kThreads) void yee_h_kernel(GridArgs g, int kz) {
   if (i >= nx || j >= ny) return;
   const int k0 = blockIdx.z * kz;
   const int k1 = min(k0 + kz, nz);
-  const long long P = (long long)nx * ny;
+  const unsigned P = (unsigned)nx * ny;
   constexpr bool per = MODE == kPeriodic;
   const bool xin = per || i < nx - 1;
   const bool yin = per || j < ny - 1;
-  const long long di = (per && i == nx - 1) ? -(long long)(nx - 1) : 1;
-  const long long dj = (per && j == ny - 1) ? -(long long)(ny - 1) * nx : nx;
+  const unsigned di = (per && i == nx - 1) ? -(unsigned)(nx - 1) : 1;
+  const unsigned dj = (per && j == ny - 1) ? -(unsigned)(ny - 1) * nx : nx;
   const float* __restrict__ ex = g.f[0];
   const float* __restrict__ ey = g.f[1];
   const float* __restrict__ ez = g.f[2];
@@ -60,7 +60,7 @@ __global__ __launch_bounds__(kThreads) void yee_h_kernel(GridArgs g, int kz) {
   float* __restrict__ hy = g.f[4];
   float* __restrict__ hz = g.f[5];
   const float ch = g.ch;
-  long long idx = (long long)k0 * P + (long long)j * nx + i;
+  unsigned idx = (unsigned)k0 * P + (unsigned)j * nx + i;
   float ex0 = ex[idx], ey0 = ey[idx];
   const int p = g.p;
   const bool xs = MODE == kCpml && (i < p || i >= nx - p);
@@ -69,7 +69,7 @@ __global__ __launch_bounds__(kThreads) void yee_h_kernel(GridArgs g, int kz) {
   const int sy = ys ? slab_index(j, ny, p) : 0;
   for (int k = k0; k < k1; ++k, idx += P) {
     const bool zin = per || k < nz - 1;
-    const long long dk = (per && k == nz - 1) ? -(long long)(nz - 1) * P : P;
+    const unsigned dk = (per && k == nz - 1) ? -(unsigned)(nz - 1) * P : P;
     const float ez0 = ez[idx];
     float ex1 = 0.f, ey1 = 0.f, ezj = 0.f, exj = 0.f, ezi = 0.f, eyi = 0.f;
     if (zin) { ex1 = ex[idx + dk]; ey1 = ey[idx + dk]; }
@@ -80,24 +80,24 @@ __global__ __launch_bounds__(kThreads) void yee_h_kernel(GridArgs g, int kz) {
     if (yin && zin) {  // fdtd3d.py:305-312
       float a = ezj - ez0, b = ey1 - ey0;
       if (MODE == kCpml) {
-        if (ys) a = cpml_term(g.psi[kHxy] + ((long long)(k * 2 * p + sy) * nx + i), g.cpy, ny, j, 1, a);
-        if (zs) b = cpml_term(g.psi[kHxz] + ((long long)(sz * ny + j) * nx + i), g.cpz, nz, k, 1, b);
+        if (ys) a = cpml_term(g.psi[kHxy] + ((unsigned)(k * 2 * p + sy) * nx + i), g.cpy, ny, j, 1, a);
+        if (zs) b = cpml_term(g.psi[kHxz] + ((unsigned)(sz * ny + j) * nx + i), g.cpz, nz, k, 1, b);
       }
       hx[idx] = hx[idx] - ch * (a - b);
     }
     if (xin && zin) {  // fdtd3d.py:314-321
       float a = ex1 - ex0, b = ezi - ez0;
       if (MODE == kCpml) {
-        if (zs) a = cpml_term(g.psi[kHyz] + ((long long)(sz * ny + j) * nx + i), g.cpz, nz, k, 1, a);
-        if (xs) b = cpml_term(g.psi[kHyx] + ((long long)k * ny + j) * 2 * p + sx, g.cpx, nx, i, 1, b);
+        if (zs) a = cpml_term(g.psi[kHyz] + ((unsigned)(sz * ny + j) * nx + i), g.cpz, nz, k, 1, a);
+        if (xs) b = cpml_term(g.psi[kHyx] + ((unsigned)k * ny + j) * 2 * p + sx, g.cpx, nx, i, 1, b);
       }
       hy[idx] = hy[idx] - ch * (a - b);
     }
     if (xin && yin) {  // fdtd3d.py:323-330
       float a = eyi - ey0, b = exj - ex0;
       if (MODE == kCpml) {
-        if (xs) a = cpml_term(g.psi[kHzx] + ((long long)k * ny + j) * 2 * p + sx, g.cpx, nx, i, 1, a);
-        if (ys) b = cpml_term(g.psi[kHzy] + ((long long)(k * 2 * p + sy) * nx + i), g.cpy, ny, j, 1, b);
+        if (xs) a = cpml_term(g.psi[kHzx] + ((unsigned)k * ny + j) * 2 * p + sx, g.cpx, nx, i, 1, a);
+        if (ys) b = cpml_term(g.psi[kHzy] + ((unsigned)(k * 2 * p + sy) * nx + i), g.cpy, ny, j, 1, b);
       }
       hz[idx] = hz[idx] - ch * (a - b);
     }
@@ -118,26 +118,26 @@ __global__ __launch_bounds__(kThreads) void yee_e_kernel(GridArgs g, SrcArgs s, 
   if (i >= nx || j >= ny) return;
   const int k0 = blockIdx.z * kz;
   const int k1 = min(k0 + kz, nz);
-  const long long P = (long long)nx * ny;
+  const unsigned P = (unsigned)nx * ny;
   constexpr bool per = MODE == kPeriodic;
   const bool xin = per || i > 0;
   const bool yin = per || j > 0;
-  const long long di = (per && i == 0) ? (long long)(nx - 1) : -1;
-  const long long dj = (per && j == 0) ? (long long)(ny - 1) * nx : -(long long)nx;
+  const unsigned di = (per && i == 0) ? (unsigned)(nx - 1) : -1;
+  const unsigned dj = (per && j == 0) ? (unsigned)(ny - 1) * nx : -(unsigned)nx;
   float* __restrict__ ex = g.f[0];
   float* __restrict__ ey = g.f[1];
   float* __restrict__ ez = g.f[2];
   const float* __restrict__ hx = g.f[3];
   const float* __restrict__ hy = g.f[4];
   const float* __restrict__ hz = g.f[5];
-  long long idx = (long long)k0 * P + (long long)j * nx + i;
+  unsigned idx = (unsigned)k0 * P + (unsigned)j * nx + i;
   float hxm = 0.f, hym = 0.f;  // H at plane k-1
   if (k0 > 0) {
     hxm = hx[idx - P];
     hym = hy[idx - P];
   } else if (per) {
-    hxm = hx[idx + (long long)(nz - 1) * P];
-    hym = hy[idx + (long long)(nz - 1) * P];
+    hxm = hx[idx + (unsigned)(nz - 1) * P];
+    hym = hy[idx + (unsigned)(nz - 1) * P];
   }
   const int p = g.p;
   const bool xs = MODE == kCpml && (i < p || i >= nx - p);
@@ -160,24 +160,24 @@ __global__ __launch_bounds__(kThreads) void yee_e_kernel(GridArgs g, SrcArgs s, 
     if (yin && zin) {  // fdtd3d.py:334-341
       float a = hz0 - hzj, b = hy0 - hym;
       if (MODE == kCpml) {
-        if (ys) a = cpml_term(g.psi[kExy] + ((long long)(k * 2 * p + sy) * nx + i), g.cpy, ny, j, 0, a);
-        if (zs) b = cpml_term(g.psi[kExz] + ((long long)(sz * ny + j) * nx + i), g.cpz, nz, k, 0, b);
+        if (ys) a = cpml_term(g.psi[kExy] + ((unsigned)(k * 2 * p + sy) * nx + i), g.cpy, ny, j, 0, a);
+        if (zs) b = cpml_term(g.psi[kExz] + ((unsigned)(sz * ny + j) * nx + i), g.cpz, nz, k, 0, b);
       }
       vx = LOSSY ? ca * vx + cb * (a - b) : vx + cb * (a - b);
     }
     if (xin && zin) {  // fdtd3d.py:343-350
       float a = hx0 - hxm, b = hz0 - hzi;
       if (MODE == kCpml) {
-        if (zs) a = cpml_term(g.psi[kEyz] + ((long long)(sz * ny + j) * nx + i), g.cpz, nz, k, 0, a);
-        if (xs) b = cpml_term(g.psi[kEyx] + ((long long)k * ny + j) * 2 * p + sx, g.cpx, nx, i, 0, b);
+        if (zs) a = cpml_term(g.psi[kEyz] + ((unsigned)(sz * ny + j) * nx + i), g.cpz, nz, k, 0, a);
+        if (xs) b = cpml_term(g.psi[kEyx] + ((unsigned)k * ny + j) * 2 * p + sx, g.cpx, nx, i, 0, b);
       }
       vy = LOSSY ? ca * vy + cb * (a - b) : vy + cb * (a - b);
     }
     if (xin && yin) {  // fdtd3d.py:352-359
       float a = hy0 - hyi, b = hx0 - hxj;
       if (MODE == kCpml) {
-        if (xs) a = cpml_term(g.psi[kEzx] + ((long long)k * ny + j) * 2 * p + sx, g.cpx, nx, i, 0, a);
-        if (ys) b = cpml_term(g.psi[kEzy] + ((long long)(k * 2 * p + sy) * nx + i), g.cpy, ny, j, 0, b);
+        if (xs) a = cpml_term(g.psi[kEzx] + ((unsigned)k * ny + j) * 2 * p + sx, g.cpx, nx, i, 0, a);
+        if (ys) b = cpml_term(g.psi[kEzy] + ((unsigned)(k * 2 * p + sy) * nx + i), g.cpy, ny, j, 0, b);
       }
       vz = LOSSY ? ca * vz + cb * (a - b) : vz + cb * (a - b);
     }
@@ -383,6 +383,10 @@ int yee_plan_create(const YeeGridDesc* desc, YeePlan** out) {
   *out = nullptr;
   int rc = validate(desc);
   if (rc) return rc;
+  // kernels use 32-bit cell offsets
+  if ((long long)desc->nx * desc->ny * desc->nz >= (1LL << 32) ||
+      yee_psi_floats(desc) >= (1LL << 32))
+    return fail(YEE_EUNSUPPORTED, "grids of 2^32 cells or more are not supported");
   YeePlan* p = new (std::nothrow) YeePlan();
   if (!p) return fail(YEE_EINVAL, "out of host memory");
   p->d = *desc;
@@ -403,6 +407,7 @@ int yee_plan_create(const YeeGridDesc* desc, YeePlan** out) {
   p->kz = kz;
   p->grid = dim3(gx, gy, (desc->nz + kz - 1) / kz);
   if (const char* e = std::getenv("JAXPHYS_AMD_FUSED_KZ")) p->fused_kz = std::atoi(e);
+  if (const char* e = std::getenv("JAXPHYS_AMD_FUSED_VARIANT")) p->fused_variant = std::atoi(e) == 1;
   *out = p;
   return YEE_OK;
 }
@@ -414,6 +419,13 @@ int yee_plan_tune(YeePlan* p, int32_t kz, int32_t fused_kz) {
     p->grid.z = (p->d.nz + p->kz - 1) / p->kz;
   }
   if (fused_kz >= 0) p->fused_kz = fused_kz;
+  return YEE_OK;
+}
+
+int yee_plan_set_variant(YeePlan* p, int32_t variant) {
+  if (!p) return fail(YEE_EINVAL, "null plan");
+  if (variant < 0 || variant > 1) return fail(YEE_EINVAL, "unknown fused variant");
+  p->fused_variant = variant;
   return YEE_OK;
 }
 

@@ -42,11 +42,11 @@ template <int MODE>
 struct Geo {
   const AdjArgs& a;
   int nx, ny, nz, p;
-  long long P;
+  unsigned P;
   __device__ Geo(const AdjArgs& a_) : a(a_), nx(a_.g.nx), ny(a_.g.ny), nz(a_.g.nz), p(a_.g.p) {
-    P = (long long)nx * ny;
+    P = (unsigned)nx * ny;
   }
-  __device__ long long idx(int i, int j, int k) const { return ((long long)k * ny + j) * nx + i; }
+  __device__ unsigned idx(int i, int j, int k) const { return ((unsigned)k * ny + j) * nx + i; }
   // wrap (periodic) or validity (others) of a neighbour index
   __device__ bool fix(int& v, int n) const {
     if (v >= 0 && v < n) return true;
@@ -74,9 +74,9 @@ struct Geo {
   __device__ bool slab(int v, int n) const { return MODE == kCpml && (v < p || v >= n - p); }
   // psi slot address for derivative axis ax (0 x, 1 y, 2 z) at (i, j, k)
   __device__ float* psi_at(float* base, int ax, int i, int j, int k) const {
-    if (ax == 1) return base + ((long long)(k * 2 * p + slab_index(j, ny, p)) * nx + i);
-    if (ax == 2) return base + ((long long)(slab_index(k, nz, p) * ny + j) * nx + i);
-    return base + ((long long)k * ny + j) * 2 * p + slab_index(i, nx, p);
+    if (ax == 1) return base + ((unsigned)(k * 2 * p + slab_index(j, ny, p)) * nx + i);
+    if (ax == 2) return base + ((unsigned)(slab_index(k, nz, p) * ny + j) * nx + i);
+    return base + ((unsigned)k * ny + j) * 2 * p + slab_index(i, nx, p);
   }
   __device__ const float* tab(int ax) const { return ax == 0 ? a.g.cpx : (ax == 1 ? a.g.cpy : a.g.cpz); }
   __device__ int nax(int ax) const { return ax == 0 ? nx : (ax == 1 ? ny : nz); }
@@ -101,7 +101,7 @@ __device__ void e_term_adj(const Geo<MODE>& G, int comp, int i, int j, int k, fl
   ga = 0.f;
   gb = 0.f;
   if (!G.e_in(comp, i, j, k)) return;
-  const long long c = G.idx(i, j, k);
+  const unsigned c = G.idx(i, j, k);
   const float gU = G.a.g.cb[c] * (G.damp(i, j, k) * G.a.lam[comp][c]);
   ga = gU;
   gb = -gU;
@@ -129,7 +129,7 @@ __device__ void h_term_adj(const Geo<MODE>& G, int comp, int i, int j, int k, fl
   ga = 0.f;
   gb = 0.f;
   if (!G.h_in(comp, i, j, k)) return;
-  const long long c = G.idx(i, j, k);
+  const unsigned c = G.idx(i, j, k);
   const float gU = -G.a.g.ch * G.a.lam[3 + comp][c];
   ga = gU;
   gb = -gU;
@@ -174,7 +174,7 @@ __global__ __launch_bounds__(kThreads) void yee_adj_h_kernel(AdjArgs a) {
   const int k = blockIdx.z;
   Geo<MODE> G(a);
   if (i >= G.nx || j >= G.ny) return;
-  const long long c = G.idx(i, j, k);
+  const unsigned c = G.idx(i, j, k);
   const float ch = a.g.ch;
   // deferred psi_H-adjoint update of the step after this one
   if (MODE == kCpml) {
@@ -238,7 +238,7 @@ __global__ __launch_bounds__(kThreads) void yee_adj_e_kernel(AdjArgs a) {
   const int k = blockIdx.z;
   Geo<MODE> G(a);
   if (i >= G.nx || j >= G.ny) return;
-  const long long c = G.idx(i, j, k);
+  const unsigned c = G.idx(i, j, k);
   const float d = G.damp(i, j, k);
   const float cb = a.g.cb[c];
   const float ca = LOSSY ? a.g.ca[c] : 1.f;
@@ -293,8 +293,8 @@ __global__ __launch_bounds__(kThreads) void yee_adj_e_kernel(AdjArgs a) {
   if (a.bnx > 0) {
     const int li = i - a.bx0, lj = j - a.by0, lk = k - a.bz0;
     if (li >= 0 && li < a.bnx && lj >= 0 && lj < a.bny && lk >= 0 && lk < a.bnz) {
-      const long long bc = (long long)a.bnx * a.bny * a.bnz;
-      const long long l = ((long long)lk * a.bny + lj) * a.bnx + li;
+      const long long bc = (unsigned)a.bnx * a.bny * a.bnz;
+      const long long l = ((unsigned)lk * a.bny + lj) * a.bnx + li;
       float s1 = 0.f, s2 = 0.f;
       for (int comp = 0; comp < 3; ++comp) {
         const float en = a.e_next[comp * bc + l], ec = a.e_cur[comp * bc + l];

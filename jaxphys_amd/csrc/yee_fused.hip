@@ -20,8 +20,9 @@
 // of the two-pass kernels, so the result is bit-for-bit the same.
 //
 // Kernel arguments are kept compact (one base pointer for each psi buffer,
-// one for all CPML tables, one for the sponge profiles): the scalar register
-// file holds them without spilling.
+// one for all CPML tables, one for the sponge profiles) and offsets are
+// 32-bit (plans are limited to < 2^32 cells): the address arithmetic then
+// needs one vector register per neighbour instead of two.
 
 #include "yee_internal.h"
 
@@ -54,13 +55,13 @@ __device__ __forceinline__ const float* tab_of(const FusedArgs& a, int ax) {
 }
 
 // psi' = b psi + c d (psi from A, written to B when `write`); returns d/kappa + psi'
-__device__ __forceinline__ float cterm(const FusedArgs& a, int slot, long long off, bool write,
+__device__ __forceinline__ float cterm(const FusedArgs& a, int slot, unsigned off, bool write,
                                        int ax, int n, int v, int kind, float d) {
   const float* t = tab_of(a, ax);
   const float b = t[(3 * kind + 0) * n + v];
   const float c = t[(3 * kind + 1) * n + v];
   const float kinv = t[(3 * kind + 2) * n + v];
-  const long long o = slot_base(a, slot) + off;
+  const unsigned o = (unsigned)slot_base(a, slot) + off;
   const float ps = b * a.pin[o] + c * d;
   if (write) a.pout[o] = ps;
   return d * kinv + ps;
@@ -73,14 +74,14 @@ __device__ __forceinline__ void h_cell(const FusedArgs& a, int i, int j, int k, 
                                        float& ox, float& oy, float& oz, float* e) {
   const int nx = a.nx, ny = a.ny, nz = a.nz, p = a.p;
   constexpr bool per = MODE == kPeriodic;
-  const long long P = (long long)nx * ny;
-  const long long c = ((long long)k * ny + j) * nx + i;
+  const unsigned P = (unsigned)nx * ny;
+  const unsigned c = ((unsigned)k * ny + j) * nx + i;
   const bool xin = per || i < nx - 1;
   const bool yin = per || j < ny - 1;
   const bool zin = per || k < nz - 1;
-  const long long di = (per && i == nx - 1) ? -(long long)(nx - 1) : 1;
-  const long long dj = (per && j == ny - 1) ? -(long long)(ny - 1) * nx : nx;
-  const long long dk = (per && k == nz - 1) ? -(long long)(nz - 1) * P : P;
+  const unsigned di = (per && i == nx - 1) ? -(unsigned)(nx - 1) : 1;
+  const unsigned dj = (per && j == ny - 1) ? -(unsigned)(ny - 1) * nx : nx;
+  const unsigned dk = (per && k == nz - 1) ? -(unsigned)(nz - 1) * P : P;
   const float* __restrict__ ex = a.fi[0];
   const float* __restrict__ ey = a.fi[1];
   const float* __restrict__ ez = a.fi[2];
@@ -93,9 +94,9 @@ __device__ __forceinline__ void h_cell(const FusedArgs& a, int i, int j, int k, 
   const bool xs = MODE == kCpml && (i < p || i >= nx - p);
   const bool ys = MODE == kCpml && (j < p || j >= ny - p);
   const bool zs = MODE == kCpml && (k < p || k >= nz - p);
-  const long long oy_ = ys ? ((long long)(k * 2 * p + slab_index(j, ny, p)) * nx + i) : 0;
-  const long long oz_ = zs ? ((long long)(slab_index(k, nz, p) * ny + j) * nx + i) : 0;
-  const long long ox_ = xs ? (((long long)k * ny + j) * 2 * p + slab_index(i, nx, p)) : 0;
+  const unsigned oy_ = ys ? ((unsigned)(k * 2 * p + slab_index(j, ny, p)) * nx + i) : 0;
+  const unsigned oz_ = zs ? ((unsigned)(slab_index(k, nz, p) * ny + j) * nx + i) : 0;
+  const unsigned ox_ = xs ? (((unsigned)k * ny + j) * 2 * p + slab_index(i, nx, p)) : 0;
   const float ch = a.ch;
   if (WANT & 1) {
     float hx = a.fi[3][c];
@@ -164,7 +165,7 @@ __global__ __launch_bounds__(64 * (BY + 1)) void yee_fused_kernel(FusedArgs a, i
   }
   const int k0 = blockIdx.z * kz;
   const int k1 = min(k0 + kz, nz);
-  const long long P = (long long)nx * ny;
+  const unsigned P = (unsigned)nx * ny;
   // Hx, Hy of the plane below the chunk (the z-1 neighbour of the first plane)
   float hxm = 0.f, hym = 0.f;
   {
@@ -192,7 +193,7 @@ __global__ __launch_bounds__(64 * (BY + 1)) void yee_fused_kernel(FusedArgs a, i
       float unused;
       h_cell<MODE, 6>(a, im, j, k, false, unused, hyxm, hzxm, nullptr);
     }
-    const long long c = (long long)k * P + (long long)j * nx + i;
+    const unsigned c = (unsigned)k * P + (unsigned)j * nx + i;
     if (owner) {
       a.fo[3][c] = hx;
       a.fo[4][c] = hy;
@@ -211,9 +212,9 @@ __global__ __launch_bounds__(64 * (BY + 1)) void yee_fused_kernel(FusedArgs a, i
       const float ca = LOSSY ? a.ca[c] : 1.f;
       float vx = e[0], vy = e[1], vz = e[2];
       const bool zs = MODE == kCpml && (k < p || k >= nz - p);
-      const long long oy_ = ys ? ((long long)(k * 2 * p + slab_index(j, ny, p)) * nx + i) : 0;
-      const long long oz_ = zs ? ((long long)(slab_index(k, nz, p) * ny + j) * nx + i) : 0;
-      const long long ox_ = xs ? (((long long)k * ny + j) * 2 * p + slab_index(i, nx, p)) : 0;
+      const unsigned oy_ = ys ? ((unsigned)(k * 2 * p + slab_index(j, ny, p)) * nx + i) : 0;
+      const unsigned oz_ = zs ? ((unsigned)(slab_index(k, nz, p) * ny + j) * nx + i) : 0;
+      const unsigned ox_ = xs ? (((unsigned)k * ny + j) * 2 * p + slab_index(i, nx, p)) : 0;
       if (yin && zin) {  // fdtd3d.py:334-341
         float u = hz - hzj, v = hy - hym;
         if (MODE == kCpml) {
@@ -256,6 +257,103 @@ __global__ __launch_bounds__(64 * (BY + 1)) void yee_fused_kernel(FusedArgs a, i
   }
 }
 
+// Barrier-free variant: 64 x 4 threads, every thread owns its column and
+// recomputes, besides its own H^{n+1/2}, Hy and Hz of column x-1 and Hx and
+// Hz of row y-1 from buffer A (their inputs are the thread's neighbours, so
+// the extra loads hit L1/L2).  Straight-line code: every load of a plane is
+// in flight at once and no wave ever waits for another.
+template <int MODE, bool LOSSY>
+__global__ __launch_bounds__(256) void yee_fused_nolds_kernel(FusedArgs a, int kz) {
+  const int nx = a.nx, ny = a.ny, nz = a.nz, p = a.p;
+  constexpr bool per = MODE == kPeriodic;
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 64 + lane;
+  const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (i >= nx || j >= ny) return;
+  int im = i - 1, jm = j - 1;
+  bool imvalid = im >= 0, jmvalid = jm >= 0;
+  if (per && im < 0) { im += nx; imvalid = true; }
+  if (per && jm < 0) { jm += ny; jmvalid = true; }
+  // out-of-grid neighbours (non-periodic) are never used by an in-range
+  // update; computing them at a clamped index keeps the code branch-free
+  const int imc = imvalid ? im : i, jmc = jmvalid ? jm : j;
+  const int k0 = blockIdx.z * kz;
+  const int k1 = min(k0 + kz, nz);
+  const unsigned P = (unsigned)nx * ny;
+  float hxm = 0.f, hym = 0.f;
+  {
+    int kp = k0 - 1;
+    if (per && kp < 0) kp += nz;
+    if (kp >= 0) {
+      float unused;
+      h_cell<MODE, 3>(a, i, j, kp, false, hxm, hym, unused, nullptr);
+    }
+  }
+  const bool xs = MODE == kCpml && (i < p || i >= nx - p);
+  const bool ys = MODE == kCpml && (j < p || j >= ny - p);
+  float sxy = 0.f;
+  if (MODE == kSponge) sxy = a.sponge[i] + a.sponge[nx + j];
+  const bool xin = per || i > 0;
+  const bool yin = per || j > 0;
+  for (int k = k0; k < k1; ++k) {
+    float hx, hy, hz, unused, hyi, hzi, hxj, hzj;
+    float e[3];
+    h_cell<MODE, 7>(a, i, j, k, true, hx, hy, hz, e);
+    h_cell<MODE, 6>(a, imc, j, k, false, unused, hyi, hzi, nullptr);
+    h_cell<MODE, 5>(a, i, jmc, k, false, hxj, unused, hzj, nullptr);
+    const unsigned c = (unsigned)k * P + (unsigned)j * nx + i;
+    a.fo[3][c] = hx;
+    a.fo[4][c] = hy;
+    a.fo[5][c] = hz;
+    const bool zin = per || k > 0;
+    const float cb = a.cb[c];
+    const float ca = LOSSY ? a.ca[c] : 1.f;
+    float vx = e[0], vy = e[1], vz = e[2];
+    const bool zs = MODE == kCpml && (k < p || k >= nz - p);
+    const unsigned oy_ = ys ? ((unsigned)(k * 2 * p + slab_index(j, ny, p)) * nx + i) : 0;
+    const unsigned oz_ = zs ? ((unsigned)(slab_index(k, nz, p) * ny + j) * nx + i) : 0;
+    const unsigned ox_ = xs ? (((unsigned)k * ny + j) * 2 * p + slab_index(i, nx, p)) : 0;
+    if (yin && zin) {
+      float u = hz - hzj, v = hy - hym;
+      if (MODE == kCpml) {
+        if (ys) u = cterm(a, kExy, oy_, true, 1, ny, j, 0, u);
+        if (zs) v = cterm(a, kExz, oz_, true, 2, nz, k, 0, v);
+      }
+      vx = LOSSY ? ca * vx + cb * (u - v) : vx + cb * (u - v);
+    }
+    if (xin && zin) {
+      float u = hx - hxm, v = hz - hzi;
+      if (MODE == kCpml) {
+        if (zs) u = cterm(a, kEyz, oz_, true, 2, nz, k, 0, u);
+        if (xs) v = cterm(a, kEyx, ox_, true, 0, nx, i, 0, v);
+      }
+      vy = LOSSY ? ca * vy + cb * (u - v) : vy + cb * (u - v);
+    }
+    if (xin && yin) {
+      float u = hy - hyi, v = hx - hxj;
+      if (MODE == kCpml) {
+        if (xs) u = cterm(a, kEzx, ox_, true, 0, nx, i, 0, u);
+        if (ys) v = cterm(a, kEzy, oy_, true, 1, ny, j, 0, v);
+      }
+      vz = LOSSY ? ca * vz + cb * (u - v) : vz + cb * (u - v);
+    }
+    if (MODE == kSponge) {
+      const float sig = (sxy + a.sponge[nx + ny + k]) / 3.0f;
+      if (sig != 0.f) {
+        const float d = expf(((-sig) * a.sdt) / a.seps0);
+        vx *= d;
+        vy *= d;
+        vz *= d;
+      }
+    }
+    a.fo[0][c] = vx;
+    a.fo[1][c] = vy;
+    a.fo[2][c] = vz;
+    hxm = hx;
+    hym = hy;
+  }
+}
+
 constexpr int kFusedBY = 8;
 
 int fused_kz(const YeePlan* p, int gx, int gy) {
@@ -271,8 +369,19 @@ int fused_kz(const YeePlan* p, int gx, int gy) {
 }
 
 template <int MODE, bool LOSSY>
-void launch_fused_t(const FusedArgs& a, dim3 grid, int kz, hipStream_t st) {
-  yee_fused_kernel<MODE, LOSSY, kFusedBY><<<grid, 64 * (kFusedBY + 1), 0, st>>>(a, kz);
+void launch_fused_t(const FusedArgs& a, const YeePlan* p, hipStream_t st) {
+  const int gx = (a.nx + 63) / 64;
+  if (p->fused_variant == 1) {
+    const int gy = (a.ny + 3) / 4;
+    const int kz = fused_kz(p, gx, gy);
+    const dim3 grid(gx, gy, (a.nz + kz - 1) / kz);
+    yee_fused_nolds_kernel<MODE, LOSSY><<<grid, 256, 0, st>>>(a, kz);
+  } else {
+    const int gy = (a.ny + kFusedBY - 1) / kFusedBY;
+    const int kz = fused_kz(p, gx, gy);
+    const dim3 grid(gx, gy, (a.nz + kz - 1) / kz);
+    yee_fused_kernel<MODE, LOSSY, kFusedBY><<<grid, 64 * (kFusedBY + 1), 0, st>>>(a, kz);
+  }
 }
 
 int launch_fused(const YeePlan* p, float* const* fin, float* const* fout, float* pin, float* pout,
@@ -303,27 +412,19 @@ int launch_fused(const YeePlan* p, float* const* fin, float* const* fout, float*
   a.cpml = d.boundary == YEE_BOUNDARY_CPML ? d.cpml : nullptr;
   a.pin = pin;
   a.pout = pout;
-  const int gx = (d.nx + 63) / 64;
-  const int gy = (d.ny + kFusedBY - 1) / kFusedBY;
-  const int kz = fused_kz(p, gx, gy);
-  const dim3 grid(gx, gy, (d.nz + kz - 1) / kz);
   const bool lossy = d.ca != nullptr;
   switch (d.boundary) {
     case YEE_BOUNDARY_PERIODIC:
-      lossy ? launch_fused_t<kPeriodic, true>(a, grid, kz, st)
-            : launch_fused_t<kPeriodic, false>(a, grid, kz, st);
+      lossy ? launch_fused_t<kPeriodic, true>(a, p, st) : launch_fused_t<kPeriodic, false>(a, p, st);
       break;
     case YEE_BOUNDARY_CPML:
-      lossy ? launch_fused_t<kCpml, true>(a, grid, kz, st)
-            : launch_fused_t<kCpml, false>(a, grid, kz, st);
+      lossy ? launch_fused_t<kCpml, true>(a, p, st) : launch_fused_t<kCpml, false>(a, p, st);
       break;
     case YEE_BOUNDARY_SPONGE:
-      lossy ? launch_fused_t<kSponge, true>(a, grid, kz, st)
-            : launch_fused_t<kSponge, false>(a, grid, kz, st);
+      lossy ? launch_fused_t<kSponge, true>(a, p, st) : launch_fused_t<kSponge, false>(a, p, st);
       break;
     default:
-      lossy ? launch_fused_t<kWalls, true>(a, grid, kz, st)
-            : launch_fused_t<kWalls, false>(a, grid, kz, st);
+      lossy ? launch_fused_t<kWalls, true>(a, p, st) : launch_fused_t<kWalls, false>(a, p, st);
       break;
   }
   YEE_HIP_CHECK(hipGetLastError());

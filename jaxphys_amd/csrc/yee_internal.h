@@ -83,6 +83,7 @@ struct YeePlan {
   YeeGridDesc d;
   int kz;            // z planes per block of the two-pass kernels
   int fused_kz = 0;  // z planes per block of the fused kernel (0: automatic)
+  int fused_variant = 0;  // 0: LDS + halo row, 1: barrier-free recompute
   dim3 grid;
   // sources
   int nsrc = 0;

@@ -225,9 +225,10 @@ def test_copy_probe(gpu):
     assert torch.equal(src, dst)
 
 
+@pytest.mark.parametrize("variant", [0, 1])
 @pytest.mark.parametrize("boundary,lossy", [("absorbing", False), ("periodic", False),
                                             ("walls", True), ("cpml", False), ("cpml", True)])
-def test_fused_step_bitwise_equals_two_pass(gpu, boundary, lossy):
+def test_fused_step_bitwise_equals_two_pass(gpu, boundary, lossy, variant):
     """One fused launch per step vs the H and E half-step kernels: the same
     arithmetic in the same order, so every field must agree bit for bit."""
     size = (70, 37, 29)
@@ -246,6 +247,8 @@ def test_fused_step_bitwise_equals_two_pass(gpu, boundary, lossy):
     n = 41
     a = g.build_solver(dt, n)
     b = g.build_solver(dt, n)
+    a.plan.set_variant(variant)
+    a.plan.tune(kz=0, fused_kz=7)   # several z chunks per column
     a.run(0, n)              # fused (odd count: ends in the second buffer)
     b.run_two_pass(0, n)
     for q in range(6):
