@@ -40,7 +40,13 @@ __device__ __forceinline__ float cpml_term(float* __restrict__ ps, const float* 
 // H half-step: H -= ch * curl(E) (forward differences).
 // ---------------------------------------------------------------------------
 template <int MODE>
-__global__ __launch_bounds__(kThreads) void yee_h_kernel(GridArgs g, int kz) {
+__global__ __launch_bounds__(kThreads) void yee_h_kernel(GridArgs g, int kz,
+                                                        const float* __restrict__ ex,
+                                                        const float* __restrict__ ey,
+                                                        const float* __restrict__ ez,
+                                                        float* __restrict__ hx,
+                                                        float* __restrict__ hy,
+                                                        float* __restrict__ hz) {
   const int i = blockIdx.x * kBX + (threadIdx.x % kBX);
   const int j = blockIdx.y * kBY + (threadIdx.x / kBX);
   const int nx = g.nx, ny = g.ny, nz = g.nz;
@@ -53,12 +59,6 @@ __global__ __launch_bounds__(kThreads) void yee_h_kernel(GridArgs g, int kz) {
   const bool yin = per || j < ny - 1;
   const unsigned di = (per && i == nx - 1) ? -(unsigned)(nx - 1) : 1;
   const unsigned dj = (per && j == ny - 1) ? -(unsigned)(ny - 1) * nx : nx;
-  const float* __restrict__ ex = g.f[0];
-  const float* __restrict__ ey = g.f[1];
-  const float* __restrict__ ez = g.f[2];
-  float* __restrict__ hx = g.f[3];
-  float* __restrict__ hy = g.f[4];
-  float* __restrict__ hz = g.f[5];
   const float ch = g.ch;
   unsigned idx = (unsigned)k0 * P + (unsigned)j * nx + i;
   float ex0 = ex[idx], ey0 = ey[idx];
@@ -67,42 +67,73 @@ __global__ __launch_bounds__(kThreads) void yee_h_kernel(GridArgs g, int kz) {
   const bool ys = MODE == kCpml && (j < p || j >= ny - p);
   const int sx = xs ? slab_index(i, nx, p) : 0;
   const int sy = ys ? slab_index(j, ny, p) : 0;
+  // branch-free neighbour offsets: an out-of-range neighbour is read at the
+  // cell itself and the update is then not selected
+  const unsigned dic = xin ? di : 0u;
+  const unsigned djc = yin ? dj : 0u;
+  // software pipeline: the loads of plane k+1 are issued before the
+  // arithmetic and stores of plane k
+  auto dk_of = [&](int k) -> unsigned {
+    return (per && k == nz - 1) ? -(unsigned)(nz - 1) * P : ((per || k < nz - 1) ? P : 0u);
+  };
+  float c_ez0, c_ex1, c_ey1, c_ezj, c_exj, c_ezi, c_eyi, c_hx0, c_hy0, c_hz0;
+  {
+    const unsigned dk = dk_of(k0);
+    c_ez0 = ez[idx];
+    c_ex1 = ex[idx + dk];
+    c_ey1 = ey[idx + dk];
+    c_ezj = ez[idx + djc];
+    c_exj = ex[idx + djc];
+    c_ezi = ez[idx + dic];
+    c_eyi = ey[idx + dic];
+    c_hx0 = hx[idx];
+    c_hy0 = hy[idx];
+    c_hz0 = hz[idx];
+  }
   for (int k = k0; k < k1; ++k, idx += P) {
+    float n_ez0 = 0.f, n_ex1 = 0.f, n_ey1 = 0.f, n_ezj = 0.f, n_exj = 0.f, n_ezi = 0.f,
+          n_eyi = 0.f, n_hx0 = 0.f, n_hy0 = 0.f, n_hz0 = 0.f;
+    if (k + 1 < k1) {
+      const unsigned q = idx + P;
+      const unsigned dk = dk_of(k + 1);
+      n_ez0 = ez[q];
+      n_ex1 = ex[q + dk];
+      n_ey1 = ey[q + dk];
+      n_ezj = ez[q + djc];
+      n_exj = ex[q + djc];
+      n_ezi = ez[q + dic];
+      n_eyi = ey[q + dic];
+      n_hx0 = hx[q];
+      n_hy0 = hy[q];
+      n_hz0 = hz[q];
+    }
     const bool zin = per || k < nz - 1;
-    const unsigned dk = (per && k == nz - 1) ? -(unsigned)(nz - 1) * P : P;
-    const float ez0 = ez[idx];
-    float ex1 = 0.f, ey1 = 0.f, ezj = 0.f, exj = 0.f, ezi = 0.f, eyi = 0.f;
-    if (zin) { ex1 = ex[idx + dk]; ey1 = ey[idx + dk]; }
-    if (yin) { ezj = ez[idx + dj]; exj = ex[idx + dj]; }
-    if (xin) { ezi = ez[idx + di]; eyi = ey[idx + di]; }
     const bool zs = MODE == kCpml && (k < p || k >= nz - p);
     const int sz = zs ? slab_index(k, nz, p) : 0;
-    if (yin && zin) {  // fdtd3d.py:305-312
-      float a = ezj - ez0, b = ey1 - ey0;
-      if (MODE == kCpml) {
-        if (ys) a = cpml_term(g.psi[kHxy] + ((unsigned)(k * 2 * p + sy) * nx + i), g.cpy, ny, j, 1, a);
-        if (zs) b = cpml_term(g.psi[kHxz] + ((unsigned)(sz * ny + j) * nx + i), g.cpz, nz, k, 1, b);
+    float ax = c_ezj - c_ez0, bx = c_ey1 - ey0;  // fdtd3d.py:305-312
+    float ay = c_ex1 - ex0, by = c_ezi - c_ez0;  // fdtd3d.py:314-321
+    float az = c_eyi - ey0, bz = c_exj - ex0;    // fdtd3d.py:323-330
+    if (MODE == kCpml) {
+      if (yin && zin) {
+        if (ys) ax = cpml_term(g.psi[kHxy] + ((unsigned)(k * 2 * p + sy) * nx + i), g.cpy, ny, j, 1, ax);
+        if (zs) bx = cpml_term(g.psi[kHxz] + ((unsigned)(sz * ny + j) * nx + i), g.cpz, nz, k, 1, bx);
       }
-      hx[idx] = hx[idx] - ch * (a - b);
-    }
-    if (xin && zin) {  // fdtd3d.py:314-321
-      float a = ex1 - ex0, b = ezi - ez0;
-      if (MODE == kCpml) {
-        if (zs) a = cpml_term(g.psi[kHyz] + ((unsigned)(sz * ny + j) * nx + i), g.cpz, nz, k, 1, a);
-        if (xs) b = cpml_term(g.psi[kHyx] + ((unsigned)k * ny + j) * 2 * p + sx, g.cpx, nx, i, 1, b);
+      if (xin && zin) {
+        if (zs) ay = cpml_term(g.psi[kHyz] + ((unsigned)(sz * ny + j) * nx + i), g.cpz, nz, k, 1, ay);
+        if (xs) by = cpml_term(g.psi[kHyx] + ((unsigned)k * ny + j) * 2 * p + sx, g.cpx, nx, i, 1, by);
       }
-      hy[idx] = hy[idx] - ch * (a - b);
-    }
-    if (xin && yin) {  // fdtd3d.py:323-330
-      float a = eyi - ey0, b = exj - ex0;
-      if (MODE == kCpml) {
-        if (xs) a = cpml_term(g.psi[kHzx] + ((unsigned)k * ny + j) * 2 * p + sx, g.cpx, nx, i, 1, a);
-        if (ys) b = cpml_term(g.psi[kHzy] + ((unsigned)(k * 2 * p + sy) * nx + i), g.cpy, ny, j, 1, b);
+      if (xin && yin) {
+        if (xs) az = cpml_term(g.psi[kHzx] + ((unsigned)k * ny + j) * 2 * p + sx, g.cpx, nx, i, 1, az);
+        if (ys) bz = cpml_term(g.psi[kHzy] + ((unsigned)(k * 2 * p + sy) * nx + i), g.cpy, ny, j, 1, bz);
       }
-      hz[idx] = hz[idx] - ch * (a - b);
     }
-    ex0 = ex1;
-    ey0 = ey1;
+    hx[idx] = (yin && zin) ? c_hx0 - ch * (ax - bx) : c_hx0;
+    hy[idx] = (xin && zin) ? c_hy0 - ch * (ay - by) : c_hy0;
+    hz[idx] = (xin && yin) ? c_hz0 - ch * (az - bz) : c_hz0;
+    ex0 = c_ex1;
+    ey0 = c_ey1;
+    c_ez0 = n_ez0; c_ex1 = n_ex1; c_ey1 = n_ey1; c_ezj = n_ezj; c_exj = n_exj;
+    c_ezi = n_ezi; c_eyi = n_eyi; c_hx0 = n_hx0; c_hy0 = n_hy0; c_hz0 = n_hz0;
   }
 }
 
@@ -111,7 +142,13 @@ __global__ __launch_bounds__(kThreads) void yee_h_kernel(GridArgs g, int kz) {
 // damping or nothing, then the fused point sources.
 // ---------------------------------------------------------------------------
 template <int MODE, bool LOSSY>
-__global__ __launch_bounds__(kThreads) void yee_e_kernel(GridArgs g, SrcArgs s, int kz) {
+__global__ __launch_bounds__(kThreads) void yee_e_kernel(GridArgs g, SrcArgs s, int kz,
+                                                        float* __restrict__ ex,
+                                                        float* __restrict__ ey,
+                                                        float* __restrict__ ez,
+                                                        const float* __restrict__ hx,
+                                                        const float* __restrict__ hy,
+                                                        const float* __restrict__ hz) {
   const int i = blockIdx.x * kBX + (threadIdx.x % kBX);
   const int j = blockIdx.y * kBY + (threadIdx.x / kBX);
   const int nx = g.nx, ny = g.ny, nz = g.nz;
@@ -124,12 +161,6 @@ __global__ __launch_bounds__(kThreads) void yee_e_kernel(GridArgs g, SrcArgs s, 
   const bool yin = per || j > 0;
   const unsigned di = (per && i == 0) ? (unsigned)(nx - 1) : -1;
   const unsigned dj = (per && j == 0) ? (unsigned)(ny - 1) * nx : -(unsigned)nx;
-  float* __restrict__ ex = g.f[0];
-  float* __restrict__ ey = g.f[1];
-  float* __restrict__ ez = g.f[2];
-  const float* __restrict__ hx = g.f[3];
-  const float* __restrict__ hy = g.f[4];
-  const float* __restrict__ hz = g.f[5];
   unsigned idx = (unsigned)k0 * P + (unsigned)j * nx + i;
   float hxm = 0.f, hym = 0.f;  // H at plane k-1
   if (k0 > 0) {
@@ -146,41 +177,69 @@ __global__ __launch_bounds__(kThreads) void yee_e_kernel(GridArgs g, SrcArgs s, 
   const int sy = ys ? slab_index(j, ny, p) : 0;
   float sxy = 0.f;
   if (MODE == kSponge) sxy = g.ssx[i] + g.ssy[j];
+  const unsigned dic = xin ? di : 0u;
+  const unsigned djc = yin ? dj : 0u;
+  float c_hx0, c_hy0, c_hz0, c_hzj, c_hxj, c_hzi, c_hyi, c_cb, c_ca, c_ex0, c_ey0, c_ez0;
+  {
+    c_hx0 = hx[idx];
+    c_hy0 = hy[idx];
+    c_hz0 = hz[idx];
+    c_hzj = hz[idx + djc];
+    c_hxj = hx[idx + djc];
+    c_hzi = hz[idx + dic];
+    c_hyi = hy[idx + dic];
+    c_cb = g.cb[idx];
+    c_ca = LOSSY ? g.ca[idx] : 1.f;
+    c_ex0 = ex[idx];
+    c_ey0 = ey[idx];
+    c_ez0 = ez[idx];
+  }
   for (int k = k0; k < k1; ++k, idx += P) {
+    float n_hx0 = 0.f, n_hy0 = 0.f, n_hz0 = 0.f, n_hzj = 0.f, n_hxj = 0.f, n_hzi = 0.f,
+          n_hyi = 0.f, n_cb = 0.f, n_ca = 1.f, n_ex0 = 0.f, n_ey0 = 0.f, n_ez0 = 0.f;
+    if (k + 1 < k1) {
+      const unsigned q = idx + P;
+      n_hx0 = hx[q];
+      n_hy0 = hy[q];
+      n_hz0 = hz[q];
+      n_hzj = hz[q + djc];
+      n_hxj = hx[q + djc];
+      n_hzi = hz[q + dic];
+      n_hyi = hy[q + dic];
+      n_cb = g.cb[q];
+      n_ca = LOSSY ? g.ca[q] : 1.f;
+      n_ex0 = ex[q];
+      n_ey0 = ey[q];
+      n_ez0 = ez[q];
+    }
     const bool zin = per || k > 0;
-    const float hx0 = hx[idx], hy0 = hy[idx], hz0 = hz[idx];
-    float hzj = 0.f, hxj = 0.f, hzi = 0.f, hyi = 0.f;
-    if (yin) { hzj = hz[idx + dj]; hxj = hx[idx + dj]; }
-    if (xin) { hzi = hz[idx + di]; hyi = hy[idx + di]; }
-    const float cb = g.cb[idx];
-    const float ca = LOSSY ? g.ca[idx] : 1.f;
-    float vx = ex[idx], vy = ey[idx], vz = ez[idx];
+    const float hx0 = c_hx0, hy0 = c_hy0, hz0 = c_hz0;
+    const float cb = c_cb, ca = c_ca, ex0 = c_ex0, ey0 = c_ey0, ez0 = c_ez0;
     const bool zs = MODE == kCpml && (k < p || k >= nz - p);
     const int sz = zs ? slab_index(k, nz, p) : 0;
-    if (yin && zin) {  // fdtd3d.py:334-341
-      float a = hz0 - hzj, b = hy0 - hym;
-      if (MODE == kCpml) {
-        if (ys) a = cpml_term(g.psi[kExy] + ((unsigned)(k * 2 * p + sy) * nx + i), g.cpy, ny, j, 0, a);
-        if (zs) b = cpml_term(g.psi[kExz] + ((unsigned)(sz * ny + j) * nx + i), g.cpz, nz, k, 0, b);
+    float ax = hz0 - c_hzj, bx = hy0 - hym;  // fdtd3d.py:334-341
+    float ay = hx0 - hxm, by = hz0 - c_hzi;  // fdtd3d.py:343-350
+    float az = hy0 - c_hyi, bz = hx0 - c_hxj;  // fdtd3d.py:352-359
+    if (MODE == kCpml) {
+      if (yin && zin) {
+        if (ys) ax = cpml_term(g.psi[kExy] + ((unsigned)(k * 2 * p + sy) * nx + i), g.cpy, ny, j, 0, ax);
+        if (zs) bx = cpml_term(g.psi[kExz] + ((unsigned)(sz * ny + j) * nx + i), g.cpz, nz, k, 0, bx);
       }
-      vx = LOSSY ? ca * vx + cb * (a - b) : vx + cb * (a - b);
-    }
-    if (xin && zin) {  // fdtd3d.py:343-350
-      float a = hx0 - hxm, b = hz0 - hzi;
-      if (MODE == kCpml) {
-        if (zs) a = cpml_term(g.psi[kEyz] + ((unsigned)(sz * ny + j) * nx + i), g.cpz, nz, k, 0, a);
-        if (xs) b = cpml_term(g.psi[kEyx] + ((unsigned)k * ny + j) * 2 * p + sx, g.cpx, nx, i, 0, b);
+      if (xin && zin) {
+        if (zs) ay = cpml_term(g.psi[kEyz] + ((unsigned)(sz * ny + j) * nx + i), g.cpz, nz, k, 0, ay);
+        if (xs) by = cpml_term(g.psi[kEyx] + ((unsigned)k * ny + j) * 2 * p + sx, g.cpx, nx, i, 0, by);
       }
-      vy = LOSSY ? ca * vy + cb * (a - b) : vy + cb * (a - b);
-    }
-    if (xin && yin) {  // fdtd3d.py:352-359
-      float a = hy0 - hyi, b = hx0 - hxj;
-      if (MODE == kCpml) {
-        if (xs) a = cpml_term(g.psi[kEzx] + ((unsigned)k * ny + j) * 2 * p + sx, g.cpx, nx, i, 0, a);
-        if (ys) b = cpml_term(g.psi[kEzy] + ((unsigned)(k * 2 * p + sy) * nx + i), g.cpy, ny, j, 0, b);
+      if (xin && yin) {
+        if (xs) az = cpml_term(g.psi[kEzx] + ((unsigned)k * ny + j) * 2 * p + sx, g.cpx, nx, i, 0, az);
+        if (ys) bz = cpml_term(g.psi[kEzy] + ((unsigned)(k * 2 * p + sy) * nx + i), g.cpy, ny, j, 0, bz);
       }
-      vz = LOSSY ? ca * vz + cb * (a - b) : vz + cb * (a - b);
     }
+    c_hx0 = n_hx0; c_hy0 = n_hy0; c_hz0 = n_hz0; c_hzj = n_hzj; c_hxj = n_hxj; c_hzi = n_hzi;
+    c_hyi = n_hyi; c_cb = n_cb; c_ca = n_ca; c_ex0 = n_ex0; c_ey0 = n_ey0; c_ez0 = n_ez0;
+    float vx = ex0, vy = ey0, vz = ez0;
+    if (yin && zin) vx = LOSSY ? ca * ex0 + cb * (ax - bx) : ex0 + cb * (ax - bx);
+    if (xin && zin) vy = LOSSY ? ca * ey0 + cb * (ay - by) : ey0 + cb * (ay - by);
+    if (xin && yin) vz = LOSSY ? ca * ez0 + cb * (az - bz) : ez0 + cb * (az - bz);
     if (MODE == kSponge) {  // fdtd3d.py:361-366, sigma_avg of :221
       const float sig = (sxy + g.ssz[k]) / 3.0f;
       if (sig != 0.f) {  // exp(-0) == 1 exactly: the multiply is a no-op
@@ -309,16 +368,18 @@ int validate(const YeeGridDesc* d) {
   return YEE_OK;
 }
 
+#define YEE_FIELDS(g) (g).f[0], (g).f[1], (g).f[2], (g).f[3], (g).f[4], (g).f[5]
+
 int launch_h(const YeePlan* p, const GridArgs& g, hipStream_t st) {
   switch (p->d.boundary) {
     case YEE_BOUNDARY_PERIODIC:
-      yee_h_kernel<kPeriodic><<<p->grid, kThreads, 0, st>>>(g, p->kz);
+      yee_h_kernel<kPeriodic><<<p->grid, kThreads, 0, st>>>(g, p->kz, YEE_FIELDS(g));
       break;
     case YEE_BOUNDARY_CPML:
-      yee_h_kernel<kCpml><<<p->grid, kThreads, 0, st>>>(g, p->kz);
+      yee_h_kernel<kCpml><<<p->grid, kThreads, 0, st>>>(g, p->kz, YEE_FIELDS(g));
       break;
     default:  // walls and sponge share the H update (sponge damps E only)
-      yee_h_kernel<kWalls><<<p->grid, kThreads, 0, st>>>(g, p->kz);
+      yee_h_kernel<kWalls><<<p->grid, kThreads, 0, st>>>(g, p->kz, YEE_FIELDS(g));
       break;
   }
   YEE_HIP_CHECK(hipGetLastError());
@@ -329,16 +390,16 @@ template <bool LOSSY>
 void launch_e_mode(const YeePlan* p, const GridArgs& g, const SrcArgs& s, hipStream_t st) {
   switch (p->d.boundary) {
     case YEE_BOUNDARY_PERIODIC:
-      yee_e_kernel<kPeriodic, LOSSY><<<p->grid, kThreads, 0, st>>>(g, s, p->kz);
+      yee_e_kernel<kPeriodic, LOSSY><<<p->grid, kThreads, 0, st>>>(g, s, p->kz, YEE_FIELDS(g));
       break;
     case YEE_BOUNDARY_CPML:
-      yee_e_kernel<kCpml, LOSSY><<<p->grid, kThreads, 0, st>>>(g, s, p->kz);
+      yee_e_kernel<kCpml, LOSSY><<<p->grid, kThreads, 0, st>>>(g, s, p->kz, YEE_FIELDS(g));
       break;
     case YEE_BOUNDARY_SPONGE:
-      yee_e_kernel<kSponge, LOSSY><<<p->grid, kThreads, 0, st>>>(g, s, p->kz);
+      yee_e_kernel<kSponge, LOSSY><<<p->grid, kThreads, 0, st>>>(g, s, p->kz, YEE_FIELDS(g));
       break;
     default:
-      yee_e_kernel<kWalls, LOSSY><<<p->grid, kThreads, 0, st>>>(g, s, p->kz);
+      yee_e_kernel<kWalls, LOSSY><<<p->grid, kThreads, 0, st>>>(g, s, p->kz, YEE_FIELDS(g));
       break;
   }
 }

@@ -98,41 +98,35 @@ __device__ __forceinline__ void h_cell(const FusedArgs& a, int i, int j, int k, 
   const unsigned oz_ = zs ? ((unsigned)(slab_index(k, nz, p) * ny + j) * nx + i) : 0;
   const unsigned ox_ = xs ? (((unsigned)k * ny + j) * 2 * p + slab_index(i, nx, p)) : 0;
   const float ch = a.ch;
+  // branch-free: out-of-range neighbours are read at the cell itself and the
+  // update is then not selected
+  const unsigned dic = xin ? di : 0u, djc = yin ? dj : 0u, dkc = zin ? dk : 0u;
   if (WANT & 1) {
-    float hx = a.fi[3][c];
-    if (yin && zin) {  // fdtd3d.py:305-312
-      float u = ez[c + dj] - ez0, v = ey[c + dk] - ey0;
-      if (MODE == kCpml) {
-        if (ys) u = cterm(a, kHxy, oy_, own, 1, ny, j, 1, u);
-        if (zs) v = cterm(a, kHxz, oz_, own, 2, nz, k, 1, v);
-      }
-      hx = hx - ch * (u - v);
+    const float hx = a.fi[3][c];
+    float u = ez[c + djc] - ez0, v = ey[c + dkc] - ey0;  // fdtd3d.py:305-312
+    if (MODE == kCpml && yin && zin) {
+      if (ys) u = cterm(a, kHxy, oy_, own, 1, ny, j, 1, u);
+      if (zs) v = cterm(a, kHxz, oz_, own, 2, nz, k, 1, v);
     }
-    ox = hx;
+    ox = (yin && zin) ? hx - ch * (u - v) : hx;
   }
   if (WANT & 2) {
-    float hy = a.fi[4][c];
-    if (xin && zin) {  // fdtd3d.py:314-321
-      float u = ex[c + dk] - ex0, v = ez[c + di] - ez0;
-      if (MODE == kCpml) {
-        if (zs) u = cterm(a, kHyz, oz_, own, 2, nz, k, 1, u);
-        if (xs) v = cterm(a, kHyx, ox_, own, 0, nx, i, 1, v);
-      }
-      hy = hy - ch * (u - v);
+    const float hy = a.fi[4][c];
+    float u = ex[c + dkc] - ex0, v = ez[c + dic] - ez0;  // fdtd3d.py:314-321
+    if (MODE == kCpml && xin && zin) {
+      if (zs) u = cterm(a, kHyz, oz_, own, 2, nz, k, 1, u);
+      if (xs) v = cterm(a, kHyx, ox_, own, 0, nx, i, 1, v);
     }
-    oy = hy;
+    oy = (xin && zin) ? hy - ch * (u - v) : hy;
   }
   if (WANT & 4) {
-    float hz = a.fi[5][c];
-    if (xin && yin) {  // fdtd3d.py:323-330
-      float u = ey[c + di] - ey0, v = ex[c + dj] - ex0;
-      if (MODE == kCpml) {
-        if (xs) u = cterm(a, kHzx, ox_, own, 0, nx, i, 1, u);
-        if (ys) v = cterm(a, kHzy, oy_, own, 1, ny, j, 1, v);
-      }
-      hz = hz - ch * (u - v);
+    const float hz = a.fi[5][c];
+    float u = ey[c + dic] - ey0, v = ex[c + djc] - ex0;  // fdtd3d.py:323-330
+    if (MODE == kCpml && xin && yin) {
+      if (xs) u = cterm(a, kHzx, ox_, own, 0, nx, i, 1, u);
+      if (ys) v = cterm(a, kHzy, oy_, own, 1, ny, j, 1, v);
     }
-    oz = hz;
+    oz = (xin && yin) ? hz - ch * (u - v) : hz;
   }
 }
 
@@ -263,7 +257,14 @@ __global__ __launch_bounds__(64 * (BY + 1)) void yee_fused_kernel(FusedArgs a, i
 // the extra loads hit L1/L2).  Straight-line code: every load of a plane is
 // in flight at once and no wave ever waits for another.
 template <int MODE, bool LOSSY>
-__global__ __launch_bounds__(256) void yee_fused_nolds_kernel(FusedArgs a, int kz) {
+__global__ __launch_bounds__(256) void yee_fused_nolds_kernel(
+    FusedArgs a, int kz, const float* __restrict__ fi0, const float* __restrict__ fi1,
+    const float* __restrict__ fi2, const float* __restrict__ fi3, const float* __restrict__ fi4,
+    const float* __restrict__ fi5, float* __restrict__ fo0, float* __restrict__ fo1,
+    float* __restrict__ fo2, float* __restrict__ fo3, float* __restrict__ fo4,
+    float* __restrict__ fo5) {
+  a.fi[0] = fi0; a.fi[1] = fi1; a.fi[2] = fi2; a.fi[3] = fi3; a.fi[4] = fi4; a.fi[5] = fi5;
+  a.fo[0] = fo0; a.fo[1] = fo1; a.fo[2] = fo2; a.fo[3] = fo3; a.fo[4] = fo4; a.fo[5] = fo5;
   const int nx = a.nx, ny = a.ny, nz = a.nz, p = a.p;
   constexpr bool per = MODE == kPeriodic;
   const int lane = threadIdx.x & 63;
@@ -375,7 +376,9 @@ void launch_fused_t(const FusedArgs& a, const YeePlan* p, hipStream_t st) {
     const int gy = (a.ny + 3) / 4;
     const int kz = fused_kz(p, gx, gy);
     const dim3 grid(gx, gy, (a.nz + kz - 1) / kz);
-    yee_fused_nolds_kernel<MODE, LOSSY><<<grid, 256, 0, st>>>(a, kz);
+    yee_fused_nolds_kernel<MODE, LOSSY><<<grid, 256, 0, st>>>(
+        a, kz, a.fi[0], a.fi[1], a.fi[2], a.fi[3], a.fi[4], a.fi[5], a.fo[0], a.fo[1], a.fo[2],
+        a.fo[3], a.fo[4], a.fo[5]);
   } else {
     const int gy = (a.ny + kFusedBY - 1) / kFusedBY;
     const int kz = fused_kz(p, gx, gy);

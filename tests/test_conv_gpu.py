@@ -163,11 +163,11 @@ def test_closure_net_backward_matches_autograd(gpu):
 def test_closure_training_reduces_loss(gpu):
     """Fit a fixed local linear map with the net's own kernels and Adam: the
     loss must fall well below that of predicting zero (the target
-    variance).  A float32 torch replica of the same net and optimiser
-    reaches about 0.47 of the target variance in these 80 steps."""
+    variance).  A float32 torch replica of the same net, initialisation and
+    optimiser reaches about 0.16 of the target variance in these 80 steps."""
     shape = (32, 32, 16)
     nx, ny, nz = shape
-    net = ClosureNet(gpu, seed=1, out_scale=1.0)
+    net = ClosureNet(gpu, seed=1)
     g = torch.Generator().manual_seed(2)
     k = torch.zeros(6, 7, 3, 3, 3, dtype=torch.float64)
     k[:, :, 1, 1, 1] = torch.randn(6, 7, generator=g, dtype=torch.float64) * 0.3
@@ -184,7 +184,7 @@ def test_closure_training_reduces_loss(gpu):
         tvar.append(float((tgt ** 2).mean()))
         dy = torch.zeros_like(y)
         dy[:, :6] = 2 * err / err.numel()
-        net.adam_step(net.backward(dy), lr=3e-3)
+        net.adam_step(net.backward(dy), lr=1e-3)
     print(f"[conv] training loss {losses[0]:.3f} -> {np.mean(losses[-5:]):.3f} "
           f"(target variance {np.mean(tvar):.3f})", flush=True)
-    assert np.mean(losses[-5:]) < 0.65 * np.mean(tvar)
+    assert np.mean(losses[-5:]) < 0.35 * np.mean(tvar)
