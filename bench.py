@@ -272,8 +272,12 @@ def main() -> None:
         torch.cuda.synchronize()
         total_ms = start.elapsed_time(stop)
         launch_ms = sum(ev[q].elapsed_time(ev[q + 1]) for q in range(K)) / K
-        dom_name = "yee_fused_kernel"
-        dom_bytes = (48 + 4 + (4 if lossy else 0)) * cells + psi_bytes
+        # one step = the fused launches (CPML: slab-free interior, then the
+        # slab shell) plus the point-source kernel; algorithmic bytes per
+        # cell-update are those of the two-pass design (76 B, 80 B with a
+        # per-cell Ca) plus 8 B per psi cell, whatever the kernel moves
+        dom_name = "yee_fused_v2_kernel (interior + shell launches of one step)"
+        dom_bytes = (80 if lossy else 76) * cells + psi_bytes
         dom_ms = launch_ms
         step_bytes = dom_bytes
         h_ms = e_ms = None

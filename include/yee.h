@@ -93,10 +93,17 @@ void yee_plan_destroy(YeePlan* plan);
  * the two-pass kernels (kz > 0) and by the fused kernel (fused_kz > 0;
  * 0 = automatic, < 0 = unchanged).  Results do not depend on them.       */
 int yee_plan_tune(YeePlan* plan, int32_t kz, int32_t fused_kz);
-/* Fused-kernel design: 0 = LDS exchange with a halo row (default), 1 =
- * barrier-free (row y-1 recomputed, column x-1 by a wave shift).  Same
- * results bit for bit.                                                   */
+/* Fused-kernel design: 0 = LDS exchange with a halo row, 1 = barrier-free
+ * (H of column x-1 and row y-1 recomputed), 2 (default) = barrier-free
+ * with straight-line planes, carried E values and, under CPML, separate
+ * launches for the slab-free interior and the slab shell.  Same results
+ * bit for bit.                                                           */
 int yee_plan_set_variant(YeePlan* plan, int32_t variant);
+/* Workgroup -> tile order (no reference counterpart): nonzero (default) =
+ * XCD-aware (each of the 8 XCDs runs a contiguous range of tiles, so
+ * neighbouring tiles share an L2), 0 = plain x-fastest order.  Applies to
+ * the two-pass kernels and fused variant 2; results do not depend on it. */
+int yee_plan_set_tile_order(YeePlan* plan, int32_t xcd_aware);
 
 #define YEE_FUSED_SOURCES 8
 

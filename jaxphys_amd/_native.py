@@ -49,6 +49,7 @@ EXPORTS = {
     "yee_plan_destroy": (None, [ctypes.c_void_p]),
     "yee_plan_tune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     "yee_plan_set_variant": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "yee_plan_set_tile_order": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "yee_set_sources": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32]),
     "yee_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
@@ -166,6 +167,10 @@ class Plan:
 
     def set_variant(self, variant: int) -> None:
         check(lib().yee_plan_set_variant(self.handle, int(variant)), "yee_plan_set_variant")
+
+    def set_tile_order(self, xcd_aware: bool) -> None:
+        check(lib().yee_plan_set_tile_order(self.handle, int(bool(xcd_aware))),
+              "yee_plan_set_tile_order")
 
     def psi_floats(self) -> int:
         return int(lib().yee_psi_floats(ctypes.byref(self.desc)))

@@ -40,18 +40,20 @@ __device__ __forceinline__ float cpml_term(float* __restrict__ ps, const float* 
 // H half-step: H -= ch * curl(E) (forward differences).
 // ---------------------------------------------------------------------------
 template <int MODE>
-__global__ __launch_bounds__(kThreads) void yee_h_kernel(GridArgs g, int kz,
+__global__ __launch_bounds__(kThreads) void yee_h_kernel(GridArgs g, Tiling tl, int kz,
                                                         const float* __restrict__ ex,
                                                         const float* __restrict__ ey,
                                                         const float* __restrict__ ez,
                                                         float* __restrict__ hx,
                                                         float* __restrict__ hy,
                                                         float* __restrict__ hz) {
-  const int i = blockIdx.x * kBX + (threadIdx.x % kBX);
-  const int j = blockIdx.y * kBY + (threadIdx.x / kBX);
+  int bx, by, bz;
+  if (!tile_of(tl, bx, by, bz)) return;
+  const int i = bx * kBX + (threadIdx.x % kBX);
+  const int j = by * kBY + (threadIdx.x / kBX);
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   if (i >= nx || j >= ny) return;
-  const int k0 = blockIdx.z * kz;
+  const int k0 = bz * kz;
   const int k1 = min(k0 + kz, nz);
   const unsigned P = (unsigned)nx * ny;
   constexpr bool per = MODE == kPeriodic;
@@ -142,18 +144,20 @@ __global__ __launch_bounds__(kThreads) void yee_h_kernel(GridArgs g, int kz,
 // damping or nothing, then the fused point sources.
 // ---------------------------------------------------------------------------
 template <int MODE, bool LOSSY>
-__global__ __launch_bounds__(kThreads) void yee_e_kernel(GridArgs g, SrcArgs s, int kz,
+__global__ __launch_bounds__(kThreads) void yee_e_kernel(GridArgs g, SrcArgs s, Tiling tl, int kz,
                                                         float* __restrict__ ex,
                                                         float* __restrict__ ey,
                                                         float* __restrict__ ez,
                                                         const float* __restrict__ hx,
                                                         const float* __restrict__ hy,
                                                         const float* __restrict__ hz) {
-  const int i = blockIdx.x * kBX + (threadIdx.x % kBX);
-  const int j = blockIdx.y * kBY + (threadIdx.x / kBX);
+  int bx, by, bz;
+  if (!tile_of(tl, bx, by, bz)) return;
+  const int i = bx * kBX + (threadIdx.x % kBX);
+  const int j = by * kBY + (threadIdx.x / kBX);
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   if (i >= nx || j >= ny) return;
-  const int k0 = blockIdx.z * kz;
+  const int k0 = bz * kz;
   const int k1 = min(k0 + kz, nz);
   const unsigned P = (unsigned)nx * ny;
   constexpr bool per = MODE == kPeriodic;
@@ -370,16 +374,22 @@ int validate(const YeeGridDesc* d) {
 
 #define YEE_FIELDS(g) (g).f[0], (g).f[1], (g).f[2], (g).f[3], (g).f[4], (g).f[5]
 
+static Tiling plan_tiling(const YeePlan* p) {
+  return make_tiling(p->d.nx, p->d.ny, p->d.nz, p->kz, p->xcd);
+}
+
 int launch_h(const YeePlan* p, const GridArgs& g, hipStream_t st) {
+  const Tiling tl = plan_tiling(p);
+  const unsigned nb = tiling_blocks(tl);
   switch (p->d.boundary) {
     case YEE_BOUNDARY_PERIODIC:
-      yee_h_kernel<kPeriodic><<<p->grid, kThreads, 0, st>>>(g, p->kz, YEE_FIELDS(g));
+      yee_h_kernel<kPeriodic><<<nb, kThreads, 0, st>>>(g, tl, p->kz, YEE_FIELDS(g));
       break;
     case YEE_BOUNDARY_CPML:
-      yee_h_kernel<kCpml><<<p->grid, kThreads, 0, st>>>(g, p->kz, YEE_FIELDS(g));
+      yee_h_kernel<kCpml><<<nb, kThreads, 0, st>>>(g, tl, p->kz, YEE_FIELDS(g));
       break;
     default:  // walls and sponge share the H update (sponge damps E only)
-      yee_h_kernel<kWalls><<<p->grid, kThreads, 0, st>>>(g, p->kz, YEE_FIELDS(g));
+      yee_h_kernel<kWalls><<<nb, kThreads, 0, st>>>(g, tl, p->kz, YEE_FIELDS(g));
       break;
   }
   YEE_HIP_CHECK(hipGetLastError());
@@ -388,18 +398,20 @@ int launch_h(const YeePlan* p, const GridArgs& g, hipStream_t st) {
 
 template <bool LOSSY>
 void launch_e_mode(const YeePlan* p, const GridArgs& g, const SrcArgs& s, hipStream_t st) {
+  const Tiling tl = plan_tiling(p);
+  const unsigned nb = tiling_blocks(tl);
   switch (p->d.boundary) {
     case YEE_BOUNDARY_PERIODIC:
-      yee_e_kernel<kPeriodic, LOSSY><<<p->grid, kThreads, 0, st>>>(g, s, p->kz, YEE_FIELDS(g));
+      yee_e_kernel<kPeriodic, LOSSY><<<nb, kThreads, 0, st>>>(g, s, tl, p->kz, YEE_FIELDS(g));
       break;
     case YEE_BOUNDARY_CPML:
-      yee_e_kernel<kCpml, LOSSY><<<p->grid, kThreads, 0, st>>>(g, s, p->kz, YEE_FIELDS(g));
+      yee_e_kernel<kCpml, LOSSY><<<nb, kThreads, 0, st>>>(g, s, tl, p->kz, YEE_FIELDS(g));
       break;
     case YEE_BOUNDARY_SPONGE:
-      yee_e_kernel<kSponge, LOSSY><<<p->grid, kThreads, 0, st>>>(g, s, p->kz, YEE_FIELDS(g));
+      yee_e_kernel<kSponge, LOSSY><<<nb, kThreads, 0, st>>>(g, s, tl, p->kz, YEE_FIELDS(g));
       break;
     default:
-      yee_e_kernel<kWalls, LOSSY><<<p->grid, kThreads, 0, st>>>(g, s, p->kz, YEE_FIELDS(g));
+      yee_e_kernel<kWalls, LOSSY><<<nb, kThreads, 0, st>>>(g, s, tl, p->kz, YEE_FIELDS(g));
       break;
   }
 }
@@ -451,42 +463,41 @@ int yee_plan_create(const YeeGridDesc* desc, YeePlan** out) {
   YeePlan* p = new (std::nothrow) YeePlan();
   if (!p) return fail(YEE_EINVAL, "out of host memory");
   p->d = *desc;
-  const int gx = (desc->nx + kBX - 1) / kBX;
-  const int gy = (desc->ny + kBY - 1) / kBY;
-  // z chunking: enough blocks to fill 256 CUs several times over, while
-  // keeping each column march long enough to amortise the carried plane.
+  // z chunking of the two-pass kernels: short column marches (many tiles)
+  // measured fastest on the 512^3 grid; the carried plane is re-read once
+  // per chunk from L2.
   int kz = 0;
   if (const char* e = std::getenv("JAXPHYS_AMD_KZ")) kz = std::atoi(e);
-  if (kz <= 0) {
-    const long long want = 8192;
-    long long chunks = (want + (long long)gx * gy - 1) / ((long long)gx * gy);
-    if (chunks < 1) chunks = 1;
-    if (chunks > desc->nz) chunks = desc->nz;
-    kz = (int)((desc->nz + chunks - 1) / chunks);
-  }
+  if (kz <= 0) kz = 4;
   if (kz > desc->nz) kz = desc->nz;
   p->kz = kz;
-  p->grid = dim3(gx, gy, (desc->nz + kz - 1) / kz);
   if (const char* e = std::getenv("JAXPHYS_AMD_FUSED_KZ")) p->fused_kz = std::atoi(e);
-  if (const char* e = std::getenv("JAXPHYS_AMD_FUSED_VARIANT")) p->fused_variant = std::atoi(e) == 1;
+  if (const char* e = std::getenv("JAXPHYS_AMD_FUSED_VARIANT")) {
+    const int v = std::atoi(e);
+    if (v >= 0 && v <= 2) p->fused_variant = v;
+  }
+  if (const char* e = std::getenv("JAXPHYS_AMD_XCD")) p->xcd = std::atoi(e) != 0;
   *out = p;
   return YEE_OK;
 }
 
 int yee_plan_tune(YeePlan* p, int32_t kz, int32_t fused_kz) {
   if (!p) return fail(YEE_EINVAL, "null plan");
-  if (kz > 0) {
-    p->kz = kz > p->d.nz ? p->d.nz : kz;
-    p->grid.z = (p->d.nz + p->kz - 1) / p->kz;
-  }
+  if (kz > 0) p->kz = kz > p->d.nz ? p->d.nz : kz;
   if (fused_kz >= 0) p->fused_kz = fused_kz;
   return YEE_OK;
 }
 
 int yee_plan_set_variant(YeePlan* p, int32_t variant) {
   if (!p) return fail(YEE_EINVAL, "null plan");
-  if (variant < 0 || variant > 1) return fail(YEE_EINVAL, "unknown fused variant");
+  if (variant < 0 || variant > 2) return fail(YEE_EINVAL, "unknown fused variant");
   p->fused_variant = variant;
+  return YEE_OK;
+}
+
+int yee_plan_set_tile_order(YeePlan* p, int32_t xcd_aware) {
+  if (!p) return fail(YEE_EINVAL, "null plan");
+  p->xcd = xcd_aware != 0;
   return YEE_OK;
 }
 

@@ -355,24 +355,358 @@ __global__ __launch_bounds__(256) void yee_fused_nolds_kernel(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Variant 2 (default): the barrier-free recompute design of variant 1,
+// reorganised so that every plane is straight-line code:
+//  * CPML work is chosen per plane by a block-uniform mask -- bit 0: the
+//    tile (with its x-1 column) touches an x slab, bit 1: a y slab, bit 2:
+//    the plane lies in a z slab.  Each mask value is its own instantiation,
+//    so interior planes carry no psi code at all, and slab planes no
+//    per-cell branches: psi of a cell outside its slab is read at a clamped
+//    offset, not selected and not written.
+//  * Ex, Ey of the column, Ex of column x-1 and Ey of row y-1 at plane k+1
+//    are loaded once and carried to the next plane in registers.
+//  * 1-D launch in the XCD-aware tile order of Tiling.
+// ---------------------------------------------------------------------------
+
+// psi recursion + curl term of one cell, branch-free: when the cell is not
+// in the slab (`in` false) psi is read at offset `base` (a valid slot
+// element), the result is `d`, and nothing is written.
+__device__ __forceinline__ float v2_psi(const float* __restrict__ pin, float* __restrict__ pout,
+                                        unsigned base, unsigned off, bool in, bool wr,
+                                        const float* __restrict__ tab, int n, int v, int kind,
+                                        float d) {
+  const float b = tab[(3 * kind + 0) * n + v];
+  const float c = tab[(3 * kind + 1) * n + v];
+  const float kinv = tab[(3 * kind + 2) * n + v];
+  const unsigned o = base + (in ? off : 0u);
+  const float ps = b * pin[o] + c * d;
+  if (wr && in) pout[o] = ps;
+  return in ? d * kinv + ps : d;
+}
+
+// per-column constants of variant 2
+struct V2Col {
+  int i, j, im, jm;            // column and its x-1 / y-1 neighbours (clamped)
+  unsigned xp, xm, yp, ym;     // neighbour offsets, 0 on a non-periodic face
+  bool xinH, yinH;             // H updates with an x+1 / y+1 neighbour
+  bool xinE, yinE;             // E updates with an x-1 / y-1 neighbour
+  // CPML: slab membership and slab index of i, i-1 (x) and j, j-1 (y)
+  bool xs, xsm, ys, ysm;
+  int sx, sxm, sy, sym;
+  float sxy;                   // sponge: sigma_x[i] + sigma_y[j]
+};
+
+// values carried from plane k-1 to plane k
+struct V2Carry {
+  float ex, ey, exm, eym;  // Ex, Ey of the column, Ex(i-1), Ey(j-1) at plane k
+  float hxm, hym;          // H^{n+1/2} x, y at plane k-1
+};
+
+template <int MODE, bool LOSSY, int SL>
+__device__ __forceinline__ void v2_plane(
+    const FusedArgs& a, const float* __restrict__ Ex, const float* __restrict__ Ey,
+    const float* __restrict__ Ez, const float* __restrict__ Hx, const float* __restrict__ Hy,
+    const float* __restrict__ Hz, float* __restrict__ oEx, float* __restrict__ oEy,
+    float* __restrict__ oEz, float* __restrict__ oHx, float* __restrict__ oHy,
+    float* __restrict__ oHz, const V2Col& q, int k, unsigned c, V2Carry& r) {
+  constexpr bool per = MODE == kPeriodic;
+  constexpr bool XS = MODE == kCpml && (SL & 1);
+  constexpr bool YS = MODE == kCpml && (SL & 2);
+  constexpr bool ZS = MODE == kCpml && (SL & 4);
+  const int nx = a.nx, ny = a.ny, nz = a.nz, p = a.p;
+  const unsigned P = (unsigned)nx * ny;
+  const bool zinH = per || k < nz - 1;
+  const bool zinE = per || k > 0;
+  const unsigned zp = (per && k == nz - 1) ? -(unsigned)(nz - 1) * P : (zinH ? P : 0u);
+  const float ch = a.ch;
+
+  // ---- loads: the column, its x+1 / y+1 / z+1 neighbours, and the inputs
+  // of H at (i-1, j, k) and (i, j-1, k)
+  const float ex = r.ex, ey = r.ey, exm = r.exm, eym = r.eym;
+  const float ez = Ez[c];
+  const float ey_xp = Ey[c + q.xp], ez_xp = Ez[c + q.xp];
+  const float ex_yp = Ex[c + q.yp], ez_yp = Ez[c + q.yp];
+  const float ex_zp = Ex[c + zp], ey_zp = Ey[c + zp];
+  const float hx0 = Hx[c], hy0 = Hy[c], hz0 = Hz[c];
+  const unsigned cxm = c + q.xm, cym = c + q.ym;
+  const float ez_xm = Ez[cxm], ey_xm = Ey[cxm], ex_xm_yp = Ex[cxm + q.yp],
+              ex_xm_zp = Ex[cxm + zp];
+  const float hy_xm0 = Hy[cxm], hz_xm0 = Hz[cxm];
+  const float ez_ym = Ez[cym], ex_ym = Ex[cym], ey_ym_xp = Ey[cym + q.xp],
+              ey_ym_zp = Ey[cym + zp];
+  const float hx_ym0 = Hx[cym], hz_ym0 = Hz[cym];
+  const float cb = a.cb[c];
+  const float ca = LOSSY ? a.ca[c] : 1.f;
+
+  // CPML addressing (only used by the XS / YS / ZS instantiations)
+  const float* tx = a.cpml;
+  const float* ty = a.cpml + 6 * nx;
+  const float* tz = a.cpml + 6 * (nx + ny);
+  const unsigned ysl = (unsigned)a.ysl, zsl = (unsigned)a.zsl, xsl = (unsigned)a.xsl;
+  const unsigned by0 = 0, bz0 = 4 * ysl, bx0 = 4 * ysl + 4 * zsl;
+  const bool zs = ZS && (k < p || k >= nz - p);
+  const int sz = zs ? slab_index(k, nz, p) : 0;
+  const unsigned tp = 2u * (unsigned)p;
+  // y group (k, s, i) ; z group (s, j, i) ; x group (k, j, s)
+  auto offy = [&](int s, int ii) { return ((unsigned)k * tp + (unsigned)s) * nx + ii; };
+  auto offz = [&](int jj, int ii) { return ((unsigned)sz * ny + jj) * nx + ii; };
+  auto offx = [&](int jj, int s) { return ((unsigned)k * ny + jj) * tp + (unsigned)s; };
+  const float* __restrict__ pin = a.pin;
+  float* __restrict__ pout = a.pout;
+
+  // ---- H^{n+1/2} of the column (fdtd3d.py:303-330)
+  float hx, hy, hz;
+  {
+    const bool upd = q.yinH && zinH;
+    float u = ez_yp - ez, v = ey_zp - ey;
+    if (YS) u = v2_psi(pin, pout, by0 + kHxy * ysl, offy(q.sy, q.i), q.ys, upd, ty, ny, q.j, 1, u);
+    if (ZS) v = v2_psi(pin, pout, bz0 + (kHxz - 4) * zsl, offz(q.j, q.i), zs, upd, tz, nz, k, 1, v);
+    hx = upd ? hx0 - ch * (u - v) : hx0;
+  }
+  {
+    const bool upd = q.xinH && zinH;
+    float u = ex_zp - ex, v = ez_xp - ez;
+    if (ZS) u = v2_psi(pin, pout, bz0 + (kHyz - 4) * zsl, offz(q.j, q.i), zs, upd, tz, nz, k, 1, u);
+    if (XS) v = v2_psi(pin, pout, bx0 + (kHyx - 8) * xsl, offx(q.j, q.sx), q.xs, upd, tx, nx, q.i, 1, v);
+    hy = upd ? hy0 - ch * (u - v) : hy0;
+  }
+  {
+    const bool upd = q.xinH && q.yinH;
+    float u = ey_xp - ey, v = ex_yp - ex;
+    if (XS) u = v2_psi(pin, pout, bx0 + (kHzx - 8) * xsl, offx(q.j, q.sx), q.xs, upd, tx, nx, q.i, 1, u);
+    if (YS) v = v2_psi(pin, pout, by0 + kHzy * ysl, offy(q.sy, q.i), q.ys, upd, ty, ny, q.j, 1, v);
+    hz = upd ? hz0 - ch * (u - v) : hz0;
+  }
+  // ---- Hy, Hz at (i-1, j, k) (recomputed, psi not written)
+  float hy_xm, hz_xm;
+  {
+    float u = ex_xm_zp - exm, v = ez - ez_xm;
+    if (ZS) u = v2_psi(pin, pout, bz0 + (kHyz - 4) * zsl, offz(q.j, q.im), zs, false, tz, nz, k, 1, u);
+    if (XS) v = v2_psi(pin, pout, bx0 + (kHyx - 8) * xsl, offx(q.j, q.sxm), q.xsm, false, tx, nx, q.im, 1, v);
+    hy_xm = zinH ? hy_xm0 - ch * (u - v) : hy_xm0;
+  }
+  {
+    float u = ey - ey_xm, v = ex_xm_yp - exm;
+    if (XS) u = v2_psi(pin, pout, bx0 + (kHzx - 8) * xsl, offx(q.j, q.sxm), q.xsm, false, tx, nx, q.im, 1, u);
+    if (YS) v = v2_psi(pin, pout, by0 + kHzy * ysl, offy(q.sy, q.im), q.ys, false, ty, ny, q.j, 1, v);
+    hz_xm = q.yinH ? hz_xm0 - ch * (u - v) : hz_xm0;
+  }
+  // ---- Hx, Hz at (i, j-1, k)
+  float hx_ym, hz_ym;
+  {
+    float u = ez - ez_ym, v = ey_ym_zp - eym;
+    if (YS) u = v2_psi(pin, pout, by0 + kHxy * ysl, offy(q.sym, q.i), q.ysm, false, ty, ny, q.jm, 1, u);
+    if (ZS) v = v2_psi(pin, pout, bz0 + (kHxz - 4) * zsl, offz(q.jm, q.i), zs, false, tz, nz, k, 1, v);
+    hx_ym = zinH ? hx_ym0 - ch * (u - v) : hx_ym0;
+  }
+  {
+    float u = ey_ym_xp - eym, v = ex - ex_ym;
+    if (XS) u = v2_psi(pin, pout, bx0 + (kHzx - 8) * xsl, offx(q.jm, q.sx), q.xs, false, tx, nx, q.i, 1, u);
+    if (YS) v = v2_psi(pin, pout, by0 + kHzy * ysl, offy(q.sym, q.i), q.ysm, false, ty, ny, q.jm, 1, v);
+    hz_ym = q.xinH ? hz_ym0 - ch * (u - v) : hz_ym0;
+  }
+  oHx[c] = hx;
+  oHy[c] = hy;
+  oHz[c] = hz;
+
+  // ---- E^{n+1} of the column (fdtd3d.py:332-359)
+  float vx = ex, vy = ey, vz = ez;
+  {
+    const bool upd = q.yinE && zinE;
+    float u = hz - hz_ym, v = hy - r.hym;
+    if (YS) u = v2_psi(pin, pout, by0 + kExy * ysl, offy(q.sy, q.i), q.ys, upd, ty, ny, q.j, 0, u);
+    if (ZS) v = v2_psi(pin, pout, bz0 + (kExz - 4) * zsl, offz(q.j, q.i), zs, upd, tz, nz, k, 0, v);
+    const float w = LOSSY ? ca * ex + cb * (u - v) : ex + cb * (u - v);
+    vx = upd ? w : ex;
+  }
+  {
+    const bool upd = q.xinE && zinE;
+    float u = hx - r.hxm, v = hz - hz_xm;
+    if (ZS) u = v2_psi(pin, pout, bz0 + (kEyz - 4) * zsl, offz(q.j, q.i), zs, upd, tz, nz, k, 0, u);
+    if (XS) v = v2_psi(pin, pout, bx0 + (kEyx - 8) * xsl, offx(q.j, q.sx), q.xs, upd, tx, nx, q.i, 0, v);
+    const float w = LOSSY ? ca * ey + cb * (u - v) : ey + cb * (u - v);
+    vy = upd ? w : ey;
+  }
+  {
+    const bool upd = q.xinE && q.yinE;
+    float u = hy - hy_xm, v = hx - hx_ym;
+    if (XS) u = v2_psi(pin, pout, bx0 + (kEzx - 8) * xsl, offx(q.j, q.sx), q.xs, upd, tx, nx, q.i, 0, u);
+    if (YS) v = v2_psi(pin, pout, by0 + kEzy * ysl, offy(q.sy, q.i), q.ys, upd, ty, ny, q.j, 0, v);
+    const float w = LOSSY ? ca * ez + cb * (u - v) : ez + cb * (u - v);
+    vz = upd ? w : ez;
+  }
+  if (MODE == kSponge) {  // fdtd3d.py:361-366; a factor exp(-0) == 1 changes no bit
+    const float sig = (q.sxy + a.sponge[nx + ny + k]) / 3.0f;
+    const float d = sig != 0.f ? expf(((-sig) * a.sdt) / a.seps0) : 1.f;
+    vx *= d;
+    vy *= d;
+    vz *= d;
+  }
+  oEx[c] = vx;
+  oEy[c] = vy;
+  oEz[c] = vz;
+
+  r.ex = ex_zp;
+  r.ey = ey_zp;
+  r.exm = ex_xm_zp;
+  r.eym = ey_ym_zp;
+  r.hxm = hx;
+  r.hym = hy;
+}
+
+// SHELL == false with CPML: the tiles of `ts` lie outside every slab (no
+// psi code is compiled in); SHELL == true: any tiles, slab work chosen per
+// plane.
+template <int MODE, bool LOSSY, bool SHELL>
+__global__ __launch_bounds__(256) void yee_fused_v2_kernel(
+    FusedArgs a, TileSet ts, const float* __restrict__ fi0,
+    const float* __restrict__ fi1, const float* __restrict__ fi2, const float* __restrict__ fi3,
+    const float* __restrict__ fi4, const float* __restrict__ fi5, float* __restrict__ fo0,
+    float* __restrict__ fo1, float* __restrict__ fo2, float* __restrict__ fo3,
+    float* __restrict__ fo4, float* __restrict__ fo5) {
+  int bx, by, k0, k1;
+  if (!tile_of_set(ts, bx, by, k0, k1)) return;
+  const int nx = a.nx, ny = a.ny, nz = a.nz, p = a.p;
+  constexpr bool per = MODE == kPeriodic;
+  const int i = bx * 64 + (threadIdx.x & 63);
+  // one y row per wavefront: j and everything derived from it is scalar
+  const int j = __builtin_amdgcn_readfirstlane(by * 4 + (int)(threadIdx.x >> 6));
+  if (i >= nx || j >= ny) return;
+  V2Col q;
+  q.i = i;
+  q.j = j;
+  q.xinH = per || i < nx - 1;
+  q.yinH = per || j < ny - 1;
+  q.xinE = per || i > 0;
+  q.yinE = per || j > 0;
+  q.im = i > 0 ? i - 1 : (per ? nx - 1 : i);
+  q.jm = j > 0 ? j - 1 : (per ? ny - 1 : j);
+  q.xp = i < nx - 1 ? 1u : (per ? -(unsigned)(nx - 1) : 0u);
+  q.yp = j < ny - 1 ? (unsigned)nx : (per ? -(unsigned)(ny - 1) * nx : 0u);
+  q.xm = (unsigned)q.im - (unsigned)i;
+  q.ym = ((unsigned)q.jm - (unsigned)j) * (unsigned)nx;
+  q.xs = MODE == kCpml && (i < p || i >= nx - p);
+  q.xsm = MODE == kCpml && (q.im < p || q.im >= nx - p);
+  q.ys = MODE == kCpml && (j < p || j >= ny - p);
+  q.ysm = MODE == kCpml && (q.jm < p || q.jm >= ny - p);
+  q.sx = q.xs ? slab_index(i, nx, p) : 0;
+  q.sxm = q.xsm ? slab_index(q.im, nx, p) : 0;
+  q.sy = q.ys ? slab_index(j, ny, p) : 0;
+  q.sym = q.ysm ? slab_index(q.jm, ny, p) : 0;
+  q.sxy = MODE == kSponge ? a.sponge[i] + a.sponge[nx + j] : 0.f;
+  // block-uniform slab mask of the tile (its columns and their x-1 / y-1
+  // neighbours)
+  int slxy = 0;
+  if (MODE == kCpml) {
+    const int i0 = bx * 64, il = min(i0 + 63, nx - 1);
+    const int j0 = by * 4, jl = min(j0 + 3, ny - 1);
+    if (i0 <= p || il >= nx - p) slxy |= 1;
+    if (j0 <= p || jl >= ny - p) slxy |= 2;
+  }
+  const unsigned P = (unsigned)nx * ny;
+  unsigned c = (unsigned)k0 * P + (unsigned)j * nx + i;
+  V2Carry r;
+  r.ex = fi0[c];
+  r.ey = fi1[c];
+  r.exm = fi0[c + q.xm];
+  r.eym = fi1[c + q.ym];
+  r.hxm = 0.f;
+  r.hym = 0.f;
+  {
+    int kp = k0 - 1;
+    if (per && kp < 0) kp += nz;
+    if (kp >= 0) {
+      float unused;
+      h_cell<MODE, 3>(a, i, j, kp, false, r.hxm, r.hym, unused, nullptr);
+    }
+  }
+#define YEE_V2_PLANE(S)                                                                   \
+  v2_plane<MODE, LOSSY, S>(a, fi0, fi1, fi2, fi3, fi4, fi5, fo0, fo1, fo2, fo3, fo4, fo5, q, \
+                           k, c, r)
+  for (int k = k0; k < k1; ++k, c += P) {
+    if (MODE != kCpml || !SHELL) {
+      YEE_V2_PLANE(0);
+    } else {
+      const int sl = slxy | ((k < p || k >= nz - p) ? 4 : 0);
+      switch (sl) {
+        case 0: YEE_V2_PLANE(0); break;
+        case 1: YEE_V2_PLANE(1); break;
+        case 2: YEE_V2_PLANE(2); break;
+        case 3: YEE_V2_PLANE(3); break;
+        case 4: YEE_V2_PLANE(4); break;
+        case 5: YEE_V2_PLANE(5); break;
+        case 6: YEE_V2_PLANE(6); break;
+        default: YEE_V2_PLANE(7); break;
+      }
+    }
+  }
+#undef YEE_V2_PLANE
+}
+
 constexpr int kFusedBY = 8;
 
-int fused_kz(const YeePlan* p, int gx, int gy) {
+int fused_kz(const YeePlan* p, int /*gx*/, int /*gy*/) {
   int kz = p->fused_kz;
-  if (kz <= 0) {
-    const long long want = 2048;
-    long long chunks = (want + (long long)gx * gy - 1) / ((long long)gx * gy);
-    if (chunks < 1) chunks = 1;
-    if (chunks > p->d.nz) chunks = p->d.nz;
-    kz = (int)((p->d.nz + chunks - 1) / chunks);
-  }
+  if (kz <= 0) kz = 16;  // z planes per tile; the fastest on the 512^3 grid
   return kz > p->d.nz ? p->d.nz : kz;
+}
+
+// Variant 2 launches.  CPML: the tiles that touch no slab (in x and y with
+// their x-1 / y-1 neighbours, planes p .. nz-p-1) form one box run by the
+// lean SHELL == false kernel; the rest, as up to six boxes, by the SHELL
+// kernel.  Both read buffer A and write disjoint cells of buffer B.
+template <int MODE, bool LOSSY>
+void launch_v2(const FusedArgs& a, const YeePlan* pl, hipStream_t st) {
+  const int nx = a.nx, ny = a.ny, nz = a.nz, p = a.p;
+  const int gx = (nx + 63) / 64, gy = (ny + 3) / 4;
+  TileSet in, sh;
+  std::memset(&in, 0, sizeof(in));
+  std::memset(&sh, 0, sizeof(sh));
+  in.kz = sh.kz = fused_kz(pl, gx, gy);
+  bool split = false;
+  int bxl = gx, bxh = 0, byl = gy, byh = 0;
+  if (MODE == kCpml) {
+    for (int b = 0; b < gx; ++b) {
+      const int i0 = 64 * b, il = std::min(i0 + 63, nx - 1);
+      if (i0 > p && il < nx - p) { bxl = std::min(bxl, b); bxh = std::max(bxh, b + 1); }
+    }
+    for (int b = 0; b < gy; ++b) {
+      const int j0 = 4 * b, jl = std::min(j0 + 3, ny - 1);
+      if (j0 > p && jl < ny - p) { byl = std::min(byl, b); byh = std::max(byh, b + 1); }
+    }
+    split = bxl < bxh && byl < byh && p < nz - p;
+  }
+  if (split) {
+    tileset_add(in, bxl, bxh, byl, byh, p, nz - p);
+    tileset_add(sh, 0, gx, 0, gy, 0, p);
+    tileset_add(sh, 0, gx, 0, gy, nz - p, nz);
+    tileset_add(sh, 0, gx, 0, byl, p, nz - p);
+    tileset_add(sh, 0, gx, byh, gy, p, nz - p);
+    tileset_add(sh, 0, bxl, byl, byh, p, nz - p);
+    tileset_add(sh, bxh, gx, byl, byh, p, nz - p);
+  } else {
+    tileset_add(MODE == kCpml ? sh : in, 0, gx, 0, gy, 0, nz);
+  }
+  tileset_finish(in, pl->xcd);
+  tileset_finish(sh, pl->xcd);
+  if (in.ntiles > 0)
+    yee_fused_v2_kernel<MODE, LOSSY, false><<<tileset_blocks(in), 256, 0, st>>>(
+        a, in, a.fi[0], a.fi[1], a.fi[2], a.fi[3], a.fi[4], a.fi[5], a.fo[0], a.fo[1], a.fo[2],
+        a.fo[3], a.fo[4], a.fo[5]);
+  if constexpr (MODE == kCpml)
+    if (sh.ntiles > 0)
+      yee_fused_v2_kernel<MODE, LOSSY, true><<<tileset_blocks(sh), 256, 0, st>>>(
+        a, sh, a.fi[0], a.fi[1], a.fi[2], a.fi[3], a.fi[4], a.fi[5], a.fo[0], a.fo[1], a.fo[2],
+        a.fo[3], a.fo[4], a.fo[5]);
 }
 
 template <int MODE, bool LOSSY>
 void launch_fused_t(const FusedArgs& a, const YeePlan* p, hipStream_t st) {
   const int gx = (a.nx + 63) / 64;
-  if (p->fused_variant == 1) {
+  if (p->fused_variant == 2) {
+    launch_v2<MODE, LOSSY>(a, p, st);
+  } else if (p->fused_variant == 1) {
     const int gy = (a.ny + 3) / 4;
     const int kz = fused_kz(p, gx, gy);
     const dim3 grid(gx, gy, (a.nz + kz - 1) / kz);

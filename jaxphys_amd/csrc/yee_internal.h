@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -74,6 +75,103 @@ inline __device__ int slab_index(int a, int n, int p) {
   return a < p ? a : a - (n - 2 * p);
 }
 
+// Tiles of a z-marching launch: gx * gy * gz tiles of 64 x 4 columns by kz
+// planes, numbered x fastest, then y, then z chunk.  The launch is 1-D.
+// Workgroups are dispatched round-robin over the chip's 8 XCDs (workgroup L
+// runs on XCD L % 8), each XCD with its own L2.  With `per` > 0 the
+// workgroup -> tile map is t = (L % 8) * per + L / 8: every XCD works
+// through its own contiguous range of tiles in order, so the x/y
+// neighbour tiles whose edge rows and columns a tile re-reads sit in the
+// same L2 at about the same time.  per == 0 is the plain order t = L.
+constexpr int kXcds = 8;
+
+struct Tiling {
+  int gx, gy, ntiles, per;
+};
+
+inline Tiling make_tiling(int nx, int ny, int nz, int kz, bool xcd) {
+  Tiling t;
+  t.gx = (nx + 63) / 64;
+  t.gy = (ny + 3) / 4;
+  const int gz = (nz + kz - 1) / kz;
+  t.ntiles = t.gx * t.gy * gz;
+  t.per = xcd ? (t.ntiles + kXcds - 1) / kXcds : 0;
+  return t;
+}
+
+inline unsigned tiling_blocks(const Tiling& t) {
+  return t.per > 0 ? (unsigned)(t.per * kXcds) : (unsigned)t.ntiles;
+}
+
+
+// A set of tiles given as up to kMaxBoxes boxes (x, y in tile units, z in
+// planes, chunked by kz); `first` is the set-wide index of a box's first
+// tile.  Used to launch the CPML interior and its slab shell separately.
+constexpr int kMaxBoxes = 6;
+
+struct TileBox {
+  int x0, nx, y0, ny, z0, z1, first;
+};
+
+struct TileSet {
+  int nbox, ntiles, per, kz;
+  TileBox b[kMaxBoxes];
+};
+
+inline void tileset_add(TileSet& ts, int x0, int x1, int y0, int y1, int z0, int z1) {
+  if (x1 <= x0 || y1 <= y0 || z1 <= z0) return;
+  TileBox& B = ts.b[ts.nbox++];
+  B.x0 = x0;
+  B.nx = x1 - x0;
+  B.y0 = y0;
+  B.ny = y1 - y0;
+  B.z0 = z0;
+  B.z1 = z1;
+  B.first = ts.ntiles;
+  ts.ntiles += B.nx * B.ny * ((z1 - z0 + ts.kz - 1) / ts.kz);
+}
+
+inline void tileset_finish(TileSet& ts, bool xcd) {
+  ts.per = xcd ? (ts.ntiles + kXcds - 1) / kXcds : 0;
+}
+
+inline unsigned tileset_blocks(const TileSet& ts) {
+  return ts.per > 0 ? (unsigned)(ts.per * kXcds) : (unsigned)ts.ntiles;
+}
+
+// (bx, by, k0, k1) of this workgroup's tile in the set; false for the
+// padding workgroups of the XCD-aware order (same map as Tiling)
+__device__ __forceinline__ bool tile_of_set(const TileSet& ts, int& bx, int& by, int& k0,
+                                            int& k1) {
+  const int L = blockIdx.x;
+  const int t = ts.per > 0 ? (L % kXcds) * ts.per + L / kXcds : L;
+  if (t >= ts.ntiles) return false;
+  TileBox B = ts.b[0];
+#pragma unroll
+  for (int q = 1; q < kMaxBoxes; ++q)
+    if (q < ts.nbox && t >= ts.b[q].first) B = ts.b[q];
+  const int lt = t - B.first;
+  bx = B.x0 + lt % B.nx;
+  const int r = lt / B.nx;
+  by = B.y0 + r % B.ny;
+  k0 = B.z0 + (r / B.ny) * ts.kz;
+  k1 = min(k0 + ts.kz, B.z1);
+  return true;
+}
+
+// (bx, by, bz) of this workgroup; false for the padding workgroups of the
+// remapped order
+__device__ __forceinline__ bool tile_of(const Tiling& tl, int& bx, int& by, int& bz) {
+  const int L = blockIdx.x;
+  const int t = tl.per > 0 ? (L % kXcds) * tl.per + L / kXcds : L;
+  if (t >= tl.ntiles) return false;
+  bx = t % tl.gx;
+  const int r = t / tl.gx;
+  by = r % tl.gy;
+  bz = r / tl.gy;
+  return true;
+}
+
 }  // namespace yee
 
 // ---------------------------------------------------------------------------
@@ -83,8 +181,8 @@ struct YeePlan {
   YeeGridDesc d;
   int kz;            // z planes per block of the two-pass kernels
   int fused_kz = 0;  // z planes per block of the fused kernel (0: automatic)
-  int fused_variant = 0;  // 0: LDS + halo row, 1: barrier-free recompute
-  dim3 grid;
+  int fused_variant = 2;  // 0: LDS + halo row, 1/2: barrier-free recompute
+  bool xcd = true;   // XCD-aware tile order (see Tiling)
   // sources
   int nsrc = 0;
   int nsteps_table = 0;

@@ -225,13 +225,15 @@ def test_copy_probe(gpu):
     assert torch.equal(src, dst)
 
 
-@pytest.mark.parametrize("variant", [0, 1])
-@pytest.mark.parametrize("boundary,lossy", [("absorbing", False), ("periodic", False),
-                                            ("walls", True), ("cpml", False), ("cpml", True)])
-def test_fused_step_bitwise_equals_two_pass(gpu, boundary, lossy, variant):
-    """One fused launch per step vs the H and E half-step kernels: the same
-    arithmetic in the same order, so every field must agree bit for bit."""
-    size = (70, 37, 29)
+@pytest.mark.parametrize("variant,xcd", [(0, True), (1, True), (2, True), (2, False)])
+@pytest.mark.parametrize("boundary,lossy,size", [
+    ("absorbing", False, (70, 37, 29)), ("periodic", False, (70, 37, 29)),
+    ("walls", True, (70, 37, 29)), ("cpml", False, (70, 37, 29)), ("cpml", True, (70, 37, 29)),
+    # wide enough for a slab-free CPML interior (split launches of variant 2)
+    ("cpml", True, (200, 41, 30)), ("cpml", False, (140, 40, 33))])
+def test_fused_step_bitwise_equals_two_pass(gpu, boundary, lossy, size, variant, xcd):
+    """One fused step vs the H and E half-step kernels: the same arithmetic
+    in the same order, so every field must agree bit for bit."""
     rng = np.random.default_rng(21)
     eps = _boxes(size, rng, 10, 3, 14, 1.0, 9.0)
     sigma = None
@@ -240,7 +242,9 @@ def test_fused_step_bitwise_equals_two_pass(gpu, boundary, lossy, variant):
         sigma[:, 10:20, :] = 0.03 * R.EPS0 * R.C0
     g = EMGrid3D(size=size, resolution=1.0, boundary=boundary, pml_layers=6)
     g.add_material(DielectricRegion(mask=np.ones(size, bool), epsilon_r=eps, sigma=sigma))
-    for q, pos in enumerate([(35, 18, 14), (0, 0, 0), (69, 36, 28), (10, 30, 5)]):
+    nx, ny, nz = size
+    for q, pos in enumerate([(nx // 2, ny // 2, nz // 2), (0, 0, 0), (nx - 1, ny - 1, nz - 1),
+                             (10, 30, 5)]):
         g.add_source(PointSource3D(0.0, pos, 1.0 + q, "xyz"[q % 3], waveform="gaussian_sine",
                                    t0=20, spread=8, wavelength=12))
     dt = 0.5 / math.sqrt(3.0) / R.C0
@@ -248,7 +252,10 @@ def test_fused_step_bitwise_equals_two_pass(gpu, boundary, lossy, variant):
     a = g.build_solver(dt, n)
     b = g.build_solver(dt, n)
     a.plan.set_variant(variant)
+    a.plan.set_tile_order(xcd)
+    b.plan.set_tile_order(xcd)
     a.plan.tune(kz=0, fused_kz=7)   # several z chunks per column
+    b.plan.tune(kz=5, fused_kz=-1)
     a.run(0, n)              # fused (odd count: ends in the second buffer)
     b.run_two_pass(0, n)
     for q in range(6):
