@@ -276,7 +276,7 @@ def main() -> None:
         # slab shell) plus the point-source kernel; algorithmic bytes per
         # cell-update are those of the two-pass design (76 B, 80 B with a
         # per-cell Ca) plus 8 B per psi cell, whatever the kernel moves
-        dom_name = "yee_fused_v2_kernel (interior + shell launches of one step)"
+        dom_name = "yee_fused_v2_kernel<..., 3> (interior + shell launches of one step)"
         dom_bytes = (80 if lossy else 76) * cells + psi_bytes
         dom_ms = launch_ms
         step_bytes = dom_bytes

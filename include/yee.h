@@ -94,10 +94,11 @@ void yee_plan_destroy(YeePlan* plan);
  * 0 = automatic, < 0 = unchanged).  Results do not depend on them.       */
 int yee_plan_tune(YeePlan* plan, int32_t kz, int32_t fused_kz);
 /* Fused-kernel design: 0 = LDS exchange with a halo row, 1 = barrier-free
- * (H of column x-1 and row y-1 recomputed), 2 (default) = barrier-free
- * with straight-line planes, carried E values and, under CPML, separate
- * launches for the slab-free interior and the slab shell.  Same results
- * bit for bit.                                                           */
+ * (H of column x-1 and row y-1 recomputed), 2 = barrier-free with
+ * straight-line planes, carried E values and, under CPML, separate
+ * launches for the slab-free interior and the slab shell, 3 (default) = 2
+ * with the x neighbours exchanged between lanes.  Same results bit for
+ * bit.                                                                   */
 int yee_plan_set_variant(YeePlan* plan, int32_t variant);
 /* Workgroup -> tile order (no reference counterpart): nonzero (default) =
  * XCD-aware (each of the 8 XCDs runs a contiguous range of tiles, so

@@ -474,7 +474,7 @@ int yee_plan_create(const YeeGridDesc* desc, YeePlan** out) {
   if (const char* e = std::getenv("JAXPHYS_AMD_FUSED_KZ")) p->fused_kz = std::atoi(e);
   if (const char* e = std::getenv("JAXPHYS_AMD_FUSED_VARIANT")) {
     const int v = std::atoi(e);
-    if (v >= 0 && v <= 2) p->fused_variant = v;
+    if (v >= 0 && v <= 3) p->fused_variant = v;
   }
   if (const char* e = std::getenv("JAXPHYS_AMD_XCD")) p->xcd = std::atoi(e) != 0;
   *out = p;
@@ -490,7 +490,7 @@ int yee_plan_tune(YeePlan* p, int32_t kz, int32_t fused_kz) {
 
 int yee_plan_set_variant(YeePlan* p, int32_t variant) {
   if (!p) return fail(YEE_EINVAL, "null plan");
-  if (variant < 0 || variant > 2) return fail(YEE_EINVAL, "unknown fused variant");
+  if (variant < 0 || variant > 3) return fail(YEE_EINVAL, "unknown fused variant");
   p->fused_variant = variant;
   return YEE_OK;
 }

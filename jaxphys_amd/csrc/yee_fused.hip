@@ -356,7 +356,7 @@ __global__ __launch_bounds__(256) void yee_fused_nolds_kernel(
 }
 
 // ---------------------------------------------------------------------------
-// Variant 2 (default): the barrier-free recompute design of variant 1,
+// Variant 2: the barrier-free recompute design of variant 1,
 // reorganised so that every plane is straight-line code:
 //  * CPML work is chosen per plane by a block-uniform mask -- bit 0: the
 //    tile (with its x-1 column) touches an x slab, bit 1: a y slab, bit 2:
@@ -555,10 +555,196 @@ __device__ __forceinline__ void v2_plane(
   r.hym = hy;
 }
 
+// ---------------------------------------------------------------------------
+// Variant 3 (default): variant 2 with the x neighbours taken from the
+// neighbouring lanes.  A wavefront holds 64 consecutive x cells of one row,
+// so E of column x+1 and the new H of column x-1 are already in registers
+// of lane+1 / lane-1 and are fetched with a cross-lane permute (ds_bpermute,
+// no LDS storage, no barrier).  Only the wave's edge lanes touch memory for
+// them: lane 0 recomputes Hy, Hz of column x-1 (the previous tile's last
+// column) from loads, and the last active lane loads E of column x+1.  Per
+// plane a wave issues 15 full-width loads instead of 24.
+// ---------------------------------------------------------------------------
+
+template <int MODE, bool LOSSY, int SL>
+__device__ __forceinline__ void v3_plane(
+    const FusedArgs& a, const float* __restrict__ Ex, const float* __restrict__ Ey,
+    const float* __restrict__ Ez, const float* __restrict__ Hx, const float* __restrict__ Hy,
+    const float* __restrict__ Hz, float* __restrict__ oEx, float* __restrict__ oEy,
+    float* __restrict__ oEz, float* __restrict__ oHx, float* __restrict__ oHy,
+    float* __restrict__ oHz, const V2Col& q, bool lane0, bool edge_r, int k, unsigned c,
+    V2Carry& r) {
+  constexpr bool per = MODE == kPeriodic;
+  constexpr bool XS = MODE == kCpml && (SL & 1);
+  constexpr bool YS = MODE == kCpml && (SL & 2);
+  constexpr bool ZS = MODE == kCpml && (SL & 4);
+  const int nx = a.nx, ny = a.ny, nz = a.nz, p = a.p;
+  const unsigned P = (unsigned)nx * ny;
+  const bool zinH = per || k < nz - 1;
+  const bool zinE = per || k > 0;
+  const unsigned zp = (per && k == nz - 1) ? -(unsigned)(nz - 1) * P : (zinH ? P : 0u);
+  const float ch = a.ch;
+
+  // ---- full-width loads: the column, its y+1 / z+1 neighbours, row y-1
+  const float ex = r.ex, ey = r.ey, eym = r.eym;
+  const float ez = Ez[c];
+  const float ex_yp = Ex[c + q.yp], ez_yp = Ez[c + q.yp];
+  const float ex_zp = Ex[c + zp], ey_zp = Ey[c + zp];
+  const float hx0 = Hx[c], hy0 = Hy[c], hz0 = Hz[c];
+  const unsigned cym = c + q.ym;
+  const float ez_ym = Ez[cym], ex_ym = Ex[cym], ey_ym_zp = Ey[cym + zp];
+  const float hx_ym0 = Hx[cym], hz_ym0 = Hz[cym];
+  const float cb = a.cb[c];
+  const float ca = LOSSY ? a.ca[c] : 1.f;
+  // ---- edge-lane loads, issued together with the others (exec-masked)
+  float ld_ey_xp = 0.f, ld_ez_xp = 0.f, ld_ey_ym_xp = 0.f;
+  if (edge_r) {
+    ld_ey_xp = Ey[c + q.xp];
+    ld_ez_xp = Ez[c + q.xp];
+    ld_ey_ym_xp = Ey[cym + q.xp];
+  }
+  const unsigned cxm = c + q.xm;
+  float ez_xm = 0.f, ey_xm = 0.f, ex_xm_yp = 0.f, ex_xm_zp = 0.f, hy_xm0 = 0.f, hz_xm0 = 0.f;
+  if (lane0) {
+    ez_xm = Ez[cxm];
+    ey_xm = Ey[cxm];
+    ex_xm_yp = Ex[cxm + q.yp];
+    ex_xm_zp = Ex[cxm + zp];
+    hy_xm0 = Hy[cxm];
+    hz_xm0 = Hz[cxm];
+  }
+  // ---- column x+1 from lane+1 (the last active lane uses its loads)
+  float ey_xp = __shfl_down(ey, 1);
+  float ez_xp = __shfl_down(ez, 1);
+  float ey_ym_xp = __shfl_down(eym, 1);
+  if (edge_r) {
+    ey_xp = ld_ey_xp;
+    ez_xp = ld_ez_xp;
+    ey_ym_xp = ld_ey_ym_xp;
+  }
+
+  const float* tx = a.cpml;
+  const float* ty = a.cpml + 6 * nx;
+  const float* tz = a.cpml + 6 * (nx + ny);
+  const unsigned ysl = (unsigned)a.ysl, zsl = (unsigned)a.zsl, xsl = (unsigned)a.xsl;
+  const unsigned by0 = 0, bz0 = 4 * ysl, bx0 = 4 * ysl + 4 * zsl;
+  const bool zs = ZS && (k < p || k >= nz - p);
+  const int sz = zs ? slab_index(k, nz, p) : 0;
+  const unsigned tp = 2u * (unsigned)p;
+  auto offy = [&](int s, int ii) { return ((unsigned)k * tp + (unsigned)s) * nx + ii; };
+  auto offz = [&](int jj, int ii) { return ((unsigned)sz * ny + jj) * nx + ii; };
+  auto offx = [&](int jj, int s) { return ((unsigned)k * ny + jj) * tp + (unsigned)s; };
+  const float* __restrict__ pin = a.pin;
+  float* __restrict__ pout = a.pout;
+
+  // ---- H^{n+1/2} of the column (fdtd3d.py:303-330)
+  float hx, hy, hz;
+  {
+    const bool upd = q.yinH && zinH;
+    float u = ez_yp - ez, v = ey_zp - ey;
+    if (YS) u = v2_psi(pin, pout, by0 + kHxy * ysl, offy(q.sy, q.i), q.ys, upd, ty, ny, q.j, 1, u);
+    if (ZS) v = v2_psi(pin, pout, bz0 + (kHxz - 4) * zsl, offz(q.j, q.i), zs, upd, tz, nz, k, 1, v);
+    hx = upd ? hx0 - ch * (u - v) : hx0;
+  }
+  {
+    const bool upd = q.xinH && zinH;
+    float u = ex_zp - ex, v = ez_xp - ez;
+    if (ZS) u = v2_psi(pin, pout, bz0 + (kHyz - 4) * zsl, offz(q.j, q.i), zs, upd, tz, nz, k, 1, u);
+    if (XS) v = v2_psi(pin, pout, bx0 + (kHyx - 8) * xsl, offx(q.j, q.sx), q.xs, upd, tx, nx, q.i, 1, v);
+    hy = upd ? hy0 - ch * (u - v) : hy0;
+  }
+  {
+    const bool upd = q.xinH && q.yinH;
+    float u = ey_xp - ey, v = ex_yp - ex;
+    if (XS) u = v2_psi(pin, pout, bx0 + (kHzx - 8) * xsl, offx(q.j, q.sx), q.xs, upd, tx, nx, q.i, 1, u);
+    if (YS) v = v2_psi(pin, pout, by0 + kHzy * ysl, offy(q.sy, q.i), q.ys, upd, ty, ny, q.j, 1, v);
+    hz = upd ? hz0 - ch * (u - v) : hz0;
+  }
+  // ---- Hy, Hz of column x-1: lane-1's new values; lane 0 recomputes them
+  float hy_xm = __shfl_up(hy, 1);
+  float hz_xm = __shfl_up(hz, 1);
+  if (lane0) {
+    const float exm = r.exm;
+    {
+      float u = ex_xm_zp - exm, v = ez - ez_xm;
+      if (ZS) u = v2_psi(pin, pout, bz0 + (kHyz - 4) * zsl, offz(q.j, q.im), zs, false, tz, nz, k, 1, u);
+      if (XS) v = v2_psi(pin, pout, bx0 + (kHyx - 8) * xsl, offx(q.j, q.sxm), q.xsm, false, tx, nx, q.im, 1, v);
+      hy_xm = zinH ? hy_xm0 - ch * (u - v) : hy_xm0;
+    }
+    {
+      float u = ey - ey_xm, v = ex_xm_yp - exm;
+      if (XS) u = v2_psi(pin, pout, bx0 + (kHzx - 8) * xsl, offx(q.j, q.sxm), q.xsm, false, tx, nx, q.im, 1, u);
+      if (YS) v = v2_psi(pin, pout, by0 + kHzy * ysl, offy(q.sy, q.im), q.ys, false, ty, ny, q.j, 1, v);
+      hz_xm = q.yinH ? hz_xm0 - ch * (u - v) : hz_xm0;
+    }
+  }
+  r.exm = ex_xm_zp;  // meaningful in lane 0 only
+  // ---- Hx, Hz at (i, j-1, k)
+  float hx_ym, hz_ym;
+  {
+    float u = ez - ez_ym, v = ey_ym_zp - eym;
+    if (YS) u = v2_psi(pin, pout, by0 + kHxy * ysl, offy(q.sym, q.i), q.ysm, false, ty, ny, q.jm, 1, u);
+    if (ZS) v = v2_psi(pin, pout, bz0 + (kHxz - 4) * zsl, offz(q.jm, q.i), zs, false, tz, nz, k, 1, v);
+    hx_ym = zinH ? hx_ym0 - ch * (u - v) : hx_ym0;
+  }
+  {
+    float u = ey_ym_xp - eym, v = ex - ex_ym;
+    if (XS) u = v2_psi(pin, pout, bx0 + (kHzx - 8) * xsl, offx(q.jm, q.sx), q.xs, false, tx, nx, q.i, 1, u);
+    if (YS) v = v2_psi(pin, pout, by0 + kHzy * ysl, offy(q.sym, q.i), q.ysm, false, ty, ny, q.jm, 1, v);
+    hz_ym = q.xinH ? hz_ym0 - ch * (u - v) : hz_ym0;
+  }
+  oHx[c] = hx;
+  oHy[c] = hy;
+  oHz[c] = hz;
+
+  // ---- E^{n+1} of the column (fdtd3d.py:332-359)
+  float vx, vy, vz;
+  {
+    const bool upd = q.yinE && zinE;
+    float u = hz - hz_ym, v = hy - r.hym;
+    if (YS) u = v2_psi(pin, pout, by0 + kExy * ysl, offy(q.sy, q.i), q.ys, upd, ty, ny, q.j, 0, u);
+    if (ZS) v = v2_psi(pin, pout, bz0 + (kExz - 4) * zsl, offz(q.j, q.i), zs, upd, tz, nz, k, 0, v);
+    const float w = LOSSY ? ca * ex + cb * (u - v) : ex + cb * (u - v);
+    vx = upd ? w : ex;
+  }
+  {
+    const bool upd = q.xinE && zinE;
+    float u = hx - r.hxm, v = hz - hz_xm;
+    if (ZS) u = v2_psi(pin, pout, bz0 + (kEyz - 4) * zsl, offz(q.j, q.i), zs, upd, tz, nz, k, 0, u);
+    if (XS) v = v2_psi(pin, pout, bx0 + (kEyx - 8) * xsl, offx(q.j, q.sx), q.xs, upd, tx, nx, q.i, 0, v);
+    const float w = LOSSY ? ca * ey + cb * (u - v) : ey + cb * (u - v);
+    vy = upd ? w : ey;
+  }
+  {
+    const bool upd = q.xinE && q.yinE;
+    float u = hy - hy_xm, v = hx - hx_ym;
+    if (XS) u = v2_psi(pin, pout, bx0 + (kEzx - 8) * xsl, offx(q.j, q.sx), q.xs, upd, tx, nx, q.i, 0, u);
+    if (YS) v = v2_psi(pin, pout, by0 + kEzy * ysl, offy(q.sy, q.i), q.ys, upd, ty, ny, q.j, 0, v);
+    const float w = LOSSY ? ca * ez + cb * (u - v) : ez + cb * (u - v);
+    vz = upd ? w : ez;
+  }
+  if (MODE == kSponge) {  // fdtd3d.py:361-366; a factor exp(-0) == 1 changes no bit
+    const float sig = (q.sxy + a.sponge[nx + ny + k]) / 3.0f;
+    const float d = sig != 0.f ? expf(((-sig) * a.sdt) / a.seps0) : 1.f;
+    vx *= d;
+    vy *= d;
+    vz *= d;
+  }
+  oEx[c] = vx;
+  oEy[c] = vy;
+  oEz[c] = vz;
+
+  r.ex = ex_zp;
+  r.ey = ey_zp;
+  r.eym = ey_ym_zp;
+  r.hxm = hx;
+  r.hym = hy;
+}
+
 // SHELL == false with CPML: the tiles of `ts` lie outside every slab (no
 // psi code is compiled in); SHELL == true: any tiles, slab work chosen per
 // plane.
-template <int MODE, bool LOSSY, bool SHELL>
+template <int MODE, bool LOSSY, bool SHELL, int VAR>
 __global__ __launch_bounds__(256) void yee_fused_v2_kernel(
     FusedArgs a, TileSet ts, const float* __restrict__ fi0,
     const float* __restrict__ fi1, const float* __restrict__ fi2, const float* __restrict__ fi3,
@@ -621,9 +807,16 @@ __global__ __launch_bounds__(256) void yee_fused_v2_kernel(
       h_cell<MODE, 3>(a, i, j, kp, false, r.hxm, r.hym, unused, nullptr);
     }
   }
-#define YEE_V2_PLANE(S)                                                                   \
-  v2_plane<MODE, LOSSY, S>(a, fi0, fi1, fi2, fi3, fi4, fi5, fo0, fo1, fo2, fo3, fo4, fo5, q, \
-                           k, c, r)
+  const int lane = threadIdx.x & 63;
+  const bool lane0 = lane == 0;
+  const bool edge_r = lane == 63 || i == nx - 1;
+#define YEE_V2_PLANE(S)                                                                       \
+  if constexpr (VAR == 3)                                                                     \
+    v3_plane<MODE, LOSSY, S>(a, fi0, fi1, fi2, fi3, fi4, fi5, fo0, fo1, fo2, fo3, fo4, fo5, q, \
+                             lane0, edge_r, k, c, r);                                         \
+  else                                                                                        \
+    v2_plane<MODE, LOSSY, S>(a, fi0, fi1, fi2, fi3, fi4, fi5, fo0, fo1, fo2, fo3, fo4, fo5, q, \
+                             k, c, r)
   for (int k = k0; k < k1; ++k, c += P) {
     if (MODE != kCpml || !SHELL) {
       YEE_V2_PLANE(0);
@@ -656,7 +849,7 @@ int fused_kz(const YeePlan* p, int /*gx*/, int /*gy*/) {
 // their x-1 / y-1 neighbours, planes p .. nz-p-1) form one box run by the
 // lean SHELL == false kernel; the rest, as up to six boxes, by the SHELL
 // kernel.  Both read buffer A and write disjoint cells of buffer B.
-template <int MODE, bool LOSSY>
+template <int MODE, bool LOSSY, int VAR>
 void launch_v2(const FusedArgs& a, const YeePlan* pl, hipStream_t st) {
   const int nx = a.nx, ny = a.ny, nz = a.nz, p = a.p;
   const int gx = (nx + 63) / 64, gy = (ny + 3) / 4;
@@ -691,12 +884,12 @@ void launch_v2(const FusedArgs& a, const YeePlan* pl, hipStream_t st) {
   tileset_finish(in, pl->xcd);
   tileset_finish(sh, pl->xcd);
   if (in.ntiles > 0)
-    yee_fused_v2_kernel<MODE, LOSSY, false><<<tileset_blocks(in), 256, 0, st>>>(
+    yee_fused_v2_kernel<MODE, LOSSY, false, VAR><<<tileset_blocks(in), 256, 0, st>>>(
         a, in, a.fi[0], a.fi[1], a.fi[2], a.fi[3], a.fi[4], a.fi[5], a.fo[0], a.fo[1], a.fo[2],
         a.fo[3], a.fo[4], a.fo[5]);
   if constexpr (MODE == kCpml)
     if (sh.ntiles > 0)
-      yee_fused_v2_kernel<MODE, LOSSY, true><<<tileset_blocks(sh), 256, 0, st>>>(
+      yee_fused_v2_kernel<MODE, LOSSY, true, VAR><<<tileset_blocks(sh), 256, 0, st>>>(
         a, sh, a.fi[0], a.fi[1], a.fi[2], a.fi[3], a.fi[4], a.fi[5], a.fo[0], a.fo[1], a.fo[2],
         a.fo[3], a.fo[4], a.fo[5]);
 }
@@ -704,8 +897,10 @@ void launch_v2(const FusedArgs& a, const YeePlan* pl, hipStream_t st) {
 template <int MODE, bool LOSSY>
 void launch_fused_t(const FusedArgs& a, const YeePlan* p, hipStream_t st) {
   const int gx = (a.nx + 63) / 64;
-  if (p->fused_variant == 2) {
-    launch_v2<MODE, LOSSY>(a, p, st);
+  if (p->fused_variant == 3) {
+    launch_v2<MODE, LOSSY, 3>(a, p, st);
+  } else if (p->fused_variant == 2) {
+    launch_v2<MODE, LOSSY, 2>(a, p, st);
   } else if (p->fused_variant == 1) {
     const int gy = (a.ny + 3) / 4;
     const int kz = fused_kz(p, gx, gy);

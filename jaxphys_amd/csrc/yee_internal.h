@@ -181,7 +181,7 @@ struct YeePlan {
   YeeGridDesc d;
   int kz;            // z planes per block of the two-pass kernels
   int fused_kz = 0;  // z planes per block of the fused kernel (0: automatic)
-  int fused_variant = 2;  // 0: LDS + halo row, 1/2: barrier-free recompute
+  int fused_variant = 3;  // 0: LDS + halo row, 1-3: barrier-free (see yee_fused.hip)
   bool xcd = true;   // XCD-aware tile order (see Tiling)
   // sources
   int nsrc = 0;

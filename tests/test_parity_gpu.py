@@ -225,7 +225,7 @@ def test_copy_probe(gpu):
     assert torch.equal(src, dst)
 
 
-@pytest.mark.parametrize("variant,xcd", [(0, True), (1, True), (2, True), (2, False)])
+@pytest.mark.parametrize("variant,xcd", [(0, True), (1, True), (2, True), (3, True), (3, False)])
 @pytest.mark.parametrize("boundary,lossy,size", [
     ("absorbing", False, (70, 37, 29)), ("periodic", False, (70, 37, 29)),
     ("walls", True, (70, 37, 29)), ("cpml", False, (70, 37, 29)), ("cpml", True, (70, 37, 29)),
