@@ -50,6 +50,7 @@ EXPORTS = {
     "yee_plan_tune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     "yee_plan_set_variant": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "yee_plan_set_tile_order": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "yee_plan_set_fused_rows": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "yee_set_sources": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32]),
     "yee_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
@@ -168,9 +169,13 @@ class Plan:
     def set_variant(self, variant: int) -> None:
         check(lib().yee_plan_set_variant(self.handle, int(variant)), "yee_plan_set_variant")
 
-    def set_tile_order(self, xcd_aware: bool) -> None:
-        check(lib().yee_plan_set_tile_order(self.handle, int(bool(xcd_aware))),
-              "yee_plan_set_tile_order")
+    def set_tile_order(self, xcd_aware) -> None:
+        """True: XCD-aware, False: plain, None: automatic (default)."""
+        v = -1 if xcd_aware is None else int(bool(xcd_aware))
+        check(lib().yee_plan_set_tile_order(self.handle, v), "yee_plan_set_tile_order")
+
+    def set_fused_rows(self, rows: int) -> None:
+        check(lib().yee_plan_set_fused_rows(self.handle, int(rows)), "yee_plan_set_fused_rows")
 
     def psi_floats(self) -> int:
         return int(lib().yee_psi_floats(ctypes.byref(self.desc)))

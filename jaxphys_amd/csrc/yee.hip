@@ -374,8 +374,9 @@ int validate(const YeeGridDesc* d) {
 
 #define YEE_FIELDS(g) (g).f[0], (g).f[1], (g).f[2], (g).f[3], (g).f[4], (g).f[5]
 
+// two-pass kernels: XCD-aware tile order unless plain order is asked for
 static Tiling plan_tiling(const YeePlan* p) {
-  return make_tiling(p->d.nx, p->d.ny, p->d.nz, p->kz, p->xcd);
+  return make_tiling(p->d.nx, p->d.ny, p->d.nz, p->kz, p->xcd != 0);
 }
 
 int launch_h(const YeePlan* p, const GridArgs& g, hipStream_t st) {
@@ -477,6 +478,10 @@ int yee_plan_create(const YeeGridDesc* desc, YeePlan** out) {
     if (v >= 0 && v <= 3) p->fused_variant = v;
   }
   if (const char* e = std::getenv("JAXPHYS_AMD_XCD")) p->xcd = std::atoi(e) != 0;
+  if (const char* e = std::getenv("JAXPHYS_AMD_FUSED_ROWS")) {
+    const int r = std::atoi(e);
+    if (r == 4 || r == 8 || r == 16) p->fused_rows = r;
+  }
   *out = p;
   return YEE_OK;
 }
@@ -497,7 +502,14 @@ int yee_plan_set_variant(YeePlan* p, int32_t variant) {
 
 int yee_plan_set_tile_order(YeePlan* p, int32_t xcd_aware) {
   if (!p) return fail(YEE_EINVAL, "null plan");
-  p->xcd = xcd_aware != 0;
+  p->xcd = xcd_aware < 0 ? -1 : (xcd_aware != 0);
+  return YEE_OK;
+}
+
+int yee_plan_set_fused_rows(YeePlan* p, int32_t rows) {
+  if (!p) return fail(YEE_EINVAL, "null plan");
+  if (rows != 4 && rows != 8 && rows != 16) return fail(YEE_EINVAL, "fused rows must be 4, 8 or 16");
+  p->fused_rows = rows;
   return YEE_OK;
 }
 

@@ -28,9 +28,14 @@
 namespace yee {
 
 struct AdjArgs {
-  GridArgs g;                  // forward coefficients (f[] unused)
-  float* lam[6];               // adjoint fields, in/out
-  float* lpsi[kNumPsi];        // adjoint psi (same layout as forward psi)
+  int nx, ny, nz, p;
+  float ch, sdt, seps0;
+  unsigned ysl, zsl, xsl;      // psi slab sizes (floats) of the y, z, x groups
+  const float* cb;
+  const float* ca;
+  const float* sponge;         // sigma_x | sigma_y | sigma_z
+  const float* cpml;           // x | y | z tables, 6 n_a floats each
+  float* lpsi;                 // adjoint psi (layout of the forward psi)
   int bx0, by0, bz0, bnx, bny, bnz;  // design box (bnx == 0: none)
   const float* e_next;         // box history E^{n+1}, comp-major
   const float* e_cur;          // box history E^n
@@ -38,276 +43,315 @@ struct AdjArgs {
   float* acc_ca;
 };
 
-template <int MODE>
-struct Geo {
-  const AdjArgs& a;
-  int nx, ny, nz, p;
-  unsigned P;
-  __device__ Geo(const AdjArgs& a_) : a(a_), nx(a_.g.nx), ny(a_.g.ny), nz(a_.g.nz), p(a_.g.p) {
-    P = (unsigned)nx * ny;
-  }
-  __device__ unsigned idx(int i, int j, int k) const { return ((unsigned)k * ny + j) * nx + i; }
-  // wrap (periodic) or validity (others) of a neighbour index
-  __device__ bool fix(int& v, int n) const {
-    if (v >= 0 && v < n) return true;
-    if (MODE != kPeriodic) return false;
-    v = v < 0 ? v + n : v - n;
-    return true;
-  }
-  __device__ bool e_in(int comp, int i, int j, int k) const {
-    if (MODE == kPeriodic) return true;
-    if (comp == 0) return j > 0 && k > 0;
-    if (comp == 1) return i > 0 && k > 0;
-    return i > 0 && j > 0;
-  }
-  __device__ bool h_in(int comp, int i, int j, int k) const {
-    if (MODE == kPeriodic) return true;
-    if (comp == 0) return j < ny - 1 && k < nz - 1;
-    if (comp == 1) return i < nx - 1 && k < nz - 1;
-    return i < nx - 1 && j < ny - 1;
-  }
-  __device__ float damp(int i, int j, int k) const {
-    if (MODE != kSponge) return 1.f;
-    const float sig = ((a.g.ssx[i] + a.g.ssy[j]) + a.g.ssz[k]) / 3.0f;
-    return sig != 0.f ? expf(((-sig) * a.g.sdt) / a.g.seps0) : 1.f;
-  }
-  __device__ bool slab(int v, int n) const { return MODE == kCpml && (v < p || v >= n - p); }
-  // psi slot address for derivative axis ax (0 x, 1 y, 2 z) at (i, j, k)
-  __device__ float* psi_at(float* base, int ax, int i, int j, int k) const {
-    if (ax == 1) return base + ((unsigned)(k * 2 * p + slab_index(j, ny, p)) * nx + i);
-    if (ax == 2) return base + ((unsigned)(slab_index(k, nz, p) * ny + j) * nx + i);
-    return base + ((unsigned)k * ny + j) * 2 * p + slab_index(i, nx, p);
-  }
-  __device__ const float* tab(int ax) const { return ax == 0 ? a.g.cpx : (ax == 1 ? a.g.cpy : a.g.cpz); }
-  __device__ int nax(int ax) const { return ax == 0 ? nx : (ax == 1 ? ny : nz); }
-};
+__device__ __forceinline__ unsigned adj_slot(const AdjArgs& a, int s) {
+  if (s < 4) return (unsigned)s * a.ysl;
+  if (s < 8) return 4 * a.ysl + (unsigned)(s - 4) * a.zsl;
+  return 4 * a.ysl + 4 * a.zsl + (unsigned)(s - 8) * a.xsl;
+}
 
-// The two curl terms of each component: (derivative axis of term a, psi slot
-// of a, derivative axis of term b, psi slot of b), for E and for H.
+// adjoint psi of slot S at (i, j, k); the cell is in the slab of S's axis
+template <int S>
+__device__ __forceinline__ float* adj_psi(const AdjArgs& a, int i, int j, int k) {
+  const int p = a.p;
+  unsigned o;
+  if (S < 4) o = ((unsigned)(k * 2 * p + slab_index(j, a.ny, p))) * a.nx + i;
+  else if (S < 8) o = ((unsigned)(slab_index(k, a.nz, p) * a.ny + j)) * a.nx + i;
+  else o = ((unsigned)k * a.ny + j) * 2 * p + slab_index(i, a.nx, p);
+  return a.lpsi + adj_slot(a, S) + o;
+}
+
+template <int AX>
+__device__ __forceinline__ const float* adj_tab(const AdjArgs& a) {
+  return AX == 0 ? a.cpml : (AX == 1 ? a.cpml + 6 * a.nx : a.cpml + 6 * (a.nx + a.ny));
+}
+
+template <int MODE>
+__device__ __forceinline__ float adj_damp(const AdjArgs& a, int i, int j, int k) {
+  if (MODE != kSponge) return 1.f;
+  const float sig = ((a.sponge[i] + a.sponge[a.nx + j]) + a.sponge[a.nx + a.ny + k]) / 3.0f;
+  return sig != 0.f ? expf(((-sig) * a.sdt) / a.seps0) : 1.f;
+}
+
+// The two curl terms of each component (derivative axis, psi slot):
 //   Ex: a = d/dy Hz, b = d/dz Hy     Hx: a = d/dy Ez, b = d/dz Ey
 //   Ey: a = d/dz Hx, b = d/dx Hz     Hy: a = d/dz Ex, b = d/dx Ez
 //   Ez: a = d/dx Hy, b = d/dy Hx     Hz: a = d/dx Ey, b = d/dy Ex
-__device__ __constant__ int kAxA[3] = {1, 2, 0};
-__device__ __constant__ int kAxB[3] = {2, 0, 1};
-__device__ __constant__ int kPsiEA[3] = {kExy, kEyz, kEzx};
-__device__ __constant__ int kPsiEB[3] = {kExz, kEyx, kEzy};
-__device__ __constant__ int kPsiHA[3] = {kHxy, kHyz, kHzx};
-__device__ __constant__ int kPsiHB[3] = {kHxz, kHyx, kHzy};
+template <int C> struct Terms {
+  static constexpr int axa = C == 0 ? 1 : (C == 1 ? 2 : 0);
+  static constexpr int axb = C == 0 ? 2 : (C == 1 ? 0 : 1);
+  static constexpr int ea = C == 0 ? kExy : (C == 1 ? kEyz : kEzx);
+  static constexpr int eb = C == 0 ? kExz : (C == 1 ? kEyx : kEzy);
+  static constexpr int ha = C == 0 ? kHxy : (C == 1 ? kHyz : kHzx);
+  static constexpr int hb = C == 0 ? kHxz : (C == 1 ? kHyx : kHzy);
+};
 
-// d J / d a  and  d J / d b  of E component `comp` at (i, j, k), from the
-// adjoint of E^{n+1} (lamE) and of psi_E^{n+1}.  Zero out of range.
-template <int MODE>
-__device__ void e_term_adj(const Geo<MODE>& G, int comp, int i, int j, int k, float& ga, float& gb) {
-  ga = 0.f;
-  gb = 0.f;
-  if (!G.e_in(comp, i, j, k)) return;
-  const unsigned c = G.idx(i, j, k);
-  const float gU = G.a.g.cb[c] * (G.damp(i, j, k) * G.a.lam[comp][c]);
+template <int AX>
+__device__ __forceinline__ int coord(int i, int j, int k) {
+  return AX == 0 ? i : (AX == 1 ? j : k);
+}
+template <int AX>
+__device__ __forceinline__ int extent(const AdjArgs& a) {
+  return AX == 0 ? a.nx : (AX == 1 ? a.ny : a.nz);
+}
+
+// whether E / H component C at (i, j, k) is updated (non-periodic ranges)
+template <int MODE, int C>
+__device__ __forceinline__ bool e_upd(const AdjArgs& a, int i, int j, int k) {
+  if (MODE == kPeriodic) return true;
+  return C == 0 ? (j > 0 && k > 0) : (C == 1 ? (i > 0 && k > 0) : (i > 0 && j > 0));
+}
+template <int MODE, int C>
+__device__ __forceinline__ bool h_upd(const AdjArgs& a, int i, int j, int k) {
+  if (MODE == kPeriodic) return true;
+  return C == 0 ? (j < a.ny - 1 && k < a.nz - 1)
+                : (C == 1 ? (i < a.nx - 1 && k < a.nz - 1) : (i < a.nx - 1 && j < a.ny - 1));
+}
+
+// dJ/d(term a), dJ/d(term b) of E component C at the in-grid cell
+// (i, j, k) from the adjoint of E^{n+1} and of psi_E^{n+1}; zero when
+// `valid` is false or the cell is not updated.  CP: CPML code compiled in.
+template <int MODE, bool CP, int C>
+__device__ __forceinline__ void eterm(const AdjArgs& a, const float* __restrict__ lamE, int i,
+                                      int j, int k, bool valid, float& ga, float& gb) {
+  using T = Terms<C>;
+  const unsigned c = ((unsigned)k * a.ny + j) * a.nx + i;
+  const float gU = a.cb[c] * (adj_damp<MODE>(a, i, j, k) * lamE[c]);
   ga = gU;
   gb = -gU;
-  const int axa = kAxA[comp], axb = kAxB[comp];
-  const int va = axa == 0 ? i : (axa == 1 ? j : k);
-  const int vb = axb == 0 ? i : (axb == 1 ? j : k);
-  if (G.slab(va, G.nax(axa))) {
-    const float* t = G.tab(axa);
-    const int n = G.nax(axa);
-    const float L = *G.psi_at(G.a.lpsi[kPsiEA[comp]], axa, i, j, k) + gU;
-    ga = t[2 * n + va] * gU + t[1 * n + va] * L;
+  if (CP) {
+    const int p = a.p;
+    const int va = coord<T::axa>(i, j, k), na = extent<T::axa>(a);
+    const int vb = coord<T::axb>(i, j, k), nb = extent<T::axb>(a);
+    if (va < p || va >= na - p) {
+      const float* t = adj_tab<T::axa>(a);
+      const float L = *adj_psi<T::ea>(a, i, j, k) + gU;
+      ga = t[2 * na + va] * gU + t[1 * na + va] * L;
+    }
+    if (vb < p || vb >= nb - p) {
+      const float* t = adj_tab<T::axb>(a);
+      const float L = *adj_psi<T::eb>(a, i, j, k) - gU;
+      gb = -t[2 * nb + vb] * gU + t[1 * nb + vb] * L;
+    }
   }
-  if (G.slab(vb, G.nax(axb))) {
-    const float* t = G.tab(axb);
-    const int n = G.nax(axb);
-    const float L = *G.psi_at(G.a.lpsi[kPsiEB[comp]], axb, i, j, k) - gU;
-    gb = -t[2 * n + vb] * gU + t[1 * n + vb] * L;
+  if (!(valid && e_upd<MODE, C>(a, i, j, k))) {
+    ga = 0.f;
+    gb = 0.f;
   }
 }
 
-// same for H component `comp`: uses lamH (already the total adjoint of
-// H^{n+1/2}) and the adjoint of psi_H^{n+1/2}.
-template <int MODE>
-__device__ void h_term_adj(const Geo<MODE>& G, int comp, int i, int j, int k, float& ga, float& gb) {
-  ga = 0.f;
-  gb = 0.f;
-  if (!G.h_in(comp, i, j, k)) return;
-  const unsigned c = G.idx(i, j, k);
-  const float gU = -G.a.g.ch * G.a.lam[3 + comp][c];
+// same for H component C from the adjoint of H^{n+1/2} and psi_H^{n+1/2}
+template <int MODE, bool CP, int C>
+__device__ __forceinline__ void hterm(const AdjArgs& a, const float* __restrict__ lamH, int i,
+                                      int j, int k, bool valid, float& ga, float& gb) {
+  using T = Terms<C>;
+  const unsigned c = ((unsigned)k * a.ny + j) * a.nx + i;
+  const float gU = -a.ch * lamH[c];
   ga = gU;
   gb = -gU;
-  const int axa = kAxA[comp], axb = kAxB[comp];
-  const int va = axa == 0 ? i : (axa == 1 ? j : k);
-  const int vb = axb == 0 ? i : (axb == 1 ? j : k);
-  if (G.slab(va, G.nax(axa))) {
-    const float* t = G.tab(axa);
-    const int n = G.nax(axa);
-    const float L = *G.psi_at(G.a.lpsi[kPsiHA[comp]], axa, i, j, k) + gU;
-    ga = t[5 * n + va] * gU + t[4 * n + va] * L;
-  }
-  if (G.slab(vb, G.nax(axb))) {
-    const float* t = G.tab(axb);
-    const int n = G.nax(axb);
-    const float L = *G.psi_at(G.a.lpsi[kPsiHB[comp]], axb, i, j, k) - gU;
-    gb = -t[5 * n + vb] * gU + t[4 * n + vb] * L;
-  }
-}
-
-template <int MODE>
-__device__ void e_term_adj_at(const Geo<MODE>& G, int comp, int i, int j, int k, float& ga, float& gb) {
-  ga = 0.f;
-  gb = 0.f;
-  if (!G.fix(i, G.nx) || !G.fix(j, G.ny) || !G.fix(k, G.nz)) return;
-  e_term_adj<MODE>(G, comp, i, j, k, ga, gb);
-}
-
-template <int MODE>
-__device__ void h_term_adj_at(const Geo<MODE>& G, int comp, int i, int j, int k, float& ga, float& gb) {
-  ga = 0.f;
-  gb = 0.f;
-  if (!G.fix(i, G.nx) || !G.fix(j, G.ny) || !G.fix(k, G.nz)) return;
-  h_term_adj<MODE>(G, comp, i, j, k, ga, gb);
-}
-
-// Kernel A: per H cell.
-template <int MODE>
-__global__ __launch_bounds__(kThreads) void yee_adj_h_kernel(AdjArgs a) {
-  const int i = blockIdx.x * kBX + (threadIdx.x % kBX);
-  const int j = blockIdx.y * kBY + (threadIdx.x / kBX);
-  const int k = blockIdx.z;
-  Geo<MODE> G(a);
-  if (i >= G.nx || j >= G.ny) return;
-  const unsigned c = G.idx(i, j, k);
-  const float ch = a.g.ch;
-  // deferred psi_H-adjoint update of the step after this one
-  if (MODE == kCpml) {
-    for (int comp = 0; comp < 3; ++comp) {
-      if (!G.h_in(comp, i, j, k)) continue;
-      const float gU = -ch * a.lam[3 + comp][c];
-      const int axa = kAxA[comp], axb = kAxB[comp];
-      const int va = axa == 0 ? i : (axa == 1 ? j : k);
-      const int vb = axb == 0 ? i : (axb == 1 ? j : k);
-      if (G.slab(va, G.nax(axa))) {
-        float* q = G.psi_at(a.lpsi[kPsiHA[comp]], axa, i, j, k);
-        *q = G.tab(axa)[3 * G.nax(axa) + va] * (*q + gU);
-      }
-      if (G.slab(vb, G.nax(axb))) {
-        float* q = G.psi_at(a.lpsi[kPsiHB[comp]], axb, i, j, k);
-        *q = G.tab(axb)[3 * G.nax(axb) + vb] * (*q - gU);
-      }
+  if (CP) {
+    const int p = a.p;
+    const int va = coord<T::axa>(i, j, k), na = extent<T::axa>(a);
+    const int vb = coord<T::axb>(i, j, k), nb = extent<T::axb>(a);
+    if (va < p || va >= na - p) {
+      const float* t = adj_tab<T::axa>(a);
+      const float L = *adj_psi<T::ha>(a, i, j, k) + gU;
+      ga = t[5 * na + va] * gU + t[4 * na + va] * L;
+    }
+    if (vb < p || vb >= nb - p) {
+      const float* t = adj_tab<T::axb>(a);
+      const float L = *adj_psi<T::hb>(a, i, j, k) - gU;
+      gb = -t[5 * nb + vb] * gU + t[4 * nb + vb] * L;
     }
   }
-  float ga0, gb0, ga1, gb1;
-  // Hx: a_y (Ey's z term, Hx[c] - Hx[c-z]) at c and c+z; b_z (Ez's y term) at c and c+y
-  {
-    float s = a.lam[3][c];
-    e_term_adj_at<MODE>(G, 1, i, j, k, ga0, gb0);
-    e_term_adj_at<MODE>(G, 1, i, j, k + 1, ga1, gb1);
-    s += ga0 - ga1;
-    e_term_adj_at<MODE>(G, 2, i, j, k, ga0, gb0);
-    e_term_adj_at<MODE>(G, 2, i, j + 1, k, ga1, gb1);
-    s += gb0 - gb1;
-    a.lam[3][c] = s;
-  }
-  // Hy: b_x (Ex's z term) at c and c+z; a_z (Ez's x term) at c and c+x
-  {
-    float s = a.lam[4][c];
-    e_term_adj_at<MODE>(G, 0, i, j, k, ga0, gb0);
-    e_term_adj_at<MODE>(G, 0, i, j, k + 1, ga1, gb1);
-    s += gb0 - gb1;
-    e_term_adj_at<MODE>(G, 2, i, j, k, ga0, gb0);
-    e_term_adj_at<MODE>(G, 2, i + 1, j, k, ga1, gb1);
-    s += ga0 - ga1;
-    a.lam[4][c] = s;
-  }
-  // Hz: a_x (Ex's y term) at c and c+y; b_y (Ey's x term) at c and c+x
-  {
-    float s = a.lam[5][c];
-    e_term_adj_at<MODE>(G, 0, i, j, k, ga0, gb0);
-    e_term_adj_at<MODE>(G, 0, i, j + 1, k, ga1, gb1);
-    s += ga0 - ga1;
-    e_term_adj_at<MODE>(G, 1, i, j, k, ga0, gb0);
-    e_term_adj_at<MODE>(G, 1, i + 1, j, k, ga1, gb1);
-    s += gb0 - gb1;
-    a.lam[5][c] = s;
+  if (!(valid && h_upd<MODE, C>(a, i, j, k))) {
+    ga = 0.f;
+    gb = 0.f;
   }
 }
 
-// Kernel B: per E cell.
-template <int MODE, bool LOSSY>
-__global__ __launch_bounds__(kThreads) void yee_adj_e_kernel(AdjArgs a) {
-  const int i = blockIdx.x * kBX + (threadIdx.x % kBX);
-  const int j = blockIdx.y * kBY + (threadIdx.x / kBX);
-  const int k = blockIdx.z;
-  Geo<MODE> G(a);
-  if (i >= G.nx || j >= G.ny) return;
-  const unsigned c = G.idx(i, j, k);
-  const float d = G.damp(i, j, k);
-  const float cb = a.g.cb[c];
-  const float ca = LOSSY ? a.g.ca[c] : 1.f;
-  float ga0, gb0, ga1, gb1;
-  float newE[3];
-  float gcomp[3];
-  for (int comp = 0; comp < 3; ++comp) {
-    const float g = d * a.lam[comp][c];
-    gcomp[comp] = g;
-    if (G.e_in(comp, i, j, k)) {
-      const float gU = cb * g;
-      if (MODE == kCpml) {
-        const int axa = kAxA[comp], axb = kAxB[comp];
-        const int va = axa == 0 ? i : (axa == 1 ? j : k);
-        const int vb = axb == 0 ? i : (axb == 1 ? j : k);
-        if (G.slab(va, G.nax(axa))) {
-          float* q = G.psi_at(a.lpsi[kPsiEA[comp]], axa, i, j, k);
-          *q = G.tab(axa)[va] * (*q + gU);
+// deferred psi_H-adjoint update of H component C at (i, j, k)
+template <int MODE, int C>
+__device__ __forceinline__ void psi_h_defer(const AdjArgs& a, float lamh, int i, int j, int k) {
+  using T = Terms<C>;
+  if (!h_upd<MODE, C>(a, i, j, k)) return;
+  const int p = a.p;
+  const float gU = -a.ch * lamh;
+  const int va = coord<T::axa>(i, j, k), na = extent<T::axa>(a);
+  const int vb = coord<T::axb>(i, j, k), nb = extent<T::axb>(a);
+  if (va < p || va >= na - p) {
+    float* q = adj_psi<T::ha>(a, i, j, k);
+    *q = adj_tab<T::axa>(a)[3 * na + va] * (*q + gU);
+  }
+  if (vb < p || vb >= nb - p) {
+    float* q = adj_psi<T::hb>(a, i, j, k);
+    *q = adj_tab<T::axb>(a)[3 * nb + vb] * (*q - gU);
+  }
+}
+
+// psi_E-adjoint update of E component C at (i, j, k) (updated cells only)
+template <int C>
+__device__ __forceinline__ void psi_e_update(const AdjArgs& a, float gU, int i, int j, int k) {
+  using T = Terms<C>;
+  const int p = a.p;
+  const int va = coord<T::axa>(i, j, k), na = extent<T::axa>(a);
+  const int vb = coord<T::axb>(i, j, k), nb = extent<T::axb>(a);
+  if (va < p || va >= na - p) {
+    float* q = adj_psi<T::ea>(a, i, j, k);
+    *q = adj_tab<T::axa>(a)[va] * (*q + gU);
+  }
+  if (vb < p || vb >= nb - p) {
+    float* q = adj_psi<T::eb>(a, i, j, k);
+    *q = adj_tab<T::axb>(a)[vb] * (*q - gU);
+  }
+}
+
+// neighbour index along an axis: wrapped (periodic) or clamped with a
+// validity flag (other boundaries)
+template <int MODE>
+__device__ __forceinline__ int nb_idx(int v, int n, bool& ok) {
+  if (v >= 0 && v < n) {
+    ok = true;
+    return v;
+  }
+  if (MODE == kPeriodic) {
+    ok = true;
+    return v < 0 ? v + n : v - n;
+  }
+  ok = false;
+  return v < 0 ? 0 : n - 1;
+}
+
+// Kernel A: per H cell, z-marching tiles.  CP: the tile may touch a CPML
+// slab (psi code compiled in); otherwise it touches none within one cell.
+template <int MODE, bool CP>
+__global__ __launch_bounds__(kThreads) void yee_adj_h_kernel(
+    AdjArgs a, TileSet ts, const float* __restrict__ lex, const float* __restrict__ ley,
+    const float* __restrict__ lez, float* __restrict__ lhx, float* __restrict__ lhy,
+    float* __restrict__ lhz) {
+  int bx, by, k0, k1;
+  if (!tile_of_set(ts, bx, by, k0, k1)) return;
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const int i = bx * kBX + (threadIdx.x % kBX);
+  const int j = by * kBY + (threadIdx.x / kBX);
+  if (i >= nx || j >= ny) return;
+  bool vx, vy;
+  const int ip = nb_idx<MODE>(i + 1, nx, vx);
+  const int jp = nb_idx<MODE>(j + 1, ny, vy);
+  for (int k = k0; k < k1; ++k) {
+    bool vz;
+    const int kp = nb_idx<MODE>(k + 1, nz, vz);
+    const unsigned c = ((unsigned)k * ny + j) * nx + i;
+    const float hx0 = lhx[c], hy0 = lhy[c], hz0 = lhz[c];
+    if (CP) {
+      psi_h_defer<MODE, 0>(a, hx0, i, j, k);
+      psi_h_defer<MODE, 1>(a, hy0, i, j, k);
+      psi_h_defer<MODE, 2>(a, hz0, i, j, k);
+    }
+    float xa, xb, ya, yb, za, zb, t0, t1;
+    eterm<MODE, CP, 0>(a, lex, i, j, k, true, xa, xb);
+    eterm<MODE, CP, 1>(a, ley, i, j, k, true, ya, yb);
+    eterm<MODE, CP, 2>(a, lez, i, j, k, true, za, zb);
+    // Hx: Ey's a (d/dz Hx) at c and c+z; Ez's b (d/dy Hx) at c and c+y
+    float s = hx0;
+    eterm<MODE, CP, 1>(a, ley, i, j, kp, vz, t0, t1);
+    s += ya - t0;
+    eterm<MODE, CP, 2>(a, lez, i, jp, k, vy, t0, t1);
+    s += zb - t1;
+    lhx[c] = s;
+    // Hy: Ex's b (d/dz Hy) at c and c+z; Ez's a (d/dx Hy) at c and c+x
+    s = hy0;
+    eterm<MODE, CP, 0>(a, lex, i, j, kp, vz, t0, t1);
+    s += xb - t1;
+    eterm<MODE, CP, 2>(a, lez, ip, j, k, vx, t0, t1);
+    s += za - t0;
+    lhy[c] = s;
+    // Hz: Ex's a (d/dy Hz) at c and c+y; Ey's b (d/dx Hz) at c and c+x
+    s = hz0;
+    eterm<MODE, CP, 0>(a, lex, i, jp, k, vy, t0, t1);
+    s += xa - t0;
+    eterm<MODE, CP, 1>(a, ley, ip, j, k, vx, t0, t1);
+    s += yb - t1;
+    lhz[c] = s;
+  }
+}
+
+// Kernel B: per E cell, z-marching tiles.
+template <int MODE, bool LOSSY, bool CP>
+__global__ __launch_bounds__(kThreads) void yee_adj_e_kernel(
+    AdjArgs a, TileSet ts, float* __restrict__ lex, float* __restrict__ ley,
+    float* __restrict__ lez, const float* __restrict__ lhx, const float* __restrict__ lhy,
+    const float* __restrict__ lhz) {
+  int bx, by, k0, k1;
+  if (!tile_of_set(ts, bx, by, k0, k1)) return;
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const int i = bx * kBX + (threadIdx.x % kBX);
+  const int j = by * kBY + (threadIdx.x / kBX);
+  if (i >= nx || j >= ny) return;
+  bool vx, vy;
+  const int im = nb_idx<MODE>(i - 1, nx, vx);
+  const int jm = nb_idx<MODE>(j - 1, ny, vy);
+  for (int k = k0; k < k1; ++k) {
+    bool vz;
+    const int km = nb_idx<MODE>(k - 1, nz, vz);
+    const unsigned c = ((unsigned)k * ny + j) * nx + i;
+    const float d = adj_damp<MODE>(a, i, j, k);
+    const float cb = a.cb[c];
+    const float ca = LOSSY ? a.ca[c] : 1.f;
+    const float g0 = d * lex[c], g1 = d * ley[c], g2 = d * lez[c];
+    float e0 = g0, e1 = g1, e2 = g2;
+    if (e_upd<MODE, 0>(a, i, j, k)) {
+      if (CP) psi_e_update<0>(a, cb * g0, i, j, k);
+      e0 = LOSSY ? ca * g0 : g0;
+    }
+    if (e_upd<MODE, 1>(a, i, j, k)) {
+      if (CP) psi_e_update<1>(a, cb * g1, i, j, k);
+      e1 = LOSSY ? ca * g1 : g1;
+    }
+    if (e_upd<MODE, 2>(a, i, j, k)) {
+      if (CP) psi_e_update<2>(a, cb * g2, i, j, k);
+      e2 = LOSSY ? ca * g2 : g2;
+    }
+    float xa, xb, ya, yb, za, zb, t0, t1;
+    hterm<MODE, CP, 0>(a, lhx, i, j, k, true, xa, xb);
+    hterm<MODE, CP, 1>(a, lhy, i, j, k, true, ya, yb);
+    hterm<MODE, CP, 2>(a, lhz, i, j, k, true, za, zb);
+    // Ex: Hy's a (Ex[c+z] - Ex[c]) at c (-) and c-z (+); Hz's b at c (-), c-y (+)
+    hterm<MODE, CP, 1>(a, lhy, i, j, km, vz, t0, t1);
+    e0 += t0 - ya;
+    hterm<MODE, CP, 2>(a, lhz, i, jm, k, vy, t0, t1);
+    e0 += t1 - zb;
+    // Ey: Hx's b (Ey[c+z] - Ey[c]) at c (-), c-z (+); Hz's a at c (-), c-x (+)
+    hterm<MODE, CP, 0>(a, lhx, i, j, km, vz, t0, t1);
+    e1 += t1 - xb;
+    hterm<MODE, CP, 2>(a, lhz, im, j, k, vx, t0, t1);
+    e1 += t0 - za;
+    // Ez: Hx's a (Ez[c+y] - Ez[c]) at c (-), c-y (+); Hy's b at c (-), c-x (+)
+    hterm<MODE, CP, 0>(a, lhx, i, jm, k, vy, t0, t1);
+    e2 += t0 - xa;
+    hterm<MODE, CP, 1>(a, lhy, im, j, k, vx, t0, t1);
+    e2 += t1 - yb;
+    // design box accumulation (cells with damp == 1, all components updated)
+    if (a.bnx > 0) {
+      const int li = i - a.bx0, lj = j - a.by0, lk = k - a.bz0;
+      if (li >= 0 && li < a.bnx && lj >= 0 && lj < a.bny && lk >= 0 && lk < a.bnz) {
+        const unsigned bc = (unsigned)a.bnx * a.bny * a.bnz;
+        const unsigned l = ((unsigned)lk * a.bny + lj) * a.bnx + li;
+        const float gc[3] = {g0, g1, g2};
+        float s1 = 0.f, s2 = 0.f;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          const float en = a.e_next[q * bc + l], ec = a.e_cur[q * bc + l];
+          s1 += gc[q] * (en - (LOSSY ? ca * ec : ec));
+          s2 += gc[q] * ec;
         }
-        if (G.slab(vb, G.nax(axb))) {
-          float* q = G.psi_at(a.lpsi[kPsiEB[comp]], axb, i, j, k);
-          *q = G.tab(axb)[vb] * (*q - gU);
-        }
+        a.acc_cb[l] += s1;
+        if (LOSSY && a.acc_ca) a.acc_ca[l] += s2;
       }
-      newE[comp] = LOSSY ? ca * g : g;
-    } else {
-      newE[comp] = g;
     }
+    lex[c] = e0;
+    ley[c] = e1;
+    lez[c] = e2;
   }
-  // Ex: Hy's a (Ex[c+z]-Ex[c]) at c (-) and c-z (+); Hz's b (Ex[c+y]-Ex[c]) at c (-), c-y (+)
-  h_term_adj_at<MODE>(G, 1, i, j, k, ga0, gb0);
-  h_term_adj_at<MODE>(G, 1, i, j, k - 1, ga1, gb1);
-  newE[0] += ga1 - ga0;
-  const float hy_b0 = gb0;  // Hy's b at c, reused for Ez
-  h_term_adj_at<MODE>(G, 2, i, j, k, ga0, gb0);
-  h_term_adj_at<MODE>(G, 2, i, j - 1, k, ga1, gb1);
-  newE[0] += gb1 - gb0;
-  const float hz_a0 = ga0;  // Hz's a at c, reused for Ey
-  // Ey: Hx's b (Ey[c+z]-Ey[c]) at c (-), c-z (+); Hz's a (Ey[c+x]-Ey[c]) at c (-), c-x (+)
-  h_term_adj_at<MODE>(G, 0, i, j, k, ga0, gb0);
-  const float hx_a0 = ga0;  // Hx's a at c, reused for Ez
-  h_term_adj_at<MODE>(G, 0, i, j, k - 1, ga1, gb1);
-  newE[1] += gb1 - gb0;
-  h_term_adj_at<MODE>(G, 2, i - 1, j, k, ga1, gb1);
-  newE[1] += ga1 - hz_a0;
-  // Ez: Hx's a (Ez[c+y]-Ez[c]) at c (-), c-y (+); Hy's b (Ez[c+x]-Ez[c]) at c (-), c-x (+)
-  h_term_adj_at<MODE>(G, 0, i, j - 1, k, ga1, gb1);
-  newE[2] += ga1 - hx_a0;
-  h_term_adj_at<MODE>(G, 1, i - 1, j, k, ga1, gb1);
-  newE[2] += gb1 - hy_b0;
-  // design box accumulation (cells with damp == 1, all components in range)
-  if (a.bnx > 0) {
-    const int li = i - a.bx0, lj = j - a.by0, lk = k - a.bz0;
-    if (li >= 0 && li < a.bnx && lj >= 0 && lj < a.bny && lk >= 0 && lk < a.bnz) {
-      const long long bc = (unsigned)a.bnx * a.bny * a.bnz;
-      const long long l = ((unsigned)lk * a.bny + lj) * a.bnx + li;
-      float s1 = 0.f, s2 = 0.f;
-      for (int comp = 0; comp < 3; ++comp) {
-        const float en = a.e_next[comp * bc + l], ec = a.e_cur[comp * bc + l];
-        s1 += gcomp[comp] * (en - (LOSSY ? ca * ec : ec));
-        s2 += gcomp[comp] * ec;
-      }
-      a.acc_cb[l] += s1;
-      if (LOSSY && a.acc_ca) a.acc_ca[l] += s2;
-    }
-  }
-  a.lam[0][c] = newE[0];
-  a.lam[1][c] = newE[1];
-  a.lam[2][c] = newE[2];
 }
 
 __global__ void probe_gather_kernel(GridArgs g, const int* comp, const long long* idx, int n,
@@ -341,10 +385,23 @@ int adjoint_args(const YeePlan* p, float* const* lam, float* lam_psi, const YeeD
                  AdjArgs& a) {
   int rc = check_fields(p, lam, lam_psi);
   if (rc) return rc;
+  const YeeGridDesc& d = p->d;
   std::memset(&a, 0, sizeof(a));
-  a.g = make_args(p, lam, lam_psi);
-  for (int c = 0; c < 6; ++c) a.lam[c] = lam[c];
-  for (int s = 0; s < kNumPsi; ++s) a.lpsi[s] = a.g.psi[s];
+  a.nx = d.nx;
+  a.ny = d.ny;
+  a.nz = d.nz;
+  a.p = d.boundary == YEE_BOUNDARY_CPML ? d.npml : 0;
+  a.ch = d.ch;
+  a.sdt = d.sponge_dt;
+  a.seps0 = d.sponge_eps0;
+  a.ysl = (unsigned)psi_slab_floats(d, 0);
+  a.zsl = (unsigned)psi_slab_floats(d, 1);
+  a.xsl = (unsigned)psi_slab_floats(d, 2);
+  a.cb = d.cb;
+  a.ca = d.ca;
+  a.sponge = d.boundary == YEE_BOUNDARY_SPONGE ? d.sponge_sigma : nullptr;
+  a.cpml = d.boundary == YEE_BOUNDARY_CPML ? d.cpml : nullptr;
+  a.lpsi = lam_psi;
   if (box && box->nx > 0) {
     if (box->x0 < 0 || box->y0 < 0 || box->z0 < 0 || box->ny <= 0 || box->nz <= 0 ||
         box->x0 + box->nx > p->d.nx || box->y0 + box->ny > p->d.ny || box->z0 + box->nz > p->d.nz)
@@ -365,26 +422,38 @@ int adjoint_args(const YeePlan* p, float* const* lam, float* lam_psi, const YeeD
   return YEE_OK;
 }
 
-int adjoint_launch(const YeePlan* p, const AdjArgs& a, hipStream_t st) {
-  const dim3 grid((p->d.nx + kBX - 1) / kBX, (p->d.ny + kBY - 1) / kBY, p->d.nz);
-  switch (p->d.boundary) {
-    case YEE_BOUNDARY_PERIODIC: yee_adj_h_kernel<kPeriodic><<<grid, kThreads, 0, st>>>(a); break;
-    case YEE_BOUNDARY_CPML: yee_adj_h_kernel<kCpml><<<grid, kThreads, 0, st>>>(a); break;
-    case YEE_BOUNDARY_SPONGE: yee_adj_h_kernel<kSponge><<<grid, kThreads, 0, st>>>(a); break;
-    default: yee_adj_h_kernel<kWalls><<<grid, kThreads, 0, st>>>(a); break;
-  }
-  YEE_HIP_CHECK(hipGetLastError());
+// Both kernels of one adjoint step, each as two launches under CPML: tiles
+// with no slab within one cell (no psi code) and the slab shell.
+template <int MODE, bool LOSSY>
+void adjoint_launch_t(const YeePlan* p, const AdjArgs& a, float* const* lam, hipStream_t st) {
+  TileSet in, sh;
+  split_tiles(a.nx, a.ny, a.nz, a.p, p->kz, 1, 1, 1, 1, p->xcd != 0, in, sh);
+  if (in.ntiles > 0)
+    yee_adj_h_kernel<MODE, false><<<tileset_blocks(in), kThreads, 0, st>>>(
+        a, in, lam[0], lam[1], lam[2], lam[3], lam[4], lam[5]);
+  if (sh.ntiles > 0)
+    yee_adj_h_kernel<MODE, MODE == kCpml><<<tileset_blocks(sh), kThreads, 0, st>>>(
+        a, sh, lam[0], lam[1], lam[2], lam[3], lam[4], lam[5]);
+  if (in.ntiles > 0)
+    yee_adj_e_kernel<MODE, LOSSY, false><<<tileset_blocks(in), kThreads, 0, st>>>(
+        a, in, lam[0], lam[1], lam[2], lam[3], lam[4], lam[5]);
+  if (sh.ntiles > 0)
+    yee_adj_e_kernel<MODE, LOSSY, MODE == kCpml><<<tileset_blocks(sh), kThreads, 0, st>>>(
+        a, sh, lam[0], lam[1], lam[2], lam[3], lam[4], lam[5]);
+}
+
+int adjoint_launch(const YeePlan* p, const AdjArgs& a, float* const* lam, hipStream_t st) {
   const bool lossy = p->d.ca != nullptr;
-#define YEE_ADJ_E(M)                                                         \
-  if (lossy) yee_adj_e_kernel<M, true><<<grid, kThreads, 0, st>>>(a);       \
-  else yee_adj_e_kernel<M, false><<<grid, kThreads, 0, st>>>(a);
+#define YEE_ADJ(M)                                          \
+  if (lossy) adjoint_launch_t<M, true>(p, a, lam, st);     \
+  else adjoint_launch_t<M, false>(p, a, lam, st);
   switch (p->d.boundary) {
-    case YEE_BOUNDARY_PERIODIC: YEE_ADJ_E(kPeriodic) break;
-    case YEE_BOUNDARY_CPML: YEE_ADJ_E(kCpml) break;
-    case YEE_BOUNDARY_SPONGE: YEE_ADJ_E(kSponge) break;
-    default: YEE_ADJ_E(kWalls) break;
+    case YEE_BOUNDARY_PERIODIC: YEE_ADJ(kPeriodic) break;
+    case YEE_BOUNDARY_CPML: YEE_ADJ(kCpml) break;
+    case YEE_BOUNDARY_SPONGE: YEE_ADJ(kSponge) break;
+    default: YEE_ADJ(kWalls) break;
   }
-#undef YEE_ADJ_E
+#undef YEE_ADJ
   YEE_HIP_CHECK(hipGetLastError());
   return YEE_OK;
 }
@@ -414,7 +483,7 @@ int yee_adjoint_step(YeePlan* p, float* const* lam, float* lam_psi, const YeeDes
   AdjArgs a;
   int rc = adjoint_args(p, lam, lam_psi, box, a);
   if (rc) return rc;
-  return adjoint_launch(p, a, (hipStream_t)stream);
+  return adjoint_launch(p, a, lam, (hipStream_t)stream);
 }
 
 int yee_probe_gather(float* const* fields, const int32_t* comp, const int64_t* idx, int32_t n,
@@ -527,7 +596,7 @@ int yee_adjoint_run(YeePlan* p, float* const* lam, float* lam_psi, int32_t n_lo,
     AdjArgs a;
     rc = adjoint_args(p, lam, lam_psi, box ? &b : nullptr, a);
     if (rc) return rc;
-    rc = adjoint_launch(p, a, st);
+    rc = adjoint_launch(p, a, lam, st);
     if (rc) return rc;
     if (n >= 1 && nprobe > 0) {
       probe_scatter_kernel<<<(nprobe + 255) / 256, 256, 0, st>>>(

@@ -744,8 +744,8 @@ __device__ __forceinline__ void v3_plane(
 // SHELL == false with CPML: the tiles of `ts` lie outside every slab (no
 // psi code is compiled in); SHELL == true: any tiles, slab work chosen per
 // plane.
-template <int MODE, bool LOSSY, bool SHELL, int VAR>
-__global__ __launch_bounds__(256) void yee_fused_v2_kernel(
+template <int MODE, bool LOSSY, bool SHELL, int VAR, int ROWS>
+__global__ __launch_bounds__(64 * ROWS) void yee_fused_v2_kernel(
     FusedArgs a, TileSet ts, const float* __restrict__ fi0,
     const float* __restrict__ fi1, const float* __restrict__ fi2, const float* __restrict__ fi3,
     const float* __restrict__ fi4, const float* __restrict__ fi5, float* __restrict__ fo0,
@@ -757,7 +757,7 @@ __global__ __launch_bounds__(256) void yee_fused_v2_kernel(
   constexpr bool per = MODE == kPeriodic;
   const int i = bx * 64 + (threadIdx.x & 63);
   // one y row per wavefront: j and everything derived from it is scalar
-  const int j = __builtin_amdgcn_readfirstlane(by * 4 + (int)(threadIdx.x >> 6));
+  const int j = __builtin_amdgcn_readfirstlane(by * ROWS + (int)(threadIdx.x >> 6));
   if (i >= nx || j >= ny) return;
   V2Col q;
   q.i = i;
@@ -786,7 +786,7 @@ __global__ __launch_bounds__(256) void yee_fused_v2_kernel(
   int slxy = 0;
   if (MODE == kCpml) {
     const int i0 = bx * 64, il = min(i0 + 63, nx - 1);
-    const int j0 = by * 4, jl = min(j0 + 3, ny - 1);
+    const int j0 = by * ROWS, jl = min(j0 + ROWS - 1, ny - 1);
     if (i0 <= p || il >= nx - p) slxy |= 1;
     if (j0 <= p || jl >= ny - p) slxy |= 2;
   }
@@ -849,49 +849,34 @@ int fused_kz(const YeePlan* p, int /*gx*/, int /*gy*/) {
 // their x-1 / y-1 neighbours, planes p .. nz-p-1) form one box run by the
 // lean SHELL == false kernel; the rest, as up to six boxes, by the SHELL
 // kernel.  Both read buffer A and write disjoint cells of buffer B.
-template <int MODE, bool LOSSY, int VAR>
-void launch_v2(const FusedArgs& a, const YeePlan* pl, hipStream_t st) {
-  const int nx = a.nx, ny = a.ny, nz = a.nz, p = a.p;
-  const int gx = (nx + 63) / 64, gy = (ny + 3) / 4;
+template <int MODE, bool LOSSY, int VAR, int ROWS>
+void launch_v2_rows(const FusedArgs& a, const YeePlan* pl, hipStream_t st) {
+  const int gx = (a.nx + 63) / 64, gy = (a.ny + ROWS - 1) / ROWS;
   TileSet in, sh;
-  std::memset(&in, 0, sizeof(in));
-  std::memset(&sh, 0, sizeof(sh));
-  in.kz = sh.kz = fused_kz(pl, gx, gy);
-  bool split = false;
-  int bxl = gx, bxh = 0, byl = gy, byh = 0;
-  if (MODE == kCpml) {
-    for (int b = 0; b < gx; ++b) {
-      const int i0 = 64 * b, il = std::min(i0 + 63, nx - 1);
-      if (i0 > p && il < nx - p) { bxl = std::min(bxl, b); bxh = std::max(bxh, b + 1); }
-    }
-    for (int b = 0; b < gy; ++b) {
-      const int j0 = 4 * b, jl = std::min(j0 + 3, ny - 1);
-      if (j0 > p && jl < ny - p) { byl = std::min(byl, b); byh = std::max(byh, b + 1); }
-    }
-    split = bxl < bxh && byl < byh && p < nz - p;
-  }
-  if (split) {
-    tileset_add(in, bxl, bxh, byl, byh, p, nz - p);
-    tileset_add(sh, 0, gx, 0, gy, 0, p);
-    tileset_add(sh, 0, gx, 0, gy, nz - p, nz);
-    tileset_add(sh, 0, gx, 0, byl, p, nz - p);
-    tileset_add(sh, 0, gx, byh, gy, p, nz - p);
-    tileset_add(sh, 0, bxl, byl, byh, p, nz - p);
-    tileset_add(sh, bxh, gx, byl, byh, p, nz - p);
-  } else {
-    tileset_add(MODE == kCpml ? sh : in, 0, gx, 0, gy, 0, nz);
-  }
-  tileset_finish(in, pl->xcd);
-  tileset_finish(sh, pl->xcd);
+  // a tile reads the x-1 / y-1 neighbours of its cells; planes p .. nz-p-1
+  // (the plane below the first is recomputed with full CPML handling)
+  split_tiles(a.nx, a.ny, a.nz, MODE == kCpml ? a.p : 0, fused_kz(pl, gx, gy), 1, 0, 0, 0,
+              pl->xcd == 1, in, sh, ROWS);
   if (in.ntiles > 0)
-    yee_fused_v2_kernel<MODE, LOSSY, false, VAR><<<tileset_blocks(in), 256, 0, st>>>(
+    yee_fused_v2_kernel<MODE, LOSSY, false, VAR, ROWS><<<tileset_blocks(in), 64 * ROWS, 0, st>>>(
         a, in, a.fi[0], a.fi[1], a.fi[2], a.fi[3], a.fi[4], a.fi[5], a.fo[0], a.fo[1], a.fo[2],
         a.fo[3], a.fo[4], a.fo[5]);
   if constexpr (MODE == kCpml)
     if (sh.ntiles > 0)
-      yee_fused_v2_kernel<MODE, LOSSY, true, VAR><<<tileset_blocks(sh), 256, 0, st>>>(
+      yee_fused_v2_kernel<MODE, LOSSY, true, VAR, ROWS><<<tileset_blocks(sh), 64 * ROWS, 0, st>>>(
         a, sh, a.fi[0], a.fi[1], a.fi[2], a.fi[3], a.fi[4], a.fi[5], a.fo[0], a.fo[1], a.fo[2],
         a.fo[3], a.fo[4], a.fo[5]);
+}
+
+// tile order: plain unless XCD-aware is asked for (measured faster for the
+// fused kernels on the 512^3 grid)
+template <int MODE, bool LOSSY, int VAR>
+void launch_v2(const FusedArgs& a, const YeePlan* pl, hipStream_t st) {
+  if constexpr (VAR == 3) {
+    if (pl->fused_rows == 16) return launch_v2_rows<MODE, LOSSY, VAR, 16>(a, pl, st);
+    if (pl->fused_rows == 8) return launch_v2_rows<MODE, LOSSY, VAR, 8>(a, pl, st);
+  }
+  launch_v2_rows<MODE, LOSSY, VAR, 4>(a, pl, st);
 }
 
 template <int MODE, bool LOSSY>

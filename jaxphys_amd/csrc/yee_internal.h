@@ -139,6 +139,46 @@ inline unsigned tileset_blocks(const TileSet& ts) {
   return ts.per > 0 ? (unsigned)(ts.per * kXcds) : (unsigned)ts.ntiles;
 }
 
+// Split the tiles of a grid (64 x `rows` columns by kz planes) into `in`, the
+// tiles whose cells widened by `lo` cells towards lower and `hi` cells
+// towards higher indices (x and y; z: zlo / zhi) meet no CPML slab of
+// thickness p, as one box; and `sh`, every other tile, as up to six boxes.
+// p == 0 puts the whole grid in `in`.
+inline void split_tiles(int nx, int ny, int nz, int p, int kz, int lo, int hi, int zlo, int zhi,
+                        bool xcd, TileSet& in, TileSet& sh, int rows = 4) {
+  std::memset(&in, 0, sizeof(in));
+  std::memset(&sh, 0, sizeof(sh));
+  in.kz = sh.kz = kz;
+  const int gx = (nx + 63) / 64, gy = (ny + rows - 1) / rows;
+  if (p <= 0) {
+    tileset_add(in, 0, gx, 0, gy, 0, nz);
+  } else {
+    int bxl = gx, bxh = 0, byl = gy, byh = 0;
+    for (int b = 0; b < gx; ++b) {
+      const int i0 = 64 * b, il = std::min(i0 + 63, nx - 1);
+      if (i0 - lo >= p && il + hi < nx - p) { bxl = std::min(bxl, b); bxh = std::max(bxh, b + 1); }
+    }
+    for (int b = 0; b < gy; ++b) {
+      const int j0 = rows * b, jl = std::min(j0 + rows - 1, ny - 1);
+      if (j0 - lo >= p && jl + hi < ny - p) { byl = std::min(byl, b); byh = std::max(byh, b + 1); }
+    }
+    const int z0 = p + zlo, z1 = nz - p - zhi;
+    if (bxl < bxh && byl < byh && z0 < z1) {
+      tileset_add(in, bxl, bxh, byl, byh, z0, z1);
+      tileset_add(sh, 0, gx, 0, gy, 0, z0);
+      tileset_add(sh, 0, gx, 0, gy, z1, nz);
+      tileset_add(sh, 0, gx, 0, byl, z0, z1);
+      tileset_add(sh, 0, gx, byh, gy, z0, z1);
+      tileset_add(sh, 0, bxl, byl, byh, z0, z1);
+      tileset_add(sh, bxh, gx, byl, byh, z0, z1);
+    } else {
+      tileset_add(sh, 0, gx, 0, gy, 0, nz);
+    }
+  }
+  tileset_finish(in, xcd);
+  tileset_finish(sh, xcd);
+}
+
 // (bx, by, k0, k1) of this workgroup's tile in the set; false for the
 // padding workgroups of the XCD-aware order (same map as Tiling)
 __device__ __forceinline__ bool tile_of_set(const TileSet& ts, int& bx, int& by, int& k0,
@@ -182,7 +222,8 @@ struct YeePlan {
   int kz;            // z planes per block of the two-pass kernels
   int fused_kz = 0;  // z planes per block of the fused kernel (0: automatic)
   int fused_variant = 3;  // 0: LDS + halo row, 1-3: barrier-free (see yee_fused.hip)
-  bool xcd = true;   // XCD-aware tile order (see Tiling)
+  int fused_rows = 4;     // y rows (wavefronts) per tile of variants 2 and 3: 4, 8 or 16
+  int xcd = -1;      // tile order: 1 XCD-aware (see Tiling), 0 plain, -1 automatic
   // sources
   int nsrc = 0;
   int nsteps_table = 0;

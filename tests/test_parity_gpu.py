@@ -225,13 +225,14 @@ def test_copy_probe(gpu):
     assert torch.equal(src, dst)
 
 
-@pytest.mark.parametrize("variant,xcd", [(0, True), (1, True), (2, True), (3, True), (3, False)])
+@pytest.mark.parametrize("variant,xcd,rows", [(0, True, 4), (1, True, 4), (2, True, 4),
+                                               (3, None, 4), (3, False, 8), (3, True, 16)])
 @pytest.mark.parametrize("boundary,lossy,size", [
     ("absorbing", False, (70, 37, 29)), ("periodic", False, (70, 37, 29)),
     ("walls", True, (70, 37, 29)), ("cpml", False, (70, 37, 29)), ("cpml", True, (70, 37, 29)),
     # wide enough for a slab-free CPML interior (split launches of variant 2)
     ("cpml", True, (200, 41, 30)), ("cpml", False, (140, 40, 33))])
-def test_fused_step_bitwise_equals_two_pass(gpu, boundary, lossy, size, variant, xcd):
+def test_fused_step_bitwise_equals_two_pass(gpu, boundary, lossy, size, variant, xcd, rows):
     """One fused step vs the H and E half-step kernels: the same arithmetic
     in the same order, so every field must agree bit for bit."""
     rng = np.random.default_rng(21)
@@ -252,6 +253,7 @@ def test_fused_step_bitwise_equals_two_pass(gpu, boundary, lossy, size, variant,
     a = g.build_solver(dt, n)
     b = g.build_solver(dt, n)
     a.plan.set_variant(variant)
+    a.plan.set_fused_rows(rows)
     a.plan.set_tile_order(xcd)
     b.plan.set_tile_order(xcd)
     a.plan.tune(kz=0, fused_kz=7)   # several z chunks per column
