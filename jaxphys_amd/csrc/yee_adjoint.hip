@@ -228,12 +228,12 @@ __global__ __launch_bounds__(kThreads) void yee_adj_h_kernel(
     AdjArgs a, TileSet ts, const float* __restrict__ lex, const float* __restrict__ ley,
     const float* __restrict__ lez, float* __restrict__ lhx, float* __restrict__ lhy,
     float* __restrict__ lhz) {
-  int bx, by, k0, k1;
-  if (!tile_of_set(ts, bx, by, k0, k1)) return;
+  int ti0, xl, tj0, yl, k0, k1;
+  if (!tile_of_set(ts, ti0, xl, tj0, yl, k0, k1)) return;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
-  const int i = bx * kBX + (threadIdx.x % kBX);
-  const int j = by * kBY + (threadIdx.x / kBX);
-  if (i >= nx || j >= ny) return;
+  const int i = ti0 + (int)(threadIdx.x % kBX);
+  const int j = tj0 + (int)(threadIdx.x / kBX);
+  if (i >= xl || j >= yl) return;
   bool vx, vy;
   const int ip = nb_idx<MODE>(i + 1, nx, vx);
   const int jp = nb_idx<MODE>(j + 1, ny, vy);
@@ -281,12 +281,12 @@ __global__ __launch_bounds__(kThreads) void yee_adj_e_kernel(
     AdjArgs a, TileSet ts, float* __restrict__ lex, float* __restrict__ ley,
     float* __restrict__ lez, const float* __restrict__ lhx, const float* __restrict__ lhy,
     const float* __restrict__ lhz) {
-  int bx, by, k0, k1;
-  if (!tile_of_set(ts, bx, by, k0, k1)) return;
+  int ti0, xl, tj0, yl, k0, k1;
+  if (!tile_of_set(ts, ti0, xl, tj0, yl, k0, k1)) return;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
-  const int i = bx * kBX + (threadIdx.x % kBX);
-  const int j = by * kBY + (threadIdx.x / kBX);
-  if (i >= nx || j >= ny) return;
+  const int i = ti0 + (int)(threadIdx.x % kBX);
+  const int j = tj0 + (int)(threadIdx.x / kBX);
+  if (i >= xl || j >= yl) return;
   bool vx, vy;
   const int im = nb_idx<MODE>(i - 1, nx, vx);
   const int jm = nb_idx<MODE>(j - 1, ny, vy);

@@ -751,14 +751,14 @@ __global__ __launch_bounds__(64 * ROWS) void yee_fused_v2_kernel(
     const float* __restrict__ fi4, const float* __restrict__ fi5, float* __restrict__ fo0,
     float* __restrict__ fo1, float* __restrict__ fo2, float* __restrict__ fo3,
     float* __restrict__ fo4, float* __restrict__ fo5) {
-  int bx, by, k0, k1;
-  if (!tile_of_set(ts, bx, by, k0, k1)) return;
+  int ti0, xl, tj0, yl, k0, k1;
+  if (!tile_of_set(ts, ti0, xl, tj0, yl, k0, k1)) return;
   const int nx = a.nx, ny = a.ny, nz = a.nz, p = a.p;
   constexpr bool per = MODE == kPeriodic;
-  const int i = bx * 64 + (threadIdx.x & 63);
+  const int i = ti0 + (int)(threadIdx.x & 63);
   // one y row per wavefront: j and everything derived from it is scalar
-  const int j = __builtin_amdgcn_readfirstlane(by * ROWS + (int)(threadIdx.x >> 6));
-  if (i >= nx || j >= ny) return;
+  const int j = __builtin_amdgcn_readfirstlane(tj0 + (int)(threadIdx.x >> 6));
+  if (i >= xl || j >= yl) return;
   V2Col q;
   q.i = i;
   q.j = j;
@@ -785,10 +785,10 @@ __global__ __launch_bounds__(64 * ROWS) void yee_fused_v2_kernel(
   // neighbours)
   int slxy = 0;
   if (MODE == kCpml) {
-    const int i0 = bx * 64, il = min(i0 + 63, nx - 1);
-    const int j0 = by * ROWS, jl = min(j0 + ROWS - 1, ny - 1);
-    if (i0 <= p || il >= nx - p) slxy |= 1;
-    if (j0 <= p || jl >= ny - p) slxy |= 2;
+    const int il = min(ti0 + 63, xl - 1);
+    const int jl = min(tj0 + ROWS - 1, yl - 1);
+    if (ti0 <= p || il >= nx - p) slxy |= 1;
+    if (tj0 <= p || jl >= ny - p) slxy |= 2;
   }
   const unsigned P = (unsigned)nx * ny;
   unsigned c = (unsigned)k0 * P + (unsigned)j * nx + i;
@@ -809,7 +809,8 @@ __global__ __launch_bounds__(64 * ROWS) void yee_fused_v2_kernel(
   }
   const int lane = threadIdx.x & 63;
   const bool lane0 = lane == 0;
-  const bool edge_r = lane == 63 || i == nx - 1;
+  // the last active lane of the wave: column x+1 belongs to another tile
+  const bool edge_r = lane == 63 || i == xl - 1;
 #define YEE_V2_PLANE(S)                                                                       \
   if constexpr (VAR == 3)                                                                     \
     v3_plane<MODE, LOSSY, S>(a, fi0, fi1, fi2, fi3, fi4, fi5, fo0, fo1, fo2, fo3, fo4, fo5, q, \

@@ -104,17 +104,19 @@ inline unsigned tiling_blocks(const Tiling& t) {
 }
 
 
-// A set of tiles given as up to kMaxBoxes boxes (x, y in tile units, z in
-// planes, chunked by kz); `first` is the set-wide index of a box's first
-// tile.  Used to launch the CPML interior and its slab shell separately.
+// A set of tiles given as up to kMaxBoxes boxes of cells [x0, x1) x
+// [y0, y1) x [z0, z1), each cut into tiles of 64 columns x `rows` rows x kz
+// planes (edge tiles partial); `first` is the set-wide index of a box's
+// first tile.  Used to launch the CPML interior and its slab shell
+// separately.
 constexpr int kMaxBoxes = 6;
 
 struct TileBox {
-  int x0, nx, y0, ny, z0, z1, first;
+  int x0, x1, nxt, y0, y1, nyt, z0, z1, first;
 };
 
 struct TileSet {
-  int nbox, ntiles, per, kz;
+  int nbox, ntiles, per, kz, rows;
   TileBox b[kMaxBoxes];
 };
 
@@ -122,13 +124,15 @@ inline void tileset_add(TileSet& ts, int x0, int x1, int y0, int y1, int z0, int
   if (x1 <= x0 || y1 <= y0 || z1 <= z0) return;
   TileBox& B = ts.b[ts.nbox++];
   B.x0 = x0;
-  B.nx = x1 - x0;
+  B.x1 = x1;
+  B.nxt = (x1 - x0 + 63) / 64;
   B.y0 = y0;
-  B.ny = y1 - y0;
+  B.y1 = y1;
+  B.nyt = (y1 - y0 + ts.rows - 1) / ts.rows;
   B.z0 = z0;
   B.z1 = z1;
   B.first = ts.ntiles;
-  ts.ntiles += B.nx * B.ny * ((z1 - z0 + ts.kz - 1) / ts.kz);
+  ts.ntiles += B.nxt * B.nyt * ((z1 - z0 + ts.kz - 1) / ts.kz);
 }
 
 inline void tileset_finish(TileSet& ts, bool xcd) {
@@ -139,50 +143,44 @@ inline unsigned tileset_blocks(const TileSet& ts) {
   return ts.per > 0 ? (unsigned)(ts.per * kXcds) : (unsigned)ts.ntiles;
 }
 
-// Split the tiles of a grid (64 x `rows` columns by kz planes) into `in`, the
-// tiles whose cells widened by `lo` cells towards lower and `hi` cells
-// towards higher indices (x and y; z: zlo / zhi) meet no CPML slab of
-// thickness p, as one box; and `sh`, every other tile, as up to six boxes.
-// p == 0 puts the whole grid in `in`.
+// Split a grid into `in`, the cells whose neighbourhood -- `lo` cells
+// towards lower and `hi` towards higher x and y indices, zlo / zhi along
+// z -- meets no CPML slab of thickness p, as one box whose x range is
+// aligned to 32 columns (128-byte rows); and `sh`, every other cell, as up
+// to six boxes.  p == 0 puts the whole grid in `in`.
 inline void split_tiles(int nx, int ny, int nz, int p, int kz, int lo, int hi, int zlo, int zhi,
                         bool xcd, TileSet& in, TileSet& sh, int rows = 4) {
   std::memset(&in, 0, sizeof(in));
   std::memset(&sh, 0, sizeof(sh));
   in.kz = sh.kz = kz;
-  const int gx = (nx + 63) / 64, gy = (ny + rows - 1) / rows;
+  in.rows = sh.rows = rows;
   if (p <= 0) {
-    tileset_add(in, 0, gx, 0, gy, 0, nz);
+    tileset_add(in, 0, nx, 0, ny, 0, nz);
   } else {
-    int bxl = gx, bxh = 0, byl = gy, byh = 0;
-    for (int b = 0; b < gx; ++b) {
-      const int i0 = 64 * b, il = std::min(i0 + 63, nx - 1);
-      if (i0 - lo >= p && il + hi < nx - p) { bxl = std::min(bxl, b); bxh = std::max(bxh, b + 1); }
-    }
-    for (int b = 0; b < gy; ++b) {
-      const int j0 = rows * b, jl = std::min(j0 + rows - 1, ny - 1);
-      if (j0 - lo >= p && jl + hi < ny - p) { byl = std::min(byl, b); byh = std::max(byh, b + 1); }
-    }
-    const int z0 = p + zlo, z1 = nz - p - zhi;
-    if (bxl < bxh && byl < byh && z0 < z1) {
-      tileset_add(in, bxl, bxh, byl, byh, z0, z1);
-      tileset_add(sh, 0, gx, 0, gy, 0, z0);
-      tileset_add(sh, 0, gx, 0, gy, z1, nz);
-      tileset_add(sh, 0, gx, 0, byl, z0, z1);
-      tileset_add(sh, 0, gx, byh, gy, z0, z1);
-      tileset_add(sh, 0, bxl, byl, byh, z0, z1);
-      tileset_add(sh, bxh, gx, byl, byh, z0, z1);
+    const int xa = (p + lo + 31) / 32 * 32, xb = (nx - p - hi) / 32 * 32;
+    const int ya = p + lo, yb = ny - p - hi;
+    const int za = p + zlo, zb = nz - p - zhi;
+    if (xa < xb && ya < yb && za < zb) {
+      tileset_add(in, xa, xb, ya, yb, za, zb);
+      tileset_add(sh, 0, nx, 0, ny, 0, za);
+      tileset_add(sh, 0, nx, 0, ny, zb, nz);
+      tileset_add(sh, 0, nx, 0, ya, za, zb);
+      tileset_add(sh, 0, nx, yb, ny, za, zb);
+      tileset_add(sh, 0, xa, ya, yb, za, zb);
+      tileset_add(sh, xb, nx, ya, yb, za, zb);
     } else {
-      tileset_add(sh, 0, gx, 0, gy, 0, nz);
+      tileset_add(sh, 0, nx, 0, ny, 0, nz);
     }
   }
   tileset_finish(in, xcd);
   tileset_finish(sh, xcd);
 }
 
-// (bx, by, k0, k1) of this workgroup's tile in the set; false for the
-// padding workgroups of the XCD-aware order (same map as Tiling)
-__device__ __forceinline__ bool tile_of_set(const TileSet& ts, int& bx, int& by, int& k0,
-                                            int& k1) {
+// First column / row / plane of this workgroup's tile in the set and the
+// exclusive limits of its box (x, y) and of the tile (k1); false for the
+// padding workgroups of the XCD-aware order (same map as Tiling).
+__device__ __forceinline__ bool tile_of_set(const TileSet& ts, int& i0, int& xl, int& j0,
+                                            int& yl, int& k0, int& k1) {
   const int L = blockIdx.x;
   const int t = ts.per > 0 ? (L % kXcds) * ts.per + L / kXcds : L;
   if (t >= ts.ntiles) return false;
@@ -191,10 +189,12 @@ __device__ __forceinline__ bool tile_of_set(const TileSet& ts, int& bx, int& by,
   for (int q = 1; q < kMaxBoxes; ++q)
     if (q < ts.nbox && t >= ts.b[q].first) B = ts.b[q];
   const int lt = t - B.first;
-  bx = B.x0 + lt % B.nx;
-  const int r = lt / B.nx;
-  by = B.y0 + r % B.ny;
-  k0 = B.z0 + (r / B.ny) * ts.kz;
+  i0 = B.x0 + (lt % B.nxt) * 64;
+  xl = B.x1;
+  const int r = lt / B.nxt;
+  j0 = B.y0 + (r % B.nyt) * ts.rows;
+  yl = B.y1;
+  k0 = B.z0 + (r / B.nyt) * ts.kz;
   k1 = min(k0 + ts.kz, B.z1);
   return true;
 }

@@ -41,13 +41,16 @@ def _gpu_grid(size, boundary, npml, eps, sigma, tab, src):
     return g
 
 
-@pytest.mark.parametrize("boundary,lossy", [("cpml", False), ("cpml", True),
-                                            ("absorbing", False), ("walls", False)])
-def test_adjoint_matches_autograd_oracle(gpu, boundary, lossy):
-    size, npml, n = (22, 20, 18), 4, 70
+@pytest.mark.parametrize("boundary,lossy,size", [
+    ("cpml", False, (22, 20, 18)), ("cpml", True, (22, 20, 18)),
+    ("absorbing", False, (22, 20, 18)), ("walls", False, (22, 20, 18)),
+    # wide enough for the slab-free interior launches
+    ("cpml", True, (76, 26, 22))])
+def test_adjoint_matches_autograd_oracle(gpu, boundary, lossy, size):
+    npml, n = 4, 70
     eps, sigma, box, tab, src, mon = _setup(boundary, lossy, size, npml, n)
     x0, y0, z0, bx, by, bz = box
-    print(f"[adjoint] {boundary} lossy={lossy}", flush=True)
+    print(f"[adjoint] {boundary} lossy={lossy} size={size}", flush=True)
     g = _gpu_grid(size, boundary, npml, eps, sigma, tab, src)
     prob = AdjointProblem(g, n, DT, box, [("ez", p) for _, p in mon], checkpoint_every=16)
     eps_box = torch.tensor(eps[x0:x0 + bx, y0:y0 + by, z0:z0 + bz], device=gpu)
