@@ -77,6 +77,25 @@ def cpu_baseline(solver, budget_s: float):
                       f"workload in {secs:.1f} s (C/OpenMP oracle, {cpu_model})"}
 
 
+def measured_traffic(workload: str, mode: str):
+    """HBM bytes per step of this workload and kernel design, measured with
+    rocprofv3 PMC counters (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by
+    the same counters on a plain copy kernel) and committed as
+    profiles/traffic.json; None when no measurement matches."""
+    import json as _json
+
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as fh:
+            rows = _json.load(fh)
+    except (OSError, ValueError):
+        return None, None
+    for r in rows:
+        if r.get("workload") == workload and r.get("mode") == mode:
+            return r.get("bytes_per_step"), r
+    return None, None
+
+
 def adjoint_bench(args, dev):
     """160^3, 12-cell CPML, 800 steps; 96^3 design region eps_r U[1,4] in
     vacuum; z dipole 20 cells below it; J = time-integrated Ez^2 on a 32x32
@@ -301,6 +320,7 @@ def main() -> None:
         dom_name, dom_bytes, dom_ms = ("yee_e_kernel", e_bytes, e_ms) if e_ms >= h_ms else \
             ("yee_h_kernel", h_bytes, h_ms)
         step_bytes = h_bytes + e_bytes
+    traffic, traffic_rec = measured_traffic(args.workload, args.mode)
     ms_step = total_ms / K
     gcells = cells * K / (total_ms / 1e3) / 1e9
     achieved = dom_bytes / (dom_ms / 1e3) / 1e9
@@ -342,7 +362,8 @@ def main() -> None:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": None if traffic_rec is None else traffic_rec.get("source"),
             "bytes_per_launch": dom_bytes,
             "ms_per_launch": round(dom_ms, 4),
         },

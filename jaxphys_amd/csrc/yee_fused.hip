@@ -741,6 +741,259 @@ __device__ __forceinline__ void v3_plane(
   r.hym = hy;
 }
 
+// ---------------------------------------------------------------------------
+// Variant 3, CPML slab shell.  Same arithmetic as v3_plane with every psi
+// term, but the y and z psi work sits behind wave-uniform branches (j is
+// one value per wavefront, k one per plane), the x table values are loaded
+// once per tile, and only the
+// x terms are selected per lane (their psi loads are issued with the field
+// loads).
+// ---------------------------------------------------------------------------
+
+// CPML x coefficients of a column (per lane, loaded once per tile)
+struct V3Tab {
+  float xbE, xcE, xkE, xbH, xcH, xkH;
+};
+
+// psi' = b psi + c d (returned in ps) and the curl term d / kappa + psi'
+__device__ __forceinline__ float pterm(float b, float c, float kinv, float psi, float d,
+                                       float& ps) {
+  ps = b * psi + c * d;
+  return d * kinv + ps;
+}
+
+template <int MODE, bool LOSSY, bool XS>
+__device__ __forceinline__ void v3s_plane(
+    const FusedArgs& a, const float* __restrict__ Ex, const float* __restrict__ Ey,
+    const float* __restrict__ Ez, const float* __restrict__ Hx, const float* __restrict__ Hy,
+    const float* __restrict__ Hz, float* __restrict__ oEx, float* __restrict__ oEy,
+    float* __restrict__ oEz, float* __restrict__ oHx, float* __restrict__ oHy,
+    float* __restrict__ oHz, const V2Col& q, const V3Tab& t, bool lane0, bool edge_r, int k,
+    unsigned c, V2Carry& r) {
+  const int nx = a.nx, ny = a.ny, nz = a.nz, p = a.p;
+  const unsigned P = (unsigned)nx * ny;
+  const bool zinH = k < nz - 1;
+  const bool zinE = k > 0;
+  const unsigned zp = zinH ? P : 0u;
+  const float ch = a.ch;
+
+  // ---- field loads (as v3_plane)
+  const float ex = r.ex, ey = r.ey, eym = r.eym;
+  const float ez = Ez[c];
+  const float ex_yp = Ex[c + q.yp], ez_yp = Ez[c + q.yp];
+  const float ex_zp = Ex[c + zp], ey_zp = Ey[c + zp];
+  const float hx0 = Hx[c], hy0 = Hy[c], hz0 = Hz[c];
+  const unsigned cym = c + q.ym;
+  const float ez_ym = Ez[cym], ex_ym = Ex[cym], ey_ym_zp = Ey[cym + zp];
+  const float hx_ym0 = Hx[cym], hz_ym0 = Hz[cym];
+  const float cb = a.cb[c];
+  const float ca = LOSSY ? a.ca[c] : 1.f;
+  float ld_ey_xp = 0.f, ld_ez_xp = 0.f, ld_ey_ym_xp = 0.f;
+  if (edge_r) {
+    ld_ey_xp = Ey[c + q.xp];
+    ld_ez_xp = Ez[c + q.xp];
+    ld_ey_ym_xp = Ey[cym + q.xp];
+  }
+  const unsigned cxm = c + q.xm;
+  float ez_xm = 0.f, ey_xm = 0.f, ex_xm_yp = 0.f, ex_xm_zp = 0.f, hy_xm0 = 0.f, hz_xm0 = 0.f;
+  if (lane0) {
+    ez_xm = Ez[cxm];
+    ey_xm = Ey[cxm];
+    ex_xm_yp = Ex[cxm + q.yp];
+    ex_xm_zp = Ex[cxm + zp];
+    hy_xm0 = Hy[cxm];
+    hz_xm0 = Hz[cxm];
+  }
+
+  // ---- psi addressing: slot bases and offsets within a slot
+  const float* __restrict__ pin = a.pin;
+  float* __restrict__ pout = a.pout;
+  const unsigned ysl = (unsigned)a.ysl, zsl = (unsigned)a.zsl, xsl = (unsigned)a.xsl;
+  const unsigned bz0 = 4 * ysl, bx0 = 4 * ysl + 4 * zsl;
+  const unsigned tp = 2u * (unsigned)p;
+  const bool zs = k < p || k >= nz - p;
+  const int sz = zs ? slab_index(k, nz, p) : 0;
+  const unsigned oy = ((unsigned)k * tp + (unsigned)q.sy) * nx + q.i;
+  const unsigned oym = ((unsigned)k * tp + (unsigned)q.sym) * nx + q.i;
+  const unsigned oz = ((unsigned)sz * ny + q.j) * nx + q.i;
+  const unsigned ozm = ((unsigned)sz * ny + q.jm) * nx + q.i;
+  const unsigned ox = q.xs ? ((unsigned)k * ny + q.j) * tp + (unsigned)q.sx : 0u;
+  const unsigned oxm = q.xs ? ((unsigned)k * ny + q.jm) * tp + (unsigned)q.sx : 0u;
+
+  // ---- x psi loads (per lane, clamped); y and z psi are loaded inside
+  // their wave-uniform branches
+  const float* ty = a.cpml + 6 * nx;
+  const float* tz = a.cpml + 6 * (nx + ny);
+  float pHyx = 0.f, pHzx = 0.f, pEyx = 0.f, pEzx = 0.f, pHzx_m = 0.f;
+  if (XS) {
+    pHyx = pin[bx0 + (kHyx - 8) * xsl + ox];
+    pHzx = pin[bx0 + (kHzx - 8) * xsl + ox];
+    pEyx = pin[bx0 + (kEyx - 8) * xsl + ox];
+    pEzx = pin[bx0 + (kEzx - 8) * xsl + ox];
+    pHzx_m = pin[bx0 + (kHzx - 8) * xsl + oxm];
+  }
+  // lane 0: psi and x tables of column x-1
+  float l_pHyx = 0.f, l_pHzx = 0.f;
+  float lxbH = 0.f, lxcH = 0.f, lxkH = 0.f;
+  if (lane0) {
+    if (XS && q.xsm) {
+      const unsigned oxl = ((unsigned)k * ny + q.j) * tp + (unsigned)q.sxm;
+      l_pHyx = pin[bx0 + (kHyx - 8) * xsl + oxl];
+      l_pHzx = pin[bx0 + (kHzx - 8) * xsl + oxl];
+      lxbH = a.cpml[3 * nx + q.im];
+      lxcH = a.cpml[4 * nx + q.im];
+      lxkH = a.cpml[5 * nx + q.im];
+    }
+  }
+  float ey_xp = __shfl_down(ey, 1);
+  float ez_xp = __shfl_down(ez, 1);
+  float ey_ym_xp = __shfl_down(eym, 1);
+  if (edge_r) {
+    ey_xp = ld_ey_xp;
+    ez_xp = ld_ez_xp;
+    ey_ym_xp = ld_ey_ym_xp;
+  }
+
+  // ---- H^{n+1/2} of the column (fdtd3d.py:303-330)
+  float hx, hy, hz, ps;
+  {
+    const bool upd = q.yinH && zinH;
+    float u = ez_yp - ez, v = ey_zp - ey;
+    if (q.ys) {
+      u = pterm(ty[3 * ny + q.j], ty[4 * ny + q.j], ty[5 * ny + q.j], pin[kHxy * ysl + oy], u, ps);
+      if (upd) pout[kHxy * ysl + oy] = ps;
+    }
+    if (zs) {
+      v = pterm(tz[3 * nz + k], tz[4 * nz + k], tz[5 * nz + k], pin[bz0 + (kHxz - 4) * zsl + oz], v, ps);
+      if (upd) pout[bz0 + (kHxz - 4) * zsl + oz] = ps;
+    }
+    hx = upd ? hx0 - ch * (u - v) : hx0;
+  }
+  {
+    const bool upd = q.xinH && zinH;
+    float u = ex_zp - ex, v = ez_xp - ez;
+    if (zs) {
+      u = pterm(tz[3 * nz + k], tz[4 * nz + k], tz[5 * nz + k], pin[bz0 + (kHyz - 4) * zsl + oz], u, ps);
+      if (upd) pout[bz0 + (kHyz - 4) * zsl + oz] = ps;
+    }
+    if (XS) {
+      const float w = pterm(t.xbH, t.xcH, t.xkH, pHyx, v, ps);
+      if (upd && q.xs) pout[bx0 + (kHyx - 8) * xsl + ox] = ps;
+      v = q.xs ? w : v;
+    }
+    hy = upd ? hy0 - ch * (u - v) : hy0;
+  }
+  {
+    const bool upd = q.xinH && q.yinH;
+    float u = ey_xp - ey, v = ex_yp - ex;
+    if (XS) {
+      const float w = pterm(t.xbH, t.xcH, t.xkH, pHzx, u, ps);
+      if (upd && q.xs) pout[bx0 + (kHzx - 8) * xsl + ox] = ps;
+      u = q.xs ? w : u;
+    }
+    if (q.ys) {
+      v = pterm(ty[3 * ny + q.j], ty[4 * ny + q.j], ty[5 * ny + q.j], pin[kHzy * ysl + oy], v, ps);
+      if (upd) pout[kHzy * ysl + oy] = ps;
+    }
+    hz = upd ? hz0 - ch * (u - v) : hz0;
+  }
+  // ---- Hy, Hz of column x-1: lane-1's new values; lane 0 recomputes them
+  float hy_xm = __shfl_up(hy, 1);
+  float hz_xm = __shfl_up(hz, 1);
+  if (lane0) {
+    const float exm = r.exm;
+    {
+      float u = ex_xm_zp - exm, v = ez - ez_xm;
+      if (zs) u = pterm(tz[3 * nz + k], tz[4 * nz + k], tz[5 * nz + k], pin[bz0 + (kHyz - 4) * zsl + oz + q.xm], u, ps);
+      if (XS && q.xsm) v = pterm(lxbH, lxcH, lxkH, l_pHyx, v, ps);
+      hy_xm = zinH ? hy_xm0 - ch * (u - v) : hy_xm0;
+    }
+    {
+      float u = ey - ey_xm, v = ex_xm_yp - exm;
+      if (XS && q.xsm) u = pterm(lxbH, lxcH, lxkH, l_pHzx, u, ps);
+      if (q.ys) v = pterm(ty[3 * ny + q.j], ty[4 * ny + q.j], ty[5 * ny + q.j], pin[kHzy * ysl + oy + q.xm], v, ps);
+      hz_xm = q.yinH ? hz_xm0 - ch * (u - v) : hz_xm0;
+    }
+  }
+  r.exm = ex_xm_zp;  // meaningful in lane 0 only
+  // ---- Hx, Hz at (i, j-1, k)
+  float hx_ym, hz_ym;
+  {
+    float u = ez - ez_ym, v = ey_ym_zp - eym;
+    if (q.ysm) u = pterm(ty[3 * ny + q.jm], ty[4 * ny + q.jm], ty[5 * ny + q.jm], pin[kHxy * ysl + oym], u, ps);
+    if (zs) v = pterm(tz[3 * nz + k], tz[4 * nz + k], tz[5 * nz + k], pin[bz0 + (kHxz - 4) * zsl + ozm], v, ps);
+    hx_ym = zinH ? hx_ym0 - ch * (u - v) : hx_ym0;
+  }
+  {
+    float u = ey_ym_xp - eym, v = ex - ex_ym;
+    if (XS) {
+      const float w = pterm(t.xbH, t.xcH, t.xkH, pHzx_m, u, ps);
+      u = q.xs ? w : u;
+    }
+    if (q.ysm) v = pterm(ty[3 * ny + q.jm], ty[4 * ny + q.jm], ty[5 * ny + q.jm], pin[kHzy * ysl + oym], v, ps);
+    hz_ym = q.xinH ? hz_ym0 - ch * (u - v) : hz_ym0;
+  }
+  oHx[c] = hx;
+  oHy[c] = hy;
+  oHz[c] = hz;
+
+  // ---- E^{n+1} of the column (fdtd3d.py:332-359)
+  float vx, vy, vz;
+  {
+    const bool upd = q.yinE && zinE;
+    float u = hz - hz_ym, v = hy - r.hym;
+    if (q.ys) {
+      u = pterm(ty[q.j], ty[ny + q.j], ty[2 * ny + q.j], pin[kExy * ysl + oy], u, ps);
+      if (upd) pout[kExy * ysl + oy] = ps;
+    }
+    if (zs) {
+      v = pterm(tz[k], tz[nz + k], tz[2 * nz + k], pin[bz0 + (kExz - 4) * zsl + oz], v, ps);
+      if (upd) pout[bz0 + (kExz - 4) * zsl + oz] = ps;
+    }
+    const float w = LOSSY ? ca * ex + cb * (u - v) : ex + cb * (u - v);
+    vx = upd ? w : ex;
+  }
+  {
+    const bool upd = q.xinE && zinE;
+    float u = hx - r.hxm, v = hz - hz_xm;
+    if (zs) {
+      u = pterm(tz[k], tz[nz + k], tz[2 * nz + k], pin[bz0 + (kEyz - 4) * zsl + oz], u, ps);
+      if (upd) pout[bz0 + (kEyz - 4) * zsl + oz] = ps;
+    }
+    if (XS) {
+      const float w = pterm(t.xbE, t.xcE, t.xkE, pEyx, v, ps);
+      if (upd && q.xs) pout[bx0 + (kEyx - 8) * xsl + ox] = ps;
+      v = q.xs ? w : v;
+    }
+    const float w = LOSSY ? ca * ey + cb * (u - v) : ey + cb * (u - v);
+    vy = upd ? w : ey;
+  }
+  {
+    const bool upd = q.xinE && q.yinE;
+    float u = hy - hy_xm, v = hx - hx_ym;
+    if (XS) {
+      const float w = pterm(t.xbE, t.xcE, t.xkE, pEzx, u, ps);
+      if (upd && q.xs) pout[bx0 + (kEzx - 8) * xsl + ox] = ps;
+      u = q.xs ? w : u;
+    }
+    if (q.ys) {
+      v = pterm(ty[q.j], ty[ny + q.j], ty[2 * ny + q.j], pin[kEzy * ysl + oy], v, ps);
+      if (upd) pout[kEzy * ysl + oy] = ps;
+    }
+    const float w = LOSSY ? ca * ez + cb * (u - v) : ez + cb * (u - v);
+    vz = upd ? w : ez;
+  }
+  oEx[c] = vx;
+  oEy[c] = vy;
+  oEz[c] = vz;
+
+  r.ex = ex_zp;
+  r.ey = ey_zp;
+  r.eym = ey_ym_zp;
+  r.hxm = hx;
+  r.hym = hy;
+}
+
 // SHELL == false with CPML: the tiles of `ts` lie outside every slab (no
 // psi code is compiled in); SHELL == true: any tiles, slab work chosen per
 // plane.
@@ -811,6 +1064,15 @@ __global__ __launch_bounds__(64 * ROWS) void yee_fused_v2_kernel(
   const bool lane0 = lane == 0;
   // the last active lane of the wave: column x+1 belongs to another tile
   const bool edge_r = lane == 63 || i == xl - 1;
+  V3Tab tb;
+  if constexpr (MODE == kCpml && SHELL && VAR == 3) {
+    const float* tx = a.cpml;
+    tb.xbE = tx[i]; tb.xcE = tx[nx + i]; tb.xkE = tx[2 * nx + i];
+    tb.xbH = tx[3 * nx + i]; tb.xcH = tx[4 * nx + i]; tb.xkH = tx[5 * nx + i];
+  }
+#define YEE_V3S_PLANE(XSB)                                                                     \
+  v3s_plane<MODE, LOSSY, XSB>(a, fi0, fi1, fi2, fi3, fi4, fi5, fo0, fo1, fo2, fo3, fo4, fo5, q, \
+                              tb, lane0, edge_r, k, c, r)
 #define YEE_V2_PLANE(S)                                                                       \
   if constexpr (VAR == 3)                                                                     \
     v3_plane<MODE, LOSSY, S>(a, fi0, fi1, fi2, fi3, fi4, fi5, fo0, fo1, fo2, fo3, fo4, fo5, q, \
@@ -819,7 +1081,10 @@ __global__ __launch_bounds__(64 * ROWS) void yee_fused_v2_kernel(
     v2_plane<MODE, LOSSY, S>(a, fi0, fi1, fi2, fi3, fi4, fi5, fo0, fo1, fo2, fo3, fo4, fo5, q, \
                              k, c, r)
   for (int k = k0; k < k1; ++k, c += P) {
-    if (MODE != kCpml || !SHELL) {
+    if constexpr (MODE == kCpml && SHELL && VAR == 3) {
+      if (slxy & 1) YEE_V3S_PLANE(true);
+      else YEE_V3S_PLANE(false);
+    } else if (MODE != kCpml || !SHELL) {
       YEE_V2_PLANE(0);
     } else {
       const int sl = slxy | ((k < p || k >= nz - p) ? 4 : 0);
@@ -836,6 +1101,7 @@ __global__ __launch_bounds__(64 * ROWS) void yee_fused_v2_kernel(
     }
   }
 #undef YEE_V2_PLANE
+#undef YEE_V3S_PLANE
 }
 
 constexpr int kFusedBY = 8;
