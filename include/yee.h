@@ -100,12 +100,14 @@ int yee_plan_tune(YeePlan* plan, int32_t kz, int32_t fused_kz);
  * with the x neighbours exchanged between lanes.  Same results bit for
  * bit.                                                                   */
 int yee_plan_set_variant(YeePlan* plan, int32_t variant);
-/* Workgroup -> tile order (no reference counterpart): 1 = XCD-aware (each
- * of the 8 XCDs runs a contiguous range of tiles, so neighbouring tiles
- * share an L2), 0 = plain x-fastest order, -1 (default) = automatic
- * (XCD-aware for the two-pass and adjoint kernels, plain for the fused
- * kernels: the faster of the two for each, measured on a 512^3 grid).
- * Results do not depend on it.                                           */
+/* Workgroup -> tile order (no reference counterpart): 0 = plain x-fastest
+ * order, 1 = XCD-contiguous (each of the 8 XCDs runs a contiguous range of
+ * tiles, so neighbouring tiles share an L2), 2 = XCD-banded (fused
+ * interior launch: each z-chunk band split into 8 contiguous shares, one
+ * per XCD; other launches use 1 for the two-pass and adjoint kernels, 0
+ * for the fused shell), -1 (default) = automatic (1 for the two-pass and
+ * adjoint kernels, 0 for the fused kernels).  Results do not depend on
+ * it.                                                                    */
 int yee_plan_set_tile_order(YeePlan* plan, int32_t xcd_aware);
 /* y rows (wavefronts) per workgroup tile of fused variants 2 and 3: 4
  * (default), 8 or 16 (no reference counterpart; results do not depend on

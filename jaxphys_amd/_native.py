@@ -169,9 +169,10 @@ class Plan:
     def set_variant(self, variant: int) -> None:
         check(lib().yee_plan_set_variant(self.handle, int(variant)), "yee_plan_set_variant")
 
-    def set_tile_order(self, xcd_aware) -> None:
-        """True: XCD-aware, False: plain, None: automatic (default)."""
-        v = -1 if xcd_aware is None else int(bool(xcd_aware))
+    def set_tile_order(self, order) -> None:
+        """None: automatic (default); False / 0: plain; True / 1:
+        XCD-contiguous; 2: XCD-banded."""
+        v = -1 if order is None else int(order)
         check(lib().yee_plan_set_tile_order(self.handle, v), "yee_plan_set_tile_order")
 
     def set_fused_rows(self, rows: int) -> None:

@@ -475,9 +475,12 @@ int yee_plan_create(const YeeGridDesc* desc, YeePlan** out) {
   if (const char* e = std::getenv("JAXPHYS_AMD_FUSED_KZ")) p->fused_kz = std::atoi(e);
   if (const char* e = std::getenv("JAXPHYS_AMD_FUSED_VARIANT")) {
     const int v = std::atoi(e);
-    if (v >= 0 && v <= 3) p->fused_variant = v;
+    if (v >= 0 && v <= 4) p->fused_variant = v;
   }
-  if (const char* e = std::getenv("JAXPHYS_AMD_XCD")) p->xcd = std::atoi(e) != 0;
+  if (const char* e = std::getenv("JAXPHYS_AMD_XCD")) {
+    const int v = std::atoi(e);
+    if (v >= -1 && v <= 2) p->xcd = v;
+  }
   if (const char* e = std::getenv("JAXPHYS_AMD_FUSED_ROWS")) {
     const int r = std::atoi(e);
     if (r == 4 || r == 8 || r == 16) p->fused_rows = r;
@@ -495,14 +498,15 @@ int yee_plan_tune(YeePlan* p, int32_t kz, int32_t fused_kz) {
 
 int yee_plan_set_variant(YeePlan* p, int32_t variant) {
   if (!p) return fail(YEE_EINVAL, "null plan");
-  if (variant < 0 || variant > 3) return fail(YEE_EINVAL, "unknown fused variant");
+  if (variant < 0 || variant > 4) return fail(YEE_EINVAL, "unknown fused variant");
   p->fused_variant = variant;
   return YEE_OK;
 }
 
 int yee_plan_set_tile_order(YeePlan* p, int32_t xcd_aware) {
   if (!p) return fail(YEE_EINVAL, "null plan");
-  p->xcd = xcd_aware < 0 ? -1 : (xcd_aware != 0);
+  if (xcd_aware < -1 || xcd_aware > 2) return fail(YEE_EINVAL, "tile order must be -1, 0, 1 or 2");
+  p->xcd = xcd_aware;
   return YEE_OK;
 }
 

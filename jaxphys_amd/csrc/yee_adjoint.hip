@@ -228,12 +228,13 @@ __global__ __launch_bounds__(kThreads) void yee_adj_h_kernel(
     AdjArgs a, TileSet ts, const float* __restrict__ lex, const float* __restrict__ ley,
     const float* __restrict__ lez, float* __restrict__ lhx, float* __restrict__ lhy,
     float* __restrict__ lhz) {
-  int ti0, xl, tj0, yl, k0, k1;
-  if (!tile_of_set(ts, ti0, xl, tj0, yl, k0, k1)) return;
+  TilePos tp;  // (no paired boxes in the adjoint tile sets)
+  if (!tile_of_set(ts, tp)) return;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
-  const int i = ti0 + (int)(threadIdx.x % kBX);
-  const int j = tj0 + (int)(threadIdx.x / kBX);
-  if (i >= xl || j >= yl) return;
+  const int k0 = tp.k0, k1 = tp.k1;
+  const int i = tp.i0 + (int)(threadIdx.x % kBX);
+  const int j = tp.j0 + (int)(threadIdx.x / kBX);
+  if (i >= tp.xl || j >= tp.yl) return;
   bool vx, vy;
   const int ip = nb_idx<MODE>(i + 1, nx, vx);
   const int jp = nb_idx<MODE>(j + 1, ny, vy);
@@ -281,12 +282,13 @@ __global__ __launch_bounds__(kThreads) void yee_adj_e_kernel(
     AdjArgs a, TileSet ts, float* __restrict__ lex, float* __restrict__ ley,
     float* __restrict__ lez, const float* __restrict__ lhx, const float* __restrict__ lhy,
     const float* __restrict__ lhz) {
-  int ti0, xl, tj0, yl, k0, k1;
-  if (!tile_of_set(ts, ti0, xl, tj0, yl, k0, k1)) return;
+  TilePos tp;  // (no paired boxes in the adjoint tile sets)
+  if (!tile_of_set(ts, tp)) return;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
-  const int i = ti0 + (int)(threadIdx.x % kBX);
-  const int j = tj0 + (int)(threadIdx.x / kBX);
-  if (i >= xl || j >= yl) return;
+  const int k0 = tp.k0, k1 = tp.k1;
+  const int i = tp.i0 + (int)(threadIdx.x % kBX);
+  const int j = tp.j0 + (int)(threadIdx.x / kBX);
+  if (i >= tp.xl || j >= tp.yl) return;
   bool vx, vy;
   const int im = nb_idx<MODE>(i - 1, nx, vx);
   const int jm = nb_idx<MODE>(j - 1, ny, vy);
@@ -427,7 +429,7 @@ int adjoint_args(const YeePlan* p, float* const* lam, float* lam_psi, const YeeD
 template <int MODE, bool LOSSY>
 void adjoint_launch_t(const YeePlan* p, const AdjArgs& a, float* const* lam, hipStream_t st) {
   TileSet in, sh;
-  split_tiles(a.nx, a.ny, a.nz, a.p, p->kz, 1, 1, 1, 1, p->xcd != 0, in, sh);
+  split_tiles(a.nx, a.ny, a.nz, a.p, p->kz, 1, 1, 1, 1, p->xcd < 0 ? 1 : p->xcd, in, sh);
   if (in.ntiles > 0)
     yee_adj_h_kernel<MODE, false><<<tileset_blocks(in), kThreads, 0, st>>>(
         a, in, lam[0], lam[1], lam[2], lam[3], lam[4], lam[5]);

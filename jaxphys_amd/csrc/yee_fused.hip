@@ -566,53 +566,87 @@ __device__ __forceinline__ void v2_plane(
 // plane a wave issues 15 full-width loads instead of 24.
 // ---------------------------------------------------------------------------
 
-template <int MODE, bool LOSSY, int SL>
-__device__ __forceinline__ void v3_plane(
+// The loads of one plane of variant 3 (edge-lane loads exec-masked).
+struct V3In {
+  float ez, ex_yp, ez_yp, ex_zp, ey_zp, hx0, hy0, hz0;
+  float ez_ym, ex_ym, ey_ym_zp, hx_ym0, hz_ym0, cb, ca;
+  float ld_ey_xp, ld_ez_xp, ld_ey_ym_xp;
+  float ez_xm, ey_xm, ex_xm_yp, ex_xm_zp, hy_xm0, hz_xm0;
+};
+
+template <int MODE, bool LOSSY>
+__device__ __forceinline__ unsigned v3_zp(const FusedArgs& a, int k) {
+  const unsigned P = (unsigned)a.nx * a.ny;
+  if (MODE == kPeriodic) return k == a.nz - 1 ? -(unsigned)(a.nz - 1) * P : P;
+  return k < a.nz - 1 ? P : 0u;
+}
+
+template <int MODE, bool LOSSY>
+__device__ __forceinline__ void v3_load(
     const FusedArgs& a, const float* __restrict__ Ex, const float* __restrict__ Ey,
     const float* __restrict__ Ez, const float* __restrict__ Hx, const float* __restrict__ Hy,
-    const float* __restrict__ Hz, float* __restrict__ oEx, float* __restrict__ oEy,
+    const float* __restrict__ Hz, const V2Col& q, bool lane0, bool edge_r, int k, unsigned c,
+    V3In& L) {
+  const unsigned zp = v3_zp<MODE, LOSSY>(a, k);
+  // full-width loads: the column, its y+1 / z+1 neighbours, row y-1
+  L.ez = Ez[c];
+  L.ex_yp = Ex[c + q.yp];
+  L.ez_yp = Ez[c + q.yp];
+  L.ex_zp = Ex[c + zp];
+  L.ey_zp = Ey[c + zp];
+  L.hx0 = Hx[c];
+  L.hy0 = Hy[c];
+  L.hz0 = Hz[c];
+  const unsigned cym = c + q.ym;
+  L.ez_ym = Ez[cym];
+  L.ex_ym = Ex[cym];
+  L.ey_ym_zp = Ey[cym + zp];
+  L.hx_ym0 = Hx[cym];
+  L.hz_ym0 = Hz[cym];
+  L.cb = a.cb[c];
+  L.ca = LOSSY ? a.ca[c] : 1.f;
+  // edge lanes (exec-masked)
+  L.ld_ey_xp = L.ld_ez_xp = L.ld_ey_ym_xp = 0.f;
+  if (edge_r) {
+    L.ld_ey_xp = Ey[c + q.xp];
+    L.ld_ez_xp = Ez[c + q.xp];
+    L.ld_ey_ym_xp = Ey[cym + q.xp];
+  }
+  const unsigned cxm = c + q.xm;
+  L.ez_xm = L.ey_xm = L.ex_xm_yp = L.ex_xm_zp = L.hy_xm0 = L.hz_xm0 = 0.f;
+  if (lane0) {
+    L.ez_xm = Ez[cxm];
+    L.ey_xm = Ey[cxm];
+    L.ex_xm_yp = Ex[cxm + q.yp];
+    L.ex_xm_zp = Ex[cxm + zp];
+    L.hy_xm0 = Hy[cxm];
+    L.hz_xm0 = Hz[cxm];
+  }
+}
+
+// Arithmetic and stores of one plane of variant 3 from its loads.
+template <int MODE, bool LOSSY, int SL>
+__device__ __forceinline__ void v3_compute(
+    const FusedArgs& a, float* __restrict__ oEx, float* __restrict__ oEy,
     float* __restrict__ oEz, float* __restrict__ oHx, float* __restrict__ oHy,
     float* __restrict__ oHz, const V2Col& q, bool lane0, bool edge_r, int k, unsigned c,
-    V2Carry& r) {
+    const V3In& L, V2Carry& r) {
   constexpr bool per = MODE == kPeriodic;
   constexpr bool XS = MODE == kCpml && (SL & 1);
   constexpr bool YS = MODE == kCpml && (SL & 2);
   constexpr bool ZS = MODE == kCpml && (SL & 4);
   const int nx = a.nx, ny = a.ny, nz = a.nz, p = a.p;
-  const unsigned P = (unsigned)nx * ny;
   const bool zinH = per || k < nz - 1;
   const bool zinE = per || k > 0;
-  const unsigned zp = (per && k == nz - 1) ? -(unsigned)(nz - 1) * P : (zinH ? P : 0u);
   const float ch = a.ch;
-
-  // ---- full-width loads: the column, its y+1 / z+1 neighbours, row y-1
   const float ex = r.ex, ey = r.ey, eym = r.eym;
-  const float ez = Ez[c];
-  const float ex_yp = Ex[c + q.yp], ez_yp = Ez[c + q.yp];
-  const float ex_zp = Ex[c + zp], ey_zp = Ey[c + zp];
-  const float hx0 = Hx[c], hy0 = Hy[c], hz0 = Hz[c];
-  const unsigned cym = c + q.ym;
-  const float ez_ym = Ez[cym], ex_ym = Ex[cym], ey_ym_zp = Ey[cym + zp];
-  const float hx_ym0 = Hx[cym], hz_ym0 = Hz[cym];
-  const float cb = a.cb[c];
-  const float ca = LOSSY ? a.ca[c] : 1.f;
-  // ---- edge-lane loads, issued together with the others (exec-masked)
-  float ld_ey_xp = 0.f, ld_ez_xp = 0.f, ld_ey_ym_xp = 0.f;
-  if (edge_r) {
-    ld_ey_xp = Ey[c + q.xp];
-    ld_ez_xp = Ez[c + q.xp];
-    ld_ey_ym_xp = Ey[cym + q.xp];
-  }
-  const unsigned cxm = c + q.xm;
-  float ez_xm = 0.f, ey_xm = 0.f, ex_xm_yp = 0.f, ex_xm_zp = 0.f, hy_xm0 = 0.f, hz_xm0 = 0.f;
-  if (lane0) {
-    ez_xm = Ez[cxm];
-    ey_xm = Ey[cxm];
-    ex_xm_yp = Ex[cxm + q.yp];
-    ex_xm_zp = Ex[cxm + zp];
-    hy_xm0 = Hy[cxm];
-    hz_xm0 = Hz[cxm];
-  }
+  const float ez = L.ez, ex_yp = L.ex_yp, ez_yp = L.ez_yp, ex_zp = L.ex_zp, ey_zp = L.ey_zp;
+  const float hx0 = L.hx0, hy0 = L.hy0, hz0 = L.hz0;
+  const float ez_ym = L.ez_ym, ex_ym = L.ex_ym, ey_ym_zp = L.ey_ym_zp;
+  const float hx_ym0 = L.hx_ym0, hz_ym0 = L.hz_ym0, cb = L.cb, ca = L.ca;
+  const float ld_ey_xp = L.ld_ey_xp, ld_ez_xp = L.ld_ez_xp, ld_ey_ym_xp = L.ld_ey_ym_xp;
+  const float ez_xm = L.ez_xm, ey_xm = L.ey_xm, ex_xm_yp = L.ex_xm_yp, ex_xm_zp = L.ex_xm_zp;
+  const float hy_xm0 = L.hy_xm0, hz_xm0 = L.hz_xm0;
   // ---- column x+1 from lane+1 (the last active lane uses its loads)
   float ey_xp = __shfl_down(ey, 1);
   float ez_xp = __shfl_down(ez, 1);
@@ -739,6 +773,19 @@ __device__ __forceinline__ void v3_plane(
   r.eym = ey_ym_zp;
   r.hxm = hx;
   r.hym = hy;
+}
+
+template <int MODE, bool LOSSY, int SL>
+__device__ __forceinline__ void v3_plane(
+    const FusedArgs& a, const float* __restrict__ Ex, const float* __restrict__ Ey,
+    const float* __restrict__ Ez, const float* __restrict__ Hx, const float* __restrict__ Hy,
+    const float* __restrict__ Hz, float* __restrict__ oEx, float* __restrict__ oEy,
+    float* __restrict__ oEz, float* __restrict__ oHx, float* __restrict__ oHy,
+    float* __restrict__ oHz, const V2Col& q, bool lane0, bool edge_r, int k, unsigned c,
+    V2Carry& r) {
+  V3In L;
+  v3_load<MODE, LOSSY>(a, Ex, Ey, Ez, Hx, Hy, Hz, q, lane0, edge_r, k, c, L);
+  v3_compute<MODE, LOSSY, SL>(a, oEx, oEy, oEz, oHx, oHy, oHz, q, lane0, edge_r, k, c, L, r);
 }
 
 // ---------------------------------------------------------------------------
@@ -1004,14 +1051,23 @@ __global__ __launch_bounds__(64 * ROWS) void yee_fused_v2_kernel(
     const float* __restrict__ fi4, const float* __restrict__ fi5, float* __restrict__ fo0,
     float* __restrict__ fo1, float* __restrict__ fo2, float* __restrict__ fo3,
     float* __restrict__ fo4, float* __restrict__ fo5) {
-  int ti0, xl, tj0, yl, k0, k1;
-  if (!tile_of_set(ts, ti0, xl, tj0, yl, k0, k1)) return;
+  TilePos tp;
+  if (!tile_of_set(ts, tp)) return;
   const int nx = a.nx, ny = a.ny, nz = a.nz, p = a.p;
   constexpr bool per = MODE == kPeriodic;
-  const int i = ti0 + (int)(threadIdx.x & 63);
+  const int k0 = tp.k0, k1 = tp.k1;
+  const int lane = threadIdx.x & 63;
+  // a paired tile holds two 32-column segments, one per half-wave; the
+  // first / last active lane of each segment exchanges its x neighbour
+  // through memory instead of a lane permute
+  const int seg = tp.pair ? (lane & 31) : lane;
+  const int i = tp.pair && lane >= 32 ? tp.ih0 + seg : tp.i0 + seg;
+  const int xl = tp.pair && lane >= 32 ? tp.ihl : tp.xl;
+  const bool lane0 = seg == 0;
+  const bool edge_r = seg == (tp.pair ? 31 : 63) || i == xl - 1;
   // one y row per wavefront: j and everything derived from it is scalar
-  const int j = __builtin_amdgcn_readfirstlane(tj0 + (int)(threadIdx.x >> 6));
-  if (i >= xl || j >= yl) return;
+  const int j = __builtin_amdgcn_readfirstlane(tp.j0 + (int)(threadIdx.x >> 6));
+  if (i >= xl || j >= tp.yl) return;
   V2Col q;
   q.i = i;
   q.j = j;
@@ -1038,10 +1094,10 @@ __global__ __launch_bounds__(64 * ROWS) void yee_fused_v2_kernel(
   // neighbours)
   int slxy = 0;
   if (MODE == kCpml) {
-    const int il = min(ti0 + 63, xl - 1);
-    const int jl = min(tj0 + ROWS - 1, yl - 1);
-    if (ti0 <= p || il >= nx - p) slxy |= 1;
-    if (tj0 <= p || jl >= ny - p) slxy |= 2;
+    const int il = min(tp.i0 + 63, tp.xl - 1);
+    const int jl = min(tp.j0 + ROWS - 1, tp.yl - 1);
+    if (tp.pair || tp.i0 <= p || il >= nx - p) slxy |= 1;
+    if (tp.j0 <= p || jl >= ny - p) slxy |= 2;
   }
   const unsigned P = (unsigned)nx * ny;
   unsigned c = (unsigned)k0 * P + (unsigned)j * nx + i;
@@ -1060,12 +1116,8 @@ __global__ __launch_bounds__(64 * ROWS) void yee_fused_v2_kernel(
       h_cell<MODE, 3>(a, i, j, kp, false, r.hxm, r.hym, unused, nullptr);
     }
   }
-  const int lane = threadIdx.x & 63;
-  const bool lane0 = lane == 0;
-  // the last active lane of the wave: column x+1 belongs to another tile
-  const bool edge_r = lane == 63 || i == xl - 1;
   V3Tab tb;
-  if constexpr (MODE == kCpml && SHELL && VAR == 3) {
+  if constexpr (MODE == kCpml && SHELL && VAR >= 3) {
     const float* tx = a.cpml;
     tb.xbE = tx[i]; tb.xcE = tx[nx + i]; tb.xkE = tx[2 * nx + i];
     tb.xbH = tx[3 * nx + i]; tb.xcH = tx[4 * nx + i]; tb.xkH = tx[5 * nx + i];
@@ -1074,14 +1126,26 @@ __global__ __launch_bounds__(64 * ROWS) void yee_fused_v2_kernel(
   v3s_plane<MODE, LOSSY, XSB>(a, fi0, fi1, fi2, fi3, fi4, fi5, fo0, fo1, fo2, fo3, fo4, fo5, q, \
                               tb, lane0, edge_r, k, c, r)
 #define YEE_V2_PLANE(S)                                                                       \
-  if constexpr (VAR == 3)                                                                     \
+  if constexpr (VAR >= 3)                                                                     \
     v3_plane<MODE, LOSSY, S>(a, fi0, fi1, fi2, fi3, fi4, fi5, fo0, fo1, fo2, fo3, fo4, fo5, q, \
                              lane0, edge_r, k, c, r);                                         \
   else                                                                                        \
     v2_plane<MODE, LOSSY, S>(a, fi0, fi1, fi2, fi3, fi4, fi5, fo0, fo1, fo2, fo3, fo4, fo5, q, \
                              k, c, r)
+  if constexpr (VAR == 4 && !(MODE == kCpml && SHELL)) {
+    // software pipeline: the loads of plane k+1 are issued before the
+    // arithmetic and stores of plane k
+    V3In cur, nxt;
+    v3_load<MODE, LOSSY>(a, fi0, fi1, fi2, fi3, fi4, fi5, q, lane0, edge_r, k0, c, cur);
+    for (int k = k0; k < k1; ++k, c += P) {
+      if (k + 1 < k1)
+        v3_load<MODE, LOSSY>(a, fi0, fi1, fi2, fi3, fi4, fi5, q, lane0, edge_r, k + 1, c + P, nxt);
+      v3_compute<MODE, LOSSY, 0>(a, fo0, fo1, fo2, fo3, fo4, fo5, q, lane0, edge_r, k, c, cur, r);
+      cur = nxt;
+    }
+  } else
   for (int k = k0; k < k1; ++k, c += P) {
-    if constexpr (MODE == kCpml && SHELL && VAR == 3) {
+    if constexpr (MODE == kCpml && SHELL && VAR >= 3) {
       if (slxy & 1) YEE_V3S_PLANE(true);
       else YEE_V3S_PLANE(false);
     } else if (MODE != kCpml || !SHELL) {
@@ -1123,7 +1187,7 @@ void launch_v2_rows(const FusedArgs& a, const YeePlan* pl, hipStream_t st) {
   // a tile reads the x-1 / y-1 neighbours of its cells; planes p .. nz-p-1
   // (the plane below the first is recomputed with full CPML handling)
   split_tiles(a.nx, a.ny, a.nz, MODE == kCpml ? a.p : 0, fused_kz(pl, gx, gy), 1, 0, 0, 0,
-              pl->xcd == 1, in, sh, ROWS);
+              pl->xcd < 0 ? 0 : pl->xcd, in, sh, ROWS, VAR >= 3);
   if (in.ntiles > 0)
     yee_fused_v2_kernel<MODE, LOSSY, false, VAR, ROWS><<<tileset_blocks(in), 64 * ROWS, 0, st>>>(
         a, in, a.fi[0], a.fi[1], a.fi[2], a.fi[3], a.fi[4], a.fi[5], a.fo[0], a.fo[1], a.fo[2],
@@ -1139,7 +1203,7 @@ void launch_v2_rows(const FusedArgs& a, const YeePlan* pl, hipStream_t st) {
 // fused kernels on the 512^3 grid)
 template <int MODE, bool LOSSY, int VAR>
 void launch_v2(const FusedArgs& a, const YeePlan* pl, hipStream_t st) {
-  if constexpr (VAR == 3) {
+  if constexpr (VAR >= 3) {
     if (pl->fused_rows == 16) return launch_v2_rows<MODE, LOSSY, VAR, 16>(a, pl, st);
     if (pl->fused_rows == 8) return launch_v2_rows<MODE, LOSSY, VAR, 8>(a, pl, st);
   }
@@ -1149,7 +1213,9 @@ void launch_v2(const FusedArgs& a, const YeePlan* pl, hipStream_t st) {
 template <int MODE, bool LOSSY>
 void launch_fused_t(const FusedArgs& a, const YeePlan* p, hipStream_t st) {
   const int gx = (a.nx + 63) / 64;
-  if (p->fused_variant == 3) {
+  if (p->fused_variant == 4) {
+    launch_v2<MODE, LOSSY, 4>(a, p, st);
+  } else if (p->fused_variant == 3) {
     launch_v2<MODE, LOSSY, 3>(a, p, st);
   } else if (p->fused_variant == 2) {
     launch_v2<MODE, LOSSY, 2>(a, p, st);

@@ -113,19 +113,26 @@ constexpr int kMaxBoxes = 6;
 
 struct TileBox {
   int x0, x1, nxt, y0, y1, nyt, z0, z1, first;
+  // paired box (xh1 > xh0): one 64-lane tile column holds [x0, x1) in lanes
+  // 0-31 and [xh0, xh1) in lanes 32-63 (both at most 32 wide)
+  int xh0, xh1;
 };
 
 struct TileSet {
   int nbox, ntiles, per, kz, rows;
+  int band, bq;  // banded XCD order: tiles per z-chunk band, per-XCD share
   TileBox b[kMaxBoxes];
 };
 
-inline void tileset_add(TileSet& ts, int x0, int x1, int y0, int y1, int z0, int z1) {
+inline void tileset_add(TileSet& ts, int x0, int x1, int y0, int y1, int z0, int z1,
+                        int xh0 = 0, int xh1 = 0) {
   if (x1 <= x0 || y1 <= y0 || z1 <= z0) return;
   TileBox& B = ts.b[ts.nbox++];
   B.x0 = x0;
   B.x1 = x1;
-  B.nxt = (x1 - x0 + 63) / 64;
+  B.xh0 = xh0;
+  B.xh1 = xh1;
+  B.nxt = xh1 > xh0 ? 1 : (x1 - x0 + 63) / 64;
   B.y0 = y0;
   B.y1 = y1;
   B.nyt = (y1 - y0 + ts.rows - 1) / ts.rows;
@@ -135,11 +142,27 @@ inline void tileset_add(TileSet& ts, int x0, int x1, int y0, int y1, int z0, int
   ts.ntiles += B.nxt * B.nyt * ((z1 - z0 + ts.kz - 1) / ts.kz);
 }
 
-inline void tileset_finish(TileSet& ts, bool xcd) {
-  ts.per = xcd ? (ts.ntiles + kXcds - 1) / kXcds : 0;
+// Tile order of a set: 0 plain (t = L), 1 XCD-contiguous (as Tiling), 2
+// XCD-banded (single-box sets): the tiles of one z-chunk band are split into
+// 8 contiguous shares, one per XCD, and every XCD moves through the bands
+// at the same pace, so all XCDs stream the same planes while each keeps
+// its neighbouring tiles in its own L2.
+inline void tileset_finish(TileSet& ts, int order) {
+  ts.per = 0;
+  ts.band = ts.bq = 0;
+  if (order == 2 && ts.nbox == 1 && ts.ntiles > 0) {
+    ts.band = ts.b[0].nxt * ts.b[0].nyt;
+    ts.bq = (ts.band + kXcds - 1) / kXcds;
+  } else if (order >= 1) {
+    ts.per = (ts.ntiles + kXcds - 1) / kXcds;
+  }
 }
 
 inline unsigned tileset_blocks(const TileSet& ts) {
+  if (ts.band > 0) {
+    const int nb = (ts.ntiles + ts.band - 1) / ts.band;
+    return (unsigned)(nb * ts.bq * kXcds);
+  }
   return ts.per > 0 ? (unsigned)(ts.per * kXcds) : (unsigned)ts.ntiles;
 }
 
@@ -149,7 +172,7 @@ inline unsigned tileset_blocks(const TileSet& ts) {
 // aligned to 32 columns (128-byte rows); and `sh`, every other cell, as up
 // to six boxes.  p == 0 puts the whole grid in `in`.
 inline void split_tiles(int nx, int ny, int nz, int p, int kz, int lo, int hi, int zlo, int zhi,
-                        bool xcd, TileSet& in, TileSet& sh, int rows = 4) {
+                        int order, TileSet& in, TileSet& sh, int rows = 4, bool pair_x = false) {
   std::memset(&in, 0, sizeof(in));
   std::memset(&sh, 0, sizeof(sh));
   in.kz = sh.kz = kz;
@@ -166,36 +189,57 @@ inline void split_tiles(int nx, int ny, int nz, int p, int kz, int lo, int hi, i
       tileset_add(sh, 0, nx, 0, ny, zb, nz);
       tileset_add(sh, 0, nx, 0, ya, za, zb);
       tileset_add(sh, 0, nx, yb, ny, za, zb);
-      tileset_add(sh, 0, xa, ya, yb, za, zb);
-      tileset_add(sh, xb, nx, ya, yb, za, zb);
+      if (pair_x && xa <= 32 && nx - xb <= 32) {
+        tileset_add(sh, 0, xa, ya, yb, za, zb, xb, nx);
+      } else {
+        tileset_add(sh, 0, xa, ya, yb, za, zb);
+        tileset_add(sh, xb, nx, ya, yb, za, zb);
+      }
     } else {
       tileset_add(sh, 0, nx, 0, ny, 0, nz);
     }
   }
-  tileset_finish(in, xcd);
-  tileset_finish(sh, xcd);
+  tileset_finish(in, order);
+  tileset_finish(sh, order == 2 ? 0 : order);
 }
 
-// First column / row / plane of this workgroup's tile in the set and the
-// exclusive limits of its box (x, y) and of the tile (k1); false for the
-// padding workgroups of the XCD-aware order (same map as Tiling).
-__device__ __forceinline__ bool tile_of_set(const TileSet& ts, int& i0, int& xl, int& j0,
-                                            int& yl, int& k0, int& k1) {
+// Position of a workgroup's tile: first column / row / plane, the exclusive
+// limits of its box (x, y) and of the tile (k1), and for a paired box the
+// high half (lanes 32-63: columns ih0 .. ihl-1).
+struct TilePos {
+  int i0, xl, j0, yl, k0, k1, ih0, ihl;
+  bool pair;
+};
+
+// false for the padding workgroups of the XCD-aware order (same map as
+// Tiling)
+__device__ __forceinline__ bool tile_of_set(const TileSet& ts, TilePos& tp) {
   const int L = blockIdx.x;
-  const int t = ts.per > 0 ? (L % kXcds) * ts.per + L / kXcds : L;
+  int t;
+  if (ts.band > 0) {
+    const int s = L / kXcds;
+    const int loc = (L % kXcds) * ts.bq + s % ts.bq;
+    if (loc >= ts.band) return false;
+    t = (s / ts.bq) * ts.band + loc;
+  } else {
+    t = ts.per > 0 ? (L % kXcds) * ts.per + L / kXcds : L;
+  }
   if (t >= ts.ntiles) return false;
   TileBox B = ts.b[0];
 #pragma unroll
   for (int q = 1; q < kMaxBoxes; ++q)
     if (q < ts.nbox && t >= ts.b[q].first) B = ts.b[q];
   const int lt = t - B.first;
-  i0 = B.x0 + (lt % B.nxt) * 64;
-  xl = B.x1;
+  tp.i0 = B.x0 + (lt % B.nxt) * 64;
+  tp.xl = B.x1;
+  tp.pair = B.xh1 > B.xh0;
+  tp.ih0 = B.xh0;
+  tp.ihl = B.xh1;
   const int r = lt / B.nxt;
-  j0 = B.y0 + (r % B.nyt) * ts.rows;
-  yl = B.y1;
-  k0 = B.z0 + (r / B.nyt) * ts.kz;
-  k1 = min(k0 + ts.kz, B.z1);
+  tp.j0 = B.y0 + (r % B.nyt) * ts.rows;
+  tp.yl = B.y1;
+  tp.k0 = B.z0 + (r / B.nyt) * ts.kz;
+  tp.k1 = min(tp.k0 + ts.kz, B.z1);
   return true;
 }
 
@@ -223,7 +267,7 @@ struct YeePlan {
   int fused_kz = 0;  // z planes per block of the fused kernel (0: automatic)
   int fused_variant = 3;  // 0: LDS + halo row, 1-3: barrier-free (see yee_fused.hip)
   int fused_rows = 4;     // y rows (wavefronts) per tile of variants 2 and 3: 4, 8 or 16
-  int xcd = -1;      // tile order: 1 XCD-aware (see Tiling), 0 plain, -1 automatic
+  int xcd = -1;      // tile order: 0 plain, 1 XCD-contiguous, 2 XCD-banded, -1 automatic
   // sources
   int nsrc = 0;
   int nsteps_table = 0;
