@@ -1,6 +1,7 @@
 """FDTD3D throughput on one MI355X: Gcell-updates/s and HBM-roofline fraction.
 
-``python bench.py [--steps K] [--warmup W] [--workload c3_512|c2_128|c1_64]``
+``python bench.py [--steps K] [--warmup W] [--workload c3_512|c3_256|c2_128|c1_64|
+adjoint_c4|closure_c5|closure_net] [--mode fused|two_pass]``
 
 A step is one full Yee time step (H half-step, then E half-step with CPML
 psi, conductivity and the three dipoles) over the whole synthetic grid; the
@@ -160,6 +161,47 @@ def adjoint_bench(args, dev):
     }
 
 
+def closure_net_bench(args, dev):
+    """The closure net's BF16-MFMA conv kernels at the coarse 128^3 grid:
+    one inference (three implicit-GEMM 3^3 convolutions, 7->32->32->6
+    channels, GELU) per step, inputs resident in HBM."""
+    import torch
+
+    from jaxphys_amd.closure import CIN, ClosureNet
+
+    n = args.closure_n // 2
+    shape = (n, n, n)
+    cells = n ** 3
+    net = ClosureNet(dev, seed=0, out_scale=1.0)
+    x = torch.randn(cells, CIN, device=dev).to(torch.bfloat16)
+    x[:, 7] = 0
+    for _ in range(args.warmup):
+        net.forward(x, shape)
+    torch.cuda.synchronize()
+    K = args.steps if args.steps is not None else 50
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(K):
+        net.forward(x, shape)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / K
+    flops = 2.0 * cells * 27 * (7 * 32 + 32 * 32 + 32 * 6)
+    tflops = flops / (ms / 1e3) / 1e12
+    return {
+        "metric": "closure net inference (3 conv3d layers, 128^3)", "value": round(tflops, 2),
+        "unit": "TFLOP/s", "higher_is_better": True, "n_gpus": 1, "steps": K,
+        "warmup": args.warmup, "ms_per_step": round(ms, 4), "dtype": "bf16 (fp32 accumulate)",
+        "data": "synthetic", "scaling": "weak", "vs_baseline": None,
+        "config": {"workload": "closure_net", "grid": [n] * 3, "channels": [7, 32, 32, 6],
+                   "kernel": "3x3x3"},
+        "roofline": {"bound": "mfma", "kernel": "conv_fwd_kernel (3 launches per inference)",
+                     "achieved": round(tflops, 2), "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": round(tflops / 2500.0, 4), "traffic": None},
+    }
+
+
 def closure_bench(args, dev):
     """Learned closure: train on fine-run/coarse pairs, evaluate on held-out
     seeds; reports the corrected coarse error and the wall-time ratio."""
@@ -255,6 +297,9 @@ def main() -> None:
         return
     if args.workload == "closure_c5":
         print(json.dumps(closure_bench(args, dev)), flush=True)
+        return
+    if args.workload == "closure_net":
+        print(json.dumps(closure_net_bench(args, dev)), flush=True)
         return
 
     t_setup = time.time()

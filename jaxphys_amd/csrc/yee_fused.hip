@@ -1041,10 +1041,11 @@ __device__ __forceinline__ void v3s_plane(
   r.hym = hy;
 }
 
-// SHELL == false with CPML: the tiles of `ts` lie outside every slab (no
-// psi code is compiled in); SHELL == true: any tiles, slab work chosen per
-// plane.
-template <int MODE, bool LOSSY, bool SHELL, int VAR, int ROWS>
+// SHELL == 0 with CPML: the tiles of `ts` lie outside every slab (no psi
+// code is compiled in).  SHELL == 1, 2: any tiles, slab work chosen per
+// plane; with VAR >= 3, 1 = tiles whose columns (and x-1 neighbours) touch
+// no x slab, 2 = x-slab columns (x psi terms compiled in).
+template <int MODE, bool LOSSY, int SHELL, int VAR, int ROWS>
 __global__ __launch_bounds__(64 * ROWS) void yee_fused_v2_kernel(
     FusedArgs a, TileSet ts, const float* __restrict__ fi0,
     const float* __restrict__ fi1, const float* __restrict__ fi2, const float* __restrict__ fi3,
@@ -1117,7 +1118,7 @@ __global__ __launch_bounds__(64 * ROWS) void yee_fused_v2_kernel(
     }
   }
   V3Tab tb;
-  if constexpr (MODE == kCpml && SHELL && VAR >= 3) {
+  if constexpr (MODE == kCpml && SHELL == 2 && VAR >= 3) {
     const float* tx = a.cpml;
     tb.xbE = tx[i]; tb.xcE = tx[nx + i]; tb.xkE = tx[2 * nx + i];
     tb.xbH = tx[3 * nx + i]; tb.xcH = tx[4 * nx + i]; tb.xkH = tx[5 * nx + i];
@@ -1132,7 +1133,7 @@ __global__ __launch_bounds__(64 * ROWS) void yee_fused_v2_kernel(
   else                                                                                        \
     v2_plane<MODE, LOSSY, S>(a, fi0, fi1, fi2, fi3, fi4, fi5, fo0, fo1, fo2, fo3, fo4, fo5, q, \
                              k, c, r)
-  if constexpr (VAR == 4 && !(MODE == kCpml && SHELL)) {
+  if constexpr (VAR == 4 && !(MODE == kCpml && SHELL != 0)) {
     // software pipeline: the loads of plane k+1 are issued before the
     // arithmetic and stores of plane k
     V3In cur, nxt;
@@ -1145,10 +1146,9 @@ __global__ __launch_bounds__(64 * ROWS) void yee_fused_v2_kernel(
     }
   } else
   for (int k = k0; k < k1; ++k, c += P) {
-    if constexpr (MODE == kCpml && SHELL && VAR >= 3) {
-      if (slxy & 1) YEE_V3S_PLANE(true);
-      else YEE_V3S_PLANE(false);
-    } else if (MODE != kCpml || !SHELL) {
+    if constexpr (MODE == kCpml && SHELL != 0 && VAR >= 3) {
+      YEE_V3S_PLANE(SHELL == 2);
+    } else if (MODE != kCpml || SHELL == 0) {
       YEE_V2_PLANE(0);
     } else {
       const int sl = slxy | ((k < p || k >= nz - p) ? 4 : 0);
@@ -1183,20 +1183,28 @@ int fused_kz(const YeePlan* p, int /*gx*/, int /*gy*/) {
 template <int MODE, bool LOSSY, int VAR, int ROWS>
 void launch_v2_rows(const FusedArgs& a, const YeePlan* pl, hipStream_t st) {
   const int gx = (a.nx + 63) / 64, gy = (a.ny + ROWS - 1) / ROWS;
-  TileSet in, sh;
+  const int kz = fused_kz(pl, gx, gy);
+  const int order = pl->xcd < 0 ? 0 : pl->xcd;
   // a tile reads the x-1 / y-1 neighbours of its cells; planes p .. nz-p-1
   // (the plane below the first is recomputed with full CPML handling)
-  split_tiles(a.nx, a.ny, a.nz, MODE == kCpml ? a.p : 0, fused_kz(pl, gx, gy), 1, 0, 0, 0,
-              pl->xcd < 0 ? 0 : pl->xcd, in, sh, ROWS, VAR >= 3);
-  if (in.ntiles > 0)
-    yee_fused_v2_kernel<MODE, LOSSY, false, VAR, ROWS><<<tileset_blocks(in), 64 * ROWS, 0, st>>>(
-        a, in, a.fi[0], a.fi[1], a.fi[2], a.fi[3], a.fi[4], a.fi[5], a.fo[0], a.fo[1], a.fo[2],
-        a.fo[3], a.fo[4], a.fo[5]);
-  if constexpr (MODE == kCpml)
-    if (sh.ntiles > 0)
-      yee_fused_v2_kernel<MODE, LOSSY, true, VAR, ROWS><<<tileset_blocks(sh), 64 * ROWS, 0, st>>>(
-        a, sh, a.fi[0], a.fi[1], a.fi[2], a.fi[3], a.fi[4], a.fi[5], a.fo[0], a.fo[1], a.fo[2],
-        a.fo[3], a.fo[4], a.fo[5]);
+  TileSet in, sh, shx;
+  if constexpr (VAR >= 3) {
+    split_tiles3(a.nx, a.ny, a.nz, MODE == kCpml ? a.p : 0, kz, 1, 0, order, in, sh, shx, ROWS);
+  } else {
+    split_tiles(a.nx, a.ny, a.nz, MODE == kCpml ? a.p : 0, kz, 1, 0, 0, 0, order, in, sh, ROWS);
+    std::memset(&shx, 0, sizeof(shx));
+  }
+#define YEE_V2_LAUNCH(SH, TS)                                                                 \
+  yee_fused_v2_kernel<MODE, LOSSY, SH, VAR, ROWS><<<tileset_blocks(TS), 64 * ROWS, 0, st>>>( \
+      a, TS, a.fi[0], a.fi[1], a.fi[2], a.fi[3], a.fi[4], a.fi[5], a.fo[0], a.fo[1], a.fo[2], \
+      a.fo[3], a.fo[4], a.fo[5])
+  if (in.ntiles > 0) YEE_V2_LAUNCH(0, in);
+  if constexpr (MODE == kCpml) {
+    if (sh.ntiles > 0) YEE_V2_LAUNCH(1, sh);
+    if constexpr (VAR >= 3)
+      if (shx.ntiles > 0) YEE_V2_LAUNCH(2, shx);
+  }
+#undef YEE_V2_LAUNCH
 }
 
 // tile order: plain unless XCD-aware is asked for (measured faster for the

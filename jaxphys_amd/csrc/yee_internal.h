@@ -203,6 +203,44 @@ inline void split_tiles(int nx, int ny, int nz, int p, int kz, int lo, int hi, i
   tileset_finish(sh, order == 2 ? 0 : order);
 }
 
+// Three-way split for the fused step: `in` as in split_tiles (x aligned to
+// 32 columns), `shx` the x-slab columns [0, xa) and [xb, nx) over all y and
+// z (one paired box when both are at most 32 wide), and `shn` the rest of
+// the shell -- y-slab rows and z-slab planes over x in [xa, xb), where no
+// cell or x-1 neighbour lies in an x slab.  When there is no interior the
+// whole grid goes to `shx`.
+inline void split_tiles3(int nx, int ny, int nz, int p, int kz, int lo, int hi, int order,
+                         TileSet& in, TileSet& shn, TileSet& shx, int rows) {
+  std::memset(&in, 0, sizeof(in));
+  std::memset(&shn, 0, sizeof(shn));
+  std::memset(&shx, 0, sizeof(shx));
+  in.kz = shn.kz = shx.kz = kz;
+  in.rows = shn.rows = shx.rows = rows;
+  const int xa = (p + lo + 31) / 32 * 32, xb = (nx - p - hi) / 32 * 32;
+  const int ya = p + lo, yb = ny - p - hi;
+  const int za = p, zb = nz - p;
+  if (p <= 0) {
+    tileset_add(in, 0, nx, 0, ny, 0, nz);
+  } else if (xa < xb && ya < yb && za < zb) {
+    tileset_add(in, xa, xb, ya, yb, za, zb);
+    if (xa <= 32 && nx - xb <= 32) {
+      tileset_add(shx, 0, xa, 0, ny, 0, nz, xb, nx);
+    } else {
+      tileset_add(shx, 0, xa, 0, ny, 0, nz);
+      tileset_add(shx, xb, nx, 0, ny, 0, nz);
+    }
+    tileset_add(shn, xa, xb, 0, ya, 0, nz);
+    tileset_add(shn, xa, xb, yb, ny, 0, nz);
+    tileset_add(shn, xa, xb, ya, yb, 0, za);
+    tileset_add(shn, xa, xb, ya, yb, zb, nz);
+  } else {
+    tileset_add(shx, 0, nx, 0, ny, 0, nz);
+  }
+  tileset_finish(in, order);
+  tileset_finish(shn, order == 2 ? 0 : order);
+  tileset_finish(shx, order == 2 ? 0 : order);
+}
+
 // Position of a workgroup's tile: first column / row / plane, the exclusive
 // limits of its box (x, y) and of the tile (k1), and for a paired box the
 // high half (lanes 32-63: columns ih0 .. ihl-1).
