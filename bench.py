@@ -213,12 +213,15 @@ def closure_bench(args, dev):
     train_seeds = list(range(args.closure_train))
     test_seeds = list(range(1000, 1000 + args.closure_test))
     t0 = time.time()
+    pkw = {"t0": args.closure_t0, "spread": args.closure_spread}
     net, hist = CL.train_closure(dev, train_seeds, n_fine=nf, n_coarse=nf * 400 // 256,
-                                 epochs=args.closure_epochs, crop=min(64, nf // 2 - 16),
-                                 lr=args.closure_lr,
-                                 log=lambda m: print("[closure]", m, file=sys.stderr, flush=True))
+                                 every=args.closure_every, epochs=args.closure_epochs,
+                                 crop=min(64, nf // 2 - 16), lr=args.closure_lr,
+                                 log=lambda m: print("[closure]", m, file=sys.stderr, flush=True),
+                                 **pkw)
     t_train = time.time() - t0
-    res = [CL.evaluate_closure(net, s, n_fine=nf, n_coarse=nf * 400 // 256) for s in test_seeds]
+    res = [CL.evaluate_closure(net, s, n_fine=nf, n_coarse=nf * 400 // 256,
+                               every=args.closure_every, **pkw) for s in test_seeds]
     err_c = [r["err_closure"] for r in res]
     err_0 = [r["err_coarse"] for r in res]
     sp = [r["speedup"] for r in res]
@@ -229,7 +232,8 @@ def closure_bench(args, dev):
         "data": "synthetic",
         "config": {"workload": "closure_c5", "fine": [nf] * 3, "coarse": [nf // 2] * 3,
                    "train_seeds": len(train_seeds), "test_seeds": len(test_seeds),
-                   "epochs": args.closure_epochs},
+                   "epochs": args.closure_epochs, "correct_every": args.closure_every,
+                   "pulse_t0": args.closure_t0, "pulse_spread": args.closure_spread},
         "err_coarse_no_closure": round(float(np.mean(err_0)), 5),
         "err_closure_max": round(float(np.max(err_c)), 5),
         "speedup_vs_fine": round(float(np.mean(sp)), 3),
@@ -257,6 +261,12 @@ def main() -> None:
     ap.add_argument("--closure-test", type=int, default=16)
     ap.add_argument("--closure-epochs", type=int, default=2)
     ap.add_argument("--closure-lr", type=float, default=1e-3)
+    ap.add_argument("--closure-every", type=int, default=10,
+                    help="coarse steps between closure corrections")
+    ap.add_argument("--closure-t0", type=float, default=60.0,
+                    help="pulse centre (coarse steps)")
+    ap.add_argument("--closure-spread", type=float, default=15.0,
+                    help="pulse width (coarse steps)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain copy of the field bytes (counter calibration)")
     args = ap.parse_args()

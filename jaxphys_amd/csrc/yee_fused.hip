@@ -879,6 +879,31 @@ __device__ __forceinline__ void v3s_plane(
     pEzx = pin[bx0 + (kEzx - 8) * xsl + ox];
     pHzx_m = pin[bx0 + (kHzx - 8) * xsl + oxm];
   }
+  // y / z psi of the column and of row y-1, loaded up front in the
+  // instantiation without x terms (its register budget allows it)
+  float eHxy = 0.f, eHzy = 0.f, eExy = 0.f, eEzy = 0.f, eHxy_m = 0.f, eHzy_m = 0.f;
+  float eHxz = 0.f, eHyz = 0.f, eExz = 0.f, eEyz = 0.f, eHxz_m = 0.f;
+  if constexpr (!XS) {
+    if (q.ys) {
+      eHxy = pin[kHxy * ysl + oy];
+      eHzy = pin[kHzy * ysl + oy];
+      eExy = pin[kExy * ysl + oy];
+      eEzy = pin[kEzy * ysl + oy];
+    }
+    if (q.ysm) {
+      eHxy_m = pin[kHxy * ysl + oym];
+      eHzy_m = pin[kHzy * ysl + oym];
+    }
+    if (zs) {
+      eHxz = pin[bz0 + (kHxz - 4) * zsl + oz];
+      eHyz = pin[bz0 + (kHyz - 4) * zsl + oz];
+      eExz = pin[bz0 + (kExz - 4) * zsl + oz];
+      eEyz = pin[bz0 + (kEyz - 4) * zsl + oz];
+      eHxz_m = pin[bz0 + (kHxz - 4) * zsl + ozm];
+    }
+  }
+  // psi value: the early load (no x terms) or a load at the point of use
+#define YEE_PSI(early, off) (XS ? pin[off] : (early))
   // lane 0: psi and x tables of column x-1
   float l_pHyx = 0.f, l_pHzx = 0.f;
   float lxbH = 0.f, lxcH = 0.f, lxkH = 0.f;
@@ -907,11 +932,11 @@ __device__ __forceinline__ void v3s_plane(
     const bool upd = q.yinH && zinH;
     float u = ez_yp - ez, v = ey_zp - ey;
     if (q.ys) {
-      u = pterm(ty[3 * ny + q.j], ty[4 * ny + q.j], ty[5 * ny + q.j], pin[kHxy * ysl + oy], u, ps);
+      u = pterm(ty[3 * ny + q.j], ty[4 * ny + q.j], ty[5 * ny + q.j], YEE_PSI(eHxy, kHxy * ysl + oy), u, ps);
       if (upd) pout[kHxy * ysl + oy] = ps;
     }
     if (zs) {
-      v = pterm(tz[3 * nz + k], tz[4 * nz + k], tz[5 * nz + k], pin[bz0 + (kHxz - 4) * zsl + oz], v, ps);
+      v = pterm(tz[3 * nz + k], tz[4 * nz + k], tz[5 * nz + k], YEE_PSI(eHxz, bz0 + (kHxz - 4) * zsl + oz), v, ps);
       if (upd) pout[bz0 + (kHxz - 4) * zsl + oz] = ps;
     }
     hx = upd ? hx0 - ch * (u - v) : hx0;
@@ -920,7 +945,7 @@ __device__ __forceinline__ void v3s_plane(
     const bool upd = q.xinH && zinH;
     float u = ex_zp - ex, v = ez_xp - ez;
     if (zs) {
-      u = pterm(tz[3 * nz + k], tz[4 * nz + k], tz[5 * nz + k], pin[bz0 + (kHyz - 4) * zsl + oz], u, ps);
+      u = pterm(tz[3 * nz + k], tz[4 * nz + k], tz[5 * nz + k], YEE_PSI(eHyz, bz0 + (kHyz - 4) * zsl + oz), u, ps);
       if (upd) pout[bz0 + (kHyz - 4) * zsl + oz] = ps;
     }
     if (XS) {
@@ -939,7 +964,7 @@ __device__ __forceinline__ void v3s_plane(
       u = q.xs ? w : u;
     }
     if (q.ys) {
-      v = pterm(ty[3 * ny + q.j], ty[4 * ny + q.j], ty[5 * ny + q.j], pin[kHzy * ysl + oy], v, ps);
+      v = pterm(ty[3 * ny + q.j], ty[4 * ny + q.j], ty[5 * ny + q.j], YEE_PSI(eHzy, kHzy * ysl + oy), v, ps);
       if (upd) pout[kHzy * ysl + oy] = ps;
     }
     hz = upd ? hz0 - ch * (u - v) : hz0;
@@ -967,8 +992,8 @@ __device__ __forceinline__ void v3s_plane(
   float hx_ym, hz_ym;
   {
     float u = ez - ez_ym, v = ey_ym_zp - eym;
-    if (q.ysm) u = pterm(ty[3 * ny + q.jm], ty[4 * ny + q.jm], ty[5 * ny + q.jm], pin[kHxy * ysl + oym], u, ps);
-    if (zs) v = pterm(tz[3 * nz + k], tz[4 * nz + k], tz[5 * nz + k], pin[bz0 + (kHxz - 4) * zsl + ozm], v, ps);
+    if (q.ysm) u = pterm(ty[3 * ny + q.jm], ty[4 * ny + q.jm], ty[5 * ny + q.jm], YEE_PSI(eHxy_m, kHxy * ysl + oym), u, ps);
+    if (zs) v = pterm(tz[3 * nz + k], tz[4 * nz + k], tz[5 * nz + k], YEE_PSI(eHxz_m, bz0 + (kHxz - 4) * zsl + ozm), v, ps);
     hx_ym = zinH ? hx_ym0 - ch * (u - v) : hx_ym0;
   }
   {
@@ -977,7 +1002,7 @@ __device__ __forceinline__ void v3s_plane(
       const float w = pterm(t.xbH, t.xcH, t.xkH, pHzx_m, u, ps);
       u = q.xs ? w : u;
     }
-    if (q.ysm) v = pterm(ty[3 * ny + q.jm], ty[4 * ny + q.jm], ty[5 * ny + q.jm], pin[kHzy * ysl + oym], v, ps);
+    if (q.ysm) v = pterm(ty[3 * ny + q.jm], ty[4 * ny + q.jm], ty[5 * ny + q.jm], YEE_PSI(eHzy_m, kHzy * ysl + oym), v, ps);
     hz_ym = q.xinH ? hz_ym0 - ch * (u - v) : hz_ym0;
   }
   oHx[c] = hx;
@@ -990,11 +1015,11 @@ __device__ __forceinline__ void v3s_plane(
     const bool upd = q.yinE && zinE;
     float u = hz - hz_ym, v = hy - r.hym;
     if (q.ys) {
-      u = pterm(ty[q.j], ty[ny + q.j], ty[2 * ny + q.j], pin[kExy * ysl + oy], u, ps);
+      u = pterm(ty[q.j], ty[ny + q.j], ty[2 * ny + q.j], YEE_PSI(eExy, kExy * ysl + oy), u, ps);
       if (upd) pout[kExy * ysl + oy] = ps;
     }
     if (zs) {
-      v = pterm(tz[k], tz[nz + k], tz[2 * nz + k], pin[bz0 + (kExz - 4) * zsl + oz], v, ps);
+      v = pterm(tz[k], tz[nz + k], tz[2 * nz + k], YEE_PSI(eExz, bz0 + (kExz - 4) * zsl + oz), v, ps);
       if (upd) pout[bz0 + (kExz - 4) * zsl + oz] = ps;
     }
     const float w = LOSSY ? ca * ex + cb * (u - v) : ex + cb * (u - v);
@@ -1004,7 +1029,7 @@ __device__ __forceinline__ void v3s_plane(
     const bool upd = q.xinE && zinE;
     float u = hx - r.hxm, v = hz - hz_xm;
     if (zs) {
-      u = pterm(tz[k], tz[nz + k], tz[2 * nz + k], pin[bz0 + (kEyz - 4) * zsl + oz], u, ps);
+      u = pterm(tz[k], tz[nz + k], tz[2 * nz + k], YEE_PSI(eEyz, bz0 + (kEyz - 4) * zsl + oz), u, ps);
       if (upd) pout[bz0 + (kEyz - 4) * zsl + oz] = ps;
     }
     if (XS) {
@@ -1024,7 +1049,7 @@ __device__ __forceinline__ void v3s_plane(
       u = q.xs ? w : u;
     }
     if (q.ys) {
-      v = pterm(ty[q.j], ty[ny + q.j], ty[2 * ny + q.j], pin[kEzy * ysl + oy], v, ps);
+      v = pterm(ty[q.j], ty[ny + q.j], ty[2 * ny + q.j], YEE_PSI(eEzy, kEzy * ysl + oy), v, ps);
       if (upd) pout[kEzy * ysl + oy] = ps;
     }
     const float w = LOSSY ? ca * ez + cb * (u - v) : ez + cb * (u - v);
@@ -1040,6 +1065,7 @@ __device__ __forceinline__ void v3s_plane(
   r.hxm = hx;
   r.hym = hy;
 }
+#undef YEE_PSI
 
 // SHELL == 0 with CPML: the tiles of `ts` lie outside every slab (no psi
 // code is compiled in).  SHELL == 1, 2: any tiles, slab work chosen per
@@ -1172,7 +1198,7 @@ constexpr int kFusedBY = 8;
 
 int fused_kz(const YeePlan* p, int /*gx*/, int /*gy*/) {
   int kz = p->fused_kz;
-  if (kz <= 0) kz = 16;  // z planes per tile; the fastest on the 512^3 grid
+  if (kz <= 0) kz = p->fused_variant >= 3 ? 4 : 16;  // the fastest on the 512^3 grid
   return kz > p->d.nz ? p->d.nz : kz;
 }
 
