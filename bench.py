@@ -224,6 +224,8 @@ def closure_bench(args, dev):
                                every=args.closure_every, **pkw) for s in test_seeds]
     err_c = [r["err_closure"] for r in res]
     err_0 = [r["err_coarse"] for r in res]
+    trj_c = [r["err_traj_closure"] for r in res]
+    trj_0 = [r["err_traj_coarse"] for r in res]
     sp = [r["speedup"] for r in res]
     return {
         "metric": "closure: coarse+net field error vs fine FDTD",
@@ -235,6 +237,8 @@ def closure_bench(args, dev):
                    "epochs": args.closure_epochs, "correct_every": args.closure_every,
                    "pulse_t0": args.closure_t0, "pulse_spread": args.closure_spread},
         "err_coarse_no_closure": round(float(np.mean(err_0)), 5),
+        "err_traj_closure": round(float(np.mean(trj_c)), 5),
+        "err_traj_coarse_no_closure": round(float(np.mean(trj_0)), 5),
         "err_closure_max": round(float(np.max(err_c)), 5),
         "speedup_vs_fine": round(float(np.mean(sp)), 3),
         "t_fine_s": round(float(np.mean([r["t_fine_s"] for r in res])), 4),

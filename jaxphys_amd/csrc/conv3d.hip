@@ -82,14 +82,17 @@ struct ConvArgs {
   void* y;
 };
 
-template <int EPI, int NT, int MT>
-__global__ __launch_bounds__(256) void conv_fwd_kernel(ConvArgs a) {
+// WAVES wavefronts per block share the staged weights (with 16, the LDS of
+// two blocks fits a CU and each SIMD runs 8 wavefronts).  The input
+// fragments of K step k+1 are loaded before the MFMAs of step k.
+template <int EPI, int NT, int MT, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void conv_fwd_kernel(ConvArgs a) {
   extern __shared__ u16 wsh[];  // [cout_pad][kpad + 8]
   const int ldw = a.kpad + 8;
   // stage weights
   {
     const int n16 = a.cout_pad * (a.kpad / 8);
-    for (int q = threadIdx.x; q < n16; q += 256) {
+    for (int q = threadIdx.x; q < n16; q += 64 * WAVES) {
       const int row = q / (a.kpad / 8), col = (q % (a.kpad / 8)) * 8;
       *reinterpret_cast<uint4*>(wsh + row * ldw + col) =
           *reinterpret_cast<const uint4*>(a.w + (long long)row * a.kpad + col);
@@ -101,11 +104,27 @@ __global__ __launch_bounds__(256) void conv_fwd_kernel(ConvArgs a) {
   const int g = t >> 4;       // k group
   const int r = t & 15;       // row / column within the tile
   const int x0 = blockIdx.x * 16 * NT;
-  const int yy = blockIdx.y * 4 + wv;
+  const int yy = blockIdx.y * WAVES + wv;
   const int z0 = blockIdx.z * a.zchunk;
   const int z1 = min(z0 + a.zchunk, a.nz);
   const int nx = a.nx, ny = a.ny, nz = a.nz, cin = a.cin;
   const bool rowok = yy < ny;
+  // the NT input fragments of K step k0 at plane zz (zero outside the grid)
+  auto load_b = [&](int zz, int k0, bf16x8* bf) {
+    const int k = k0 + 8 * g;
+    const int tap = k / cin;
+    const int c0 = k - tap * cin;
+    const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+    const int sz = zz + dz, sy = yy + dy;
+    const bool pl = tap < 27 && sz >= 0 && sz < nz && sy >= 0 && sy < ny;
+    const u16* rowp = a.x + ((long long)(pl ? sz : 0) * ny + (pl ? sy : 0)) * nx * cin;
+#pragma unroll
+    for (int n = 0; n < NT; ++n) {
+      const int sx = x0 + n * 16 + r + dx;
+      if (pl && sx >= 0 && sx < nx) bf[n] = load8(rowp + (long long)sx * cin + c0);
+      else bf[n] = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
+    }
+  };
   for (int zz = z0; zz < z1; ++zz) {
     f32x4 acc[MT][NT];
 #pragma unroll
@@ -113,30 +132,21 @@ __global__ __launch_bounds__(256) void conv_fwd_kernel(ConvArgs a) {
 #pragma unroll
       for (int n = 0; n < NT; ++n) acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
     if (rowok) {
+      bf16x8 cur[NT], nxt[NT];
+      load_b(zz, 0, cur);
       for (int k0 = 0; k0 < a.kpad; k0 += 32) {
+        if (k0 + 32 < a.kpad) load_b(zz, k0 + 32, nxt);
         const int k = k0 + 8 * g;
-        const int tap = k / cin;
-        const int c0 = k - tap * cin;
         bf16x8 af[MT];
 #pragma unroll
         for (int m = 0; m < MT; ++m) af[m] = load8(wsh + (m * 16 + r) * ldw + k);
-        const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
-        const int sz = zz + dz, sy = yy + dy;
-        const bool pl = tap < 27 && sz >= 0 && sz < nz && sy >= 0 && sy < ny;
-        const u16* rowp = a.x + ((long long)(pl ? sz : 0) * ny + (pl ? sy : 0)) * nx * cin;
 #pragma unroll
-        for (int n = 0; n < NT; ++n) {
-          const int sx = x0 + n * 16 + r + dx;
-          bf16x8 bf;
-          if (pl && sx >= 0 && sx < nx) {
-            bf = load8(rowp + (long long)sx * cin + c0);
-          } else {
-            bf = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
-          }
+        for (int n = 0; n < NT; ++n)
 #pragma unroll
           for (int m = 0; m < MT; ++m)
-            acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[m], bf, acc[m][n], 0, 0, 0);
-        }
+            acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[m], cur[n], acc[m][n], 0, 0, 0);
+#pragma unroll
+        for (int n = 0; n < NT; ++n) cur[n] = nxt[n];
       }
     }
     if (!rowok) continue;
@@ -390,8 +400,9 @@ int conv3d_fwd(const ConvDesc* d, const void* x, const void* w, const float* bia
   a.aux = (const u16*)aux;
   a.y = y;
   constexpr int NT = 4;
+  constexpr int WAVES = 16;
   const int gx = (d->nx + 16 * NT - 1) / (16 * NT);
-  const int gy = (d->ny + 3) / 4;
+  const int gy = (d->ny + WAVES - 1) / WAVES;
   long long blocks_xy = (long long)gx * gy;
   int zchunk = (int)((blocks_xy * d->nz + 2047) / 2048);  // ~2048 blocks
   zchunk = zchunk < 1 ? 1 : (zchunk > 8 ? 8 : zchunk);
@@ -401,7 +412,7 @@ int conv3d_fwd(const ConvDesc* d, const void* x, const void* w, const float* bia
   if (shm > 160 * 1024) return fail(-1, "weights exceed LDS");
   hipStream_t st = (hipStream_t)stream;
 #define CONV_LAUNCH(E, MT)                                                             \
-  hipLaunchKernelGGL((conv_fwd_kernel<E, NT, MT>), grid, dim3(256), shm, st, a)
+  hipLaunchKernelGGL((conv_fwd_kernel<E, NT, MT, WAVES>), grid, dim3(64 * WAVES), shm, st, a)
   const int mt = d->cout_pad / 16;
   switch (epi) {
     case CONV_EPI_GELU:
