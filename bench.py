@@ -213,7 +213,8 @@ def closure_bench(args, dev):
     train_seeds = list(range(args.closure_train))
     test_seeds = list(range(1000, 1000 + args.closure_test))
     t0 = time.time()
-    pkw = {"t0": args.closure_t0, "spread": args.closure_spread}
+    pkw = {"t0": args.closure_t0, "spread": args.closure_spread,
+           "waveform": args.closure_waveform}
     net, hist = CL.train_closure(dev, train_seeds, n_fine=nf, n_coarse=nf * 400 // 256,
                                  every=args.closure_every, epochs=args.closure_epochs,
                                  crop=min(64, nf // 2 - 16), lr=args.closure_lr,
@@ -235,7 +236,8 @@ def closure_bench(args, dev):
         "config": {"workload": "closure_c5", "fine": [nf] * 3, "coarse": [nf // 2] * 3,
                    "train_seeds": len(train_seeds), "test_seeds": len(test_seeds),
                    "epochs": args.closure_epochs, "correct_every": args.closure_every,
-                   "pulse_t0": args.closure_t0, "pulse_spread": args.closure_spread},
+                   "pulse_t0": args.closure_t0, "pulse_spread": args.closure_spread,
+                   "pulse": args.closure_waveform},
         "err_coarse_no_closure": round(float(np.mean(err_0)), 5),
         "err_traj_closure": round(float(np.mean(trj_c)), 5),
         "err_traj_coarse_no_closure": round(float(np.mean(trj_0)), 5),
@@ -271,6 +273,8 @@ def main() -> None:
                     help="pulse centre (coarse steps)")
     ap.add_argument("--closure-spread", type=float, default=15.0,
                     help="pulse width (coarse steps)")
+    ap.add_argument("--closure-waveform", default="gaussian",
+                    choices=["gaussian", "gaussian_derivative"])
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain copy of the field bytes (counter calibration)")
     args = ap.parse_args()

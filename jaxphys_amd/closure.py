@@ -226,16 +226,16 @@ class ClosureProblem:
     grids with random dielectric boxes (eps_r in [1, emax]), a CPML of
     ``npml`` fine cells and a z-directed Gaussian-pulse dipole.
 
-    The default pulse is the derivative of a Gaussian: its E increments sum
-    to zero, so the source leaves no static charge behind.  (A plain
-    Gaussian leaves a static near field around the source whose grid
-    representation differs between the resolutions by a fixed ratio; the
-    coarse-vs-fine error is then dominated by that remnant whatever the
-    pulse width.)"""
+    With the default Gaussian pulse the E increments do not sum to zero: the
+    source leaves a static near field behind, whose grid representation
+    differs between the two resolutions by a fixed ratio, and at late times
+    that remnant dominates the coarse-vs-fine difference.
+    ``waveform="gaussian_derivative"`` (zero-sum increments) removes it and
+    leaves the dispersive error of the travelling pulse."""
 
     def __init__(self, seed: int, n_fine: int = 256, npml_fine: int = 16, emax: float = 6.0,
                  nbox: int = 24, t0: float = 60.0, spread: float = 15.0,
-                 waveform: str = "gaussian_derivative", device=None):
+                 waveform: str = "gaussian", device=None):
         self.seed = seed
         self.nf = n_fine
         self.nc = n_fine // 2
