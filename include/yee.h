@@ -75,6 +75,13 @@ typedef struct YeeGridDesc {
    * i.e. 6*n_a floats per axis, axes concatenated x, y, z.                 */
   int32_t npml;
   const float* cpml;
+  /* Optional material palette (no reference counterpart): when `mat` is not
+   * NULL, mat[cell] (uint8) indexes `palette` = cb[256] then ca[256] (the
+   * ca half is read only when `ca` is set) holding exactly the values of the
+   * cb / ca arrays, which stay required.  The fused variants 3-4 and the
+   * two-pass E kernel then read 1 byte per cell instead of 4 or 8.         */
+  const uint8_t* mat;
+  const float* palette;
 } YeeGridDesc;
 
 typedef struct YeePlan YeePlan;
@@ -109,6 +116,9 @@ int yee_plan_set_variant(YeePlan* plan, int32_t variant);
  * adjoint kernels, 0 for the fused kernels).  Results do not depend on
  * it.                                                                    */
 int yee_plan_set_tile_order(YeePlan* plan, int32_t xcd_aware);
+/* Replace the palette of the plan (both NULL: per-cell arrays only), e.g.
+ * after the cb / ca arrays were changed in place.                        */
+int yee_plan_set_palette(YeePlan* plan, const uint8_t* mat, const float* palette);
 /* y rows (wavefronts) per workgroup tile of fused variants 2 and 3: 4
  * (default), 8 or 16 (no reference counterpart; results do not depend on
  * it).                                                                   */

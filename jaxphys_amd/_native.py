@@ -38,6 +38,7 @@ class YeeGridDesc(ctypes.Structure):
         ("sponge_dt", ctypes.c_float), ("sponge_eps0", ctypes.c_float),
         ("npml", ctypes.c_int32),
         ("cpml", ctypes.c_void_p),
+        ("mat", ctypes.c_void_p), ("palette", ctypes.c_void_p),
     ]
 
 
@@ -51,6 +52,7 @@ EXPORTS = {
     "yee_plan_set_variant": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "yee_plan_set_tile_order": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "yee_plan_set_fused_rows": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "yee_plan_set_palette": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "yee_set_sources": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32]),
     "yee_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
@@ -174,6 +176,14 @@ class Plan:
         XCD-contiguous; 2: XCD-banded."""
         v = -1 if order is None else int(order)
         check(lib().yee_plan_set_tile_order(self.handle, v), "yee_plan_set_tile_order")
+
+    def set_palette(self, mat, palette) -> None:
+        """uint8 material ids per cell and the 512-float (cb | ca) table, or
+        None, None for the per-cell arrays only."""
+        check(lib().yee_plan_set_palette(self.handle, None if mat is None else ptr(mat),
+                                         None if palette is None else ptr(palette)),
+              "yee_plan_set_palette")
+        self._palette_refs = (mat, palette)
 
     def set_fused_rows(self, rows: int) -> None:
         check(lib().yee_plan_set_fused_rows(self.handle, int(rows)), "yee_plan_set_fused_rows")

@@ -152,7 +152,13 @@ __global__ __launch_bounds__(kThreads) void yee_e_kernel(GridArgs g, SrcArgs s, 
                                                         const float* __restrict__ hy,
                                                         const float* __restrict__ hz) {
   int bx, by, bz;
-  if (!tile_of(tl, bx, by, bz)) return;
+  if (!tile_of(tl, bx, by, bz)) return;  // (the whole workgroup)
+  // material palette in LDS
+  __shared__ float spal[512];
+  if (g.mat) {
+    for (int q = threadIdx.x; q < 512; q += kThreads) spal[q] = g.pal[q];
+    __syncthreads();
+  }
   const int i = bx * kBX + (threadIdx.x % kBX);
   const int j = by * kBY + (threadIdx.x / kBX);
   const int nx = g.nx, ny = g.ny, nz = g.nz;
@@ -183,7 +189,7 @@ __global__ __launch_bounds__(kThreads) void yee_e_kernel(GridArgs g, SrcArgs s, 
   if (MODE == kSponge) sxy = g.ssx[i] + g.ssy[j];
   const unsigned dic = xin ? di : 0u;
   const unsigned djc = yin ? dj : 0u;
-  float c_hx0, c_hy0, c_hz0, c_hzj, c_hxj, c_hzi, c_hyi, c_cb, c_ca, c_ex0, c_ey0, c_ez0;
+  float c_hx0, c_hy0, c_hz0, c_hzj, c_hxj, c_hzi, c_hyi, c_cb, c_ca = 1.f, c_ex0, c_ey0, c_ez0;
   {
     c_hx0 = hx[idx];
     c_hy0 = hy[idx];
@@ -192,8 +198,12 @@ __global__ __launch_bounds__(kThreads) void yee_e_kernel(GridArgs g, SrcArgs s, 
     c_hxj = hx[idx + djc];
     c_hzi = hz[idx + dic];
     c_hyi = hy[idx + dic];
-    c_cb = g.cb[idx];
-    c_ca = LOSSY ? g.ca[idx] : 1.f;
+    if (g.mat) {
+      c_cb = __uint_as_float(g.mat[idx]);  // material id, looked up at use
+    } else {
+      c_cb = g.cb[idx];
+      c_ca = LOSSY ? g.ca[idx] : 1.f;
+    }
     c_ex0 = ex[idx];
     c_ey0 = ey[idx];
     c_ez0 = ez[idx];
@@ -210,15 +220,25 @@ __global__ __launch_bounds__(kThreads) void yee_e_kernel(GridArgs g, SrcArgs s, 
       n_hxj = hx[q + djc];
       n_hzi = hz[q + dic];
       n_hyi = hy[q + dic];
-      n_cb = g.cb[q];
-      n_ca = LOSSY ? g.ca[q] : 1.f;
+      if (g.mat) {
+        n_cb = __uint_as_float(g.mat[q]);
+      } else {
+        n_cb = g.cb[q];
+        n_ca = LOSSY ? g.ca[q] : 1.f;
+      }
       n_ex0 = ex[q];
       n_ey0 = ey[q];
       n_ez0 = ez[q];
     }
     const bool zin = per || k > 0;
     const float hx0 = c_hx0, hy0 = c_hy0, hz0 = c_hz0;
-    const float cb = c_cb, ca = c_ca, ex0 = c_ex0, ey0 = c_ey0, ez0 = c_ez0;
+    float cb = c_cb, ca = c_ca;
+    if (g.mat) {
+      const unsigned m = __float_as_uint(c_cb);
+      cb = spal[m];
+      ca = LOSSY ? spal[256 + m] : 1.f;
+    }
+    const float ex0 = c_ex0, ey0 = c_ey0, ez0 = c_ez0;
     const bool zs = MODE == kCpml && (k < p || k >= nz - p);
     const int sz = zs ? slab_index(k, nz, p) : 0;
     float ax = hz0 - c_hzj, bx = hy0 - hym;  // fdtd3d.py:334-341
@@ -361,6 +381,8 @@ int validate(const YeeGridDesc* d) {
   if (d->boundary < YEE_BOUNDARY_WALLS || d->boundary > YEE_BOUNDARY_CPML)
     return fail(YEE_EINVAL, "unknown boundary code");
   if (!d->cb) return fail(YEE_EINVAL, "cb (ce/eps_r) array is required");
+  if ((d->mat == nullptr) != (d->palette == nullptr))
+    return fail(YEE_EINVAL, "mat and palette go together");
   if (d->boundary == YEE_BOUNDARY_SPONGE && !d->sponge_sigma)
     return fail(YEE_EINVAL, "sponge boundary needs sponge_sigma");
   if (d->boundary == YEE_BOUNDARY_CPML) {
@@ -507,6 +529,15 @@ int yee_plan_set_tile_order(YeePlan* p, int32_t xcd_aware) {
   if (!p) return fail(YEE_EINVAL, "null plan");
   if (xcd_aware < -1 || xcd_aware > 2) return fail(YEE_EINVAL, "tile order must be -1, 0, 1 or 2");
   p->xcd = xcd_aware;
+  return YEE_OK;
+}
+
+int yee_plan_set_palette(YeePlan* p, const uint8_t* mat, const float* palette) {
+  if (!p) return fail(YEE_EINVAL, "null plan");
+  if ((mat == nullptr) != (palette == nullptr))
+    return fail(YEE_EINVAL, "mat and palette go together");
+  p->d.mat = mat;
+  p->d.palette = palette;
   return YEE_OK;
 }
 

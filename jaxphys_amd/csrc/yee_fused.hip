@@ -40,6 +40,8 @@ struct FusedArgs {
   const float* cpml;            // x | y | z tables, 6 n_a floats each
   const float* pin;             // psi of buffer A
   float* pout;                  // psi of buffer B
+  const unsigned char* mat;     // material palette (variants 3-4; see YeeGridDesc)
+  const float* pal;
 };
 
 // psi slot offsets: y group (Exy Ezy Hxy Hzy), z group (Exz Eyz Hxz Hyz),
@@ -570,6 +572,7 @@ __device__ __forceinline__ void v2_plane(
 struct V3In {
   float ez, ex_yp, ez_yp, ex_zp, ey_zp, hx0, hy0, hz0;
   float ez_ym, ex_ym, ey_ym_zp, hx_ym0, hz_ym0, cb, ca;
+  unsigned mid;  // material id (palette)
   float ld_ey_xp, ld_ez_xp, ld_ey_ym_xp;
   float ez_xm, ey_xm, ex_xm_yp, ex_xm_zp, hy_xm0, hz_xm0;
 };
@@ -603,8 +606,12 @@ __device__ __forceinline__ void v3_load(
   L.ey_ym_zp = Ey[cym + zp];
   L.hx_ym0 = Hx[cym];
   L.hz_ym0 = Hz[cym];
-  L.cb = a.cb[c];
-  L.ca = LOSSY ? a.ca[c] : 1.f;
+  if (a.mat) {
+    L.mid = a.mat[c];
+  } else {
+    L.cb = a.cb[c];
+    L.ca = LOSSY ? a.ca[c] : 1.f;
+  }
   // edge lanes (exec-masked)
   L.ld_ey_xp = L.ld_ez_xp = L.ld_ey_ym_xp = 0.f;
   if (edge_r) {
@@ -630,7 +637,7 @@ __device__ __forceinline__ void v3_compute(
     const FusedArgs& a, float* __restrict__ oEx, float* __restrict__ oEy,
     float* __restrict__ oEz, float* __restrict__ oHx, float* __restrict__ oHy,
     float* __restrict__ oHz, const V2Col& q, bool lane0, bool edge_r, int k, unsigned c,
-    const V3In& L, V2Carry& r) {
+    const V3In& L, V2Carry& r, const float* spal) {
   constexpr bool per = MODE == kPeriodic;
   constexpr bool XS = MODE == kCpml && (SL & 1);
   constexpr bool YS = MODE == kCpml && (SL & 2);
@@ -643,7 +650,9 @@ __device__ __forceinline__ void v3_compute(
   const float ez = L.ez, ex_yp = L.ex_yp, ez_yp = L.ez_yp, ex_zp = L.ex_zp, ey_zp = L.ey_zp;
   const float hx0 = L.hx0, hy0 = L.hy0, hz0 = L.hz0;
   const float ez_ym = L.ez_ym, ex_ym = L.ex_ym, ey_ym_zp = L.ey_ym_zp;
-  const float hx_ym0 = L.hx_ym0, hz_ym0 = L.hz_ym0, cb = L.cb, ca = L.ca;
+  const float hx_ym0 = L.hx_ym0, hz_ym0 = L.hz_ym0;
+  const float cb = a.mat ? spal[L.mid] : L.cb;
+  const float ca = a.mat ? (LOSSY ? spal[256 + L.mid] : 1.f) : L.ca;
   const float ld_ey_xp = L.ld_ey_xp, ld_ez_xp = L.ld_ez_xp, ld_ey_ym_xp = L.ld_ey_ym_xp;
   const float ez_xm = L.ez_xm, ey_xm = L.ey_xm, ex_xm_yp = L.ex_xm_yp, ex_xm_zp = L.ex_xm_zp;
   const float hy_xm0 = L.hy_xm0, hz_xm0 = L.hz_xm0;
@@ -782,10 +791,11 @@ __device__ __forceinline__ void v3_plane(
     const float* __restrict__ Hz, float* __restrict__ oEx, float* __restrict__ oEy,
     float* __restrict__ oEz, float* __restrict__ oHx, float* __restrict__ oHy,
     float* __restrict__ oHz, const V2Col& q, bool lane0, bool edge_r, int k, unsigned c,
-    V2Carry& r) {
+    V2Carry& r, const float* spal) {
   V3In L;
   v3_load<MODE, LOSSY>(a, Ex, Ey, Ez, Hx, Hy, Hz, q, lane0, edge_r, k, c, L);
-  v3_compute<MODE, LOSSY, SL>(a, oEx, oEy, oEz, oHx, oHy, oHz, q, lane0, edge_r, k, c, L, r);
+  v3_compute<MODE, LOSSY, SL>(a, oEx, oEy, oEz, oHx, oHy, oHz, q, lane0, edge_r, k, c, L, r,
+                              spal);
 }
 
 // ---------------------------------------------------------------------------
@@ -816,7 +826,7 @@ __device__ __forceinline__ void v3s_plane(
     const float* __restrict__ Hz, float* __restrict__ oEx, float* __restrict__ oEy,
     float* __restrict__ oEz, float* __restrict__ oHx, float* __restrict__ oHy,
     float* __restrict__ oHz, const V2Col& q, const V3Tab& t, bool lane0, bool edge_r, int k,
-    unsigned c, V2Carry& r) {
+    unsigned c, V2Carry& r, const float* spal) {
   const int nx = a.nx, ny = a.ny, nz = a.nz, p = a.p;
   const unsigned P = (unsigned)nx * ny;
   const bool zinH = k < nz - 1;
@@ -833,8 +843,15 @@ __device__ __forceinline__ void v3s_plane(
   const unsigned cym = c + q.ym;
   const float ez_ym = Ez[cym], ex_ym = Ex[cym], ey_ym_zp = Ey[cym + zp];
   const float hx_ym0 = Hx[cym], hz_ym0 = Hz[cym];
-  const float cb = a.cb[c];
-  const float ca = LOSSY ? a.ca[c] : 1.f;
+  float cb, ca;
+  if (a.mat) {
+    const unsigned m = a.mat[c];
+    cb = spal[m];
+    ca = LOSSY ? spal[256 + m] : 1.f;
+  } else {
+    cb = a.cb[c];
+    ca = LOSSY ? a.ca[c] : 1.f;
+  }
   float ld_ey_xp = 0.f, ld_ez_xp = 0.f, ld_ey_ym_xp = 0.f;
   if (edge_r) {
     ld_ey_xp = Ey[c + q.xp];
@@ -1079,7 +1096,13 @@ __global__ __launch_bounds__(64 * ROWS) void yee_fused_v2_kernel(
     float* __restrict__ fo1, float* __restrict__ fo2, float* __restrict__ fo3,
     float* __restrict__ fo4, float* __restrict__ fo5) {
   TilePos tp;
-  if (!tile_of_set(ts, tp)) return;
+  if (!tile_of_set(ts, tp)) return;  // (the whole workgroup)
+  // material palette in LDS (variants 3-4)
+  __shared__ float spal[512];
+  if (VAR >= 3 && a.mat) {
+    for (int q = threadIdx.x; q < 512; q += 64 * ROWS) spal[q] = a.pal[q];
+    __syncthreads();
+  }
   const int nx = a.nx, ny = a.ny, nz = a.nz, p = a.p;
   constexpr bool per = MODE == kPeriodic;
   const int k0 = tp.k0, k1 = tp.k1;
@@ -1151,11 +1174,11 @@ __global__ __launch_bounds__(64 * ROWS) void yee_fused_v2_kernel(
   }
 #define YEE_V3S_PLANE(XSB)                                                                     \
   v3s_plane<MODE, LOSSY, XSB>(a, fi0, fi1, fi2, fi3, fi4, fi5, fo0, fo1, fo2, fo3, fo4, fo5, q, \
-                              tb, lane0, edge_r, k, c, r)
+                              tb, lane0, edge_r, k, c, r, spal)
 #define YEE_V2_PLANE(S)                                                                       \
   if constexpr (VAR >= 3)                                                                     \
     v3_plane<MODE, LOSSY, S>(a, fi0, fi1, fi2, fi3, fi4, fi5, fo0, fo1, fo2, fo3, fo4, fo5, q, \
-                             lane0, edge_r, k, c, r);                                         \
+                             lane0, edge_r, k, c, r, spal);                                   \
   else                                                                                        \
     v2_plane<MODE, LOSSY, S>(a, fi0, fi1, fi2, fi3, fi4, fi5, fo0, fo1, fo2, fo3, fo4, fo5, q, \
                              k, c, r)
@@ -1167,7 +1190,8 @@ __global__ __launch_bounds__(64 * ROWS) void yee_fused_v2_kernel(
     for (int k = k0; k < k1; ++k, c += P) {
       if (k + 1 < k1)
         v3_load<MODE, LOSSY>(a, fi0, fi1, fi2, fi3, fi4, fi5, q, lane0, edge_r, k + 1, c + P, nxt);
-      v3_compute<MODE, LOSSY, 0>(a, fo0, fo1, fo2, fo3, fo4, fo5, q, lane0, edge_r, k, c, cur, r);
+      v3_compute<MODE, LOSSY, 0>(a, fo0, fo1, fo2, fo3, fo4, fo5, q, lane0, edge_r, k, c, cur, r,
+                                 spal);
       cur = nxt;
     }
   } else
@@ -1292,6 +1316,8 @@ int launch_fused(const YeePlan* p, float* const* fin, float* const* fout, float*
   }
   a.cb = d.cb;
   a.ca = d.ca;
+  a.mat = d.mat;
+  a.pal = d.mat ? d.palette : nullptr;
   a.sponge = d.boundary == YEE_BOUNDARY_SPONGE ? d.sponge_sigma : nullptr;
   a.cpml = d.boundary == YEE_BOUNDARY_CPML ? d.cpml : nullptr;
   a.pin = pin;

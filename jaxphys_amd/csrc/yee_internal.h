@@ -62,6 +62,8 @@ struct GridArgs {
   const float* cpy;
   const float* cpz;
   float* psi[kNumPsi];
+  const unsigned char* mat;  // material palette (see YeeGridDesc)
+  const float* pal;
 };
 
 struct SrcArgs {
@@ -339,6 +341,8 @@ inline GridArgs make_args(const YeePlan* p, float* const* fields, float* psi) {
   for (int c = 0; c < 6; ++c) g.f[c] = fields[c];
   g.cb = d.cb;
   g.ca = d.ca;
+  g.mat = d.mat;
+  g.pal = d.mat ? d.palette : nullptr;
   if (d.boundary == YEE_BOUNDARY_SPONGE) {
     g.ssx = d.sponge_sigma;
     g.ssy = d.sponge_sigma + d.nx;

@@ -83,7 +83,8 @@ class YeeSolver:
 
     def __init__(self, size, dx: float, dt: float, boundary: str, pml_layers: int,
                  eps_r: torch.Tensor, sigma: torch.Tensor | None = None,
-                 cpml: dict | None = None, device=None, fused: bool = True):
+                 cpml: dict | None = None, device=None, fused: bool = True,
+                 palette: bool = True):
         self.device = torch.device(device) if device is not None else _native.require_device()
         _native.lib()
         self.nx, self.ny, self.nz = (int(s) for s in size)
@@ -107,6 +108,9 @@ class YeeSolver:
         self.eps_r = torch.empty_like(eps)
         self.cb = torch.empty_like(eps)
         self.ca = torch.empty_like(eps) if self.sigma is not None else None
+        self.plan = None
+        self.use_palette = bool(palette)
+        self.mat = self.palette = None
         self.set_eps(eps)
         desc = _native.YeeGridDesc()
         desc.nx, desc.ny, desc.nz = nx, ny, nz
@@ -136,6 +140,7 @@ class YeeSolver:
         else:  # any other string: the reference's non-periodic, undamped update
             desc.boundary = _native.BOUNDARY_WALLS
         self.plan = _native.Plan(desc, keep=[self.cb, self.ca, self.sponge, self.cpml])
+        self._update_palette()
         npsi = self.plan.psi_floats()
         # two state buffers: the fused kernel reads one and writes the other
         self.fused = bool(fused)
@@ -164,6 +169,33 @@ class YeeSolver:
             lossy = self.sigma != 0
             self.ca.copy_(torch.where(lossy, (1.0 - x) / (1.0 + x), torch.ones_like(x)))
             self.cb.copy_(torch.where(lossy, ((ce / e64) / (1.0 + x)).to(torch.float32), self.cb))
+        if self.plan is not None:
+            self._update_palette()
+
+    def _update_palette(self) -> None:
+        """When the grid holds at most 256 distinct (cb, ca) pairs (bit
+        patterns), give the plan a uint8 material id per cell and the table
+        of those exact values: the fused and two-pass E updates then read one
+        byte per cell instead of four or eight."""
+        self.mat = self.palette = None
+        if self.use_palette:
+            bits = self.cb.view(torch.int32).reshape(-1)
+            if self.ca is not None:
+                key = torch.stack([bits, self.ca.view(torch.int32).reshape(-1)], dim=1)
+                uniq, inv = torch.unique(key, dim=0, return_inverse=True)
+            else:
+                uniq, inv = torch.unique(bits, return_inverse=True)
+                uniq = uniq.view(-1, 1)
+            m = uniq.shape[0]
+            if m <= 256:
+                pal = torch.zeros(512, dtype=torch.int32, device=self.device)
+                pal[:m] = uniq[:, 0]
+                if self.ca is not None:
+                    pal[256:256 + m] = uniq[:, 1]
+                self.palette = pal.view(torch.float32)
+                self.mat = inv.to(torch.uint8).view(self.cb.shape).contiguous()
+                del inv, uniq
+        self.plan.set_palette(self.mat, self.palette)
 
     def d_coeffs_d_eps(self, region=None):
         """(dCb/deps_r, dCa/deps_r) per cell (float64) for the adjoint chain

@@ -36,6 +36,8 @@ def _desc(**kw):
     d.npml = kw.get("npml", 0)
     d.cpml = kw.get("cpml", None)
     d.sponge_sigma = kw.get("sponge", None)
+    d.mat = kw.get("mat", None)
+    d.palette = kw.get("palette", None)
     return d
 
 
@@ -54,6 +56,7 @@ def test_psi_floats_formula():
     (dict(boundary=_native.BOUNDARY_SPONGE), "sponge"),
     (dict(boundary=_native.BOUNDARY_CPML, npml=6, cpml=0x2000), "overlap"),
     (dict(boundary=_native.BOUNDARY_CPML, npml=2), "tables"),
+    (dict(mat=0x3000), "palette"),
 ])
 def test_plan_create_rejects_bad_descriptors(kw, msg):
     d = _desc(**kw)

@@ -254,6 +254,9 @@ def test_fused_step_bitwise_equals_two_pass(gpu, boundary, lossy, size, variant,
     n = 41
     a = g.build_solver(dt, n)
     b = g.build_solver(dt, n)
+    # per-cell coefficient arrays on one side, the material palette on the
+    # other (when the grid has few enough materials)
+    b.plan.set_palette(None, None)
     a.plan.set_variant(variant)
     a.plan.set_fused_rows(rows)
     a.plan.set_tile_order(xcd)
@@ -267,3 +270,29 @@ def test_fused_step_bitwise_equals_two_pass(gpu, boundary, lossy, size, variant,
     if a.psi is not None:
         assert torch.equal(a.psi, b.psi)
     assert float(a.fields[2].abs().max()) > 0
+
+
+@pytest.mark.parametrize("boundary,lossy", [("cpml", True), ("absorbing", False)])
+def test_two_pass_palette_bitwise_equals_arrays(gpu, boundary, lossy):
+    """The E kernel reading uint8 material ids and the LDS table gives the
+    same bits as reading the per-cell cb / ca arrays."""
+    size = (70, 37, 29)
+    rng = np.random.default_rng(5)
+    eps = _boxes(size, rng, 10, 3, 14, 1.0, 9.0)
+    sigma = None
+    if lossy:
+        sigma = np.zeros(size)
+        sigma[:, 10:20, :] = 0.03 * R.EPS0 * R.C0
+    g = EMGrid3D(size=size, resolution=1.0, boundary=boundary, pml_layers=6)
+    g.add_material(DielectricRegion(mask=np.ones(size, bool), epsilon_r=eps, sigma=sigma))
+    g.add_source(PointSource3D(0.0, (30, 18, 14), 1.0, "z", waveform="gaussian_sine",
+                               t0=20, spread=8, wavelength=12))
+    dt = 0.5 / math.sqrt(3.0) / R.C0
+    a = g.build_solver(dt, 30)
+    b = g.build_solver(dt, 30)
+    assert a.mat is not None and a.palette is not None
+    b.plan.set_palette(None, None)
+    a.run_two_pass(0, 30)
+    b.run_two_pass(0, 30)
+    for q in range(6):
+        assert torch.equal(a.fields[q], b.fields[q]), NAMES[q]
