@@ -53,7 +53,10 @@ def cpu_baseline(solver, budget_s: float):
     cb = solver.cb.cpu().numpy()
     ca = solver.ca.cpu().numpy() if solver.ca is not None else None
     cp = solver.cpml.cpu().numpy() if solver.cpml is not None else None
-    tab = solver.source_table
+    # the source values do not change the cost of a step: the table is
+    # repeated so that the sample can run for the whole budget
+    tab = np.resize(solver.source_table, (max(4096, solver.source_table.shape[0]),
+                                          solver.source_table.shape[1]))
     comps = solver.source_comps
     nx, ny, _ = solver.nx, solver.ny, solver.nz
     sidx = [(k * ny + j) * nx + i for i, j, k in solver.source_positions]

@@ -843,11 +843,12 @@ __device__ __forceinline__ void v3s_plane(
   const unsigned cym = c + q.ym;
   const float ez_ym = Ez[cym], ex_ym = Ex[cym], ey_ym_zp = Ey[cym + zp];
   const float hx_ym0 = Hx[cym], hz_ym0 = Hz[cym];
-  float cb, ca;
+  // coefficients: per-cell arrays, or a material id looked up in the LDS
+  // table at the E update
+  float cb = 0.f, ca = 1.f;
+  unsigned mid = 0;
   if (a.mat) {
-    const unsigned m = a.mat[c];
-    cb = spal[m];
-    ca = LOSSY ? spal[256 + m] : 1.f;
+    mid = a.mat[c];
   } else {
     cb = a.cb[c];
     ca = LOSSY ? a.ca[c] : 1.f;
@@ -1026,6 +1027,10 @@ __device__ __forceinline__ void v3s_plane(
   oHy[c] = hy;
   oHz[c] = hz;
 
+  if (a.mat) {
+    cb = spal[mid];
+    ca = LOSSY ? spal[256 + mid] : 1.f;
+  }
   // ---- E^{n+1} of the column (fdtd3d.py:332-359)
   float vx, vy, vz;
   {

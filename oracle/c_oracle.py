@@ -164,6 +164,7 @@ def time_arrays(size, cb, ca, cpml_tabs, npml, ch, sources_table, comps, sidx, b
                              n0, n, threads)
         return time.perf_counter() - t
 
-    t1 = go(0, 1)  # also first-touch of the arrays
-    n = max(1, min(tab.shape[0] - 1, int(budget_s / max(t1, 1e-6))))
-    return n, go(1, n)
+    go(0, 1)  # first touch of the arrays
+    t2 = go(1, 2) / 2.0
+    n = max(1, min(tab.shape[0] - 3, int(budget_s / max(t2, 1e-6))))
+    return n, go(3, n)
