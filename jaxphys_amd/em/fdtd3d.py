@@ -242,6 +242,7 @@ class EMGrid3D:
         *,
         n_steps: int | None = None,
         energy_every: int = 0,
+        snapshots: str = "device",
     ) -> EMFieldHistory3D:
         """Run the 3D FDTD simulation (fdtd3d.py:169-417).
 
@@ -252,6 +253,10 @@ class EMGrid3D:
             n_steps: extension -- explicit step count (overrides t_span).
             energy_every: extension -- record the float64 field energy every
                 N steps in ``self.energy_log`` (0 = off).
+            snapshots: extension -- "device" keeps the saved steps in GPU
+                memory; "host" streams them into pinned host memory while
+                the run continues (a two-slot device staging ring, copies on
+                a side stream), for histories larger than HBM.
 
         Returns:
             EMFieldHistory3D with time-series of all six field components.
@@ -278,6 +283,10 @@ class EMGrid3D:
             nx, ny, nz, dt, n_steps,
         )
 
+        if snapshots not in ("device", "host"):
+            raise ConfigurationError(
+                f"snapshots must be 'device' or 'host', got {snapshots!r}"
+            )
         solver = self.build_solver(dt, n_steps, t_start)
         dev = solver.device
         # fdtd3d.py:395-404: ys[0::save_every] (all steps when save_every <= 1)
@@ -286,8 +295,17 @@ class EMGrid3D:
         else:
             saves = list(range(n_steps))
         n_snap = len(saves)
-        hist = [torch.empty((n_snap, nz, ny, nx), dtype=torch.float32, device=dev)
-                for _ in range(6)]
+        host = snapshots == "host"
+        if host:
+            hist = [torch.empty((n_snap, nz, ny, nx), dtype=torch.float32, pin_memory=True)
+                    for _ in range(6)]
+            stage = [[torch.empty((nz, ny, nx), dtype=torch.float32, device=dev)
+                      for _ in range(6)] for _ in range(2)]
+            copy_stream = torch.cuda.Stream(device=dev)
+            slot_free = [None, None]
+        else:
+            hist = [torch.empty((n_snap, nz, ny, nx), dtype=torch.float32, device=dev)
+                    for _ in range(6)]
         self.energy_log = []
         marks = sorted(set(saves) | ({*range(energy_every - 1, n_steps, energy_every)}
                                      if energy_every > 0 else set()))
@@ -298,12 +316,31 @@ class EMGrid3D:
             cur = n + 1
             if n in save_pos:
                 q = save_pos[n]
-                for c in range(6):
-                    hist[c][q].copy_(solver.fields[c])
+                if host:
+                    slot = q % 2
+                    compute = torch.cuda.current_stream(dev)
+                    if slot_free[slot] is not None:
+                        compute.wait_event(slot_free[slot])
+                    for c in range(6):
+                        stage[slot][c].copy_(solver.fields[c])
+                    ready = torch.cuda.Event()
+                    ready.record(compute)
+                    copy_stream.wait_event(ready)
+                    with torch.cuda.stream(copy_stream):
+                        for c in range(6):
+                            hist[c][q].copy_(stage[slot][c], non_blocking=True)
+                    slot_free[slot] = torch.cuda.Event()
+                    slot_free[slot].record(copy_stream)
+                else:
+                    for c in range(6):
+                        hist[c][q].copy_(solver.fields[c])
             if energy_every > 0 and (n + 1) % energy_every == 0:
                 self.energy_log.append((n + 1, solver.em_energy_joules()))
         solver.run(cur, n_steps - cur)
         self.last_solver = solver
+        if host:
+            copy_stream.synchronize()
+            dev = torch.device("cpu")
 
         # fdtd3d.py:288, 378: t = t_c + dt with t_c = t_start + n*dt in float32
         f32 = np.float32

@@ -296,3 +296,23 @@ def test_two_pass_palette_bitwise_equals_arrays(gpu, boundary, lossy):
     b.run_two_pass(0, 30)
     for q in range(6):
         assert torch.equal(a.fields[q], b.fields[q]), NAMES[q]
+
+
+def test_host_snapshots_equal_device_snapshots(gpu):
+    """simulate(snapshots="host") streams the saved steps into pinned host
+    memory on a side stream; the values are those of the device path."""
+    size = (40, 24, 20)
+    srcs = [R.Src(5e9, (20, 12, 10)), R.Src(7e9, (9, 5, 14), 0.5, "x")]
+    dt = R.default_dt(0.01, np.float32)
+    out = []
+    for mode in ("device", "host"):
+        g = EMGrid3D(size=size, resolution=0.01, boundary="absorbing", pml_layers=4)
+        for s in srcs:
+            g.add_source(PointSource3D(s.frequency, s.position, s.amplitude, s.polarization))
+        out.append(g.simulate(dt=dt, save_every=6, n_steps=40, snapshots=mode))
+    dev_h, host_h = out
+    assert host_h.ez.device.type == "cpu"
+    assert host_h.ez.shape == dev_h.ez.shape == (7, 40, 24, 20)
+    for c in NAMES:
+        assert torch.equal(getattr(host_h, c), getattr(dev_h, c).cpu()), c
+    assert torch.equal(host_h.t, dev_h.t.cpu())
