@@ -52,7 +52,8 @@ class AdjointProblem:
         self.n_steps = int(n_steps)
         self.K = max(1, int(checkpoint_every))
         self.dt = float(dt)
-        sol = grid.build_solver(self.dt, self.n_steps)
+        # the design eps changes every evaluation: per-cell arrays only
+        sol = grid.build_solver(self.dt, self.n_steps, palette=False)
         self.solver = sol
         dev = sol.device
         nx, ny, nz = sol.nx, sol.ny, sol.nz

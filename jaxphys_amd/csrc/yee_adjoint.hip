@@ -107,10 +107,18 @@ __device__ __forceinline__ bool h_upd(const AdjArgs& a, int i, int j, int k) {
                 : (C == 1 ? (i < a.nx - 1 && k < a.nz - 1) : (i < a.nx - 1 && j < a.ny - 1));
 }
 
+// Which psi terms a launch compiles in (CPX): 0 none, 1 the y and z terms
+// (j is one value per wavefront and k one per plane, so their slab tests
+// are uniform branches), 2 all (x terms tested per lane).
+template <int CPX, int AX>
+__device__ __forceinline__ constexpr bool axis_on() {
+  return AX == 0 ? CPX >= 2 : CPX >= 1;
+}
+
 // dJ/d(term a), dJ/d(term b) of E component C at the in-grid cell
 // (i, j, k) from the adjoint of E^{n+1} and of psi_E^{n+1}; zero when
-// `valid` is false or the cell is not updated.  CP: CPML code compiled in.
-template <int MODE, bool CP, int C>
+// `valid` is false or the cell is not updated.
+template <int MODE, int CPX, int C>
 __device__ __forceinline__ void eterm(const AdjArgs& a, const float* __restrict__ lamE, int i,
                                       int j, int k, bool valid, float& ga, float& gb) {
   using T = Terms<C>;
@@ -118,16 +126,16 @@ __device__ __forceinline__ void eterm(const AdjArgs& a, const float* __restrict_
   const float gU = a.cb[c] * (adj_damp<MODE>(a, i, j, k) * lamE[c]);
   ga = gU;
   gb = -gU;
-  if (CP) {
+  {
     const int p = a.p;
     const int va = coord<T::axa>(i, j, k), na = extent<T::axa>(a);
     const int vb = coord<T::axb>(i, j, k), nb = extent<T::axb>(a);
-    if (va < p || va >= na - p) {
+    if (axis_on<CPX, T::axa>() && (va < p || va >= na - p)) {
       const float* t = adj_tab<T::axa>(a);
       const float L = *adj_psi<T::ea>(a, i, j, k) + gU;
       ga = t[2 * na + va] * gU + t[1 * na + va] * L;
     }
-    if (vb < p || vb >= nb - p) {
+    if (axis_on<CPX, T::axb>() && (vb < p || vb >= nb - p)) {
       const float* t = adj_tab<T::axb>(a);
       const float L = *adj_psi<T::eb>(a, i, j, k) - gU;
       gb = -t[2 * nb + vb] * gU + t[1 * nb + vb] * L;
@@ -140,7 +148,7 @@ __device__ __forceinline__ void eterm(const AdjArgs& a, const float* __restrict_
 }
 
 // same for H component C from the adjoint of H^{n+1/2} and psi_H^{n+1/2}
-template <int MODE, bool CP, int C>
+template <int MODE, int CPX, int C>
 __device__ __forceinline__ void hterm(const AdjArgs& a, const float* __restrict__ lamH, int i,
                                       int j, int k, bool valid, float& ga, float& gb) {
   using T = Terms<C>;
@@ -148,16 +156,16 @@ __device__ __forceinline__ void hterm(const AdjArgs& a, const float* __restrict_
   const float gU = -a.ch * lamH[c];
   ga = gU;
   gb = -gU;
-  if (CP) {
+  {
     const int p = a.p;
     const int va = coord<T::axa>(i, j, k), na = extent<T::axa>(a);
     const int vb = coord<T::axb>(i, j, k), nb = extent<T::axb>(a);
-    if (va < p || va >= na - p) {
+    if (axis_on<CPX, T::axa>() && (va < p || va >= na - p)) {
       const float* t = adj_tab<T::axa>(a);
       const float L = *adj_psi<T::ha>(a, i, j, k) + gU;
       ga = t[5 * na + va] * gU + t[4 * na + va] * L;
     }
-    if (vb < p || vb >= nb - p) {
+    if (axis_on<CPX, T::axb>() && (vb < p || vb >= nb - p)) {
       const float* t = adj_tab<T::axb>(a);
       const float L = *adj_psi<T::hb>(a, i, j, k) - gU;
       gb = -t[5 * nb + vb] * gU + t[4 * nb + vb] * L;
@@ -170,7 +178,7 @@ __device__ __forceinline__ void hterm(const AdjArgs& a, const float* __restrict_
 }
 
 // deferred psi_H-adjoint update of H component C at (i, j, k)
-template <int MODE, int C>
+template <int MODE, int CPX, int C>
 __device__ __forceinline__ void psi_h_defer(const AdjArgs& a, float lamh, int i, int j, int k) {
   using T = Terms<C>;
   if (!h_upd<MODE, C>(a, i, j, k)) return;
@@ -178,28 +186,28 @@ __device__ __forceinline__ void psi_h_defer(const AdjArgs& a, float lamh, int i,
   const float gU = -a.ch * lamh;
   const int va = coord<T::axa>(i, j, k), na = extent<T::axa>(a);
   const int vb = coord<T::axb>(i, j, k), nb = extent<T::axb>(a);
-  if (va < p || va >= na - p) {
+  if (axis_on<CPX, T::axa>() && (va < p || va >= na - p)) {
     float* q = adj_psi<T::ha>(a, i, j, k);
     *q = adj_tab<T::axa>(a)[3 * na + va] * (*q + gU);
   }
-  if (vb < p || vb >= nb - p) {
+  if (axis_on<CPX, T::axb>() && (vb < p || vb >= nb - p)) {
     float* q = adj_psi<T::hb>(a, i, j, k);
     *q = adj_tab<T::axb>(a)[3 * nb + vb] * (*q - gU);
   }
 }
 
 // psi_E-adjoint update of E component C at (i, j, k) (updated cells only)
-template <int C>
+template <int CPX, int C>
 __device__ __forceinline__ void psi_e_update(const AdjArgs& a, float gU, int i, int j, int k) {
   using T = Terms<C>;
   const int p = a.p;
   const int va = coord<T::axa>(i, j, k), na = extent<T::axa>(a);
   const int vb = coord<T::axb>(i, j, k), nb = extent<T::axb>(a);
-  if (va < p || va >= na - p) {
+  if (axis_on<CPX, T::axa>() && (va < p || va >= na - p)) {
     float* q = adj_psi<T::ea>(a, i, j, k);
     *q = adj_tab<T::axa>(a)[va] * (*q + gU);
   }
-  if (vb < p || vb >= nb - p) {
+  if (axis_on<CPX, T::axb>() && (vb < p || vb >= nb - p)) {
     float* q = adj_psi<T::eb>(a, i, j, k);
     *q = adj_tab<T::axb>(a)[vb] * (*q - gU);
   }
@@ -221,9 +229,10 @@ __device__ __forceinline__ int nb_idx(int v, int n, bool& ok) {
   return v < 0 ? 0 : n - 1;
 }
 
-// Kernel A: per H cell, z-marching tiles.  CP: the tile may touch a CPML
-// slab (psi code compiled in); otherwise it touches none within one cell.
-template <int MODE, bool CP>
+// Kernel A: per H cell, z-marching tiles.  CPX as axis_on: 0 for tiles with
+// no slab within one cell, 1 for tiles whose columns are at least one cell
+// away from the x slabs, 2 for any tile.
+template <int MODE, int CPX>
 __global__ __launch_bounds__(kThreads) void yee_adj_h_kernel(
     AdjArgs a, TileSet ts, const float* __restrict__ lex, const float* __restrict__ ley,
     const float* __restrict__ lez, float* __restrict__ lhx, float* __restrict__ lhy,
@@ -233,7 +242,8 @@ __global__ __launch_bounds__(kThreads) void yee_adj_h_kernel(
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const int k0 = tp.k0, k1 = tp.k1;
   const int i = tp.i0 + (int)(threadIdx.x % kBX);
-  const int j = tp.j0 + (int)(threadIdx.x / kBX);
+  // one y row per wavefront: j is scalar, so y-slab tests are uniform
+  const int j = __builtin_amdgcn_readfirstlane(tp.j0 + (int)(threadIdx.x / kBX));
   if (i >= tp.xl || j >= tp.yl) return;
   bool vx, vy;
   const int ip = nb_idx<MODE>(i + 1, nx, vx);
@@ -243,41 +253,41 @@ __global__ __launch_bounds__(kThreads) void yee_adj_h_kernel(
     const int kp = nb_idx<MODE>(k + 1, nz, vz);
     const unsigned c = ((unsigned)k * ny + j) * nx + i;
     const float hx0 = lhx[c], hy0 = lhy[c], hz0 = lhz[c];
-    if (CP) {
-      psi_h_defer<MODE, 0>(a, hx0, i, j, k);
-      psi_h_defer<MODE, 1>(a, hy0, i, j, k);
-      psi_h_defer<MODE, 2>(a, hz0, i, j, k);
+    if (CPX > 0) {
+      psi_h_defer<MODE, CPX, 0>(a, hx0, i, j, k);
+      psi_h_defer<MODE, CPX, 1>(a, hy0, i, j, k);
+      psi_h_defer<MODE, CPX, 2>(a, hz0, i, j, k);
     }
     float xa, xb, ya, yb, za, zb, t0, t1;
-    eterm<MODE, CP, 0>(a, lex, i, j, k, true, xa, xb);
-    eterm<MODE, CP, 1>(a, ley, i, j, k, true, ya, yb);
-    eterm<MODE, CP, 2>(a, lez, i, j, k, true, za, zb);
+    eterm<MODE, CPX, 0>(a, lex, i, j, k, true, xa, xb);
+    eterm<MODE, CPX, 1>(a, ley, i, j, k, true, ya, yb);
+    eterm<MODE, CPX, 2>(a, lez, i, j, k, true, za, zb);
     // Hx: Ey's a (d/dz Hx) at c and c+z; Ez's b (d/dy Hx) at c and c+y
     float s = hx0;
-    eterm<MODE, CP, 1>(a, ley, i, j, kp, vz, t0, t1);
+    eterm<MODE, CPX, 1>(a, ley, i, j, kp, vz, t0, t1);
     s += ya - t0;
-    eterm<MODE, CP, 2>(a, lez, i, jp, k, vy, t0, t1);
+    eterm<MODE, CPX, 2>(a, lez, i, jp, k, vy, t0, t1);
     s += zb - t1;
     lhx[c] = s;
     // Hy: Ex's b (d/dz Hy) at c and c+z; Ez's a (d/dx Hy) at c and c+x
     s = hy0;
-    eterm<MODE, CP, 0>(a, lex, i, j, kp, vz, t0, t1);
+    eterm<MODE, CPX, 0>(a, lex, i, j, kp, vz, t0, t1);
     s += xb - t1;
-    eterm<MODE, CP, 2>(a, lez, ip, j, k, vx, t0, t1);
+    eterm<MODE, CPX, 2>(a, lez, ip, j, k, vx, t0, t1);
     s += za - t0;
     lhy[c] = s;
     // Hz: Ex's a (d/dy Hz) at c and c+y; Ey's b (d/dx Hz) at c and c+x
     s = hz0;
-    eterm<MODE, CP, 0>(a, lex, i, jp, k, vy, t0, t1);
+    eterm<MODE, CPX, 0>(a, lex, i, jp, k, vy, t0, t1);
     s += xa - t0;
-    eterm<MODE, CP, 1>(a, ley, ip, j, k, vx, t0, t1);
+    eterm<MODE, CPX, 1>(a, ley, ip, j, k, vx, t0, t1);
     s += yb - t1;
     lhz[c] = s;
   }
 }
 
-// Kernel B: per E cell, z-marching tiles.
-template <int MODE, bool LOSSY, bool CP>
+// Kernel B: per E cell, z-marching tiles (CPX as in kernel A).
+template <int MODE, bool LOSSY, int CPX>
 __global__ __launch_bounds__(kThreads) void yee_adj_e_kernel(
     AdjArgs a, TileSet ts, float* __restrict__ lex, float* __restrict__ ley,
     float* __restrict__ lez, const float* __restrict__ lhx, const float* __restrict__ lhy,
@@ -287,7 +297,8 @@ __global__ __launch_bounds__(kThreads) void yee_adj_e_kernel(
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const int k0 = tp.k0, k1 = tp.k1;
   const int i = tp.i0 + (int)(threadIdx.x % kBX);
-  const int j = tp.j0 + (int)(threadIdx.x / kBX);
+  // one y row per wavefront: j is scalar, so y-slab tests are uniform
+  const int j = __builtin_amdgcn_readfirstlane(tp.j0 + (int)(threadIdx.x / kBX));
   if (i >= tp.xl || j >= tp.yl) return;
   bool vx, vy;
   const int im = nb_idx<MODE>(i - 1, nx, vx);
@@ -302,35 +313,35 @@ __global__ __launch_bounds__(kThreads) void yee_adj_e_kernel(
     const float g0 = d * lex[c], g1 = d * ley[c], g2 = d * lez[c];
     float e0 = g0, e1 = g1, e2 = g2;
     if (e_upd<MODE, 0>(a, i, j, k)) {
-      if (CP) psi_e_update<0>(a, cb * g0, i, j, k);
+      if (CPX > 0) psi_e_update<CPX, 0>(a, cb * g0, i, j, k);
       e0 = LOSSY ? ca * g0 : g0;
     }
     if (e_upd<MODE, 1>(a, i, j, k)) {
-      if (CP) psi_e_update<1>(a, cb * g1, i, j, k);
+      if (CPX > 0) psi_e_update<CPX, 1>(a, cb * g1, i, j, k);
       e1 = LOSSY ? ca * g1 : g1;
     }
     if (e_upd<MODE, 2>(a, i, j, k)) {
-      if (CP) psi_e_update<2>(a, cb * g2, i, j, k);
+      if (CPX > 0) psi_e_update<CPX, 2>(a, cb * g2, i, j, k);
       e2 = LOSSY ? ca * g2 : g2;
     }
     float xa, xb, ya, yb, za, zb, t0, t1;
-    hterm<MODE, CP, 0>(a, lhx, i, j, k, true, xa, xb);
-    hterm<MODE, CP, 1>(a, lhy, i, j, k, true, ya, yb);
-    hterm<MODE, CP, 2>(a, lhz, i, j, k, true, za, zb);
+    hterm<MODE, CPX, 0>(a, lhx, i, j, k, true, xa, xb);
+    hterm<MODE, CPX, 1>(a, lhy, i, j, k, true, ya, yb);
+    hterm<MODE, CPX, 2>(a, lhz, i, j, k, true, za, zb);
     // Ex: Hy's a (Ex[c+z] - Ex[c]) at c (-) and c-z (+); Hz's b at c (-), c-y (+)
-    hterm<MODE, CP, 1>(a, lhy, i, j, km, vz, t0, t1);
+    hterm<MODE, CPX, 1>(a, lhy, i, j, km, vz, t0, t1);
     e0 += t0 - ya;
-    hterm<MODE, CP, 2>(a, lhz, i, jm, k, vy, t0, t1);
+    hterm<MODE, CPX, 2>(a, lhz, i, jm, k, vy, t0, t1);
     e0 += t1 - zb;
     // Ey: Hx's b (Ey[c+z] - Ey[c]) at c (-), c-z (+); Hz's a at c (-), c-x (+)
-    hterm<MODE, CP, 0>(a, lhx, i, j, km, vz, t0, t1);
+    hterm<MODE, CPX, 0>(a, lhx, i, j, km, vz, t0, t1);
     e1 += t1 - xb;
-    hterm<MODE, CP, 2>(a, lhz, im, j, k, vx, t0, t1);
+    hterm<MODE, CPX, 2>(a, lhz, im, j, k, vx, t0, t1);
     e1 += t0 - za;
     // Ez: Hx's a (Ez[c+y] - Ez[c]) at c (-), c-y (+); Hy's b at c (-), c-x (+)
-    hterm<MODE, CP, 0>(a, lhx, i, jm, k, vy, t0, t1);
+    hterm<MODE, CPX, 0>(a, lhx, i, jm, k, vy, t0, t1);
     e2 += t0 - xa;
-    hterm<MODE, CP, 1>(a, lhy, im, j, k, vx, t0, t1);
+    hterm<MODE, CPX, 1>(a, lhy, im, j, k, vx, t0, t1);
     e2 += t1 - yb;
     // design box accumulation (cells with damp == 1, all components updated)
     if (a.bnx > 0) {
@@ -428,20 +439,35 @@ int adjoint_args(const YeePlan* p, float* const* lam, float* lam_psi, const YeeD
 // with no slab within one cell (no psi code) and the slab shell.
 template <int MODE, bool LOSSY>
 void adjoint_launch_t(const YeePlan* p, const AdjArgs& a, float* const* lam, hipStream_t st) {
-  TileSet in, sh;
-  split_tiles(a.nx, a.ny, a.nz, a.p, p->kz, 1, 1, 1, 1, p->xcd < 0 ? 1 : p->xcd, in, sh);
-  if (in.ntiles > 0)
-    yee_adj_h_kernel<MODE, false><<<tileset_blocks(in), kThreads, 0, st>>>(
-        a, in, lam[0], lam[1], lam[2], lam[3], lam[4], lam[5]);
-  if (sh.ntiles > 0)
-    yee_adj_h_kernel<MODE, MODE == kCpml><<<tileset_blocks(sh), kThreads, 0, st>>>(
-        a, sh, lam[0], lam[1], lam[2], lam[3], lam[4], lam[5]);
-  if (in.ntiles > 0)
-    yee_adj_e_kernel<MODE, LOSSY, false><<<tileset_blocks(in), kThreads, 0, st>>>(
-        a, in, lam[0], lam[1], lam[2], lam[3], lam[4], lam[5]);
-  if (sh.ntiles > 0)
-    yee_adj_e_kernel<MODE, LOSSY, MODE == kCpml><<<tileset_blocks(sh), kThreads, 0, st>>>(
-        a, sh, lam[0], lam[1], lam[2], lam[3], lam[4], lam[5]);
+  TileSet in, shn, shx;
+  const int order = p->xcd < 0 ? 1 : p->xcd;
+  if (a.p > 0) {
+    // kernel A reads +1 and kernel B -1 neighbours: one cell of margin on
+    // every side; the x-slab columns get their own launch
+    split_tiles3(a.nx, a.ny, a.nz, a.p, p->kz, 1, 1, order == 2 ? 1 : order, in, shn, shx,
+                 kBY, 1, 1, false);
+  } else {
+    split_tiles(a.nx, a.ny, a.nz, 0, p->kz, 1, 1, 1, 1, order == 2 ? 1 : order, in, shn);
+    std::memset(&shx, 0, sizeof(shx));
+  }
+#define YEE_ADJ_H(CPX, TS)                                                      \
+  yee_adj_h_kernel<MODE, CPX><<<tileset_blocks(TS), kThreads, 0, st>>>(          \
+      a, TS, lam[0], lam[1], lam[2], lam[3], lam[4], lam[5])
+#define YEE_ADJ_E(CPX, TS)                                                      \
+  yee_adj_e_kernel<MODE, LOSSY, CPX><<<tileset_blocks(TS), kThreads, 0, st>>>(   \
+      a, TS, lam[0], lam[1], lam[2], lam[3], lam[4], lam[5])
+  if (in.ntiles > 0) YEE_ADJ_H(0, in);
+  if constexpr (MODE == kCpml) {
+    if (shn.ntiles > 0) YEE_ADJ_H(1, shn);
+    if (shx.ntiles > 0) YEE_ADJ_H(2, shx);
+  }
+  if (in.ntiles > 0) YEE_ADJ_E(0, in);
+  if constexpr (MODE == kCpml) {
+    if (shn.ntiles > 0) YEE_ADJ_E(1, shn);
+    if (shx.ntiles > 0) YEE_ADJ_E(2, shx);
+  }
+#undef YEE_ADJ_H
+#undef YEE_ADJ_E
 }
 
 int adjoint_launch(const YeePlan* p, const AdjArgs& a, float* const* lam, hipStream_t st) {

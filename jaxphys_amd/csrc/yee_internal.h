@@ -212,7 +212,8 @@ inline void split_tiles(int nx, int ny, int nz, int p, int kz, int lo, int hi, i
 // cell or x-1 neighbour lies in an x slab.  When there is no interior the
 // whole grid goes to `shx`.
 inline void split_tiles3(int nx, int ny, int nz, int p, int kz, int lo, int hi, int order,
-                         TileSet& in, TileSet& shn, TileSet& shx, int rows) {
+                         TileSet& in, TileSet& shn, TileSet& shx, int rows, int zlo = 0,
+                         int zhi = 0, bool pair_x = true) {
   std::memset(&in, 0, sizeof(in));
   std::memset(&shn, 0, sizeof(shn));
   std::memset(&shx, 0, sizeof(shx));
@@ -220,12 +221,12 @@ inline void split_tiles3(int nx, int ny, int nz, int p, int kz, int lo, int hi, 
   in.rows = shn.rows = shx.rows = rows;
   const int xa = (p + lo + 31) / 32 * 32, xb = (nx - p - hi) / 32 * 32;
   const int ya = p + lo, yb = ny - p - hi;
-  const int za = p, zb = nz - p;
+  const int za = p + zlo, zb = nz - p - zhi;
   if (p <= 0) {
     tileset_add(in, 0, nx, 0, ny, 0, nz);
   } else if (xa < xb && ya < yb && za < zb) {
     tileset_add(in, xa, xb, ya, yb, za, zb);
-    if (xa <= 32 && nx - xb <= 32) {
+    if (pair_x && xa <= 32 && nx - xb <= 32) {
       tileset_add(shx, 0, xa, 0, ny, 0, nz, xb, nx);
     } else {
       tileset_add(shx, 0, xa, 0, ny, 0, nz);

@@ -219,14 +219,15 @@ class EMGrid3D:
         return np.stack(cols, axis=1) if cols else np.zeros((n_steps, 0), np.float32)
 
     def build_solver(self, dt: float, n_steps: int, t_start: float = 0.0,
-                     device=None) -> YeeSolver:
+                     device=None, palette: bool = True) -> YeeSolver:
         """Device state for this grid (extension; used by simulate, the adjoint
-        and the closure trainer)."""
+        and the closure trainer).  ``palette=False`` keeps the per-cell
+        coefficient arrays only (for callers that change eps_r often)."""
         from jaxphys_amd import _native
         dev = device or self._device or _native.require_device()
         eps, sigma = self._build_media(dev)
         solver = YeeSolver(self.size, self._dx, dt, self._boundary, self._pml_layers,
-                           eps, sigma, cpml=self._cpml, device=dev)
+                           eps, sigma, cpml=self._cpml, device=dev, palette=palette)
         # fdtd3d.py:239-244: "z" -> Ez, "x" -> Ex, anything else -> Ey
         comps = [2 if s.polarization == "z" else (0 if s.polarization == "x" else 1)
                  for s in self._sources]

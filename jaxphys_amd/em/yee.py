@@ -239,18 +239,41 @@ class YeeSolver:
         """The current CPML psi buffer (None unless CPML)."""
         return self._psis[self._cur]
 
+    # Fused step only when the slab-free interior (the box of the fused
+    # interior launch) holds at least this share of the cells: its CPML shell
+    # launches cost about twice the two-pass kernels per cell (on MI355X:
+    # 512^3 with 16-cell slabs, 77 % interior: fused 1.94 ms vs 2.31 ms;
+    # 160^3 with 12-cell slabs, 42 %: fused 0.123 ms vs 0.075 ms).
+    FUSED_MIN_INTERIOR = 0.6
+
+    def interior_fraction(self) -> float:
+        """Share of the cells in the fused step's slab-free interior box."""
+        if self.cpml is None:
+            return 1.0
+        p = self.pml_layers
+        xa, xb = (p + 1 + 31) // 32 * 32, (self.nx - p) // 32 * 32
+        n = (max(0, xb - xa) * max(0, self.ny - p - (p + 1)) * max(0, self.nz - 2 * p))
+        return n / float(self.nx * self.ny * self.nz)
+
     def run(self, n0: int, nsteps: int, stream=None) -> None:
-        """Steps n0 .. n0+nsteps-1: one fused launch per step (default) or the
-        two half-step launches."""
+        """Steps n0 .. n0+nsteps-1 with the faster path for this grid: one
+        fused step per launch set, or the H and E half-step kernels (the same
+        results bit for bit)."""
+        if self.fused and self.interior_fraction() >= self.FUSED_MIN_INTERIOR:
+            self.run_fused(n0, nsteps, stream)
+        else:
+            self.run_two_pass(n0, nsteps, stream)
+
+    def run_fused(self, n0: int, nsteps: int, stream=None) -> None:
+        """Steps with the fused kernels (ping-pong between the two buffers)."""
         if nsteps <= 0:
             return
-        if self.fused:
-            a, b = self._cur, 1 - self._cur
-            _native.run_fused(self.plan, self._bufs[a], self._bufs[b], self._psis[a],
-                              self._psis[b], n0, nsteps, stream)
-            self._cur = a if nsteps % 2 == 0 else b
-        else:
-            self.plan.run(self.fields, self.psi, n0, nsteps, stream)
+        if not self.fused:
+            raise RuntimeError("solver built with fused=False has one state buffer")
+        a, b = self._cur, 1 - self._cur
+        _native.run_fused(self.plan, self._bufs[a], self._bufs[b], self._psis[a],
+                          self._psis[b], n0, nsteps, stream)
+        self._cur = a if nsteps % 2 == 0 else b
 
     def run_two_pass(self, n0: int, nsteps: int, stream=None) -> None:
         """Steps with the separate H and E kernels (same results bit for bit)."""

@@ -263,7 +263,7 @@ def test_fused_step_bitwise_equals_two_pass(gpu, boundary, lossy, size, variant,
     b.plan.set_tile_order(xcd)
     a.plan.tune(kz=0, fused_kz=7)   # several z chunks per column
     b.plan.tune(kz=5, fused_kz=-1)
-    a.run(0, n)              # fused (odd count: ends in the second buffer)
+    a.run_fused(0, n)        # fused (odd count: ends in the second buffer)
     b.run_two_pass(0, n)
     for q in range(6):
         assert torch.equal(a.fields[q], b.fields[q]), NAMES[q]
