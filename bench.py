@@ -103,7 +103,8 @@ def measured_traffic(workload: str, mode: str):
 def adjoint_bench(args, dev):
     """160^3, 12-cell CPML, 800 steps; 96^3 design region eps_r U[1,4] in
     vacuum; z dipole 20 cells below it; J = time-integrated Ez^2 on a 32x32
-    patch 20 cells above it; checkpoints every 100 steps.  Reports the cost
+    patch 20 cells above it; the design-box history recorded in the forward
+    run when it fits (else checkpoints every 100 steps).  Reports the cost
     of value+gradient relative to one plain forward run and central finite
     differences (h=1e-3) on 16 seeded design cells."""
     import numpy as np
@@ -114,8 +115,9 @@ def adjoint_bench(args, dev):
 
     n, steps = 160, args.steps or 800
     g, box, probes, eps_box = W.c4_design(n, dev)
+    rec = {"auto": None, "on": True, "off": False}[args.record_box]
     prob = AdjointProblem(g, steps, W.dt_for(1.0), box, probes,
-                          checkpoint_every=args.checkpoint_every)
+                          checkpoint_every=args.checkpoint_every, record_box=rec)
     # plain forward (the run being differentiated), timed alone
     plain = g.build_solver(W.dt_for(1.0), steps)
     eps_full = prob.background.clone()
@@ -156,9 +158,10 @@ def adjoint_bench(args, dev):
         "n_gpus": 1, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "adjoint_c4", "grid": [n, n, n], "steps": steps, "npml": 12,
                    "design_box": list(box), "checkpoint_every": args.checkpoint_every,
+                   "record_box": prob.record_box,
                    "probes": len(probes)},
         "forward_s": round(t_fwd, 4), "value_and_grad_s": round(t2 - t0, 4),
-        "checkpointed_forward_s": round(t1 - t0, 4), "adjoint_s": round(t2 - t1, 4),
+        "recording_forward_s": round(t1 - t0, 4), "adjoint_s": round(t2 - t1, 4),
         "J": J, "fd_check": {"h": h, "cells": len(fd_err),
                               "max_abs_err_over_max_grad": worst, "samples": fd_err[:4]},
     }
@@ -264,6 +267,9 @@ def main() -> None:
     ap.add_argument("--mode", default="fused", choices=["fused", "two_pass"],
                     help="one fused launch per step (default) or the H and E half-step kernels")
     ap.add_argument("--checkpoint-every", type=int, default=100)
+    ap.add_argument("--record-box", default="auto", choices=["auto", "on", "off"],
+                    help="adjoint: record the design-box history in the forward run "
+                         "(auto: when it fits) or recompute it from checkpoints")
     ap.add_argument("--fd-cells", type=int, default=16)
     ap.add_argument("--closure-n", type=int, default=256)
     ap.add_argument("--closure-train", type=int, default=64)

@@ -5,6 +5,12 @@ No reference counterpart: the reference would differentiate
 reverse mode through ``lax.scan`` at src/neurosim/em/fdtd3d.py:393), which
 keeps residuals of every step (O(n_steps * N^3) memory).  Here:
 
+The adjoint sweep needs, of the forward run, only the probe values and E
+on the design box after every step.  When that box history fits in a
+quarter of the free device memory (``record_box``, automatic), the forward
+run records it for all steps and the backward pass is a single adjoint
+sweep.  Otherwise, checkpointing:
+
 1. forward run on the GPU, saving the full state (6 fields + CPML psi) every
    ``checkpoint_every`` steps and the probe values after every step;
 2. for each checkpoint segment, last first: restore the checkpoint, recompute
@@ -12,7 +18,9 @@ keeps residuals of every step (O(n_steps * N^3) memory).  Here:
    then run the adjoint steps of the segment in reverse
    (``yee_adjoint_run``), which accumulate ``sum_n g (E^{n+1} - Ca E^n)`` and
    ``sum_n g E^n`` per design cell;
-3. chain rule to eps_r: ``dJ/deps = acc_cb/Cb * dCb/deps + acc_ca * dCa/deps``.
+
+and in both cases the chain rule to eps_r:
+``dJ/deps = acc_cb/Cb * dCb/deps + acc_ca * dCa/deps``.
 
 Objective: ``J = sum_n sum_q w_q F_q(after step n)^2`` over probes q
 (component, cell), e.g. the time-integrated |Ez|^2 on a monitor patch.
@@ -43,11 +51,15 @@ class AdjointProblem:
             and contain no source.
         probes: ``[(component, (i, j, k)), ...]`` with component in
             ex..hz, and optional ``weights``.
-        checkpoint_every: forward checkpoint interval K.
+        checkpoint_every: forward checkpoint interval K (when the box
+            history is not recorded).
+        record_box: record E on the design box for every step during the
+            forward run (no recomputation); None = when it fits in a quarter
+            of the free device memory.
     """
 
     def __init__(self, grid, n_steps: int, dt: float, design_box, probes, weights=None,
-                 checkpoint_every: int = 100):
+                 checkpoint_every: int = 100, record_box: bool | None = None):
         self.grid = grid
         self.n_steps = int(n_steps)
         self.K = max(1, int(checkpoint_every))
@@ -101,6 +113,12 @@ class AdjointProblem:
         self.probe_hist = torch.zeros((self.n_steps, self.nprobe), dtype=torch.float32, device=dev)
         self.checkpoints = []
         self.box_hist = None
+        need = (self.n_steps + 1) * 3 * self.bcells * 4
+        if record_box is None:
+            free, _ = torch.cuda.mem_get_info(dev)
+            record_box = need <= free // 4
+        self.record_box = bool(record_box)
+        self._recorded = False
 
     # ------------------------------------------------------------------
     def set_design(self, eps_box: torch.Tensor) -> None:
@@ -120,6 +138,19 @@ class AdjointProblem:
         sol.reset()
         self.checkpoints = []
         N, K = self.n_steps, self.K
+        if self.record_box:
+            if self.box_hist is None or self.box_hist.shape[0] != N + 1:
+                self.box_hist = torch.empty((N + 1, 3 * self.bcells), dtype=torch.float32,
+                                            device=sol.device)
+            _native.box_gather(sol.fields, (sol.nx, sol.ny, sol.nz), self.box_int,
+                               self.box_hist[0])
+            _native.run_record(sol.plan, sol.fields, sol.psi, 0, N, self.probe_comp,
+                               self.probe_idx, self.probe_hist, box=self.box_int,
+                               box_out=self.box_hist[1:])
+            self._recorded = True
+            ph = self.probe_hist.to(torch.float64)
+            return float((self.w.to(torch.float64) * ph * ph).sum())
+        self._recorded = False
         for s0 in range(0, N, K):
             self.checkpoints.append(([f.clone() for f in sol.fields],
                                      sol.psi.clone() if sol.psi is not None else None))
@@ -134,18 +165,23 @@ class AdjointProblem:
         sol = self.solver
         dev = sol.device
         N, K = self.n_steps, self.K
-        if not self.checkpoints:
+        if not self.checkpoints and not self._recorded:
             raise RuntimeError("backward() needs a forward() first")
         lam = [torch.zeros_like(f) for f in sol.fields]
         lam_psi = torch.zeros_like(sol.psi) if sol.psi is not None else None
         acc_cb = torch.zeros(self.bcells, dtype=torch.float32, device=dev)
         acc_ca = torch.zeros(self.bcells, dtype=torch.float32, device=dev) \
             if sol.ca is not None else None
-        if self.box_hist is None or self.box_hist.shape[0] < K + 1:
-            self.box_hist = torch.empty((K + 1, 3 * self.bcells), dtype=torch.float32, device=dev)
         # adjoint of the final E: 2 w F(after step N-1)
         _native.probe_scatter_add(lam, self.probe_comp, self.probe_idx, self.w,
                                   self.probe_hist[N - 1], 2.0)
+        if self._recorded:
+            _native.adjoint_run(sol.plan, lam, lam_psi, 0, N, self.probe_comp, self.probe_idx,
+                                self.w, self.probe_hist, 2.0, box=self.box_int,
+                                box_hist=self.box_hist, acc_cb=acc_cb, acc_ca=acc_ca)
+            return self._chain_rule(acc_cb, acc_ca)
+        if self.box_hist is None or self.box_hist.shape[0] < K + 1:
+            self.box_hist = torch.empty((K + 1, 3 * self.bcells), dtype=torch.float32, device=dev)
         for seg in range(len(self.checkpoints) - 1, -1, -1):
             s0 = seg * K
             s1 = min(N, s0 + K)
@@ -161,6 +197,10 @@ class AdjointProblem:
             _native.adjoint_run(sol.plan, lam, lam_psi, s0, s1, self.probe_comp, self.probe_idx,
                                 self.w, self.probe_hist, 2.0, box=self.box_int,
                                 box_hist=self.box_hist, acc_cb=acc_cb, acc_ca=acc_ca)
+        return self._chain_rule(acc_cb, acc_ca)
+
+    def _chain_rule(self, acc_cb, acc_ca) -> torch.Tensor:
+        sol = self.solver
         x0, y0, z0, bx, by, bz = self.box_ref
         shape = (bz, by, bx)
         dcb, dca = sol.d_coeffs_d_eps(self.region)

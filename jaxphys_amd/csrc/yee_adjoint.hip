@@ -229,6 +229,16 @@ __device__ __forceinline__ int nb_idx(int v, int n, bool& ok) {
   return v < 0 ? 0 : n - 1;
 }
 
+// Column of this lane.  A paired tile holds two 32-column segments, one per
+// half-wave (the low and the high x slab); every neighbour is loaded from
+// memory, so the two halves need no exchange.
+__device__ __forceinline__ void adj_column(const TilePos& tp, int& i, int& xl) {
+  const int lane = (int)(threadIdx.x % kBX);
+  const bool hi = tp.pair && lane >= 32;
+  i = hi ? tp.ih0 + (lane & 31) : tp.i0 + (tp.pair ? lane & 31 : lane);
+  xl = hi ? tp.ihl : tp.xl;
+}
+
 // Kernel A: per H cell, z-marching tiles.  CPX as axis_on: 0 for tiles with
 // no slab within one cell, 1 for tiles whose columns are at least one cell
 // away from the x slabs, 2 for any tile.
@@ -237,14 +247,15 @@ __global__ __launch_bounds__(kThreads) void yee_adj_h_kernel(
     AdjArgs a, TileSet ts, const float* __restrict__ lex, const float* __restrict__ ley,
     const float* __restrict__ lez, float* __restrict__ lhx, float* __restrict__ lhy,
     float* __restrict__ lhz) {
-  TilePos tp;  // (no paired boxes in the adjoint tile sets)
+  TilePos tp;
   if (!tile_of_set(ts, tp)) return;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const int k0 = tp.k0, k1 = tp.k1;
-  const int i = tp.i0 + (int)(threadIdx.x % kBX);
+  int i, xl;
+  adj_column(tp, i, xl);
   // one y row per wavefront: j is scalar, so y-slab tests are uniform
   const int j = __builtin_amdgcn_readfirstlane(tp.j0 + (int)(threadIdx.x / kBX));
-  if (i >= tp.xl || j >= tp.yl) return;
+  if (i >= xl || j >= tp.yl) return;
   bool vx, vy;
   const int ip = nb_idx<MODE>(i + 1, nx, vx);
   const int jp = nb_idx<MODE>(j + 1, ny, vy);
@@ -292,14 +303,15 @@ __global__ __launch_bounds__(kThreads) void yee_adj_e_kernel(
     AdjArgs a, TileSet ts, float* __restrict__ lex, float* __restrict__ ley,
     float* __restrict__ lez, const float* __restrict__ lhx, const float* __restrict__ lhy,
     const float* __restrict__ lhz) {
-  TilePos tp;  // (no paired boxes in the adjoint tile sets)
+  TilePos tp;
   if (!tile_of_set(ts, tp)) return;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const int k0 = tp.k0, k1 = tp.k1;
-  const int i = tp.i0 + (int)(threadIdx.x % kBX);
+  int i, xl;
+  adj_column(tp, i, xl);
   // one y row per wavefront: j is scalar, so y-slab tests are uniform
   const int j = __builtin_amdgcn_readfirstlane(tp.j0 + (int)(threadIdx.x / kBX));
-  if (i >= tp.xl || j >= tp.yl) return;
+  if (i >= xl || j >= tp.yl) return;
   bool vx, vy;
   const int im = nb_idx<MODE>(i - 1, nx, vx);
   const int jm = nb_idx<MODE>(j - 1, ny, vy);
@@ -443,9 +455,10 @@ void adjoint_launch_t(const YeePlan* p, const AdjArgs& a, float* const* lam, hip
   const int order = p->xcd < 0 ? 1 : p->xcd;
   if (a.p > 0) {
     // kernel A reads +1 and kernel B -1 neighbours: one cell of margin on
-    // every side; the x-slab columns get their own launch
+    // every side; the x-slab columns get their own launch, both slabs in
+    // one paired box when each is at most 32 columns wide
     split_tiles3(a.nx, a.ny, a.nz, a.p, p->kz, 1, 1, order == 2 ? 1 : order, in, shn, shx,
-                 kBY, 1, 1, false);
+                 kBY, 1, 1, true);
   } else {
     split_tiles(a.nx, a.ny, a.nz, 0, p->kz, 1, 1, 1, 1, order == 2 ? 1 : order, in, shn);
     std::memset(&shx, 0, sizeof(shx));
