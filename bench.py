@@ -1,7 +1,7 @@
 """FDTD3D throughput on one MI355X: Gcell-updates/s and HBM-roofline fraction.
 
 ``python bench.py [--steps K] [--warmup W] [--workload c3_512|c3_256|c2_128|c1_64|
-adjoint_c4|closure_c5|closure_net] [--mode fused|two_pass]``
+adjoint_c4|closure_c5|closure_net|tm2d] [--mode fused|two_pass]``
 
 A step is one full Yee time step (H half-step, then E half-step with CPML
 psi, conductivity and the three dipoles) over the whole synthetic grid; the
@@ -167,6 +167,91 @@ def adjoint_bench(args, dev):
     }
 
 
+def tm2d_bench(args, dev):
+    """2D TM grid (the reference's EMGrid, src/neurosim/em/fdtd.py) at
+    16384 x 16384: absorbing boundary with 10 damped layers, a conducting
+    wall with a 400-cell slit across the middle and two plane-wave rows.
+    One fused kernel per step; HIP events around every launch."""
+    import numpy as np
+    import torch
+
+    from jaxphys_amd import EMGrid, PlaneWave, Wall
+
+    n = args.tm2d_n
+    K = args.steps if args.steps is not None else 200
+    Wm = args.warmup
+    g = EMGrid(size=(n, n), resolution=0.01, boundary="absorbing", pml_layers=10)
+    g.add_source(PlaneWave(frequency=3e9, y=n // 8, amplitude=1.0))
+    g.add_source(PlaneWave(frequency=5e9, y=n // 4, amplitude=0.5))
+    g.add_conductor(Wall(y=n // 2, gap_start=n // 2 - 200, gap_end=n // 2 + 200))
+    dt = 0.99 * 0.01 / (2.99792458e8 * float(np.sqrt(np.float32(2.0))))
+    t_setup = time.time()
+    sol = g.build_solver(dt, K + Wm)
+    sol.run(0, Wm)
+    torch.cuda.synchronize()
+    setup_s = time.time() - t_setup
+    stream = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(K + 1)]
+    ev[0].record(stream)
+    for q in range(K):
+        sol.run(Wm + q, 1, stream)
+        ev[q + 1].record(stream)
+    torch.cuda.synchronize()
+    total_ms = ev[0].elapsed_time(ev[K])
+    launch_ms = total_ms / K
+    cells = n * n
+    nbytes = sol.bytes_per_step()
+    achieved = nbytes / (launch_ms / 1e3) / 1e9
+    finite = bool(all(torch.isfinite(f).all().item() for f in sol.fields))
+    cpu = None
+    if args.cpu_budget > 0:
+        cpu = tm2d_cpu_baseline(g, sol, dt, args.cpu_budget)
+    traffic, traffic_rec = measured_traffic("tm2d", "fused")
+    return {
+        "metric": "FDTD2D TM Gcell-upd/s & %HBM roofline", "value": round(cells / launch_ms / 1e6, 3),
+        "unit": "Gcell-upd/s", "n_gpus": 1, "steps": K, "warmup": Wm,
+        "ms_per_step": round(launch_ms, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "tm2d", "grid": [n, n], "boundary": "absorbing", "npml": 10,
+                   "walls": 1, "sources": 2},
+        "roofline": {"bound": "hbm", "kernel": "tm2d_step_kernel<0>", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": None if traffic_rec is None else traffic_rec.get("source"),
+                     "bytes_per_launch": nbytes, "ms_per_launch": round(launch_ms, 4)},
+        "cpu_baseline": cpu, "fields_finite": finite, "setup_s": round(setup_s, 2),
+    }
+
+
+def tm2d_cpu_baseline(grid, sol, dt, budget_s):
+    """The NumPy float32 restatement of the 2D step (the oracle), one
+    thread, on the same grid, tables and sources, for about ``budget_s``."""
+    import numpy as np
+
+    from oracle import ref_tm2d as R
+
+    nx, ny = grid.size
+    f32 = np.float32
+    waves = [R.Wave(s.frequency, s.y, s.amplitude) for s in grid._sources]
+    walls = [R.Wall(w.y, w.gap_start, w.gap_end) for w in grid._conductors]
+    mask = R.conductor_mask(nx, ny, walls)
+    damp = R.damping(nx, ny, grid._config.pml_layers, grid._dx, dt, f32)
+    vals = R.source_values(waves, 4096, 0.0, dt, f32)
+    ez, hx, hy = (np.zeros((nx, ny), f32) for _ in range(3))
+    ch, ce = dt / (R.MU0 * grid._dx), dt / (R.EPS0 * grid._dx)
+    steps = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < budget_s and steps < 4096:
+        ez, hx, hy = R.step(ez, hx, hy, steps, boundary="absorbing", ch=ch, ce=ce, damp=damp,
+                            mask=mask, src_vals=vals, waves=waves, dtype=f32)
+        steps += 1
+    secs = time.perf_counter() - t0
+    return {"value": round(nx * ny * steps / secs / 1e9, 4), "unit": "Gcell-upd/s", "cores": 1,
+            "kind": "port",
+            "sample": f"{steps} float32 steps of the same {nx}x{ny} workload in {secs:.1f} s "
+                      "(NumPy oracle, one thread)"}
+
+
 def closure_net_bench(args, dev):
     """The closure net's BF16-MFMA conv kernels at the coarse 128^3 grid:
     one inference (three implicit-GEMM 3^3 convolutions, 7->32->32->6
@@ -224,6 +309,10 @@ def closure_bench(args, dev):
     net, hist = CL.train_closure(dev, train_seeds, n_fine=nf, n_coarse=nf * 400 // 256,
                                  every=args.closure_every, epochs=args.closure_epochs,
                                  crop=min(64, nf // 2 - 16), lr=args.closure_lr,
+                                 lr_final=args.closure_lr_final,
+                                 onpolicy_from=(None if args.closure_onpolicy_from < 0
+                                                else args.closure_onpolicy_from),
+                                 forcing=not args.closure_no_forcing,
                                  log=lambda m: print("[closure]", m, file=sys.stderr, flush=True),
                                  **pkw)
     t_train = time.time() - t0
@@ -243,8 +332,15 @@ def closure_bench(args, dev):
                    "train_seeds": len(train_seeds), "test_seeds": len(test_seeds),
                    "epochs": args.closure_epochs, "correct_every": args.closure_every,
                    "pulse_t0": args.closure_t0, "pulse_spread": args.closure_spread,
-                   "pulse": args.closure_waveform},
+                   "pulse": args.closure_waveform, "forcing": not args.closure_no_forcing,
+                   "onpolicy_from_epoch": args.closure_onpolicy_from,
+                   "lr": [args.closure_lr, args.closure_lr_final]},
         "err_coarse_no_closure": round(float(np.mean(err_0)), 5),
+        "err_far_closure": round(float(np.mean([r["err_far_closure"] for r in res])), 5),
+        "err_far_coarse_no_closure": round(float(np.mean([r["err_far_coarse"] for r in res])), 5),
+        "err_traj_far_closure": round(float(np.mean([r["err_traj_far_closure"] for r in res])), 5),
+        "err_traj_far_coarse_no_closure": round(float(np.mean([r["err_traj_far_coarse"]
+                                                               for r in res])), 5),
         "err_traj_closure": round(float(np.mean(trj_c)), 5),
         "err_traj_coarse_no_closure": round(float(np.mean(trj_0)), 5),
         "err_closure_max": round(float(np.max(err_c)), 5),
@@ -276,6 +372,13 @@ def main() -> None:
     ap.add_argument("--closure-test", type=int, default=16)
     ap.add_argument("--closure-epochs", type=int, default=2)
     ap.add_argument("--closure-lr", type=float, default=1e-3)
+    ap.add_argument("--closure-lr-final", type=float, default=1e-4,
+                    help="learning rate at the end of the cosine schedule")
+    ap.add_argument("--closure-onpolicy-from", type=int, default=1,
+                    help="epoch from which training uses the corrected rollout's own states "
+                         "(-1: teacher-forced only)")
+    ap.add_argument("--closure-no-forcing", action="store_true",
+                    help="no forcing input channel (fields and eps_r only)")
     ap.add_argument("--closure-every", type=int, default=10,
                     help="coarse steps between closure corrections")
     ap.add_argument("--closure-t0", type=float, default=60.0,
@@ -284,6 +387,7 @@ def main() -> None:
                     help="pulse width (coarse steps)")
     ap.add_argument("--closure-waveform", default="gaussian",
                     choices=["gaussian", "gaussian_derivative"])
+    ap.add_argument("--tm2d-n", type=int, default=16384, help="tm2d workload: cells per side")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain copy of the field bytes (counter calibration)")
     args = ap.parse_args()
@@ -327,6 +431,9 @@ def main() -> None:
         return
     if args.workload == "closure_net":
         print(json.dumps(closure_net_bench(args, dev)), flush=True)
+        return
+    if args.workload == "tm2d":
+        print(json.dumps(tm2d_bench(args, dev)), flush=True)
         return
 
     t_setup = time.time()

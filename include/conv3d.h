@@ -67,10 +67,12 @@ int conv3d_flip_weights(const ConvDesc* d, const void* w, int32_t dy_cpad, void*
  * c < 6, eps channel (eps - eps_shift) / eps_scale, channel 7 = 0.       */
 int conv3d_pack_fields(int32_t nx, int32_t ny, int32_t nz, float* const* fields, const float* eps,
                        const float* scale, float eps_shift, float eps_scale, void* out, void* stream);
-/* fields[c] += gain * scale[c] * y[..][c], y float32 channels last [..][ystride] */
+/* fields[c] += gain * scale[c] * y[..][c], y float32 channels last [..][ystride],
+ * for the cells at least `margin` cells from every face; the others (the
+ * absorbing layers, where the closure is not trained) are left unchanged. */
 int conv3d_apply_correction(int32_t nx, int32_t ny, int32_t nz, float* const* fields,
                             const float* y, int32_t ystride, const float* scale, float gain,
-                            void* stream);
+                            int32_t margin, void* stream);
 
 const char* conv3d_last_error(void);
 

@@ -37,7 +37,8 @@ EXPORTS = {
                                           ctypes.c_void_p]),
     "conv3d_apply_correction": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                                ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
-                                               ctypes.c_void_p, ctypes.c_float, ctypes.c_void_p]),
+                                               ctypes.c_void_p, ctypes.c_float, ctypes.c_int32,
+                                               ctypes.c_void_p]),
     "conv3d_last_error": (ctypes.c_char_p, []),
 }
 
@@ -104,8 +105,10 @@ def pack_fields(shape_xyz, fields, eps, scale, eps_shift, eps_scale, out, stream
                                    stream_handle(stream)), "conv3d_pack_fields")
 
 
-def apply_correction(shape_xyz, fields, y, ystride, scale, gain=1.0, stream=None) -> None:
+def apply_correction(shape_xyz, fields, y, ystride, scale, gain=1.0, margin=0,
+                     stream=None) -> None:
     nx, ny, nz = shape_xyz
     check(lib().conv3d_apply_correction(nx, ny, nz, field_array(fields), ptr(y), int(ystride),
-                                        ptr(scale), float(gain), stream_handle(stream)),
+                                        ptr(scale), float(gain), int(margin),
+                                        stream_handle(stream)),
           "conv3d_apply_correction")

@@ -30,6 +30,7 @@ HIPFLAGS = [
 LIBS = {
     "libyee_hip.so": ["csrc/yee.hip", "csrc/yee_fused.hip", "csrc/yee_adjoint.hip"],
     "libconv3d_hip.so": ["csrc/conv3d.hip"],
+    "libtm2d_hip.so": ["csrc/tm2d.hip"],
 }
 
 

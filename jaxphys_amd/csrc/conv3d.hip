@@ -354,10 +354,14 @@ __global__ void pack_fields_kernel(int nx, int ny, int nz, const float* f0, cons
 
 __global__ void apply_correction_kernel(int nx, int ny, int nz, float* f0, float* f1, float* f2,
                                         float* f3, float* f4, float* f5, const float* y, int ys,
-                                        const float* scale, float gain) {
+                                        const float* scale, float gain, int margin) {
   const long long n = (long long)nx * ny * nz;
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const int x = (int)(i % nx), yy = (int)((i / nx) % ny), z = (int)(i / ((long long)nx * ny));
+  if (x < margin || x >= nx - margin || yy < margin || yy >= ny - margin || z < margin ||
+      z >= nz - margin)
+    return;
   float* f[6] = {f0, f1, f2, f3, f4, f5};
 #pragma unroll
   for (int c = 0; c < 6; ++c) f[c][i] += gain * scale[c] * y[i * ys + c];
@@ -527,12 +531,12 @@ int conv3d_pack_fields(int32_t nx, int32_t ny, int32_t nz, float* const* fields,
 
 int conv3d_apply_correction(int32_t nx, int32_t ny, int32_t nz, float* const* fields,
                             const float* y, int32_t ystride, const float* scale, float gain,
-                            void* stream) {
-  if (!fields || !y || !scale || ystride < 6) return fail(-1, "bad argument");
+                            int32_t margin, void* stream) {
+  if (!fields || !y || !scale || ystride < 6 || margin < 0) return fail(-1, "bad argument");
   const long long n = (long long)nx * ny * nz;
   hipLaunchKernelGGL(apply_correction_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                      (hipStream_t)stream, nx, ny, nz, fields[0], fields[1], fields[2], fields[3],
-                     fields[4], fields[5], y, ystride, scale, gain);
+                     fields[4], fields[5], y, ystride, scale, gain, margin);
   CONV_CHECK(hipGetLastError());
   return 0;
 }

@@ -1,0 +1,222 @@
+// 2D TM (Ez, Hx, Hy) Yee step for gfx950: the reference's 2D solver
+// (src/neurosim/em/fdtd.py:298-353) in one pass per step.
+//
+// One launch per step reads buffer A and writes buffer B (ping-pong), so no
+// workgroup reads a value another one already replaced.  The reference's
+// step is H update -> E update -> damping or edge copies -> sources ->
+// conductors; every output cell (i, j) is a pure function of A:
+//
+//   * the edge copies (fdtd.py:256-296: rows 0 and nx-1 first, then columns
+//     0 and ny-1) give cell (i, j) the freshly updated values of cell
+//     (ri(i), rj(j)), with r the identity inside and, at the edges,
+//     reflecting 0 -> 1, n-1 -> n-2, periodic 0 -> n-2, n-1 -> 1;
+//   * E at (a, b) needs the new H at (a, b), (a-1, b) and (a, b-1): the
+//     thread recomputes the two neighbour values from A (their loads hit the
+//     cache lines its wavefront and the previous row already brought in);
+//   * damping, the row sources and the conductor mask act at (i, j).
+//
+// Each float32 operation is the reference's, in its order, and the build
+// uses -ffp-contract=off: the result is bit-identical to the float32
+// restatement.  A workgroup is one 64-lane x row segment by 4 wavefronts;
+// each wavefront walks 4 consecutive rows, so the tile is 64 x 16 cells and
+// the row above a wavefront's current row is still in L1.  HBM traffic per
+// cell: Ez, Hx, Hy read once and written once (24 B), plus one damping
+// float in the absorbing band and one conductor byte when walls exist.
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "tm2d.h"
+
+namespace tm2d {
+
+thread_local std::string g_err;
+
+static int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define TM2D_CHECK(expr)                                                   \
+  do {                                                                     \
+    hipError_t e_ = (expr);                                                \
+    if (e_ != hipSuccess) return fail(TM2D_EHIP, hipGetErrorString(e_));   \
+  } while (0)
+
+constexpr int kTx = 64;  // lanes along x
+constexpr int kTy = 4;   // wavefronts per workgroup
+constexpr int kRows = 4; // rows per wavefront
+
+struct StepArgs {
+  int nx, ny, npml;
+  float ch, ce;
+  const float* damp;
+  const uint8_t* cond;
+  int n_src;
+  const int* rows;
+  const float* vals;  // the source values of this step, n_src floats
+};
+
+template <int MODE>
+__device__ __forceinline__ int remap(int i, int n) {
+  if (MODE == TM2D_REFLECTING) return i == 0 ? 1 : (i == n - 1 ? n - 2 : i);
+  if (MODE == TM2D_PERIODIC) return i == 0 ? n - 2 : (i == n - 1 ? 1 : i);
+  return i;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kTx * kTy) void tm2d_step_kernel(
+    StepArgs g, const float* __restrict__ ez, const float* __restrict__ hx,
+    const float* __restrict__ hy, float* __restrict__ ez_out, float* __restrict__ hx_out,
+    float* __restrict__ hy_out) {
+  constexpr bool kPer = MODE == TM2D_PERIODIC;
+  const int nx = g.nx, ny = g.ny;
+  const int i = blockIdx.x * kTx + threadIdx.x;
+  if (i >= nx) return;
+  const int a = remap<MODE>(i, nx);
+  // x neighbours of column a: wrapped (periodic) or clamped; a clamped one is
+  // loaded but not used (the update it would feed is the identity there)
+  const int ap = kPer ? (a + 1 == nx ? 0 : a + 1) : min(a + 1, nx - 1);
+  const int am = kPer ? (a == 0 ? nx - 1 : a - 1) : max(a - 1, 0);
+  const bool upd_hy = kPer || a <= nx - 2;
+  const bool in_x_band = i < g.npml || i >= nx - g.npml;
+  const int j0 = (blockIdx.y * kTy + threadIdx.y) * kRows;
+#pragma unroll
+  for (int r = 0; r < kRows; ++r) {
+    const int j = j0 + r;
+    if (j >= ny) break;
+    const int b = remap<MODE>(j, ny);
+    const int bp = kPer ? (b + 1 == ny ? 0 : b + 1) : min(b + 1, ny - 1);
+    const int bm = kPer ? (b == 0 ? ny - 1 : b - 1) : max(b - 1, 0);
+    const int c = b * nx + a;
+    const float e0 = ez[c];
+    const float e_xp = ez[b * nx + ap];
+    const float e_yp = ez[bp * nx + a];
+    const float hy0 = hy[c];
+    const float hx0 = hx[c];
+    // fdtd.py:308-309 / 316-325: the new H at (a, b)
+    const float hyn = upd_hy ? hy0 + g.ch * (e_xp - e0) : hy0;
+    const float hxn = (kPer || b <= ny - 2) ? hx0 - g.ch * (e_yp - e0) : hx0;
+    float ezn = e0;
+    if (kPer || (a >= 1 && b >= 1)) {
+      // the new H at (a-1, b) and (a, b-1), recomputed from A
+      const float e_xm = ez[b * nx + am];
+      const float e_ym = ez[bm * nx + a];
+      const float hyn_m = hy[b * nx + am] + g.ch * (e0 - e_xm);
+      const float hxn_m = hx[bm * nx + a] - g.ch * (e0 - e_ym);
+      // fdtd.py:311-313 / 328-335
+      ezn = e0 + g.ce * ((hyn - hyn_m) - (hxn - hxn_m));
+    }
+    const int o = j * nx + i;
+    if (MODE == TM2D_ABSORBING) {
+      // fdtd.py:337-339; outside the band the factor is exp(-0) = 1
+      if (in_x_band || j < g.npml || j >= ny - g.npml) ezn = ezn * g.damp[o];
+    }
+    // fdtd.py:249-254, 349: row sources in insertion order
+    for (int s = 0; s < g.n_src; ++s)
+      if (g.rows[s] == j) ezn = ezn + g.vals[s];
+    // fdtd.py:351-352
+    if (g.cond != nullptr && g.cond[o]) ezn = 0.0f;
+    ez_out[o] = ezn;
+    hx_out[o] = hxn;
+    hy_out[o] = hyn;
+  }
+}
+
+static int validate(const Tm2dDesc* d) {
+  if (!d) return fail(TM2D_EINVAL, "null descriptor");
+  if (d->nx < 3 || d->ny < 3) return fail(TM2D_EINVAL, "grid must be at least 3x3");
+  if ((long long)d->nx * d->ny >= (1LL << 31))
+    return fail(TM2D_EINVAL, "grid must hold fewer than 2^31 cells");
+  if (d->boundary < TM2D_ABSORBING || d->boundary > TM2D_REFLECTING)
+    return fail(TM2D_EINVAL, "unknown boundary");
+  if (d->boundary == TM2D_ABSORBING && d->npml > 0 && !d->damp)
+    return fail(TM2D_EINVAL, "absorbing boundary with npml > 0 needs the damping table");
+  if (d->npml < 0) return fail(TM2D_EINVAL, "npml must be >= 0");
+  if (d->n_src < 0) return fail(TM2D_EINVAL, "n_src must be >= 0");
+  if (d->n_src > 0 && (!d->src_rows || !d->src_table || d->table_steps <= 0))
+    return fail(TM2D_EINVAL, "sources need src_rows and src_table");
+  return TM2D_OK;
+}
+
+static int launch(const Tm2dDesc* d, float* const* in, float* const* out, int n,
+                  hipStream_t stream) {
+  StepArgs g;
+  g.nx = d->nx;
+  g.ny = d->ny;
+  g.npml = d->boundary == TM2D_ABSORBING ? d->npml : 0;
+  g.ch = d->ch;
+  g.ce = d->ce;
+  g.damp = d->damp;
+  g.cond = d->conductor;
+  g.n_src = d->n_src;
+  g.rows = d->src_rows;
+  g.vals = d->n_src > 0 ? d->src_table + (long long)n * d->n_src : nullptr;
+  const dim3 block(kTx, kTy);
+  const dim3 grid((d->nx + kTx - 1) / kTx, (d->ny + kTy * kRows - 1) / (kTy * kRows));
+  switch (d->boundary) {
+    case TM2D_ABSORBING:
+      hipLaunchKernelGGL(tm2d_step_kernel<TM2D_ABSORBING>, grid, block, 0, stream, g, in[0],
+                         in[1], in[2], out[0], out[1], out[2]);
+      break;
+    case TM2D_PERIODIC:
+      hipLaunchKernelGGL(tm2d_step_kernel<TM2D_PERIODIC>, grid, block, 0, stream, g, in[0],
+                         in[1], in[2], out[0], out[1], out[2]);
+      break;
+    default:
+      hipLaunchKernelGGL(tm2d_step_kernel<TM2D_REFLECTING>, grid, block, 0, stream, g, in[0],
+                         in[1], in[2], out[0], out[1], out[2]);
+      break;
+  }
+  TM2D_CHECK(hipGetLastError());
+  return TM2D_OK;
+}
+
+static int check_fields(float* const* f, const char* what) {
+  if (!f || !f[0] || !f[1] || !f[2]) return fail(TM2D_EINVAL, std::string("null ") + what);
+  return TM2D_OK;
+}
+
+}  // namespace tm2d
+
+using namespace tm2d;
+
+extern "C" {
+
+const char* tm2d_last_error(void) { return g_err.c_str(); }
+
+int tm2d_step(const Tm2dDesc* d, float* const* in, float* const* out, int32_t n,
+              void* stream) {
+  int rc = validate(d);
+  if (rc) return rc;
+  if ((rc = check_fields(in, "input fields")) || (rc = check_fields(out, "output fields")))
+    return rc;
+  for (int p = 0; p < 3; ++p)
+    for (int q = 0; q < 3; ++q)
+      if (in[p] == out[q]) return fail(TM2D_EINVAL, "input and output buffers must differ");
+  if (d->n_src > 0 && (n < 0 || n >= d->table_steps))
+    return fail(TM2D_EINVAL, "step index outside the source table");
+  return launch(d, in, out, n, (hipStream_t)stream);
+}
+
+int tm2d_run(const Tm2dDesc* d, float* const* a, float* const* b, int32_t n0, int32_t nsteps,
+             void* stream) {
+  int rc = validate(d);
+  if (rc) return rc;
+  if ((rc = check_fields(a, "buffer set a")) || (rc = check_fields(b, "buffer set b"))) return rc;
+  for (int p = 0; p < 3; ++p)
+    for (int q = 0; q < 3; ++q)
+      if (a[p] == b[q]) return fail(TM2D_EINVAL, "the two buffer sets must differ");
+  if (nsteps < 0 || n0 < 0) return fail(TM2D_EINVAL, "negative step range");
+  if (d->n_src > 0 && (long long)n0 + nsteps > d->table_steps)
+    return fail(TM2D_EINVAL, "step range outside the source table");
+  for (int s = 0; s < nsteps; ++s) {
+    float* const* src = (s % 2 == 0) ? a : b;
+    float* const* dst = (s % 2 == 0) ? b : a;
+    if ((rc = launch(d, src, dst, n0 + s, (hipStream_t)stream))) return rc;
+  }
+  return TM2D_OK;
+}
+
+}  // extern "C"

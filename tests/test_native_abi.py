@@ -79,3 +79,46 @@ def test_plan_and_source_validation_host_only():
         plan.set_sources([4], [(1, 1, 1)], np.zeros((3, 1), np.float32), 3)
     # (a valid call uploads the table to the device: exercised by the GPU tests)
     plan.close()
+
+
+def test_tm2d_library_exports_all_declared_symbols():
+    from jaxphys_amd import _tm2d
+
+    lib = _tm2d.lib()
+    names = _declared("tm2d.h")
+    assert {"tm2d_step", "tm2d_run", "tm2d_last_error"} <= names
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(_tm2d.EXPORTS) == names
+
+
+@pytest.mark.parametrize("kw,msg", [
+    (dict(nx=2), "at least 3x3"),
+    (dict(boundary=5), "unknown boundary"),
+    (dict(npml=3), "damping table"),
+    (dict(n_src=1), "src_rows"),
+    (dict(nx=70000, ny=70000), "2^31"),
+])
+def test_tm2d_rejects_bad_descriptors_host_only(kw, msg):
+    from jaxphys_amd import _tm2d
+
+    d = _tm2d.Tm2dDesc()
+    d.nx, d.ny = kw.get("nx", 16), kw.get("ny", 12)
+    d.boundary = kw.get("boundary", _tm2d.ABSORBING)
+    d.npml = kw.get("npml", 0)
+    d.n_src = kw.get("n_src", 0)
+    f = _tm2d.fields3([None, None, None])
+    rc = _tm2d.lib().tm2d_run(ctypes.byref(d), f, f, 0, 1, None)
+    assert rc == _tm2d.TM2D_EINVAL
+    assert msg in _tm2d.lib().tm2d_last_error().decode()
+
+
+def test_tm2d_rejects_aliased_buffers_host_only():
+    from jaxphys_amd import _tm2d
+
+    d = _tm2d.Tm2dDesc()
+    d.nx, d.ny, d.boundary = 16, 12, _tm2d.REFLECTING
+    a = (ctypes.c_void_p * 3)(0x1000, 0x2000, 0x3000)
+    b = (ctypes.c_void_p * 3)(0x4000, 0x2000, 0x5000)  # shares one buffer with a
+    rc = _tm2d.lib().tm2d_run(ctypes.byref(d), a, b, 0, 2, None)
+    assert rc == _tm2d.TM2D_EINVAL and "differ" in _tm2d.lib().tm2d_last_error().decode()
