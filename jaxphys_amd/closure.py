@@ -91,6 +91,8 @@ class ClosureNet:
         self.eps_shift, self.eps_scale = 1.0, 1.0
         self._packed = None
         self._ws = {}
+        self._saved = {}
+        self.version = 0  # bumped by every weight update
 
     # -- weights in the kernel layout -----------------------------------
     @staticmethod
@@ -117,9 +119,12 @@ class ClosureNet:
         return b
 
     # -- forward / backward ----------------------------------------------
-    def forward(self, x: torch.Tensor, shape_xyz, train: bool = False) -> torch.Tensor:
+    def forward(self, x: torch.Tensor, shape_xyz, train: bool = False,
+                tag: str = "") -> torch.Tensor:
         """``x``: bf16 (npos, 8) channels last over (nz, ny, nx); returns the
-        float32 correction (npos, 8) (channels 0..5 real)."""
+        float32 network output (npos, 8) (channels 0..5 real).  ``tag``
+        names the set of buffers (and saved activations for ``backward``)
+        used, so that two forward passes can be kept at once."""
         if self._packed is None:
             self.pack()
         nx, ny, nz = shape_xyz
@@ -128,22 +133,23 @@ class ClosureNet:
         d1 = _conv.desc(shape_xyz, CIN, HID, 32, HID)
         d2 = _conv.desc(shape_xyz, HID, HID, 32, HID)
         d3 = _conv.desc(shape_xyz, HID, COUT, 16, OUT_STRIDE)
-        a1 = self._buf("a1", (n, HID), torch.bfloat16)
-        a2 = self._buf("a2", (n, HID), torch.bfloat16)
-        z1 = self._buf("z1", (n, HID), torch.bfloat16) if train else None
-        z2 = self._buf("z2", (n, HID), torch.bfloat16) if train else None
-        y = self._buf("y", (n, OUT_STRIDE), torch.float32)
+        a1 = self._buf("a1" + tag, (n, HID), torch.bfloat16)
+        a2 = self._buf("a2" + tag, (n, HID), torch.bfloat16)
+        z1 = self._buf("z1" + tag, (n, HID), torch.bfloat16) if train else None
+        z2 = self._buf("z2" + tag, (n, HID), torch.bfloat16) if train else None
+        y = self._buf("y" + tag, (n, OUT_STRIDE), torch.float32)
         _conv.fwd(d1, x, W["w1"], P["b1"], _conv.EPI_GELU, a1, z_out=z1)
         _conv.fwd(d2, a1, W["w2"], P["b2"], _conv.EPI_GELU, a2, z_out=z2)
         _conv.fwd(d3, a2, W["w3"], P["b3"], _conv.EPI_LINEAR_F32, y)
         if train:
-            self._saved = (x, a1, a2, z1, z2, shape_xyz)
+            self._saved[tag] = (x, a1, a2, z1, z2, shape_xyz)
         return y
 
-    def backward(self, dy: torch.Tensor) -> dict:
-        """``dy``: float32 (npos, 8) gradient of the loss w.r.t. the output.
-        Returns float32 gradients shaped like ``params``."""
-        x, a1, a2, z1, z2, shape_xyz = self._saved
+    def backward(self, dy: torch.Tensor, tag: str = "") -> dict:
+        """``dy``: float32 (npos, 8) gradient of the loss w.r.t. the output
+        of the forward pass ``tag``.  Returns float32 gradients shaped like
+        the network's weights and biases."""
+        x, a1, a2, z1, z2, shape_xyz = self._saved[tag]
         nx, ny, nz = shape_xyz
         n = nx * ny * nz
         W = self._packed
@@ -196,6 +202,7 @@ class ClosureNet:
             vh = v / (1 - b2 ** self.step_count)
             self.params[k].sub_(lr * mh / (vh.sqrt() + eps))
         self._packed = None
+        self.version += 1
 
     # -- fields <-> network tensors --------------------------------------
     def pack_fields(self, solver, m: int | None = None, every: int | None = None,
@@ -224,12 +231,40 @@ class ClosureNet:
                 every: int | None = None) -> None:
         """fields += net(fields, eps[, forcing of steps m .. m+every-1])
         (denormalised)."""
+        y0 = self.zero_response(solver)
         x = self.pack_fields(solver, m, every)
         y = self.forward(x, (solver.nx, solver.ny, solver.nz))
+        y.sub_(y0)
         if self.forcing:
             self.add_stencil(y, solver, (solver.nx, solver.ny, solver.nz), (0, 0, 0), m, every)
         _conv.apply_correction((solver.nx, solver.ny, solver.nz), solver.fields, y, OUT_STRIDE,
                                self.scale, gain, self.margin)
+
+    def pack_zero(self, solver) -> torch.Tensor:
+        """Network input of the zero state of the solver's grid (eps_r only):
+        a bf16 (npos, 8) tensor of its own."""
+        shape = (solver.nz, solver.ny, solver.nx)
+        zeros = [torch.zeros(shape, dtype=torch.float32, device=self.device)] * 6
+        x0 = torch.empty(solver.nx * solver.ny * solver.nz, CIN, dtype=torch.bfloat16,
+                         device=self.device)
+        _conv.pack_fields((solver.nx, solver.ny, solver.nz), zeros, solver.eps_r, self.scale,
+                          self.eps_shift, self.eps_scale, x0)
+        return x0
+
+    def zero_response(self, solver) -> torch.Tensor:
+        """The network's output for zero fields and zero forcing on this
+        grid (it depends on eps_r only), cached on the solver until the
+        weights change.  Every correction subtracts it, so the correction
+        of a zero state is exactly zero: no constant is added to the
+        fields at every correction."""
+        key = (id(self), self.version)
+        cache = getattr(solver, "_closure_y0", None)
+        if cache is not None and cache[0] == key:
+            return cache[1]
+        x0 = self.pack_zero(solver)
+        y0 = self.forward(x0, (solver.nx, solver.ny, solver.nz), tag="0").clone()
+        solver._closure_y0 = (key, y0)
+        return y0
 
     def _stencil_terms(self, solver, shape_xyz, origin, every):
         """(flat indices into the box, stencil slot, source-cell number) of
@@ -526,6 +561,7 @@ def train_closure(device, seeds, n_fine: int = 256, n_coarse: int = 400, every: 
             # crops whose loss region (crop minus its b-cell border) starts at
             # the correction margin
             margin = max(0, net.margin - b)
+            x0 = net.pack_zero(csol).view(nc, nc, nc, CIN)
             inputs = None
             if onpolicy:
                 # the corrected rollout's own states before each correction;
@@ -563,7 +599,11 @@ def train_closure(device, seeds, n_fine: int = 256, n_coarse: int = 400, every: 
                 sl = (slice(cz, cz + crop), slice(cy, cy + crop), slice(cx, cx + crop))
                 xc = x[sl].contiguous().view(-1, CIN)
                 tc = tgt[sl]
+                # correction = net(state) - net(zero state)
                 y = net.forward(xc, (crop, crop, crop), train=True).view(crop, crop, crop, 8)
+                y0 = net.forward(x0[sl].contiguous().view(-1, CIN), (crop, crop, crop),
+                                 train=True, tag="0").view(crop, crop, crop, 8)
+                y = y - y0
                 terms = (net.add_stencil(y, csol, (crop, crop, crop), (cx, cy, cz), m, every)
                          if net.forcing else None)
                 err = torch.zeros_like(y)
@@ -576,7 +616,11 @@ def train_closure(device, seeds, n_fine: int = 256, n_coarse: int = 400, every: 
                 upd += 1
                 if not math.isfinite(loss):
                     continue  # a sample from a diverged state: no update
-                grads = net.backward((2.0 / n_in) * err.view(-1, 8))
+                g_out = (2.0 / n_in) * err.view(-1, 8)
+                grads = net.backward(g_out)
+                grads0 = net.backward(g_out, tag="0")
+                for k_ in grads:
+                    grads[k_] = grads[k_] - grads0[k_]
                 if net.forcing:
                     grads["ks"] = (net.stencil_grad(err, terms, 2.0 / n_in) if terms is not None
                                    else torch.zeros_like(net.params["ks"]))

@@ -11,17 +11,17 @@
 //     (ri(i), rj(j)), with r the identity inside and, at the edges,
 //     reflecting 0 -> 1, n-1 -> n-2, periodic 0 -> n-2, n-1 -> 1;
 //   * E at (a, b) needs the new H at (a, b), (a-1, b) and (a, b-1): the
-//     thread recomputes the two neighbour values from A (their loads hit the
-//     cache lines its wavefront and the previous row already brought in);
+//     thread recomputes the two neighbour values from A (or carries / takes
+//     them from the adjacent lane, see tm2d_step_kernel);
 //   * damping, the row sources and the conductor mask act at (i, j).
 //
 // Each float32 operation is the reference's, in its order, and the build
 // uses -ffp-contract=off: the result is bit-identical to the float32
 // restatement.  A workgroup is one 64-lane x row segment by 4 wavefronts;
-// each wavefront walks 4 consecutive rows, so the tile is 64 x 16 cells and
-// the row above a wavefront's current row is still in L1.  HBM traffic per
-// cell: Ez, Hx, Hy read once and written once (24 B), plus one damping
-// float in the absorbing band and one conductor byte when walls exist.
+// each wavefront walks 8 consecutive rows, so the tile is 64 x 32 cells.
+// HBM traffic per cell: Ez, Hx, Hy read once and written once (24 B), plus
+// one damping float in the absorbing band and one conductor byte when
+// walls exist.
 
 #include <hip/hip_runtime.h>
 
@@ -46,7 +46,7 @@ static int fail(int code, const std::string& msg) {
 
 constexpr int kTx = 64;  // lanes along x
 constexpr int kTy = 4;   // wavefronts per workgroup
-constexpr int kRows = 4; // rows per wavefront
+constexpr int kRows = 8; // rows per wavefront
 
 struct StepArgs {
   int nx, ny, npml;
@@ -65,6 +65,79 @@ __device__ __forceinline__ int remap(int i, int n) {
   return i;
 }
 
+// The new (Ez, Hx, Hy) of output cell (i, j), from buffer A, for any cell
+// (edge copies, wrap and sliced ranges included).
+template <int MODE>
+__device__ __forceinline__ void cell_general(const StepArgs& g, const float* __restrict__ ez,
+                                             const float* __restrict__ hx,
+                                             const float* __restrict__ hy, int i, int j,
+                                             float& ezn_out, float& hxn_out, float& hyn_out) {
+  constexpr bool kPer = MODE == TM2D_PERIODIC;
+  const int nx = g.nx, ny = g.ny;
+  const int a = remap<MODE>(i, nx);
+  // x / y neighbours of (a, b): wrapped (periodic) or clamped; a clamped one
+  // is loaded but not used (the update it would feed is the identity there)
+  const int ap = kPer ? (a + 1 == nx ? 0 : a + 1) : min(a + 1, nx - 1);
+  const int am = kPer ? (a == 0 ? nx - 1 : a - 1) : max(a - 1, 0);
+  const int b = remap<MODE>(j, ny);
+  const int bp = kPer ? (b + 1 == ny ? 0 : b + 1) : min(b + 1, ny - 1);
+  const int bm = kPer ? (b == 0 ? ny - 1 : b - 1) : max(b - 1, 0);
+  const int c = b * nx + a;
+  const float e0 = ez[c];
+  const float e_xp = ez[b * nx + ap];
+  const float e_yp = ez[bp * nx + a];
+  const float hy0 = hy[c];
+  const float hx0 = hx[c];
+  // fdtd.py:308-309 / 316-325: the new H at (a, b)
+  const float hyn = (kPer || a <= nx - 2) ? hy0 + g.ch * (e_xp - e0) : hy0;
+  const float hxn = (kPer || b <= ny - 2) ? hx0 - g.ch * (e_yp - e0) : hx0;
+  float ezn = e0;
+  if (kPer || (a >= 1 && b >= 1)) {
+    // the new H at (a-1, b) and (a, b-1), recomputed from A
+    const float e_xm = ez[b * nx + am];
+    const float e_ym = ez[bm * nx + a];
+    const float hyn_m = hy[b * nx + am] + g.ch * (e0 - e_xm);
+    const float hxn_m = hx[bm * nx + a] - g.ch * (e0 - e_ym);
+    // fdtd.py:311-313 / 328-335
+    ezn = e0 + g.ce * ((hyn - hyn_m) - (hxn - hxn_m));
+  }
+  ezn_out = ezn;
+  hxn_out = hxn;
+  hyn_out = hyn;
+}
+
+// Damping, row sources and conductors at output cell (i, j), then the store.
+template <int MODE>
+__device__ __forceinline__ void finish_cell(const StepArgs& g, int i, int j, bool in_x_band,
+                                            float ezn, float hxn, float hyn,
+                                            float* __restrict__ ez_out,
+                                            float* __restrict__ hx_out,
+                                            float* __restrict__ hy_out) {
+  const int o = j * g.nx + i;
+  if (MODE == TM2D_ABSORBING) {
+    // fdtd.py:337-339; outside the band the factor is exp(-0) = 1
+    if (in_x_band || j < g.npml || j >= g.ny - g.npml) ezn = ezn * g.damp[o];
+  }
+  // fdtd.py:249-254, 349: row sources in insertion order
+  for (int s = 0; s < g.n_src; ++s)
+    if (g.rows[s] == j) ezn = ezn + g.vals[s];
+  // fdtd.py:351-352
+  if (g.cond != nullptr && g.cond[o]) ezn = 0.0f;
+  ez_out[o] = ezn;
+  hx_out[o] = hxn;
+  hy_out[o] = hyn;
+}
+
+// One workgroup: kTy wavefronts, each a 64-column strip of kRows rows.  A
+// wavefront whose 64 columns all exist and whose rows all lie in
+// [1, ny-2] (no edge copies, no wrap in y) marches down its rows carrying
+// Ez(y-1), Ez(y) and the new Hx(y-1) in registers and taking the x
+// neighbours Ez(x+-1), Hy(x-1) from the adjacent lanes (their loads are the
+// same addresses); it loads Ez(y+1), Hx and Hy once per row.  Lanes whose
+// x neighbour is not the adjacent lane's column (the wavefront's first and
+// last lane, the edge-copy columns) load it.  Every other wavefront runs
+// the general per-cell path.  Both paths perform the same float32
+// operations in the same order.
 template <int MODE>
 __global__ __launch_bounds__(kTx * kTy) void tm2d_step_kernel(
     StepArgs g, const float* __restrict__ ez, const float* __restrict__ hx,
@@ -72,55 +145,58 @@ __global__ __launch_bounds__(kTx * kTy) void tm2d_step_kernel(
     float* __restrict__ hy_out) {
   constexpr bool kPer = MODE == TM2D_PERIODIC;
   const int nx = g.nx, ny = g.ny;
-  const int i = blockIdx.x * kTx + threadIdx.x;
-  if (i >= nx) return;
+  const int lane = threadIdx.x;
+  const int x0 = blockIdx.x * kTx;
+  const int i = x0 + lane;
+  const int j0 = (blockIdx.y * kTy + threadIdx.y) * kRows;
+  const bool in_x_band = i < g.npml || i >= nx - g.npml;
+  const bool fast = x0 + kTx <= nx && j0 >= 1 && j0 + kRows <= ny - 1;
+  if (!fast) {
+    if (i >= nx) return;
+    for (int r = 0; r < kRows; ++r) {
+      const int j = j0 + r;
+      if (j >= ny) break;
+      float ezn, hxn, hyn;
+      cell_general<MODE>(g, ez, hx, hy, i, j, ezn, hxn, hyn);
+      finish_cell<MODE>(g, i, j, in_x_band, ezn, hxn, hyn, ez_out, hx_out, hy_out);
+    }
+    return;
+  }
   const int a = remap<MODE>(i, nx);
-  // x neighbours of column a: wrapped (periodic) or clamped; a clamped one is
-  // loaded but not used (the update it would feed is the identity there)
   const int ap = kPer ? (a + 1 == nx ? 0 : a + 1) : min(a + 1, nx - 1);
   const int am = kPer ? (a == 0 ? nx - 1 : a - 1) : max(a - 1, 0);
+  // the adjacent lane holds column ap / am exactly when its own column is it
+  const bool xp_lane = lane < kTx - 1 && remap<MODE>(i + 1, nx) == ap;
+  const bool xm_lane = lane > 0 && remap<MODE>(i - 1, nx) == am;
   const bool upd_hy = kPer || a <= nx - 2;
-  const bool in_x_band = i < g.npml || i >= nx - g.npml;
-  const int j0 = (blockIdx.y * kTy + threadIdx.y) * kRows;
+  const bool upd_e = kPer || a >= 1;
+  int b = j0;
+  float e_ym = ez[(b - 1) * nx + a];
+  float e_c = ez[b * nx + a];
+  // the new Hx at (a, b-1); b-1 <= ny-2 always holds here
+  float hxn_m = hx[(b - 1) * nx + a] - g.ch * (e_c - e_ym);
 #pragma unroll
-  for (int r = 0; r < kRows; ++r) {
-    const int j = j0 + r;
-    if (j >= ny) break;
-    const int b = remap<MODE>(j, ny);
-    const int bp = kPer ? (b + 1 == ny ? 0 : b + 1) : min(b + 1, ny - 1);
-    const int bm = kPer ? (b == 0 ? ny - 1 : b - 1) : max(b - 1, 0);
-    const int c = b * nx + a;
-    const float e0 = ez[c];
-    const float e_xp = ez[b * nx + ap];
-    const float e_yp = ez[bp * nx + a];
-    const float hy0 = hy[c];
-    const float hx0 = hx[c];
-    // fdtd.py:308-309 / 316-325: the new H at (a, b)
-    const float hyn = upd_hy ? hy0 + g.ch * (e_xp - e0) : hy0;
-    const float hxn = (kPer || b <= ny - 2) ? hx0 - g.ch * (e_yp - e0) : hx0;
-    float ezn = e0;
-    if (kPer || (a >= 1 && b >= 1)) {
-      // the new H at (a-1, b) and (a, b-1), recomputed from A
-      const float e_xm = ez[b * nx + am];
-      const float e_ym = ez[bm * nx + a];
-      const float hyn_m = hy[b * nx + am] + g.ch * (e0 - e_xm);
-      const float hxn_m = hx[bm * nx + a] - g.ch * (e0 - e_ym);
-      // fdtd.py:311-313 / 328-335
-      ezn = e0 + g.ce * ((hyn - hyn_m) - (hxn - hxn_m));
+  for (int r = 0; r < kRows; ++r, ++b) {
+    const float e_yp = ez[(b + 1) * nx + a];
+    const float hy0 = hy[b * nx + a];
+    const float hx0 = hx[b * nx + a];
+    const float sh_xp = __shfl_down(e_c, 1);
+    const float sh_xm = __shfl_up(e_c, 1);
+    const float sh_hym = __shfl_up(hy0, 1);
+    const float e_xp = xp_lane ? sh_xp : ez[b * nx + ap];
+    const float e_xm = xm_lane ? sh_xm : ez[b * nx + am];
+    const float hy_m = xm_lane ? sh_hym : hy[b * nx + am];
+    const float hyn = upd_hy ? hy0 + g.ch * (e_xp - e_c) : hy0;
+    const float hxn = hx0 - g.ch * (e_yp - e_c);
+    float ezn = e_c;
+    if (upd_e) {
+      const float hyn_m = hy_m + g.ch * (e_c - e_xm);
+      ezn = e_c + g.ce * ((hyn - hyn_m) - (hxn - hxn_m));
     }
-    const int o = j * nx + i;
-    if (MODE == TM2D_ABSORBING) {
-      // fdtd.py:337-339; outside the band the factor is exp(-0) = 1
-      if (in_x_band || j < g.npml || j >= ny - g.npml) ezn = ezn * g.damp[o];
-    }
-    // fdtd.py:249-254, 349: row sources in insertion order
-    for (int s = 0; s < g.n_src; ++s)
-      if (g.rows[s] == j) ezn = ezn + g.vals[s];
-    // fdtd.py:351-352
-    if (g.cond != nullptr && g.cond[o]) ezn = 0.0f;
-    ez_out[o] = ezn;
-    hx_out[o] = hxn;
-    hy_out[o] = hyn;
+    finish_cell<MODE>(g, i, b, in_x_band, ezn, hxn, hyn, ez_out, hx_out, hy_out);
+    e_ym = e_c;
+    e_c = e_yp;
+    hxn_m = hxn;
   }
 }
 
