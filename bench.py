@@ -337,6 +337,9 @@ def closure_bench(args, dev):
                    "lr": [args.closure_lr, args.closure_lr_final]},
         "err_coarse_no_closure": round(float(np.mean(err_0)), 5),
         "err_far_closure": round(float(np.mean([r["err_far_closure"] for r in res])), 5),
+        "source_model_only": {k: round(float(np.mean([r.get(f"{k}_source_only", float("nan"))
+                                                      for r in res])), 5)
+                              for k in ("err", "err_far", "err_traj", "err_traj_far")},
         "err_far_coarse_no_closure": round(float(np.mean([r["err_far_coarse"] for r in res])), 5),
         "err_traj_far_closure": round(float(np.mean([r["err_traj_far_closure"] for r in res])), 5),
         "err_traj_far_coarse_no_closure": round(float(np.mean([r["err_traj_far_coarse"]
@@ -344,6 +347,11 @@ def closure_bench(args, dev):
         "err_traj_closure": round(float(np.mean(trj_c)), 5),
         "err_traj_coarse_no_closure": round(float(np.mean(trj_0)), 5),
         "err_closure_max": round(float(np.max(err_c)), 5),
+        "per_seed": [{"seed": r["seed"], "err": round(r["err_closure"], 5),
+                      "err_source_only": round(r.get("err_source_only", float("nan")), 5),
+                      "err_far": round(r["err_far_closure"], 5),
+                      "err_far_source_only": round(r.get("err_far_source_only", float("nan")), 5)}
+                     for r in res],
         "speedup_vs_fine": round(float(np.mean(sp)), 3),
         "t_fine_s": round(float(np.mean([r["t_fine_s"] for r in res])), 4),
         "t_closure_s": round(float(np.mean([r["t_closure_s"] for r in res])), 4),

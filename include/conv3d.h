@@ -64,9 +64,14 @@ int conv3d_flip_weights(const ConvDesc* d, const void* w, int32_t dy_cpad, void*
 
 /* Pack the six Yee fields and eps_r (float32, x fastest, (nz,ny,nx)) into a
  * bf16 channels-last tensor with 8 channels: c = field / scale[c] for
- * c < 6, eps channel (eps - eps_shift) / eps_scale, channel 7 = 0.       */
+ * c < 6 (with CONV_PACK_ASINH in `flags`: asinh(field / scale[c]), which
+ * keeps the values next to a point source, thousands of times the RMS, in
+ * the range of the rest), eps channel (eps - eps_shift) / eps_scale,
+ * channel 7 = 0.                                                          */
+#define CONV_PACK_ASINH 1
 int conv3d_pack_fields(int32_t nx, int32_t ny, int32_t nz, float* const* fields, const float* eps,
-                       const float* scale, float eps_shift, float eps_scale, void* out, void* stream);
+                       const float* scale, float eps_shift, float eps_scale, int32_t flags,
+                       void* out, void* stream);
 /* fields[c] += gain * scale[c] * y[..][c], y float32 channels last [..][ystride],
  * for the cells at least `margin` cells from every face; the others (the
  * absorbing layers, where the closure is not trained) are left unchanged. */

@@ -227,6 +227,16 @@ int yee_adjoint_run(YeePlan* plan, float* const* lam, float* lam_psi, int32_t n_
 int yee_energy_fp64(int32_t nx, int32_t ny, int32_t nz, float* const* fields,
                     const float* eps_r, double* work, void* stream);
 
+/* Float64 Poynting flux through the plane normal to `axis` (0 x, 1 y, 2 z)
+ * halfway between the E planes `plane` and `plane+1`, where the tangential
+ * H components live: work[0] = sum over the plane of (E x H) . n, each
+ * tangential E the mean of its two planes (no reference counterpart; for
+ * energy-balance checks and field diagnostics, dx^2 times it is the power
+ * through the plane).  `work` holds YEE_FLUX_WORK doubles; deterministic.  */
+#define YEE_FLUX_WORK (1 + 1024)
+int yee_poynting_flux_fp64(int32_t nx, int32_t ny, int32_t nz, float* const* fields,
+                           int32_t axis, int32_t plane, double* work, void* stream);
+
 /* Plain device copy of n floats, used to calibrate HBM byte counters
  * (no reference counterpart).                                            */
 int yee_copy_probe(const float* src, float* dst, int64_t n, void* stream);

@@ -33,8 +33,8 @@ EXPORTS = {
                                            ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p]),
     "conv3d_pack_fields": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
-                                          ctypes.c_float, ctypes.c_float, ctypes.c_void_p,
-                                          ctypes.c_void_p]),
+                                          ctypes.c_float, ctypes.c_float, ctypes.c_int32,
+                                          ctypes.c_void_p, ctypes.c_void_p]),
     "conv3d_apply_correction": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                                ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
                                                ctypes.c_void_p, ctypes.c_float, ctypes.c_int32,
@@ -98,10 +98,14 @@ def flip_weights(d: ConvDesc, w, dy_cpad, wt, stream=None) -> None:
                                     stream_handle(stream)), "conv3d_flip_weights")
 
 
-def pack_fields(shape_xyz, fields, eps, scale, eps_shift, eps_scale, out, stream=None) -> None:
+PACK_ASINH = 1
+
+
+def pack_fields(shape_xyz, fields, eps, scale, eps_shift, eps_scale, out, flags=0,
+                stream=None) -> None:
     nx, ny, nz = shape_xyz
     check(lib().conv3d_pack_fields(nx, ny, nz, field_array(fields), ptr(eps), ptr(scale),
-                                   float(eps_shift), float(eps_scale), ptr(out),
+                                   float(eps_shift), float(eps_scale), int(flags), ptr(out),
                                    stream_handle(stream)), "conv3d_pack_fields")
 
 

@@ -65,6 +65,9 @@ EXPORTS = {
                                        ctypes.c_void_p]),
     "yee_copy_probe": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
                                       ctypes.c_void_p]),
+    "yee_poynting_flux_fp64": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                              ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
+                                              ctypes.c_void_p, ctypes.c_void_p]),
     "yee_last_error": (ctypes.c_char_p, []),
     "yee_step_fused": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
@@ -296,6 +299,21 @@ def energy_fp64(shape_xyz, fields, eps_r=None, work=None, stream=None):
     check(lib().yee_energy_fp64(nx, ny, nz, field_array(fields), ptr(eps_r) or None,
                                 ptr(work), stream_handle(stream)), "yee_energy_fp64")
     return work[:2]
+
+
+FLUX_WORK = 1 + 1024
+
+
+def poynting_flux_fp64(shape_xyz, fields, axis: int, plane: int, work=None, stream=None):
+    """float64 sum over a Yee H plane of (E x H) . n (see include/yee.h);
+    returns a device tensor of one double (asynchronous)."""
+    nx, ny, nz = shape_xyz
+    if work is None:
+        work = torch.empty(FLUX_WORK, dtype=torch.float64, device=fields[0].device)
+    check(lib().yee_poynting_flux_fp64(nx, ny, nz, field_array(fields), int(axis), int(plane),
+                                       ptr(work), stream_handle(stream)),
+          "yee_poynting_flux_fp64")
+    return work[:1]
 
 
 def copy_probe(src: torch.Tensor, dst: torch.Tensor, stream=None) -> None:

@@ -13,9 +13,19 @@ charge behind, and the fine- and coarse-grid near fields of that charge
 differ by a fixed local pattern times the injected amount, which the net can
 only place where it knows a source acted.  That part of the correction is
 linear in the forcing and as large as the fields themselves next to the
-source, far outside the range the bf16 GELU layers resolve, so it goes
-through a separate float32 linear 5x5x5 stencil of the forcing
-(``params["ks"]``, trained with the rest).  Corrections are applied only
+source, far outside the range the bf16 GELU layers resolve, so it is a
+separate float32 linear source model: in the 15x15x15 box around each
+source cell, correction = K0 f0 + K1 f1 + K2 f0 (eps_s - 1), with f0 the
+interval's injected E increment, f1 its first time moment and eps_s the
+permittivity at the source cell (the sources add to E itself, so in a
+uniform medium the static field they leave per increment does not depend
+on eps; the third term lets the fit learn how a dielectric at the source
+changes it); K0..K2 (``ks``) are fitted by least squares on the training
+samples (a linear regression, solved exactly after every epoch).  The net
+is trained on, and corrects, the cells outside those boxes.  The field
+channels enter as asinh(field / RMS), so the near-source values (thousands
+of RMS) stay in the range of the rest.  The net's response to the zero
+state (eps_r only) is subtracted, so a zero field gets no correction.  Corrections are applied only
 ``margin`` cells or more inside the faces: the absorbing layers, where the
 net is never trained, are left to the solver.  Weights and activations are bfloat16 with
 float32 accumulation on the matrix cores (libconv3d_hip.so); float32 master
@@ -48,7 +58,8 @@ from jaxphys_amd.em.fdtd3d import DielectricRegion, EMGrid3D, PointSource3D
 CIN, HID, COUT = 8, 32, 6          # padded input channels (7 real), hidden, output
 KP1, KP2 = _conv.kpad_of(CIN), _conv.kpad_of(HID)
 OUT_STRIDE = 8                      # float32 output channel stride
-STENCIL = 5                         # side of the linear forcing stencil
+STENCIL = 15                        # side of the linear source-model box
+NFEAT = 3                           # source features: sum, first moment, sum * (eps - 1)
 _OFFS = [(dx, dy, dz) for dz in range(-(STENCIL // 2), STENCIL // 2 + 1)
          for dy in range(-(STENCIL // 2), STENCIL // 2 + 1)
          for dx in range(-(STENCIL // 2), STENCIL // 2 + 1)]
@@ -65,6 +76,7 @@ class ClosureNet:
         self.device = torch.device(device)
         self.forcing = bool(forcing)
         self.force_scale = 1.0
+        self.pack_flags = _conv.PACK_ASINH
         self.n_in = 8 if self.forcing else 7
         g = torch.Generator(device="cpu").manual_seed(seed)
 
@@ -81,8 +93,8 @@ class ClosureNet:
             "w3": init(COUT, HID, HID, out_scale * math.sqrt(1.0 / (27 * HID))),
             "b3": torch.zeros(COUT, device=self.device),
         }
-        if self.forcing:
-            self.params["ks"] = torch.zeros(COUT, STENCIL ** 3, device=self.device)
+        # the linear source model (physical units; least squares, not Adam)
+        self.ks = torch.zeros(NFEAT, COUT, STENCIL ** 3, device=self.device)
         self.margin = 0
         self.adam = {k: (torch.zeros_like(v), torch.zeros_like(v)) for k, v in self.params.items()}
         self.step_count = 0
@@ -214,29 +226,38 @@ class ClosureNet:
         x = self._buf("xin", (n, CIN), torch.bfloat16)
         _conv.pack_fields((solver.nx, solver.ny, solver.nz),
                           solver.fields if fields is None else fields, solver.eps_r,
-                          self.scale, self.eps_shift, self.eps_scale, x)
+                          self.scale, self.eps_shift, self.eps_scale, x, self.pack_flags)
         if self.forcing:
             if m is None or every is None:
                 raise ValueError("a closure net with the forcing channel needs the interval")
-            idx, vals, _ = interval_forcing(solver, every)
-            if idx is not None:
+            fc = interval_forcing(solver, every)
+            if fc is not None:
                 q = m // every
-                if m % every or q >= vals.shape[0]:
+                if m % every or q >= fc.feats.shape[0]:
                     raise ValueError(f"interval start {m} is not a multiple of {every} "
                                      "inside the source table")
-                x[idx, 7] = (vals[q] * (1.0 / self.force_scale)).to(torch.bfloat16)
+                x[fc.flat, 7] = (fc.feats[q, :, 0] * (1.0 / self.force_scale)).to(torch.bfloat16)
         return x
 
     def correct(self, solver, gain: float = 1.0, m: int | None = None,
-                every: int | None = None) -> None:
+                every: int | None = None, use_net: bool = True) -> None:
         """fields += net(fields, eps[, forcing of steps m .. m+every-1])
-        (denormalised)."""
-        y0 = self.zero_response(solver)
-        x = self.pack_fields(solver, m, every)
-        y = self.forward(x, (solver.nx, solver.ny, solver.nz))
-        y.sub_(y0)
+        (denormalised) outside the source boxes, plus the linear source
+        model inside them.  ``use_net=False`` applies the source model only
+        (for measuring what the net adds)."""
+        if use_net:
+            y0 = self.zero_response(solver)
+            x = self.pack_fields(solver, m, every)
+            y = self.forward(x, (solver.nx, solver.ny, solver.nz))
+            y.sub_(y0)
+        else:
+            y = self._buf("y", (solver.nx * solver.ny * solver.nz, OUT_STRIDE), torch.float32)
+            y.zero_()
         if self.forcing:
-            self.add_stencil(y, solver, (solver.nx, solver.ny, solver.nz), (0, 0, 0), m, every)
+            shape = (solver.nx, solver.ny, solver.nz)
+            box = self._box_terms(solver, shape, (0, 0, 0), every)[0]
+            y.view(-1, OUT_STRIDE)[box] = 0.0
+            self.add_source_model(y, solver, shape, (0, 0, 0), m, every)
         _conv.apply_correction((solver.nx, solver.ny, solver.nz), solver.fields, y, OUT_STRIDE,
                                self.scale, gain, self.margin)
 
@@ -248,7 +269,7 @@ class ClosureNet:
         x0 = torch.empty(solver.nx * solver.ny * solver.nz, CIN, dtype=torch.bfloat16,
                          device=self.device)
         _conv.pack_fields((solver.nx, solver.ny, solver.nz), zeros, solver.eps_r, self.scale,
-                          self.eps_shift, self.eps_scale, x0)
+                          self.eps_shift, self.eps_scale, x0, self.pack_flags)
         return x0
 
     def zero_response(self, solver) -> torch.Tensor:
@@ -266,18 +287,18 @@ class ClosureNet:
         solver._closure_y0 = (key, y0)
         return y0
 
-    def _stencil_terms(self, solver, shape_xyz, origin, every):
-        """(flat indices into the box, stencil slot, source-cell number) of
-        the 5^3 neighbourhoods of the source cells inside a box of
-        ``shape_xyz`` at ``origin`` (device tensors, cached per box)."""
+    def _box_terms(self, solver, shape_xyz, origin, every):
+        """(flat indices into the box of ``shape_xyz`` at ``origin``, slot in
+        the source-model box, source-cell number) of the cells of each
+        source's box that lie inside it (device tensors, cached)."""
         key = (tuple(shape_xyz), tuple(origin), every)
-        cache = getattr(solver, "_closure_stencil", None)
-        if cache is not None and cache[0] == key:
-            return cache[1]
-        _, _, cells = interval_forcing(solver, every)
+        cache = getattr(solver, "_closure_boxes", {})
+        if key in cache:
+            return cache[key]
+        fc = interval_forcing(solver, every)
         nx, ny, nz = shape_xyz
         idx, slot, which = [], [], []
-        for c_i, (i, j, k) in enumerate(cells or []):
+        for c_i, (i, j, k) in enumerate(fc.xyz if fc is not None else []):
             for o, (dx, dy, dz) in enumerate(_OFFS):
                 x_, y_, z_ = i + dx - origin[0], j + dy - origin[1], k + dz - origin[2]
                 if 0 <= x_ < nx and 0 <= y_ < ny and 0 <= z_ < nz:
@@ -288,63 +309,74 @@ class ClosureNet:
         out = (torch.tensor(idx, dtype=torch.long, device=dev),
                torch.tensor(slot, dtype=torch.long, device=dev),
                torch.tensor(which, dtype=torch.long, device=dev))
-        solver._closure_stencil = (key, out)
+        if len(cache) > 64:
+            cache = {}
+        cache[key] = out
+        solver._closure_boxes = cache
         return out
 
-    def add_stencil(self, y, solver, shape_xyz, origin, m, every):
-        """y[.., :6] += ks * forcing around each source cell (box-relative)."""
-        _, vals, _ = interval_forcing(solver, every)
-        if vals is None:
-            return None
-        idx, slot, which = self._stencil_terms(solver, shape_xyz, origin, every)
+    def add_source_model(self, y, solver, shape_xyz, origin, m, every) -> None:
+        """y[.., :6] += (K0 f0 + K1 f1) / scale in the box around each source
+        cell (box-relative indices)."""
+        fc = interval_forcing(solver, every)
+        if fc is None:
+            return
+        idx, slot, which = self._box_terms(solver, shape_xyz, origin, every)
         if idx.numel() == 0:
-            return None
-        f = vals[m // every] * (1.0 / self.force_scale)
-        contrib = self.params["ks"][:, slot].T * f[which][:, None]
+            return
+        phi = source_features(fc, m // every)                    # (cells, NFEAT)
+        k = self.ks[:, :, slot]                                 # (NFEAT, 6, E)
+        contrib = (k * phi[which].T[:, None, :]).sum(0).T / self.scale[None, :]
         y.view(-1, OUT_STRIDE)[:, :COUT].index_add_(0, idx, contrib)
-        return idx, slot, which, f
-
-    def stencil_grad(self, err, terms, scale):
-        """d loss / d ks for ``err`` = d loss / d y (crop, 8)."""
-        idx, slot, which, f = terms
-        g = err.reshape(-1, OUT_STRIDE)[idx, :COUT] * f[which][:, None] * scale
-        d = torch.zeros(STENCIL ** 3, COUT, device=self.device)
-        d.index_add_(0, slot, g)
-        return d.T.contiguous()
 
     def state_dict(self):
         return {"params": {k: v.detach().cpu() for k, v in self.params.items()},
                 "scale": self.scale.cpu(), "eps_shift": self.eps_shift,
                 "eps_scale": self.eps_scale, "forcing": self.forcing,
-                "force_scale": self.force_scale, "margin": self.margin}
+                "force_scale": self.force_scale, "margin": self.margin,
+                "ks": self.ks.cpu()}
+
+
+class _Forcing:
+    """Per-interval source features of one solver (see interval_forcing)."""
+
+    def __init__(self, flat, feats, xyz, eps):
+        self.flat, self.feats, self.xyz, self.eps = flat, feats, xyz, eps
 
 
 def interval_forcing(solver, every: int):
-    """The point sources' E increments summed over each interval of
-    ``every`` steps (intervals start at multiples of ``every``), per distinct
-    source cell: (flat cell indices, float32 (n_intervals, n_cells)) on the
-    solver's device and the cells' (i, j, k), or (None, None, None) without
-    sources.  Cached on the solver."""
+    """The point sources' E increments over each interval of ``every`` steps
+    (intervals start at multiples of ``every``), per distinct source cell:
+    ``flat`` cell indices, ``feats`` float32 (n_intervals, n_cells, 2) =
+    (sum of the increments, their first moment sum((n - n_mid)/every * s_n)),
+    the cells' (i, j, k) and eps_r there; None without sources.  Cached on
+    the solver."""
     cache = getattr(solver, "_closure_forcing", None)
     if cache is not None and cache[0] == every:
         return cache[1]
     tab = np.asarray(getattr(solver, "source_table", np.zeros((0, 0))), np.float64)
     if tab.size == 0:
-        out = (None, None, None)
+        out = None
     else:
         nq = -(-tab.shape[0] // every)
         pad = np.zeros((nq * every, tab.shape[1]))
         pad[:tab.shape[0]] = tab
-        per = pad.reshape(nq, every, -1).sum(axis=1)
+        per = pad.reshape(nq, every, -1)
+        w = (np.arange(every) - (every - 1) / 2.0) / every
+        f0 = per.sum(axis=1)
+        f1 = (per * w[None, :, None]).sum(axis=1)
         cells: dict = {}
         for s_i, (i, j, k) in enumerate(solver.source_positions):
             cells.setdefault((k * solver.ny + j) * solver.nx + i, []).append(s_i)
         flat = sorted(cells)
-        vals = np.stack([per[:, cells[c]].sum(axis=1) for c in flat], axis=1)
+        feats = np.stack([np.stack([f[:, cells[c]].sum(axis=1) for c in flat], axis=1)
+                          for f in (f0, f1)], axis=2)
         xyz = [(c % solver.nx, (c // solver.nx) % solver.ny, c // (solver.nx * solver.ny))
                for c in flat]
-        out = (torch.tensor(flat, dtype=torch.long, device=solver.device),
-               torch.tensor(vals, dtype=torch.float32, device=solver.device), xyz)
+        dev = solver.device
+        flat_t = torch.tensor(flat, dtype=torch.long, device=dev)
+        eps = solver.eps_r.reshape(-1)[flat_t].to(torch.float32)
+        out = _Forcing(flat_t, torch.tensor(feats, dtype=torch.float32, device=dev), xyz, eps)
     solver._closure_forcing = (every, out)
     return out
 
@@ -503,21 +535,65 @@ def set_normalisation(net: ClosureNet, states: list) -> None:
 def _crop_start(n: int, margin: int, crop: int, rng, centre: int | None = None,
                 border: int = 3) -> int:
     """Start of a crop of ``crop`` cells inside [margin, n - margin); with
-    ``centre``, one whose loss region (the crop minus ``border``) holds it."""
+    ``centre``, one whose loss region (the crop minus ``border``) holds the
+    whole source-model box around it when the crop is large enough, else
+    the centre itself."""
     lo, hi = margin, n - margin - crop
     if hi < lo:
         return max(0, (n - crop) // 2)
     if centre is not None:
-        lo_c, hi_c = max(lo, centre - crop + border + 1), min(hi, centre - border)
-        if hi_c >= lo_c:
-            return int(rng.integers(lo_c, hi_c + 1))
+        for pad in (border + STENCIL // 2, border):
+            lo_c, hi_c = max(lo, centre - crop + pad + 1), min(hi, centre - pad)
+            if hi_c >= lo_c:
+                return int(rng.integers(lo_c, hi_c + 1))
     return int(rng.integers(lo, hi + 1))
 
 
 def _set_force_scale(net: ClosureNet, solver, every: int) -> None:
-    _, vals, _ = interval_forcing(solver, every)
-    peak = float(vals.abs().max()) if vals is not None and vals.numel() else 0.0
+    fc = interval_forcing(solver, every)
+    peak = float(fc.feats[..., 0].abs().max()) if fc is not None else 0.0
     net.force_scale = peak if peak > 0 else 1.0
+
+
+def source_features(fc, q: int) -> torch.Tensor:
+    """(cells, NFEAT) features of interval q: f0, f1, f0 (eps_s - 1)."""
+    f0, f1 = fc.feats[q, :, 0], fc.feats[q, :, 1]
+    return torch.stack([f0, f1, f0 * (fc.eps - 1.0)], dim=1)
+
+
+def _accumulate_source_fit(net, solver, every, q, tc, box, slot, which, origin, crop, b, ls):
+    """Normal equations of the linear source model: for every source cell
+    whose whole box lies in the crop's loss region, G += phi phi^T and
+    B[:, c, o] += phi * target_c(o) (physical units)."""
+    fc = interval_forcing(solver, every)
+    if fc is None or box.numel() == 0:
+        return
+    h = STENCIL // 2
+    full = [c_i for c_i, (i, j, k) in enumerate(fc.xyz)
+            if all(b + h <= v - o < crop - b - h for v, o in zip((i, j, k), origin))]
+    if not full:
+        return
+    t_phys = tc.reshape(-1, COUT)[box].double() * net.scale.double()[None, :]   # (E, 6)
+    phi = source_features(fc, q).double()                                      # (cells, F)
+    sel = torch.zeros(len(fc.xyz), dtype=torch.bool, device=box.device)
+    sel[full] = True
+    keep = sel[which]
+    for c_i in full:
+        ls["G"] += torch.outer(phi[c_i], phi[c_i])
+    pw = phi[which[keep]]                                                       # (E', F)
+    for f in range(NFEAT):
+        ls["B"][f].index_add_(1, slot[keep], (t_phys[keep] * pw[:, f:f + 1]).T)
+
+
+def _solve_source_fit(net, ls) -> None:
+    """K = G^-1 B for every (component, box cell); a tiny ridge keeps G
+    invertible when a feature never varies."""
+    G = ls["G"]
+    if float(G.trace()) <= 0:
+        return
+    ridge = 1e-9 * float(G.trace()) * torch.eye(NFEAT, dtype=G.dtype, device=G.device)
+    K = torch.linalg.solve(G + ridge, ls["B"].reshape(NFEAT, -1))
+    net.ks = K.reshape(NFEAT, COUT, STENCIL ** 3).to(torch.float32)
 
 
 def train_closure(device, seeds, n_fine: int = 256, n_coarse: int = 400, every: int = 10,
@@ -540,7 +616,9 @@ def train_closure(device, seeds, n_fine: int = 256, n_coarse: int = 400, every: 
     Adam, learning rate cosine-annealed from ``lr`` to ``lr_final``."""
     rng = np.random.default_rng(seed)
     net = net or ClosureNet(device, seed=seed, forcing=forcing)
-    history = []
+    history, target_var = [], []
+    ls = {"G": torch.zeros(NFEAT, NFEAT, dtype=torch.float64, device=net.device),
+          "B": torch.zeros(NFEAT, COUT, STENCIL ** 3, dtype=torch.float64, device=net.device)}
     t_start = time.time()
     lr_final = lr if lr_final is None else lr_final
     n_updates = epochs * len(seeds) * (n_coarse // every)
@@ -604,13 +682,21 @@ def train_closure(device, seeds, n_fine: int = 256, n_coarse: int = 400, every: 
                 y0 = net.forward(x0[sl].contiguous().view(-1, CIN), (crop, crop, crop),
                                  train=True, tag="0").view(crop, crop, crop, 8)
                 y = y - y0
-                terms = (net.add_stencil(y, csol, (crop, crop, crop), (cx, cy, cz), m, every)
-                         if net.forcing else None)
+                # the net's loss region: the crop minus its b-cell border and
+                # minus the source-model boxes (where its output is zeroed)
+                w = torch.zeros((crop,) * 3, device=y.device)
+                w[(slice(b, crop - b),) * 3] = 1.0
+                if net.forcing:
+                    box, slot, which = net._box_terms(csol, (crop,) * 3, (cx, cy, cz), every)
+                    w.view(-1)[box] = 0.0
+                    y.view(-1, OUT_STRIDE)[box] = 0.0
+                    _accumulate_source_fit(net, csol, every, q, tc, box, slot, which,
+                                           (cx, cy, cz), crop, b, ls)
                 err = torch.zeros_like(y)
-                inner = (slice(b, crop - b),) * 3
-                err[inner + (slice(0, 6),)] = y[inner + (slice(0, 6),)] - tc[inner]
-                n_in = (crop - 2 * b) ** 3 * 6
+                err[..., :6] = (y[..., :6] - tc) * w[..., None]
+                n_in = float(w.sum()) * 6
                 loss = float((err ** 2).sum()) / n_in
+                target_var.append(float(((tc * w[..., None]) ** 2).sum()) / n_in)
                 frac = upd / max(1, n_updates - 1)
                 lr_now = lr_final + 0.5 * (lr - lr_final) * (1.0 + math.cos(math.pi * frac))
                 upd += 1
@@ -621,14 +707,14 @@ def train_closure(device, seeds, n_fine: int = 256, n_coarse: int = 400, every: 
                 grads0 = net.backward(g_out, tag="0")
                 for k_ in grads:
                     grads[k_] = grads[k_] - grads0[k_]
-                if net.forcing:
-                    grads["ks"] = (net.stencil_grad(err, terms, 2.0 / n_in) if terms is not None
-                                   else torch.zeros_like(net.params["ks"]))
                 net.adam_step(grads, lr=lr_now)
                 history.append(loss)
+            if net.forcing and s_i == len(seeds) - 1:
+                _solve_source_fit(net, ls)
             if log:
                 log(f"epoch {ep}{' on-policy' if onpolicy else ''} seed {sd}: mean loss "
-                    f"{np.mean(history[-len(order):]):.4e} ({time.time() - t_start:.1f} s)")
+                    f"{np.mean(history[-len(order):]):.4e} (zero correction "
+                    f"{np.mean(target_var[-len(order):]):.4e}) ({time.time() - t_start:.1f} s)")
             del states, inputs
     return net, history
 
@@ -668,7 +754,10 @@ def evaluate_closure(net: ClosureNet, seed: int, n_fine: int = 256, n_coarse: in
     # reference states at every correction time (restricted fine states)
     states, fsol = prob.fine_states(n_coarse, every)
     del fsol
-    for label, model in (("coarse", None), ("closure", net)):
+    runs = [("coarse", None, True), ("closure", net, True)]
+    if net.forcing:
+        runs.append(("source_only", net, False))
+    for label, model, use_net in runs:
         csol = prob.coarse_grid().build_solver(prob.dt_coarse(), n_coarse + 1)
         if model is not None:
             model.pack()
@@ -677,7 +766,7 @@ def evaluate_closure(net: ClosureNet, seed: int, n_fine: int = 256, n_coarse: in
         for m in range(0, n_coarse, every):
             csol.run(m, every)
             if model is not None:
-                model.correct(csol, gain, m, every)
+                model.correct(csol, gain, m, every, use_net)
         torch.cuda.synchronize()
         out[f"t_{label}_s"] = time.perf_counter() - t0
         a_, b_ = _err_terms(csol.fields, ref, margin)
@@ -690,7 +779,7 @@ def evaluate_closure(net: ClosureNet, seed: int, n_fine: int = 256, n_coarse: in
         for q, m in enumerate(range(0, n_coarse, every)):
             csol.run(m, every)
             if model is not None:
-                model.correct(csol, gain, m, every)
+                model.correct(csol, gain, m, every, use_net)
             a_, b_ = _err_terms(csol.fields, states[q + 1], margin)
             num, den = num + a_, den + b_
             a_, b_ = _err_terms(csol.fields, states[q + 1], margin, far)

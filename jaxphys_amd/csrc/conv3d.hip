@@ -337,14 +337,17 @@ __global__ void flip_weights_kernel(const u16* w, int cout, int cout_pad, int ci
 __global__ void pack_fields_kernel(int nx, int ny, int nz, const float* f0, const float* f1,
                                    const float* f2, const float* f3, const float* f4,
                                    const float* f5, const float* eps, const float* scale,
-                                   float eshift, float escale, u16* out) {
+                                   float eshift, float escale, int flags, u16* out) {
   const long long n = (long long)nx * ny * nz;
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float* f[6] = {f0, f1, f2, f3, f4, f5};
   u16 v[8];
 #pragma unroll
-  for (int c = 0; c < 6; ++c) v[c] = f2bf(f[c][i] / scale[c]);
+  for (int c = 0; c < 6; ++c) {
+    const float u = f[c][i] / scale[c];
+    v[c] = f2bf((flags & CONV_PACK_ASINH) ? asinhf(u) : u);
+  }
   v[6] = f2bf((eps[i] - eshift) / escale);
   v[7] = 0;
   uint4 pk;
@@ -518,13 +521,13 @@ int conv3d_flip_weights(const ConvDesc* d, const void* w, int32_t dy_cpad, void*
 }
 
 int conv3d_pack_fields(int32_t nx, int32_t ny, int32_t nz, float* const* fields, const float* eps,
-                       const float* scale, float eps_shift, float eps_scale, void* out,
-                       void* stream) {
+                       const float* scale, float eps_shift, float eps_scale, int32_t flags,
+                       void* out, void* stream) {
   if (!fields || !eps || !scale || !out) return fail(-1, "null argument");
   const long long n = (long long)nx * ny * nz;
   hipLaunchKernelGGL(pack_fields_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                      (hipStream_t)stream, nx, ny, nz, fields[0], fields[1], fields[2], fields[3],
-                     fields[4], fields[5], eps, scale, eps_shift, eps_scale, (u16*)out);
+                     fields[4], fields[5], eps, scale, eps_shift, eps_scale, flags, (u16*)out);
   CONV_CHECK(hipGetLastError());
   return 0;
 }

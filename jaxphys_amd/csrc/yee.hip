@@ -353,6 +353,48 @@ __global__ __launch_bounds__(256) void energy_final_kernel(const double* part, i
   }
 }
 
+// Poynting flux through the plane normal to `axis` halfway between E planes
+// p and p+1 (where the two tangential H components live): per cell of the
+// plane, (E_a x H)_n with each tangential E averaged over its two planes.
+//   axis x: S = Ey Hz - Ez Hy,  axis y: S = Ez Hx - Ex Hz,  axis z: S = Ex Hy - Ey Hx
+__global__ __launch_bounds__(256) void flux_partial_kernel(GridArgs g, int axis, int p,
+                                                          double* part) {
+  __shared__ double sh[4];
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  // plane cells (u, v): axis x -> (j, k), y -> (i, k), z -> (i, j)
+  const int nu = axis == 0 ? ny : nx;
+  const int nv = axis == 2 ? ny : nz;
+  const long long ncell = (long long)nu * nv;
+  // first / second tangential E and the H each pairs with (right-handed)
+  const int ea = axis == 0 ? 1 : (axis == 1 ? 2 : 0);
+  const int eb = axis == 0 ? 2 : (axis == 1 ? 0 : 1);
+  const int ha = axis == 0 ? 5 : (axis == 1 ? 3 : 4);
+  const int hb = axis == 0 ? 4 : (axis == 1 ? 5 : 3);
+  const long long step = axis == 0 ? 1 : (axis == 1 ? (long long)nx : (long long)nx * ny);
+  double acc = 0.0;
+  for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < ncell;
+       c += (long long)gridDim.x * blockDim.x) {
+    const int u = (int)(c % nu), v = (int)(c / nu);
+    const int i = axis == 0 ? p : u;
+    const int j = axis == 0 ? u : (axis == 1 ? p : v);
+    const int k = axis == 2 ? p : v;
+    const long long o = ((long long)k * ny + j) * nx + i;
+    const double ua = 0.5 * ((double)g.f[ea][o] + (double)g.f[ea][o + step]);
+    const double ub = 0.5 * ((double)g.f[eb][o] + (double)g.f[eb][o + step]);
+    acc += ua * (double)g.f[ha][o] - ub * (double)g.f[hb][o];
+  }
+  acc = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+__global__ __launch_bounds__(256) void flux_final_kernel(const double* part, int nb, double* out) {
+  __shared__ double sh[4];
+  double acc = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) acc += part[b];
+  acc = block_sum(acc, sh);
+  if (threadIdx.x == 0) out[0] = acc;
+}
+
 __global__ __launch_bounds__(256) void copy_probe_kernel(const float4* __restrict__ src,
                                                          float4* __restrict__ dst, long long n4) {
   for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n4;
@@ -632,6 +674,31 @@ int yee_energy_fp64(int32_t nx, int32_t ny, int32_t nz, float* const* fields, co
   energy_partial_kernel<<<kEnergyBlocks, 256, 0, st>>>(g, eps_r, ncell, work + 2);
   YEE_HIP_CHECK(hipGetLastError());
   energy_final_kernel<<<1, 256, 0, st>>>(work + 2, kEnergyBlocks, work);
+  YEE_HIP_CHECK(hipGetLastError());
+  return YEE_OK;
+}
+
+int yee_poynting_flux_fp64(int32_t nx, int32_t ny, int32_t nz, float* const* fields,
+                           int32_t axis, int32_t plane, double* work, void* stream) {
+  if (nx < 1 || ny < 1 || nz < 1) return fail(YEE_EINVAL, "bad grid size");
+  if (!fields || !work) return fail(YEE_EINVAL, "null pointer");
+  if (axis < 0 || axis > 2) return fail(YEE_EINVAL, "axis must be 0, 1 or 2");
+  const int n_axis = axis == 0 ? nx : (axis == 1 ? ny : nz);
+  if (plane < 0 || plane + 1 >= n_axis)
+    return fail(YEE_EINVAL, "plane must satisfy 0 <= plane < n - 1 along the axis");
+  GridArgs g;
+  std::memset(&g, 0, sizeof(g));
+  g.nx = nx;
+  g.ny = ny;
+  g.nz = nz;
+  for (int c = 0; c < 6; ++c) {
+    if (!fields[c]) return fail(YEE_EINVAL, "null field pointer");
+    g.f[c] = fields[c];
+  }
+  hipStream_t st = (hipStream_t)stream;
+  flux_partial_kernel<<<kEnergyBlocks, 256, 0, st>>>(g, axis, plane, work + 1);
+  YEE_HIP_CHECK(hipGetLastError());
+  flux_final_kernel<<<1, 256, 0, st>>>(work + 1, kEnergyBlocks, work);
   YEE_HIP_CHECK(hipGetLastError());
   return YEE_OK;
 }

@@ -26,3 +26,8 @@ class ConfigurationError(NeurosimError):
 class NativeLibraryError(SimulationError):
     """The HIP library is missing or cannot run here (no reference
     counterpart: the reference has no native code)."""
+
+
+class VisualizationError(NeurosimError):
+    """Plotting is unavailable or was asked for something that does not
+    exist (exceptions.py:44)."""

@@ -63,6 +63,27 @@ def test_random_state_bit_exact(gpu, boundary):
         np.testing.assert_array_equal(f.T.cpu().numpy(), ref[c][-1], err_msg=c)
 
 
+@pytest.mark.parametrize("boundary", ["absorbing", "periodic", "reflecting"])
+@pytest.mark.parametrize("size", [(264, 70), (512, 41)])
+def test_marching_wavefronts_bit_exact(gpu, boundary, size):
+    """Several full 64-column marching wavefronts (the edge columns in the
+    first / last lane) and partial ones, rows inside and at the edges,
+    random state, walls."""
+    rng = np.random.default_rng(size[0])
+    init = [rng.standard_normal(size).astype(F32) for _ in range(3)]
+    waves = [R.Wave(2e9, 17, 0.3), R.Wave(3e9, 17, 0.2)]
+    walls = [R.Wall(20, 100, 140), R.Wall(33)]
+    g = _grid(size, boundary, 7, waves, walls)
+    dt = R.default_dt(0.01, F32)
+    sol = g.build_solver(dt, 14)
+    sol.set_state(*init)
+    sol.run(0, 14)
+    _, ref = R.simulate(size, 0.01, boundary, 7, waves, walls, dt=dt, n_steps=14, save_every=1,
+                        dtype=F32, init=init)
+    for c, f in zip(("ez", "hx", "hy"), sol.fields):
+        np.testing.assert_array_equal(f.T.cpu().numpy(), ref[c][-1], err_msg=c)
+
+
 def test_step_entry_point_matches_run(gpu):
     """tm2d_step (one step, explicit buffers) equals tm2d_run step by step."""
     from jaxphys_amd import _tm2d

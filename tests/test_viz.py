@@ -1,0 +1,94 @@
+"""Field slices, plots (matplotlib, Agg) and the float64 Poynting flux of
+the GPU solver vs the same staggered formula in NumPy float64."""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from jaxphys_amd import viz
+from jaxphys_amd.exceptions import VisualizationError
+from jaxphys_amd.state import EMFieldHistory, EMFieldHistory3D
+
+
+def _hist3d():
+    g = torch.Generator().manual_seed(0)
+    f = [torch.randn(3, 6, 5, 4, generator=g) for _ in range(6)]
+    grid = [torch.arange(n, dtype=torch.float32) * 0.01 for n in (6, 5, 4)]
+    return EMFieldHistory3D(torch.arange(3.0), *f, *grid)
+
+
+def test_field_slice_selects_the_plane():
+    h = _hist3d()
+    s = viz.field_slice(h, "Hy", "y", 2, step=1)
+    assert s.shape == (6, 4)
+    assert torch.equal(s, h.hy[1][:, 2, :])
+    assert torch.equal(viz.field_slice(h, "Ez"), h.ez[-1][:, :, 2])
+    with pytest.raises(VisualizationError):
+        viz.field_slice(h, "Bz")
+    with pytest.raises(VisualizationError):
+        viz.field_slice(h, "Ez", "z", 9)
+
+
+def test_plots_render():
+    if not viz.HAS_MATPLOTLIB:
+        with pytest.raises(VisualizationError):
+            viz.plot_field_slice(_hist3d())
+        return
+    import matplotlib
+
+    matplotlib.use("Agg")
+    fig = viz.plot_field_slice(_hist3d(), "Ex", "x", 3)
+    assert fig is not None
+    g = torch.Generator().manual_seed(1)
+    h2 = EMFieldHistory(torch.arange(2.0), *[torch.randn(2, 7, 5, generator=g) for _ in range(3)],
+                        torch.arange(7.0), torch.arange(5.0))
+    assert viz.plot_field_snapshot(h2, component="Hx") is not None
+    with pytest.raises(VisualizationError):
+        viz.plot_field_snapshot(h2, component="Ex")
+
+
+def _flux_ref(f, axis, p):
+    """sum over the H plane p+1/2 of (E x H).n with the tangential E averaged
+    over planes p, p+1; arrays in the reference (x, y, z) order."""
+    ex, ey, ez, hx, hy, hz = [a.astype(np.float64) for a in f]
+    sl = [slice(None)] * 3
+    s0, s1 = list(sl), list(sl)
+    s0[axis], s1[axis] = p, p + 1
+    s0, s1 = tuple(s0), tuple(s1)
+    av = lambda a: 0.5 * (a[s0] + a[s1])  # noqa: E731
+    if axis == 0:
+        return float((av(ey) * hz[s0] - av(ez) * hy[s0]).sum())
+    if axis == 1:
+        return float((av(ez) * hx[s0] - av(ex) * hz[s0]).sum())
+    return float((av(ex) * hy[s0] - av(ey) * hx[s0]).sum())
+
+
+@pytest.mark.gpu
+def test_poynting_flux_matches_numpy_and_plane_wave(gpu):
+    from jaxphys_amd import _native
+
+    rng = np.random.default_rng(5)
+    shape = (19, 14, 11)                           # (nx, ny, nz)
+    f = [rng.standard_normal(shape).astype(np.float32) for _ in range(6)]
+    dev = [torch.from_numpy(a).permute(2, 1, 0).contiguous().to(gpu) for a in f]
+    for axis in range(3):
+        for p in (0, shape[axis] // 2, shape[axis] - 2):
+            got = float(_native.poynting_flux_fp64(shape, dev, axis, p).item())
+            want = _flux_ref(f, axis, p)
+            assert abs(got - want) <= 1e-12 * max(1.0, abs(want)), (axis, p, got, want)
+    # a +z plane wave (Ex, Hy in phase): positive flux through z planes,
+    # none through x and y planes
+    nz = 32
+    k = np.arange(nz)
+    ex = np.broadcast_to(np.cos(2 * math.pi * k / nz)[None, None, :], (8, 8, nz))
+    hy = np.broadcast_to(np.cos(2 * math.pi * (k + 0.5) / nz)[None, None, :] / 376.73,
+                         (8, 8, nz))
+    z = np.zeros((8, 8, nz))
+    pw = [torch.from_numpy(np.ascontiguousarray(a, np.float32)).permute(2, 1, 0).contiguous()
+          .to(gpu) for a in (ex, z, z, z, hy, z)]
+    sz = [float(_native.poynting_flux_fp64((8, 8, nz), pw, 2, p).item()) for p in range(nz - 1)]
+    assert min(sz) >= 0 and max(sz) > 0
+    assert float(_native.poynting_flux_fp64((8, 8, nz), pw, 0, 3).item()) == 0.0
+    assert float(_native.poynting_flux_fp64((8, 8, nz), pw, 1, 3).item()) == 0.0
