@@ -25,6 +25,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 
@@ -72,6 +73,39 @@ __device__ __forceinline__ float gelu_grad(float z) {
          z * 0.39894228040143268f * expf(-0.5f * z * z);
 }
 
+// Cross-lane moves inside the 16-lane DPP rows, which are exactly the
+// column groups of the 16x16 MFMA B operand (lane t holds column t % 16):
+// shift_right gives lane r the value of lane r-1 and lane 0 the value of
+// lane 15 of `prev`; shift_left gives lane r the value of lane r+1 and lane
+// 15 the value of lane 0 of `next`.  Used to derive the x-1 / x+1 taps of
+// a conv row from the x+0 fragments already in registers.
+__device__ __forceinline__ bf16x8 shift_right(bf16x8 cur, bf16x8 prev) {
+  int c[4], p[4], o[4];
+  __builtin_memcpy(c, &cur, 16);
+  __builtin_memcpy(p, &prev, 16);
+#pragma unroll
+  for (int d = 0; d < 4; ++d) {
+    const int e = __builtin_amdgcn_update_dpp(0, p[d], 0x121 /* row_ror:1 */, 0xF, 0xF, false);
+    o[d] = __builtin_amdgcn_update_dpp(e, c[d], 0x111 /* row_shr:1 */, 0xF, 0xF, false);
+  }
+  bf16x8 r;
+  __builtin_memcpy(&r, o, 16);
+  return r;
+}
+__device__ __forceinline__ bf16x8 shift_left(bf16x8 cur, bf16x8 next) {
+  int c[4], n[4], o[4];
+  __builtin_memcpy(c, &cur, 16);
+  __builtin_memcpy(n, &next, 16);
+#pragma unroll
+  for (int d = 0; d < 4; ++d) {
+    const int e = __builtin_amdgcn_update_dpp(0, n[d], 0x12F /* row_ror:15 */, 0xF, 0xF, false);
+    o[d] = __builtin_amdgcn_update_dpp(e, c[d], 0x101 /* row_shl:1 */, 0xF, 0xF, false);
+  }
+  bf16x8 r;
+  __builtin_memcpy(&r, o, 16);
+  return r;
+}
+
 struct ConvArgs {
   int nx, ny, nz, cin, cout, cout_pad, ycp, kpad, zchunk;
   const u16* x;
@@ -85,7 +119,12 @@ struct ConvArgs {
 // WAVES wavefronts per block share the staged weights (with 16, the LDS of
 // two blocks fits a CU and each SIMD runs 8 wavefronts).  The input
 // fragments of K step k+1 are loaded before the MFMAs of step k.
-template <int EPI, int NT, int MT, int WAVES>
+// SHIFT (input channels a multiple of 32, so a K step is one tap): the K
+// loop walks the 9 (dz, dy) pairs; per pair and 32-channel block it loads
+// the NT fragments of the x+0 tap plus one fragment each side, and makes the
+// x-1 and x+1 fragments with shift_right / shift_left, so each input row is
+// read once instead of three times.
+template <int EPI, int NT, int MT, int WAVES, bool SHIFT>
 __global__ __launch_bounds__(64 * WAVES) void conv_fwd_kernel(ConvArgs a) {
   extern __shared__ u16 wsh[];  // [cout_pad][kpad + 8]
   const int ldw = a.kpad + 8;
@@ -131,7 +170,59 @@ __global__ __launch_bounds__(64 * WAVES) void conv_fwd_kernel(ConvArgs a) {
     for (int m = 0; m < MT; ++m)
 #pragma unroll
       for (int n = 0; n < NT; ++n) acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
-    if (rowok) {
+    if (rowok && SHIFT) {
+      const int ncb = cin / 32;
+      const int nsteps = 9 * ncb;
+      // x+0 fragments of (pair p, channel block cb) at plane zz and the
+      // fragments just left of the first / right of the last column tile
+      auto load_c = [&](int st, bf16x8* c, bf16x8& el, bf16x8& er) {
+        const int p = st / ncb, cb = st - p * ncb;
+        const int sz = zz + p / 3 - 1, sy = yy + p % 3 - 1;
+        const bool pl = sz >= 0 && sz < nz && sy >= 0 && sy < ny;
+        const u16* rowp = a.x + ((long long)(pl ? sz : 0) * ny + (pl ? sy : 0)) * nx * cin +
+                          cb * 32 + 8 * g;
+#pragma unroll
+        for (int n = 0; n < NT; ++n) {
+          const int sx = x0 + n * 16 + r;
+          if (pl && sx < nx) c[n] = load8(rowp + (long long)sx * cin);
+          else c[n] = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
+        }
+        const int xl = x0 - 1, xr = x0 + 16 * NT;
+        if (pl && xl >= 0) el = load8(rowp + (long long)xl * cin);
+        else el = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
+        if (pl && xr < nx) er = load8(rowp + (long long)xr * cin);
+        else er = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
+      };
+      bf16x8 cur[NT], nxt[NT], el, er, nel, ner;
+      load_c(0, cur, el, er);
+      for (int st = 0; st < nsteps; ++st) {
+        if (st + 1 < nsteps) load_c(st + 1, nxt, nel, ner);
+        const int p = st / ncb, cb = st - p * ncb;
+#pragma unroll
+        for (int dx = 0; dx < 3; ++dx) {
+          bf16x8 bfr[NT];
+#pragma unroll
+          for (int n = 0; n < NT; ++n) {
+            if (dx == 0) bfr[n] = shift_right(cur[n], n == 0 ? el : cur[n - 1]);
+            else if (dx == 1) bfr[n] = cur[n];
+            else bfr[n] = shift_left(cur[n], n == NT - 1 ? er : cur[n + 1]);
+          }
+          const int k = (3 * p + dx) * cin + cb * 32 + 8 * g;
+          bf16x8 af[MT];
+#pragma unroll
+          for (int m = 0; m < MT; ++m) af[m] = load8(wsh + (m * 16 + r) * ldw + k);
+#pragma unroll
+          for (int n = 0; n < NT; ++n)
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+              acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[m], bfr[n], acc[m][n], 0, 0, 0);
+        }
+#pragma unroll
+        for (int n = 0; n < NT; ++n) cur[n] = nxt[n];
+        el = nel;
+        er = ner;
+      }
+    } else if (rowok) {
       bf16x8 cur[NT], nxt[NT];
       load_b(zz, 0, cur);
       for (int k0 = 0; k0 < a.kpad; k0 += 32) {
@@ -372,6 +463,16 @@ __global__ void apply_correction_kernel(int nx, int ny, int nz, float* f0, float
 
 int kpad_of(int cin) { return ((27 * cin + 31) / 32) * 32; }
 
+// JAXPHYS_AMD_CONV_NOSHIFT=1 turns the shifted-fragment loop off (same
+// results; for comparison).
+bool shift_enabled() {
+  static const bool on = [] {
+    const char* e = std::getenv("JAXPHYS_AMD_CONV_NOSHIFT");
+    return !(e && e[0] == '1');
+  }();
+  return on;
+}
+
 }  // namespace conv
 
 using namespace conv;
@@ -418,8 +519,16 @@ int conv3d_fwd(const ConvDesc* d, const void* x, const void* w, const float* bia
   const size_t shm = (size_t)d->cout_pad * (a.kpad + 8) * sizeof(u16);
   if (shm > 160 * 1024) return fail(-1, "weights exceed LDS");
   hipStream_t st = (hipStream_t)stream;
-#define CONV_LAUNCH(E, MT)                                                             \
-  hipLaunchKernelGGL((conv_fwd_kernel<E, NT, MT, WAVES>), grid, dim3(64 * WAVES), shm, st, a)
+  const bool shift = d->cin_pad % 32 == 0 && shift_enabled();
+#define CONV_LAUNCH(E, MT)                                                                   \
+  do {                                                                                       \
+    if (shift)                                                                               \
+      hipLaunchKernelGGL((conv_fwd_kernel<E, NT, MT, WAVES, true>), grid, dim3(64 * WAVES),  \
+                         shm, st, a);                                                        \
+    else                                                                                     \
+      hipLaunchKernelGGL((conv_fwd_kernel<E, NT, MT, WAVES, false>), grid, dim3(64 * WAVES), \
+                         shm, st, a);                                                        \
+  } while (0)
   const int mt = d->cout_pad / 16;
   switch (epi) {
     case CONV_EPI_GELU:
