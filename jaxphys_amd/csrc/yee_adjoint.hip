@@ -259,6 +259,16 @@ __global__ __launch_bounds__(kThreads) void yee_adj_h_kernel(
   bool vx, vy;
   const int ip = nb_idx<MODE>(i + 1, nx, vx);
   const int jp = nb_idx<MODE>(j + 1, ny, vy);
+  // lane + 1 holds column i + 1 of this tile row (same half of a paired
+  // tile, inside the tile, no wrap): its own terms at x+1 are taken from it
+  const int lane = (int)(threadIdx.x % kBX);
+  const int seg = tp.pair ? 32 : kBX;
+  const bool xp_lane = (lane % seg) < seg - 1 && ip == i + 1 && i + 1 < xl;
+  // the Ex and Ey terms of plane k are carried from the previous plane,
+  // where they were the z+1 neighbours (k + 1 < k1 <= nz: no wrap there)
+  float cxa, cxb, cya, cyb;
+  eterm<MODE, CPX, 0>(a, lex, i, j, k0, true, cxa, cxb);
+  eterm<MODE, CPX, 1>(a, ley, i, j, k0, true, cya, cyb);
   for (int k = k0; k < k1; ++k) {
     bool vz;
     const int kp = nb_idx<MODE>(k + 1, nz, vz);
@@ -269,30 +279,39 @@ __global__ __launch_bounds__(kThreads) void yee_adj_h_kernel(
       psi_h_defer<MODE, CPX, 1>(a, hy0, i, j, k);
       psi_h_defer<MODE, CPX, 2>(a, hz0, i, j, k);
     }
-    float xa, xb, ya, yb, za, zb, t0, t1;
-    eterm<MODE, CPX, 0>(a, lex, i, j, k, true, xa, xb);
-    eterm<MODE, CPX, 1>(a, ley, i, j, k, true, ya, yb);
+    const float xa = cxa, xb = cxb, ya = cya, yb = cyb;
+    float za, zb, t0, t1;
     eterm<MODE, CPX, 2>(a, lez, i, j, k, true, za, zb);
+    // z+1 neighbours (also the carried terms of the next plane)
+    eterm<MODE, CPX, 0>(a, lex, i, j, kp, vz, cxa, cxb);
+    eterm<MODE, CPX, 1>(a, ley, i, j, kp, vz, cya, cyb);
+    // x+1 neighbours: Ez's a and Ey's b at (ip, j, k)
+    const float sh_za = __shfl_down(za, 1, kBX);
+    const float sh_yb = __shfl_down(yb, 1, kBX);
+    float zpa, ypb;
+    if (xp_lane) {
+      zpa = sh_za;
+      ypb = sh_yb;
+    } else {
+      eterm<MODE, CPX, 2>(a, lez, ip, j, k, vx, zpa, t1);
+      eterm<MODE, CPX, 1>(a, ley, ip, j, k, vx, t0, ypb);
+    }
     // Hx: Ey's a (d/dz Hx) at c and c+z; Ez's b (d/dy Hx) at c and c+y
     float s = hx0;
-    eterm<MODE, CPX, 1>(a, ley, i, j, kp, vz, t0, t1);
-    s += ya - t0;
+    s += ya - cya;
     eterm<MODE, CPX, 2>(a, lez, i, jp, k, vy, t0, t1);
     s += zb - t1;
     lhx[c] = s;
     // Hy: Ex's b (d/dz Hy) at c and c+z; Ez's a (d/dx Hy) at c and c+x
     s = hy0;
-    eterm<MODE, CPX, 0>(a, lex, i, j, kp, vz, t0, t1);
-    s += xb - t1;
-    eterm<MODE, CPX, 2>(a, lez, ip, j, k, vx, t0, t1);
-    s += za - t0;
+    s += xb - cxb;
+    s += za - zpa;
     lhy[c] = s;
     // Hz: Ex's a (d/dy Hz) at c and c+y; Ey's b (d/dx Hz) at c and c+x
     s = hz0;
     eterm<MODE, CPX, 0>(a, lex, i, jp, k, vy, t0, t1);
     s += xa - t0;
-    eterm<MODE, CPX, 1>(a, ley, ip, j, k, vx, t0, t1);
-    s += yb - t1;
+    s += yb - ypb;
     lhz[c] = s;
   }
 }
@@ -315,9 +334,21 @@ __global__ __launch_bounds__(kThreads) void yee_adj_e_kernel(
   bool vx, vy;
   const int im = nb_idx<MODE>(i - 1, nx, vx);
   const int jm = nb_idx<MODE>(j - 1, ny, vy);
+  // lane - 1 holds column i - 1 of this tile row (same half of a paired
+  // tile, no wrap): its own terms at x-1 are taken from it
+  const int lane = (int)(threadIdx.x % kBX);
+  const int seg = tp.pair ? 32 : kBX;
+  const bool xm_lane = (lane % seg) > 0 && im == i - 1;
+  // the Hx and Hy terms of plane k-1: computed for the first plane, then
+  // carried from the previous plane's own terms (k - 1 >= k0 >= 0 there)
+  float pxa, pxb, pya, pyb;
+  {
+    bool vz0;
+    const int km0 = nb_idx<MODE>(k0 - 1, nz, vz0);
+    hterm<MODE, CPX, 0>(a, lhx, i, j, km0, vz0, pxa, pxb);
+    hterm<MODE, CPX, 1>(a, lhy, i, j, km0, vz0, pya, pyb);
+  }
   for (int k = k0; k < k1; ++k) {
-    bool vz;
-    const int km = nb_idx<MODE>(k - 1, nz, vz);
     const unsigned c = ((unsigned)k * ny + j) * nx + i;
     const float d = adj_damp<MODE>(a, i, j, k);
     const float cb = a.cb[c];
@@ -340,21 +371,32 @@ __global__ __launch_bounds__(kThreads) void yee_adj_e_kernel(
     hterm<MODE, CPX, 0>(a, lhx, i, j, k, true, xa, xb);
     hterm<MODE, CPX, 1>(a, lhy, i, j, k, true, ya, yb);
     hterm<MODE, CPX, 2>(a, lhz, i, j, k, true, za, zb);
+    // x-1 neighbours: Hz's a and Hy's b at (im, j, k)
+    const float sh_za = __shfl_up(za, 1, kBX);
+    const float sh_yb = __shfl_up(yb, 1, kBX);
+    float zma, ymb;
+    if (xm_lane) {
+      zma = sh_za;
+      ymb = sh_yb;
+    } else {
+      hterm<MODE, CPX, 2>(a, lhz, im, j, k, vx, zma, t1);
+      hterm<MODE, CPX, 1>(a, lhy, im, j, k, vx, t0, ymb);
+    }
     // Ex: Hy's a (Ex[c+z] - Ex[c]) at c (-) and c-z (+); Hz's b at c (-), c-y (+)
-    hterm<MODE, CPX, 1>(a, lhy, i, j, km, vz, t0, t1);
-    e0 += t0 - ya;
+    e0 += pya - ya;
     hterm<MODE, CPX, 2>(a, lhz, i, jm, k, vy, t0, t1);
     e0 += t1 - zb;
     // Ey: Hx's b (Ey[c+z] - Ey[c]) at c (-), c-z (+); Hz's a at c (-), c-x (+)
-    hterm<MODE, CPX, 0>(a, lhx, i, j, km, vz, t0, t1);
-    e1 += t1 - xb;
-    hterm<MODE, CPX, 2>(a, lhz, im, j, k, vx, t0, t1);
-    e1 += t0 - za;
+    e1 += pxb - xb;
+    e1 += zma - za;
     // Ez: Hx's a (Ez[c+y] - Ez[c]) at c (-), c-y (+); Hy's b at c (-), c-x (+)
     hterm<MODE, CPX, 0>(a, lhx, i, jm, k, vy, t0, t1);
     e2 += t0 - xa;
-    hterm<MODE, CPX, 1>(a, lhy, im, j, k, vx, t0, t1);
-    e2 += t1 - yb;
+    e2 += ymb - yb;
+    pxa = xa;
+    pxb = xb;
+    pya = ya;
+    pyb = yb;
     // design box accumulation (cells with damp == 1, all components updated)
     if (a.bnx > 0) {
       const int li = i - a.bx0, lj = j - a.by0, lk = k - a.bz0;
