@@ -536,6 +536,11 @@ int yee_plan_create(const YeeGridDesc* desc, YeePlan** out) {
   if (kz <= 0) kz = 4;
   if (kz > desc->nz) kz = desc->nz;
   p->kz = kz;
+  if (const char* e = std::getenv("JAXPHYS_AMD_ADJ_KZ")) {
+    const int v = std::atoi(e);
+    if (v > 0) p->adj_kz = v;
+  }
+  if (p->adj_kz > desc->nz) p->adj_kz = desc->nz;
   if (const char* e = std::getenv("JAXPHYS_AMD_FUSED_KZ")) p->fused_kz = std::atoi(e);
   if (const char* e = std::getenv("JAXPHYS_AMD_FUSED_VARIANT")) {
     const int v = std::atoi(e);

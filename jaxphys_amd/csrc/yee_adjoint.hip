@@ -499,10 +499,10 @@ void adjoint_launch_t(const YeePlan* p, const AdjArgs& a, float* const* lam, hip
     // kernel A reads +1 and kernel B -1 neighbours: one cell of margin on
     // every side; the x-slab columns get their own launch, both slabs in
     // one paired box when each is at most 32 columns wide
-    split_tiles3(a.nx, a.ny, a.nz, a.p, p->kz, 1, 1, order == 2 ? 1 : order, in, shn, shx,
+    split_tiles3(a.nx, a.ny, a.nz, a.p, p->adj_kz, 1, 1, order == 2 ? 1 : order, in, shn, shx,
                  kBY, 1, 1, true);
   } else {
-    split_tiles(a.nx, a.ny, a.nz, 0, p->kz, 1, 1, 1, 1, order == 2 ? 1 : order, in, shn);
+    split_tiles(a.nx, a.ny, a.nz, 0, p->adj_kz, 1, 1, 1, 1, order == 2 ? 1 : order, in, shn);
     std::memset(&shx, 0, sizeof(shx));
   }
 #define YEE_ADJ_H(CPX, TS)                                                      \
