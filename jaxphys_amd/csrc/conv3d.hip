@@ -124,7 +124,9 @@ struct ConvArgs {
 // the NT fragments of the x+0 tap plus one fragment each side, and makes the
 // x-1 and x+1 fragments with shift_right / shift_left, so each input row is
 // read once instead of three times.
-template <int EPI, int NT, int MT, int WAVES, bool SHIFT>
+// CIN > 0 fixes the input channel count at compile time (the divisions of
+// the tap / channel split become shifts); 0 reads it from the arguments.
+template <int EPI, int NT, int MT, int WAVES, bool SHIFT, int CIN = 0>
 __global__ __launch_bounds__(64 * WAVES) void conv_fwd_kernel(ConvArgs a) {
   extern __shared__ u16 wsh[];  // [cout_pad][kpad + 8]
   const int ldw = a.kpad + 8;
@@ -146,7 +148,7 @@ __global__ __launch_bounds__(64 * WAVES) void conv_fwd_kernel(ConvArgs a) {
   const int yy = blockIdx.y * WAVES + wv;
   const int z0 = blockIdx.z * a.zchunk;
   const int z1 = min(z0 + a.zchunk, a.nz);
-  const int nx = a.nx, ny = a.ny, nz = a.nz, cin = a.cin;
+  const int nx = a.nx, ny = a.ny, nz = a.nz, cin = CIN > 0 ? CIN : a.cin;
   const bool rowok = yy < ny;
   // the NT input fragments of K step k0 at plane zz (zero outside the grid)
   auto load_b = [&](int zz, int k0, bf16x8* bf) {
@@ -525,6 +527,9 @@ int conv3d_fwd(const ConvDesc* d, const void* x, const void* w, const float* bia
     if (shift)                                                                               \
       hipLaunchKernelGGL((conv_fwd_kernel<E, NT, MT, WAVES, true>), grid, dim3(64 * WAVES),  \
                          shm, st, a);                                                        \
+    else if (d->cin_pad == 8)                                                                \
+      hipLaunchKernelGGL((conv_fwd_kernel<E, NT, MT, WAVES, false, 8>), grid,                \
+                         dim3(64 * WAVES), shm, st, a);                                      \
     else                                                                                     \
       hipLaunchKernelGGL((conv_fwd_kernel<E, NT, MT, WAVES, false>), grid, dim3(64 * WAVES), \
                          shm, st, a);                                                        \

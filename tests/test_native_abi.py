@@ -122,3 +122,26 @@ def test_tm2d_rejects_aliased_buffers_host_only():
     b = (ctypes.c_void_p * 3)(0x4000, 0x2000, 0x5000)  # shares one buffer with a
     rc = _tm2d.lib().tm2d_run(ctypes.byref(d), a, b, 0, 2, None)
     assert rc == _tm2d.TM2D_EINVAL and "differ" in _tm2d.lib().tm2d_last_error().decode()
+
+
+def test_conv3d_library_exports_all_declared_symbols():
+    from jaxphys_amd import _conv
+
+    lib = _conv.lib()
+    names = _declared("conv3d.h")
+    assert {"conv3d_fwd", "conv3d_wgrad", "conv3d_pack_fields"} <= names
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(_conv.EXPORTS) == names
+
+
+def test_conv3d_rejects_bad_channel_configs_host_only():
+    from jaxphys_amd import _conv
+
+    d = _conv.desc((16, 8, 4), 12, 32, 32, 32)   # 12 input channels: not a multiple of 8
+    rc = _conv.lib().conv3d_fwd(ctypes.byref(d), 0x1000, 0x2000, None, 0, None, None, 0x3000,
+                                None)
+    assert rc == -1 and "channel" in _conv.lib().conv3d_last_error().decode()
+    rc = _conv.lib().conv3d_apply_correction(4, 4, 4, (ctypes.c_void_p * 6)(), 0x1000, 8,
+                                             0x2000, 1.0, -1, None)
+    assert rc == -1

@@ -145,3 +145,19 @@ def test_periodic_translation_invariance_large(gpu):
     for a, b in zip(outs[0], outs[1]):
         assert torch.equal(torch.roll(a, shifts=(sy, sx), dims=(0, 1)), b)
     assert float(outs[0][0].abs().max()) > 0
+
+
+def test_empty_history_and_every_step(gpu):
+    """fdtd.py:362-368: zero steps give empty histories; save_every <= 1
+    keeps every step (t = t_c + dt)."""
+    g = _grid((20, 12), "reflecting", 0, [R.Wave(3e9, 5, 1.0)], [])
+    h = g.simulate(t_span=(0.0, 1e-12))
+    assert h.t.shape == (0,) and h.ez.shape == (0, 20, 12) and h.hy.shape == (0, 20, 12)
+    for every in (0, 1):
+        h = g.simulate(t_span=(0.0, 1e-10), save_every=every)
+        t_ref, ref = R.simulate((20, 12), 0.01, "reflecting", 0, [R.Wave(3e9, 5, 1.0)],
+                                t_span=(0.0, 1e-10), save_every=every, dtype=F32)
+        assert h.ez.shape[0] == ref["ez"].shape[0] == 4
+        np.testing.assert_array_equal(h.t.cpu().numpy(), t_ref)
+        for c in ("ez", "hx", "hy"):
+            np.testing.assert_array_equal(getattr(h, c).cpu().numpy(), ref[c])
