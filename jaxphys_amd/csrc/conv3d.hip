@@ -173,7 +173,7 @@ __global__ __launch_bounds__(64 * WAVES) void conv_fwd_kernel(ConvArgs a) {
 #pragma unroll
       for (int n = 0; n < NT; ++n) acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
     if (rowok && SHIFT) {
-      const int ncb = cin / 32;
+      const int ncb = cin >= 32 ? cin / 32 : 1;  // (SHIFT implies cin % 32 == 0)
       const int nsteps = 9 * ncb;
       // x+0 fragments of (pair p, channel block cb) at plane zz and the
       // fragments just left of the first / right of the last column tile

@@ -243,12 +243,12 @@ __device__ __forceinline__ void adj_column(const TilePos& tp, int& i, int& xl) {
 // no slab within one cell, 1 for tiles whose columns are at least one cell
 // away from the x slabs, 2 for any tile.
 template <int MODE, int CPX>
-__global__ __launch_bounds__(kThreads) void yee_adj_h_kernel(
-    AdjArgs a, TileSet ts, const float* __restrict__ lex, const float* __restrict__ ley,
-    const float* __restrict__ lez, float* __restrict__ lhx, float* __restrict__ lhy,
-    float* __restrict__ lhz) {
+__device__ __forceinline__ void adj_h_body(
+    const AdjArgs& a, const TileSet& ts, int L, const float* __restrict__ lex,
+    const float* __restrict__ ley, const float* __restrict__ lez, float* __restrict__ lhx,
+    float* __restrict__ lhy, float* __restrict__ lhz) {
   TilePos tp;
-  if (!tile_of_set(ts, tp)) return;
+  if (!tile_of_set_at(ts, L, tp)) return;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const int k0 = tp.k0, k1 = tp.k1;
   int i, xl;
@@ -318,12 +318,12 @@ __global__ __launch_bounds__(kThreads) void yee_adj_h_kernel(
 
 // Kernel B: per E cell, z-marching tiles (CPX as in kernel A).
 template <int MODE, bool LOSSY, int CPX>
-__global__ __launch_bounds__(kThreads) void yee_adj_e_kernel(
-    AdjArgs a, TileSet ts, float* __restrict__ lex, float* __restrict__ ley,
-    float* __restrict__ lez, const float* __restrict__ lhx, const float* __restrict__ lhy,
-    const float* __restrict__ lhz) {
+__device__ __forceinline__ void adj_e_body(
+    const AdjArgs& a, const TileSet& ts, int L, float* __restrict__ lex,
+    float* __restrict__ ley, float* __restrict__ lez, const float* __restrict__ lhx,
+    const float* __restrict__ lhy, const float* __restrict__ lhz) {
   TilePos tp;
-  if (!tile_of_set(ts, tp)) return;
+  if (!tile_of_set_at(ts, L, tp)) return;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const int k0 = tp.k0, k1 = tp.k1;
   int i, xl;
@@ -421,6 +421,69 @@ __global__ __launch_bounds__(kThreads) void yee_adj_e_kernel(
   }
 }
 
+// One launch per kernel and tile set (no CPML: the whole grid is one set).
+template <int MODE, int CPX>
+__global__ __launch_bounds__(kThreads) void yee_adj_h_kernel(
+    AdjArgs a, TileSet ts, const float* __restrict__ lex, const float* __restrict__ ley,
+    const float* __restrict__ lez, float* __restrict__ lhx, float* __restrict__ lhy,
+    float* __restrict__ lhz) {
+  adj_h_body<MODE, CPX>(a, ts, (int)blockIdx.x, lex, ley, lez, lhx, lhy, lhz);
+}
+
+template <int MODE, bool LOSSY, int CPX>
+__global__ __launch_bounds__(kThreads) void yee_adj_e_kernel(
+    AdjArgs a, TileSet ts, float* __restrict__ lex, float* __restrict__ ley,
+    float* __restrict__ lez, const float* __restrict__ lhx, const float* __restrict__ lhy,
+    const float* __restrict__ lhz) {
+  adj_e_body<MODE, LOSSY, CPX>(a, ts, (int)blockIdx.x, lex, ley, lez, lhx, lhy, lhz);
+}
+
+// CPML: the three tile sets (slab-free interior, y/z-slab shell, x-slab
+// columns) of one kernel in a single launch.  The workgroup index picks the
+// set (each set's block count is a multiple of 8, so the XCD-aware order
+// within every set is kept); the small, latency-bound shell tiles then run
+// beside the interior tiles instead of in launches of their own.
+struct AdjSets {
+  TileSet in, shn, shx;
+  int nb_in, nb_shn;
+};
+
+template <int MODE>
+__global__ __launch_bounds__(kThreads) void yee_adj_h3_kernel(
+    AdjArgs a, AdjSets sets, const float* __restrict__ lex, const float* __restrict__ ley,
+    const float* __restrict__ lez, float* __restrict__ lhx, float* __restrict__ lhy,
+    float* __restrict__ lhz) {
+  int L = (int)blockIdx.x;
+  if (L < sets.nb_in) {
+    adj_h_body<MODE, 0>(a, sets.in, L, lex, ley, lez, lhx, lhy, lhz);
+    return;
+  }
+  L -= sets.nb_in;
+  if (L < sets.nb_shn) {
+    adj_h_body<MODE, 1>(a, sets.shn, L, lex, ley, lez, lhx, lhy, lhz);
+    return;
+  }
+  adj_h_body<MODE, 2>(a, sets.shx, L - sets.nb_shn, lex, ley, lez, lhx, lhy, lhz);
+}
+
+template <int MODE, bool LOSSY>
+__global__ __launch_bounds__(kThreads) void yee_adj_e3_kernel(
+    AdjArgs a, AdjSets sets, float* __restrict__ lex, float* __restrict__ ley,
+    float* __restrict__ lez, const float* __restrict__ lhx, const float* __restrict__ lhy,
+    const float* __restrict__ lhz) {
+  int L = (int)blockIdx.x;
+  if (L < sets.nb_in) {
+    adj_e_body<MODE, LOSSY, 0>(a, sets.in, L, lex, ley, lez, lhx, lhy, lhz);
+    return;
+  }
+  L -= sets.nb_in;
+  if (L < sets.nb_shn) {
+    adj_e_body<MODE, LOSSY, 1>(a, sets.shn, L, lex, ley, lez, lhx, lhy, lhz);
+    return;
+  }
+  adj_e_body<MODE, LOSSY, 2>(a, sets.shx, L - sets.nb_shn, lex, ley, lez, lhx, lhy, lhz);
+}
+
 __global__ void probe_gather_kernel(GridArgs g, const int* comp, const long long* idx, int n,
                                     float* out) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -505,24 +568,35 @@ void adjoint_launch_t(const YeePlan* p, const AdjArgs& a, float* const* lam, hip
     split_tiles(a.nx, a.ny, a.nz, 0, p->adj_kz, 1, 1, 1, 1, order == 2 ? 1 : order, in, shn);
     std::memset(&shx, 0, sizeof(shx));
   }
-#define YEE_ADJ_H(CPX, TS)                                                      \
-  yee_adj_h_kernel<MODE, CPX><<<tileset_blocks(TS), kThreads, 0, st>>>(          \
-      a, TS, lam[0], lam[1], lam[2], lam[3], lam[4], lam[5])
-#define YEE_ADJ_E(CPX, TS)                                                      \
-  yee_adj_e_kernel<MODE, LOSSY, CPX><<<tileset_blocks(TS), kThreads, 0, st>>>(   \
-      a, TS, lam[0], lam[1], lam[2], lam[3], lam[4], lam[5])
-  if (in.ntiles > 0) YEE_ADJ_H(0, in);
   if constexpr (MODE == kCpml) {
-    if (shn.ntiles > 0) YEE_ADJ_H(1, shn);
-    if (shx.ntiles > 0) YEE_ADJ_H(2, shx);
+    // block counts rounded up to multiples of 8 (the extra workgroups of a
+    // plain-order set find no tile and exit)
+    auto nblocks = [](const TileSet& ts) {
+      if (ts.ntiles <= 0) return 0;
+      const int nb = (int)tileset_blocks(ts);
+      return (ts.per > 0 || ts.band > 0) ? nb : (nb + kXcds - 1) / kXcds * kXcds;
+    };
+    AdjSets sets;
+    sets.in = in;
+    sets.shn = shn;
+    sets.shx = shx;
+    sets.nb_in = nblocks(in);
+    sets.nb_shn = nblocks(shn);
+    const int total = sets.nb_in + sets.nb_shn + nblocks(shx);
+    if (total > 0) {
+      yee_adj_h3_kernel<MODE><<<total, kThreads, 0, st>>>(a, sets, lam[0], lam[1], lam[2],
+                                                          lam[3], lam[4], lam[5]);
+      yee_adj_e3_kernel<MODE, LOSSY><<<total, kThreads, 0, st>>>(a, sets, lam[0], lam[1],
+                                                                 lam[2], lam[3], lam[4], lam[5]);
+    }
+  } else {
+    if (in.ntiles > 0) {
+      yee_adj_h_kernel<MODE, 0><<<tileset_blocks(in), kThreads, 0, st>>>(
+          a, in, lam[0], lam[1], lam[2], lam[3], lam[4], lam[5]);
+      yee_adj_e_kernel<MODE, LOSSY, 0><<<tileset_blocks(in), kThreads, 0, st>>>(
+          a, in, lam[0], lam[1], lam[2], lam[3], lam[4], lam[5]);
+    }
   }
-  if (in.ntiles > 0) YEE_ADJ_E(0, in);
-  if constexpr (MODE == kCpml) {
-    if (shn.ntiles > 0) YEE_ADJ_E(1, shn);
-    if (shx.ntiles > 0) YEE_ADJ_E(2, shx);
-  }
-#undef YEE_ADJ_H
-#undef YEE_ADJ_E
 }
 
 int adjoint_launch(const YeePlan* p, const AdjArgs& a, float* const* lam, hipStream_t st) {

@@ -254,8 +254,7 @@ struct TilePos {
 
 // false for the padding workgroups of the XCD-aware order (same map as
 // Tiling)
-__device__ __forceinline__ bool tile_of_set(const TileSet& ts, TilePos& tp) {
-  const int L = blockIdx.x;
+__device__ __forceinline__ bool tile_of_set_at(const TileSet& ts, int L, TilePos& tp) {
   int t;
   if (ts.band > 0) {
     const int s = L / kXcds;
@@ -282,6 +281,10 @@ __device__ __forceinline__ bool tile_of_set(const TileSet& ts, TilePos& tp) {
   tp.k0 = B.z0 + (r / B.nyt) * ts.kz;
   tp.k1 = min(tp.k0 + ts.kz, B.z1);
   return true;
+}
+
+__device__ __forceinline__ bool tile_of_set(const TileSet& ts, TilePos& tp) {
+  return tile_of_set_at(ts, (int)blockIdx.x, tp);
 }
 
 // (bx, by, bz) of this workgroup; false for the padding workgroups of the
