@@ -200,23 +200,37 @@ int yee_box_gather(float* const* fields, int32_t nx, int32_t ny, int32_t nz,
 /* yee_run plus recording after every step s = 0..nsteps-1 (step n0+s):
  * probe_out[s*nprobe + q] = field value of probe q (as yee_probe_gather)
  * and box_out[s*3*boxcells ...] = E on the box (as yee_box_gather).  Either
- * part may be disabled (nprobe == 0 / box == NULL).                      */
+ * part may be disabled (nprobe == 0 / box == NULL).  probe_box / probe_map
+ * (optional, both or neither): the cell -> probe map of yee_adjoint_run,
+ * for E probes at distinct (component, cell) pairs.  When the probes are
+ * mapped (or absent) and every source is fused into the E kernel, that
+ * kernel writes the records itself; otherwise one record launch follows
+ * each step.  The recorded values are the same either way.              */
 int yee_run_record(YeePlan* plan, float* const* fields, float* psi, int32_t n0,
                    int32_t nsteps, int32_t nprobe, const int32_t* probe_comp,
                    const int64_t* probe_idx, float* probe_out, const int32_t* box,
-                   float* box_out, void* stream);
+                   float* box_out, const int32_t* probe_box, const int32_t* probe_map,
+                   void* stream);
 
 /* Adjoint sweep over steps n_hi-1 down to n_lo (yee_adjoint_step each),
  * adding after step n (for n >= 1) the objective term
  * scale * w[q] * probe_hist[(n-1)*nprobe + q] to the adjoint of probe q
  * (the objective sum_n sum_q w_q F_q(after step n)^2 gives scale = 2).
+ * Probes are E components at distinct (component, cell) pairs (merge
+ * duplicates by adding their weights).
  * box history: box_hist[m] = E on the box after step n_lo-1+m, m = 0 ..
- * n_hi-n_lo (so box_hist[0] is E^{n_lo}).                               */
+ * n_hi-n_lo (so box_hist[0] is E^{n_lo}).
+ * probe_box / probe_map (optional, both or neither): a box (x0, y0, z0,
+ * nx, ny, nz; host array) holding every probe cell and, on the device,
+ * probe_map[((c*nz + k)*ny + j)*nx + i] = q for probe q of E component c
+ * at (x0+i, y0+j, z0+k), -1 elsewhere.  With it the E-adjoint kernel adds
+ * the objective terms itself (same values, no scatter launch per step).  */
 int yee_adjoint_run(YeePlan* plan, float* const* lam, float* lam_psi, int32_t n_lo,
                     int32_t n_hi, int32_t nprobe, const int32_t* probe_comp,
                     const int64_t* probe_idx, const float* w, const float* probe_hist,
                     float scale, const int32_t* box, const float* box_hist,
-                    float* acc_cb, float* acc_ca, void* stream);
+                    float* acc_cb, float* acc_ca, const int32_t* probe_box,
+                    const int32_t* probe_map, void* stream);
 
 /* Float64 accumulators for the checks (no reference counterpart):
  * work[0] = sum_cells eps_r |E|^2 (eps_r from an explicit per-cell array,

@@ -88,13 +88,14 @@ EXPORTS = {
     "yee_run_record": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                       ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
-                                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+                                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                      ctypes.c_void_p, ctypes.c_void_p]),
     "yee_adjoint_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                        ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                        ctypes.c_void_p, ctypes.c_float, ctypes.c_void_p,
                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
-                                       ctypes.c_void_p]),
+                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
 }
 
 
@@ -245,24 +246,28 @@ def box_array(box):
 
 
 def run_record(plan: "Plan", fields, psi, n0, nsteps, probe_comp=None, probe_idx=None,
-               probe_out=None, box=None, box_out=None, stream=None) -> None:
+               probe_out=None, box=None, box_out=None, probe_box=None, probe_map=None,
+               stream=None) -> None:
     """``yee_run_record``: steps with per-step probe / design-box recording."""
     nprobe = 0 if probe_comp is None else int(probe_comp.numel())
     check(lib().yee_run_record(plan.handle, field_array(fields), ptr(psi) or None, int(n0),
                                int(nsteps), nprobe, ptr(probe_comp) or None,
                                ptr(probe_idx) or None, ptr(probe_out) or None,
-                               box_array(box), ptr(box_out) or None, stream_handle(stream)),
+                               box_array(box), ptr(box_out) or None, box_array(probe_box),
+                               ptr(probe_map) or None, stream_handle(stream)),
           "yee_run_record")
 
 
 def adjoint_run(plan: "Plan", lam, lam_psi, n_lo, n_hi, probe_comp, probe_idx, w, probe_hist,
-                scale, box=None, box_hist=None, acc_cb=None, acc_ca=None, stream=None) -> None:
+                scale, box=None, box_hist=None, acc_cb=None, acc_ca=None, probe_box=None,
+                probe_map=None, stream=None) -> None:
     nprobe = 0 if probe_comp is None else int(probe_comp.numel())
     check(lib().yee_adjoint_run(plan.handle, field_array(lam), ptr(lam_psi) or None, int(n_lo),
                                 int(n_hi), nprobe, ptr(probe_comp) or None,
                                 ptr(probe_idx) or None, ptr(w) or None, ptr(probe_hist) or None,
                                 float(scale), box_array(box), ptr(box_hist) or None,
-                                ptr(acc_cb) or None, ptr(acc_ca) or None, stream_handle(stream)),
+                                ptr(acc_cb) or None, ptr(acc_ca) or None, box_array(probe_box),
+                                ptr(probe_map) or None, stream_handle(stream)),
           "yee_adjoint_run")
 
 

@@ -93,8 +93,14 @@ class AdjointProblem:
         self.region = (slice(z0, z0 + bz), slice(y0, y0 + by), slice(x0, x0 + bx))
         self.box_int = (x0, y0, z0, bx, by, bz)
         self.bcells = bx * by * bz
-        comps, idx = [], []
-        for comp, (i, j, k) in probes:
+        probes = list(probes)
+        w_in = np.ones(len(probes)) if weights is None else np.asarray(weights, np.float64)
+        if w_in.shape != (len(probes),):
+            raise ConfigurationError("one weight per probe")
+        # probes at the same (component, cell) are one term with the summed
+        # weight: the adjoint adds one objective term per (component, cell)
+        merged: dict[tuple[int, int, int, int], float] = {}
+        for (comp, (i, j, k)), wq in zip(probes, w_in):
             c = _COMP[comp] if isinstance(comp, str) else int(comp)
             if c not in (0, 1, 2):
                 # the psi_H adjoint update is deferred by one step and reads
@@ -102,13 +108,29 @@ class AdjointProblem:
                 raise ConfigurationError("adjoint objectives are sums over E components")
             if not (0 <= i < nx and 0 <= j < ny and 0 <= k < nz):
                 raise ConfigurationError(f"probe {(i, j, k)} outside the grid")
-            comps.append(c)
-            idx.append((k * ny + j) * nx + i)
-        self.nprobe = len(comps)
+            key = (c, int(i), int(j), int(k))
+            merged[key] = merged.get(key, 0.0) + float(wq)
+        keys = list(merged)
+        comps = [c for c, _, _, _ in keys]
+        idx = [(k * ny + j) * nx + i for _, i, j, k in keys]
+        self.nprobe = len(keys)
         self.probe_comp = torch.tensor(comps, dtype=torch.int32, device=dev)
         self.probe_idx = torch.tensor(idx, dtype=torch.int64, device=dev)
-        w = np.ones(self.nprobe) if weights is None else np.asarray(weights, dtype=np.float64)
-        self.w = torch.tensor(w, dtype=torch.float32, device=dev)
+        self.w = torch.tensor([merged[key] for key in keys], dtype=torch.float32, device=dev)
+        # cell -> probe map over the probes' bounding box (when it is small):
+        # the E-adjoint kernel then adds the objective terms itself
+        self.probe_box, self.probe_map = None, None
+        if keys:
+            pi = np.array([[i, j, k] for _, i, j, k in keys])
+            lo, hi = pi.min(0), pi.max(0) + 1
+            ext = hi - lo
+            if int(np.prod(ext)) <= max(1 << 20, 8 * self.nprobe):
+                pmap = np.full((3, ext[2], ext[1], ext[0]), -1, dtype=np.int32)
+                for q, (c, i, j, k) in enumerate(keys):
+                    pmap[c, k - lo[2], j - lo[1], i - lo[0]] = q
+                self.probe_box = (int(lo[0]), int(lo[1]), int(lo[2]),
+                                  int(ext[0]), int(ext[1]), int(ext[2]))
+                self.probe_map = torch.from_numpy(pmap.reshape(-1)).to(dev)
         self.background = sol.eps_r.clone()
         self.probe_hist = torch.zeros((self.n_steps, self.nprobe), dtype=torch.float32, device=dev)
         self.checkpoints = []
@@ -146,7 +168,8 @@ class AdjointProblem:
                                self.box_hist[0])
             _native.run_record(sol.plan, sol.fields, sol.psi, 0, N, self.probe_comp,
                                self.probe_idx, self.probe_hist, box=self.box_int,
-                               box_out=self.box_hist[1:])
+                               box_out=self.box_hist[1:], probe_box=self.probe_box,
+                               probe_map=self.probe_map)
             self._recorded = True
             ph = self.probe_hist.to(torch.float64)
             return float((self.w.to(torch.float64) * ph * ph).sum())
@@ -156,7 +179,8 @@ class AdjointProblem:
                                      sol.psi.clone() if sol.psi is not None else None))
             ns = min(K, N - s0)
             _native.run_record(sol.plan, sol.fields, sol.psi, s0, ns, self.probe_comp,
-                               self.probe_idx, self.probe_hist[s0:s0 + ns])
+                               self.probe_idx, self.probe_hist[s0:s0 + ns],
+                               probe_box=self.probe_box, probe_map=self.probe_map)
         ph = self.probe_hist.to(torch.float64)
         return float((self.w.to(torch.float64) * ph * ph).sum())
 
@@ -178,7 +202,8 @@ class AdjointProblem:
         if self._recorded:
             _native.adjoint_run(sol.plan, lam, lam_psi, 0, N, self.probe_comp, self.probe_idx,
                                 self.w, self.probe_hist, 2.0, box=self.box_int,
-                                box_hist=self.box_hist, acc_cb=acc_cb, acc_ca=acc_ca)
+                                box_hist=self.box_hist, acc_cb=acc_cb, acc_ca=acc_ca,
+                                probe_box=self.probe_box, probe_map=self.probe_map)
             return self._chain_rule(acc_cb, acc_ca)
         if self.box_hist is None or self.box_hist.shape[0] < K + 1:
             self.box_hist = torch.empty((K + 1, 3 * self.bcells), dtype=torch.float32, device=dev)
@@ -196,7 +221,8 @@ class AdjointProblem:
                                box_out=self.box_hist[1:])
             _native.adjoint_run(sol.plan, lam, lam_psi, s0, s1, self.probe_comp, self.probe_idx,
                                 self.w, self.probe_hist, 2.0, box=self.box_int,
-                                box_hist=self.box_hist, acc_cb=acc_cb, acc_ca=acc_ca)
+                                box_hist=self.box_hist, acc_cb=acc_cb, acc_ca=acc_ca,
+                                probe_box=self.probe_box, probe_map=self.probe_map)
         return self._chain_rule(acc_cb, acc_ca)
 
     def _chain_rule(self, acc_cb, acc_ca) -> torch.Tensor:

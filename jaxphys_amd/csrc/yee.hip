@@ -143,14 +143,15 @@ __global__ __launch_bounds__(kThreads) void yee_h_kernel(GridArgs g, Tiling tl, 
 // E half-step: E = [Ca] E + Cb * curl(H) (backward differences), then sponge
 // damping or nothing, then the fused point sources.
 // ---------------------------------------------------------------------------
-template <int MODE, bool LOSSY>
+template <int MODE, bool LOSSY, bool REC = false>
 __global__ __launch_bounds__(kThreads) void yee_e_kernel(GridArgs g, SrcArgs s, Tiling tl, int kz,
                                                         float* __restrict__ ex,
                                                         float* __restrict__ ey,
                                                         float* __restrict__ ez,
                                                         const float* __restrict__ hx,
                                                         const float* __restrict__ hy,
-                                                        const float* __restrict__ hz) {
+                                                        const float* __restrict__ hz,
+                                                        RecArgs r) {
   int bx, by, bz;
   if (!tile_of(tl, bx, by, bz)) return;  // (the whole workgroup)
   // material palette in LDS
@@ -189,6 +190,17 @@ __global__ __launch_bounds__(kThreads) void yee_e_kernel(GridArgs g, SrcArgs s, 
   if (MODE == kSponge) sxy = g.ssx[i] + g.ssy[j];
   const unsigned dic = xin ? di : 0u;
   const unsigned djc = yin ? dj : 0u;
+  // recording: this column's offsets in the box and in the probe map
+  bool rb = false, rp = false;
+  unsigned rl = 0, pl = 0;
+  if (REC) {
+    const unsigned li = (unsigned)(i - r.bx0), lj = (unsigned)(j - r.by0);
+    rb = r.box_out && li < (unsigned)r.bnx && lj < (unsigned)r.bny;
+    rl = lj * r.bnx + li;
+    const unsigned pi = (unsigned)(i - r.px0), pj = (unsigned)(j - r.py0);
+    rp = r.probe_out && pi < (unsigned)r.pnx && pj < (unsigned)r.pny;
+    pl = pj * r.pnx + pi;
+  }
   float c_hx0, c_hy0, c_hz0, c_hzj, c_hxj, c_hzi, c_hyi, c_cb, c_ca = 1.f, c_ex0, c_ey0, c_ez0;
   {
     c_hx0 = hx[idx];
@@ -284,6 +296,24 @@ __global__ __launch_bounds__(kThreads) void yee_e_kernel(GridArgs g, SrcArgs s, 
     ex[idx] = vx;
     ey[idx] = vy;
     ez[idx] = vz;
+    if (REC) {
+      const unsigned lk = (unsigned)(k - r.bz0);
+      if (rb && lk < (unsigned)r.bnz) {
+        const unsigned l = lk * r.bny * r.bnx + rl;
+        r.box_out[l] = vx;
+        r.box_out[r.bc + l] = vy;
+        r.box_out[2u * r.bc + l] = vz;
+      }
+      const unsigned pk = (unsigned)(k - r.pz0);
+      if (rp && pk < (unsigned)r.pnz) {
+        const unsigned pc = (unsigned)r.pnx * r.pny * r.pnz;
+        const unsigned l = pk * r.pny * r.pnx + pl;
+        const int q0 = r.pmap[l], q1 = r.pmap[pc + l], q2 = r.pmap[2 * pc + l];
+        if (q0 >= 0) r.probe_out[q0] = vx;
+        if (q1 >= 0) r.probe_out[q1] = vy;
+        if (q2 >= 0) r.probe_out[q2] = vz;
+      }
+    }
     hxm = hx0;
     hym = hy0;
   }
@@ -461,34 +491,47 @@ int launch_h(const YeePlan* p, const GridArgs& g, hipStream_t st) {
   return YEE_OK;
 }
 
-template <bool LOSSY>
-void launch_e_mode(const YeePlan* p, const GridArgs& g, const SrcArgs& s, hipStream_t st) {
+template <bool LOSSY, bool REC>
+void launch_e_mode(const YeePlan* p, const GridArgs& g, const SrcArgs& s, const RecArgs& r,
+                   hipStream_t st) {
   const Tiling tl = plan_tiling(p);
   const unsigned nb = tiling_blocks(tl);
   switch (p->d.boundary) {
     case YEE_BOUNDARY_PERIODIC:
-      yee_e_kernel<kPeriodic, LOSSY><<<nb, kThreads, 0, st>>>(g, s, tl, p->kz, YEE_FIELDS(g));
+      yee_e_kernel<kPeriodic, LOSSY, REC><<<nb, kThreads, 0, st>>>(g, s, tl, p->kz,
+                                                                   YEE_FIELDS(g), r);
       break;
     case YEE_BOUNDARY_CPML:
-      yee_e_kernel<kCpml, LOSSY><<<nb, kThreads, 0, st>>>(g, s, tl, p->kz, YEE_FIELDS(g));
+      yee_e_kernel<kCpml, LOSSY, REC><<<nb, kThreads, 0, st>>>(g, s, tl, p->kz, YEE_FIELDS(g),
+                                                               r);
       break;
     case YEE_BOUNDARY_SPONGE:
-      yee_e_kernel<kSponge, LOSSY><<<nb, kThreads, 0, st>>>(g, s, tl, p->kz, YEE_FIELDS(g));
+      yee_e_kernel<kSponge, LOSSY, REC><<<nb, kThreads, 0, st>>>(g, s, tl, p->kz,
+                                                                 YEE_FIELDS(g), r);
       break;
     default:
-      yee_e_kernel<kWalls, LOSSY><<<nb, kThreads, 0, st>>>(g, s, tl, p->kz, YEE_FIELDS(g));
+      yee_e_kernel<kWalls, LOSSY, REC><<<nb, kThreads, 0, st>>>(g, s, tl, p->kz, YEE_FIELDS(g),
+                                                                r);
       break;
   }
 }
 
-int launch_e(const YeePlan* p, const GridArgs& g, int n, hipStream_t st) {
+int launch_e(const YeePlan* p, const GridArgs& g, int n, hipStream_t st, const RecArgs* rec) {
   SrcArgs s;
   int rc = make_src_args(p, n, s);
   if (rc) return rc;
-  if (p->d.ca)
-    launch_e_mode<true>(p, g, s, st);
-  else
-    launch_e_mode<false>(p, g, s, st);
+  if (rec && p->nsrc > YEE_FUSED_SOURCES)
+    return fail(YEE_EINVAL, "recording inside the E kernel needs every source fused");
+  RecArgs r;
+  std::memset(&r, 0, sizeof(r));
+  if (rec) r = *rec;
+  if (p->d.ca) {
+    if (rec) launch_e_mode<true, true>(p, g, s, r, st);
+    else launch_e_mode<true, false>(p, g, s, r, st);
+  } else {
+    if (rec) launch_e_mode<false, true>(p, g, s, r, st);
+    else launch_e_mode<false, false>(p, g, s, r, st);
+  }
   YEE_HIP_CHECK(hipGetLastError());
   return launch_inject(p, g.f[0], g.f[1], g.f[2], n, YEE_FUSED_SOURCES, st);
 }

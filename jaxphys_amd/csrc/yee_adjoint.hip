@@ -41,6 +41,14 @@ struct AdjArgs {
   const float* e_cur;          // box history E^n
   float* acc_cb;
   float* acc_ca;
+  // objective terms added to the E adjoint after this step (pnx == 0: none):
+  // pmap[(c * pnz + k) * pny + j) * pnx + i] = probe of E component c at
+  // (px0 + i, py0 + j, pz0 + k), or -1; term = pscale * pw[q] * pvals[q]
+  int px0, py0, pz0, pnx, pny, pnz;
+  const int* pmap;
+  const float* pw;
+  const float* pvals;
+  float pscale;
 };
 
 __device__ __forceinline__ unsigned adj_slot(const AdjArgs& a, int s) {
@@ -419,11 +427,34 @@ __device__ __forceinline__ void adj_e_body(
     ley[c] = e1;
     lez[c] = e2;
   }
+  // objective terms of the previous step's probes, added by the thread that
+  // wrote these cells (after the march, out of its loop) exactly as the
+  // probe scatter would add them
+  if (a.pnx > 0) {
+    const unsigned pi = (unsigned)(i - a.px0), pj = (unsigned)(j - a.py0);
+    if (pi < (unsigned)a.pnx && pj < (unsigned)a.pny) {
+      const unsigned pc = (unsigned)a.pnx * a.pny * a.pnz;
+      const int kb = max(k0, a.pz0), ke = min(k1, a.pz0 + a.pnz);
+      for (int k = kb; k < ke; ++k) {
+        const unsigned c = ((unsigned)k * ny + j) * nx + i;
+        const unsigned pl = ((unsigned)(k - a.pz0) * a.pny + pj) * a.pnx + pi;
+        const int q0 = a.pmap[pl], q1 = a.pmap[pc + pl], q2 = a.pmap[2 * pc + pl];
+        if (q0 >= 0) lex[c] += a.pscale * (a.pw ? a.pw[q0] : 1.f) * a.pvals[q0];
+        if (q1 >= 0) ley[c] += a.pscale * (a.pw ? a.pw[q1] : 1.f) * a.pvals[q1];
+        if (q2 >= 0) lez[c] += a.pscale * (a.pw ? a.pw[q2] : 1.f) * a.pvals[q2];
+      }
+    }
+  }
 }
+
+// The adjoint kernels are latency bound (short z marches, gathers from
+// neighbour rows): they are built for 8 waves per SIMD.  Of their uniform
+// values the ones beyond that SGPR budget live in VGPR lanes.
+#define YEE_ADJ_OCCUPANCY __attribute__((amdgpu_waves_per_eu(8)))
 
 // One launch per kernel and tile set (no CPML: the whole grid is one set).
 template <int MODE, int CPX>
-__global__ __launch_bounds__(kThreads) void yee_adj_h_kernel(
+__global__ __launch_bounds__(kThreads) YEE_ADJ_OCCUPANCY void yee_adj_h_kernel(
     AdjArgs a, TileSet ts, const float* __restrict__ lex, const float* __restrict__ ley,
     const float* __restrict__ lez, float* __restrict__ lhx, float* __restrict__ lhy,
     float* __restrict__ lhz) {
@@ -431,7 +462,7 @@ __global__ __launch_bounds__(kThreads) void yee_adj_h_kernel(
 }
 
 template <int MODE, bool LOSSY, int CPX>
-__global__ __launch_bounds__(kThreads) void yee_adj_e_kernel(
+__global__ __launch_bounds__(kThreads) YEE_ADJ_OCCUPANCY void yee_adj_e_kernel(
     AdjArgs a, TileSet ts, float* __restrict__ lex, float* __restrict__ ley,
     float* __restrict__ lez, const float* __restrict__ lhx, const float* __restrict__ lhy,
     const float* __restrict__ lhz) {
@@ -449,7 +480,7 @@ struct AdjSets {
 };
 
 template <int MODE>
-__global__ __launch_bounds__(kThreads) void yee_adj_h3_kernel(
+__global__ __launch_bounds__(kThreads) YEE_ADJ_OCCUPANCY void yee_adj_h3_kernel(
     AdjArgs a, AdjSets sets, const float* __restrict__ lex, const float* __restrict__ ley,
     const float* __restrict__ lez, float* __restrict__ lhx, float* __restrict__ lhy,
     float* __restrict__ lhz) {
@@ -467,7 +498,7 @@ __global__ __launch_bounds__(kThreads) void yee_adj_h3_kernel(
 }
 
 template <int MODE, bool LOSSY>
-__global__ __launch_bounds__(kThreads) void yee_adj_e3_kernel(
+__global__ __launch_bounds__(kThreads) YEE_ADJ_OCCUPANCY void yee_adj_e3_kernel(
     AdjArgs a, AdjSets sets, float* __restrict__ lex, float* __restrict__ ley,
     float* __restrict__ lez, const float* __restrict__ lhx, const float* __restrict__ lhy,
     const float* __restrict__ lhz) {
@@ -510,6 +541,42 @@ __global__ void box_gather_kernel(GridArgs g, int x0, int y0, int z0, int bnx, i
   out[2 * bc + l] = g.f[2][c];
 }
 
+// One launch per recorded step: blocks [0, nb_box) copy the design box of
+// E (one cell per thread over the flattened box, three components), the
+// rest gather the probes.  Box cells fit 32 bits (checked on the host).
+struct RecordArgs {
+  int x0, y0, z0, bnx, bny, bnz;
+  unsigned bc;        // box cells
+  int nb_box;         // box blocks
+  float* box_out;
+  int nprobe;
+  const int* comp;
+  const long long* idx;
+  float* probe_out;
+};
+
+constexpr int kRecThreads = 256;
+
+__global__ __launch_bounds__(kRecThreads) void record_kernel(GridArgs g, RecordArgs r) {
+  const int b = blockIdx.x;
+  if (b < r.nb_box) {
+    const unsigned l = (unsigned)b * kRecThreads + threadIdx.x;
+    if (l >= r.bc) return;
+    const unsigned li = l % (unsigned)r.bnx;
+    const unsigned row = l / (unsigned)r.bnx;
+    const unsigned lj = row % (unsigned)r.bny;
+    const unsigned lk = row / (unsigned)r.bny;
+    const long long c =
+        ((long long)(r.z0 + (int)lk) * g.ny + (r.y0 + (int)lj)) * g.nx + (r.x0 + (int)li);
+    const float e0 = g.f[0][c], e1 = g.f[1][c], e2 = g.f[2][c];
+    r.box_out[l] = e0;
+    r.box_out[r.bc + l] = e1;
+    r.box_out[2ull * r.bc + l] = e2;
+    return;
+  }
+  const int q = (b - r.nb_box) * kRecThreads + threadIdx.x;
+  if (q < r.nprobe) r.probe_out[q] = g.f[r.comp[q]][r.idx[q]];
+}
 
 int adjoint_args(const YeePlan* p, float* const* lam, float* lam_psi, const YeeDesignBox* box,
                  AdjArgs& a) {
@@ -624,10 +691,6 @@ int box_check(const YeePlan* p, const int32_t* box) {
   return YEE_OK;
 }
 
-void box_gather_launch(const GridArgs& g, const int32_t* box, float* out, hipStream_t st) {
-  const dim3 grid((box[3] + 63) / 64, box[4], box[5]);
-  box_gather_kernel<<<grid, 64, 0, st>>>(g, box[0], box[1], box[2], box[3], box[4], box[5], out);
-}
 
 }  // namespace yee
 
@@ -692,7 +755,8 @@ int yee_box_gather(float* const* fields, int32_t nx, int32_t ny, int32_t nz, con
 
 int yee_run_record(YeePlan* p, float* const* fields, float* psi, int32_t n0, int32_t nsteps,
                    int32_t nprobe, const int32_t* probe_comp, const int64_t* probe_idx,
-                   float* probe_out, const int32_t* box, float* box_out, void* stream) {
+                   float* probe_out, const int32_t* box, float* box_out,
+                   const int32_t* probe_box, const int32_t* probe_map, void* stream) {
   int rc = check_fields(p, fields, psi);
   if (rc) return rc;
   if (nsteps < 0 || nprobe < 0) return fail(YEE_EINVAL, "negative count");
@@ -701,21 +765,63 @@ int yee_run_record(YeePlan* p, float* const* fields, float* psi, int32_t n0, int
   rc = box_check(p, box);
   if (rc) return rc;
   if (box && !box_out) return fail(YEE_EINVAL, "box output missing");
+  const bool mapped = nprobe > 0 && probe_box;
+  if (mapped) {
+    rc = box_check(p, probe_box);
+    if (rc) return rc;
+    if (!probe_map) return fail(YEE_EINVAL, "probe map missing");
+    if (3LL * probe_box[3] * probe_box[4] * probe_box[5] >= (1LL << 31))
+      return fail(YEE_EINVAL, "probe map box too large");
+  }
   const GridArgs g = make_args(p, fields, psi);
   hipStream_t st = (hipStream_t)stream;
   const long long bc3 = box ? 3LL * box[3] * box[4] * box[5] : 0;
+  if (bc3 >= (1LL << 32)) return fail(YEE_EINVAL, "design box too large to record");
+  // inside the E kernel when no source is injected after it and the probes
+  // (if any) come with a map; else one record launch after each step
+  const bool in_kernel = (box || nprobe > 0) && p->nsrc <= YEE_FUSED_SOURCES &&
+                         (nprobe == 0 || mapped);
+  RecArgs e;
+  std::memset(&e, 0, sizeof(e));
+  RecordArgs r;
+  std::memset(&r, 0, sizeof(r));
+  if (box) {
+    e.bx0 = r.x0 = box[0];
+    e.by0 = r.y0 = box[1];
+    e.bz0 = r.z0 = box[2];
+    e.bnx = r.bnx = box[3];
+    e.bny = r.bny = box[4];
+    e.bnz = r.bnz = box[5];
+    e.bc = r.bc = (unsigned)(bc3 / 3);
+    r.nb_box = (int)((r.bc + kRecThreads - 1) / kRecThreads);
+  }
+  if (mapped) {
+    e.pmap = probe_map;
+    e.px0 = probe_box[0]; e.py0 = probe_box[1]; e.pz0 = probe_box[2];
+    e.pnx = probe_box[3]; e.pny = probe_box[4]; e.pnz = probe_box[5];
+  }
+  r.nprobe = nprobe;
+  r.comp = probe_comp;
+  r.idx = (const long long*)probe_idx;
+  const int nblocks = r.nb_box + (nprobe + kRecThreads - 1) / kRecThreads;
   for (int s = 0; s < nsteps; ++s) {
     rc = launch_h(p, g, st);
     if (rc) return rc;
+    float* bo = box ? box_out + s * bc3 : nullptr;
+    float* po = nprobe > 0 ? probe_out + (long long)s * nprobe : nullptr;
+    if (in_kernel) {
+      e.box_out = bo;
+      e.probe_out = po;
+      rc = launch_e(p, g, n0 + s, st, &e);
+      if (rc) return rc;
+      continue;
+    }
     rc = launch_e(p, g, n0 + s, st);
     if (rc) return rc;
-    if (nprobe > 0) {
-      probe_gather_kernel<<<(nprobe + 255) / 256, 256, 0, st>>>(
-          g, probe_comp, (const long long*)probe_idx, nprobe, probe_out + (long long)s * nprobe);
-      YEE_HIP_CHECK(hipGetLastError());
-    }
-    if (box) {
-      box_gather_launch(g, box, box_out + s * bc3, st);
+    if (nblocks > 0) {
+      r.box_out = bo;
+      r.probe_out = po;
+      record_kernel<<<nblocks, kRecThreads, 0, st>>>(g, r);
       YEE_HIP_CHECK(hipGetLastError());
     }
   }
@@ -725,12 +831,21 @@ int yee_run_record(YeePlan* p, float* const* fields, float* psi, int32_t n0, int
 int yee_adjoint_run(YeePlan* p, float* const* lam, float* lam_psi, int32_t n_lo, int32_t n_hi,
                     int32_t nprobe, const int32_t* probe_comp, const int64_t* probe_idx,
                     const float* w, const float* probe_hist, float scale, const int32_t* box,
-                    const float* box_hist, float* acc_cb, float* acc_ca, void* stream) {
+                    const float* box_hist, float* acc_cb, float* acc_ca,
+                    const int32_t* probe_box, const int32_t* probe_map, void* stream) {
   if (n_lo < 0 || n_hi < n_lo) return fail(YEE_EINVAL, "bad step range");
   if (nprobe > 0 && (!probe_comp || !probe_idx || !probe_hist))
     return fail(YEE_EINVAL, "probe arrays missing");
   int rc = box_check(p, box);
   if (rc) return rc;
+  const bool mapped = nprobe > 0 && probe_box;
+  if (mapped) {
+    rc = box_check(p, probe_box);
+    if (rc) return rc;
+    if (!probe_map) return fail(YEE_EINVAL, "probe map missing");
+    if (3LL * probe_box[3] * probe_box[4] * probe_box[5] >= (1LL << 31))
+      return fail(YEE_EINVAL, "probe map box too large");
+  }
   YeeDesignBox b;
   std::memset(&b, 0, sizeof(b));
   long long bc3 = 0;
@@ -753,9 +868,17 @@ int yee_adjoint_run(YeePlan* p, float* const* lam, float* lam_psi, int32_t n_lo,
     AdjArgs a;
     rc = adjoint_args(p, lam, lam_psi, box ? &b : nullptr, a);
     if (rc) return rc;
+    if (mapped && n >= 1) {
+      a.px0 = probe_box[0]; a.py0 = probe_box[1]; a.pz0 = probe_box[2];
+      a.pnx = probe_box[3]; a.pny = probe_box[4]; a.pnz = probe_box[5];
+      a.pmap = probe_map;
+      a.pw = w;
+      a.pvals = probe_hist + (long long)(n - 1) * nprobe;
+      a.pscale = scale;
+    }
     rc = adjoint_launch(p, a, lam, st);
     if (rc) return rc;
-    if (n >= 1 && nprobe > 0) {
+    if (!mapped && n >= 1 && nprobe > 0) {
       probe_scatter_kernel<<<(nprobe + 255) / 256, 256, 0, st>>>(
           gl, probe_comp, (const long long*)probe_idx, w,
           probe_hist + (long long)(n - 1) * nprobe, scale, nprobe);

@@ -73,6 +73,19 @@ struct SrcArgs {
   float val[YEE_FUSED_SOURCES];
 };
 
+// What an E launch records of the E it writes (the forward run of the
+// adjoint): E on a box (comp-major, 3 * bc floats, x fastest) and the
+// values of probes given by a cell -> probe map over their bounding box,
+// pmap[((c * pnz + k) * pny + j) * pnx + i] = probe of E component c or -1.
+struct RecArgs {
+  float* box_out;  // null: no box
+  int bx0, by0, bz0, bnx, bny, bnz;
+  unsigned bc;
+  float* probe_out;  // null: no probes
+  const int* pmap;
+  int px0, py0, pz0, pnx, pny, pnz;
+};
+
 inline __device__ int slab_index(int a, int n, int p) {
   return a < p ? a : a - (n - 2 * p);
 }
@@ -400,6 +413,9 @@ inline int make_src_args(const YeePlan* p, int n, SrcArgs& s) {
 int launch_inject(const YeePlan* p, float* ex, float* ey, float* ez, int n, int first,
                   hipStream_t st);
 int launch_h(const YeePlan* p, const GridArgs& g, hipStream_t st);
-int launch_e(const YeePlan* p, const GridArgs& g, int n, hipStream_t st);
+// rec (optional): record inside the E kernel; only when every source of the
+// plan is fused into it (no injection launch after the kernel)
+int launch_e(const YeePlan* p, const GridArgs& g, int n, hipStream_t st,
+             const RecArgs* rec = nullptr);
 
 }  // namespace yee

@@ -85,6 +85,43 @@ def test_adjoint_checkpoint_interval_invariance(gpu):
         assert torch.equal(grads[0], grads[q]), q
 
 
+@pytest.mark.parametrize("rec", [True, False])
+def test_probe_map_equals_probe_scatter(gpu, rec):
+    """The objective terms added by the E-adjoint kernel through the cell ->
+    probe map equal those of the per-step scatter launch bit for bit, on
+    probes of all three E components; a probe listed twice is one term with
+    the summed weight."""
+    size, npml, n = (22, 20, 18), 4, 40
+    eps, sigma, box, tab, src, mon = _setup("cpml", True, size, npml, n)
+    x0, y0, z0, bx, by, bz = box
+    eps_box = torch.tensor(eps[x0:x0 + bx, y0:y0 + by, z0:z0 + bz], device=gpu)
+    probes = [("ex", (9, 8, 13)), ("ey", (10, 9, 13)), ("ez", (10, 9, 13)),
+              ("ez", (12, 10, 14)), ("ex", (3, 15, 12))]
+    weights = [1.0, 0.5, 2.0, 1.5, 0.25]
+    res = []
+    for mapped in (True, False):
+        g = _gpu_grid(size, "cpml", npml, eps, sigma, tab, src)
+        prob = AdjointProblem(g, n, DT, box, probes, weights, checkpoint_every=16,
+                              record_box=rec)
+        assert prob.probe_map is not None
+        if not mapped:
+            prob.probe_box, prob.probe_map = None, None
+        res.append(prob.value_and_grad(eps_box))
+    assert res[0][0] == res[1][0]
+    assert torch.equal(res[0][1], res[1][1])
+    g = _gpu_grid(size, "cpml", npml, eps, sigma, tab, src)
+    dup = AdjointProblem(g, n, DT, box, probes + [probes[3]], weights + [0.5],
+                         checkpoint_every=16, record_box=rec)
+    assert dup.nprobe == len(probes)
+    g = _gpu_grid(size, "cpml", npml, eps, sigma, tab, src)
+    one = AdjointProblem(g, n, DT, box, probes, weights[:3] + [2.0, 0.25],
+                         checkpoint_every=16, record_box=rec)
+    Jd, gd = dup.value_and_grad(eps_box)
+    Jo, go = one.value_and_grad(eps_box)
+    assert Jd == Jo
+    assert torch.equal(gd, go)
+
+
 def test_autograd_function_and_finite_differences(gpu):
     size, npml, n = (22, 20, 18), 4, 60
     eps, sigma, box, tab, src, mon = _setup("cpml", False, size, npml, n)
