@@ -382,7 +382,7 @@ def main() -> None:
     ap.add_argument("--closure-lr", type=float, default=1e-3)
     ap.add_argument("--closure-lr-final", type=float, default=1e-4,
                     help="learning rate at the end of the cosine schedule")
-    ap.add_argument("--closure-onpolicy-from", type=int, default=1,
+    ap.add_argument("--closure-onpolicy-from", type=int, default=-1,
                     help="epoch from which training uses the corrected rollout's own states "
                          "(-1: teacher-forced only)")
     ap.add_argument("--closure-no-forcing", action="store_true",
