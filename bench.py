@@ -1,7 +1,7 @@
 """FDTD3D throughput on one MI355X: Gcell-updates/s and HBM-roofline fraction.
 
 ``python bench.py [--steps K] [--warmup W] [--workload c3_512|c3_256|c2_128|c1_64|
-adjoint_c4|closure_c5|closure_net|tm2d] [--mode fused|two_pass]``
+c1_64_ref|adjoint_c4|closure_c5|closure_net|tm2d|snapshots_c3] [--mode fused|two_pass]``
 
 A step is one full Yee time step (H half-step, then E half-step with CPML
 psi, conductivity and the three dipoles) over the whole synthetic grid; the
@@ -37,6 +37,8 @@ def build_workload(name: str, device):
         g, steps = W.c2_boxes(128)
     elif name == "c1_64":
         g, steps = W.c1_sphere(64)
+    elif name == "c1_64_ref":
+        g, steps = W.c1_sphere(64, reference_compat=True)
     else:
         raise SystemExit(f"unknown workload {name}")
     return g, steps
@@ -79,6 +81,42 @@ def cpu_baseline(solver, budget_s: float):
             "cores": cores, "kind": "port",
             "sample": f"{steps} float32 steps of the same {solver.nx}x{solver.ny}x{solver.nz} "
                       f"workload in {secs:.1f} s (C/OpenMP oracle, {cpu_model})"}
+
+
+def cpu_baseline_numpy(budget_s: float, max_steps: int = 500):
+    """The NumPy float32 restatement of the reference's step (the oracle,
+    oracle/ref_fdtd.py: one thread, array at a time, the closest analogue of
+    the reference's XLA program) on the reference-compatible 64^3 sphere
+    workload (10-layer sponge, sine source), timed for about ``budget_s``
+    seconds of steps."""
+    import numpy as np
+
+    from jaxphys_amd import workloads as W
+    from oracle import ref_fdtd as R
+
+    n, dtype = 64, np.float32
+    size, c = (n, n, n), n // 2
+    dt = W.dt_for(1.0)
+    I, J, K = np.indices(size)
+    mats = [R.Mat(mask=(I - c) ** 2 + (J - c) ** 2 + (K - c) ** 2 <= 144, epsilon_r=4.0)]
+    srcs = [R.Src(R.C0 / 20.0, (c + 8, c, c))]
+    ch, ce = dt / (R.MU0 * 1.0), dt / (R.EPS0 * 1.0)
+    eps_r = R.build_eps_r(size, mats, dtype)
+    damp = R.damping_factor(R.sigma_avg(size, 10, 1.0, dtype), dt, dtype)
+    fields = [np.zeros(size, dtype=dtype) for _ in range(6)]
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        fields = R.step(fields, eps_r, damp, "absorbing", ch, ce, dtype)
+        vals, _ = R.source_values(srcs, steps, 0.0, dt, dtype)
+        fields = R.add_sources(fields, srcs, vals)
+        steps += 1
+        secs = time.perf_counter() - t0
+        if secs >= budget_s or steps >= max_steps:
+            break
+    return {"value": round(n ** 3 * steps / secs / 1e9, 5), "unit": "Gcell-upd/s", "cores": 1,
+            "kind": "port",
+            "sample": f"{steps} float32 steps of the reference-compatible 64^3 sphere workload "
+                      f"in {secs:.1f} s (NumPy restatement of the reference's step, one thread)"}
 
 
 def measured_traffic(workload: str, mode: str):
@@ -164,6 +202,70 @@ def adjoint_bench(args, dev):
         "recording_forward_s": round(t1 - t0, 4), "adjoint_s": round(t2 - t1, 4),
         "J": J, "fd_check": {"h": h, "cells": len(fd_err),
                               "max_abs_err_over_max_grad": worst, "samples": fd_err[:4]},
+    }
+
+
+def snapshots_bench(args, dev):
+    """The reference API's saved history at the roofline size: the 512^3
+    Gaussian-random-field CPML workload through ``EMGrid3D.simulate``, 100
+    steps with every 20th step saved (5 snapshots of 6 fields, 16.1 GB),
+    kept in HBM (``snapshots="device"``) or streamed into pinned host memory
+    while the run continues (``snapshots="host"``), against the same call
+    saving only step 0.  Wall time of the whole call (set-up included, the
+    same in all three)."""
+    import torch
+
+    from jaxphys_amd import workloads as W
+
+    n = 512
+    steps, every = args.steps or 100, 20
+    grid, _ = W.c3_grf(n, device=dev)
+    dt = W.dt_for(1.0)
+
+    def timed(save_every, mode):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        hist = grid.simulate(dt=dt, save_every=save_every, n_steps=steps, snapshots=mode)
+        torch.cuda.synchronize()
+        secs = time.perf_counter() - t0
+        nsnap = int(hist.ex.shape[0])
+        del hist
+        return secs, nsnap
+
+    timed(steps, "device")  # warm-up: kernels, allocator
+    bare, _ = timed(steps + 1, "device")
+    on_dev, nsnap = timed(every, "device")
+    torch.cuda.empty_cache()
+    # the first host call pins its history buffers; torch's caching host
+    # allocator hands the same pinned blocks to the next call
+    first_host, _ = timed(every, "host")
+    on_host, nsnap_h = timed(every, "host")
+    # the plain device-to-host copy rate of one snapshot-sized buffer
+    a = torch.empty(n ** 3, dtype=torch.float32, device=dev)
+    b = torch.empty(n ** 3, dtype=torch.float32, pin_memory=True)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(4):
+        b.copy_(a, non_blocking=True)
+    torch.cuda.synchronize()
+    d2h = 4 * 4 * n ** 3 / (time.perf_counter() - t0) / 1e9
+    del a, b
+    snap_bytes = 6 * 4 * n ** 3
+    return {
+        "metric": "FDTD3D history streaming (saved steps to pinned host memory)",
+        "value": round(nsnap_h * snap_bytes / on_host / 1e9, 2),
+        "unit": "GB/s of history delivered to host memory (whole call)",
+        "higher_is_better": True, "n_gpus": 1, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "snapshots_c3", "grid": [n, n, n], "steps": steps,
+                   "save_every": every, "snapshots": nsnap, "bytes_per_snapshot": snap_bytes},
+        "wall_s": {"no_history": round(bare, 3), "device_history": round(on_dev, 3),
+                   "host_history": round(on_host, 3),
+                   "host_history_first_call": round(first_host, 3)},
+        "overhead_vs_no_history": {"device": round(on_dev / bare, 3),
+                                   "host": round(on_host / bare, 3)},
+        "d2h_copy_GBps": round(d2h, 1),
+        "copy_bound_s": round(nsnap_h * snap_bytes / (d2h * 1e9), 3),
     }
 
 
@@ -443,6 +545,9 @@ def main() -> None:
     if args.workload == "tm2d":
         print(json.dumps(tm2d_bench(args, dev)), flush=True)
         return
+    if args.workload == "snapshots_c3":
+        print(json.dumps(snapshots_bench(args, dev)), flush=True)
+        return
 
     t_setup = time.time()
     grid, default_steps = build_workload(args.workload, dev)
@@ -520,6 +625,13 @@ def main() -> None:
         except Exception as exc:  # reported, never silently replaced
             cpu = {"value": None, "error": f"{type(exc).__name__}: {exc}"}
 
+    cpu_np = None
+    if args.cpu_budget > 0 and args.workload == "c1_64_ref":
+        try:
+            cpu_np = cpu_baseline_numpy(args.cpu_budget)
+        except Exception as exc:  # reported, never silently replaced
+            cpu_np = {"value": None, "error": f"{type(exc).__name__}: {exc}"}
+
     out = {
         "metric": "FDTD3D Gcell-upd/s & %HBM roofline",
         "value": round(gcells, 3),
@@ -566,6 +678,8 @@ def main() -> None:
         "fields_finite": finite,
         "setup_s": round(setup_s, 2),
     }
+    if cpu_np is not None:
+        out["cpu_baseline_numpy"] = cpu_np
     print(json.dumps(out), flush=True)
 
 
