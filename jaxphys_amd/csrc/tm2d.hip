@@ -17,8 +17,8 @@
 //
 // Each float32 operation is the reference's, in its order, and the build
 // uses -ffp-contract=off: the result is bit-identical to the float32
-// restatement.  A workgroup is 4 wavefronts stacked along y, each walking
-// 16 consecutive rows of one 64-column strip: a 64 x 64 tile.
+// restatement.  A workgroup is 16 wavefronts stacked along y, each walking
+// 4 consecutive rows of one 64-column strip: a 64 x 64 tile.
 // HBM traffic per cell: Ez, Hx, Hy read once and written once (24 B), plus
 // one damping float in the absorbing band and one conductor byte when
 // walls exist.
@@ -45,8 +45,8 @@ static int fail(int code, const std::string& msg) {
   } while (0)
 
 constexpr int kTx = 64;   // lanes along x
-constexpr int kTy = 4;    // wavefronts per workgroup, stacked along y
-constexpr int kRows = 16; // rows marched by each wavefront
+constexpr int kTy = 16;   // wavefronts per workgroup, stacked along y
+constexpr int kRows = 4;  // rows marched by each wavefront
 
 struct StepArgs {
   int nx, ny, npml;
@@ -106,13 +106,14 @@ __device__ __forceinline__ void cell_general(const StepArgs& g, const float* __r
   hyn_out = hyn;
 }
 
-// Damping, row sources and conductors at output cell (i, j), then the store.
+// Damping, row sources and conductors at output cell (i, j), then the store
+// (`conductor`: the cell's conductor flag, already loaded).
 template <int MODE>
-__device__ __forceinline__ void finish_cell(const StepArgs& g, int i, int j, bool in_x_band,
-                                            float ezn, float hxn, float hyn,
-                                            float* __restrict__ ez_out,
-                                            float* __restrict__ hx_out,
-                                            float* __restrict__ hy_out) {
+__device__ __forceinline__ void finish_cell_c(const StepArgs& g, int i, int j, bool in_x_band,
+                                              bool conductor, float ezn, float hxn, float hyn,
+                                              float* __restrict__ ez_out,
+                                              float* __restrict__ hx_out,
+                                              float* __restrict__ hy_out) {
   const int o = j * g.nx + i;
   if (MODE == TM2D_ABSORBING) {
     // fdtd.py:337-339; outside the band the factor is exp(-0) = 1
@@ -122,15 +123,28 @@ __device__ __forceinline__ void finish_cell(const StepArgs& g, int i, int j, boo
   for (int s = 0; s < g.n_src; ++s)
     if (g.rows[s] == j) ezn = ezn + g.vals[s];
   // fdtd.py:351-352
-  if (g.cond != nullptr && g.cond[o]) ezn = 0.0f;
+  if (conductor) ezn = 0.0f;
   ez_out[o] = ezn;
   hx_out[o] = hxn;
   hy_out[o] = hyn;
 }
 
+template <int MODE>
+__device__ __forceinline__ void finish_cell(const StepArgs& g, int i, int j, bool in_x_band,
+                                            float ezn, float hxn, float hyn,
+                                            float* __restrict__ ez_out,
+                                            float* __restrict__ hx_out,
+                                            float* __restrict__ hy_out) {
+  const bool conductor = g.cond != nullptr && g.cond[j * g.nx + i];
+  finish_cell_c<MODE>(g, i, j, in_x_band, conductor, ezn, hxn, hyn, ez_out, hx_out, hy_out);
+}
+
 // One workgroup: kTy wavefronts stacked along y, each a 64-column strip of
-// kRows rows (a 64 x 64 tile; measured on the 16384^2 grid against 256 x 8,
-// 256 x 16, 256 x 32 and 64 x 32 tiles, all within 3 %).  A
+// kRows rows, all of whose loads are issued before the march (a 64 x 64
+// tile; measured on the 16384^2 grid: 16 x 4 rows 1.288 ms per step,
+// 8 x 8 1.324, 8 x 4 1.316, 4 x 8 1.424, 4 x 16 1.507, and 4 x 16 with
+// the loads inside the march 1.461; the rows a wavefront shares with its
+// neighbours are then read from cache).  A
 // wavefront whose 64 columns all exist and whose rows all lie in
 // [1, ny-2] (no edge copies, no wrap in y) marches down its rows carrying
 // Ez(y-1), Ez(y) and the new Hx(y-1) in registers and taking the x
@@ -173,15 +187,32 @@ __global__ __launch_bounds__(kTx * kTy) void tm2d_step_kernel(
   const bool upd_hy = kPer || a <= nx - 2;
   const bool upd_e = kPer || a >= 1;
   int b = j0;
+  // every row's loads are issued before the march: kRows rows x 3 arrays in
+  // flight per wavefront (the march itself only needs the adjacent lanes)
+  float ezr[kRows], hxr[kRows], hyr[kRows];
+  bool cr[kRows];
+#pragma unroll
+  for (int r = 0; r < kRows; ++r) {
+    ezr[r] = ez[(b + r + 1) * nx + a];
+    hxr[r] = hx[(b + r) * nx + a];
+    hyr[r] = hy[(b + r) * nx + a];
+  }
+  if (g.cond != nullptr) {
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) cr[r] = g.cond[(b + r) * nx + i] != 0;
+  } else {
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) cr[r] = false;
+  }
   float e_ym = ez[(b - 1) * nx + a];
   float e_c = ez[b * nx + a];
   // the new Hx at (a, b-1); b-1 <= ny-2 always holds here
   float hxn_m = hx[(b - 1) * nx + a] - g.ch * (e_c - e_ym);
 #pragma unroll
   for (int r = 0; r < kRows; ++r, ++b) {
-    const float e_yp = ez[(b + 1) * nx + a];
-    const float hy0 = hy[b * nx + a];
-    const float hx0 = hx[b * nx + a];
+    const float e_yp = ezr[r];
+    const float hy0 = hyr[r];
+    const float hx0 = hxr[r];
     const float sh_xp = __shfl_down(e_c, 1);
     const float sh_xm = __shfl_up(e_c, 1);
     const float sh_hym = __shfl_up(hy0, 1);
@@ -195,7 +226,7 @@ __global__ __launch_bounds__(kTx * kTy) void tm2d_step_kernel(
       const float hyn_m = hy_m + g.ch * (e_c - e_xm);
       ezn = e_c + g.ce * ((hyn - hyn_m) - (hxn - hxn_m));
     }
-    finish_cell<MODE>(g, i, b, in_x_band, ezn, hxn, hyn, ez_out, hx_out, hy_out);
+    finish_cell_c<MODE>(g, i, b, in_x_band, cr[r], ezn, hxn, hyn, ez_out, hx_out, hy_out);
     e_ym = e_c;
     e_c = e_yp;
     hxn_m = hxn;
