@@ -122,6 +122,47 @@ def test_probe_map_equals_probe_scatter(gpu, rec):
     assert torch.equal(gd, go)
 
 
+@pytest.mark.parametrize("nsrc", [1, 10])
+def test_run_record_equals_steps_and_gathers(gpu, nsrc):
+    """yee_run_record equals plain steps followed by the box and probe
+    gathers, bit for bit: recording inside the E kernel (every source fused
+    into it, probes mapped) and the record launch after the step (probes
+    without the map, or more sources than the E kernel fuses)."""
+    from jaxphys_amd import _native
+
+    size, npml, n = (22, 20, 18), 4, 30
+    eps, sigma, box, tab, src, mon = _setup("cpml", True, size, npml, n)
+    g = EMGrid3D(size=size, resolution=1.0, boundary="cpml", pml_layers=npml)
+    g.add_material(DielectricRegion(mask=np.ones(size, bool), epsilon_r=eps, sigma=sigma))
+    rng = np.random.default_rng(7)
+    for q in range(nsrc):  # outside the design box (z 6..9)
+        pos = (int(rng.integers(2, 20)), int(rng.integers(2, 18)), int(rng.integers(2, 5)))
+        g.add_source(PointSource3D(0.0, pos, 1.0, "xyz"[q % 3], waveform=tab))
+    probes = [("ex", (9, 8, 13)), ("ey", (10, 9, 13)), ("ez", (10, 9, 13)), ("ez", (3, 15, 12))]
+    prob = AdjointProblem(g, n, DT, box, probes, checkpoint_every=n)
+    sol = prob.solver
+    nx, ny, nz = sol.nx, sol.ny, sol.nz
+    bc3 = 3 * prob.bcells
+    ref_box = torch.empty((n, bc3), device=gpu)
+    ref_probe = torch.empty((n, prob.nprobe), device=gpu)
+    sol.reset()
+    for s in range(n):
+        sol.run_two_pass(s, 1)
+        _native.box_gather(sol.fields, (nx, ny, nz), prob.box_int, ref_box[s])
+        _native.probe_gather(sol.fields, prob.probe_comp, prob.probe_idx, ref_probe[s])
+    for mapped in (True, False):
+        out_box = torch.full((n, bc3), float("nan"), device=gpu)
+        out_probe = torch.full((n, prob.nprobe), float("nan"), device=gpu)
+        sol.reset()
+        _native.run_record(sol.plan, sol.fields, sol.psi, 0, n, prob.probe_comp,
+                           prob.probe_idx, out_probe, box=prob.box_int, box_out=out_box,
+                           probe_box=prob.probe_box if mapped else None,
+                           probe_map=prob.probe_map if mapped else None)
+        torch.cuda.synchronize()
+        assert torch.equal(out_box, ref_box), mapped
+        assert torch.equal(out_probe, ref_probe), mapped
+
+
 def test_autograd_function_and_finite_differences(gpu):
     size, npml, n = (22, 20, 18), 4, 60
     eps, sigma, box, tab, src, mon = _setup("cpml", False, size, npml, n)
