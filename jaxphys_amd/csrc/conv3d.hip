@@ -49,6 +49,7 @@ int fail(int code, const std::string& m) {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned short u16;
+typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ float bf2f(u16 h) { return __uint_as_float((unsigned)h << 16); }
 __device__ __forceinline__ u16 f2bf(float f) {  // round to nearest even
@@ -150,6 +151,9 @@ __global__ __launch_bounds__(64 * WAVES) void conv_fwd_kernel(ConvArgs a) {
   const int z1 = min(z0 + a.zchunk, a.nz);
   const int nx = a.nx, ny = a.ny, nz = a.nz, cin = CIN > 0 ? CIN : a.cin;
   const bool rowok = yy < ny;
+  // 8-byte epilogue stores: channel pitch a multiple of 4, aligned outputs
+  const bool vec4 = (a.ycp & 3) == 0 && ((uintptr_t)a.y & 7) == 0 &&
+                    ((uintptr_t)a.zout & 7) == 0 && ((uintptr_t)a.aux & 7) == 0;
   // the NT input fragments of K step k0 at plane zz (zero outside the grid)
   auto load_b = [&](int zz, int k0, bf16x8* bf) {
     const int k = k0 + 8 * g;
@@ -251,6 +255,37 @@ __global__ __launch_bounds__(64 * WAVES) void conv_fwd_kernel(ConvArgs a) {
       const long long pos = ((long long)zz * ny + yy) * nx + px;
 #pragma unroll
       for (int m = 0; m < MT; ++m) {
+        // bf16 outputs: the lane's 4 consecutive channels in one 8-byte
+        // store (same values as the per-channel path below)
+        if (EPI != CONV_EPI_LINEAR_F32 && vec4) {
+          const int cb = m * 16 + 4 * g;
+          if (cb >= a.ycp) continue;
+          const long long o = pos * a.ycp + cb;
+          u16 yv[4], zv4[4];
+          u16x4 aux4 = {0, 0, 0, 0};
+          if (EPI == CONV_EPI_GELU_BWD) aux4 = *reinterpret_cast<const u16x4*>(a.aux + o);
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int co = cb + v;
+            const bool real = co < a.cout;
+            const float val = acc[m][n][v];
+            if (EPI == CONV_EPI_GELU) {
+              const float zv = real ? val + (a.bias ? a.bias[co] : 0.f) : 0.f;
+              const u16 zb = f2bf(zv);
+              zv4[v] = zb;
+              yv[v] = real ? f2bf(gelu(bf2f(zb))) : (u16)0;
+            } else if (EPI == CONV_EPI_LINEAR_BF16) {
+              yv[v] = real ? f2bf(val + (a.bias ? a.bias[co] : 0.f)) : (u16)0;
+            } else {  // CONV_EPI_GELU_BWD
+              yv[v] = real ? f2bf(val * gelu_grad(bf2f(aux4[v]))) : (u16)0;
+            }
+          }
+          *reinterpret_cast<u16x4*>(reinterpret_cast<u16*>(a.y) + o) =
+              u16x4{yv[0], yv[1], yv[2], yv[3]};
+          if (EPI == CONV_EPI_GELU && a.zout)
+            *reinterpret_cast<u16x4*>(a.zout + o) = u16x4{zv4[0], zv4[1], zv4[2], zv4[3]};
+          continue;
+        }
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int co = m * 16 + 4 * g + v;
