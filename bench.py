@@ -415,6 +415,7 @@ def closure_bench(args, dev):
                                  onpolicy_from=(None if args.closure_onpolicy_from < 0
                                                 else args.closure_onpolicy_from),
                                  forcing=not args.closure_no_forcing,
+                                 seed=args.closure_seed,
                                  log=lambda m: print("[closure]", m, file=sys.stderr, flush=True),
                                  **pkw)
     t_train = time.time() - t0
@@ -435,6 +436,7 @@ def closure_bench(args, dev):
                    "epochs": args.closure_epochs, "correct_every": args.closure_every,
                    "pulse_t0": args.closure_t0, "pulse_spread": args.closure_spread,
                    "pulse": args.closure_waveform, "forcing": not args.closure_no_forcing,
+                   "train_seed": args.closure_seed,
                    "onpolicy_from_epoch": args.closure_onpolicy_from,
                    "lr": [args.closure_lr, args.closure_lr_final]},
         "err_coarse_no_closure": round(float(np.mean(err_0)), 5),
@@ -487,6 +489,8 @@ def main() -> None:
     ap.add_argument("--closure-onpolicy-from", type=int, default=-1,
                     help="epoch from which training uses the corrected rollout's own states "
                          "(-1: teacher-forced only)")
+    ap.add_argument("--closure-seed", type=int, default=0,
+                    help="seed of the net's initial weights and of the crop sampling")
     ap.add_argument("--closure-no-forcing", action="store_true",
                     help="no forcing input channel (fields and eps_r only)")
     ap.add_argument("--closure-every", type=int, default=10,

@@ -4,8 +4,8 @@ has no neural-network code); the FDTD solver itself generates the training
 data and stays the fallback and the judge of the error.
 
 Network (``ClosureNet``): channels 7 -> 32 -> 32 -> 6 (E, H normalised and
-eps_r in; a field correction out), 3x3x3 zero-padded convolutions, exact-erf
-GELU after the first two layers.  The eighth input channel (the padding of
+eps_r in; a field correction out), 3x3x3 zero-padded convolutions, GELU
+(erf to 1.5e-7) after the first two layers.  The eighth input channel (the padding of
 the 7 to the kernels' multiple of 8) optionally carries the forcing: the E
 increment the coarse grid's point sources injected during the interval just
 run, at their cells.  A pulse whose increments do not sum to zero leaves

@@ -10,8 +10,8 @@
 // accumulation.  The layer's weights are staged once per block in LDS; each
 // wave owns one y row of 16*NT x positions and marches a chunk of z planes;
 // input fragments are 16-byte channel-contiguous loads (channels last).
-// Epilogues fuse bias, exact-erf GELU, the GELU derivative of the backward
-// pass and the float32 output of the last layer.
+// Epilogues fuse bias, erf GELU (erf to 1.5e-7, see erf_as), the GELU
+// derivative of the backward pass and the float32 output of the last layer.
 //
 // Weight gradient: dW[co][k] = sum_p dY[p][co] X[p + off(k)][c(k)], an MFMA
 // GEMM with K = positions.  Each block reduces a set of 32-position row
@@ -52,11 +52,12 @@ typedef unsigned short u16;
 typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ float bf2f(u16 h) { return __uint_as_float((unsigned)h << 16); }
-__device__ __forceinline__ u16 f2bf(float f) {  // round to nearest even
-  unsigned u = __float_as_uint(f);
-  if ((u & 0x7f800000u) == 0x7f800000u) return (u16)(u >> 16);  // inf / nan
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (u16)(u >> 16);
+// float -> bf16, round to nearest even (v_cvt_pk_bf16_f32 on gfx950)
+__device__ __forceinline__ u16 f2bf(float f) {
+  const __bf16 h = (__bf16)f;
+  u16 u;
+  __builtin_memcpy(&u, &h, 2);
+  return u;
 }
 
 __device__ __forceinline__ bf16x8 load8(const u16* p) {
@@ -66,12 +67,29 @@ __device__ __forceinline__ bf16x8 load8(const u16* p) {
   return r;
 }
 
+// erf by Abramowitz & Stegun, Handbook of Mathematical Functions (1964),
+// formula 7.1.26: erf(x) = 1 - t (a1 + t (a2 + t (a3 + t (a4 + t a5))))
+// exp(-x^2), t = 1 / (1 + p |x|), odd in x; |error| <= 1.5e-7, far below
+// the bf16 rounding of the activations.  Branch-free: a reciprocal, five
+// FMAs and one exp2 per value, where a library erff costs several times as
+// many VALU cycles (the GELU epilogue competes with the MFMAs for issue).
+__device__ __forceinline__ float erf_as(float x) {
+  const float ax = fabsf(x);
+  const float t = __builtin_amdgcn_rcpf(__builtin_fmaf(0.3275911f, ax, 1.0f));
+  float q = __builtin_fmaf(t, 1.061405429f, -1.453152027f);
+  q = __builtin_fmaf(t, q, 1.421413741f);
+  q = __builtin_fmaf(t, q, -0.284496736f);
+  q = __builtin_fmaf(t, q, 0.254829592f);
+  const float y = __builtin_fmaf(-(q * t), __expf(-ax * ax), 1.0f);
+  return __builtin_copysignf(y, x);
+}
+// GELU(z) = z Phi(z) = z (1 + erf(z / sqrt 2)) / 2 and its derivative
 __device__ __forceinline__ float gelu(float z) {
-  return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f));
+  return 0.5f * z * (1.0f + erf_as(z * 0.70710678118654752f));
 }
 __device__ __forceinline__ float gelu_grad(float z) {
-  return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) +
-         z * 0.39894228040143268f * expf(-0.5f * z * z);
+  return 0.5f * (1.0f + erf_as(z * 0.70710678118654752f)) +
+         z * 0.39894228040143268f * __expf(-0.5f * z * z);
 }
 
 // Cross-lane moves inside the 16-lane DPP rows, which are exactly the
@@ -117,9 +135,9 @@ struct ConvArgs {
   void* y;
 };
 
-// WAVES wavefronts per block share the staged weights (with 16, the LDS of
-// two blocks fits a CU and each SIMD runs 8 wavefronts).  The input
-// fragments of K step k+1 are loaded before the MFMAs of step k.
+// WAVES wavefronts per block share the staged weights (with 16 and up to
+// 128 VGPRs per lane, one block fills a CU: 4 wavefronts per SIMD).  The
+// input fragments of K step k+1 are loaded before the MFMAs of step k.
 // SHIFT (input channels a multiple of 32, so a K step is one tap): the K
 // loop walks the 9 (dz, dy) pairs; per pair and 32-channel block it loads
 // the NT fragments of the x+0 tap plus one fragment each side, and makes the
