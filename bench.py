@@ -416,6 +416,8 @@ def closure_bench(args, dev):
                                                 else args.closure_onpolicy_from),
                                  forcing=not args.closure_no_forcing,
                                  seed=args.closure_seed,
+                                 far_field=not args.closure_no_far_field,
+                                 val_seeds=list(range(900, 900 + args.closure_val)),
                                  log=lambda m: print("[closure]", m, file=sys.stderr, flush=True),
                                  **pkw)
     t_train = time.time() - t0
@@ -437,6 +439,8 @@ def closure_bench(args, dev):
                    "pulse_t0": args.closure_t0, "pulse_spread": args.closure_spread,
                    "pulse": args.closure_waveform, "forcing": not args.closure_no_forcing,
                    "train_seed": args.closure_seed,
+                   "far_field": not args.closure_no_far_field, "far_gain": net.far_gain,
+                   "val_seeds": args.closure_val, "net_gain": net.net_gain,
                    "onpolicy_from_epoch": args.closure_onpolicy_from,
                    "lr": [args.closure_lr, args.closure_lr_final]},
         "err_coarse_no_closure": round(float(np.mean(err_0)), 5),
@@ -444,6 +448,13 @@ def closure_bench(args, dev):
         "source_model_only": {k: round(float(np.mean([r.get(f"{k}_source_only", float("nan"))
                                                       for r in res])), 5)
                               for k in ("err", "err_far", "err_traj", "err_traj_far")},
+        "source_model_and_far_field": {k: round(float(np.mean([r.get(f"{k}_far_field",
+                                                                     float("nan"))
+                                                               for r in res])), 5)
+                                       for k in ("err", "err_far", "err_traj", "err_traj_far")},
+        "median_err_far": {k: round(float(np.median([r.get(f"err_far_{k}", float("nan"))
+                                                     for r in res])), 5)
+                           for k in ("coarse", "source_only", "far_field", "closure")},
         "err_far_coarse_no_closure": round(float(np.mean([r["err_far_coarse"] for r in res])), 5),
         "err_traj_far_closure": round(float(np.mean([r["err_traj_far_closure"] for r in res])), 5),
         "err_traj_far_coarse_no_closure": round(float(np.mean([r["err_traj_far_coarse"]
@@ -454,7 +465,9 @@ def closure_bench(args, dev):
         "per_seed": [{"seed": r["seed"], "err": round(r["err_closure"], 5),
                       "err_source_only": round(r.get("err_source_only", float("nan")), 5),
                       "err_far": round(r["err_far_closure"], 5),
-                      "err_far_source_only": round(r.get("err_far_source_only", float("nan")), 5)}
+                      "err_far_source_only": round(r.get("err_far_source_only", float("nan")), 5),
+                      "err_far_far_field": round(r.get("err_far_far_field", float("nan")), 5),
+                      "err_far_coarse": round(r["err_far_coarse"], 5)}
                      for r in res],
         "speedup_vs_fine": round(float(np.mean(sp)), 3),
         "t_fine_s": round(float(np.mean([r["t_fine_s"] for r in res])), 4),
@@ -491,6 +504,10 @@ def main() -> None:
                          "(-1: teacher-forced only)")
     ap.add_argument("--closure-seed", type=int, default=0,
                     help="seed of the net's initial weights and of the crop sampling")
+    ap.add_argument("--closure-val", type=int, default=2,
+                    help="validation seeds (900, 901, ...) that choose the net's gain")
+    ap.add_argument("--closure-no-far-field", action="store_true",
+                    help="no linear far-field operator (net and source model only)")
     ap.add_argument("--closure-no-forcing", action="store_true",
                     help="no forcing input channel (fields and eps_r only)")
     ap.add_argument("--closure-every", type=int, default=10,

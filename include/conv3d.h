@@ -79,6 +79,17 @@ int conv3d_apply_correction(int32_t nx, int32_t ny, int32_t nz, float* const* fi
                             const float* y, int32_t ystride, const float* scale, float gain,
                             int32_t margin, void* stream);
 
+/* The closure's linear far-field operator (float32; no reference
+ * counterpart): y[i][co] += sum over the 27 taps t = ((dz+1)*3 + dy+1)*3 +
+ * dx+1 and the six fields ci of
+ *   (w[t][ci][0][co] + (eps[i] - 1) w[t][ci][1][co]) * f_ci(i + (dx,dy,dz)) / scale[ci],
+ * taps outside the grid contributing 0.  fields: 6 device pointers
+ * (float32, x fastest); eps (per cell), scale (6), w (27*6*2*6 floats) and
+ * y (float32 channels last [..][ystride]) on the device.                  */
+int conv3d_stencil_f32(int32_t nx, int32_t ny, int32_t nz, float* const* fields,
+                       const float* eps, const float* scale, const float* w, float* y,
+                       int32_t ystride, void* stream);
+
 const char* conv3d_last_error(void);
 
 #ifdef __cplusplus

@@ -39,6 +39,10 @@ EXPORTS = {
                                                ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
                                                ctypes.c_void_p, ctypes.c_float, ctypes.c_int32,
                                                ctypes.c_void_p]),
+    "conv3d_stencil_f32": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                          ctypes.c_void_p]),
     "conv3d_last_error": (ctypes.c_char_p, []),
 }
 
@@ -116,3 +120,12 @@ def apply_correction(shape_xyz, fields, y, ystride, scale, gain=1.0, margin=0,
                                         ptr(scale), float(gain), int(margin),
                                         stream_handle(stream)),
           "conv3d_apply_correction")
+
+
+def stencil_f32(shape_xyz, fields, eps, scale, w, y, ystride, stream=None) -> None:
+    """``conv3d_stencil_f32``: y[..][co] += the linear far-field operator of
+    the six fields (w: 27*6*2*6 float32, [tap][field][plain, (eps-1)][co])."""
+    nx, ny, nz = shape_xyz
+    check(lib().conv3d_stencil_f32(nx, ny, nz, field_array(fields), ptr(eps), ptr(scale), ptr(w),
+                                   ptr(y), int(ystride), stream_handle(stream)),
+          "conv3d_stencil_f32")

@@ -50,6 +50,7 @@ import time
 
 import numpy as np
 import torch
+import torch.nn.functional as F
 
 from jaxphys_amd import _conv
 from jaxphys_amd.em.constants import C0
@@ -63,6 +64,13 @@ NFEAT = 3                           # source features: sum, first moment, sum * 
 _OFFS = [(dx, dy, dz) for dz in range(-(STENCIL // 2), STENCIL // 2 + 1)
          for dy in range(-(STENCIL // 2), STENCIL // 2 + 1)
          for dx in range(-(STENCIL // 2), STENCIL // 2 + 1)]
+# the linear far-field operator: 27 taps x 6 fields x (plain, eps - 1) -> 6
+FAR_TAPS = 27
+FAR_FEAT = 2 * 6 * FAR_TAPS
+FAR_RIDGE = 3e-3        # ridge, relative to the mean diagonal of the normal matrix
+FAR_GAIN = 0.75         # share of the fitted correction applied per interval
+FAR_SAMPLES = 60000     # cells sampled per training interval
+FAR_GROWTH = 1.1        # largest accepted energy ratio of the stability check
 
 
 def _bf16(t: torch.Tensor) -> torch.Tensor:
@@ -95,6 +103,13 @@ class ClosureNet:
         }
         # the linear source model (physical units; least squares, not Adam)
         self.ks = torch.zeros(NFEAT, COUT, STENCIL ** 3, device=self.device)
+        # the linear far-field operator (float32 stencil, least squares; the
+        # applied gain is folded into the weights); off until fitted
+        self.far_w = torch.zeros(FAR_TAPS * 6 * 2 * 6, device=self.device)
+        self.far_gain = 0.0
+        # scale of the net's correction (select_net_gain picks it on
+        # validation seeds after training)
+        self.net_gain = 1.0
         self.margin = 0
         self.adam = {k: (torch.zeros_like(v), torch.zeros_like(v)) for k, v in self.params.items()}
         self.step_count = 0
@@ -239,20 +254,32 @@ class ClosureNet:
                 x[fc.flat, 7] = (fc.feats[q, :, 0] * (1.0 / self.force_scale)).to(torch.bfloat16)
         return x
 
+    def add_far_field(self, y: torch.Tensor, solver, fields=None) -> None:
+        """y[.., :6] += the linear far-field operator of the solver's state
+        (or of ``fields``), normalised units, gain included."""
+        _conv.stencil_f32((solver.nx, solver.ny, solver.nz),
+                          solver.fields if fields is None else fields, solver.eps_r,
+                          self.scale, self.far_w, y, OUT_STRIDE)
+
     def correct(self, solver, gain: float = 1.0, m: int | None = None,
-                every: int | None = None, use_net: bool = True) -> None:
+                every: int | None = None, use_net: bool = True, use_far: bool = True) -> None:
         """fields += net(fields, eps[, forcing of steps m .. m+every-1])
-        (denormalised) outside the source boxes, plus the linear source
-        model inside them.  ``use_net=False`` applies the source model only
-        (for measuring what the net adds)."""
-        if use_net:
+        plus the linear far-field operator (denormalised) outside the source
+        boxes, plus the linear source model inside them.  ``use_net=False``
+        / ``use_far=False`` leave the net / the far-field operator out (for
+        measuring what each part adds)."""
+        if use_net and self.net_gain > 0:
             y0 = self.zero_response(solver)
             x = self.pack_fields(solver, m, every)
             y = self.forward(x, (solver.nx, solver.ny, solver.nz))
             y.sub_(y0)
+            if self.net_gain != 1.0:
+                y.mul_(self.net_gain)
         else:
             y = self._buf("y", (solver.nx * solver.ny * solver.nz, OUT_STRIDE), torch.float32)
             y.zero_()
+        if use_far and self.far_gain > 0:
+            self.add_far_field(y, solver)
         if self.forcing:
             shape = (solver.nx, solver.ny, solver.nz)
             box = self._box_terms(solver, shape, (0, 0, 0), every)[0]
@@ -334,7 +361,8 @@ class ClosureNet:
                 "scale": self.scale.cpu(), "eps_shift": self.eps_shift,
                 "eps_scale": self.eps_scale, "forcing": self.forcing,
                 "force_scale": self.force_scale, "margin": self.margin,
-                "ks": self.ks.cpu()}
+                "ks": self.ks.cpu(), "far_w": self.far_w.cpu(), "far_gain": self.far_gain,
+                "net_gain": self.net_gain}
 
 
 class _Forcing:
@@ -596,11 +624,156 @@ def _solve_source_fit(net, ls) -> None:
     net.ks = K.reshape(NFEAT, COUT, STENCIL ** 3).to(torch.float32)
 
 
+def _init_net(net: ClosureNet, prob: ClosureProblem, states: list, csol, every: int) -> None:
+    """Normalisation, forcing scale and correction margin from the first
+    training problem."""
+    set_normalisation(net, states[1:])
+    if net.forcing:
+        _set_force_scale(net, csol, every)
+    net.margin = prob.npml_c + 4
+
+
+def _far_cells(prob: ClosureProblem, net: ClosureNet) -> torch.Tensor:
+    """Flat indices of the cells the far-field operator is fitted on: the
+    corrected region (``net.margin`` from every face) minus the source-model
+    box around the source."""
+    nc = prob.nc
+    m = torch.zeros((nc, nc, nc), dtype=torch.bool, device=net.device)
+    a, b = net.margin, nc - net.margin
+    m[a:b, a:b, a:b] = True
+    i, j, k = prob.src_c
+    h = STENCIL // 2
+    m[max(0, k - h):k + h + 1, max(0, j - h):j + h + 1, max(0, i - h):i + h + 1] = False
+    return torch.nonzero(m.view(-1)).view(-1)
+
+
+def _far_features(net: ClosureNet, fields, eps_m1: torch.Tensor,
+                  sel: torch.Tensor) -> torch.Tensor:
+    """(n, FAR_FEAT) float64 features of the far-field operator at the flat
+    cells ``sel``: f_ci(cell + off(t)) / scale[ci] for tap t and field ci
+    (column t*6 + ci), then the same times eps - 1; zero outside the grid."""
+    nz, ny, nx = fields[0].shape
+    f = torch.stack([fields[c].double() / float(net.scale[c]) for c in range(6)])
+    fp = F.pad(f, (1, 1, 1, 1, 1, 1))
+    z, y, x = sel // (ny * nx), (sel // nx) % ny, sel % nx
+    cols = [fp[:, z + 1 + (t // 9 - 1), y + 1 + ((t // 3) % 3 - 1), x + 1 + (t % 3 - 1)]
+            for t in range(FAR_TAPS)]
+    x1 = torch.cat(cols, 0).T
+    return torch.cat([x1, x1 * eps_m1.reshape(-1)[sel][:, None]], 1)
+
+
+def _far_energy_ratio(net: ClosureNet, prob: ClosureProblem, every: int, n_steps: int) -> float:
+    """E energy after ``n_steps`` coarse steps of ``prob`` with the far-field
+    operator alone applied every ``every`` steps, over the same run without
+    it: a ratio well above 1 means the operator amplifies some mode."""
+    energy = []
+    for use in (False, True):
+        sol = prob.coarse_grid().build_solver(prob.dt_coarse(), n_steps + 1)
+        shape = (sol.nx, sol.ny, sol.nz)
+        y = torch.zeros(sol.nx * sol.ny * sol.nz, OUT_STRIDE, device=net.device)
+        for m in range(0, n_steps, every):
+            sol.run(m, every)
+            if use:
+                y.zero_()
+                net.add_far_field(y, sol)
+                _conv.apply_correction(shape, sol.fields, y, OUT_STRIDE, net.scale, 1.0,
+                                       net.margin)
+        energy.append(sum(float((f.double() ** 2).sum()) for f in sol.fields[:3]))
+    return energy[1] / energy[0] if energy[0] > 0 else float("inf")
+
+
+def fit_far_field(net: ClosureNet, device, seeds, n_fine: int, n_coarse: int, every: int,
+                  log=None, seed: int = 0, **problem_kw) -> None:
+    """Least-squares fit of the linear far-field operator (float32 27-tap
+    stencil of the six normalised fields and of the same times eps - 1) to
+    the teacher-forced one-interval residuals, restricted fine state minus
+    the coarse state ``every`` steps after starting from the restricted fine
+    state, on FAR_SAMPLES random cells per interval outside the source-model
+    boxes.  Ridge FAR_RIDGE; gain FAR_GAIN, halved until a run three times
+    the training horizon with the operator alone keeps its field energy
+    within FAR_GROWTH of the uncorrected run (an unconstrained fit amplifies
+    modes the training states do not contain); off if no gain passes.
+    Needs the normalisation set (``_init_net``)."""
+    rng = np.random.default_rng(seed + 1)
+    G = torch.zeros(FAR_FEAT, FAR_FEAT, dtype=torch.float64, device=net.device)
+    B = torch.zeros(FAR_FEAT, COUT, dtype=torch.float64, device=net.device)
+    t0 = time.time()
+    for sd in seeds:
+        prob = ClosureProblem(sd, n_fine, device=device, **problem_kw)
+        states, _ = prob.fine_states(n_coarse, every)
+        csol = prob.coarse_grid().build_solver(prob.dt_coarse(), n_coarse + every + 1)
+        eps_m1 = csol.eps_r.double() - 1.0
+        cand = _far_cells(prob, net)
+        for q in range(len(states) - 1):
+            csol.reset()
+            for f, v in zip(csol.fields, states[q]):
+                f.copy_(v)
+            csol.run(q * every, every)
+            cur = csol.fields
+            pick = torch.from_numpy(rng.integers(0, cand.numel(), FAR_SAMPLES)).to(net.device)
+            sel = cand[pick]
+            X = _far_features(net, cur, eps_m1, sel)
+            Y = torch.stack([((states[q + 1][c] - cur[c]).double() / float(net.scale[c]))
+                             .reshape(-1)[sel] for c in range(COUT)], 1)
+            G += X.T @ X
+            B += X.T @ Y
+        del states
+    lam = FAR_RIDGE * float(G.trace()) / FAR_FEAT
+    W = torch.linalg.solve(G + lam * torch.eye(FAR_FEAT, dtype=G.dtype, device=G.device), B)
+    # (plain/eps, tap, field, out) -> the kernel's (tap, field, plain/eps, out)
+    wk = W.reshape(2, FAR_TAPS, 6, COUT).permute(1, 2, 0, 3).reshape(-1).to(torch.float32)
+    check = ClosureProblem(seeds[-1], n_fine, device=device, **problem_kw)
+    gain = FAR_GAIN
+    for _ in range(4):
+        net.far_w = (wk * gain).contiguous()
+        net.far_gain = gain
+        ratio = _far_energy_ratio(net, check, every, 3 * n_coarse)
+        if log:
+            log(f"far-field operator: gain {gain:g}, energy ratio {ratio:.4f} after "
+                f"{3 * n_coarse} steps ({time.time() - t0:.1f} s)")
+        if ratio <= FAR_GROWTH:
+            return
+        gain *= 0.5
+    net.far_w.zero_()
+    net.far_gain = 0.0
+
+
+def select_net_gain(net: ClosureNet, device, seeds, n_fine: int, n_coarse: int, every: int,
+                    gains=(0.0, 0.25, 0.5, 1.0), log=None, **problem_kw) -> dict:
+    """Scale of the net's correction, chosen on validation seeds (neither
+    training nor test seeds): the gain whose corrected coarse runs have the
+    lowest trajectory error of E against the restricted fine states, summed
+    over the seeds.  Sets ``net.net_gain``; returns the errors per gain."""
+    errs = {g: 0.0 for g in gains}
+    for sd in seeds:
+        prob = ClosureProblem(sd, n_fine, device=device, **problem_kw)
+        states, _ = prob.fine_states(n_coarse, every)
+        for g in gains:
+            net.net_gain = g
+            net.pack()
+            csol = prob.coarse_grid().build_solver(prob.dt_coarse(), n_coarse + 1)
+            num = den = 0.0
+            for q, m in enumerate(range(0, n_coarse, every)):
+                csol.run(m, every)
+                net.correct(csol, 1.0, m, every)
+                a_, b_ = _err_terms(csol.fields, states[q + 1], prob.npml_c)
+                num, den = num + a_, den + b_
+            errs[g] += math.sqrt(num / den) if den > 0 else 0.0
+        del states
+    net.net_gain = min(gains, key=lambda g: errs[g])
+    if log:
+        log("net gain on validation seeds " + ", ".join(f"{g:g}: {e / len(seeds):.5f}"
+                                                        for g, e in errs.items())
+            + f" -> {net.net_gain:g}")
+    return errs
+
+
 def train_closure(device, seeds, n_fine: int = 256, n_coarse: int = 400, every: int = 10,
                   epochs: int = 2, crop: int = 64, lr: float = 1e-3, net: ClosureNet | None = None,
                   log=None, seed: int = 0, onpolicy_from: int | None = None,
                   lr_final: float | None = None, source_crops: float = 0.5,
-                  forcing: bool = True, reset_tol: float = 1.0, **problem_kw):
+                  forcing: bool = True, reset_tol: float = 1.0, far_field: bool = True,
+                  val_seeds=None, **problem_kw):
     """Train on fine-run/coarse pairs generated by the FDTD solver.
 
     A sample is a coarse state at coarse step m + every and its target, the
@@ -613,9 +786,23 @@ def train_closure(device, seeds, n_fine: int = 256, n_coarse: int = 400, every: 
     corrections left.  Loss: mean squared normalised error on an interior
     crop (a 3-cell border, where zero padding differs from the full domain,
     is excluded); a share ``source_crops`` of the crops holds the source.
-    Adam, learning rate cosine-annealed from ``lr`` to ``lr_final``."""
+    Adam, learning rate cosine-annealed from ``lr`` to ``lr_final``.
+    With ``far_field`` the linear far-field operator is fitted first
+    (``fit_far_field``) and the net learns what remains after it; with
+    ``val_seeds`` the net's gain is then chosen on them
+    (``select_net_gain``)."""
     rng = np.random.default_rng(seed)
     net = net or ClosureNet(device, seed=seed, forcing=forcing)
+    initialised = False
+    if far_field:
+        prob0 = ClosureProblem(seeds[0], n_fine, device=device, **problem_kw)
+        states0, _ = prob0.fine_states(n_coarse, every)
+        _init_net(net, prob0, states0,
+                  prob0.coarse_grid().build_solver(prob0.dt_coarse(), n_coarse + every + 1), every)
+        del states0
+        initialised = True
+        fit_far_field(net, device, seeds, n_fine, n_coarse, every, log=log, seed=seed,
+                      **problem_kw)
     history, target_var = [], []
     ls = {"G": torch.zeros(NFEAT, NFEAT, dtype=torch.float64, device=net.device),
           "B": torch.zeros(NFEAT, COUT, STENCIL ** 3, dtype=torch.float64, device=net.device)}
@@ -630,11 +817,9 @@ def train_closure(device, seeds, n_fine: int = 256, n_coarse: int = 400, every: 
             prob = ClosureProblem(sd, n_fine, device=device, **problem_kw)
             states, _ = prob.fine_states(n_coarse, every)
             csol = prob.coarse_grid().build_solver(prob.dt_coarse(), n_coarse + every + 1)
-            if ep == 0 and s_i == 0:
-                set_normalisation(net, states[1:])
-                if net.forcing:
-                    _set_force_scale(net, csol, every)
-                net.margin = prob.npml_c + 4
+            if not initialised:
+                _init_net(net, prob, states, csol, every)
+                initialised = True
             nc = prob.nc
             # crops whose loss region (crop minus its b-cell border) starts at
             # the correction margin
@@ -670,6 +855,15 @@ def train_closure(device, seeds, n_fine: int = 256, n_coarse: int = 400, every: 
                 x = net.pack_fields(csol, m, every, fields=cur).view(nc, nc, nc, CIN)
                 tgt = torch.stack([(states[q + 1][c] - cur[c]) / net.scale[c]
                                    for c in range(6)], dim=-1)
+                far = None
+                if net.far_gain > 0:
+                    # the net learns what the far-field operator leaves (the
+                    # source model, inside the boxes where the operator is
+                    # not applied, keeps the full residual)
+                    far = net._buf("far", (nc ** 3, OUT_STRIDE), torch.float32)
+                    far.zero_()
+                    net.add_far_field(far, csol, fields=cur)
+                    far = far.view(nc, nc, nc, OUT_STRIDE)[..., :6]
                 near = rng.random() < source_crops
                 cz = _crop_start(nc, margin, crop, rng, src[2] if near else None, b)
                 cy = _crop_start(nc, margin, crop, rng, src[1] if near else None, b)
@@ -677,6 +871,7 @@ def train_closure(device, seeds, n_fine: int = 256, n_coarse: int = 400, every: 
                 sl = (slice(cz, cz + crop), slice(cy, cy + crop), slice(cx, cx + crop))
                 xc = x[sl].contiguous().view(-1, CIN)
                 tc = tgt[sl]
+                tn = tc if far is None else tc - far[sl]
                 # correction = net(state) - net(zero state)
                 y = net.forward(xc, (crop, crop, crop), train=True).view(crop, crop, crop, 8)
                 y0 = net.forward(x0[sl].contiguous().view(-1, CIN), (crop, crop, crop),
@@ -693,10 +888,10 @@ def train_closure(device, seeds, n_fine: int = 256, n_coarse: int = 400, every: 
                     _accumulate_source_fit(net, csol, every, q, tc, box, slot, which,
                                            (cx, cy, cz), crop, b, ls)
                 err = torch.zeros_like(y)
-                err[..., :6] = (y[..., :6] - tc) * w[..., None]
+                err[..., :6] = (y[..., :6] - tn) * w[..., None]
                 n_in = float(w.sum()) * 6
                 loss = float((err ** 2).sum()) / n_in
-                target_var.append(float(((tc * w[..., None]) ** 2).sum()) / n_in)
+                target_var.append(float(((tn * w[..., None]) ** 2).sum()) / n_in)
                 frac = upd / max(1, n_updates - 1)
                 lr_now = lr_final + 0.5 * (lr - lr_final) * (1.0 + math.cos(math.pi * frac))
                 upd += 1
@@ -716,6 +911,9 @@ def train_closure(device, seeds, n_fine: int = 256, n_coarse: int = 400, every: 
                     f"{np.mean(history[-len(order):]):.4e} (zero correction "
                     f"{np.mean(target_var[-len(order):]):.4e}) ({time.time() - t_start:.1f} s)")
             del states, inputs
+    if val_seeds:
+        select_net_gain(net, device, list(val_seeds), n_fine, n_coarse, every, log=log,
+                        **problem_kw)
     return net, history
 
 
@@ -754,10 +952,12 @@ def evaluate_closure(net: ClosureNet, seed: int, n_fine: int = 256, n_coarse: in
     # reference states at every correction time (restricted fine states)
     states, fsol = prob.fine_states(n_coarse, every)
     del fsol
-    runs = [("coarse", None, True), ("closure", net, True)]
+    runs = [("coarse", None, True, True), ("closure", net, True, True)]
     if net.forcing:
-        runs.append(("source_only", net, False))
-    for label, model, use_net in runs:
+        runs.append(("source_only", net, False, False))
+    if net.far_gain > 0:
+        runs.append(("far_field", net, False, True))
+    for label, model, use_net, use_far in runs:
         csol = prob.coarse_grid().build_solver(prob.dt_coarse(), n_coarse + 1)
         if model is not None:
             model.pack()
@@ -766,7 +966,7 @@ def evaluate_closure(net: ClosureNet, seed: int, n_fine: int = 256, n_coarse: in
         for m in range(0, n_coarse, every):
             csol.run(m, every)
             if model is not None:
-                model.correct(csol, gain, m, every, use_net)
+                model.correct(csol, gain, m, every, use_net, use_far)
         torch.cuda.synchronize()
         out[f"t_{label}_s"] = time.perf_counter() - t0
         a_, b_ = _err_terms(csol.fields, ref, margin)
@@ -779,7 +979,7 @@ def evaluate_closure(net: ClosureNet, seed: int, n_fine: int = 256, n_coarse: in
         for q, m in enumerate(range(0, n_coarse, every)):
             csol.run(m, every)
             if model is not None:
-                model.correct(csol, gain, m, every, use_net)
+                model.correct(csol, gain, m, every, use_net, use_far)
             a_, b_ = _err_terms(csol.fields, states[q + 1], margin)
             num, den = num + a_, den + b_
             a_, b_ = _err_terms(csol.fields, states[q + 1], margin, far)

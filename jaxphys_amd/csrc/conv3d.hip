@@ -516,6 +516,64 @@ __global__ void apply_correction_kernel(int nx, int ny, int nz, float* f0, float
   for (int c = 0; c < 6; ++c) f[c][i] += gain * scale[c] * y[i * ys + c];
 }
 
+// The closure's linear far-field operator, float32:
+// y[i][co] += sum_{t, ci} (w[t][ci][0][co] + (eps[i] - 1) w[t][ci][1][co])
+//             * f_ci(i + off(t)) / scale[ci]
+// over the 27 taps off(t) = (dx, dy, dz) in {-1, 0, 1}^3, t = ((dz + 1) * 3
+// + dy + 1) * 3 + dx + 1, zero outside the grid.  One thread per cell; per
+// field the 27 taps are summed first and scaled once by 1 / scale[ci]; the
+// 12 weights of a (tap, field) pair are contiguous and uniform across the
+// workgroup (scalar loads); the neighbour values come from cache.
+__global__ __launch_bounds__(256) void stencil_f32_kernel(
+    int nx, int ny, int nz, const float* __restrict__ f0, const float* __restrict__ f1,
+    const float* __restrict__ f2, const float* __restrict__ f3, const float* __restrict__ f4,
+    const float* __restrict__ f5, const float* __restrict__ eps,
+    const float* __restrict__ scale, const float* __restrict__ w, float* __restrict__ y,
+    int ys) {
+  const long long n = (long long)nx * ny * nz;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int x = (int)(i % nx), yy = (int)((i / nx) % ny), z = (int)(i / ((long long)nx * ny));
+  // taps inside the grid
+  unsigned valid = 0;
+#pragma unroll
+  for (int t = 0; t < 27; ++t) {
+    const int sx = x + t % 3 - 1, sy = yy + (t / 3) % 3 - 1, sz = z + t / 9 - 1;
+    if (sx >= 0 && sx < nx && sy >= 0 && sy < ny && sz >= 0 && sz < nz) valid |= 1u << t;
+  }
+  const float* f[6] = {f0, f1, f2, f3, f4, f5};
+  float a[6], b[6];
+#pragma unroll
+  for (int co = 0; co < 6; ++co) a[co] = b[co] = 0.f;
+#pragma unroll
+  for (int ci = 0; ci < 6; ++ci) {
+    float pa[6], pb[6];
+#pragma unroll
+    for (int co = 0; co < 6; ++co) pa[co] = pb[co] = 0.f;
+#pragma unroll
+    for (int t = 0; t < 27; ++t) {
+      if (!(valid >> t & 1u)) continue;
+      const long long j = i + ((long long)(t / 9 - 1) * ny + ((t / 3) % 3 - 1)) * nx + (t % 3 - 1);
+      const float v = f[ci][j];
+      const float* wt = w + (t * 6 + ci) * 12;
+#pragma unroll
+      for (int co = 0; co < 6; ++co) {
+        pa[co] += wt[co] * v;
+        pb[co] += wt[6 + co] * v;
+      }
+    }
+    const float inv = 1.0f / scale[ci];
+#pragma unroll
+    for (int co = 0; co < 6; ++co) {
+      a[co] += pa[co] * inv;
+      b[co] += pb[co] * inv;
+    }
+  }
+  const float e = eps[i] - 1.0f;
+#pragma unroll
+  for (int co = 0; co < 6; ++co) y[i * ys + co] += a[co] + e * b[co];
+}
+
 int kpad_of(int cin) { return ((27 * cin + 31) / 32) * 32; }
 
 // JAXPHYS_AMD_CONV_NOSHIFT=1 turns the shifted-fragment loop off (same
@@ -712,6 +770,21 @@ int conv3d_apply_correction(int32_t nx, int32_t ny, int32_t nz, float* const* fi
   hipLaunchKernelGGL(apply_correction_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                      (hipStream_t)stream, nx, ny, nz, fields[0], fields[1], fields[2], fields[3],
                      fields[4], fields[5], y, ystride, scale, gain, margin);
+  CONV_CHECK(hipGetLastError());
+  return 0;
+}
+
+int conv3d_stencil_f32(int32_t nx, int32_t ny, int32_t nz, float* const* fields,
+                       const float* eps, const float* scale, const float* w, float* y,
+                       int32_t ystride, void* stream) {
+  if (!fields || !eps || !scale || !w || !y || ystride < 6 || nx <= 0 || ny <= 0 || nz <= 0)
+    return fail(-1, "bad argument");
+  for (int c = 0; c < 6; ++c)
+    if (!fields[c]) return fail(-1, "null field");
+  const long long n = (long long)nx * ny * nz;
+  hipLaunchKernelGGL(stencil_f32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, nx, ny, nz, fields[0], fields[1], fields[2], fields[3],
+                     fields[4], fields[5], eps, scale, w, y, ystride);
   CONV_CHECK(hipGetLastError());
   return 0;
 }

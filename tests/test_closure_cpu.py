@@ -43,3 +43,29 @@ def test_problem_media_agree_between_grids():
     assert 1.0 <= p.eps_c.min() and p.eps_c.max() <= 6.0
     i, j, k = p.src_c
     assert all(8 <= v < 24 for v in (i, j, k))
+
+
+def test_far_field_stencil_oracle_pins():
+    """The array-at-a-time far-field operator equals the cell-by-cell loop,
+    and a single weight reproduces a shifted, scaled field (so an offset
+    sign, a swapped channel or the eps term's sign shows)."""
+    from oracle import closure_ops as O
+
+    rng = np.random.default_rng(3)
+    shape = (5, 4, 6)                               # (z, y, x)
+    fields = [rng.standard_normal(shape) for _ in range(6)]
+    eps = 1.0 + 3.0 * rng.random(shape)
+    scale = 0.5 + rng.random(6)
+    w = rng.standard_normal((27, 6, 2, 6))
+    np.testing.assert_allclose(O.stencil(fields, eps, scale, w),
+                               O.stencil_bruteforce(fields, eps, scale, w), rtol=1e-12, atol=1e-12)
+    # one tap: off = (dx, dy, dz) = (+1, 0, -1) -> t = (0*3 + 1)*3 + 2 = 5
+    w1 = np.zeros((27, 6, 2, 6))
+    w1[5, 2, 0, 4] = 2.0
+    w1[5, 2, 1, 4] = 0.5
+    y = O.stencil(fields, eps, scale, w1)
+    expect = np.zeros(shape)
+    expect[1:, :, :-1] = fields[2][:-1, :, 1:] / scale[2]
+    expect *= 2.0 + 0.5 * (eps - 1.0)
+    np.testing.assert_allclose(y[..., 4], expect, rtol=1e-12, atol=1e-12)
+    assert np.abs(np.delete(y, 4, axis=-1)).max() == 0.0

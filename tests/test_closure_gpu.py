@@ -37,3 +37,39 @@ def test_closure_training_beats_plain_coarse(gpu):
         print(f"[closure] seed {r['seed']}: coarse {r['err_coarse']:.4f} "
               f"closure {r['err_closure']:.4f}", flush=True)
     assert np.mean([r["err_closure"] for r in res]) < np.mean([r["err_coarse"] for r in res])
+
+
+def test_far_field_operator_reduces_held_out_residual(gpu):
+    """The least-squares far-field operator, fitted on four seeds, lowers
+    the one-interval residual (restricted fine state minus the coarse state
+    run from the restricted fine state) of a held-out seed away from the
+    source, and passes its own long-run stability check."""
+    n_fine, n_coarse, every, kw = 64, 100, 10, dict(npml_fine=8)
+    net = CL.ClosureNet(gpu, seed=0)
+    prob0 = CL.ClosureProblem(0, n_fine, device=gpu, **kw)
+    states0, _ = prob0.fine_states(n_coarse, every)
+    CL._init_net(net, prob0, states0,
+                 prob0.coarse_grid().build_solver(prob0.dt_coarse(), n_coarse + every + 1), every)
+    CL.fit_far_field(net, gpu, [0, 1, 2, 3], n_fine, n_coarse, every, **kw)
+    assert net.far_gain > 0
+    prob = CL.ClosureProblem(1000, n_fine, device=gpu, **kw)
+    states, _ = prob.fine_states(n_coarse, every)
+    csol = prob.coarse_grid().build_solver(prob.dt_coarse(), n_coarse + every + 1)
+    cells = CL._far_cells(prob, net)
+    n = csol.nx * csol.ny * csol.nz
+    y = torch.zeros(n, CL.OUT_STRIDE, device=gpu)
+    before = after = 0.0
+    for q in range(len(states) - 1):
+        csol.reset()
+        for f, v in zip(csol.fields, states[q]):
+            f.copy_(v)
+        csol.run(q * every, every)
+        y.zero_()
+        net.add_far_field(y, csol)
+        for c in range(6):
+            r = ((states[q + 1][c] - csol.fields[c]) / net.scale[c]).reshape(-1)[cells].double()
+            before += float((r ** 2).sum())
+            after += float(((r - y[cells, c].double()) ** 2).sum())
+    print(f"[closure] held-out far residual with the operator: {after / before:.3f} of without",
+          flush=True)
+    assert after < 0.97 * before

@@ -189,3 +189,27 @@ def test_closure_training_reduces_loss(gpu):
     print(f"[conv] training loss {losses[0]:.3f} -> {np.mean(losses[-5:]):.3f} "
           f"(target variance {np.mean(tvar):.3f})", flush=True)
     assert np.mean(losses[-5:]) < 0.35 * np.mean(tvar)
+
+
+def test_far_field_stencil_matches_oracle(gpu):
+    """conv3d_stencil_f32 (the closure's float32 far-field operator) against
+    the float64 oracle on a grid with unequal sides; y is accumulated into."""
+    from jaxphys_amd import _conv
+    from oracle import closure_ops as O
+
+    rng = np.random.default_rng(11)
+    nx, ny, nz = 20, 13, 9
+    fields_np = [rng.standard_normal((nz, ny, nx)).astype(np.float32) for _ in range(6)]
+    eps_np = (1.0 + 5.0 * rng.random((nz, ny, nx))).astype(np.float32)
+    scale_np = (0.5 + rng.random(6)).astype(np.float32)
+    w_np = (0.1 * rng.standard_normal((27, 6, 2, 6))).astype(np.float32)
+    y0 = rng.standard_normal((nz * ny * nx, 8)).astype(np.float32)
+    fields = [torch.from_numpy(f).to(gpu) for f in fields_np]
+    y = torch.from_numpy(y0).to(gpu)
+    _conv.stencil_f32((nx, ny, nz), fields, torch.from_numpy(eps_np).to(gpu),
+                      torch.from_numpy(scale_np).to(gpu), torch.from_numpy(w_np).to(gpu), y, 8)
+    torch.cuda.synchronize()
+    ref = O.stencil(fields_np, eps_np, scale_np, w_np).reshape(-1, 6)
+    got = y.cpu().numpy()
+    np.testing.assert_allclose(got[:, :6] - y0[:, :6], ref, rtol=0, atol=2e-5 * np.abs(ref).max())
+    np.testing.assert_array_equal(got[:, 6:], y0[:, 6:])

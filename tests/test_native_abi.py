@@ -145,3 +145,9 @@ def test_conv3d_rejects_bad_channel_configs_host_only():
     rc = _conv.lib().conv3d_apply_correction(4, 4, 4, (ctypes.c_void_p * 6)(), 0x1000, 8,
                                              0x2000, 1.0, -1, None)
     assert rc == -1
+    rc = _conv.lib().conv3d_stencil_f32(4, 4, 4, (ctypes.c_void_p * 6)(), 0x1000, 0x2000,
+                                        0x3000, 0x4000, 8, None)
+    assert rc == -1 and "null field" in _conv.lib().conv3d_last_error().decode()
+    rc = _conv.lib().conv3d_stencil_f32(4, 4, 4, (ctypes.c_void_p * 6)(*[0x1000] * 6), 0x1000,
+                                        0x2000, 0x3000, 0x4000, 5, None)
+    assert rc == -1
