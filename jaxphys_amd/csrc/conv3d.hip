@@ -520,58 +520,80 @@ __global__ void apply_correction_kernel(int nx, int ny, int nz, float* f0, float
 // y[i][co] += sum_{t, ci} (w[t][ci][0][co] + (eps[i] - 1) w[t][ci][1][co])
 //             * f_ci(i + off(t)) / scale[ci]
 // over the 27 taps off(t) = (dx, dy, dz) in {-1, 0, 1}^3, t = ((dz + 1) * 3
-// + dy + 1) * 3 + dx + 1, zero outside the grid.  One thread per cell; per
-// field the 27 taps are summed first and scaled once by 1 / scale[ci]; the
-// 12 weights of a (tap, field) pair are contiguous and uniform across the
-// workgroup (scalar loads); the neighbour values come from cache.
+// + dy + 1) * 3 + dx + 1, zero outside the grid.  One thread per column pair
+// (x, y, z0) and (x, y, z0 + 1): per field and (dx, dy) it loads the four
+// planes z0 - 1 .. z0 + 2 once and feeds both cells, and each (tap, field)
+// weight set (12 floats, contiguous, uniform across the workgroup: scalar
+// loads) serves two cells.  Per field the taps are summed first and scaled
+// once by 1 / scale[ci].
 __global__ __launch_bounds__(256) void stencil_f32_kernel(
     int nx, int ny, int nz, const float* __restrict__ f0, const float* __restrict__ f1,
     const float* __restrict__ f2, const float* __restrict__ f3, const float* __restrict__ f4,
     const float* __restrict__ f5, const float* __restrict__ eps,
     const float* __restrict__ scale, const float* __restrict__ w, float* __restrict__ y,
     int ys) {
-  const long long n = (long long)nx * ny * nz;
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int x = (int)(i % nx), yy = (int)((i / nx) % ny), z = (int)(i / ((long long)nx * ny));
-  // taps inside the grid
-  unsigned valid = 0;
-#pragma unroll
-  for (int t = 0; t < 27; ++t) {
-    const int sx = x + t % 3 - 1, sy = yy + (t / 3) % 3 - 1, sz = z + t / 9 - 1;
-    if (sx >= 0 && sx < nx && sy >= 0 && sy < ny && sz >= 0 && sz < nz) valid |= 1u << t;
-  }
+  const int nzp = (nz + 1) / 2;
+  const long long npairs = (long long)nx * ny * nzp;
+  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npairs) return;
+  const int x = (int)(p % nx), yy = (int)((p / nx) % ny), z0 = 2 * (int)(p / ((long long)nx * ny));
+  const bool two = z0 + 1 < nz;
+  const long long plane = (long long)nx * ny;
+  const long long i0 = (long long)z0 * plane + (long long)yy * nx + x;
   const float* f[6] = {f0, f1, f2, f3, f4, f5};
-  float a[6], b[6];
+  float a0[6], b0[6], a1[6], b1[6];
 #pragma unroll
-  for (int co = 0; co < 6; ++co) a[co] = b[co] = 0.f;
-#pragma unroll
+  for (int co = 0; co < 6; ++co) a0[co] = b0[co] = a1[co] = b1[co] = 0.f;
+  // the field loop stays rolled: 81 instead of 168 VGPRs, 5 wavefronts per
+  // SIMD instead of 3 (0.211 against 0.252 ms at 128^3)
+#pragma unroll 1
   for (int ci = 0; ci < 6; ++ci) {
-    float pa[6], pb[6];
+    float pa0[6], pb0[6], pa1[6], pb1[6];
 #pragma unroll
-    for (int co = 0; co < 6; ++co) pa[co] = pb[co] = 0.f;
+    for (int co = 0; co < 6; ++co) pa0[co] = pb0[co] = pa1[co] = pb1[co] = 0.f;
 #pragma unroll
-    for (int t = 0; t < 27; ++t) {
-      if (!(valid >> t & 1u)) continue;
-      const long long j = i + ((long long)(t / 9 - 1) * ny + ((t / 3) % 3 - 1)) * nx + (t % 3 - 1);
-      const float v = f[ci][j];
-      const float* wt = w + (t * 6 + ci) * 12;
+    for (int q = 0; q < 9; ++q) {
+      const int dx = q % 3 - 1, dy = q / 3 - 1;
+      const int sx = x + dx, sy = yy + dy;
+      if (sx < 0 || sx >= nx || sy < 0 || sy >= ny) continue;
+      // planes z0 - 1 .. z0 + 2 of column (sx, sy); zero outside the grid
+      float v[4];
 #pragma unroll
-      for (int co = 0; co < 6; ++co) {
-        pa[co] += wt[co] * v;
-        pb[co] += wt[6 + co] * v;
+      for (int pz = 0; pz < 4; ++pz) {
+        const int sz = z0 - 1 + pz;
+        v[pz] = (sz >= 0 && sz < nz) ? f[ci][i0 + (long long)(pz - 1) * plane + dy * nx + dx]
+                                     : 0.f;
+      }
+#pragma unroll
+      for (int dz = 0; dz < 3; ++dz) {
+        const float* wt = w + ((dz * 9 + q) * 6 + ci) * 12;
+#pragma unroll
+        for (int co = 0; co < 6; ++co) {
+          pa0[co] += wt[co] * v[dz];
+          pb0[co] += wt[6 + co] * v[dz];
+          pa1[co] += wt[co] * v[dz + 1];
+          pb1[co] += wt[6 + co] * v[dz + 1];
+        }
       }
     }
     const float inv = 1.0f / scale[ci];
 #pragma unroll
     for (int co = 0; co < 6; ++co) {
-      a[co] += pa[co] * inv;
-      b[co] += pb[co] * inv;
+      a0[co] += pa0[co] * inv;
+      b0[co] += pb0[co] * inv;
+      a1[co] += pa1[co] * inv;
+      b1[co] += pb1[co] * inv;
     }
   }
-  const float e = eps[i] - 1.0f;
+  const float e0 = eps[i0] - 1.0f;
 #pragma unroll
-  for (int co = 0; co < 6; ++co) y[i * ys + co] += a[co] + e * b[co];
+  for (int co = 0; co < 6; ++co) y[i0 * ys + co] += a0[co] + e0 * b0[co];
+  if (two) {
+    const long long i1 = i0 + plane;
+    const float e1 = eps[i1] - 1.0f;
+#pragma unroll
+    for (int co = 0; co < 6; ++co) y[i1 * ys + co] += a1[co] + e1 * b1[co];
+  }
 }
 
 int kpad_of(int cin) { return ((27 * cin + 31) / 32) * 32; }
@@ -781,8 +803,8 @@ int conv3d_stencil_f32(int32_t nx, int32_t ny, int32_t nz, float* const* fields,
     return fail(-1, "bad argument");
   for (int c = 0; c < 6; ++c)
     if (!fields[c]) return fail(-1, "null field");
-  const long long n = (long long)nx * ny * nz;
-  hipLaunchKernelGGL(stencil_f32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+  const long long npairs = (long long)nx * ny * ((nz + 1) / 2);
+  hipLaunchKernelGGL(stencil_f32_kernel, dim3((unsigned)((npairs + 255) / 256)), dim3(256), 0,
                      (hipStream_t)stream, nx, ny, nz, fields[0], fields[1], fields[2], fields[3],
                      fields[4], fields[5], eps, scale, w, y, ystride);
   CONV_CHECK(hipGetLastError());
