@@ -382,6 +382,24 @@ def closure_net_bench(args, dev):
     ms = e0.elapsed_time(e1) / K
     flops = 2.0 * cells * 27 * (7 * 32 + 32 * 32 + 32 * 6)
     tflops = flops / (ms / 1e3) / 1e12
+    # the float32 far-field operator applied with the net (27 taps x 6
+    # fields x 12 outputs = 1944 FMAs per cell)
+    from jaxphys_amd import _conv
+    fields = [torch.randn(shape, device=dev) for _ in range(6)]
+    eps = 1.0 + torch.rand(shape, device=dev)
+    sc = torch.ones(6, device=dev)
+    w = 0.01 * torch.randn(27 * 72, device=dev)
+    y = torch.zeros(cells, 8, device=dev)
+    for _ in range(3):
+        _conv.stencil_f32((n, n, n), fields, eps, sc, w, y, 8)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(K):
+        _conv.stencil_f32((n, n, n), fields, eps, sc, w, y, 8)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    st_ms = e0.elapsed_time(e1) / K
+    st_tflops = 2.0 * 1944 * cells / (st_ms / 1e3) / 1e12
     return {
         "metric": "closure net inference (3 conv3d layers, 128^3)", "value": round(tflops, 2),
         "unit": "TFLOP/s", "higher_is_better": True, "n_gpus": 1, "steps": K,
@@ -392,6 +410,9 @@ def closure_net_bench(args, dev):
         "roofline": {"bound": "mfma", "kernel": "conv_fwd_kernel (3 launches per inference)",
                      "achieved": round(tflops, 2), "peak": 2500.0, "unit": "TFLOP/s",
                      "frac": round(tflops / 2500.0, 4), "traffic": None},
+        "far_field_stencil": {"kernel": "stencil_f32_kernel", "ms": round(st_ms, 4),
+                              "TFLOP/s": round(st_tflops, 2),
+                              "frac_of_fp32_vector_peak": round(st_tflops / 78.6, 4)},
     }
 
 
