@@ -167,14 +167,15 @@ int yee_step_e(YeePlan* plan, float* const* fields, float* psi, int32_t n,
  * Objective terms are added between steps with yee_probe_scatter_add.
  * Optional design box: cells where dJ/d(eps_r) is wanted, all with damping
  * 1 (outside sponge/CPML slabs) and every E component updated; the caller
- * passes the forward E^{n+1} and E^n on the box (comp-major, x fastest)
- * and the kernel accumulates acc_cb += sum_c g_c (E^{n+1}_c - Ca E^n_c),
- * acc_ca += sum_c g_c E^n_c, with g = lam(E^{n+1}); then dJ/dCb =
- * acc_cb / Cb and dJ/dCa = acc_ca.                                        */
+ * passes d = E^{n+1} - Ca E^n of the forward step on the box (comp-major,
+ * x fastest; what yee_run_record records) and, when acc_ca is given, E^n;
+ * the kernel accumulates acc_cb += sum_c g_c d_c, acc_ca += sum_c g_c
+ * E^n_c, with g = lam(E^{n+1}); then dJ/dCb = acc_cb / Cb and dJ/dCa =
+ * acc_ca.                                                                 */
 typedef struct YeeDesignBox {
   int32_t x0, y0, z0, nx, ny, nz;
-  const float* e_next;
-  const float* e_cur;
+  const float* d;
+  const float* e_cur; /* E^n: needed only with acc_ca */
   float* acc_cb;
   float* acc_ca; /* may be NULL when lossless */
 } YeeDesignBox;
@@ -197,10 +198,13 @@ int yee_probe_scatter_add(float* const* fields, const int32_t* comp, const int64
 int yee_box_gather(float* const* fields, int32_t nx, int32_t ny, int32_t nz,
                    const int32_t* box, float* out, void* stream);
 
-/* yee_run plus recording after every step s = 0..nsteps-1 (step n0+s):
- * probe_out[s*nprobe + q] = field value of probe q (as yee_probe_gather)
- * and box_out[s*3*boxcells ...] = E on the box (as yee_box_gather).  Either
- * part may be disabled (nprobe == 0 / box == NULL).  probe_box / probe_map
+/* yee_run plus recording after every step s = 0..nsteps-1 (step n = n0+s):
+ * probe_out[s*nprobe + q] = field value of probe q after the step (as
+ * yee_probe_gather), and the step's box record at box_out[s*R ...] with R =
+ * 3*boxcells (R = 6*boxcells for lossy plans): d = E^{n+1} - Ca E^n on the
+ * box (comp-major, x fastest), followed for lossy plans by E^n — what the
+ * adjoint's eps gradient uses.  Either part may be disabled (nprobe == 0 /
+ * box == NULL).  probe_box / probe_map
  * (optional, both or neither): the cell -> probe map of yee_adjoint_run,
  * for E probes at distinct (component, cell) pairs.  When the probes are
  * mapped (or absent) and every source is fused into the E kernel, that
@@ -218,8 +222,8 @@ int yee_run_record(YeePlan* plan, float* const* fields, float* psi, int32_t n0,
  * (the objective sum_n sum_q w_q F_q(after step n)^2 gives scale = 2).
  * Probes are E components at distinct (component, cell) pairs (merge
  * duplicates by adding their weights).
- * box history: box_hist[m] = E on the box after step n_lo-1+m, m = 0 ..
- * n_hi-n_lo (so box_hist[0] is E^{n_lo}).
+ * box history: the box records of steps n_lo .. n_hi-1 as yee_run_record
+ * writes them (record m of step n_lo+m, R floats each).
  * probe_box / probe_map (optional, both or neither): a box (x0, y0, z0,
  * nx, ny, nz; host array) holding every probe cell and, on the device,
  * probe_map[((c*nz + k)*ny + j)*nx + i] = q for probe q of E component c

@@ -104,7 +104,7 @@ class YeeDesignBox(ctypes.Structure):
 
     _fields_ = [("x0", ctypes.c_int32), ("y0", ctypes.c_int32), ("z0", ctypes.c_int32),
                 ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
-                ("e_next", ctypes.c_void_p), ("e_cur", ctypes.c_void_p),
+                ("d", ctypes.c_void_p), ("e_cur", ctypes.c_void_p),
                 ("acc_cb", ctypes.c_void_p), ("acc_ca", ctypes.c_void_p)]
 
 

@@ -5,19 +5,19 @@ No reference counterpart: the reference would differentiate
 reverse mode through ``lax.scan`` at src/neurosim/em/fdtd3d.py:393), which
 keeps residuals of every step (O(n_steps * N^3) memory).  Here:
 
-The adjoint sweep needs, of the forward run, only the probe values and E
-on the design box after every step.  When that box history fits in a
-quarter of the free device memory (``record_box``, automatic), the forward
-run records it for all steps and the backward pass is a single adjoint
-sweep.  Otherwise, checkpointing:
+The adjoint sweep needs, of the forward run, only the probe values and,
+per step, d = E^{n+1} - Ca E^n on the design box (plus E^n for lossy
+plans), which the forward E kernel records from its registers.  When that
+box history fits in a quarter of the free device memory (``record_box``,
+automatic), the forward run records it for all steps and the backward pass
+is a single adjoint sweep.  Otherwise, checkpointing:
 
 1. forward run on the GPU, saving the full state (6 fields + CPML psi) every
    ``checkpoint_every`` steps and the probe values after every step;
 2. for each checkpoint segment, last first: restore the checkpoint, recompute
-   the segment forward while recording E on the design box after every step,
-   then run the adjoint steps of the segment in reverse
-   (``yee_adjoint_run``), which accumulate ``sum_n g (E^{n+1} - Ca E^n)`` and
-   ``sum_n g E^n`` per design cell;
+   the segment forward while recording the design box for every step, then
+   run the adjoint steps of the segment in reverse (``yee_adjoint_run``),
+   which accumulate ``sum_n g d`` and ``sum_n g E^n`` per design cell;
 
 and in both cases the chain rule to eps_r:
 ``dJ/deps = acc_cb/Cb * dCb/deps + acc_ca * dCa/deps``.
@@ -135,7 +135,9 @@ class AdjointProblem:
         self.probe_hist = torch.zeros((self.n_steps, self.nprobe), dtype=torch.float32, device=dev)
         self.checkpoints = []
         self.box_hist = None
-        need = (self.n_steps + 1) * 3 * self.bcells * 4
+        # one step's box record: d = E^{n+1} - Ca E^n, and E^n for lossy plans
+        self.rec_floats = 3 * self.bcells * (2 if sol.ca is not None else 1)
+        need = self.n_steps * self.rec_floats * 4
         if record_box is None:
             free, _ = torch.cuda.mem_get_info(dev)
             record_box = need <= free // 4
@@ -161,14 +163,12 @@ class AdjointProblem:
         self.checkpoints = []
         N, K = self.n_steps, self.K
         if self.record_box:
-            if self.box_hist is None or self.box_hist.shape[0] != N + 1:
-                self.box_hist = torch.empty((N + 1, 3 * self.bcells), dtype=torch.float32,
+            if self.box_hist is None or self.box_hist.shape[0] != N:
+                self.box_hist = torch.empty((N, self.rec_floats), dtype=torch.float32,
                                             device=sol.device)
-            _native.box_gather(sol.fields, (sol.nx, sol.ny, sol.nz), self.box_int,
-                               self.box_hist[0])
             _native.run_record(sol.plan, sol.fields, sol.psi, 0, N, self.probe_comp,
                                self.probe_idx, self.probe_hist, box=self.box_int,
-                               box_out=self.box_hist[1:], probe_box=self.probe_box,
+                               box_out=self.box_hist, probe_box=self.probe_box,
                                probe_map=self.probe_map)
             self._recorded = True
             ph = self.probe_hist.to(torch.float64)
@@ -205,8 +205,8 @@ class AdjointProblem:
                                 box_hist=self.box_hist, acc_cb=acc_cb, acc_ca=acc_ca,
                                 probe_box=self.probe_box, probe_map=self.probe_map)
             return self._chain_rule(acc_cb, acc_ca)
-        if self.box_hist is None or self.box_hist.shape[0] < K + 1:
-            self.box_hist = torch.empty((K + 1, 3 * self.bcells), dtype=torch.float32, device=dev)
+        if self.box_hist is None or self.box_hist.shape[0] < K:
+            self.box_hist = torch.empty((K, self.rec_floats), dtype=torch.float32, device=dev)
         for seg in range(len(self.checkpoints) - 1, -1, -1):
             s0 = seg * K
             s1 = min(N, s0 + K)
@@ -215,10 +215,8 @@ class AdjointProblem:
                 f.copy_(c)
             if psi is not None:
                 sol.psi.copy_(psi)
-            _native.box_gather(sol.fields, (sol.nx, sol.ny, sol.nz), self.box_int,
-                               self.box_hist[0])
             _native.run_record(sol.plan, sol.fields, sol.psi, s0, s1 - s0, box=self.box_int,
-                               box_out=self.box_hist[1:])
+                               box_out=self.box_hist)
             _native.adjoint_run(sol.plan, lam, lam_psi, s0, s1, self.probe_comp, self.probe_idx,
                                 self.w, self.probe_hist, 2.0, box=self.box_int,
                                 box_hist=self.box_hist, acc_cb=acc_cb, acc_ca=acc_ca,

@@ -299,10 +299,17 @@ __global__ __launch_bounds__(kThreads) void yee_e_kernel(GridArgs g, SrcArgs s, 
     if (REC) {
       const unsigned lk = (unsigned)(k - r.bz0);
       if (rb && lk < (unsigned)r.bnz) {
+        // d = E^{n+1} - Ca E^n, the same float operations as the adjoint's
+        // accumulation applies to the two states
         const unsigned l = lk * r.bny * r.bnx + rl;
-        r.box_out[l] = vx;
-        r.box_out[r.bc + l] = vy;
-        r.box_out[2u * r.bc + l] = vz;
+        r.box_out[l] = vx - (LOSSY ? ca * ex0 : ex0);
+        r.box_out[r.bc + l] = vy - (LOSSY ? ca * ey0 : ey0);
+        r.box_out[2u * r.bc + l] = vz - (LOSSY ? ca * ez0 : ez0);
+        if (LOSSY) {
+          r.box_out[3u * r.bc + l] = ex0;
+          r.box_out[4u * r.bc + l] = ey0;
+          r.box_out[5u * r.bc + l] = ez0;
+        }
       }
       const unsigned pk = (unsigned)(k - r.pz0);
       if (rp && pk < (unsigned)r.pnz) {

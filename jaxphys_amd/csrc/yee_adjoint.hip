@@ -18,8 +18,9 @@
 //   kernel B (per E cell): g = damp lamE; psi_E-adjoint update; lamE = Ca g
 //     (in range) plus the transpose of the H pass gathered from the H cells
 //     at the cell and its -x/-y/-z neighbours; design-box accumulation
-//     acc_cb += g (E^{n+1} - Ca E^n), acc_ca += g E^n, from which
-//     dJ/dCb = acc_cb / Cb and dJ/dCa = acc_ca.
+//     acc_cb += g d with d = E^{n+1} - Ca E^n (recorded by the forward E
+//     kernel), acc_ca += g E^n, from which dJ/dCb = acc_cb / Cb and
+//     dJ/dCa = acc_ca.
 // Reads and writes never collide: A writes only lamH and psi_H adjoints at
 // its own cell and reads lamE / psi_E adjoints; B the other way round.
 
@@ -37,8 +38,8 @@ struct AdjArgs {
   const float* cpml;           // x | y | z tables, 6 n_a floats each
   float* lpsi;                 // adjoint psi (layout of the forward psi)
   int bx0, by0, bz0, bnx, bny, bnz;  // design box (bnx == 0: none)
-  const float* e_next;         // box history E^{n+1}, comp-major
-  const float* e_cur;          // box history E^n
+  const float* hist_d;         // box record of the step: E^{n+1} - Ca E^n, comp-major
+  const float* hist_e;         // E^n on the box (lossy plans, for acc_ca)
   float* acc_cb;
   float* acc_ca;
   // objective terms added to the E adjoint after this step (pnx == 0: none):
@@ -412,15 +413,16 @@ __device__ __forceinline__ void adj_e_body(
         const unsigned bc = (unsigned)a.bnx * a.bny * a.bnz;
         const unsigned l = ((unsigned)lk * a.bny + lj) * a.bnx + li;
         const float gc[3] = {g0, g1, g2};
-        float s1 = 0.f, s2 = 0.f;
+        float s1 = 0.f;
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
-          const float en = a.e_next[q * bc + l], ec = a.e_cur[q * bc + l];
-          s1 += gc[q] * (en - (LOSSY ? ca * ec : ec));
-          s2 += gc[q] * ec;
-        }
+        for (int q = 0; q < 3; ++q) s1 += gc[q] * a.hist_d[q * bc + l];
         a.acc_cb[l] += s1;
-        if (LOSSY && a.acc_ca) a.acc_ca[l] += s2;
+        if (LOSSY && a.acc_ca) {
+          float s2 = 0.f;
+#pragma unroll
+          for (int q = 0; q < 3; ++q) s2 += gc[q] * a.hist_e[q * bc + l];
+          a.acc_ca[l] += s2;
+        }
       }
     }
     lex[c] = e0;
@@ -541,13 +543,19 @@ __global__ void box_gather_kernel(GridArgs g, int x0, int y0, int z0, int bnx, i
   out[2 * bc + l] = g.f[2][c];
 }
 
-// One launch per recorded step: blocks [0, nb_box) copy the design box of
-// E (one cell per thread over the flattened box, three components), the
-// rest gather the probes.  Box cells fit 32 bits (checked on the host).
+// The record launches of a step when the E kernel cannot record itself
+// (sources injected after it, or probes without a map).  Before the step,
+// record_pre copies E^n of the box into the step's record (both halves for
+// lossy plans); after it, record_post turns the first half into d =
+// E^{n+1} - Ca E^n in place (the E kernel's arithmetic) and gathers the
+// probes: blocks [0, nb_box) work on the box (one cell per thread over the
+// flattened box, three components), the rest on the probes.  Box cells fit
+// 32 bits (checked on the host).
 struct RecordArgs {
   int x0, y0, z0, bnx, bny, bnz;
   unsigned bc;        // box cells
   int nb_box;         // box blocks
+  int lossy;          // record E^n after d
   float* box_out;
   int nprobe;
   const int* comp;
@@ -557,21 +565,40 @@ struct RecordArgs {
 
 constexpr int kRecThreads = 256;
 
-__global__ __launch_bounds__(kRecThreads) void record_kernel(GridArgs g, RecordArgs r) {
+__device__ __forceinline__ long long record_cell(const GridArgs& g, const RecordArgs& r,
+                                                 unsigned l) {
+  const unsigned li = l % (unsigned)r.bnx;
+  const unsigned row = l / (unsigned)r.bnx;
+  const unsigned lj = row % (unsigned)r.bny;
+  const unsigned lk = row / (unsigned)r.bny;
+  return ((long long)(r.z0 + (int)lk) * g.ny + (r.y0 + (int)lj)) * g.nx + (r.x0 + (int)li);
+}
+
+__global__ __launch_bounds__(kRecThreads) void record_pre_kernel(GridArgs g, RecordArgs r) {
+  const unsigned l = (unsigned)blockIdx.x * kRecThreads + threadIdx.x;
+  if (l >= r.bc) return;
+  const long long c = record_cell(g, r, l);
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const float e = g.f[q][c];
+    r.box_out[(unsigned long long)q * r.bc + l] = e;
+    if (r.lossy) r.box_out[(unsigned long long)(3 + q) * r.bc + l] = e;
+  }
+}
+
+__global__ __launch_bounds__(kRecThreads) void record_post_kernel(GridArgs g, RecordArgs r) {
   const int b = blockIdx.x;
   if (b < r.nb_box) {
     const unsigned l = (unsigned)b * kRecThreads + threadIdx.x;
     if (l >= r.bc) return;
-    const unsigned li = l % (unsigned)r.bnx;
-    const unsigned row = l / (unsigned)r.bnx;
-    const unsigned lj = row % (unsigned)r.bny;
-    const unsigned lk = row / (unsigned)r.bny;
-    const long long c =
-        ((long long)(r.z0 + (int)lk) * g.ny + (r.y0 + (int)lj)) * g.nx + (r.x0 + (int)li);
-    const float e0 = g.f[0][c], e1 = g.f[1][c], e2 = g.f[2][c];
-    r.box_out[l] = e0;
-    r.box_out[r.bc + l] = e1;
-    r.box_out[2ull * r.bc + l] = e2;
+    const long long c = record_cell(g, r, l);
+    float ca = 1.f;
+    if (r.lossy) ca = g.mat ? g.pal[256 + g.mat[c]] : g.ca[c];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      float* o = r.box_out + (unsigned long long)q * r.bc + l;
+      *o = g.f[q][c] - (r.lossy ? ca * *o : *o);
+    }
     return;
   }
   const int q = (b - r.nb_box) * kRecThreads + threadIdx.x;
@@ -603,16 +630,16 @@ int adjoint_args(const YeePlan* p, float* const* lam, float* lam_psi, const YeeD
     if (box->x0 < 0 || box->y0 < 0 || box->z0 < 0 || box->ny <= 0 || box->nz <= 0 ||
         box->x0 + box->nx > p->d.nx || box->y0 + box->ny > p->d.ny || box->z0 + box->nz > p->d.nz)
       return fail(YEE_EINVAL, "design box outside the grid");
-    if (!box->e_next || !box->e_cur || !box->acc_cb)
-      return fail(YEE_EINVAL, "design box needs e_next, e_cur and acc_cb");
+    if (!box->d || !box->acc_cb || (d.ca && box->acc_ca && !box->e_cur))
+      return fail(YEE_EINVAL, "design box needs d, acc_cb (and e_cur with acc_ca)");
     a.bx0 = box->x0;
     a.by0 = box->y0;
     a.bz0 = box->z0;
     a.bnx = box->nx;
     a.bny = box->ny;
     a.bnz = box->nz;
-    a.e_next = box->e_next;
-    a.e_cur = box->e_cur;
+    a.hist_d = box->d;
+    a.hist_e = box->e_cur;
     a.acc_cb = box->acc_cb;
     a.acc_ca = box->acc_ca;
   }
@@ -776,9 +803,11 @@ int yee_run_record(YeePlan* p, float* const* fields, float* psi, int32_t n0, int
   const GridArgs g = make_args(p, fields, psi);
   hipStream_t st = (hipStream_t)stream;
   const long long bc3 = box ? 3LL * box[3] * box[4] * box[5] : 0;
-  if (bc3 >= (1LL << 32)) return fail(YEE_EINVAL, "design box too large to record");
+  const bool lossy = p->d.ca != nullptr;
+  const long long slot = lossy ? 2 * bc3 : bc3;  // floats of one step's box record
+  if (slot >= (1LL << 32)) return fail(YEE_EINVAL, "design box too large to record");
   // inside the E kernel when no source is injected after it and the probes
-  // (if any) come with a map; else one record launch after each step
+  // (if any) come with a map; else record launches before and after the step
   const bool in_kernel = (box || nprobe > 0) && p->nsrc <= YEE_FUSED_SOURCES &&
                          (nprobe == 0 || mapped);
   RecArgs e;
@@ -794,6 +823,7 @@ int yee_run_record(YeePlan* p, float* const* fields, float* psi, int32_t n0, int
     e.bnz = r.bnz = box[5];
     e.bc = r.bc = (unsigned)(bc3 / 3);
     r.nb_box = (int)((r.bc + kRecThreads - 1) / kRecThreads);
+    r.lossy = lossy ? 1 : 0;
   }
   if (mapped) {
     e.pmap = probe_map;
@@ -805,10 +835,15 @@ int yee_run_record(YeePlan* p, float* const* fields, float* psi, int32_t n0, int
   r.idx = (const long long*)probe_idx;
   const int nblocks = r.nb_box + (nprobe + kRecThreads - 1) / kRecThreads;
   for (int s = 0; s < nsteps; ++s) {
+    float* bo = box ? box_out + s * slot : nullptr;
+    float* po = nprobe > 0 ? probe_out + (long long)s * nprobe : nullptr;
+    if (!in_kernel && box) {
+      r.box_out = bo;
+      record_pre_kernel<<<r.nb_box, kRecThreads, 0, st>>>(g, r);
+      YEE_HIP_CHECK(hipGetLastError());
+    }
     rc = launch_h(p, g, st);
     if (rc) return rc;
-    float* bo = box ? box_out + s * bc3 : nullptr;
-    float* po = nprobe > 0 ? probe_out + (long long)s * nprobe : nullptr;
     if (in_kernel) {
       e.box_out = bo;
       e.probe_out = po;
@@ -821,7 +856,7 @@ int yee_run_record(YeePlan* p, float* const* fields, float* psi, int32_t n0, int
     if (nblocks > 0) {
       r.box_out = bo;
       r.probe_out = po;
-      record_kernel<<<nblocks, kRecThreads, 0, st>>>(g, r);
+      record_post_kernel<<<nblocks, kRecThreads, 0, st>>>(g, r);
       YEE_HIP_CHECK(hipGetLastError());
     }
   }
@@ -856,14 +891,16 @@ int yee_adjoint_run(YeePlan* p, float* const* lam, float* lam_psi, int32_t n_lo,
     b.acc_ca = acc_ca;
     bc3 = 3LL * box[3] * box[4] * box[5];
   }
+  // one step's box record: d (and E^n for lossy plans), as yee_run_record
+  const long long slot = p->d.ca ? 2 * bc3 : bc3;
   hipStream_t st = (hipStream_t)stream;
   GridArgs gl;
   std::memset(&gl, 0, sizeof(gl));
   for (int c = 0; c < 6; ++c) gl.f[c] = lam[c];
   for (int n = n_hi - 1; n >= n_lo; --n) {
     if (box) {
-      b.e_next = box_hist + (long long)(n - n_lo + 1) * bc3;
-      b.e_cur = box_hist + (long long)(n - n_lo) * bc3;
+      b.d = box_hist + (long long)(n - n_lo) * slot;
+      b.e_cur = p->d.ca ? b.d + bc3 : nullptr;
     }
     AdjArgs a;
     rc = adjoint_args(p, lam, lam_psi, box ? &b : nullptr, a);

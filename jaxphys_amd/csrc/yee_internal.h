@@ -73,9 +73,11 @@ struct SrcArgs {
   float val[YEE_FUSED_SOURCES];
 };
 
-// What an E launch records of the E it writes (the forward run of the
-// adjoint): E on a box (comp-major, 3 * bc floats, x fastest) and the
-// values of probes given by a cell -> probe map over their bounding box,
+// What an E launch records of the step it computes (the forward run of the
+// adjoint): on a box, d = E^{n+1} - Ca E^n (comp-major, 3 * bc floats, x
+// fastest; the difference the eps gradient needs), followed, for lossy
+// plans, by E^n (3 * bc more); and the values of probes given by a cell ->
+// probe map over their bounding box,
 // pmap[((c * pnz + k) * pny + j) * pnx + i] = probe of E component c or -1.
 struct RecArgs {
   float* box_out;  // null: no box
