@@ -475,7 +475,11 @@ def closure_bench(args, dev):
                                        for k in ("err", "err_far", "err_traj", "err_traj_far")},
         "median_err_far": {k: round(float(np.median([r.get(f"err_far_{k}", float("nan"))
                                                      for r in res])), 5)
-                           for k in ("coarse", "source_only", "far_field", "closure")},
+                           for k in ("coarse", "source_only", "far_field", "closure",
+                                     "net_gain1")},
+        "with_net_gain1": {k: round(float(np.mean([r.get(f"{k}_net_gain1", float("nan"))
+                                                   for r in res])), 5)
+                           for k in ("err", "err_far", "err_traj", "err_traj_far")},
         "err_far_coarse_no_closure": round(float(np.mean([r["err_far_coarse"] for r in res])), 5),
         "err_traj_far_closure": round(float(np.mean([r["err_traj_far_closure"] for r in res])), 5),
         "err_traj_far_coarse_no_closure": round(float(np.mean([r["err_traj_far_coarse"]
