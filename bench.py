@@ -439,6 +439,7 @@ def closure_bench(args, dev):
                                  seed=args.closure_seed,
                                  far_field=not args.closure_no_far_field,
                                  val_seeds=list(range(900, 900 + args.closure_val)),
+                                 warm_start=args.closure_warm_start,
                                  log=lambda m: print("[closure]", m, file=sys.stderr, flush=True),
                                  **pkw)
     t_train = time.time() - t0
@@ -462,6 +463,7 @@ def closure_bench(args, dev):
                    "train_seed": args.closure_seed,
                    "far_field": not args.closure_no_far_field, "far_gain": net.far_gain,
                    "val_seeds": args.closure_val, "net_gain": net.net_gain,
+                   "warm_start": args.closure_warm_start,
                    "onpolicy_from_epoch": args.closure_onpolicy_from,
                    "lr": [args.closure_lr, args.closure_lr_final]},
         "err_coarse_no_closure": round(float(np.mean(err_0)), 5),
@@ -529,6 +531,8 @@ def main() -> None:
                          "(-1: teacher-forced only)")
     ap.add_argument("--closure-seed", type=int, default=0,
                     help="seed of the net's initial weights and of the crop sampling")
+    ap.add_argument("--closure-warm-start", action="store_true",
+                    help="teacher-forced samples start one corrected interval earlier")
     ap.add_argument("--closure-val", type=int, default=2,
                     help="validation seeds (900, 901, ...) that choose the net's gain")
     ap.add_argument("--closure-no-far-field", action="store_true",

@@ -684,7 +684,9 @@ def _far_energy_ratio(net: ClosureNet, prob: ClosureProblem, every: int, n_steps
 
 def fit_far_field(net: ClosureNet, device, seeds, n_fine: int, n_coarse: int, every: int,
                   log=None, seed: int = 0, **problem_kw) -> None:
-    """Least-squares fit of the linear far-field operator (float32 27-tap
+    """Least-squares fits of the two linear parts on the same teacher-forced
+    intervals: the source model (when the net has the forcing channel) and
+    the linear far-field operator (float32 27-tap
     stencil of the six normalised fields and of the same times eps - 1) to
     the teacher-forced one-interval residuals, restricted fine state minus
     the coarse state ``every`` steps after starting from the restricted fine
@@ -697,6 +699,8 @@ def fit_far_field(net: ClosureNet, device, seeds, n_fine: int, n_coarse: int, ev
     rng = np.random.default_rng(seed + 1)
     G = torch.zeros(FAR_FEAT, FAR_FEAT, dtype=torch.float64, device=net.device)
     B = torch.zeros(FAR_FEAT, COUT, dtype=torch.float64, device=net.device)
+    ls = {"G": torch.zeros(NFEAT, NFEAT, dtype=torch.float64, device=net.device),
+          "B": torch.zeros(NFEAT, COUT, STENCIL ** 3, dtype=torch.float64, device=net.device)}
     t0 = time.time()
     for sd in seeds:
         prob = ClosureProblem(sd, n_fine, device=device, **problem_kw)
@@ -704,6 +708,7 @@ def fit_far_field(net: ClosureNet, device, seeds, n_fine: int, n_coarse: int, ev
         csol = prob.coarse_grid().build_solver(prob.dt_coarse(), n_coarse + every + 1)
         eps_m1 = csol.eps_r.double() - 1.0
         cand = _far_cells(prob, net)
+        nc = prob.nc
         for q in range(len(states) - 1):
             csol.reset()
             for f, v in zip(csol.fields, states[q]):
@@ -717,7 +722,16 @@ def fit_far_field(net: ClosureNet, device, seeds, n_fine: int, n_coarse: int, ev
                              .reshape(-1)[sel] for c in range(COUT)], 1)
             G += X.T @ X
             B += X.T @ Y
+            if net.forcing:
+                # the source model, on the same teacher-forced interval
+                box, slot, which = net._box_terms(csol, (nc,) * 3, (0, 0, 0), every)
+                tgt = torch.stack([(states[q + 1][c] - cur[c]) / net.scale[c]
+                                   for c in range(COUT)], dim=-1)
+                _accumulate_source_fit(net, csol, every, q, tgt, box, slot, which, (0, 0, 0),
+                                       nc, 0, ls)
         del states
+    if net.forcing:
+        _solve_source_fit(net, ls)
     lam = FAR_RIDGE * float(G.trace()) / FAR_FEAT
     W = torch.linalg.solve(G + lam * torch.eye(FAR_FEAT, dtype=G.dtype, device=G.device), B)
     # (plain/eps, tap, field, out) -> the kernel's (tap, field, plain/eps, out)
@@ -777,7 +791,8 @@ def train_closure(device, seeds, n_fine: int = 256, n_coarse: int = 400, every: 
                   log=None, seed: int = 0, onpolicy_from: int | None = None,
                   lr_final: float | None = None, source_crops: float = 0.5,
                   forcing: bool = True, reset_tol: float = 1.0, far_field: bool = True,
-                  val_seeds=None, onpolicy_with_net: bool = False, **problem_kw):
+                  val_seeds=None, onpolicy_with_net: bool = False, warm_start: bool = False,
+                  **problem_kw):
     """Train on fine-run/coarse pairs generated by the FDTD solver.
 
     A sample is a coarse state at coarse step m + every and its target, the
@@ -789,7 +804,11 @@ def train_closure(device, seeds, n_fine: int = 256, n_coarse: int = 400, every: 
     corrected by the source model and the far-field operator only (the net
     then learns the error that accumulates over many intervals of that
     stable run), or with ``onpolicy_with_net`` by the whole closure.  The
-    source model is fitted on teacher-forced samples only.  Loss: mean squared normalised error on an interior
+    source model is fitted on teacher-forced samples only.  With
+    ``warm_start`` a teacher-forced sample starts one interval earlier and
+    applies the linear parts (source model, far-field operator) after that
+    interval, so the transient of a restart from a restricted fine state
+    has passed when the sample's interval runs.  Loss: mean squared normalised error on an interior
     crop (a 3-cell border, where zero padding differs from the full domain,
     is excluded); a share ``source_crops`` of the crops holds the source.
     Adam, learning rate cosine-annealed from ``lr`` to ``lr_final``.
@@ -852,8 +871,16 @@ def train_closure(device, seeds, n_fine: int = 256, n_coarse: int = 400, every: 
                 m = int(q) * every
                 if inputs is None:
                     csol.reset()
-                    for f, v in zip(csol.fields, states[q]):
-                        f.copy_(v)
+                    if warm_start and q >= 1:
+                        # one interval earlier, corrected by the linear parts,
+                        # so the restart transient has passed before the sample
+                        for f, v in zip(csol.fields, states[q - 1]):
+                            f.copy_(v)
+                        csol.run(m - every, every)
+                        net.correct(csol, 1.0, m - every, every, use_net=False)
+                    else:
+                        for f, v in zip(csol.fields, states[q]):
+                            f.copy_(v)
                     csol.run(m, every)
                     cur = csol.fields
                 else:
@@ -891,7 +918,7 @@ def train_closure(device, seeds, n_fine: int = 256, n_coarse: int = 400, every: 
                     box, slot, which = net._box_terms(csol, (crop,) * 3, (cx, cy, cz), every)
                     w.view(-1)[box] = 0.0
                     y.view(-1, OUT_STRIDE)[box] = 0.0
-                    if not onpolicy:
+                    if not onpolicy and not far_field:
                         _accumulate_source_fit(net, csol, every, q, tc, box, slot, which,
                                                (cx, cy, cz), crop, b, ls)
                 err = torch.zeros_like(y)
@@ -911,7 +938,7 @@ def train_closure(device, seeds, n_fine: int = 256, n_coarse: int = 400, every: 
                     grads[k_] = grads[k_] - grads0[k_]
                 net.adam_step(grads, lr=lr_now)
                 history.append(loss)
-            if net.forcing and s_i == len(seeds) - 1:
+            if net.forcing and not far_field and s_i == len(seeds) - 1:
                 _solve_source_fit(net, ls)
             if log:
                 log(f"epoch {ep}{' on-policy' if onpolicy else ''} seed {sd}: mean loss "
