@@ -715,13 +715,14 @@ def fit_far_field(net: ClosureNet, device, seeds, n_fine: int, n_coarse: int, ev
                 f.copy_(v)
             csol.run(q * every, every)
             cur = csol.fields
-            pick = torch.from_numpy(rng.integers(0, cand.numel(), FAR_SAMPLES)).to(net.device)
-            sel = cand[pick]
-            X = _far_features(net, cur, eps_m1, sel)
-            Y = torch.stack([((states[q + 1][c] - cur[c]).double() / float(net.scale[c]))
-                             .reshape(-1)[sel] for c in range(COUT)], 1)
-            G += X.T @ X
-            B += X.T @ Y
+            pick = torch.from_numpy(rng.integers(0, max(1, cand.numel()), FAR_SAMPLES))
+            sel = cand[pick.to(net.device)] if cand.numel() > 0 else cand
+            if cand.numel() > 0:
+                X = _far_features(net, cur, eps_m1, sel)
+                Y = torch.stack([((states[q + 1][c] - cur[c]).double() / float(net.scale[c]))
+                                 .reshape(-1)[sel] for c in range(COUT)], 1)
+                G += X.T @ X
+                B += X.T @ Y
             if net.forcing:
                 # the source model, on the same teacher-forced interval
                 box, slot, which = net._box_terms(csol, (nc,) * 3, (0, 0, 0), every)
@@ -732,6 +733,13 @@ def fit_far_field(net: ClosureNet, device, seeds, n_fine: int, n_coarse: int, ev
         del states
     if net.forcing:
         _solve_source_fit(net, ls)
+    if float(G.trace()) <= 0:
+        # no cell outside the correction margin and the source boxes
+        net.far_w.zero_()
+        net.far_gain = 0.0
+        if log:
+            log("far-field operator: no cells to fit on, left off")
+        return
     lam = FAR_RIDGE * float(G.trace()) / FAR_FEAT
     W = torch.linalg.solve(G + lam * torch.eye(FAR_FEAT, dtype=G.dtype, device=G.device), B)
     # (plain/eps, tap, field, out) -> the kernel's (tap, field, plain/eps, out)
