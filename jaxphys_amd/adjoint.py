@@ -39,6 +39,48 @@ from jaxphys_amd.exceptions import ConfigurationError
 _COMP = {"ex": 0, "ey": 1, "ez": 2, "hx": 3, "hy": 4, "hz": 5}
 
 
+def probe_terms(probes, weights, size):
+    """Objective terms of ``probes`` (``[(component, (i, j, k)), ...]``, E
+    components only) with ``weights`` (None: all 1) on a grid of ``size``
+    (nx, ny, nz): probes at the same (component, cell) merge into one term
+    with the summed weight (the adjoint adds one term per component and
+    cell).  Returns (keys [(c, i, j, k)], weights, probe_box, probe_map):
+    the box (x0, y0, z0, nx, ny, nz) bounding the probe cells and the map
+    over it, int32 (3, nz, ny, nx) holding each term's index at its
+    component and cell and -1 elsewhere, when the box holds at most
+    max(2^20, 8 * terms) cells; else (.., None, None)."""
+    nx, ny, nz = size
+    probes = list(probes)
+    w_in = np.ones(len(probes)) if weights is None else np.asarray(weights, np.float64)
+    if w_in.shape != (len(probes),):
+        raise ConfigurationError("one weight per probe")
+    merged: dict[tuple[int, int, int, int], float] = {}
+    for (comp, (i, j, k)), wq in zip(probes, w_in):
+        c = _COMP[comp] if isinstance(comp, str) else int(comp)
+        if c not in (0, 1, 2):
+            # the psi_H adjoint update is deferred by one step and reads
+            # lamH unmodified; objective terms are therefore E-only
+            raise ConfigurationError("adjoint objectives are sums over E components")
+        if not (0 <= i < nx and 0 <= j < ny and 0 <= k < nz):
+            raise ConfigurationError(f"probe {(i, j, k)} outside the grid")
+        key = (c, int(i), int(j), int(k))
+        merged[key] = merged.get(key, 0.0) + float(wq)
+    keys = list(merged)
+    wts = [merged[key] for key in keys]
+    if not keys:
+        return keys, wts, None, None
+    pi = np.array([[i, j, k] for _, i, j, k in keys])
+    lo, hi = pi.min(0), pi.max(0) + 1
+    ext = hi - lo
+    if int(np.prod(ext)) > max(1 << 20, 8 * len(keys)):
+        return keys, wts, None, None
+    pmap = np.full((3, ext[2], ext[1], ext[0]), -1, dtype=np.int32)
+    for q, (c, i, j, k) in enumerate(keys):
+        pmap[c, k - lo[2], j - lo[1], i - lo[0]] = q
+    box = (int(lo[0]), int(lo[1]), int(lo[2]), int(ext[0]), int(ext[1]), int(ext[2]))
+    return keys, wts, box, pmap
+
+
 class AdjointProblem:
     """Objective and gradient w.r.t. eps_r on a design box of an EMGrid3D.
 
@@ -93,44 +135,13 @@ class AdjointProblem:
         self.region = (slice(z0, z0 + bz), slice(y0, y0 + by), slice(x0, x0 + bx))
         self.box_int = (x0, y0, z0, bx, by, bz)
         self.bcells = bx * by * bz
-        probes = list(probes)
-        w_in = np.ones(len(probes)) if weights is None else np.asarray(weights, np.float64)
-        if w_in.shape != (len(probes),):
-            raise ConfigurationError("one weight per probe")
-        # probes at the same (component, cell) are one term with the summed
-        # weight: the adjoint adds one objective term per (component, cell)
-        merged: dict[tuple[int, int, int, int], float] = {}
-        for (comp, (i, j, k)), wq in zip(probes, w_in):
-            c = _COMP[comp] if isinstance(comp, str) else int(comp)
-            if c not in (0, 1, 2):
-                # the psi_H adjoint update is deferred by one step and reads
-                # lamH unmodified; objective terms are therefore E-only
-                raise ConfigurationError("adjoint objectives are sums over E components")
-            if not (0 <= i < nx and 0 <= j < ny and 0 <= k < nz):
-                raise ConfigurationError(f"probe {(i, j, k)} outside the grid")
-            key = (c, int(i), int(j), int(k))
-            merged[key] = merged.get(key, 0.0) + float(wq)
-        keys = list(merged)
-        comps = [c for c, _, _, _ in keys]
-        idx = [(k * ny + j) * nx + i for _, i, j, k in keys]
+        keys, wts, self.probe_box, pmap = probe_terms(probes, weights, (nx, ny, nz))
         self.nprobe = len(keys)
-        self.probe_comp = torch.tensor(comps, dtype=torch.int32, device=dev)
-        self.probe_idx = torch.tensor(idx, dtype=torch.int64, device=dev)
-        self.w = torch.tensor([merged[key] for key in keys], dtype=torch.float32, device=dev)
-        # cell -> probe map over the probes' bounding box (when it is small):
-        # the E-adjoint kernel then adds the objective terms itself
-        self.probe_box, self.probe_map = None, None
-        if keys:
-            pi = np.array([[i, j, k] for _, i, j, k in keys])
-            lo, hi = pi.min(0), pi.max(0) + 1
-            ext = hi - lo
-            if int(np.prod(ext)) <= max(1 << 20, 8 * self.nprobe):
-                pmap = np.full((3, ext[2], ext[1], ext[0]), -1, dtype=np.int32)
-                for q, (c, i, j, k) in enumerate(keys):
-                    pmap[c, k - lo[2], j - lo[1], i - lo[0]] = q
-                self.probe_box = (int(lo[0]), int(lo[1]), int(lo[2]),
-                                  int(ext[0]), int(ext[1]), int(ext[2]))
-                self.probe_map = torch.from_numpy(pmap.reshape(-1)).to(dev)
+        self.probe_comp = torch.tensor([c for c, _, _, _ in keys], dtype=torch.int32, device=dev)
+        self.probe_idx = torch.tensor([(k * ny + j) * nx + i for _, i, j, k in keys],
+                                      dtype=torch.int64, device=dev)
+        self.w = torch.tensor(wts, dtype=torch.float32, device=dev)
+        self.probe_map = None if pmap is None else torch.from_numpy(pmap.reshape(-1)).to(dev)
         self.background = sol.eps_r.clone()
         self.probe_hist = torch.zeros((self.n_steps, self.nprobe), dtype=torch.float32, device=dev)
         self.checkpoints = []
