@@ -662,6 +662,13 @@ def _far_features(net: ClosureNet, fields, eps_m1: torch.Tensor,
     return torch.cat([x1, x1 * eps_m1.reshape(-1)[sel][:, None]], 1)
 
 
+def far_kernel_weights(W: torch.Tensor) -> torch.Tensor:
+    """Least-squares solution (FAR_FEAT, 6) in the feature order of
+    ``_far_features`` (plain/eps, tap, field) -> the flat float32 weights
+    of ``conv3d_stencil_f32`` (tap, field, plain/eps, out)."""
+    return W.reshape(2, FAR_TAPS, 6, COUT).permute(1, 2, 0, 3).reshape(-1).to(torch.float32)
+
+
 def _far_energy_ratio(net: ClosureNet, prob: ClosureProblem, every: int, n_steps: int) -> float:
     """E energy after ``n_steps`` coarse steps of ``prob`` with the far-field
     operator alone applied every ``every`` steps, over the same run without
@@ -742,8 +749,7 @@ def fit_far_field(net: ClosureNet, device, seeds, n_fine: int, n_coarse: int, ev
         return
     lam = FAR_RIDGE * float(G.trace()) / FAR_FEAT
     W = torch.linalg.solve(G + lam * torch.eye(FAR_FEAT, dtype=G.dtype, device=G.device), B)
-    # (plain/eps, tap, field, out) -> the kernel's (tap, field, plain/eps, out)
-    wk = W.reshape(2, FAR_TAPS, 6, COUT).permute(1, 2, 0, 3).reshape(-1).to(torch.float32)
+    wk = far_kernel_weights(W)
     check = ClosureProblem(seeds[-1], n_fine, device=device, **problem_kw)
     gain = FAR_GAIN
     for _ in range(4):

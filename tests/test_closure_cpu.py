@@ -69,3 +69,23 @@ def test_far_field_stencil_oracle_pins():
     expect *= 2.0 + 0.5 * (eps - 1.0)
     np.testing.assert_allclose(y[..., 4], expect, rtol=1e-12, atol=1e-12)
     assert np.abs(np.delete(y, 4, axis=-1)).max() == 0.0
+
+
+def test_far_field_features_and_kernel_weight_layout_agree_with_the_oracle():
+    """Features times least-squares weights (the fit's view) equal the
+    stencil oracle applied with the weights converted to the kernel layout
+    (the kernel's view): a swapped tap, field or plain/eps index fails."""
+    from jaxphys_amd import closure as CL
+    from oracle import closure_ops as O
+
+    net = CL.ClosureNet("cpu", seed=0)
+    net.scale = torch.tensor([0.5, 1.0, 2.0, 1.5, 0.7, 1.2])
+    rng = np.random.default_rng(0)
+    nz, ny, nx = 5, 6, 7
+    fields = [torch.from_numpy(rng.standard_normal((nz, ny, nx))) for _ in range(6)]
+    eps = torch.from_numpy(1.0 + 3.0 * rng.random((nz, ny, nx)))
+    X = CL._far_features(net, fields, eps - 1.0, torch.arange(nz * ny * nx))
+    W = torch.from_numpy(rng.standard_normal((CL.FAR_FEAT, 6)))
+    wk = CL.far_kernel_weights(W).double().reshape(27, 6, 2, 6).numpy()
+    ref = O.stencil([f.numpy() for f in fields], eps.numpy(), net.scale.numpy(), wk)
+    np.testing.assert_allclose((X @ W).numpy(), ref.reshape(-1, 6), rtol=1e-6, atol=1e-6)
