@@ -1,7 +1,14 @@
-"""Learned dispersion closure: a coarse (2x) Yee grid plus a small 3D conv net
-that reproduces fine-grid fields.  No reference counterpart (the reference
-has no neural-network code); the FDTD solver itself generates the training
-data and stays the fallback and the judge of the error.
+"""Learned dispersion closure: a coarse (2x) Yee grid plus a learned
+correction every few steps that reproduces fine-grid fields.  No reference
+counterpart (the reference has no neural-network code); the FDTD solver
+itself generates the training data and stays the fallback and the judge of
+the error.  The correction has three parts, all fitted to the solver's own
+fine runs: a linear source model in a box around each source, a float32
+linear far-field operator (a 27-tap stencil of the fields, fitted by
+least squares, which corrects the coarse grid's dispersion away from the
+source; ``fit_far_field``), and a BF16 3D conv net trained on what those
+two leave, whose gain validation seeds choose (``select_net_gain``; zero
+when it does not lower their error by more than 1 %).
 
 Network (``ClosureNet``): channels 7 -> 32 -> 32 -> 6 (E, H normalised and
 eps_r in; a field correction out), 3x3x3 zero-padded convolutions, GELU
@@ -20,8 +27,9 @@ interval's injected E increment, f1 its first time moment and eps_s the
 permittivity at the source cell (the sources add to E itself, so in a
 uniform medium the static field they leave per increment does not depend
 on eps; the third term lets the fit learn how a dielectric at the source
-changes it); K0..K2 (``ks``) are fitted by least squares on the training
-samples (a linear regression, solved exactly after every epoch).  The net
+changes it); K0..K2 (``ks``) are fitted by least squares on the
+teacher-forced training intervals (a linear regression, solved exactly,
+before the net is trained).  The net
 is trained on, and corrects, the cells outside those boxes.  The field
 channels enter as asinh(field / RMS), so the near-source values (thousands
 of RMS) stay in the range of the rest.  The net's response to the zero
