@@ -176,6 +176,50 @@ def test_run_record_equals_steps_and_gathers(gpu, nsrc):
         assert torch.equal(out_probe, ref_probe), mapped
 
 
+def test_adjoint_step_equals_one_step_of_adjoint_run(gpu):
+    """yee_adjoint_step with a YeeDesignBox (d and E^n of the step's record,
+    a lossy plan) equals yee_adjoint_run over that single step, bit for bit,
+    on a random adjoint state (fields, psi adjoints and both accumulators)."""
+    from jaxphys_amd import _native
+
+    size, npml, n = (22, 20, 18), 4, 20
+    eps, sigma, box, tab, src, mon = _setup("cpml", True, size, npml, n)
+    x0, y0, z0, bx, by, bz = box
+    g = _gpu_grid(size, "cpml", npml, eps, sigma, tab, src)
+    prob = AdjointProblem(g, n, DT, box, [("ez", p) for _, p in mon], checkpoint_every=n,
+                          record_box=True)
+    prob.forward(torch.tensor(eps[x0:x0 + bx, y0:y0 + by, z0:z0 + bz], device=gpu))
+    sol = prob.solver
+    gen = torch.Generator().manual_seed(0)
+    lam0 = [torch.randn(tuple(f.shape), generator=gen).to(gpu) for f in sol.fields]
+    lpsi0 = torch.randn(tuple(sol.psi.shape), generator=gen).to(gpu)
+    m = n - 1
+    rec = prob.box_hist[m]
+    out = []
+    for mode in ("run", "step"):
+        lam = [t.clone() for t in lam0]
+        lpsi = lpsi0.clone()
+        acc_cb = torch.zeros(prob.bcells, device=gpu)
+        acc_ca = torch.zeros(prob.bcells, device=gpu)
+        if mode == "run":
+            _native.adjoint_run(sol.plan, lam, lpsi, m, m + 1, None, None, None, None, 0.0,
+                                box=prob.box_int, box_hist=prob.box_hist[m:], acc_cb=acc_cb,
+                                acc_ca=acc_ca)
+        else:
+            b = _native.YeeDesignBox()
+            b.x0, b.y0, b.z0, b.nx, b.ny, b.nz = prob.box_int
+            b.d = rec.data_ptr()
+            b.e_cur = rec[3 * prob.bcells:].data_ptr()
+            b.acc_cb = acc_cb.data_ptr()
+            b.acc_ca = acc_ca.data_ptr()
+            _native.adjoint_step(sol.plan, lam, lpsi, box=b)
+        torch.cuda.synchronize()
+        out.append(lam + [lpsi, acc_cb, acc_ca])
+    assert float(out[0][-2].abs().max()) > 0
+    for a, b_ in zip(*out):
+        assert torch.equal(a, b_)
+
+
 def test_autograd_function_and_finite_differences(gpu):
     size, npml, n = (22, 20, 18), 4, 60
     eps, sigma, box, tab, src, mon = _setup("cpml", False, size, npml, n)
