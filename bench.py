@@ -553,7 +553,7 @@ def main() -> None:
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
-              "(the z-slab decomposition is described in DESIGN.md)", file=sys.stderr)
+              "(a z-slab decomposition with one halo exchange per half-step is the design for more)", file=sys.stderr)
         raise SystemExit(2)
 
     import numpy as np
