@@ -552,7 +552,7 @@ def main() -> None:
                     help="run only a plain copy of the field bytes (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
+        print("bench.py measures one GPU; multi-GPU runs are not implemented "
               "(a z-slab decomposition with one halo exchange per half-step is the design for more)", file=sys.stderr)
         raise SystemExit(2)
 
