@@ -323,8 +323,10 @@ __device__ __forceinline__ bool tile_of(const Tiling& tl, int& bx, int& by, int&
 struct YeePlan {
   YeeGridDesc d;
   int kz;            // z planes per block of the two-pass kernels
-  int adj_kz = 2;    // z planes per block of the adjoint kernels (latency
-                     // bound: short marches, many wavefronts, measured best)
+  int adj_kz = 3;    // z planes per block of the adjoint kernels (latency
+                     // bound: short marches, many wavefronts; on the 160^3
+                     // design workload 1-6 planes give 3.12, 2.88, 2.80,
+                     // 3.02, 3.13, 3.03x one forward run)
   int fused_kz = 0;  // z planes per block of the fused kernel (0: automatic)
   int fused_variant = 3;  // 0: LDS + halo row, 1-3: barrier-free (see yee_fused.hip)
   int fused_rows = 4;     // y rows (wavefronts) per tile of variants 2 and 3: 4, 8 or 16
