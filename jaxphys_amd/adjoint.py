@@ -92,7 +92,10 @@ class AdjointProblem:
             index order.  Must lie outside absorbing slabs, at i, j, k >= 1,
             and contain no source.
         probes: ``[(component, (i, j, k)), ...]`` with component in
-            ex..hz, and optional ``weights``.
+            ex, ey, ez, and optional ``weights``.  H components are
+            rejected: the adjoint of the H-part psi update is applied one
+            step late, so an objective term cannot be added to the H
+            adjoint at the step it belongs to.
         checkpoint_every: forward checkpoint interval K (when the box
             history is not recorded).
         record_box: record E on the design box for every step during the
@@ -154,6 +157,10 @@ class AdjointProblem:
             record_box = need <= free // 4
         self.record_box = bool(record_box)
         self._recorded = False
+        # number of forward() calls so far: backward() uses what the latest
+        # forward left (probe history, checkpoints or box history, eps), so
+        # a caller holding an older forward can check it is still current
+        self.forward_count = 0
 
     # ------------------------------------------------------------------
     def set_design(self, eps_box: torch.Tensor) -> None:
@@ -168,6 +175,7 @@ class AdjointProblem:
     def forward(self, eps_box: torch.Tensor | None = None) -> float:
         """Forward run with checkpoints; returns J (float64 on the host)."""
         sol = self.solver
+        self.forward_count += 1
         if eps_box is not None:
             self.set_design(eps_box)
         sol.reset()
@@ -252,17 +260,28 @@ class AdjointProblem:
 
 class FDTDObjective(torch.autograd.Function):
     """``J = FDTDObjective.apply(eps_box, problem)``; ``J.backward()`` runs
-    the checkpointed adjoint and returns dJ/d eps_box."""
+    the checkpointed adjoint and returns dJ/d eps_box.
+
+    The adjoint sweep uses the state of the problem's latest forward run, so
+    the backward of an ``apply`` must come before the next ``apply`` on the
+    same problem (one problem per objective term otherwise); a backward
+    whose forward is no longer the latest raises ``RuntimeError``."""
 
     @staticmethod
     def forward(ctx, eps_box, problem):
         J = problem.forward(eps_box)
         ctx.problem = problem
+        ctx.count = problem.forward_count
         ctx.dtype = eps_box.dtype
         ctx.device = eps_box.device
         return torch.tensor(J, dtype=torch.float64, device=eps_box.device)
 
     @staticmethod
     def backward(ctx, gJ):
+        if ctx.problem.forward_count != ctx.count:
+            raise RuntimeError(
+                "FDTDObjective.backward: the AdjointProblem ran another forward pass after "
+                "this one; its adjoint would be taken at the wrong design (use one "
+                "AdjointProblem per objective term, or call backward before the next apply)")
         g = ctx.problem.backward().to(device=ctx.device)
         return (gJ.to(g.dtype) * g).to(ctx.dtype), None

@@ -663,32 +663,60 @@ int yee_set_sources(YeePlan* p, int32_t nsrc, const int32_t* comp, const int32_t
   if (nsrc < 0 || nsteps < 0) return fail(YEE_EINVAL, "negative source or step count");
   if (nsrc > 0 && (!comp || !pos || (nsteps > 0 && !table)))
     return fail(YEE_EINVAL, "null source arrays");
-  free_sources(p);
-  p->comp.assign(nsrc, 0);
-  p->idx.assign(nsrc, 0);
+  // every source is checked before the plan changes: a rejected call leaves
+  // the previous sources in place
+  std::vector<int> new_comp(nsrc, 0);
+  std::vector<long long> new_idx(nsrc, 0);
   for (int q = 0; q < nsrc; ++q) {
     const int i = pos[3 * q], j = pos[3 * q + 1], k = pos[3 * q + 2];
     if (i < 0 || i >= p->d.nx || j < 0 || j >= p->d.ny || k < 0 || k >= p->d.nz)
       return fail(YEE_EINVAL, "source position out of grid bounds");
     if (comp[q] < YEE_EX || comp[q] > YEE_EZ)
       return fail(YEE_EINVAL, "source component must be EX, EY or EZ");
-    p->comp[q] = comp[q];
-    p->idx[q] = ((long long)k * p->d.ny + j) * p->d.nx + i;
+    new_comp[q] = comp[q];
+    new_idx[q] = ((long long)k * p->d.ny + j) * p->d.nx + i;
   }
-  p->table.assign(table ? table : (const float*)nullptr,
-                  table ? table + (size_t)nsrc * nsteps : (const float*)nullptr);
+  std::vector<float> new_table;
+  if (table) new_table.assign(table, table + (size_t)nsrc * nsteps);
+  // from here on the old device copies are gone: until the new ones are in
+  // place the plan has no sources, so a failure below leaves a plan that
+  // runs without sources rather than one that points at freed memory
+  free_sources(p);
+  p->nsrc = 0;
+  p->nsteps_table = 0;
+  p->comp.clear();
+  p->idx.clear();
+  p->table.clear();
+  if (nsrc > 0) {  // device copies: the fused step injects every source itself
+    int* dc = nullptr;
+    long long* di = nullptr;
+    float* dt = nullptr;
+    const size_t tab_floats = (size_t)nsrc * (nsteps > 0 ? nsteps : 1);
+    hipError_t e = hipMalloc(&dc, sizeof(int) * nsrc);
+    if (e == hipSuccess) e = hipMalloc(&di, sizeof(long long) * nsrc);
+    if (e == hipSuccess) e = hipMalloc(&dt, sizeof(float) * tab_floats);
+    if (e == hipSuccess)
+      e = hipMemcpy(dc, new_comp.data(), sizeof(int) * nsrc, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+      e = hipMemcpy(di, new_idx.data(), sizeof(long long) * nsrc, hipMemcpyHostToDevice);
+    if (e == hipSuccess && nsteps > 0)
+      e = hipMemcpy(dt, new_table.data(), sizeof(float) * (size_t)nsrc * nsteps,
+                    hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      if (dc) (void)hipFree(dc);
+      if (di) (void)hipFree(di);
+      if (dt) (void)hipFree(dt);
+      return fail(YEE_EHIP, std::string("source table upload: ") + hipGetErrorString(e));
+    }
+    p->d_comp = dc;
+    p->d_idx = di;
+    p->d_table = dt;
+  }
+  p->comp.swap(new_comp);
+  p->idx.swap(new_idx);
+  p->table.swap(new_table);
   p->nsrc = nsrc;
   p->nsteps_table = nsteps;
-  if (nsrc > 0) {  // device copies: the fused step injects every source itself
-    YEE_HIP_CHECK(hipMalloc(&p->d_comp, sizeof(int) * nsrc));
-    YEE_HIP_CHECK(hipMalloc(&p->d_idx, sizeof(long long) * nsrc));
-    YEE_HIP_CHECK(hipMalloc(&p->d_table, sizeof(float) * (size_t)nsrc * (nsteps > 0 ? nsteps : 1)));
-    YEE_HIP_CHECK(hipMemcpy(p->d_comp, p->comp.data(), sizeof(int) * nsrc, hipMemcpyHostToDevice));
-    YEE_HIP_CHECK(hipMemcpy(p->d_idx, p->idx.data(), sizeof(long long) * nsrc, hipMemcpyHostToDevice));
-    if (nsteps > 0)
-      YEE_HIP_CHECK(hipMemcpy(p->d_table, p->table.data(), sizeof(float) * (size_t)nsrc * nsteps,
-                              hipMemcpyHostToDevice));
-  }
   return YEE_OK;
 }
 

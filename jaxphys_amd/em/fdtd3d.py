@@ -22,6 +22,7 @@ Extensions are keyword-only and default to the reference's behaviour:
 from __future__ import annotations
 
 import logging
+import math
 from dataclasses import dataclass, field
 from typing import Any, Literal
 
@@ -31,7 +32,7 @@ import torch
 from jaxphys_amd.em import sources as _src
 from jaxphys_amd.em.constants import C0, EPS0, MU0  # noqa: F401  (re-exported)
 from jaxphys_amd.em.yee import YeeSolver
-from jaxphys_amd.exceptions import ConfigurationError
+from jaxphys_amd.exceptions import ConfigurationError, NumericalInstabilityError
 from jaxphys_amd.state import EMFieldHistory3D
 
 logger = logging.getLogger(__name__)
@@ -92,6 +93,32 @@ def _as_tensor(a, device, dtype=None) -> torch.Tensor:
 
 def _shape_of(a) -> tuple:
     return tuple(a.shape) if hasattr(a, "shape") else tuple(np.shape(a))
+
+
+# margin of stability_energy_bound over the largest energy / bound ratio
+# measured on stable runs (0.65: sponge, CPML, walls and periodic grids,
+# sine and pulse sources, Courant numbers up to 0.999 of the 3D limit)
+ENERGY_GUARD_MARGIN = 4.0
+
+
+def stability_energy_bound(table, eps_r, positions, dx: float) -> np.ndarray:
+    """Upper bound of the field energy (J) after each step of a stable run.
+
+    Each source value v added to E at a cell starts a field of energy
+    eps0 eps_r v^2 dx^3 / 2; a stable Yee scheme keeps the energy of each
+    such field (walls, periodic) or lowers it (sponge, CPML, losses), and
+    the field of the run is their sum, so sqrt(energy) after step n is at
+    most sqrt(eps0 eps_s dx^3 / 2) * sum_{m <= n} sum_s |v_s(m)| with eps_s
+    the largest permittivity at a source cell.  The naive energy the solver
+    measures (E and H at staggered times) differs from the conserved one by
+    a factor that depends on the Courant number; ``ENERGY_GUARD_MARGIN``
+    covers it.  Returns float64 bounds indexed by step."""
+    tab = np.abs(np.asarray(table, dtype=np.float64))
+    if tab.ndim != 2 or tab.shape[1] == 0:
+        return np.zeros(tab.shape[0] if tab.ndim else 0)
+    cum = np.cumsum(tab.sum(axis=1))
+    eps_s = max(float(eps_r[k, j, i]) for i, j, k in positions)
+    return ENERGY_GUARD_MARGIN * 0.5 * EPS0 * eps_s * dx ** 3 * cum ** 2
 
 
 class EMGrid3D:
@@ -243,6 +270,7 @@ class EMGrid3D:
         *,
         n_steps: int | None = None,
         energy_every: int = 0,
+        energy_guard: bool = True,
         snapshots: str = "device",
     ) -> EMFieldHistory3D:
         """Run the 3D FDTD simulation (fdtd3d.py:169-417).
@@ -254,6 +282,10 @@ class EMGrid3D:
             n_steps: extension -- explicit step count (overrides t_span).
             energy_every: extension -- record the float64 field energy every
                 N steps in ``self.energy_log`` (0 = off).
+            energy_guard: extension -- with ``energy_every > 0``, raise
+                ``NumericalInstabilityError`` when a recorded energy is not
+                finite or exceeds ``stability_energy_bound`` (what a stable
+                run driven by these sources can reach).
             snapshots: extension -- "device" keeps the saved steps in GPU
                 memory; "host" streams them into pinned host memory while
                 the run continues (a two-slot device staging ring, copies on
@@ -308,6 +340,9 @@ class EMGrid3D:
             hist = [torch.empty((n_snap, nz, ny, nx), dtype=torch.float32, device=dev)
                     for _ in range(6)]
         self.energy_log = []
+        if energy_every > 0 and energy_guard:
+            guard_bound = stability_energy_bound(
+                solver.source_table, solver.eps_r, solver.source_positions, dx)
         marks = sorted(set(saves) | ({*range(energy_every - 1, n_steps, energy_every)}
                                      if energy_every > 0 else set()))
         save_pos = {n: q for q, n in enumerate(saves)}
@@ -336,7 +371,16 @@ class EMGrid3D:
                     for c in range(6):
                         hist[c][q].copy_(solver.fields[c])
             if energy_every > 0 and (n + 1) % energy_every == 0:
-                self.energy_log.append((n + 1, solver.em_energy_joules()))
+                energy = solver.em_energy_joules()
+                self.energy_log.append((n + 1, energy))
+                if energy_guard:
+                    bound = float(guard_bound[n])
+                    if not math.isfinite(energy) or energy > bound:
+                        raise NumericalInstabilityError(
+                            f"field energy {energy:.4g} J after step {n + 1} exceeds "
+                            f"{bound:.4g} J, the most a stable run driven by these "
+                            f"sources can hold (dt = {dt:.4g} s; the 3D CFL limit is "
+                            f"{dx / (C0 * math.sqrt(3.0)):.4g} s)")
         solver.run(cur, n_steps - cur)
         self.last_solver = solver
         if host:

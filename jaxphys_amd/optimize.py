@@ -10,7 +10,9 @@ for electromagnetic inverse design, ``FDTDObjective`` wraps the checkpointed
 hand-written adjoint of the Yee step (jaxphys_amd/adjoint.py), so one
 gradient costs about three forward runs instead of the stored scan.
 
-``design_permittivity`` is the inverse-design entry: it maximises or
+``sensitivity`` mirrors the reference's Jacobian helper
+(src/neurosim/optimize.py:210-227).  ``design_permittivity`` is the
+inverse-design entry: it maximises or
 minimises the probe objective of an ``AdjointProblem`` over the permittivity
 of its design box, with optional bounds.
 """
@@ -76,9 +78,9 @@ def optimize(
     ``objective`` maps a float64 torch tensor to a scalar torch tensor and
     must be differentiable by torch autograd.  ``project`` (extension,
     keyword-only) is applied to the iterate after every update, e.g. a
-    clamp to admissible permittivities."""
-    if method not in ("gradient_descent", "adam"):
-        raise ValueError(f"unknown method {method!r}; expected 'gradient_descent' or 'adam'")
+    clamp to admissible permittivities.  As in the reference
+    (optimize.py:95-134), any ``method`` other than "adam" runs gradient
+    descent."""
     x = torch.as_tensor(initial_guess, dtype=torch.float64)
     if torch.is_tensor(initial_guess):
         x = initial_guess.detach().to(torch.float64).clone()
@@ -121,6 +123,18 @@ def optimize(
             traj.append(x.clone())
     logger.warning("Optimization did not converge after %d iterations", max_iterations)
     return result(x, max_iterations, False)
+
+
+def sensitivity(simulation_fn: Callable, params) -> torch.Tensor:
+    """Jacobian d(outputs)/d(params) of ``simulation_fn`` at ``params``
+    (optimize.py:210-227, ``jax.jacobian`` there; torch autograd here, one
+    backward pass per output).  ``params`` is taken as float64.  For an FDTD
+    objective through ``FDTDObjective``, each output must come from its own
+    ``AdjointProblem`` (the adjoint uses its problem's latest forward run)."""
+    x = torch.as_tensor(params, dtype=torch.float64)
+    if torch.is_tensor(params):
+        x = params.detach().to(torch.float64).clone()
+    return torch.autograd.functional.jacobian(simulation_fn, x)
 
 
 def design_permittivity(problem, eps_init: torch.Tensor, *, maximize: bool = True,

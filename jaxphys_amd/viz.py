@@ -2,7 +2,8 @@
 
 The reference's visualisation (src/neurosim/viz/fields.py:32-167) plots the
 2D ``EMFieldHistory`` only; this module keeps its ``plot_field_snapshot``
-for the 2D grid and adds the 3D counterparts: ``field_slice`` (a plane of
+and ``animate_field`` for the 2D grid (``animate_field`` also takes an
+``EMFieldHistory3D`` and animates one plane of it) and adds the 3D counterparts: ``field_slice`` (a plane of
 an ``EMFieldHistory3D`` snapshot), ``plot_field_slice`` and
 ``poynting_flux`` (the power through a Yee plane of a running solver,
 accumulated in float64 on the GPU by ``yee_poynting_flux_fp64``).
@@ -19,6 +20,7 @@ import torch
 from jaxphys_amd.exceptions import VisualizationError
 
 try:  # optional, as in the reference
+    import matplotlib.animation as animation
     import matplotlib.pyplot as plt
 
     HAS_MATPLOTLIB = True
@@ -112,3 +114,64 @@ def plot_field_snapshot(fields, step: int = -1, component: str = "Ez",
     fig.colorbar(im, ax=ax, label=f"{component} (a.u.)")
     fig.tight_layout()
     return fig
+
+
+def animate_field(fields, component: str = "Ez", save: str | None = None, fps: int = 30,
+                  figsize: tuple[float, float] = (8, 6), cmap: str = "RdBu_r", *,
+                  axis: str = "z", index: int | None = None) -> Any:
+    """Animate a field component over the saved snapshots
+    (viz/fields.py:94-167): one frame per snapshot, symmetric colour scale
+    fixed by the largest magnitude over all frames, title with the time.
+    A 2D ``EMFieldHistory`` takes Ez, Hx or Hy as the reference does; an
+    ``EMFieldHistory3D`` (extension) takes any of Ex..Hz and animates the
+    plane ``index`` along ``axis`` (keyword-only; default the middle).
+    ``save`` writes the animation with matplotlib's writers.  Returns the
+    ``FuncAnimation``."""
+    _check_matplotlib()
+    if hasattr(fields, "grid_z"):
+        if component not in _COMPONENTS_3D:
+            raise VisualizationError(f"Unknown component '{component}'")
+        if axis not in _AXES:
+            raise VisualizationError(f"Unknown axis '{axis}'. Available: ['x', 'y', 'z']")
+        full = getattr(fields, component.lower())
+        a = _AXES[axis]
+        n = full.shape[1 + a]
+        index = n // 2 if index is None else int(index)
+        if not 0 <= index < n:
+            raise VisualizationError(f"index {index} outside [0, {n}) along {axis}")
+        data = full.select(1 + a, index)
+        rest = [ax for ax in "xyz" if ax != axis]
+        g0 = getattr(fields, f"grid_{rest[0]}")
+        g1 = getattr(fields, f"grid_{rest[1]}")
+        labels = (f"{rest[0]} (m)", f"{rest[1]} (m)")
+    else:
+        component_map = {"Ez": fields.ez, "Hx": fields.hx, "Hy": fields.hy}
+        if component not in component_map:
+            raise VisualizationError(f"Unknown component '{component}'")
+        data = component_map[component]
+        g0, g1 = fields.grid_x, fields.grid_y
+        labels = ("x (m)", "y (m)")
+    data = data.detach().float().cpu()
+    t = fields.t.detach().cpu()
+    vmax = float(data.abs().max()) if data.numel() else 0.0
+    if vmax == 0:
+        vmax = 1.0
+    g0, g1 = g0.cpu(), g1.cpu()
+    fig, ax = plt.subplots(figsize=figsize)
+    im = ax.imshow(data[0].T.numpy(), origin="lower", cmap=cmap, vmin=-vmax, vmax=vmax,
+                   extent=(float(g0[0]), float(g0[-1]), float(g1[0]), float(g1[-1])))
+    ax.set_xlabel(labels[0])
+    ax.set_ylabel(labels[1])
+    title = ax.set_title(f"{component} at t = 0")
+    fig.colorbar(im, ax=ax)
+
+    def update(frame: int) -> list:
+        im.set_data(data[frame].T.numpy())
+        title.set_text(f"{component} at t = {float(t[frame]):.2e} s")
+        return [im, title]
+
+    anim = animation.FuncAnimation(fig, update, frames=int(data.shape[0]),
+                                   interval=1000 / fps, blit=False)
+    if save:
+        anim.save(save, fps=fps, dpi=100)
+    return anim

@@ -70,7 +70,10 @@ class COracle:
     """State of one oracle run: fields (x fastest), psi, coefficients."""
 
     def __init__(self, size, dx, dt, boundary, eps_r, sigma=None, npml=0, cpml_kw=None,
-                 dtype=np.float64, sponge_layers=None):
+                 dtype=np.float64, sponge_layers=None, cpml_coeffs=None):
+        """``cpml_coeffs`` (tests only) replaces ``ext_fdtd.cpml_coeffs`` as
+        the function that builds each axis's CPML tables, e.g. to plant a
+        mistake and show that a test catches it."""
         self.dtype = np.dtype(dtype).type
         self.size = tuple(size)
         nx, ny, nz = size
@@ -85,8 +88,9 @@ class COracle:
         self.cp = None
         if boundary == "cpml":
             tabs = []
+            fn = X.cpml_coeffs if cpml_coeffs is None else cpml_coeffs
             for n in size:
-                t = X.cpml_coeffs(n, npml, dx, dt, **(cpml_kw or {}))
+                t = fn(n, npml, dx, dt, **(cpml_kw or {}))
                 tabs.append(np.concatenate([np.asarray(v) for v in (*t["e"], *t["h"])]))
             self.cp = np.ascontiguousarray(np.concatenate(tabs).astype(self.dtype))
         d = OracleDesc()

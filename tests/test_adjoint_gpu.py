@@ -258,3 +258,25 @@ def test_design_permittivity_increases_probe_energy(gpu):
     print(f"[design] J {J0:.4e} -> {res.fun:.4e}", flush=True)
     assert res.fun > J0
     assert float(res.x.min()) >= 1.0 and float(res.x.max()) <= 4.0
+
+
+def test_objective_backward_after_another_forward_raises(gpu):
+    """FDTDObjective's backward uses the problem's latest forward run; a
+    second apply on the same problem before the first backward would give
+    the gradient at the wrong design, so it raises instead."""
+    size, npml, n = (22, 20, 18), 4, 30
+    eps, sigma, box, tab, src, mon = _setup("cpml", False, size, npml, n)
+    x0, y0, z0, bx, by, bz = box
+    g = _gpu_grid(size, "cpml", npml, eps, sigma, tab, src)
+    prob = AdjointProblem(g, n, DT, box, [("ez", p) for _, p in mon], checkpoint_every=8)
+    e1 = torch.tensor(eps[x0:x0 + bx, y0:y0 + by, z0:z0 + bz], device=gpu,
+                      dtype=torch.float64).requires_grad_(True)
+    e2 = (e1.detach() + 0.5).requires_grad_(True)
+    J = FDTDObjective.apply(e1, prob) + FDTDObjective.apply(e2, prob)
+    with pytest.raises(RuntimeError, match="another forward"):
+        J.backward()
+    # one apply, then backward: the gradient of the problem's own forward
+    J1 = FDTDObjective.apply(e1, prob)
+    J1.backward()
+    _, ref = prob.value_and_grad(e1.detach())
+    assert torch.equal(e1.grad.to(ref.dtype).cpu(), ref.cpu())

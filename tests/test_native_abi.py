@@ -78,6 +78,32 @@ def test_plan_and_source_validation_host_only():
     with pytest.raises(ConfigurationError, match="component"):
         plan.set_sources([4], [(1, 1, 1)], np.zeros((3, 1), np.float32), 3)
     # (a valid call uploads the table to the device: exercised by the GPU tests)
+
+
+def test_rejected_set_sources_leaves_the_plan_usable_host_only():
+    """A rejected yee_set_sources changes nothing: repeated rejected calls
+    (bad position after good ones in the same call, bad component, null
+    arrays) each return EINVAL with their message, and the plan is then
+    destroyed cleanly.  The GPU tests check that a run after a rejected call
+    uses the previous sources."""
+    import numpy as np
+
+    lib = _native.lib()
+    plan = _native.Plan(_desc())
+    comp = np.array([2, 2], np.int32)
+    pos = np.array([1, 1, 1, 99, 0, 0], np.int32)
+    tab = np.zeros(6, np.float32)
+    for _ in range(2):
+        rc = lib.yee_set_sources(plan.handle, 2, comp.ctypes.data, pos.ctypes.data,
+                                 tab.ctypes.data, 3)
+        assert rc == _native.YEE_EINVAL and b"out of grid" in lib.yee_last_error()
+    bad = np.array([5], np.int32)
+    rc = lib.yee_set_sources(plan.handle, 1, bad.ctypes.data, pos.ctypes.data,
+                             tab.ctypes.data, 3)
+    assert rc == _native.YEE_EINVAL and b"component" in lib.yee_last_error()
+    rc = lib.yee_set_sources(plan.handle, 1, None, None, None, 3)
+    assert rc == _native.YEE_EINVAL and b"null" in lib.yee_last_error()
+    plan.close()
     plan.close()
 
 

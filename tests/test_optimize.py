@@ -51,5 +51,30 @@ def test_projection_and_errors():
                    max_iterations=10, tolerance=0.0,
                    project=lambda x: x.clamp(max=4.0))
     assert float(res.x) == 4.0
-    with pytest.raises(ValueError, match="unknown method"):
-        optimize(_quad, 0.0, method="newton")
+    # optimize.py:95-134: any method other than "adam" is gradient descent
+    a = optimize(_quad, 0.0, learning_rate=0.1, max_iterations=7, tolerance=0.0,
+                 method="newton")
+    b = optimize(_quad, 0.0, learning_rate=0.1, max_iterations=7, tolerance=0.0)
+    assert float(a.x) == float(b.x) and a.n_iterations == b.n_iterations == 7
+
+
+def test_sensitivity_is_the_jacobian():
+    # optimize.py:210-227: d(outputs)/d(params), float64
+    from jaxphys_amd.optimize import sensitivity
+
+    A = torch.tensor([[1.0, 2.0, -1.0], [0.5, 0.0, 3.0]], dtype=torch.float64)
+
+    def f(p):
+        return A @ p + torch.stack([p[0] * p[1], p[2] ** 3])
+
+    p = torch.tensor([1.5, -2.0, 0.5])
+    J = sensitivity(f, p)
+    assert J.dtype == torch.float64 and J.shape == (2, 3)
+    expect = A.clone()
+    expect[0, 0] += -2.0
+    expect[0, 1] += 1.5
+    expect[1, 2] += 3 * 0.25
+    assert torch.allclose(J, expect, rtol=0, atol=1e-12)
+    # a scalar output gives the gradient
+    g = sensitivity(lambda p: (p ** 2).sum(), [1.0, 2.0])
+    assert torch.allclose(g, torch.tensor([2.0, 4.0], dtype=torch.float64))

@@ -316,3 +316,37 @@ def test_host_snapshots_equal_device_snapshots(gpu):
     for c in NAMES:
         assert torch.equal(getattr(host_h, c), getattr(dev_h, c).cpu()), c
     assert torch.equal(host_h.t, dev_h.t.cpu())
+
+
+@pytest.mark.parametrize("boundary", ["absorbing", "periodic"])
+def test_host_snapshots_match_oracle(gpu, boundary):
+    """simulate(snapshots="host") against the oracle's own restatement of
+    fdtd3d.py:169-417 (float32): every saved step, its time stamp and the
+    grid coordinates; a 40x24x20 grid, two sources, a dielectric slab,
+    snapshots every 6 of 41 steps (the last save not on the final step)."""
+    size = (40, 24, 20)
+    srcs = [R.Src(5e9, (20, 12, 10)), R.Src(7e9, (9, 5, 14), 0.5, "x")]
+    m = np.zeros(size, bool)
+    m[28:] = True
+    mats = [R.Mat(m, 4.0)]
+    dt = R.default_dt(0.01, np.float32)
+    t_span = (0.0, 41.5 * dt)
+    pml = 4 if boundary == "absorbing" else 0
+    g = EMGrid3D(size=size, resolution=0.01, boundary=boundary, pml_layers=pml)
+    for s in srcs:
+        g.add_source(PointSource3D(s.frequency, s.position, s.amplitude, s.polarization))
+    g.add_material(DielectricRegion(mask=m, epsilon_r=4.0))
+    h = g.simulate(t_span=t_span, dt=dt, save_every=6, snapshots="host")
+    t_ref, ref = R.simulate(size, 0.01, boundary, pml, srcs, mats, t_span=t_span, dt=dt,
+                            save_every=6, dtype=np.float32)
+    assert h.ez.device.type == "cpu" and h.ez.shape[0] == 7
+    np.testing.assert_array_equal(h.t.numpy(), t_ref)
+    np.testing.assert_allclose(h.grid_x.numpy(), np.arange(40) * 0.01, rtol=1e-6)
+    for c in NAMES:
+        got = getattr(h, c).numpy()
+        assert got.shape == ref[c].shape
+        for q in range(got.shape[0]):
+            if boundary == "periodic":
+                np.testing.assert_array_equal(got[q], ref[c][q])
+            else:
+                assert rel_l2(got[q], ref[c][q]) <= 1e-5, (c, q)

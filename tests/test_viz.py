@@ -49,6 +49,38 @@ def test_plots_render():
         viz.plot_field_snapshot(h2, component="Ex")
 
 
+def test_animate_field_frames():
+    if not viz.HAS_MATPLOTLIB:
+        with pytest.raises(VisualizationError):
+            viz.animate_field(_hist3d())
+        return
+    import matplotlib
+
+    matplotlib.use("Agg")
+    g = torch.Generator().manual_seed(2)
+    h2 = EMFieldHistory(torch.arange(4.0) * 1e-10,
+                        *[torch.randn(4, 7, 5, generator=g) for _ in range(3)],
+                        torch.arange(7.0), torch.arange(5.0))
+    anim = viz.animate_field(h2, "Hy")
+    # frame q shows snapshot q (transposed to (y, x) for imshow), scale fixed
+    # by the largest magnitude over all frames
+    anim._func(2)
+    img = anim._fig.axes[0].images[0]
+    assert np.array_equal(np.asarray(img.get_array()), h2.hy[2].T.numpy())
+    vmax = float(h2.hy.abs().max())
+    assert img.get_clim() == pytest.approx((-vmax, vmax))
+    assert "2.00e-10" in anim._fig.axes[0].get_title()
+    with pytest.raises(VisualizationError):
+        viz.animate_field(h2, "Ex")
+    h3 = _hist3d()
+    anim3 = viz.animate_field(h3, "Hy", axis="y", index=2)
+    anim3._func(1)
+    img3 = anim3._fig.axes[0].images[0]
+    assert np.array_equal(np.asarray(img3.get_array()), h3.hy[1][:, 2, :].T.numpy())
+    with pytest.raises(VisualizationError):
+        viz.animate_field(h3, "Ez", axis="w")
+
+
 def _flux_ref(f, axis, p):
     """sum over the H plane p+1/2 of (E x H).n with the tangential E averaged
     over planes p, p+1; arrays in the reference (x, y, z) order."""
