@@ -245,6 +245,21 @@ int yee_adjoint_run(YeePlan* plan, float* const* lam, float* lam_psi, int32_t n_
 int yee_energy_fp64(int32_t nx, int32_t ny, int32_t nz, float* const* fields,
                     const float* eps_r, double* work, void* stream);
 
+/* yee_energy_fp64 for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  Same kernels and the same
+ * summation order as yee_energy_fp64, so the same bits; the 2*1024 float64
+ * partial sums live in a static device array of the library (16 KiB per
+ * GPU, held until the process ends) instead of a caller buffer, and `out`
+ * (device, 2 doubles) receives sum eps_r |E|^2 and sum |H|^2.  Calls no
+ * hipMalloc/hipFree.  Grids above YEE_NOALLOC_MAX_CELLS cells are rejected
+ * with YEE_EINVAL.  All calls on one GPU share the array: calls that
+ * overlap in time on one GPU (different streams or host threads) race;
+ * the caller's stream must belong to the current device.                 */
+#define YEE_NOALLOC_MAX_CELLS (1LL << 36)
+int yee_energy_fp64_noalloc(int32_t nx, int32_t ny, int32_t nz, float* const* fields,
+                            const float* eps_r, double* out, void* stream);
+
 /* Float64 Poynting flux through the plane normal to `axis` (0 x, 1 y, 2 z)
  * halfway between the E planes `plane` and `plane+1`, where the tangential
  * H components live: work[0] = sum over the plane of (E x H) . n, each

@@ -63,6 +63,9 @@ EXPORTS = {
     "yee_energy_fp64": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                        ctypes.c_void_p]),
+    "yee_energy_fp64_noalloc": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                               ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                               ctypes.c_void_p]),
     "yee_copy_probe": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
                                       ctypes.c_void_p]),
     "yee_poynting_flux_fp64": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
@@ -304,6 +307,16 @@ def energy_fp64(shape_xyz, fields, eps_r=None, work=None, stream=None):
     check(lib().yee_energy_fp64(nx, ny, nz, field_array(fields), ptr(eps_r) or None,
                                 ptr(work), stream_handle(stream)), "yee_energy_fp64")
     return work[:2]
+
+
+def energy_fp64_noalloc(shape_xyz, fields, eps_r, out, stream=None) -> None:
+    """``energy_fp64`` through the entry point that allocates nothing: the
+    partial sums stay in the library's static device array; ``out`` (device
+    float64, 2 values) receives (sum eps_r |E|^2, sum |H|^2).  Asynchronous."""
+    nx, ny, nz = shape_xyz
+    check(lib().yee_energy_fp64_noalloc(nx, ny, nz, field_array(fields), ptr(eps_r) or None,
+                                        ptr(out), stream_handle(stream)),
+          "yee_energy_fp64_noalloc")
 
 
 FLUX_WORK = 1 + 1024

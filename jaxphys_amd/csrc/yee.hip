@@ -21,6 +21,8 @@
 
 #include "yee_internal.h"
 
+#include <mutex>
+
 namespace yee {
 
 thread_local std::string g_err;
@@ -553,6 +555,40 @@ int launch_inject(const YeePlan* p, float* ex, float* ey, float* ez, int n, int 
   return YEE_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Workspace of yee_energy_fp64_noalloc: the float64 partial sums of the
+// energy reduction in a file-scope device array instead of a caller buffer.
+// Every GPU the library runs on holds its own copy (16 KiB) until the
+// process ends.  In host code `scratch` names a host-side stand-in, never
+// the device memory: the device address is looked up once per device with
+// hipGetSymbolAddress and only that address is passed to kernels.
+// ---------------------------------------------------------------------------
+constexpr int kScratchDoubles = 2 * kEnergyBlocks;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kMaxDevices] = {};
+
+// device address of `scratch` on the current device (looked up on the
+// first call per device, with that device current)
+static int scratch_on_current_device(double** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(YEE_EHIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDevices) return fail(YEE_EUNSUPPORTED, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(YEE_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = (double*)ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return YEE_OK;
+}
+
 }  // namespace yee
 
 using namespace yee;
@@ -763,6 +799,35 @@ int yee_energy_fp64(int32_t nx, int32_t ny, int32_t nz, float* const* fields, co
   energy_partial_kernel<<<kEnergyBlocks, 256, 0, st>>>(g, eps_r, ncell, work + 2);
   YEE_HIP_CHECK(hipGetLastError());
   energy_final_kernel<<<1, 256, 0, st>>>(work + 2, kEnergyBlocks, work);
+  YEE_HIP_CHECK(hipGetLastError());
+  return YEE_OK;
+}
+
+int yee_energy_fp64_noalloc(int32_t nx, int32_t ny, int32_t nz, float* const* fields,
+                            const float* eps_r, double* out, void* stream) {
+  if (nx < 1 || ny < 1 || nz < 1) return fail(YEE_EINVAL, "bad grid size");
+  if (!fields || !out) return fail(YEE_EINVAL, "null pointer");
+  const long long ncell = (long long)nx * ny * nz;
+  if (ncell > YEE_NOALLOC_MAX_CELLS)
+    return fail(YEE_EINVAL, "yee_energy_fp64_noalloc: grid of " + std::to_string(ncell) +
+                                " cells exceeds its maximum of " +
+                                std::to_string((long long)YEE_NOALLOC_MAX_CELLS) + " cells");
+  GridArgs g;
+  std::memset(&g, 0, sizeof(g));
+  for (int c = 0; c < 6; ++c) {
+    if (!fields[c]) return fail(YEE_EINVAL, "null field pointer");
+    g.f[c] = fields[c];
+  }
+  double* part = nullptr;
+  const int rc = scratch_on_current_device(&part);
+  if (rc) return rc;
+  static_assert(2 * kEnergyBlocks <= kScratchDoubles, "energy partials exceed the scratch array");
+  hipStream_t st = (hipStream_t)stream;
+  // every partial slot is written by its block before the final kernel
+  // reads it, so nothing left by an earlier call is read
+  energy_partial_kernel<<<kEnergyBlocks, 256, 0, st>>>(g, eps_r, ncell, part);
+  YEE_HIP_CHECK(hipGetLastError());
+  energy_final_kernel<<<1, 256, 0, st>>>(part, kEnergyBlocks, out);
   YEE_HIP_CHECK(hipGetLastError());
   return YEE_OK;
 }
