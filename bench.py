@@ -25,6 +25,7 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "FDTD3D Gcell-upd/s & %HBM roofline; fwd+adjoint s; closure speedup@equal err"
 
 
 def build_workload(name: str, device):
@@ -138,24 +139,23 @@ def measured_traffic(workload: str, mode: str):
     return None, None
 
 
-def adjoint_bench(args, dev):
-    """160^3, 12-cell CPML, 800 steps; 96^3 design region eps_r U[1,4] in
-    vacuum; z dipole 20 cells below it; J = time-integrated Ez^2 on a 32x32
-    patch 20 cells above it; the design-box history recorded in the forward
-    run when it fits (else checkpoints every 100 steps).  Reports the cost
-    of value+gradient relative to one plain forward run and central finite
-    differences (h=1e-3) on 16 seeded design cells."""
-    import numpy as np
+def adjoint_measure(dev, steps: int = 800, checkpoint_every: int = 100, record_box=False):
+    """The adjoint workload: 160^3, 12-cell CPML, ``steps`` steps; 96^3
+    design region eps_r U[1,4] in vacuum; z dipole 20 cells below it; J =
+    time-integrated Ez^2 on a 32x32 patch 20 cells above it.  Times one
+    plain forward run and one value + gradient (``record_box=False``:
+    checkpoints every ``checkpoint_every`` steps, each segment recomputed
+    while recording the design box, then its adjoint steps).  Returns the
+    timings, the problem, the gradient and the inputs."""
     import torch
 
     from jaxphys_amd import workloads as W
     from jaxphys_amd.adjoint import AdjointProblem
 
-    n, steps = 160, args.steps or 800
+    n = 160
     g, box, probes, eps_box = W.c4_design(n, dev)
-    rec = {"auto": None, "on": True, "off": False}[args.record_box]
     prob = AdjointProblem(g, steps, W.dt_for(1.0), box, probes,
-                          checkpoint_every=args.checkpoint_every, record_box=rec)
+                          checkpoint_every=checkpoint_every, record_box=record_box)
     # plain forward (the run being differentiated), timed alone
     plain = g.build_solver(W.dt_for(1.0), steps)
     eps_full = prob.background.clone()
@@ -178,30 +178,91 @@ def adjoint_bench(args, dev):
     grad = prob.backward()
     torch.cuda.synchronize()
     t2 = time.perf_counter()
-    rng = np.random.default_rng(42)
+    mode = ("design-box history recorded in the forward run" if prob.record_box else
+            f"checkpoints every {prob.K} steps, segments recomputed")
+    summary = {"ratio": round((t2 - t0) / t_fwd, 3), "forward_s": round(t_fwd, 4),
+               "value_and_grad_s": round(t2 - t0, 4), "checkpoint_forward_s": round(t1 - t0, 4),
+               "backward_s": round(t2 - t1, 4), "mode": mode,
+               "checkpoint_every": prob.K, "record_box": prob.record_box,
+               "grid": [n, n, n], "steps": steps, "design_box": list(box), "J": J}
+    return summary, prob, grad, (g, box, probes, eps_box)
+
+
+def adjoint_fd_float64(g, box, probes, eps_box, steps, cells):
+    """Central differences (h = 1e-3) of the objective of the float64 C
+    oracle (the checker) at design cells ``cells`` (box-relative)."""
+    import numpy as np
+
+    from jaxphys_amd import workloads as W
+    from oracle import c_oracle as CO
+    from oracle import ext_fdtd as X
+
+    n = g.size[0]
+    x0, y0, z0, bx, by, bz = box
+    eps64 = np.ones((n, n, n))
+    eps64[x0:x0 + bx, y0:y0 + by, z0:z0 + bz] = eps_box.cpu().numpy().astype(np.float64)
+    tab = X.waveform_table("gaussian_sine", steps, t0=90, spread=30, wavelength=20,
+                           courant=W.COURANT)
+    zmon = probes[0][1][2]
+    lo, hi = probes[0][1][0], probes[-1][1][0] + 1
+    threads = CO.host_cores()
+
+    def J64(eps):
+        o = CO.COracle((n,) * 3, 1.0, W.dt_for(1.0), "cpml", eps, npml=g._pml_layers,
+                       dtype=np.float64)
+        o.set_sources([(2, g._sources[0].position, tab)])
+        tot = 0.0
+        for m in range(steps):
+            o.run(m, 1, threads=threads)
+            patch = o.f[2][zmon, lo:hi, lo:hi]
+            tot += float((patch * patch).sum())
+        return tot
+
     h = 1e-3
-    fd_err = []
-    for _ in range(args.fd_cells):
-        c = tuple(int(rng.integers(0, s)) for s in eps_box.shape)
-        ep = eps_box.clone(); ep[c] += h  # noqa: E702
-        em = eps_box.clone(); em[c] -= h  # noqa: E702
-        fd = (prob.forward(ep) - prob.forward(em)) / (2 * h)
-        gv = float(grad[c])
-        fd_err.append({"cell": c, "adjoint": gv, "fd": fd})
-    scale = float(grad.abs().max())
-    worst = max((abs(e["adjoint"] - e["fd"]) / scale for e in fd_err), default=None)
+    out = []
+    for cell in cells:
+        ijk = (x0 + cell[0], y0 + cell[1], z0 + cell[2])
+        ep, em = eps64.copy(), eps64.copy()
+        ep[ijk] += h
+        em[ijk] -= h
+        out.append((J64(ep) - J64(em)) / (2 * h))
+    return out
+
+
+def adjoint_bench(args, dev):
+    """The adjoint workload (``adjoint_measure``) in the mode asked for
+    (default: checkpoints every 100 steps with recomputation), with central
+    finite differences (h=1e-3) of the float64 oracle's objective on
+    ``--fd-cells`` seeded design cells, reported per cell."""
+    import numpy as np
+
+    steps = args.steps or 800
+    rec = {"auto": None, "on": True, "off": False}[args.record_box]
+    summary, prob, grad, (g, box, probes, eps_box) = adjoint_measure(
+        dev, steps, args.checkpoint_every, rec)
+    gnp = grad.cpu().numpy()
+    rng = np.random.default_rng(42)
+    cells = [tuple(int(rng.integers(0, s)) for s in eps_box.shape) for _ in range(args.fd_cells)]
+    fd = adjoint_fd_float64(g, box, probes, eps_box, steps, cells) if cells else []
+    samples = [{"cell": c, "adjoint": float(gnp[c]), "fd_float64": f,
+                "rel_err": abs(float(gnp[c]) - f) / abs(f) if f != 0 else None}
+               for c, f in zip(cells, fd)]
+    scale = float(np.abs(gnp).max())
     return {
         "metric": "adjoint dJ/deps_r cost (x one forward run)",
-        "value": round((t2 - t0) / t_fwd, 3), "unit": "x forward", "higher_is_better": False,
+        "value": summary["ratio"], "unit": "x forward", "higher_is_better": False,
         "n_gpus": 1, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "adjoint_c4", "grid": [n, n, n], "steps": steps, "npml": 12,
-                   "design_box": list(box), "checkpoint_every": args.checkpoint_every,
-                   "record_box": prob.record_box,
-                   "probes": len(probes)},
-        "forward_s": round(t_fwd, 4), "value_and_grad_s": round(t2 - t0, 4),
-        "recording_forward_s": round(t1 - t0, 4), "adjoint_s": round(t2 - t1, 4),
-        "J": J, "fd_check": {"h": h, "cells": len(fd_err),
-                              "max_abs_err_over_max_grad": worst, "samples": fd_err[:4]},
+        "config": {"workload": "adjoint_c4", "grid": summary["grid"], "steps": steps, "npml": 12,
+                   "design_box": list(box), "checkpoint_every": summary["checkpoint_every"],
+                   "record_box": summary["record_box"], "probes": len(probes)},
+        "adjoint": summary,
+        "fd_check": {"h": 1e-3, "oracle": "float64 C oracle", "cells": len(samples),
+                     "max_rel_err": max((x["rel_err"] for x in samples
+                                         if x["rel_err"] is not None), default=None),
+                     "max_abs_err_over_max_grad": max(
+                         (abs(x["adjoint"] - x["fd_float64"]) / scale for x in samples),
+                         default=None),
+                     "samples": samples},
     }
 
 
@@ -416,6 +477,50 @@ def closure_net_bench(args, dev):
     }
 
 
+CLOSURE_FILE = os.path.join(ROOT, "jaxphys_amd", "data", "closure_c5.safetensors")
+
+
+def closure_summary(net, dev, seeds, nf: int = 256, every: int = 10, **pkw):
+    """Evaluate a trained closure on held-out seeds (fine nf^3 dx=1 vs coarse
+    (nf/2)^3 dx=2, 400 coarse steps): mean / max interior field error of E
+    at the final step against the restricted fine run, the far-field error
+    (cells more than 3 coarse cells from the source), the same for plain
+    coarse FDTD and for the linear parts alone, and the wall-time ratio of
+    the fine run to the corrected coarse run."""
+    import numpy as np
+
+    from jaxphys_amd import closure as CL
+
+    res = [CL.evaluate_closure(net, s, n_fine=nf, n_coarse=nf * 400 // 256, every=every, **pkw)
+           for s in seeds]
+
+    def col(k):
+        return np.array([r.get(k, float("nan")) for r in res], dtype=np.float64)
+
+    far_c, far_0 = col("err_far_closure"), col("err_far_coarse")
+    return {
+        "seeds": [int(s) for s in seeds],
+        "speedup_vs_fine": round(float(np.mean(col("speedup"))), 3),
+        "err": round(float(np.mean(col("err_closure"))), 5),
+        "err_max": round(float(np.max(col("err_closure"))), 5),
+        "err_far_median": round(float(np.median(far_c)), 5),
+        "err_far_max": round(float(np.max(far_c)), 5),
+        "coarse_err": round(float(np.mean(col("err_coarse"))), 5),
+        "coarse_err_far_median": round(float(np.median(far_0)), 5),
+        "seeds_far_worse_than_coarse": int(np.sum(far_c > far_0)),
+        "linear_parts_err_far_median": round(float(np.median(col("err_far_far_field"))), 5),
+        "net_gain": net.net_gain, "far_gain": net.far_gain,
+        "t_fine_s": round(float(np.mean(col("t_fine_s"))), 4),
+        "t_closure_s": round(float(np.mean(col("t_closure_s"))), 4),
+        "per_seed": [{"seed": r["seed"], "err": round(r["err_closure"], 5),
+                      "err_far": round(r["err_far_closure"], 5),
+                      "err_far_coarse": round(r["err_far_coarse"], 5),
+                      "err_far_linear": round(r.get("err_far_far_field", float("nan")), 5)}
+                     for r in res],
+        "_res": res,
+    }
+
+
 def closure_bench(args, dev):
     """Learned closure: train on fine-run/coarse pairs, evaluate on held-out
     seeds; reports the corrected coarse error and the wall-time ratio."""
@@ -429,7 +534,10 @@ def closure_bench(args, dev):
     t0 = time.time()
     pkw = {"t0": args.closure_t0, "spread": args.closure_spread,
            "waveform": args.closure_waveform}
-    net, hist = CL.train_closure(dev, train_seeds, n_fine=nf, n_coarse=nf * 400 // 256,
+    if args.closure_load:
+        net, hist = CL.ClosureNet.load(args.closure_load, dev), []
+    else:
+        net, hist = CL.train_closure(dev, train_seeds, n_fine=nf, n_coarse=nf * 400 // 256,
                                  every=args.closure_every, epochs=args.closure_epochs,
                                  crop=min(64, nf // 2 - 16), lr=args.closure_lr,
                                  lr_final=args.closure_lr_final,
@@ -443,8 +551,15 @@ def closure_bench(args, dev):
                                  log=lambda m: print("[closure]", m, file=sys.stderr, flush=True),
                                  **pkw)
     t_train = time.time() - t0
-    res = [CL.evaluate_closure(net, s, n_fine=nf, n_coarse=nf * 400 // 256,
-                               every=args.closure_every, **pkw) for s in test_seeds]
+    if args.closure_save:
+        os.makedirs(os.path.dirname(os.path.abspath(args.closure_save)), exist_ok=True)
+        net.save(args.closure_save, info={
+            "workload": "closure_c5", "n_fine": nf, "train_seeds": train_seeds,
+            "val_seeds": list(range(900, 900 + args.closure_val)),
+            "epochs": args.closure_epochs, "every": args.closure_every, "seed": args.closure_seed,
+            "train_s": round(t_train, 1), **pkw})
+    summ = closure_summary(net, dev, test_seeds, nf, args.closure_every, **pkw)
+    res = summ.pop("_res")
     err_c = [r["err_closure"] for r in res]
     err_0 = [r["err_coarse"] for r in res]
     trj_c = [r["err_traj_closure"] for r in res]
@@ -501,6 +616,7 @@ def closure_bench(args, dev):
         "t_closure_s": round(float(np.mean([r["t_closure_s"] for r in res])), 4),
         "t_coarse_s": round(float(np.mean([r["t_coarse_s"] for r in res])), 4),
         "train_s": round(t_train, 1), "final_loss": float(np.mean(hist[-40:])) if hist else None,
+        "summary": summ,
     }
 
 
@@ -515,9 +631,10 @@ def main() -> None:
     ap.add_argument("--mode", default="fused", choices=["fused", "two_pass"],
                     help="one fused launch per step (default) or the H and E half-step kernels")
     ap.add_argument("--checkpoint-every", type=int, default=100)
-    ap.add_argument("--record-box", default="auto", choices=["auto", "on", "off"],
-                    help="adjoint: record the design-box history in the forward run "
-                         "(auto: when it fits) or recompute it from checkpoints")
+    ap.add_argument("--record-box", default="off", choices=["auto", "on", "off"],
+                    help="adjoint: recompute the design-box history from checkpoints (off, "
+                         "the configured mode) or record it in the forward run (on; auto: "
+                         "when it fits)")
     ap.add_argument("--fd-cells", type=int, default=16)
     ap.add_argument("--closure-n", type=int, default=256)
     ap.add_argument("--closure-train", type=int, default=64)
@@ -539,6 +656,10 @@ def main() -> None:
                     help="no linear far-field operator (net and source model only)")
     ap.add_argument("--closure-no-forcing", action="store_true",
                     help="no forcing input channel (fields and eps_r only)")
+    ap.add_argument("--closure-save", default=None,
+                    help="closure_c5: write the trained closure to this safetensors file")
+    ap.add_argument("--closure-load", default=None,
+                    help="closure_c5: evaluate this saved closure instead of training one")
     ap.add_argument("--closure-every", type=int, default=10,
                     help="coarse steps between closure corrections")
     ap.add_argument("--closure-t0", type=float, default=60.0,
@@ -548,6 +669,10 @@ def main() -> None:
     ap.add_argument("--closure-waveform", default="gaussian",
                     choices=["gaussian", "gaussian_derivative"])
     ap.add_argument("--tm2d-n", type=int, default=16384, help="tm2d workload: cells per side")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="default workload: leave out the adjoint and closure objects")
+    ap.add_argument("--closure-seeds", type=int, default=16,
+                    help="default workload: held-out seeds (1000, 1001, ...) of the closure object")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain copy of the field bytes (counter calibration)")
     args = ap.parse_args()
@@ -642,6 +767,10 @@ def main() -> None:
         dom_ms = launch_ms
         step_bytes = dom_bytes
         h_ms = e_ms = None
+        # what the fused design itself must move: E and H read and written
+        # once (48 B), the material id (1 B) or Cb (and Ca) per cell, psi
+        coef = 1 if solver.mat is not None else (8 if lossy else 4)
+        own_min_bytes = (48 + coef) * cells + psi_bytes
     else:
         ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(K)]
         torch.cuda.synchronize()
@@ -662,6 +791,7 @@ def main() -> None:
         dom_name, dom_bytes, dom_ms = ("yee_e_kernel", e_bytes, e_ms) if e_ms >= h_ms else \
             ("yee_h_kernel", h_bytes, h_ms)
         step_bytes = h_bytes + e_bytes
+        own_min_bytes = None
     traffic, traffic_rec = measured_traffic(args.workload, args.mode)
     ms_step = total_ms / K
     gcells = cells * K / (total_ms / 1e3) / 1e9
@@ -683,7 +813,7 @@ def main() -> None:
             cpu_np = {"value": None, "error": f"{type(exc).__name__}: {exc}"}
 
     out = {
-        "metric": "FDTD3D Gcell-upd/s & %HBM roofline",
+        "metric": METRIC,
         "value": round(gcells, 3),
         "unit": "Gcell-upd/s",
         "n_gpus": 1,
@@ -715,6 +845,9 @@ def main() -> None:
             "traffic_source": None if traffic_rec is None else traffic_rec.get("source"),
             "bytes_per_launch": dom_bytes,
             "ms_per_launch": round(dom_ms, 4),
+            "own_min_bytes": own_min_bytes,
+            "frac_own_min": None if own_min_bytes is None else
+            round(own_min_bytes / (dom_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
         },
         "step": {
             "mode": args.mode,
@@ -730,6 +863,25 @@ def main() -> None:
     }
     if cpu_np is not None:
         out["cpu_baseline_numpy"] = cpu_np
+    if not args.no_extras and args.workload == "c3_512":
+        # the rest of the metric: gradient cost and the closure, each on its
+        # own configuration (after the timed region above)
+        del solver, grid
+        torch.cuda.empty_cache()
+        try:
+            out["adjoint"] = adjoint_measure(dev)[0]
+        except Exception as exc:  # reported, never silently replaced
+            out["adjoint"] = {"error": f"{type(exc).__name__}: {exc}"}
+        torch.cuda.empty_cache()
+        try:
+            from jaxphys_amd.closure import ClosureNet
+            net = ClosureNet.load(CLOSURE_FILE, dev)
+            summ = closure_summary(net, dev, list(range(1000, 1000 + args.closure_seeds)))
+            summ.pop("_res")
+            summ["model"] = os.path.relpath(CLOSURE_FILE, ROOT)
+            out["closure"] = summ
+        except Exception as exc:  # reported, never silently replaced
+            out["closure"] = {"error": f"{type(exc).__name__}: {exc}"}
     print(json.dumps(out), flush=True)
 
 

@@ -372,6 +372,64 @@ class ClosureNet:
                 "ks": self.ks.cpu(), "far_w": self.far_w.cpu(), "far_gain": self.far_gain,
                 "net_gain": self.net_gain}
 
+    # scalar settings stored beside the tensors of a saved closure
+    _META = ("eps_shift", "eps_scale", "force_scale", "margin", "far_gain", "net_gain")
+
+    def save(self, path: str, info: dict | None = None) -> None:
+        """Write the trained closure (weights, normalisation, the two linear
+        parts and the gains) as a safetensors file: tensors only, plus the
+        scalar settings and ``info`` as JSON metadata (nothing executable)."""
+        import json
+
+        from safetensors.torch import save_file
+
+        tensors = {f"params.{k}": v.detach().float().cpu().contiguous()
+                   for k, v in self.params.items()}
+        tensors.update({"scale": self.scale.float().cpu().contiguous(),
+                        "ks": self.ks.float().cpu().contiguous(),
+                        "far_w": self.far_w.float().cpu().contiguous()})
+        meta = {k: float(getattr(self, k)) for k in self._META}
+        meta["forcing"] = bool(self.forcing)
+        save_file(tensors, path, metadata={"closure": json.dumps(meta),
+                                           "info": json.dumps(info or {})})
+
+    @classmethod
+    def load(cls, path: str, device) -> "ClosureNet":
+        """A closure saved by ``save``, on ``device``, ready to ``correct``."""
+        import json
+
+        from safetensors import safe_open
+
+        with safe_open(path, framework="pt") as fh:
+            meta = json.loads(fh.metadata()["closure"])
+            tensors = {k: fh.get_tensor(k) for k in fh.keys()}
+        net = cls(device, forcing=bool(meta["forcing"]))
+        for k in net.params:
+            t = tensors[f"params.{k}"]
+            if t.shape != net.params[k].shape:
+                raise ValueError(f"saved closure: {k} has shape {tuple(t.shape)}, "
+                                 f"expected {tuple(net.params[k].shape)}")
+            net.params[k] = t.to(net.device)
+        net.scale = tensors["scale"].to(net.device)
+        net.ks = tensors["ks"].to(net.device)
+        net.far_w = tensors["far_w"].to(net.device)
+        for k in cls._META:
+            setattr(net, k, int(meta[k]) if k == "margin" else float(meta[k]))
+        net.adam = {k: (torch.zeros_like(v), torch.zeros_like(v)) for k, v in net.params.items()}
+        net.pack()
+        net.version += 1
+        return net
+
+    @staticmethod
+    def info(path: str) -> dict:
+        """The ``info`` metadata a closure file was saved with."""
+        import json
+
+        from safetensors import safe_open
+
+        with safe_open(path, framework="pt") as fh:
+            return json.loads(fh.metadata().get("info", "{}"))
+
 
 class _Forcing:
     """Per-interval source features of one solver (see interval_forcing)."""

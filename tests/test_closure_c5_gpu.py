@@ -1,0 +1,35 @@
+"""The committed trained closure (jaxphys_amd/data/closure_c5.safetensors,
+weights-only) on the C5 configuration at its own size: fine 256^3 (dx=1)
+against coarse 128^3 (dx=2), 400 coarse steps, random dielectric boxes and
+a Gaussian-pulse dipole, on held-out seeds that neither training (0-63) nor
+validation (900-901) used.  The fine FDTD run is the judge of the error."""
+
+import os
+
+import pytest
+
+from jaxphys_amd import closure as CL
+
+pytestmark = pytest.mark.gpu
+
+MODEL = os.path.join(os.path.dirname(CL.__file__), "data", "closure_c5.safetensors")
+
+
+@pytest.mark.parametrize("seed", [1000, 1007])
+def test_committed_closure_on_held_out_seed(gpu, seed):
+    net = CL.ClosureNet.load(MODEL, gpu)
+    info = CL.ClosureNet.info(MODEL)
+    assert seed not in info["train_seeds"] and seed not in info["val_seeds"]
+    r = CL.evaluate_closure(net, seed, n_fine=256, n_coarse=400, every=10, t0=info["t0"],
+                            spread=info["spread"], waveform=info["waveform"])
+    print(f"[c5] seed {seed}: closure {r['err_closure']:.4%} (far {r['err_far_closure']:.3%}), "
+          f"plain coarse {r['err_coarse']:.3%} (far {r['err_far_coarse']:.3%}), "
+          f"speedup {r['speedup']:.2f}x", flush=True)
+    # field error at the final step against the restricted fine run
+    assert r["err_closure"] <= 0.02
+    assert r["err_closure"] < 0.1 * r["err_coarse"]
+    # away from the source cells the closure also beats plain coarse FDTD
+    assert r["err_far_closure"] < 0.5 * r["err_far_coarse"]
+    assert r["err_traj_closure"] <= 0.02
+    # wall time: measured 9-10x faster than the fine grid
+    assert r["speedup"] >= 4.0

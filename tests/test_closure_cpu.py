@@ -89,3 +89,27 @@ def test_far_field_features_and_kernel_weight_layout_agree_with_the_oracle():
     wk = CL.far_kernel_weights(W).double().reshape(27, 6, 2, 6).numpy()
     ref = O.stencil([f.numpy() for f in fields], eps.numpy(), net.scale.numpy(), wk)
     np.testing.assert_allclose((X @ W).numpy(), ref.reshape(-1, 6), rtol=1e-6, atol=1e-6)
+
+
+def test_closure_save_load_roundtrip(tmp_path):
+    """A saved closure (safetensors: tensors and JSON metadata only) loads
+    back with the same weights, linear parts, normalisation and gains."""
+    import torch
+
+    from jaxphys_amd.closure import ClosureNet
+
+    net = ClosureNet("cpu", seed=3)
+    net.far_gain, net.net_gain, net.margin, net.force_scale = 0.75, 0.25, 12, 2.5
+    g = torch.Generator().manual_seed(0)
+    net.ks = torch.randn(net.ks.shape, generator=g)
+    net.far_w = torch.randn(net.far_w.shape, generator=g)
+    net.scale = torch.rand(6, generator=g) + 0.5
+    path = str(tmp_path / "closure.safetensors")
+    net.save(path, info={"train_seeds": 64})
+    back = ClosureNet.load(path, "cpu")
+    for k in net.params:
+        assert torch.equal(net.params[k], back.params[k]), k
+    assert torch.equal(net.ks, back.ks) and torch.equal(net.far_w, back.far_w)
+    assert torch.equal(net.scale, back.scale)
+    assert (back.far_gain, back.net_gain, back.margin, back.force_scale) == (0.75, 0.25, 12, 2.5)
+    assert back.forcing and ClosureNet.info(path) == {"train_seeds": 64}
