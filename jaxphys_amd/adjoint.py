@@ -17,7 +17,9 @@ is a single adjoint sweep.  Otherwise, checkpointing:
 2. for each checkpoint segment, last first: restore the checkpoint, recompute
    the segment forward while recording the design box for every step, then
    run the adjoint steps of the segment in reverse (``yee_adjoint_run``),
-   which accumulate ``sum_n g d`` and ``sum_n g E^n`` per design cell;
+   which accumulate ``sum_n g d`` and ``sum_n g E^n`` per design cell.  The
+   last segment's box is recorded during the forward run itself (the sweep
+   starts there, so it needs no recomputation);
 
 and in both cases the chain rule to eps_r:
 ``dJ/deps = acc_cb/Cb * dCb/deps + acc_ca * dCa/deps``.
@@ -156,6 +158,7 @@ class AdjointProblem:
             free, _ = torch.cuda.mem_get_info(dev)
             record_box = need <= free // 4
         self.record_box = bool(record_box)
+        self.seg_buf = None
         self._recorded = False
         # number of forward() calls so far: backward() uses what the latest
         # forward left (probe history, checkpoints or box history, eps), so
@@ -193,12 +196,21 @@ class AdjointProblem:
             ph = self.probe_hist.to(torch.float64)
             return float((self.w.to(torch.float64) * ph * ph).sum())
         self._recorded = False
-        for s0 in range(0, N, K):
+        if self.seg_buf is None or self.seg_buf.shape[0] < K:
+            self.seg_buf = torch.empty((K, self.rec_floats), dtype=torch.float32,
+                                       device=sol.device)
+        starts = list(range(0, N, K))
+        for q, s0 in enumerate(starts):
             self.checkpoints.append(([f.clone() for f in sol.fields],
                                      sol.psi.clone() if sol.psi is not None else None))
             ns = min(K, N - s0)
+            last = q == len(starts) - 1
+            # the last segment is recorded on the way (the backward pass
+            # starts there and needs no recomputation of it)
             _native.run_record(sol.plan, sol.fields, sol.psi, s0, ns, self.probe_comp,
                                self.probe_idx, self.probe_hist[s0:s0 + ns],
+                               box=self.box_int if last else None,
+                               box_out=self.seg_buf if last else None,
                                probe_box=self.probe_box, probe_map=self.probe_map)
         ph = self.probe_hist.to(torch.float64)
         return float((self.w.to(torch.float64) * ph * ph).sum())
@@ -218,28 +230,29 @@ class AdjointProblem:
         # adjoint of the final E: 2 w F(after step N-1)
         _native.probe_scatter_add(lam, self.probe_comp, self.probe_idx, self.w,
                                   self.probe_hist[N - 1], 2.0)
+
+        def sweep(n_lo, n_hi, box_hist):
+            _native.adjoint_run(sol.plan, lam, lam_psi, n_lo, n_hi, self.probe_comp,
+                                self.probe_idx, self.w, self.probe_hist, 2.0, box=self.box_int,
+                                box_hist=box_hist, acc_cb=acc_cb, acc_ca=acc_ca,
+                                probe_box=self.probe_box, probe_map=self.probe_map)
+
         if self._recorded:
-            _native.adjoint_run(sol.plan, lam, lam_psi, 0, N, self.probe_comp, self.probe_idx,
-                                self.w, self.probe_hist, 2.0, box=self.box_int,
-                                box_hist=self.box_hist, acc_cb=acc_cb, acc_ca=acc_ca,
-                                probe_box=self.probe_box, probe_map=self.probe_map)
+            sweep(0, N, self.box_hist)
             return self._chain_rule(acc_cb, acc_ca)
-        if self.box_hist is None or self.box_hist.shape[0] < K:
-            self.box_hist = torch.empty((K, self.rec_floats), dtype=torch.float32, device=dev)
         for seg in range(len(self.checkpoints) - 1, -1, -1):
-            s0 = seg * K
-            s1 = min(N, s0 + K)
-            fields, psi = self.checkpoints[seg]
-            for f, c in zip(sol.fields, fields):
-                f.copy_(c)
-            if psi is not None:
-                sol.psi.copy_(psi)
-            _native.run_record(sol.plan, sol.fields, sol.psi, s0, s1 - s0, box=self.box_int,
-                               box_out=self.box_hist)
-            _native.adjoint_run(sol.plan, lam, lam_psi, s0, s1, self.probe_comp, self.probe_idx,
-                                self.w, self.probe_hist, 2.0, box=self.box_int,
-                                box_hist=self.box_hist, acc_cb=acc_cb, acc_ca=acc_ca,
-                                probe_box=self.probe_box, probe_map=self.probe_map)
+            s0, s1 = seg * K, min(N, seg * K + K)
+            if seg < len(self.checkpoints) - 1:
+                # restore the checkpoint and recompute the segment, recording
+                # the design box (the last one was recorded by forward())
+                fields, psi = self.checkpoints[seg]
+                for f, c in zip(sol.fields, fields):
+                    f.copy_(c)
+                if psi is not None:
+                    sol.psi.copy_(psi)
+                _native.run_record(sol.plan, sol.fields, sol.psi, s0, s1 - s0, box=self.box_int,
+                                   box_out=self.seg_buf)
+            sweep(s0, s1, self.seg_buf)
         return self._chain_rule(acc_cb, acc_ca)
 
     def _chain_rule(self, acc_cb, acc_ca) -> torch.Tensor:
