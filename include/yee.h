@@ -112,9 +112,10 @@ int yee_plan_set_variant(YeePlan* plan, int32_t variant);
  * tiles, so neighbouring tiles share an L2), 2 = XCD-banded (fused
  * interior launch: each z-chunk band split into 8 contiguous shares, one
  * per XCD; other launches use 1 for the two-pass and adjoint kernels, 0
- * for the fused shell), -1 (default) = automatic (1 for the two-pass and
- * adjoint kernels, 0 for the fused kernels).  Results do not depend on
- * it.                                                                    */
+ * for the fused shell), 3..10 = XCD row runs (each XCD takes runs of
+ * order-2 consecutive rows of tiles while the 8 XCDs work on adjacent
+ * runs), -1 (default) = automatic (1 for the two-pass and adjoint kernels,
+ * 3 for the fused kernels).  Results do not depend on it.               */
 int yee_plan_set_tile_order(YeePlan* plan, int32_t xcd_aware);
 /* Replace the palette of the plan (both NULL: per-cell arrays only), e.g.
  * after the cb / ca arrays were changed in place.                        */

@@ -479,7 +479,7 @@ int validate(const YeeGridDesc* d) {
 
 // two-pass kernels: XCD-aware tile order unless plain order is asked for
 static Tiling plan_tiling(const YeePlan* p) {
-  return make_tiling(p->d.nx, p->d.ny, p->d.nz, p->kz, p->xcd != 0);
+  return make_tiling(p->d.nx, p->d.ny, p->d.nz, p->kz, p->xcd);
 }
 
 int launch_h(const YeePlan* p, const GridArgs& g, hipStream_t st) {
@@ -640,7 +640,7 @@ int yee_plan_create(const YeeGridDesc* desc, YeePlan** out) {
   }
   if (const char* e = std::getenv("JAXPHYS_AMD_XCD")) {
     const int v = std::atoi(e);
-    if (v >= -1 && v <= 2) p->xcd = v;
+    if (v >= -1 && v <= 10) p->xcd = v;
   }
   if (const char* e = std::getenv("JAXPHYS_AMD_FUSED_ROWS")) {
     const int r = std::atoi(e);
@@ -666,7 +666,7 @@ int yee_plan_set_variant(YeePlan* p, int32_t variant) {
 
 int yee_plan_set_tile_order(YeePlan* p, int32_t xcd_aware) {
   if (!p) return fail(YEE_EINVAL, "null plan");
-  if (xcd_aware < -1 || xcd_aware > 2) return fail(YEE_EINVAL, "tile order must be -1, 0, 1 or 2");
+  if (xcd_aware < -1 || xcd_aware > 10) return fail(YEE_EINVAL, "tile order must be in -1 .. 10");
   p->xcd = xcd_aware;
   return YEE_OK;
 }

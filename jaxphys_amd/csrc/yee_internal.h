@@ -103,20 +103,27 @@ inline __device__ int slab_index(int a, int n, int p) {
 constexpr int kXcds = 8;
 
 struct Tiling {
-  int gx, gy, ntiles, per;
+  int gx, gy, ntiles, per, run;
 };
 
-inline Tiling make_tiling(int nx, int ny, int nz, int kz, bool xcd) {
+// order: 0 plain, 1 (or -1) XCD-contiguous, >= 3 XCD row runs of
+// (order - 2) rows of tiles (see tileset_finish)
+inline Tiling make_tiling(int nx, int ny, int nz, int kz, int order) {
   Tiling t;
   t.gx = (nx + 63) / 64;
   t.gy = (ny + 3) / 4;
   const int gz = (nz + kz - 1) / kz;
   t.ntiles = t.gx * t.gy * gz;
-  t.per = xcd ? (t.ntiles + kXcds - 1) / kXcds : 0;
+  t.run = order >= 3 ? (order - 2) * t.gx : 0;
+  t.per = (order != 0 && t.run == 0) ? (t.ntiles + kXcds - 1) / kXcds : 0;
   return t;
 }
 
 inline unsigned tiling_blocks(const Tiling& t) {
+  if (t.run > 0) {
+    const int g = kXcds * t.run;
+    return (unsigned)((t.ntiles + g - 1) / g * g);
+  }
   return t.per > 0 ? (unsigned)(t.per * kXcds) : (unsigned)t.ntiles;
 }
 
@@ -138,6 +145,7 @@ struct TileBox {
 struct TileSet {
   int nbox, ntiles, per, kz, rows;
   int band, bq;  // banded XCD order: tiles per z-chunk band, per-XCD share
+  int run;       // XCD row runs: tiles per run (0: not this order)
   TileBox b[kMaxBoxes];
 };
 
@@ -164,10 +172,20 @@ inline void tileset_add(TileSet& ts, int x0, int x1, int y0, int y1, int z0, int
 // 8 contiguous shares, one per XCD, and every XCD moves through the bands
 // at the same pace, so all XCDs stream the same planes while each keeps
 // its neighbouring tiles in its own L2.
+//
+// Orders 3 and up: XCD row runs.  Consecutive groups of 8 * run workgroups
+// cover 8 * run consecutive tiles; within a group XCD x takes the run of
+// `run` tiles starting at x * run, with run = (order - 2) rows of tiles of
+// the first box.  Each XCD then computes whole rows of x-neighbouring tiles
+// (their shared edge columns stay in its L2) while the 8 XCDs stream
+// adjacent runs of memory at the same time.
 inline void tileset_finish(TileSet& ts, int order) {
   ts.per = 0;
   ts.band = ts.bq = 0;
-  if (order == 2 && ts.nbox == 1 && ts.ntiles > 0) {
+  ts.run = 0;
+  if (order >= 3 && ts.ntiles > 0) {
+    ts.run = (order - 2) * (ts.b[0].nxt > 0 ? ts.b[0].nxt : 1);
+  } else if (order == 2 && ts.nbox == 1 && ts.ntiles > 0) {
     ts.band = ts.b[0].nxt * ts.b[0].nyt;
     ts.bq = (ts.band + kXcds - 1) / kXcds;
   } else if (order >= 1) {
@@ -176,6 +194,10 @@ inline void tileset_finish(TileSet& ts, int order) {
 }
 
 inline unsigned tileset_blocks(const TileSet& ts) {
+  if (ts.run > 0) {
+    const int g = kXcds * ts.run;
+    return (unsigned)((ts.ntiles + g - 1) / g * g);
+  }
   if (ts.band > 0) {
     const int nb = (ts.ntiles + ts.band - 1) / ts.band;
     return (unsigned)(nb * ts.bq * kXcds);
@@ -217,7 +239,7 @@ inline void split_tiles(int nx, int ny, int nz, int p, int kz, int lo, int hi, i
     }
   }
   tileset_finish(in, order);
-  tileset_finish(sh, order == 2 ? 0 : order);
+  tileset_finish(sh, order == 2 ? 0 : (order > 2 ? 3 : order));
 }
 
 // Three-way split for the fused step: `in` as in split_tiles (x aligned to
@@ -255,8 +277,8 @@ inline void split_tiles3(int nx, int ny, int nz, int p, int kz, int lo, int hi, 
     tileset_add(shx, 0, nx, 0, ny, 0, nz);
   }
   tileset_finish(in, order);
-  tileset_finish(shn, order == 2 ? 0 : order);
-  tileset_finish(shx, order == 2 ? 0 : order);
+  tileset_finish(shn, order == 2 ? 0 : (order > 2 ? 3 : order));
+  tileset_finish(shx, order == 2 ? 0 : (order > 2 ? 3 : order));
 }
 
 // Position of a workgroup's tile: first column / row / plane, the exclusive
@@ -271,7 +293,10 @@ struct TilePos {
 // Tiling)
 __device__ __forceinline__ bool tile_of_set_at(const TileSet& ts, int L, TilePos& tp) {
   int t;
-  if (ts.band > 0) {
+  if (ts.run > 0) {
+    const int g = kXcds * ts.run;
+    t = (L / g) * g + (L % kXcds) * ts.run + (L / kXcds) % ts.run;
+  } else if (ts.band > 0) {
     const int s = L / kXcds;
     const int loc = (L % kXcds) * ts.bq + s % ts.bq;
     if (loc >= ts.band) return false;
@@ -306,7 +331,13 @@ __device__ __forceinline__ bool tile_of_set(const TileSet& ts, TilePos& tp) {
 // remapped order
 __device__ __forceinline__ bool tile_of(const Tiling& tl, int& bx, int& by, int& bz) {
   const int L = blockIdx.x;
-  const int t = tl.per > 0 ? (L % kXcds) * tl.per + L / kXcds : L;
+  int t;
+  if (tl.run > 0) {
+    const int g = kXcds * tl.run;
+    t = (L / g) * g + (L % kXcds) * tl.run + (L / kXcds) % tl.run;
+  } else {
+    t = tl.per > 0 ? (L % kXcds) * tl.per + L / kXcds : L;
+  }
   if (t >= tl.ntiles) return false;
   bx = t % tl.gx;
   const int r = t / tl.gx;
