@@ -180,7 +180,8 @@ class Plan:
 
     def set_tile_order(self, order) -> None:
         """None: automatic (default); False / 0: plain; True / 1:
-        XCD-contiguous; 2: XCD-banded."""
+        XCD-contiguous; 2: XCD-banded; 3 .. 10: XCD runs of order - 2
+        rows of tiles."""
         v = -1 if order is None else int(order)
         check(lib().yee_plan_set_tile_order(self.handle, v), "yee_plan_set_tile_order")
 

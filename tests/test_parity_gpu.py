@@ -228,7 +228,7 @@ def test_copy_probe(gpu):
 @pytest.mark.parametrize("variant,xcd,rows", [(0, True, 4), (1, True, 4), (2, True, 4),
                                                (3, None, 4), (3, False, 8), (3, True, 16),
                                                (4, None, 4), (4, None, 16), (4, 2, 4),
-                                               (3, 2, 8)])
+                                               (3, 2, 8), (3, 3, 4), (3, 4, 8), (3, 6, 4)])
 @pytest.mark.parametrize("boundary,lossy,size", [
     ("absorbing", False, (70, 37, 29)), ("periodic", False, (70, 37, 29)),
     ("walls", True, (70, 37, 29)), ("cpml", False, (70, 37, 29)), ("cpml", True, (70, 37, 29)),
