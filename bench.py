@@ -548,6 +548,7 @@ def closure_bench(args, dev):
                                  far_field=not args.closure_no_far_field,
                                  val_seeds=list(range(900, 900 + args.closure_val)),
                                  warm_start=args.closure_warm_start,
+                                 net_in_box=args.closure_net_in_box,
                                  log=lambda m: print("[closure]", m, file=sys.stderr, flush=True),
                                  **pkw)
     t_train = time.time() - t0
@@ -579,6 +580,7 @@ def closure_bench(args, dev):
                    "far_field": not args.closure_no_far_field, "far_gain": net.far_gain,
                    "val_seeds": args.closure_val, "net_gain": net.net_gain,
                    "warm_start": args.closure_warm_start,
+                   "net_in_box": net.net_in_box,
                    "onpolicy_from_epoch": args.closure_onpolicy_from,
                    "lr": [args.closure_lr, args.closure_lr_final]},
         "err_coarse_no_closure": round(float(np.mean(err_0)), 5),
@@ -652,6 +654,8 @@ def main() -> None:
                     help="teacher-forced samples start one corrected interval earlier")
     ap.add_argument("--closure-val", type=int, default=2,
                     help="validation seeds (900, 901, ...) that choose the net's gain")
+    ap.add_argument("--closure-net-in-box", action="store_true",
+                    help="the net also corrects the source boxes (on top of the source model)")
     ap.add_argument("--closure-no-far-field", action="store_true",
                     help="no linear far-field operator (net and source model only)")
     ap.add_argument("--closure-no-forcing", action="store_true",

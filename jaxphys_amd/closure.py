@@ -118,6 +118,9 @@ class ClosureNet:
         # scale of the net's correction (select_net_gain picks it on
         # validation seeds after training)
         self.net_gain = 1.0
+        # whether the net also corrects inside the source boxes (on top of
+        # the source model) or only outside them
+        self.net_in_box = False
         self.margin = 0
         self.adam = {k: (torch.zeros_like(v), torch.zeros_like(v)) for k, v in self.params.items()}
         self.step_count = 0
@@ -286,12 +289,22 @@ class ClosureNet:
         else:
             y = self._buf("y", (solver.nx * solver.ny * solver.nz, OUT_STRIDE), torch.float32)
             y.zero_()
+        shape = (solver.nx, solver.ny, solver.nz)
+        box = self._box_terms(solver, shape, (0, 0, 0), every)[0] if self.forcing else None
         if use_far and self.far_gain > 0:
-            self.add_far_field(y, solver)
+            if self.net_in_box and box is not None:
+                # the far-field operator stays out of the source boxes; the
+                # net's correction is kept there
+                far = self._buf("far_c", tuple(y.shape), torch.float32)
+                far.zero_()
+                self.add_far_field(far, solver)
+                far.view(-1, OUT_STRIDE)[box] = 0.0
+                y.add_(far)
+            else:
+                self.add_far_field(y, solver)
         if self.forcing:
-            shape = (solver.nx, solver.ny, solver.nz)
-            box = self._box_terms(solver, shape, (0, 0, 0), every)[0]
-            y.view(-1, OUT_STRIDE)[box] = 0.0
+            if not self.net_in_box:
+                y.view(-1, OUT_STRIDE)[box] = 0.0
             self.add_source_model(y, solver, shape, (0, 0, 0), m, every)
         _conv.apply_correction((solver.nx, solver.ny, solver.nz), solver.fields, y, OUT_STRIDE,
                                self.scale, gain, self.margin)
@@ -373,7 +386,8 @@ class ClosureNet:
                 "net_gain": self.net_gain}
 
     # scalar settings stored beside the tensors of a saved closure
-    _META = ("eps_shift", "eps_scale", "force_scale", "margin", "far_gain", "net_gain")
+    _META = ("eps_shift", "eps_scale", "force_scale", "margin", "far_gain", "net_gain",
+             "net_in_box")
 
     def save(self, path: str, info: dict | None = None) -> None:
         """Write the trained closure (weights, normalisation, the two linear
@@ -414,7 +428,11 @@ class ClosureNet:
         net.ks = tensors["ks"].to(net.device)
         net.far_w = tensors["far_w"].to(net.device)
         for k in cls._META:
-            setattr(net, k, int(meta[k]) if k == "margin" else float(meta[k]))
+            if k not in meta:
+                continue
+            v = meta[k]
+            setattr(net, k, int(v) if k == "margin" else (bool(v) if k == "net_in_box"
+                                                           else float(v)))
         net.adam = {k: (torch.zeros_like(v), torch.zeros_like(v)) for k, v in net.params.items()}
         net.pack()
         net.version += 1
@@ -872,7 +890,7 @@ def train_closure(device, seeds, n_fine: int = 256, n_coarse: int = 400, every: 
                   lr_final: float | None = None, source_crops: float = 0.5,
                   forcing: bool = True, reset_tol: float = 1.0, far_field: bool = True,
                   val_seeds=None, onpolicy_with_net: bool = False, warm_start: bool = False,
-                  **problem_kw):
+                  net_in_box: bool = False, **problem_kw):
     """Train on fine-run/coarse pairs generated by the FDTD solver.
 
     A sample is a coarse state at coarse step m + every and its target, the
@@ -895,9 +913,11 @@ def train_closure(device, seeds, n_fine: int = 256, n_coarse: int = 400, every: 
     With ``far_field`` the linear far-field operator is fitted first
     (``fit_far_field``) and the net learns what remains after it; with
     ``val_seeds`` the net's gain is then chosen on them
-    (``select_net_gain``)."""
+    (``select_net_gain``).  With ``net_in_box`` the net also corrects the
+    source boxes, learning there what the source model leaves."""
     rng = np.random.default_rng(seed)
     net = net or ClosureNet(device, seed=seed, forcing=forcing)
+    net.net_in_box = bool(net_in_box) and net.forcing
     initialised = False
     if far_field:
         prob0 = ClosureProblem(seeds[0], n_fine, device=device, **problem_kw)
@@ -996,8 +1016,21 @@ def train_closure(device, seeds, n_fine: int = 256, n_coarse: int = 400, every: 
                 w[(slice(b, crop - b),) * 3] = 1.0
                 if net.forcing:
                     box, slot, which = net._box_terms(csol, (crop,) * 3, (cx, cy, cz), every)
-                    w.view(-1)[box] = 0.0
-                    y.view(-1, OUT_STRIDE)[box] = 0.0
+                    if net.net_in_box:
+                        # inside the boxes the net learns what the source
+                        # model leaves (the far-field operator is not
+                        # applied there)
+                        sm = net._buf("sm", (crop ** 3, OUT_STRIDE), torch.float32)
+                        sm.zero_()
+                        net.add_source_model(sm, csol, (crop,) * 3, (cx, cy, cz), m, every)
+                        tn = tn.clone()
+                        tnv = tn.view(-1, 6)
+                        if far is not None:
+                            tnv[box] += far[sl].reshape(-1, 6)[box]
+                        tnv[box] -= sm[box, :6]
+                    else:
+                        w.view(-1)[box] = 0.0
+                        y.view(-1, OUT_STRIDE)[box] = 0.0
                     if not onpolicy and not far_field:
                         _accumulate_source_fit(net, csol, every, q, tc, box, slot, which,
                                                (cx, cy, cz), crop, b, ls)
