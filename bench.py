@@ -461,6 +461,31 @@ def closure_net_bench(args, dev):
     torch.cuda.synchronize()
     st_ms = e0.elapsed_time(e1) / K
     st_tflops = 2.0 * 1944 * cells / (st_ms / 1e3) / 1e12
+    # one training step of the net (the trainer's work per sample): forward
+    # keeping the activations, backward (input gradients through the
+    # flipped-weight forward kernel, weight gradients), on a 64^3 crop and
+    # on the whole 128^3 grid
+    train = {}
+    for m in (64, n):
+        shp = (m, m, m)
+        xt = torch.randn(m ** 3, CIN, device=dev).to(torch.bfloat16)
+        xt[:, 7] = 0
+        gy = torch.randn(m ** 3, 8, device=dev) * 1e-3
+        for _ in range(3):
+            net.forward(xt, shp, train=True)
+            net.backward(gy)
+        torch.cuda.synchronize()
+        reps = max(5, K // 5)
+        e0.record(stream)
+        for _ in range(reps):
+            net.forward(xt, shp, train=True)
+            net.backward(gy)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        t_ms = e0.elapsed_time(e1) / reps
+        fl = 3.0 * 2.0 * m ** 3 * 27 * (7 * 32 + 32 * 32 + 32 * 6)
+        train[f"{m}^3"] = {"ms": round(t_ms, 4), "TFLOP/s": round(fl / (t_ms / 1e3) / 1e12, 2),
+                           "frac_of_bf16_peak": round(fl / (t_ms / 1e3) / 1e12 / 2500.0, 4)}
     return {
         "metric": "closure net inference (3 conv3d layers, 128^3)", "value": round(tflops, 2),
         "unit": "TFLOP/s", "higher_is_better": True, "n_gpus": 1, "steps": K,
@@ -471,6 +496,7 @@ def closure_net_bench(args, dev):
         "roofline": {"bound": "mfma", "kernel": "conv_fwd_kernel (3 launches per inference)",
                      "achieved": round(tflops, 2), "peak": 2500.0, "unit": "TFLOP/s",
                      "frac": round(tflops / 2500.0, 4), "traffic": None},
+        "training_step": train,
         "far_field_stencil": {"kernel": "stencil_f32_kernel", "ms": round(st_ms, 4),
                               "TFLOP/s": round(st_tflops, 2),
                               "frac_of_fp32_vector_peak": round(st_tflops / 78.6, 4)},

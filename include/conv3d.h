@@ -48,9 +48,10 @@ int conv3d_fwd(const ConvDesc* d, const void* x, const void* w, const float* bia
                void* z_out, const void* aux, void* y, void* stream);
 
 /* dw[co][k] = sum_p dy[p][co] * x[p + off(tap(k))][c(k)]; db[co] = sum_p dy[p][co].
- * dy: bf16 channels last with channel stride dy_cpad (>= cout).  `work` is a
- * float32 device buffer of conv3d_wgrad_work_floats(d) floats (deterministic
- * two-stage reduction); dw, db are overwritten.                          */
+ * dy: bf16 channels last with channel stride dy_cpad (>= cout, a multiple
+ * of 8); x and dy 16-byte aligned.  `work` is a float32 device buffer of
+ * conv3d_wgrad_work_floats(d) floats (per-block partial sums of dw and db,
+ * then a fixed-order reduction: deterministic); dw, db are overwritten.  */
 int64_t conv3d_wgrad_work_floats(const ConvDesc* d);
 int conv3d_wgrad(const ConvDesc* d, const void* x, const void* dy, int32_t dy_cpad, float* work,
                  float* dw, float* db, void* stream);

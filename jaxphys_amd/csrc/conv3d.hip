@@ -364,6 +364,7 @@ __global__ __launch_bounds__(256) void conv_wgrad_kernel(WgradArgs a) {
 #pragma unroll
     for (int q = 0; q < kMaxTiles; ++q) acc[m][q] = f32x4{0.f, 0.f, 0.f, 0.f};
   const int cin = a.cin, nx = a.nx, ny = a.ny, nz = a.nz;
+  float bsum = 0.f;
   const int s_begin = blockIdx.x * a.segs_per_block;
   const int s_end = min(s_begin + a.segs_per_block, a.total_segs);
   for (int s = s_begin; s < s_end; ++s) {
@@ -371,26 +372,50 @@ __global__ __launch_bounds__(256) void conv_wgrad_kernel(WgradArgs a) {
     const int row = s / a.nseg_x;
     const int yy = row % ny, zz = row / ny;
     __syncthreads();
-    // stage dY^T: dyT[co][p] = dY[pos(sx0 + p)][co]
-    for (int q = threadIdx.x; q < a.cout_pad * kSeg; q += 256) {
-      const int co = q / kSeg, p = q % kSeg;
+    // stage dY^T: dyT[co][p] = dY[pos(sx0 + p)][co]; 16-byte loads of 8
+    // channels of one position (consecutive threads read consecutive
+    // chunks), transposed on the way into LDS
+    const long long rowpos = ((long long)zz * ny + yy) * nx;
+    const int dch = a.dycp / 8;
+    for (int q = threadIdx.x; q < kSeg * dch; q += 256) {
+      const int p = q / dch, c8 = (q - p * dch) * 8;
       const int px = sx0 + p;
-      u16 v = 0;
-      if (px < nx && co < a.dycp) v = a.dy[(((long long)zz * ny + yy) * nx + px) * a.dycp + co];
-      dyT[co * kSeg + p] = v;
+      u16 v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      if (px < nx) {
+        const uint4 raw = *reinterpret_cast<const uint4*>(a.dy + (rowpos + px) * a.dycp + c8);
+        __builtin_memcpy(v, &raw, 16);
+      }
+#pragma unroll
+      for (int e = 0; e < 8; ++e) dyT[(c8 + e) * kSeg + p] = v[e];
     }
-    // stage X^T of the 9 neighbour rows, x from sx0-1 .. sx0+kSeg
-    for (int q = threadIdx.x; q < 9 * cin * xs; q += 256) {
-      const int rr = q / (cin * xs);
-      const int rem = q % (cin * xs);
-      const int c = rem / xs, p = rem % xs;
+    for (int q = threadIdx.x; q < (a.cout_pad - a.dycp) * kSeg; q += 256)
+      dyT[a.dycp * kSeg + q] = 0;
+    // stage X^T of the 9 neighbour rows, x from sx0-1 .. sx0+kSeg (each
+    // row's positions are contiguous in memory: 16-byte loads of 8
+    // channels, transposed into LDS)
+    const int cch = cin / 8;
+    const int per_row = xs * cch;
+    for (int q = threadIdx.x; q < 9 * per_row; q += 256) {
+      const int rr = q / per_row;
+      const int rem = q - rr * per_row;
+      const int p = rem / cch, c8 = (rem - p * cch) * 8;
       const int sz = zz + rr / 3 - 1, sy = yy + rr % 3 - 1, px = sx0 + p - 1;
-      u16 v = 0;
-      if (sz >= 0 && sz < nz && sy >= 0 && sy < ny && px >= 0 && px < nx)
-        v = a.x[(((long long)sz * ny + sy) * nx + px) * cin + c];
-      xT[(rr * cin + c) * xs + p] = v;
+      u16 v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      if (sz >= 0 && sz < nz && sy >= 0 && sy < ny && px >= 0 && px < nx) {
+        const uint4 raw =
+            *reinterpret_cast<const uint4*>(a.x + (((long long)sz * ny + sy) * nx + px) * cin + c8);
+        __builtin_memcpy(v, &raw, 16);
+      }
+#pragma unroll
+      for (int e = 0; e < 8; ++e) xT[(rr * cin + c8 + e) * xs + p] = v[e];
     }
     __syncthreads();
+    // bias gradient: this block's sum of dY over its positions, per channel
+    // (fixed order)
+    if ((int)threadIdx.x < a.cout_pad) {
+      const u16* row = dyT + threadIdx.x * kSeg;
+      for (int p = 0; p < kSeg; ++p) bsum += bf2f(row[p]);
+    }
     bf16x8 af[MT];
 #pragma unroll
     for (int m = 0; m < MT; ++m) af[m] = load8(dyT + (m * 16 + r) * kSeg + 8 * g);
@@ -429,42 +454,29 @@ __global__ __launch_bounds__(256) void conv_wgrad_kernel(WgradArgs a) {
       for (int v = 0; v < 4; ++v)
         out[(long long)(m * 16 + 4 * g + v) * a.kpad + nt * 16 + r] = acc[m][q][v];
   }
+  // bias partials after the gridDim.x weight partials
+  if ((int)threadIdx.x < a.cout_pad)
+    a.part[(long long)gridDim.x * a.cout_pad * a.kpad + (long long)blockIdx.x * a.cout_pad +
+           threadIdx.x] = bsum;
 }
 
-__global__ void wgrad_reduce_kernel(const float* part, int nparts, long long n, float* dw) {
+// out[i] = sum_b part[b * n + i] for i < n_out, in a fixed order: four
+// interleaved partial sums (so that four loads are in flight), then
+// (s0 + s1) + (s2 + s3)
+__global__ void wgrad_reduce_kernel(const float* part, int nparts, long long n, long long n_out,
+                                    float* out) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  float s = 0.f;
-  for (int b = 0; b < nparts; ++b) s += part[(long long)b * n + i];
-  dw[i] = s;
-}
-
-// db[co] = sum_p dY[p][co] (fixed order: per-block partials then a sum)
-__global__ __launch_bounds__(256) void bias_grad_partial(const u16* dy, int dycp, int cout,
-                                                        long long npos, float* part) {
-  __shared__ float sh[256];
-  for (int co = 0; co < cout; ++co) {
-    float s = 0.f;
-    for (long long p = (long long)blockIdx.x * 256 + threadIdx.x; p < npos;
-         p += (long long)gridDim.x * 256)
-      s += bf2f(dy[p * dycp + co]);
-    sh[threadIdx.x] = s;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-      if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) part[blockIdx.x * cout + co] = sh[0];
-    __syncthreads();
+  if (i >= n_out) return;
+  float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+  int b = 0;
+  for (; b + 3 < nparts; b += 4) {
+    s0 += part[(long long)b * n + i];
+    s1 += part[(long long)(b + 1) * n + i];
+    s2 += part[(long long)(b + 2) * n + i];
+    s3 += part[(long long)(b + 3) * n + i];
   }
-}
-
-__global__ void bias_grad_final(const float* part, int nb, int cout, float* db) {
-  const int co = threadIdx.x;
-  if (co >= cout) return;
-  float s = 0.f;
-  for (int b = 0; b < nb; ++b) s += part[b * cout + co];
-  db[co] = s;
+  for (; b < nparts; ++b) s0 += part[(long long)b * n + i];
+  out[i] = (s0 + s1) + (s2 + s3);
 }
 
 __global__ void flip_weights_kernel(const u16* w, int cout, int cout_pad, int cin, int kpad,
@@ -706,15 +718,17 @@ int64_t conv3d_wgrad_work_floats(const ConvDesc* d) {
   int nseg_x, total, spb, nb;
   wgrad_shape(d, nseg_x, total, spb, nb);
   const long long kp = kpad_of(d->cin_pad);
-  return (long long)nb * d->cout_pad * kp + 2048LL * 32;
+  return (long long)nb * d->cout_pad * kp + (long long)nb * d->cout_pad;
 }
 
 int conv3d_wgrad(const ConvDesc* d, const void* x, const void* dy, int32_t dy_cpad, float* work,
                  float* dw, float* db, void* stream) {
   if (!d || !x || !dy || !work || !dw) return fail(-1, "null argument");
   if (d->cin_pad % 8 || d->cout_pad % 16 || d->cout_pad > 32 || dy_cpad > d->cout_pad ||
-      dy_cpad < d->cout)
+      dy_cpad < d->cout || dy_cpad % 8)
     return fail(-1, "unsupported channel configuration");
+  if (((uintptr_t)x | (uintptr_t)dy) % 16)
+    return fail(-1, "x and dy must be 16-byte aligned");
   const int kpad = kpad_of(d->cin_pad);
   if (kpad / 16 > 4 * 16) return fail(-1, "too many input channels for the weight gradient");
   int nseg_x, total, spb, nb;
@@ -742,17 +756,13 @@ int conv3d_wgrad(const ConvDesc* d, const void* x, const void* dy, int32_t dy_cp
   CONV_CHECK(hipGetLastError());
   const long long n = (long long)d->cout_pad * kpad;
   hipLaunchKernelGGL(wgrad_reduce_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
-                     (const float*)work, nb, n, dw);
+                     (const float*)work, nb, n, n, dw);
   CONV_CHECK(hipGetLastError());
   if (db) {
-    float* bpart = work + (long long)nb * n;
-    const long long npos = (long long)d->nx * d->ny * d->nz;
-    const int bb = 2048;
-    hipLaunchKernelGGL(bias_grad_partial, dim3(bb), dim3(256), 0, st, (const u16*)dy, dy_cpad,
-                       d->cout, npos, bpart);
-    CONV_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(bias_grad_final, dim3(1), dim3(64), 0, st, (const float*)bpart, bb, d->cout,
-                       db);
+    // the kernel's per-block bias partials follow its weight partials
+    hipLaunchKernelGGL(wgrad_reduce_kernel, dim3(1), dim3(64), 0, st,
+                       (const float*)(work + (long long)nb * n), nb, (long long)d->cout_pad,
+                       (long long)d->cout, db);
     CONV_CHECK(hipGetLastError());
   }
   return 0;
