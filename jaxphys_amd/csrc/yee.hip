@@ -634,6 +634,7 @@ int yee_plan_create(const YeeGridDesc* desc, YeePlan** out) {
   }
   if (p->adj_kz > desc->nz) p->adj_kz = desc->nz;
   if (const char* e = std::getenv("JAXPHYS_AMD_FUSED_KZ")) p->fused_kz = std::atoi(e);
+  if (const char* e = std::getenv("JAXPHYS_AMD_ADJ_PIPE")) p->adj_pipe = std::atoi(e) != 0;
   if (const char* e = std::getenv("JAXPHYS_AMD_FUSED_VARIANT")) {
     const int v = std::atoi(e);
     if (v >= 0 && v <= 4) p->fused_variant = v;

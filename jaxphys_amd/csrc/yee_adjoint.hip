@@ -449,26 +449,359 @@ __device__ __forceinline__ void adj_e_body(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Software-pipelined bodies of the two adjoint kernels: the same terms, sums
+// and order as adj_h_body / adj_e_body (so the same bits), but every
+// full-width load of plane k+1 is issued before the arithmetic of plane k,
+// as the forward two-pass kernels do, instead of each plane waiting for its
+// own loads.  The E (H) terms are evaluated from loaded values by eterm_v
+// (hterm_v), which perform eterm's (hterm's) arithmetic.
+// ---------------------------------------------------------------------------
+
+template <int MODE, int CPX, int C>
+__device__ __forceinline__ void eterm_v(const AdjArgs& a, float lam, float cb, int i, int j,
+                                        int k, bool valid, float& ga, float& gb) {
+  using T = Terms<C>;
+  const float gU = cb * (adj_damp<MODE>(a, i, j, k) * lam);
+  ga = gU;
+  gb = -gU;
+  const int p = a.p;
+  const int va = coord<T::axa>(i, j, k), na = extent<T::axa>(a);
+  const int vb = coord<T::axb>(i, j, k), nb = extent<T::axb>(a);
+  if (axis_on<CPX, T::axa>() && (va < p || va >= na - p)) {
+    const float* t = adj_tab<T::axa>(a);
+    const float L = *adj_psi<T::ea>(a, i, j, k) + gU;
+    ga = t[2 * na + va] * gU + t[1 * na + va] * L;
+  }
+  if (axis_on<CPX, T::axb>() && (vb < p || vb >= nb - p)) {
+    const float* t = adj_tab<T::axb>(a);
+    const float L = *adj_psi<T::eb>(a, i, j, k) - gU;
+    gb = -t[2 * nb + vb] * gU + t[1 * nb + vb] * L;
+  }
+  if (!(valid && e_upd<MODE, C>(a, i, j, k))) {
+    ga = 0.f;
+    gb = 0.f;
+  }
+}
+
+template <int MODE, int CPX, int C>
+__device__ __forceinline__ void hterm_v(const AdjArgs& a, float lamh, int i, int j, int k,
+                                        bool valid, float& ga, float& gb) {
+  using T = Terms<C>;
+  const float gU = -a.ch * lamh;
+  ga = gU;
+  gb = -gU;
+  const int p = a.p;
+  const int va = coord<T::axa>(i, j, k), na = extent<T::axa>(a);
+  const int vb = coord<T::axb>(i, j, k), nb = extent<T::axb>(a);
+  if (axis_on<CPX, T::axa>() && (va < p || va >= na - p)) {
+    const float* t = adj_tab<T::axa>(a);
+    const float L = *adj_psi<T::ha>(a, i, j, k) + gU;
+    ga = t[5 * na + va] * gU + t[4 * na + va] * L;
+  }
+  if (axis_on<CPX, T::axb>() && (vb < p || vb >= nb - p)) {
+    const float* t = adj_tab<T::axb>(a);
+    const float L = *adj_psi<T::hb>(a, i, j, k) - gU;
+    gb = -t[5 * nb + vb] * gU + t[4 * nb + vb] * L;
+  }
+  if (!(valid && h_upd<MODE, C>(a, i, j, k))) {
+    ga = 0.f;
+    gb = 0.f;
+  }
+}
+
+// loads of one plane of kernel A: E adjoint (and Cb) of the cell's z+1, y+1
+// and (edge lanes) x+1 neighbours and of the cell itself, H adjoint of the
+// cell
+struct AdjHRaw {
+  float ez, ex1, ey1, cb1, ezy, exy, cby, hx, hy, hz, ezx, eyx, cbx;
+};
+
+template <int MODE, int CPX>
+__device__ __forceinline__ void adj_h_body_p(
+    const AdjArgs& a, const TileSet& ts, int L, const float* __restrict__ lex,
+    const float* __restrict__ ley, const float* __restrict__ lez, float* __restrict__ lhx,
+    float* __restrict__ lhy, float* __restrict__ lhz) {
+  TilePos tp;
+  if (!tile_of_set_at(ts, L, tp)) return;
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const int k0 = tp.k0, k1 = tp.k1;
+  int i, xl;
+  adj_column(tp, i, xl);
+  const int j = __builtin_amdgcn_readfirstlane(tp.j0 + (int)(threadIdx.x / kBX));
+  if (i >= xl || j >= tp.yl) return;
+  bool vx, vy;
+  const int ip = nb_idx<MODE>(i + 1, nx, vx);
+  const int jp = nb_idx<MODE>(j + 1, ny, vy);
+  const int lane = (int)(threadIdx.x % kBX);
+  const int seg = tp.pair ? 32 : kBX;
+  const bool xp_lane = (lane % seg) < seg - 1 && ip == i + 1 && i + 1 < xl;
+  const unsigned P = (unsigned)nx * ny;
+  auto load = [&](int k) __attribute__((always_inline)) {
+    AdjHRaw r;
+    bool vz;
+    const int kp = nb_idx<MODE>(k + 1, nz, vz);
+    const unsigned c = ((unsigned)k * ny + j) * nx + i;
+    const unsigned cp = ((unsigned)kp * ny + j) * nx + i;
+    const unsigned cy = ((unsigned)k * ny + jp) * nx + i;
+    r.ez = lez[c];
+    r.ex1 = lex[cp];
+    r.ey1 = ley[cp];
+    r.cb1 = a.cb[cp];
+    r.ezy = lez[cy];
+    r.exy = lex[cy];
+    r.cby = a.cb[cy];
+    r.hx = lhx[c];
+    r.hy = lhy[c];
+    r.hz = lhz[c];
+    r.ezx = r.eyx = r.cbx = 0.f;
+    if (!xp_lane) {
+      const unsigned cx = ((unsigned)k * ny + j) * nx + ip;
+      r.ezx = lez[cx];
+      r.eyx = ley[cx];
+      r.cbx = a.cb[cx];
+    }
+    return r;
+  };
+  float cxa, cxb, cya, cyb;
+  float cbc;
+  {
+    const unsigned c0 = ((unsigned)k0 * ny + j) * nx + i;
+    cbc = a.cb[c0];
+    eterm_v<MODE, CPX, 0>(a, lex[c0], cbc, i, j, k0, true, cxa, cxb);
+    eterm_v<MODE, CPX, 1>(a, ley[c0], cbc, i, j, k0, true, cya, cyb);
+  }
+  AdjHRaw cur = load(k0);
+  for (int k = k0; k < k1; ++k) {
+    const AdjHRaw nxt = load(k + 1 < k1 ? k + 1 : k);
+    bool vz;
+    const int kp = nb_idx<MODE>(k + 1, nz, vz);
+    const unsigned c = ((unsigned)k * ny + j) * nx + i;
+    const float hx0 = cur.hx, hy0 = cur.hy, hz0 = cur.hz;
+    if (CPX > 0) {
+      psi_h_defer<MODE, CPX, 0>(a, hx0, i, j, k);
+      psi_h_defer<MODE, CPX, 1>(a, hy0, i, j, k);
+      psi_h_defer<MODE, CPX, 2>(a, hz0, i, j, k);
+    }
+    const float xa = cxa, xb = cxb, ya = cya, yb = cyb;
+    float za, zb, t0, t1;
+    eterm_v<MODE, CPX, 2>(a, cur.ez, cbc, i, j, k, true, za, zb);
+    eterm_v<MODE, CPX, 0>(a, cur.ex1, cur.cb1, i, j, kp, vz, cxa, cxb);
+    eterm_v<MODE, CPX, 1>(a, cur.ey1, cur.cb1, i, j, kp, vz, cya, cyb);
+    const float sh_za = __shfl_down(za, 1, kBX);
+    const float sh_yb = __shfl_down(yb, 1, kBX);
+    float zpa, ypb;
+    if (xp_lane) {
+      zpa = sh_za;
+      ypb = sh_yb;
+    } else {
+      eterm_v<MODE, CPX, 2>(a, cur.ezx, cur.cbx, ip, j, k, vx, zpa, t1);
+      eterm_v<MODE, CPX, 1>(a, cur.eyx, cur.cbx, ip, j, k, vx, t0, ypb);
+    }
+    float s = hx0;
+    s += ya - cya;
+    eterm_v<MODE, CPX, 2>(a, cur.ezy, cur.cby, i, jp, k, vy, t0, t1);
+    s += zb - t1;
+    lhx[c] = s;
+    s = hy0;
+    s += xb - cxb;
+    s += za - zpa;
+    lhy[c] = s;
+    s = hz0;
+    eterm_v<MODE, CPX, 0>(a, cur.exy, cur.cby, i, jp, k, vy, t0, t1);
+    s += xa - t0;
+    s += yb - ypb;
+    lhz[c] = s;
+    cbc = cur.cb1;
+    cur = nxt;
+  }
+}
+
+// loads of one plane of kernel B: E adjoint, Cb (Ca) and H adjoint of the
+// cell, H adjoint of the y-1 neighbour and (edge lanes) of the x-1
+// neighbour
+struct AdjERaw {
+  float ex, ey, ez, cb, ca, hx, hy, hz, hzm, hxm, hzx, hyx;
+};
+
+template <int MODE, bool LOSSY, int CPX>
+__device__ __forceinline__ void adj_e_body_p(
+    const AdjArgs& a, const TileSet& ts, int L, float* __restrict__ lex,
+    float* __restrict__ ley, float* __restrict__ lez, const float* __restrict__ lhx,
+    const float* __restrict__ lhy, const float* __restrict__ lhz) {
+  TilePos tp;
+  if (!tile_of_set_at(ts, L, tp)) return;
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const int k0 = tp.k0, k1 = tp.k1;
+  int i, xl;
+  adj_column(tp, i, xl);
+  const int j = __builtin_amdgcn_readfirstlane(tp.j0 + (int)(threadIdx.x / kBX));
+  if (i >= xl || j >= tp.yl) return;
+  bool vx, vy;
+  const int im = nb_idx<MODE>(i - 1, nx, vx);
+  const int jm = nb_idx<MODE>(j - 1, ny, vy);
+  const int lane = (int)(threadIdx.x % kBX);
+  const int seg = tp.pair ? 32 : kBX;
+  const bool xm_lane = (lane % seg) > 0 && im == i - 1;
+  auto load = [&](int k) __attribute__((always_inline)) {
+    AdjERaw r;
+    const unsigned c = ((unsigned)k * ny + j) * nx + i;
+    const unsigned cm = ((unsigned)k * ny + jm) * nx + i;
+    r.ex = lex[c];
+    r.ey = ley[c];
+    r.ez = lez[c];
+    r.cb = a.cb[c];
+    r.ca = LOSSY ? a.ca[c] : 1.f;
+    r.hx = lhx[c];
+    r.hy = lhy[c];
+    r.hz = lhz[c];
+    r.hzm = lhz[cm];
+    r.hxm = lhx[cm];
+    r.hzx = r.hyx = 0.f;
+    if (!xm_lane) {
+      const unsigned cx = ((unsigned)k * ny + j) * nx + im;
+      r.hzx = lhz[cx];
+      r.hyx = lhy[cx];
+    }
+    return r;
+  };
+  float pxa, pxb, pya, pyb;
+  {
+    bool vz0;
+    const int km0 = nb_idx<MODE>(k0 - 1, nz, vz0);
+    hterm<MODE, CPX, 0>(a, lhx, i, j, km0, vz0, pxa, pxb);
+    hterm<MODE, CPX, 1>(a, lhy, i, j, km0, vz0, pya, pyb);
+  }
+  AdjERaw cur = load(k0);
+  for (int k = k0; k < k1; ++k) {
+    const AdjERaw nxt = load(k + 1 < k1 ? k + 1 : k);
+    const unsigned c = ((unsigned)k * ny + j) * nx + i;
+    const float d = adj_damp<MODE>(a, i, j, k);
+    const float cb = cur.cb;
+    const float ca = cur.ca;
+    const float g0 = d * cur.ex, g1 = d * cur.ey, g2 = d * cur.ez;
+    float e0 = g0, e1 = g1, e2 = g2;
+    if (e_upd<MODE, 0>(a, i, j, k)) {
+      if (CPX > 0) psi_e_update<CPX, 0>(a, cb * g0, i, j, k);
+      e0 = LOSSY ? ca * g0 : g0;
+    }
+    if (e_upd<MODE, 1>(a, i, j, k)) {
+      if (CPX > 0) psi_e_update<CPX, 1>(a, cb * g1, i, j, k);
+      e1 = LOSSY ? ca * g1 : g1;
+    }
+    if (e_upd<MODE, 2>(a, i, j, k)) {
+      if (CPX > 0) psi_e_update<CPX, 2>(a, cb * g2, i, j, k);
+      e2 = LOSSY ? ca * g2 : g2;
+    }
+    float xa, xb, ya, yb, za, zb, t0, t1;
+    hterm_v<MODE, CPX, 0>(a, cur.hx, i, j, k, true, xa, xb);
+    hterm_v<MODE, CPX, 1>(a, cur.hy, i, j, k, true, ya, yb);
+    hterm_v<MODE, CPX, 2>(a, cur.hz, i, j, k, true, za, zb);
+    const float sh_za = __shfl_up(za, 1, kBX);
+    const float sh_yb = __shfl_up(yb, 1, kBX);
+    float zma, ymb;
+    if (xm_lane) {
+      zma = sh_za;
+      ymb = sh_yb;
+    } else {
+      hterm_v<MODE, CPX, 2>(a, cur.hzx, im, j, k, vx, zma, t1);
+      hterm_v<MODE, CPX, 1>(a, cur.hyx, im, j, k, vx, t0, ymb);
+    }
+    e0 += pya - ya;
+    hterm_v<MODE, CPX, 2>(a, cur.hzm, i, jm, k, vy, t0, t1);
+    e0 += t1 - zb;
+    e1 += pxb - xb;
+    e1 += zma - za;
+    hterm_v<MODE, CPX, 0>(a, cur.hxm, i, jm, k, vy, t0, t1);
+    e2 += t0 - xa;
+    e2 += ymb - yb;
+    pxa = xa;
+    pxb = xb;
+    pya = ya;
+    pyb = yb;
+    if (a.bnx > 0) {
+      const int li = i - a.bx0, lj = j - a.by0, lk = k - a.bz0;
+      if (li >= 0 && li < a.bnx && lj >= 0 && lj < a.bny && lk >= 0 && lk < a.bnz) {
+        const unsigned bc = (unsigned)a.bnx * a.bny * a.bnz;
+        const unsigned l = ((unsigned)lk * a.bny + lj) * a.bnx + li;
+        const float gc[3] = {g0, g1, g2};
+        float s1 = 0.f;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) s1 += gc[q] * a.hist_d[q * bc + l];
+        a.acc_cb[l] += s1;
+        if (LOSSY && a.acc_ca) {
+          float s2 = 0.f;
+#pragma unroll
+          for (int q = 0; q < 3; ++q) s2 += gc[q] * a.hist_e[q * bc + l];
+          a.acc_ca[l] += s2;
+        }
+      }
+    }
+    lex[c] = e0;
+    ley[c] = e1;
+    lez[c] = e2;
+    cur = nxt;
+  }
+  if (a.pnx > 0) {
+    const unsigned pi = (unsigned)(i - a.px0), pj = (unsigned)(j - a.py0);
+    if (pi < (unsigned)a.pnx && pj < (unsigned)a.pny) {
+      const unsigned pc = (unsigned)a.pnx * a.pny * a.pnz;
+      const int kb = max(k0, a.pz0), ke = min(k1, a.pz0 + a.pnz);
+      for (int k = kb; k < ke; ++k) {
+        const unsigned c = ((unsigned)k * ny + j) * nx + i;
+        const unsigned pl = ((unsigned)(k - a.pz0) * a.pny + pj) * a.pnx + pi;
+        const int q0 = a.pmap[pl], q1 = a.pmap[pc + pl], q2 = a.pmap[2 * pc + pl];
+        if (q0 >= 0) lex[c] += a.pscale * (a.pw ? a.pw[q0] : 1.f) * a.pvals[q0];
+        if (q1 >= 0) ley[c] += a.pscale * (a.pw ? a.pw[q1] : 1.f) * a.pvals[q1];
+        if (q2 >= 0) lez[c] += a.pscale * (a.pw ? a.pw[q2] : 1.f) * a.pvals[q2];
+      }
+    }
+  }
+}
+
 // The adjoint kernels are latency bound (short z marches, gathers from
 // neighbour rows): they are built for 8 waves per SIMD.  Of their uniform
 // values the ones beyond that SGPR budget live in VGPR lanes.
 #define YEE_ADJ_OCCUPANCY __attribute__((amdgpu_waves_per_eu(8)))
+// the pipelined bodies hold two planes' loads: at least 6 waves per SIMD,
+// which their registers allow without spills
+#define YEE_ADJ_OCCUPANCY_P __attribute__((amdgpu_waves_per_eu(6)))
 
 // One launch per kernel and tile set (no CPML: the whole grid is one set).
-template <int MODE, int CPX>
+template <int MODE, int CPX, bool PIPE>
 __global__ __launch_bounds__(kThreads) YEE_ADJ_OCCUPANCY void yee_adj_h_kernel(
     AdjArgs a, TileSet ts, const float* __restrict__ lex, const float* __restrict__ ley,
     const float* __restrict__ lez, float* __restrict__ lhx, float* __restrict__ lhy,
     float* __restrict__ lhz) {
-  adj_h_body<MODE, CPX>(a, ts, (int)blockIdx.x, lex, ley, lez, lhx, lhy, lhz);
+  if (PIPE) adj_h_body_p<MODE, CPX>(a, ts, (int)blockIdx.x, lex, ley, lez, lhx, lhy, lhz);
+  else adj_h_body<MODE, CPX>(a, ts, (int)blockIdx.x, lex, ley, lez, lhx, lhy, lhz);
 }
 
-template <int MODE, bool LOSSY, int CPX>
+template <int MODE, bool LOSSY, int CPX, bool PIPE>
 __global__ __launch_bounds__(kThreads) YEE_ADJ_OCCUPANCY void yee_adj_e_kernel(
     AdjArgs a, TileSet ts, float* __restrict__ lex, float* __restrict__ ley,
     float* __restrict__ lez, const float* __restrict__ lhx, const float* __restrict__ lhy,
     const float* __restrict__ lhz) {
-  adj_e_body<MODE, LOSSY, CPX>(a, ts, (int)blockIdx.x, lex, ley, lez, lhx, lhy, lhz);
+  if (PIPE) adj_e_body_p<MODE, LOSSY, CPX>(a, ts, (int)blockIdx.x, lex, ley, lez, lhx, lhy, lhz);
+  else adj_e_body<MODE, LOSSY, CPX>(a, ts, (int)blockIdx.x, lex, ley, lez, lhx, lhy, lhz);
+}
+
+template <int MODE, int CPX, bool PIPE>
+__global__ __launch_bounds__(kThreads) YEE_ADJ_OCCUPANCY_P void yee_adj_hp_kernel(
+    AdjArgs a, TileSet ts, const float* __restrict__ lex, const float* __restrict__ ley,
+    const float* __restrict__ lez, float* __restrict__ lhx, float* __restrict__ lhy,
+    float* __restrict__ lhz) {
+  if (PIPE) adj_h_body_p<MODE, CPX>(a, ts, (int)blockIdx.x, lex, ley, lez, lhx, lhy, lhz);
+  else adj_h_body<MODE, CPX>(a, ts, (int)blockIdx.x, lex, ley, lez, lhx, lhy, lhz);
+}
+
+template <int MODE, bool LOSSY, int CPX, bool PIPE>
+__global__ __launch_bounds__(kThreads) YEE_ADJ_OCCUPANCY_P void yee_adj_ep_kernel(
+    AdjArgs a, TileSet ts, float* __restrict__ lex, float* __restrict__ ley,
+    float* __restrict__ lez, const float* __restrict__ lhx, const float* __restrict__ lhy,
+    const float* __restrict__ lhz) {
+  if (PIPE) adj_e_body_p<MODE, LOSSY, CPX>(a, ts, (int)blockIdx.x, lex, ley, lez, lhx, lhy, lhz);
+  else adj_e_body<MODE, LOSSY, CPX>(a, ts, (int)blockIdx.x, lex, ley, lez, lhx, lhy, lhz);
 }
 
 // CPML: the three tile sets (slab-free interior, y/z-slab shell, x-slab
@@ -481,40 +814,89 @@ struct AdjSets {
   int nb_in, nb_shn;
 };
 
-template <int MODE>
+
+template <int MODE, bool PIPE>
 __global__ __launch_bounds__(kThreads) YEE_ADJ_OCCUPANCY void yee_adj_h3_kernel(
     AdjArgs a, AdjSets sets, const float* __restrict__ lex, const float* __restrict__ ley,
     const float* __restrict__ lez, float* __restrict__ lhx, float* __restrict__ lhy,
     float* __restrict__ lhz) {
   int L = (int)blockIdx.x;
   if (L < sets.nb_in) {
-    adj_h_body<MODE, 0>(a, sets.in, L, lex, ley, lez, lhx, lhy, lhz);
+    if (PIPE) adj_h_body_p<MODE, 0>(a, sets.in, L, lex, ley, lez, lhx, lhy, lhz);
+    else adj_h_body<MODE, 0>(a, sets.in, L, lex, ley, lez, lhx, lhy, lhz);
     return;
   }
   L -= sets.nb_in;
   if (L < sets.nb_shn) {
-    adj_h_body<MODE, 1>(a, sets.shn, L, lex, ley, lez, lhx, lhy, lhz);
+    if (PIPE) adj_h_body_p<MODE, 1>(a, sets.shn, L, lex, ley, lez, lhx, lhy, lhz);
+    else adj_h_body<MODE, 1>(a, sets.shn, L, lex, ley, lez, lhx, lhy, lhz);
     return;
   }
-  adj_h_body<MODE, 2>(a, sets.shx, L - sets.nb_shn, lex, ley, lez, lhx, lhy, lhz);
+  if (PIPE) adj_h_body_p<MODE, 2>(a, sets.shx, L - sets.nb_shn, lex, ley, lez, lhx, lhy, lhz);
+  else adj_h_body<MODE, 2>(a, sets.shx, L - sets.nb_shn, lex, ley, lez, lhx, lhy, lhz);
 }
 
-template <int MODE, bool LOSSY>
+template <int MODE, bool LOSSY, bool PIPE>
 __global__ __launch_bounds__(kThreads) YEE_ADJ_OCCUPANCY void yee_adj_e3_kernel(
     AdjArgs a, AdjSets sets, float* __restrict__ lex, float* __restrict__ ley,
     float* __restrict__ lez, const float* __restrict__ lhx, const float* __restrict__ lhy,
     const float* __restrict__ lhz) {
   int L = (int)blockIdx.x;
   if (L < sets.nb_in) {
-    adj_e_body<MODE, LOSSY, 0>(a, sets.in, L, lex, ley, lez, lhx, lhy, lhz);
+    if (PIPE) adj_e_body_p<MODE, LOSSY, 0>(a, sets.in, L, lex, ley, lez, lhx, lhy, lhz);
+    else adj_e_body<MODE, LOSSY, 0>(a, sets.in, L, lex, ley, lez, lhx, lhy, lhz);
     return;
   }
   L -= sets.nb_in;
   if (L < sets.nb_shn) {
-    adj_e_body<MODE, LOSSY, 1>(a, sets.shn, L, lex, ley, lez, lhx, lhy, lhz);
+    if (PIPE) adj_e_body_p<MODE, LOSSY, 1>(a, sets.shn, L, lex, ley, lez, lhx, lhy, lhz);
+    else adj_e_body<MODE, LOSSY, 1>(a, sets.shn, L, lex, ley, lez, lhx, lhy, lhz);
     return;
   }
-  adj_e_body<MODE, LOSSY, 2>(a, sets.shx, L - sets.nb_shn, lex, ley, lez, lhx, lhy, lhz);
+  if (PIPE) adj_e_body_p<MODE, LOSSY, 2>(a, sets.shx, L - sets.nb_shn, lex, ley, lez, lhx, lhy, lhz);
+  else adj_e_body<MODE, LOSSY, 2>(a, sets.shx, L - sets.nb_shn, lex, ley, lez, lhx, lhy, lhz);
+}
+
+template <int MODE, bool PIPE>
+__global__ __launch_bounds__(kThreads) YEE_ADJ_OCCUPANCY_P void yee_adj_h3p_kernel(
+    AdjArgs a, AdjSets sets, const float* __restrict__ lex, const float* __restrict__ ley,
+    const float* __restrict__ lez, float* __restrict__ lhx, float* __restrict__ lhy,
+    float* __restrict__ lhz) {
+  int L = (int)blockIdx.x;
+  if (L < sets.nb_in) {
+    if (PIPE) adj_h_body_p<MODE, 0>(a, sets.in, L, lex, ley, lez, lhx, lhy, lhz);
+    else adj_h_body<MODE, 0>(a, sets.in, L, lex, ley, lez, lhx, lhy, lhz);
+    return;
+  }
+  L -= sets.nb_in;
+  if (L < sets.nb_shn) {
+    if (PIPE) adj_h_body_p<MODE, 1>(a, sets.shn, L, lex, ley, lez, lhx, lhy, lhz);
+    else adj_h_body<MODE, 1>(a, sets.shn, L, lex, ley, lez, lhx, lhy, lhz);
+    return;
+  }
+  if (PIPE) adj_h_body_p<MODE, 2>(a, sets.shx, L - sets.nb_shn, lex, ley, lez, lhx, lhy, lhz);
+  else adj_h_body<MODE, 2>(a, sets.shx, L - sets.nb_shn, lex, ley, lez, lhx, lhy, lhz);
+}
+
+template <int MODE, bool LOSSY, bool PIPE>
+__global__ __launch_bounds__(kThreads) YEE_ADJ_OCCUPANCY_P void yee_adj_e3p_kernel(
+    AdjArgs a, AdjSets sets, float* __restrict__ lex, float* __restrict__ ley,
+    float* __restrict__ lez, const float* __restrict__ lhx, const float* __restrict__ lhy,
+    const float* __restrict__ lhz) {
+  int L = (int)blockIdx.x;
+  if (L < sets.nb_in) {
+    if (PIPE) adj_e_body_p<MODE, LOSSY, 0>(a, sets.in, L, lex, ley, lez, lhx, lhy, lhz);
+    else adj_e_body<MODE, LOSSY, 0>(a, sets.in, L, lex, ley, lez, lhx, lhy, lhz);
+    return;
+  }
+  L -= sets.nb_in;
+  if (L < sets.nb_shn) {
+    if (PIPE) adj_e_body_p<MODE, LOSSY, 1>(a, sets.shn, L, lex, ley, lez, lhx, lhy, lhz);
+    else adj_e_body<MODE, LOSSY, 1>(a, sets.shn, L, lex, ley, lez, lhx, lhy, lhz);
+    return;
+  }
+  if (PIPE) adj_e_body_p<MODE, LOSSY, 2>(a, sets.shx, L - sets.nb_shn, lex, ley, lez, lhx, lhy, lhz);
+  else adj_e_body<MODE, LOSSY, 2>(a, sets.shx, L - sets.nb_shn, lex, ley, lez, lhx, lhy, lhz);
 }
 
 __global__ void probe_gather_kernel(GridArgs g, const int* comp, const long long* idx, int n,
@@ -648,7 +1030,7 @@ int adjoint_args(const YeePlan* p, float* const* lam, float* lam_psi, const YeeD
 
 // Both kernels of one adjoint step, each as two launches under CPML: tiles
 // with no slab within one cell (no psi code) and the slab shell.
-template <int MODE, bool LOSSY>
+template <int MODE, bool LOSSY, bool PIPE>
 void adjoint_launch_t(const YeePlan* p, const AdjArgs& a, float* const* lam, hipStream_t st) {
   TileSet in, shn, shx;
   const int order = p->xcd < 0 ? 1 : p->xcd;
@@ -678,26 +1060,45 @@ void adjoint_launch_t(const YeePlan* p, const AdjArgs& a, float* const* lam, hip
     sets.nb_shn = nblocks(shn);
     const int total = sets.nb_in + sets.nb_shn + nblocks(shx);
     if (total > 0) {
-      yee_adj_h3_kernel<MODE><<<total, kThreads, 0, st>>>(a, sets, lam[0], lam[1], lam[2],
-                                                          lam[3], lam[4], lam[5]);
-      yee_adj_e3_kernel<MODE, LOSSY><<<total, kThreads, 0, st>>>(a, sets, lam[0], lam[1],
-                                                                 lam[2], lam[3], lam[4], lam[5]);
+      if constexpr (PIPE) {
+        yee_adj_h3p_kernel<MODE, true><<<total, kThreads, 0, st>>>(a, sets, lam[0], lam[1],
+                                                                   lam[2], lam[3], lam[4], lam[5]);
+        yee_adj_e3p_kernel<MODE, LOSSY, true><<<total, kThreads, 0, st>>>(
+            a, sets, lam[0], lam[1], lam[2], lam[3], lam[4], lam[5]);
+      } else {
+        yee_adj_h3_kernel<MODE, false><<<total, kThreads, 0, st>>>(a, sets, lam[0], lam[1],
+                                                                   lam[2], lam[3], lam[4], lam[5]);
+        yee_adj_e3_kernel<MODE, LOSSY, false><<<total, kThreads, 0, st>>>(
+            a, sets, lam[0], lam[1], lam[2], lam[3], lam[4], lam[5]);
+      }
     }
   } else {
     if (in.ntiles > 0) {
-      yee_adj_h_kernel<MODE, 0><<<tileset_blocks(in), kThreads, 0, st>>>(
-          a, in, lam[0], lam[1], lam[2], lam[3], lam[4], lam[5]);
-      yee_adj_e_kernel<MODE, LOSSY, 0><<<tileset_blocks(in), kThreads, 0, st>>>(
-          a, in, lam[0], lam[1], lam[2], lam[3], lam[4], lam[5]);
+      if constexpr (PIPE) {
+        yee_adj_hp_kernel<MODE, 0, true><<<tileset_blocks(in), kThreads, 0, st>>>(
+            a, in, lam[0], lam[1], lam[2], lam[3], lam[4], lam[5]);
+        yee_adj_ep_kernel<MODE, LOSSY, 0, true><<<tileset_blocks(in), kThreads, 0, st>>>(
+            a, in, lam[0], lam[1], lam[2], lam[3], lam[4], lam[5]);
+      } else {
+        yee_adj_h_kernel<MODE, 0, false><<<tileset_blocks(in), kThreads, 0, st>>>(
+            a, in, lam[0], lam[1], lam[2], lam[3], lam[4], lam[5]);
+        yee_adj_e_kernel<MODE, LOSSY, 0, false><<<tileset_blocks(in), kThreads, 0, st>>>(
+            a, in, lam[0], lam[1], lam[2], lam[3], lam[4], lam[5]);
+      }
     }
   }
 }
 
 int adjoint_launch(const YeePlan* p, const AdjArgs& a, float* const* lam, hipStream_t st) {
   const bool lossy = p->d.ca != nullptr;
-#define YEE_ADJ(M)                                          \
-  if (lossy) adjoint_launch_t<M, true>(p, a, lam, st);     \
-  else adjoint_launch_t<M, false>(p, a, lam, st);
+#define YEE_ADJ(M)                                                       \
+  if (p->adj_pipe) {                                                     \
+    if (lossy) adjoint_launch_t<M, true, true>(p, a, lam, st);           \
+    else adjoint_launch_t<M, false, true>(p, a, lam, st);                \
+  } else {                                                               \
+    if (lossy) adjoint_launch_t<M, true, false>(p, a, lam, st);          \
+    else adjoint_launch_t<M, false, false>(p, a, lam, st);               \
+  }
   switch (p->d.boundary) {
     case YEE_BOUNDARY_PERIODIC: YEE_ADJ(kPeriodic) break;
     case YEE_BOUNDARY_CPML: YEE_ADJ(kCpml) break;

@@ -354,11 +354,13 @@ __device__ __forceinline__ bool tile_of(const Tiling& tl, int& bx, int& by, int&
 struct YeePlan {
   YeeGridDesc d;
   int kz;            // z planes per block of the two-pass kernels
-  int adj_kz = 3;    // z planes per block of the adjoint kernels (latency
-                     // bound: short marches, many wavefronts; on the 160^3
-                     // design workload 1-6 planes give 3.12, 2.88, 2.80,
-                     // 3.02, 3.13, 3.03x one forward run)
+  int adj_kz = 4;    // z planes per block of the adjoint kernels (160^3
+                     // design workload, software-pipelined bodies: 2, 3,
+                     // 4, 6 planes give 3.67, 3.62, 3.60, 3.72x one
+                     // forward run in the checkpointed mode)
   int fused_kz = 0;  // z planes per block of the fused kernel (0: automatic)
+  int adj_pipe = 1;  // adjoint kernels: software-pipelined bodies (1) or
+                     // loads issued per plane (0); same results
   int fused_variant = 3;  // 0: LDS + halo row, 1-3: barrier-free (see yee_fused.hip)
   int fused_rows = 4;     // y rows (wavefronts) per tile of variants 2 and 3: 4, 8 or 16
   int xcd = -1;      // tile order: 0 plain, 1 XCD-contiguous, 2 XCD-banded, -1 automatic

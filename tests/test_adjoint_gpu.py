@@ -280,3 +280,26 @@ def test_objective_backward_after_another_forward_raises(gpu):
     J1.backward()
     _, ref = prob.value_and_grad(e1.detach())
     assert torch.equal(e1.grad.to(ref.dtype).cpu(), ref.cpu())
+
+
+@pytest.mark.parametrize("boundary,lossy,size", [
+    ("cpml", False, (22, 20, 18)), ("cpml", True, (76, 26, 22)),
+    ("absorbing", False, (22, 20, 18)), ("walls", True, (22, 20, 18))])
+def test_pipelined_adjoint_kernels_equal_plain(gpu, boundary, lossy, size, monkeypatch):
+    """The software-pipelined adjoint bodies (default) and the plain ones
+    (JAXPHYS_AMD_ADJ_PIPE=0) perform the same arithmetic in the same order:
+    the same J and gradient bit for bit."""
+    npml, n = 4, 60
+    eps, sigma, box, tab, src, mon = _setup(boundary, lossy, size, npml, n)
+    x0, y0, z0, bx, by, bz = box
+    out = []
+    for pipe in ("1", "0"):
+        monkeypatch.setenv("JAXPHYS_AMD_ADJ_PIPE", pipe)
+        g = _gpu_grid(size, boundary, npml, eps, sigma, tab, src)
+        prob = AdjointProblem(g, n, DT, box, [("ez", p) for _, p in mon], checkpoint_every=13)
+        eps_box = torch.tensor(eps[x0:x0 + bx, y0:y0 + by, z0:z0 + bz], device=gpu)
+        out.append(prob.value_and_grad(eps_box))
+    (Ja, ga), (Jb, gb) = out
+    assert Ja == Jb
+    assert torch.equal(ga, gb), float((ga - gb).abs().max())
+    assert float(ga.abs().max()) > 0
