@@ -562,6 +562,8 @@ def closure_bench(args, dev):
            "waveform": args.closure_waveform}
     if args.closure_load:
         net, hist = CL.ClosureNet.load(args.closure_load, dev), []
+        net.source_patch = args.closure_source == "patch"
+        net.patch_half = args.closure_patch_half
     else:
         net, hist = CL.train_closure(dev, train_seeds, n_fine=nf, n_coarse=nf * 400 // 256,
                                  every=args.closure_every, epochs=args.closure_epochs,
@@ -575,6 +577,8 @@ def closure_bench(args, dev):
                                  val_seeds=list(range(900, 900 + args.closure_val)),
                                  warm_start=args.closure_warm_start,
                                  net_in_box=args.closure_net_in_box,
+                                 source_patch=args.closure_source == "patch",
+                                 patch_half=args.closure_patch_half,
                                  log=lambda m: print("[closure]", m, file=sys.stderr, flush=True),
                                  **pkw)
     t_train = time.time() - t0
@@ -607,6 +611,8 @@ def closure_bench(args, dev):
                    "val_seeds": args.closure_val, "net_gain": net.net_gain,
                    "warm_start": args.closure_warm_start,
                    "net_in_box": net.net_in_box,
+                   "source_correction": "near-source patch" if net.source_patch else
+                   "linear source model", "patch_half": net.patch_half,
                    "onpolicy_from_epoch": args.closure_onpolicy_from,
                    "lr": [args.closure_lr, args.closure_lr_final]},
         "err_coarse_no_closure": round(float(np.mean(err_0)), 5),
@@ -682,6 +688,11 @@ def main() -> None:
                     help="validation seeds (900, 901, ...) that choose the net's gain")
     ap.add_argument("--closure-net-in-box", action="store_true",
                     help="the net also corrects the source boxes (on top of the source model)")
+    ap.add_argument("--closure-source", default="patch", choices=["patch", "linear"],
+                    help="correction of the source boxes: the near-source patch (local "
+                    "fine/coarse grids) or the least-squares linear source model")
+    ap.add_argument("--closure-patch-half", type=int, default=16,
+                    help="half side (coarse cells) of the near-source patch's local grids")
     ap.add_argument("--closure-no-far-field", action="store_true",
                     help="no linear far-field operator (net and source model only)")
     ap.add_argument("--closure-no-forcing", action="store_true",

@@ -53,6 +53,7 @@ H^{2m-3/2} and H^{2m-1/2}.
 
 from __future__ import annotations
 
+import dataclasses
 import math
 import time
 
@@ -63,6 +64,7 @@ import torch.nn.functional as F
 from jaxphys_amd import _conv
 from jaxphys_amd.em.constants import C0
 from jaxphys_amd.em.fdtd3d import DielectricRegion, EMGrid3D, PointSource3D
+from jaxphys_amd.exceptions import ConfigurationError
 
 CIN, HID, COUT = 8, 32, 6          # padded input channels (7 real), hidden, output
 KP1, KP2 = _conv.kpad_of(CIN), _conv.kpad_of(HID)
@@ -79,6 +81,7 @@ FAR_RIDGE = 3e-3        # ridge, relative to the mean diagonal of the normal mat
 FAR_GAIN = 0.75         # share of the fitted correction applied per interval
 FAR_SAMPLES = 60000     # cells sampled per training interval
 FAR_GROWTH = 1.1        # largest accepted energy ratio of the stability check
+PATCH_HALF = 16         # half side (coarse cells) of the near-source patch's local grids
 
 
 def _bf16(t: torch.Tensor) -> torch.Tensor:
@@ -121,6 +124,11 @@ class ClosureNet:
         # whether the net also corrects inside the source boxes (on top of
         # the source model) or only outside them
         self.net_in_box = False
+        # the source boxes' correction: the near-source patch computed on
+        # local fine/coarse grids (True, attach_patch), or the linear source
+        # model ks (False)
+        self.source_patch = True
+        self.patch_half = PATCH_HALF
         self.margin = 0
         self.adam = {k: (torch.zeros_like(v), torch.zeros_like(v)) for k, v in self.params.items()}
         self.step_count = 0
@@ -302,12 +310,22 @@ class ClosureNet:
                 y.add_(far)
             else:
                 self.add_far_field(y, solver)
+        patch = None
         if self.forcing:
             if not self.net_in_box:
                 y.view(-1, OUT_STRIDE)[box] = 0.0
-            self.add_source_model(y, solver, shape, (0, 0, 0), m, every)
+            if self.source_patch:
+                patch = getattr(solver, "_closure_patch", None)
+                if patch is None and box.numel() > 0:
+                    raise RuntimeError("this closure corrects the source boxes with the "
+                                       "near-source patch: call closure.attach_patch(solver, "
+                                       "grid, n_steps, every) first")
+            else:
+                self.add_source_model(y, solver, shape, (0, 0, 0), m, every)
         _conv.apply_correction((solver.nx, solver.ny, solver.nz), solver.fields, y, OUT_STRIDE,
                                self.scale, gain, self.margin)
+        if patch is not None:
+            patch.apply(solver, m // every, gain)
 
     def pack_zero(self, solver) -> torch.Tensor:
         """Network input of the zero state of the solver's grid (eps_r only):
@@ -387,7 +405,7 @@ class ClosureNet:
 
     # scalar settings stored beside the tensors of a saved closure
     _META = ("eps_shift", "eps_scale", "force_scale", "margin", "far_gain", "net_gain",
-             "net_in_box")
+             "net_in_box", "source_patch", "patch_half")
 
     def save(self, path: str, info: dict | None = None) -> None:
         """Write the trained closure (weights, normalisation, the two linear
@@ -431,8 +449,8 @@ class ClosureNet:
             if k not in meta:
                 continue
             v = meta[k]
-            setattr(net, k, int(v) if k == "margin" else (bool(v) if k == "net_in_box"
-                                                           else float(v)))
+            setattr(net, k, int(v) if k in ("margin", "patch_half") else
+                    (bool(v) if k in ("net_in_box", "source_patch") else float(v)))
         net.adam = {k: (torch.zeros_like(v), torch.zeros_like(v)) for k, v in net.params.items()}
         net.pack()
         net.version += 1
@@ -585,21 +603,176 @@ class ClosureProblem:
         coarse states at coarse step counts 0, every, 2*every, ..., as lists
         of six (nz/2, ny/2, nx/2) tensors, and the fine solver."""
         g = self.fine_grid()
-        nf = 2 * n_coarse_steps
-        sol = g.build_solver(self.dt_fine(), nf + 2)
+        sol = g.build_solver(self.dt_fine(), 2 * n_coarse_steps + 2)
         shape = (self.nc,) * 3
         states = [[torch.zeros(shape, device=sol.device) for _ in range(6)]]
-        done = 0  # fine steps run so far: the state is (E^done, H^{done-1/2})
-        for m in range(every, n_coarse_steps + 1, every):
-            sol.run(done, 2 * m - 1 - done)          # -> H^{2m-3/2}
-            h_prev = restrict_h(sol.fields[3:])
-            sol.run(2 * m - 1, 1)                    # -> E^{2m}, H^{2m-1/2}
-            e = restrict_e(sol.fields[:3])
-            h = restrict_h(sol.fields[3:])
-            states.append([t.contiguous() for t in e] +
-                          [(0.5 * (a + b)).contiguous() for a, b in zip(h_prev, h)])
-            done = 2 * m
+        states += list(restricted_states(sol, n_coarse_steps, every))
         return states, sol
+
+
+def restricted_states(sol, n_coarse_steps: int, every: int):
+    """Advance a fine solver (from step 0) and yield the restricted coarse
+    state at coarse step counts every, 2*every, ... <= n_coarse_steps: six
+    tensors, E^{2m} restricted and the mean of the restricted H^{2m-3/2} and
+    H^{2m-1/2} (the module docstring's staggering in time)."""
+    done = 0  # fine steps run so far: the state is (E^done, H^{done-1/2})
+    for m in range(every, n_coarse_steps + 1, every):
+        sol.run(done, 2 * m - 1 - done)          # -> H^{2m-3/2}
+        h_prev = restrict_h(sol.fields[3:])
+        sol.run(2 * m - 1, 1)                    # -> E^{2m}, H^{2m-1/2}
+        e = restrict_e(sol.fields[:3])
+        h = restrict_h(sol.fields[3:])
+        yield ([t.contiguous() for t in e] +
+               [(0.5 * (a + b)).contiguous() for a, b in zip(h_prev, h)])
+        done = 2 * m
+
+
+# ---------------------------------------------------------------------------
+# near-source patch
+# ---------------------------------------------------------------------------
+
+_AXIS = {"x": 0, "y": 1, "z": 2}
+
+
+def _fine_twin(s: PointSource3D, pos_c) -> list:
+    """The fine-grid (dx/2, dt/2) sources that inject what the coarse source
+    ``s`` injects at coarse position ``pos_c``: two fine cells along the
+    driven component's axis (the coarse node sits between them), twice the
+    amplitude each (the same dipole moment per unit time: (dt_c / dt_f)
+    (dx_f / dx_c)^3 = 1/4 of the two cells' sum), the pulse's centre, width
+    and carrier wavelength in fine steps / cells."""
+    wf = s.waveform
+    if wf is not None and not isinstance(wf, str):
+        raise ConfigurationError("the near-source patch needs a named waveform "
+                                 "(an array of per-step values has no fine-step samples)")
+    # fdtd3d.py:239-244: "z" -> Ez, "x" -> Ex, anything else -> Ey
+    a = _AXIS.get(s.polarization, 1)
+    out = []
+    for d in (0, 1):
+        p = [2 * int(v) for v in pos_c]
+        p[a] += d
+        out.append(dataclasses.replace(
+            s, position=tuple(p), amplitude=2.0 * s.amplitude, t0=2.0 * s.t0,
+            spread=2.0 * s.spread,
+            wavelength=None if s.wavelength is None else 2.0 * s.wavelength))
+    return out
+
+
+class NearSourcePatch:
+    """Corrections of the (2R+1)^3 box around each point-source cell of a
+    coarse grid, one per correction interval, computed on a local
+    fine/coarse pair of grids instead of fitted.
+
+    Next to a point source the coarse grid misrepresents the field by as
+    much as the field itself (the source's own near field and the static
+    remnant of a pulse whose increments do not sum to zero), and how it does
+    depends on the permittivity around the source.  For each source cell a
+    local coarse grid of side 2 ``half`` around it and its 2x-refined twin
+    (the same permittivity repeated 2x2x2, the same source injected as
+    ``_fine_twin`` does, CPML of the coarse grid's thickness and twice it)
+    run side by side: after every interval the local coarse state in the
+    box is replaced by the restricted fine state, and the difference is the
+    interval's correction.  The global coarse run adds the same corrections
+    to the same cells (``apply``), so in the box it follows the local fine
+    run, which for the fields the source itself makes is the fine run.
+    Corrections of separate sources add (the field equations are linear);
+    each source's local grids hold only the sources at its own cell."""
+
+    def __init__(self, grid: EMGrid3D, dt: float, n_coarse: int, every: int,
+                 half: int = PATCH_HALF, radius: int = STENCIL // 2, device=None):
+        from jaxphys_amd import _native
+        if grid._boundary != "cpml":
+            raise ConfigurationError("the near-source patch needs a cpml grid")
+        dev = torch.device(device or grid._device or _native.require_device())
+        npml = int(grid._pml_layers)
+        n_loc = 2 * half
+        if n_loc - 2 * npml < 2 * radius + 1:
+            raise ConfigurationError(
+                f"near-source patch: local grid {n_loc} with {npml}-cell CPML leaves no room "
+                f"for the {2 * radius + 1}-cell box")
+        eps, sigma = grid._build_media(dev)           # internal (z, y, x)
+        size = grid.size
+        cells: dict = {}
+        for s in grid._sources:
+            cells.setdefault(tuple(int(v) for v in s.position), []).append(s)
+        self.radius, self.half, self.every = radius, half, every
+        self.boxes = []      # (z, y, x) slices of the global grid per source cell
+        self.corr = []       # (n_intervals, 6, 2R+1, 2R+1, 2R+1) per source cell
+        cpml = dict(pml_R0=grid._cpml["R0"], pml_grading=grid._cpml["grading"],
+                    pml_kappa_max=grid._cpml["kappa_max"], pml_alpha_max=grid._cpml["alpha_max"])
+        for pos, srcs in cells.items():
+            o = [min(max(p - half, 0), n - n_loc) for p, n in zip(pos, size)]
+            if min(o) < 0:
+                raise ConfigurationError(f"near-source patch: grid {size} is smaller than "
+                                         f"its {n_loc}-cell local grid")
+            loc = [p - a for p, a in zip(pos, o)]
+            if any(v - radius < npml or v + radius >= n_loc - npml for v in loc):
+                raise ConfigurationError(
+                    f"near-source patch: source at {pos} is too close to the grid's faces")
+            sl = (slice(o[2], o[2] + n_loc), slice(o[1], o[1] + n_loc), slice(o[0], o[0] + n_loc))
+            eps_c = eps[sl].permute(2, 1, 0).contiguous()      # (x, y, z)
+            sig_c = None if sigma is None else sigma[sl].permute(2, 1, 0).contiguous()
+
+            def rep(t):
+                return t.repeat_interleave(2, 0).repeat_interleave(2, 1).repeat_interleave(2, 2)
+
+            gc = EMGrid3D((n_loc,) * 3, grid._dx, "cpml", npml, device=dev, **cpml)
+            gf = EMGrid3D((2 * n_loc,) * 3, 0.5 * grid._dx, "cpml", 2 * npml, device=dev, **cpml)
+            ones = np.ones((n_loc,) * 3, bool)
+            gc.add_material(DielectricRegion(mask=ones, epsilon_r=eps_c, sigma=sig_c))
+            gf.add_material(DielectricRegion(
+                mask=np.ones((2 * n_loc,) * 3, bool), epsilon_r=rep(eps_c),
+                sigma=None if sig_c is None else rep(sig_c)))
+            for s in srcs:
+                gc.add_source(dataclasses.replace(s, position=tuple(loc)))
+                for t in _fine_twin(s, loc):
+                    gf.add_source(t)
+            csol = gc.build_solver(dt, n_coarse + every + 1)
+            fsol = gf.build_solver(0.5 * dt, 2 * n_coarse + 2)
+            bl = tuple(slice(v - radius, v + radius + 1) for v in (loc[2], loc[1], loc[0]))
+            out = []
+            for q, st in enumerate(restricted_states(fsol, n_coarse, every)):
+                csol.run(q * every, every)
+                c = torch.stack([st[k][bl] - csol.fields[k][bl] for k in range(6)])
+                for k in range(6):
+                    csol.fields[k][bl] += c[k]
+                out.append(c)
+            del csol, fsol
+            self.corr.append(torch.stack(out) if out else
+                             torch.zeros((0, 6) + (2 * radius + 1,) * 3, device=dev))
+            self.boxes.append(tuple(slice(v - radius, v + radius + 1)
+                                    for v in (pos[2], pos[1], pos[0])))
+
+    def apply(self, solver, q: int, gain: float = 1.0) -> None:
+        """fields += gain * correction of interval q in every source box."""
+        for box, c in zip(self.boxes, self.corr):
+            if q < c.shape[0]:
+                for k in range(6):
+                    solver.fields[k][box].add_(c[q, k], alpha=gain)
+
+
+def attach_patch(solver, grid: EMGrid3D, n_coarse: int, every: int,
+                 half: int = PATCH_HALF) -> NearSourcePatch:
+    """Compute the near-source patch of ``grid`` for ``n_coarse`` steps of
+    ``solver`` (built from that grid) and keep it on the solver, where
+    ``ClosureNet.correct`` uses it in place of the linear source model."""
+    patch = NearSourcePatch(grid, solver.dt, n_coarse, every, half=half, device=solver.device)
+    solver._closure_patch = patch
+    return patch
+
+
+def coarse_solver(prob: "ClosureProblem", net, n_steps: int, every: int, patch=None):
+    """The coarse solver of ``prob`` for ``n_steps`` steps; with the
+    near-source patch attached when ``net`` corrects its source boxes with
+    one (``patch``: one computed before for the same problem)."""
+    g = prob.coarse_grid()
+    sol = g.build_solver(prob.dt_coarse(), n_steps + every + 1)
+    if net is not None and net.forcing and net.source_patch:
+        if patch is None:
+            attach_patch(sol, g, n_steps, every, half=net.patch_half)
+        else:
+            sol._closure_patch = patch
+    return sol
 
 
 def rel_err(a: list, b: list, comps=(0, 1, 2), margin: int = 0) -> float:
@@ -619,7 +792,7 @@ def coarse_rollout(prob: ClosureProblem, net: ClosureNet | None, n_coarse: int, 
                    init=None, gain: float = 1.0):
     """Coarse run of n_coarse steps with the correction every ``every`` steps
     (net=None: plain coarse FDTD).  Returns the final coarse solver."""
-    sol = prob.coarse_grid().build_solver(prob.dt_coarse(), n_coarse + 1)
+    sol = coarse_solver(prob, net, n_coarse, every)
     if init is not None:
         for f, v in zip(sol.fields, init):
             f.copy_(v)
@@ -863,10 +1036,12 @@ def select_net_gain(net: ClosureNet, device, seeds, n_fine: int, n_coarse: int, 
     for sd in seeds:
         prob = ClosureProblem(sd, n_fine, device=device, **problem_kw)
         states, _ = prob.fine_states(n_coarse, every)
+        patch = None
         for g in gains:
             net.net_gain = g
             net.pack()
-            csol = prob.coarse_grid().build_solver(prob.dt_coarse(), n_coarse + 1)
+            csol = coarse_solver(prob, net, n_coarse, every, patch)
+            patch = getattr(csol, "_closure_patch", None)
             num = den = 0.0
             for q, m in enumerate(range(0, n_coarse, every)):
                 csol.run(m, every)
@@ -890,7 +1065,8 @@ def train_closure(device, seeds, n_fine: int = 256, n_coarse: int = 400, every: 
                   lr_final: float | None = None, source_crops: float = 0.5,
                   forcing: bool = True, reset_tol: float = 1.0, far_field: bool = True,
                   val_seeds=None, onpolicy_with_net: bool = False, warm_start: bool = False,
-                  net_in_box: bool = False, **problem_kw):
+                  net_in_box: bool = False, source_patch: bool = True,
+                  patch_half: int = PATCH_HALF, **problem_kw):
     """Train on fine-run/coarse pairs generated by the FDTD solver.
 
     A sample is a coarse state at coarse step m + every and its target, the
@@ -918,6 +1094,7 @@ def train_closure(device, seeds, n_fine: int = 256, n_coarse: int = 400, every: 
     rng = np.random.default_rng(seed)
     net = net or ClosureNet(device, seed=seed, forcing=forcing)
     net.net_in_box = bool(net_in_box) and net.forcing
+    net.source_patch, net.patch_half = bool(source_patch), int(patch_half)
     initialised = False
     if far_field:
         prob0 = ClosureProblem(seeds[0], n_fine, device=device, **problem_kw)
@@ -941,7 +1118,10 @@ def train_closure(device, seeds, n_fine: int = 256, n_coarse: int = 400, every: 
         for s_i, sd in enumerate(seeds):
             prob = ClosureProblem(sd, n_fine, device=device, **problem_kw)
             states, _ = prob.fine_states(n_coarse, every)
-            csol = prob.coarse_grid().build_solver(prob.dt_coarse(), n_coarse + every + 1)
+            if onpolicy or warm_start:
+                csol = coarse_solver(prob, net, n_coarse, every)
+            else:
+                csol = prob.coarse_grid().build_solver(prob.dt_coarse(), n_coarse + every + 1)
             if not initialised:
                 _init_net(net, prob, states, csol, every)
                 initialised = True
@@ -1108,13 +1288,22 @@ def evaluate_closure(net: ClosureNet, seed: int, n_fine: int = 256, n_coarse: in
         # what the net would do at full gain (validation switched it off)
         runs.append(("net_gain1", net, True, True))
     saved_gain = net.net_gain
+    patch = None
     for label, model, use_net, use_far in runs:
         net.net_gain = 1.0 if label == "net_gain1" else saved_gain
-        csol = prob.coarse_grid().build_solver(prob.dt_coarse(), n_coarse + 1)
+        grid = prob.coarse_grid()
+        csol = grid.build_solver(prob.dt_coarse(), n_coarse + every + 1)
         if model is not None:
             model.pack()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
+        if model is not None and model.forcing and model.source_patch:
+            # the patch is part of the closure's cost: computed inside the
+            # first timed run, reused by the others
+            if patch is None:
+                patch = attach_patch(csol, grid, n_coarse, every, half=model.patch_half)
+            else:
+                csol._closure_patch = patch
         for m in range(0, n_coarse, every):
             csol.run(m, every)
             if model is not None:

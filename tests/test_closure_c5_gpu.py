@@ -2,7 +2,9 @@
 weights-only) on the C5 configuration at its own size: fine 256^3 (dx=1)
 against coarse 128^3 (dx=2), 400 coarse steps, random dielectric boxes and
 a Gaussian-pulse dipole, on held-out seeds that neither training (0-63) nor
-validation (900-901) used.  The fine FDTD run is the judge of the error."""
+validation (900-901) used.  The fine FDTD run is the judge of the error.
+Seed 1005 puts the source inside an eps = 1.6 box next to its face, the case
+a source model fitted on sources in vacuum got wrong."""
 
 import os
 
@@ -15,7 +17,7 @@ pytestmark = pytest.mark.gpu
 MODEL = os.path.join(os.path.dirname(CL.__file__), "data", "closure_c5.safetensors")
 
 
-@pytest.mark.parametrize("seed", [1000, 1007])
+@pytest.mark.parametrize("seed", [1000, 1005])
 def test_committed_closure_on_held_out_seed(gpu, seed):
     net = CL.ClosureNet.load(MODEL, gpu)
     info = CL.ClosureNet.info(MODEL)
@@ -29,7 +31,10 @@ def test_committed_closure_on_held_out_seed(gpu, seed):
     assert r["err_closure"] <= 0.02
     assert r["err_closure"] < 0.1 * r["err_coarse"]
     # away from the source cells the closure also beats plain coarse FDTD
+    # (measured: 1.7 % and 3.1 % against 11.6 % and 11.4 %)
     assert r["err_far_closure"] < 0.5 * r["err_far_coarse"]
+    assert r["err_far_closure"] <= 0.04
     assert r["err_traj_closure"] <= 0.02
-    # wall time: measured 9-10x faster than the fine grid
+    # wall time, the near-source patch's local runs included: measured 5x
+    # faster than the fine grid
     assert r["speedup"] >= 4.0

@@ -113,3 +113,59 @@ def test_closure_save_load_roundtrip(tmp_path):
     assert torch.equal(net.scale, back.scale)
     assert (back.far_gain, back.net_gain, back.margin, back.force_scale) == (0.75, 0.25, 12, 2.5)
     assert back.forcing and ClosureNet.info(path) == {"train_seeds": 64}
+
+
+def test_fine_twin_matches_the_fine_grid_sources():
+    """The near-source patch's fine sources for the coarse dipole are the
+    fine grid's own sources of the closure problem: two cells along the
+    driven axis, twice the amplitude, the pulse in fine steps."""
+    from jaxphys_amd.closure import ClosureProblem, _fine_twin
+
+    prob = ClosureProblem(7, 64, npml_fine=8)
+    (src_c,) = prob.coarse_grid()._sources
+    twin = _fine_twin(src_c, src_c.position)
+    fine = prob.fine_grid()._sources
+    assert [(s.position, s.amplitude, s.polarization, s.waveform, s.t0, s.spread)
+            for s in twin] == \
+        [(s.position, s.amplitude, s.polarization, s.waveform, s.t0, s.spread) for s in fine]
+
+
+def test_fine_twin_axes_and_errors():
+    import pytest
+
+    from jaxphys_amd.closure import _fine_twin
+    from jaxphys_amd.em.fdtd3d import PointSource3D
+    from jaxphys_amd.exceptions import ConfigurationError
+
+    s = PointSource3D(1e9, (3, 4, 5), 0.5, "x", waveform="gaussian_sine", t0=10, spread=3,
+                      wavelength=8.0)
+    a, b = _fine_twin(s, (3, 4, 5))
+    assert (a.position, b.position) == ((6, 8, 10), (7, 8, 10))
+    assert a.amplitude == 1.0 and a.t0 == 20.0 and a.spread == 6.0 and a.wavelength == 16.0
+    # anything but "x" / "z" drives Ey (fdtd3d.py:239-244)
+    a, b = _fine_twin(PointSource3D(1e9, (1, 1, 1), polarization="q"), (1, 1, 1))
+    assert (a.position, b.position) == ((2, 2, 2), (2, 3, 2))
+    with pytest.raises(ConfigurationError):
+        _fine_twin(PointSource3D(1e9, (1, 1, 1), waveform=[0.0, 1.0]), (1, 1, 1))
+
+
+def test_near_source_patch_needs_cpml():
+    import pytest
+
+    from jaxphys_amd.closure import NearSourcePatch
+    from jaxphys_amd.em.fdtd3d import EMGrid3D
+    from jaxphys_amd.exceptions import ConfigurationError
+
+    with pytest.raises(ConfigurationError):
+        NearSourcePatch(EMGrid3D((32, 32, 32), 2.0, "absorbing", 4), 1e-9, 10, 10)
+
+
+def test_saved_closure_keeps_the_source_correction_mode(tmp_path):
+    from jaxphys_amd.closure import ClosureNet
+
+    net = ClosureNet("cpu", seed=0)
+    net.source_patch, net.patch_half = False, 20
+    path = str(tmp_path / "c.safetensors")
+    net.save(path)
+    back = ClosureNet.load(path, "cpu")
+    assert back.source_patch is False and back.patch_half == 20

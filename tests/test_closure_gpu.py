@@ -73,3 +73,57 @@ def test_far_field_operator_reduces_held_out_residual(gpu):
     print(f"[closure] held-out far residual with the operator: {after / before:.3f} of without",
           flush=True)
     assert after < 0.97 * before
+
+
+def test_near_source_patch_reproduces_fine_run_in_source_box(gpu):
+    """When the global grids are no larger than the patch's local grids
+    (coarse 32^3 = 2 x half 16, the same CPML), the local pair is the global
+    pair: the coarse run corrected by the patch alone then equals the
+    restricted fine run in the source box after every interval (to the
+    rounding of one addition), while the plain coarse run does not."""
+    n_fine, n_coarse, every, kw = 64, 100, 10, dict(npml_fine=8)
+    net = CL.ClosureNet(gpu, seed=0)
+    net.far_gain, net.net_gain = 0.0, 0.0
+    prob = CL.ClosureProblem(3, n_fine, device=gpu, **kw)
+    states, _ = prob.fine_states(n_coarse, every)
+    sol = CL.coarse_solver(prob, net, n_coarse, every)
+    plain = prob.coarse_grid().build_solver(prob.dt_coarse(), n_coarse + every + 1)
+    patch = sol._closure_patch
+    assert len(patch.boxes) == 1 and patch.corr[0].shape[0] == n_coarse // every
+    box = patch.boxes[0]
+    worst = worst_plain = 0.0
+    for q, m in enumerate(range(0, n_coarse, every)):
+        sol.run(m, every)
+        net.correct(sol, 1.0, m, every, use_net=False, use_far=False)
+        plain.run(m, every)
+        for c in range(6):
+            ref = states[q + 1][c][box]
+            scale = float(ref.abs().max()) + 1e-30
+            worst = max(worst, float((sol.fields[c][box] - ref).abs().max()) / scale)
+            worst_plain = max(worst_plain, float((plain.fields[c][box] - ref).abs().max()) / scale)
+    print(f"[closure] patch box vs restricted fine: {worst:.2e} (plain coarse {worst_plain:.2e})",
+          flush=True)
+    assert worst <= 1e-5
+    assert worst_plain > 0.1
+
+
+def test_near_source_patch_beats_plain_coarse_far_from_source(gpu):
+    """With the near-source patch (and nothing else) the coarse run's field
+    error away from the source (more than 3 cells) on held-out seeds is
+    well below the plain coarse run's, at a size where the local grids are
+    smaller than the domain (coarse 48^3, local 32^3)."""
+    n_fine, n_coarse, every, kw = 96, 120, 10, dict(npml_fine=8)
+    net = CL.ClosureNet(gpu, seed=0)
+    net.far_gain, net.net_gain = 0.0, 0.0
+    for seed in (1000, 1005):
+        prob = CL.ClosureProblem(seed, n_fine, device=gpu, **kw)
+        states, _ = prob.fine_states(n_coarse, every)
+        far = CL.far_mask(prob.nc, prob.npml_c, prob.src_c, 3.0, gpu)
+        errs = {}
+        for label, model in (("plain", None), ("patch", net)):
+            sol = CL.coarse_rollout(prob, model, n_coarse, every)
+            a, b = CL._err_terms(sol.fields, states[-1], prob.npml_c, far)
+            errs[label] = (a / b) ** 0.5
+        print(f"[closure] seed {seed} far-field error: patch {errs['patch']:.4f}, "
+              f"plain {errs['plain']:.4f}", flush=True)
+        assert errs["patch"] < 0.5 * errs["plain"]
