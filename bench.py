@@ -564,6 +564,8 @@ def closure_bench(args, dev):
         net, hist = CL.ClosureNet.load(args.closure_load, dev), []
         net.source_patch = args.closure_source == "patch"
         net.patch_half = args.closure_patch_half
+        if args.closure_patch_tail is not None:
+            net.patch_tail = args.closure_patch_tail
     else:
         net, hist = CL.train_closure(dev, train_seeds, n_fine=nf, n_coarse=nf * 400 // 256,
                                  every=args.closure_every, epochs=args.closure_epochs,
@@ -579,6 +581,8 @@ def closure_bench(args, dev):
                                  net_in_box=args.closure_net_in_box,
                                  source_patch=args.closure_source == "patch",
                                  patch_half=args.closure_patch_half,
+                                 patch_tail=(CL.PATCH_TAIL if args.closure_patch_tail is None
+                                             else args.closure_patch_tail),
                                  log=lambda m: print("[closure]", m, file=sys.stderr, flush=True),
                                  **pkw)
     t_train = time.time() - t0
@@ -613,6 +617,7 @@ def closure_bench(args, dev):
                    "net_in_box": net.net_in_box,
                    "source_correction": "near-source patch" if net.source_patch else
                    "linear source model", "patch_half": net.patch_half,
+                   "patch_tail": net.patch_tail,
                    "onpolicy_from_epoch": args.closure_onpolicy_from,
                    "lr": [args.closure_lr, args.closure_lr_final]},
         "err_coarse_no_closure": round(float(np.mean(err_0)), 5),
@@ -693,6 +698,9 @@ def main() -> None:
                     "fine/coarse grids) or the least-squares linear source model")
     ap.add_argument("--closure-patch-half", type=int, default=16,
                     help="half side (coarse cells) of the near-source patch's local grids")
+    ap.add_argument("--closure-patch-tail", type=int, default=None,
+                    help="near-source patch: intervals computed after the source's last "
+                    "(-1: all; default: the closure's own setting)")
     ap.add_argument("--closure-no-far-field", action="store_true",
                     help="no linear far-field operator (net and source model only)")
     ap.add_argument("--closure-no-forcing", action="store_true",
