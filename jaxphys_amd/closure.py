@@ -2,13 +2,17 @@
 correction every few steps that reproduces fine-grid fields.  No reference
 counterpart (the reference has no neural-network code); the FDTD solver
 itself generates the training data and stays the fallback and the judge of
-the error.  The correction has three parts, all fitted to the solver's own
-fine runs: a linear source model in a box around each source, a float32
+the error.  The correction has three parts: in the 15^3 box around each
+source, the near-source patch (``NearSourcePatch``: per-interval
+corrections computed on a local fine/coarse pair of grids around the
+source; or, with ``source_patch=False``, the linear source model below,
+fitted to the solver's own fine runs); outside those boxes a float32
 linear far-field operator (a 27-tap stencil of the fields, fitted by
 least squares, which corrects the coarse grid's dispersion away from the
-source; ``fit_far_field``), and a BF16 3D conv net trained on what those
-two leave, whose gain validation seeds choose (``select_net_gain``; zero
-when it does not lower their error by more than 1 %).
+source; ``fit_far_field``); and a BF16 3D conv net trained on what the
+far-field operator leaves, whose gain validation seeds choose
+(``select_net_gain``; zero when it does not lower their error by more
+than 1 %).
 
 Network (``ClosureNet``): channels 7 -> 32 -> 32 -> 6 (E, H normalised and
 eps_r in; a field correction out), 3x3x3 zero-padded convolutions, GELU
@@ -20,8 +24,9 @@ charge behind, and the fine- and coarse-grid near fields of that charge
 differ by a fixed local pattern times the injected amount, which the net can
 only place where it knows a source acted.  That part of the correction is
 linear in the forcing and as large as the fields themselves next to the
-source, far outside the range the bf16 GELU layers resolve, so it is a
-separate float32 linear source model: in the 15x15x15 box around each
+source, far outside the range the bf16 GELU layers resolve, so it is left
+to the near-source patch, or to a separate float32 linear source model: in
+the 15x15x15 box around each
 source cell, correction = K0 f0 + K1 f1 + K2 f0 (eps_s - 1), with f0 the
 interval's injected E increment, f1 its first time moment and eps_s the
 permittivity at the source cell (the sources add to E itself, so in a
