@@ -121,6 +121,14 @@ def test_conv_wgrad_and_dgrad_match_autograd(gpu):
 
 
 def test_closure_net_backward_matches_autograd(gpu):
+    """The whole net (three layers, forward and backward on the MFMA
+    kernels) against float64 autograd on the same bf16-rounded weights and
+    input.  The kernels keep activations and the output gradient in bf16
+    (2^-9 relative rounding per value) where the reference keeps float64,
+    so the two differ by that rounding carried through the layers:
+    measured 3.7e-3 (output) and 3e-4 .. 1.9e-3 (gradients, largest for w1,
+    two layers of rounded activations below the loss); bounds 8e-3 and
+    5e-3."""
     shape = (24, 20, 12)
     nx, ny, nz = shape
     net = ClosureNet(gpu, seed=3, out_scale=1.0, forcing=False)
@@ -150,7 +158,8 @@ def test_closure_net_backward_matches_autograd(gpu):
     loss = ((out - target[:, :COUT].double()) ** 2).mean()
     loss.backward()
     rel_y = float((y[:, :COUT].double().cpu() - out.detach()).norm() / out.detach().norm())
-    assert rel_y < 2e-2, rel_y
+    print(f"[conv] net output rel {rel_y:.2e}", flush=True)
+    assert rel_y < 8e-3, rel_y
     for k in ("w1", "b1", "w2", "b2", "w3", "b3"):
         ref = P[k].grad
         if k == "w1":
@@ -158,7 +167,7 @@ def test_closure_net_backward_matches_autograd(gpu):
             ref[:, :, 7:] = 0
         rel = float((grads[k].double().cpu() - ref).norm() / ref.norm())
         print(f"[conv] grad {k} rel {rel:.2e}", flush=True)
-        assert rel < 5e-2, (k, rel)
+        assert rel < 5e-3, (k, rel)
 
 
 def test_closure_training_reduces_loss(gpu):
