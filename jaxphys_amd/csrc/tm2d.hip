@@ -25,6 +25,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
 #include <string>
 
 #include "tm2d.h"
@@ -56,7 +57,23 @@ struct StepArgs {
   int n_src;
   const int* rows;
   const float* vals;  // the source values of this step, n_src floats
+  // tile order: workgroup L -> tile t; gx tiles per tile row, run > 0 gives
+  // each XCD runs of `run` x-consecutive tiles (see tile_of)
+  int gx, ntiles, run;
 };
+
+constexpr int kXcds = 8;  // MI355X: 8 XCDs, workgroups dealt round-robin
+
+// Tile of workgroup L.  With run > 0 every group of 8 * run consecutive
+// workgroups covers 8 * run consecutive tiles and XCD (L % 8) takes the
+// run-long stretch (L % 8) * run .. of them: with run = gx / 8 each XCD
+// holds the same eighth of every tile row, so a tile's x neighbours and
+// (one group later) its y neighbours are read from the XCD's own L2.
+__device__ __forceinline__ int tile_of(const StepArgs& g, int L) {
+  if (g.run <= 0) return L;
+  const int grp = kXcds * g.run;
+  return (L / grp) * grp + (L % kXcds) * g.run + (L / kXcds) % g.run;
+}
 
 template <int MODE>
 __device__ __forceinline__ int remap(int i, int n) {
@@ -162,9 +179,11 @@ __global__ __launch_bounds__(kTx * kTy) void tm2d_step_kernel(
   constexpr bool kPer = MODE == TM2D_PERIODIC;
   const int nx = g.nx, ny = g.ny;
   const int lane = threadIdx.x;
-  const int x0 = blockIdx.x * kTx;
+  const int t = tile_of(g, (int)blockIdx.x);
+  if (t >= g.ntiles) return;  // padding workgroups of the tile order
+  const int x0 = (t % g.gx) * kTx;
   const int i = x0 + lane;
-  const int j0 = (blockIdx.y * kTy + threadIdx.y) * kRows;
+  const int j0 = ((t / g.gx) * kTy + threadIdx.y) * kRows;
   const bool in_x_band = i < g.npml || i >= nx - g.npml;
   const bool fast = x0 + kTx <= nx && j0 >= 1 && j0 + kRows <= ny - 1;
   if (!fast) {
@@ -249,6 +268,16 @@ static int validate(const Tm2dDesc* d) {
   return TM2D_OK;
 }
 
+// run length of the tile order: an eighth of a tile row (0, the plain
+// order, for grids under 8 tiles wide); TM2D_TILE_RUN overrides it
+static int tile_run(int gx) {
+  if (const char* e = std::getenv("TM2D_TILE_RUN")) {
+    const int v = std::atoi(e);
+    if (v >= 0 && v <= 4096) return v;
+  }
+  return gx / kXcds;
+}
+
 static int launch(const Tm2dDesc* d, float* const* in, float* const* out, int n,
                   hipStream_t stream) {
   StepArgs g;
@@ -263,7 +292,12 @@ static int launch(const Tm2dDesc* d, float* const* in, float* const* out, int n,
   g.rows = d->src_rows;
   g.vals = d->n_src > 0 ? d->src_table + (long long)n * d->n_src : nullptr;
   const dim3 block(kTx, kTy);
-  const dim3 grid((d->nx + kTx - 1) / kTx, (d->ny + kTy * kRows - 1) / (kTy * kRows));
+  g.gx = (d->nx + kTx - 1) / kTx;
+  const int gy = (d->ny + kTy * kRows - 1) / (kTy * kRows);
+  g.ntiles = g.gx * gy;
+  g.run = tile_run(g.gx);
+  const int grp = g.run > 0 ? kXcds * g.run : 1;
+  const dim3 grid((unsigned)((g.ntiles + grp - 1) / grp * grp));
   switch (d->boundary) {
     case TM2D_ABSORBING:
       hipLaunchKernelGGL(tm2d_step_kernel<TM2D_ABSORBING>, grid, block, 0, stream, g, in[0],

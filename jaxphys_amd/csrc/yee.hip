@@ -643,6 +643,10 @@ int yee_plan_create(const YeeGridDesc* desc, YeePlan** out) {
     const int v = std::atoi(e);
     if (v >= -1 && v <= 10) p->xcd = v;
   }
+  if (const char* e = std::getenv("JAXPHYS_AMD_XCD_SHELL")) {
+    const int v = std::atoi(e);
+    if (v >= -1 && v <= 10) p->xcd_shell = v;
+  }
   if (const char* e = std::getenv("JAXPHYS_AMD_FUSED_ROWS")) {
     const int r = std::atoi(e);
     if (r == 4 || r == 8 || r == 16) p->fused_rows = r;

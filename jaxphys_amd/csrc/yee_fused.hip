@@ -1259,7 +1259,8 @@ void launch_v2_rows(const FusedArgs& a, const YeePlan* pl, hipStream_t st) {
   // (the plane below the first is recomputed with full CPML handling)
   TileSet in, sh, shx;
   if constexpr (VAR >= 3) {
-    split_tiles3(a.nx, a.ny, a.nz, MODE == kCpml ? a.p : 0, kz, 1, 0, order, in, sh, shx, ROWS);
+    split_tiles3(a.nx, a.ny, a.nz, MODE == kCpml ? a.p : 0, kz, 1, 0, order, in, sh, shx, ROWS,
+                 0, 0, true, pl->xcd_shell);
   } else {
     split_tiles(a.nx, a.ny, a.nz, MODE == kCpml ? a.p : 0, kz, 1, 0, 0, 0, order, in, sh, ROWS);
     std::memset(&shx, 0, sizeof(shx));

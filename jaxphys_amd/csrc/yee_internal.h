@@ -250,7 +250,7 @@ inline void split_tiles(int nx, int ny, int nz, int p, int kz, int lo, int hi, i
 // whole grid goes to `shx`.
 inline void split_tiles3(int nx, int ny, int nz, int p, int kz, int lo, int hi, int order,
                          TileSet& in, TileSet& shn, TileSet& shx, int rows, int zlo = 0,
-                         int zhi = 0, bool pair_x = true) {
+                         int zhi = 0, bool pair_x = true, int shell_order = -1) {
   std::memset(&in, 0, sizeof(in));
   std::memset(&shn, 0, sizeof(shn));
   std::memset(&shx, 0, sizeof(shx));
@@ -277,8 +277,9 @@ inline void split_tiles3(int nx, int ny, int nz, int p, int kz, int lo, int hi, 
     tileset_add(shx, 0, nx, 0, ny, 0, nz);
   }
   tileset_finish(in, order);
-  tileset_finish(shn, order == 2 ? 0 : (order > 2 ? 3 : order));
-  tileset_finish(shx, order == 2 ? 0 : (order > 2 ? 3 : order));
+  const int so = shell_order >= 0 ? shell_order : (order == 2 ? 0 : (order > 2 ? 3 : order));
+  tileset_finish(shn, so);
+  tileset_finish(shx, so);
 }
 
 // Position of a workgroup's tile: first column / row / plane, the exclusive
@@ -364,6 +365,7 @@ struct YeePlan {
   int fused_variant = 3;  // 0: LDS + halo row, 1-3: barrier-free (see yee_fused.hip)
   int fused_rows = 4;     // y rows (wavefronts) per tile of variants 2 and 3: 4, 8 or 16
   int xcd = -1;      // tile order: 0 plain, 1 XCD-contiguous, 2 XCD-banded, -1 automatic
+  int xcd_shell = -1;  // tile order of the fused CPML shell launches (-1: from xcd)
   // sources
   int nsrc = 0;
   int nsteps_table = 0;
