@@ -56,6 +56,14 @@ __device__ __forceinline__ const float* tab_of(const FusedArgs& a, int ax) {
   return ax == 0 ? a.cpml : (ax == 1 ? a.cpml + 6LL * a.nx : a.cpml + 6LL * (a.nx + a.ny));
 }
 
+// Stores of a step's outputs (fields and psi) are non-temporal: each value
+// is written once and read again only by the next step, after the whole
+// grid has streamed through the caches, so keeping it there only evicts
+// the input lines that neighbouring tiles still read.  Measured on the
+// 512^3 CPML workload: 1.911 -> 1.779 ms per fused step (interior launch
+// 1134 -> 1049 us), the same bits.
+__device__ __forceinline__ void put(float* p, float v) { __builtin_nontemporal_store(v, p); }
+
 // psi' = b psi + c d (psi from A, written to B when `write`); returns d/kappa + psi'
 __device__ __forceinline__ float cterm(const FusedArgs& a, int slot, unsigned off, bool write,
                                        int ax, int n, int v, int kind, float d) {
@@ -65,7 +73,7 @@ __device__ __forceinline__ float cterm(const FusedArgs& a, int slot, unsigned of
   const float kinv = t[(3 * kind + 2) * n + v];
   const unsigned o = (unsigned)slot_base(a, slot) + off;
   const float ps = b * a.pin[o] + c * d;
-  if (write) a.pout[o] = ps;
+  if (write) put(&a.pout[o], ps);
   return d * kinv + ps;
 }
 
@@ -383,7 +391,7 @@ __device__ __forceinline__ float v2_psi(const float* __restrict__ pin, float* __
   const float kinv = tab[(3 * kind + 2) * n + v];
   const unsigned o = base + (in ? off : 0u);
   const float ps = b * pin[o] + c * d;
-  if (wr && in) pout[o] = ps;
+  if (wr && in) put(&pout[o], ps);
   return in ? d * kinv + ps : d;
 }
 
@@ -508,9 +516,9 @@ __device__ __forceinline__ void v2_plane(
     if (YS) v = v2_psi(pin, pout, by0 + kHzy * ysl, offy(q.sym, q.i), q.ysm, false, ty, ny, q.jm, 1, v);
     hz_ym = q.xinH ? hz_ym0 - ch * (u - v) : hz_ym0;
   }
-  oHx[c] = hx;
-  oHy[c] = hy;
-  oHz[c] = hz;
+  put(&oHx[c], hx);
+  put(&oHy[c], hy);
+  put(&oHz[c], hz);
 
   // ---- E^{n+1} of the column (fdtd3d.py:332-359)
   float vx = ex, vy = ey, vz = ez;
@@ -545,9 +553,9 @@ __device__ __forceinline__ void v2_plane(
     vy *= d;
     vz *= d;
   }
-  oEx[c] = vx;
-  oEy[c] = vy;
-  oEz[c] = vz;
+  put(&oEx[c], vx);
+  put(&oEy[c], vy);
+  put(&oEz[c], vz);
 
   r.ex = ex_zp;
   r.ey = ey_zp;
@@ -736,9 +744,9 @@ __device__ __forceinline__ void v3_compute(
     if (YS) v = v2_psi(pin, pout, by0 + kHzy * ysl, offy(q.sym, q.i), q.ysm, false, ty, ny, q.jm, 1, v);
     hz_ym = q.xinH ? hz_ym0 - ch * (u - v) : hz_ym0;
   }
-  oHx[c] = hx;
-  oHy[c] = hy;
-  oHz[c] = hz;
+  put(&oHx[c], hx);
+  put(&oHy[c], hy);
+  put(&oHz[c], hz);
 
   // ---- E^{n+1} of the column (fdtd3d.py:332-359)
   float vx, vy, vz;
@@ -773,9 +781,9 @@ __device__ __forceinline__ void v3_compute(
     vy *= d;
     vz *= d;
   }
-  oEx[c] = vx;
-  oEy[c] = vy;
-  oEz[c] = vz;
+  put(&oEx[c], vx);
+  put(&oEy[c], vy);
+  put(&oEz[c], vz);
 
   r.ex = ex_zp;
   r.ey = ey_zp;
@@ -951,11 +959,11 @@ __device__ __forceinline__ void v3s_plane(
     float u = ez_yp - ez, v = ey_zp - ey;
     if (q.ys) {
       u = pterm(ty[3 * ny + q.j], ty[4 * ny + q.j], ty[5 * ny + q.j], YEE_PSI(eHxy, kHxy * ysl + oy), u, ps);
-      if (upd) pout[kHxy * ysl + oy] = ps;
+      if (upd) put(&pout[kHxy * ysl + oy], ps);
     }
     if (zs) {
       v = pterm(tz[3 * nz + k], tz[4 * nz + k], tz[5 * nz + k], YEE_PSI(eHxz, bz0 + (kHxz - 4) * zsl + oz), v, ps);
-      if (upd) pout[bz0 + (kHxz - 4) * zsl + oz] = ps;
+      if (upd) put(&pout[bz0 + (kHxz - 4) * zsl + oz], ps);
     }
     hx = upd ? hx0 - ch * (u - v) : hx0;
   }
@@ -964,11 +972,11 @@ __device__ __forceinline__ void v3s_plane(
     float u = ex_zp - ex, v = ez_xp - ez;
     if (zs) {
       u = pterm(tz[3 * nz + k], tz[4 * nz + k], tz[5 * nz + k], YEE_PSI(eHyz, bz0 + (kHyz - 4) * zsl + oz), u, ps);
-      if (upd) pout[bz0 + (kHyz - 4) * zsl + oz] = ps;
+      if (upd) put(&pout[bz0 + (kHyz - 4) * zsl + oz], ps);
     }
     if (XS) {
       const float w = pterm(t.xbH, t.xcH, t.xkH, pHyx, v, ps);
-      if (upd && q.xs) pout[bx0 + (kHyx - 8) * xsl + ox] = ps;
+      if (upd && q.xs) put(&pout[bx0 + (kHyx - 8) * xsl + ox], ps);
       v = q.xs ? w : v;
     }
     hy = upd ? hy0 - ch * (u - v) : hy0;
@@ -978,12 +986,12 @@ __device__ __forceinline__ void v3s_plane(
     float u = ey_xp - ey, v = ex_yp - ex;
     if (XS) {
       const float w = pterm(t.xbH, t.xcH, t.xkH, pHzx, u, ps);
-      if (upd && q.xs) pout[bx0 + (kHzx - 8) * xsl + ox] = ps;
+      if (upd && q.xs) put(&pout[bx0 + (kHzx - 8) * xsl + ox], ps);
       u = q.xs ? w : u;
     }
     if (q.ys) {
       v = pterm(ty[3 * ny + q.j], ty[4 * ny + q.j], ty[5 * ny + q.j], YEE_PSI(eHzy, kHzy * ysl + oy), v, ps);
-      if (upd) pout[kHzy * ysl + oy] = ps;
+      if (upd) put(&pout[kHzy * ysl + oy], ps);
     }
     hz = upd ? hz0 - ch * (u - v) : hz0;
   }
@@ -1023,9 +1031,9 @@ __device__ __forceinline__ void v3s_plane(
     if (q.ysm) v = pterm(ty[3 * ny + q.jm], ty[4 * ny + q.jm], ty[5 * ny + q.jm], YEE_PSI(eHzy_m, kHzy * ysl + oym), v, ps);
     hz_ym = q.xinH ? hz_ym0 - ch * (u - v) : hz_ym0;
   }
-  oHx[c] = hx;
-  oHy[c] = hy;
-  oHz[c] = hz;
+  put(&oHx[c], hx);
+  put(&oHy[c], hy);
+  put(&oHz[c], hz);
 
   if (a.mat) {
     cb = spal[mid];
@@ -1038,11 +1046,11 @@ __device__ __forceinline__ void v3s_plane(
     float u = hz - hz_ym, v = hy - r.hym;
     if (q.ys) {
       u = pterm(ty[q.j], ty[ny + q.j], ty[2 * ny + q.j], YEE_PSI(eExy, kExy * ysl + oy), u, ps);
-      if (upd) pout[kExy * ysl + oy] = ps;
+      if (upd) put(&pout[kExy * ysl + oy], ps);
     }
     if (zs) {
       v = pterm(tz[k], tz[nz + k], tz[2 * nz + k], YEE_PSI(eExz, bz0 + (kExz - 4) * zsl + oz), v, ps);
-      if (upd) pout[bz0 + (kExz - 4) * zsl + oz] = ps;
+      if (upd) put(&pout[bz0 + (kExz - 4) * zsl + oz], ps);
     }
     const float w = LOSSY ? ca * ex + cb * (u - v) : ex + cb * (u - v);
     vx = upd ? w : ex;
@@ -1052,11 +1060,11 @@ __device__ __forceinline__ void v3s_plane(
     float u = hx - r.hxm, v = hz - hz_xm;
     if (zs) {
       u = pterm(tz[k], tz[nz + k], tz[2 * nz + k], YEE_PSI(eEyz, bz0 + (kEyz - 4) * zsl + oz), u, ps);
-      if (upd) pout[bz0 + (kEyz - 4) * zsl + oz] = ps;
+      if (upd) put(&pout[bz0 + (kEyz - 4) * zsl + oz], ps);
     }
     if (XS) {
       const float w = pterm(t.xbE, t.xcE, t.xkE, pEyx, v, ps);
-      if (upd && q.xs) pout[bx0 + (kEyx - 8) * xsl + ox] = ps;
+      if (upd && q.xs) put(&pout[bx0 + (kEyx - 8) * xsl + ox], ps);
       v = q.xs ? w : v;
     }
     const float w = LOSSY ? ca * ey + cb * (u - v) : ey + cb * (u - v);
@@ -1067,19 +1075,19 @@ __device__ __forceinline__ void v3s_plane(
     float u = hy - hy_xm, v = hx - hx_ym;
     if (XS) {
       const float w = pterm(t.xbE, t.xcE, t.xkE, pEzx, u, ps);
-      if (upd && q.xs) pout[bx0 + (kEzx - 8) * xsl + ox] = ps;
+      if (upd && q.xs) put(&pout[bx0 + (kEzx - 8) * xsl + ox], ps);
       u = q.xs ? w : u;
     }
     if (q.ys) {
       v = pterm(ty[q.j], ty[ny + q.j], ty[2 * ny + q.j], YEE_PSI(eEzy, kEzy * ysl + oy), v, ps);
-      if (upd) pout[kEzy * ysl + oy] = ps;
+      if (upd) put(&pout[kEzy * ysl + oy], ps);
     }
     const float w = LOSSY ? ca * ez + cb * (u - v) : ez + cb * (u - v);
     vz = upd ? w : ez;
   }
-  oEx[c] = vx;
-  oEy[c] = vy;
-  oEz[c] = vz;
+  put(&oEx[c], vx);
+  put(&oEy[c], vy);
+  put(&oEz[c], vz);
 
   r.ex = ex_zp;
   r.ey = ey_zp;
