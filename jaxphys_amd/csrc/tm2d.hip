@@ -141,9 +141,12 @@ __device__ __forceinline__ void finish_cell_c(const StepArgs& g, int i, int j, b
     if (g.rows[s] == j) ezn = ezn + g.vals[s];
   // fdtd.py:351-352
   if (conductor) ezn = 0.0f;
-  ez_out[o] = ezn;
-  hx_out[o] = hxn;
-  hy_out[o] = hyn;
+  // non-temporal: outputs are read again only by the next step, after the
+  // whole grid has streamed through the caches (16384^2 absorbing grid:
+  // 1.272 -> 1.237 ms per step, the same bits)
+  __builtin_nontemporal_store(ezn, &ez_out[o]);
+  __builtin_nontemporal_store(hxn, &hx_out[o]);
+  __builtin_nontemporal_store(hyn, &hy_out[o]);
 }
 
 template <int MODE>
