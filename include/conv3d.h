@@ -91,6 +91,28 @@ int conv3d_stencil_f32(int32_t nx, int32_t ny, int32_t nz, float* const* fields,
                        const float* eps, const float* scale, const float* w, float* y,
                        int32_t ystride, void* stream);
 
+/* The closure's near-source patch (no reference counterpart): a local grid
+ * and its 2x-refined twin run side by side, and the coarse box of side^3
+ * cells at coarse (x0, y0, z0) is kept equal to the restricted fine state.
+ * Fine fields are float32, x fastest, nxf x nyf x nzf with even sides; fine
+ * index of coarse (K, J, I) is (2K, 2J, 2I).  Restriction: coarse Ex is the
+ * mean of fine Ex at x and x+1 (Ey, Ez likewise along their own axes);
+ * coarse Hx the mean of the four fine Hx at (y, z) + {0,1}^2 (Hy, Hz
+ * likewise).
+ * conv3d_patch_restrict_h: out[3][side^3] = the restricted fine H (hf: 3
+ * device pointers), kept for the mean with the next fine H.
+ * conv3d_patch_correct: st = restricted fine E (ef) and the mean of hprev and
+ * the restricted fine H (hf); for each of the six coarse fields (coarse: 6
+ * device pointers, nxc x nyc x nzc), corr[c][i] = st - coarse and coarse +=
+ * corr over the box.                                                       */
+int conv3d_patch_restrict_h(const float* const* hf, int32_t nxf, int32_t nyf, int32_t nzf,
+                            int32_t x0, int32_t y0, int32_t z0, int32_t side, float* out,
+                            void* stream);
+int conv3d_patch_correct(const float* const* ef, const float* const* hf, const float* hprev,
+                         int32_t nxf, int32_t nyf, int32_t nzf, float* const* coarse,
+                         int32_t nxc, int32_t nyc, int32_t nzc, int32_t x0, int32_t y0,
+                         int32_t z0, int32_t side, float* corr, void* stream);
+
 const char* conv3d_last_error(void);
 
 #ifdef __cplusplus

@@ -43,6 +43,11 @@ EXPORTS = {
                                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
                                           ctypes.c_void_p]),
+    "conv3d_patch_restrict_h": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_int32] * 7
+                                + [ctypes.c_void_p, ctypes.c_void_p]),
+    "conv3d_patch_correct": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+                             + [ctypes.c_int32] * 3 + [ctypes.c_void_p]
+                             + [ctypes.c_int32] * 7 + [ctypes.c_void_p, ctypes.c_void_p]),
     "conv3d_last_error": (ctypes.c_char_p, []),
 }
 
@@ -129,3 +134,26 @@ def stencil_f32(shape_xyz, fields, eps, scale, w, y, ystride, stream=None) -> No
     check(lib().conv3d_stencil_f32(nx, ny, nz, field_array(fields), ptr(eps), ptr(scale), ptr(w),
                                    ptr(y), int(ystride), stream_handle(stream)),
           "conv3d_stencil_f32")
+
+
+def patch_restrict_h(fine_h, fine_shape_xyz, origin_xyz, side, out, stream=None) -> None:
+    """``conv3d_patch_restrict_h``: out (3, side, side, side) = the fine H
+    fields restricted to the coarse box at ``origin_xyz``."""
+    nx, ny, nz = fine_shape_xyz
+    x0, y0, z0 = origin_xyz
+    check(lib().conv3d_patch_restrict_h(field_array(fine_h), nx, ny, nz, x0, y0, z0, int(side),
+                                        ptr(out), stream_handle(stream)),
+          "conv3d_patch_restrict_h")
+
+
+def patch_correct(fine_e, fine_h, hprev, fine_shape_xyz, coarse, coarse_shape_xyz, origin_xyz,
+                  side, corr, stream=None) -> None:
+    """``conv3d_patch_correct``: corr (6, side, side, side) = restricted fine
+    state minus the coarse fields in the box, and the coarse fields += corr."""
+    nx, ny, nz = fine_shape_xyz
+    cx, cy, cz = coarse_shape_xyz
+    x0, y0, z0 = origin_xyz
+    check(lib().conv3d_patch_correct(field_array(fine_e), field_array(fine_h), ptr(hprev), nx, ny,
+                                     nz, field_array(coarse), cx, cy, cz, x0, y0, z0, int(side),
+                                     ptr(corr), stream_handle(stream)),
+          "conv3d_patch_correct")

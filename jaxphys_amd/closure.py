@@ -750,17 +750,28 @@ class NearSourcePatch:
                 last = int(live[-1]) // every if live.size else -1
                 n_run = min(n_coarse, (last + 1 + tail) * every)
             fsol = gf.build_solver(0.5 * dt, 2 * n_run + 2)
-            bl = tuple(slice(v - radius, v + radius + 1) for v in (loc[2], loc[1], loc[0]))
-            out = []
-            for q, st in enumerate(restricted_states(fsol, n_run, every)):
+            side = 2 * radius + 1
+            org = tuple(v - radius for v in loc)                # box origin (x, y, z)
+            fshape, cshape = (2 * n_loc,) * 3, (n_loc,) * 3
+            n_int = n_run // every
+            corr = torch.empty((n_int, 6, side, side, side), device=dev)
+            hprev = torch.empty((3, side, side, side), device=dev)
+            # the fine run advanced interval by interval: H^{2m-3/2} restricted
+            # into hprev, then one more step to (E^{2m}, H^{2m-1/2}); the
+            # kernels restrict, average and correct the coarse box in place
+            # (the steps of restricted_states, on the box only)
+            done = 0
+            for q in range(n_int):
+                m = (q + 1) * every
+                fsol.run(done, 2 * m - 1 - done)
+                _conv.patch_restrict_h(fsol.fields[3:], fshape, org, side, hprev)
+                fsol.run(2 * m - 1, 1)
                 csol.run(q * every, every)
-                c = torch.stack([st[k][bl] - csol.fields[k][bl] for k in range(6)])
-                for k in range(6):
-                    csol.fields[k][bl] += c[k]
-                out.append(c)
+                _conv.patch_correct(fsol.fields[:3], fsol.fields[3:], hprev, fshape,
+                                    csol.fields, cshape, org, side, corr[q])
+                done = 2 * m
             del csol, fsol
-            self.corr.append(torch.stack(out) if out else
-                             torch.zeros((0, 6) + (2 * radius + 1,) * 3, device=dev))
+            self.corr.append(corr)
             self.boxes.append(tuple(slice(v - radius, v + radius + 1)
                                     for v in (pos[2], pos[1], pos[0])))
 

@@ -608,6 +608,82 @@ __global__ __launch_bounds__(256) void stencil_f32_kernel(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Near-source patch of the closure: restriction of a 2x-refined local grid
+// to the coarse box around a source, and the coarse box's correction.  The
+// same float32 operations in the same order as the torch restriction in
+// jaxphys_amd/closure.py (restrict_e / restrict_h, the mean of two H
+// restrictions), so the two give the same bits.  One thread per coarse cell
+// of the side^3 box at coarse (x0, y0, z0); fine index of coarse (K, J, I)
+// is (2K, 2J, 2I).
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ long long fidx(int nxf, int nyf, int z, int y, int x) {
+  return ((long long)z * nyf + y) * nxf + x;
+}
+
+// coarse Hx, Hy, Hz of the box (each the mean of four fine values)
+__device__ __forceinline__ void restrict_h_cell(const float* __restrict__ hx,
+                                                const float* __restrict__ hy,
+                                                const float* __restrict__ hz, int nxf, int nyf,
+                                                int K, int J, int I, float& ox, float& oy,
+                                                float& oz) {
+  const int z = 2 * K, y = 2 * J, x = 2 * I;
+  ox = 0.25f * (((hx[fidx(nxf, nyf, z, y, x)] + hx[fidx(nxf, nyf, z, y + 1, x)]) +
+                 hx[fidx(nxf, nyf, z + 1, y, x)]) + hx[fidx(nxf, nyf, z + 1, y + 1, x)]);
+  oy = 0.25f * (((hy[fidx(nxf, nyf, z, y, x)] + hy[fidx(nxf, nyf, z, y, x + 1)]) +
+                 hy[fidx(nxf, nyf, z + 1, y, x)]) + hy[fidx(nxf, nyf, z + 1, y, x + 1)]);
+  oz = 0.25f * (((hz[fidx(nxf, nyf, z, y, x)] + hz[fidx(nxf, nyf, z, y, x + 1)]) +
+                 hz[fidx(nxf, nyf, z, y + 1, x)]) + hz[fidx(nxf, nyf, z, y + 1, x + 1)]);
+}
+
+__global__ __launch_bounds__(256) void patch_restrict_h_kernel(
+    const float* __restrict__ hx, const float* __restrict__ hy, const float* __restrict__ hz,
+    int nxf, int nyf, int x0, int y0, int z0, int side, float* __restrict__ out) {
+  const int n = side * side * side;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = t % side, j = (t / side) % side, k = t / (side * side);
+  float a, b, c;
+  restrict_h_cell(hx, hy, hz, nxf, nyf, z0 + k, y0 + j, x0 + i, a, b, c);
+  out[t] = a;
+  out[n + t] = b;
+  out[2 * n + t] = c;
+}
+
+__global__ __launch_bounds__(256) void patch_correct_kernel(
+    const float* __restrict__ ex, const float* __restrict__ ey, const float* __restrict__ ez,
+    const float* __restrict__ hx, const float* __restrict__ hy, const float* __restrict__ hz,
+    const float* __restrict__ hprev, int nxf, int nyf, float* __restrict__ c0,
+    float* __restrict__ c1, float* __restrict__ c2, float* __restrict__ c3,
+    float* __restrict__ c4, float* __restrict__ c5, int nxc, int nyc, int x0, int y0, int z0,
+    int side, float* __restrict__ corr) {
+  const int n = side * side * side;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = t % side, j = (t / side) % side, k = t / (side * side);
+  const int K = z0 + k, J = y0 + j, I = x0 + i;
+  const int z = 2 * K, y = 2 * J, x = 2 * I;
+  float st[6];
+  st[0] = 0.5f * (ex[fidx(nxf, nyf, z, y, x)] + ex[fidx(nxf, nyf, z, y, x + 1)]);
+  st[1] = 0.5f * (ey[fidx(nxf, nyf, z, y, x)] + ey[fidx(nxf, nyf, z, y + 1, x)]);
+  st[2] = 0.5f * (ez[fidx(nxf, nyf, z, y, x)] + ez[fidx(nxf, nyf, z + 1, y, x)]);
+  float h0, h1, h2;
+  restrict_h_cell(hx, hy, hz, nxf, nyf, K, J, I, h0, h1, h2);
+  st[3] = 0.5f * (hprev[t] + h0);
+  st[4] = 0.5f * (hprev[n + t] + h1);
+  st[5] = 0.5f * (hprev[2 * n + t] + h2);
+  float* cf[6] = {c0, c1, c2, c3, c4, c5};
+  const long long o = ((long long)K * nyc + J) * nxc + I;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    const float x_ = cf[q][o];
+    const float d = st[q] - x_;
+    cf[q][o] = x_ + d;
+    corr[q * n + t] = d;
+  }
+}
+
 int kpad_of(int cin) { return ((27 * cin + 31) / 32) * 32; }
 
 // JAXPHYS_AMD_CONV_NOSHIFT=1 turns the shifted-fragment loop off (same
@@ -817,6 +893,47 @@ int conv3d_stencil_f32(int32_t nx, int32_t ny, int32_t nz, float* const* fields,
   hipLaunchKernelGGL(stencil_f32_kernel, dim3((unsigned)((npairs + 255) / 256)), dim3(256), 0,
                      (hipStream_t)stream, nx, ny, nz, fields[0], fields[1], fields[2], fields[3],
                      fields[4], fields[5], eps, scale, w, y, ystride);
+  CONV_CHECK(hipGetLastError());
+  return 0;
+}
+
+static bool box_inside(int x0, int y0, int z0, int side, int nx, int ny, int nz) {
+  return side > 0 && x0 >= 0 && y0 >= 0 && z0 >= 0 && x0 + side <= nx && y0 + side <= ny &&
+         z0 + side <= nz;
+}
+
+int conv3d_patch_restrict_h(const float* const* hf, int32_t nxf, int32_t nyf, int32_t nzf,
+                            int32_t x0, int32_t y0, int32_t z0, int32_t side, float* out,
+                            void* stream) {
+  if (!hf || !hf[0] || !hf[1] || !hf[2] || !out) return fail(-1, "bad argument");
+  if (nxf % 2 || nyf % 2 || nzf % 2) return fail(-1, "fine grid sides must be even");
+  if (!box_inside(x0, y0, z0, side, nxf / 2, nyf / 2, nzf / 2))
+    return fail(-1, "box outside the coarse grid of the fine one");
+  const int n = side * side * side;
+  hipLaunchKernelGGL(patch_restrict_h_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, hf[0], hf[1], hf[2], nxf, nyf, x0, y0, z0, side, out);
+  CONV_CHECK(hipGetLastError());
+  return 0;
+}
+
+int conv3d_patch_correct(const float* const* ef, const float* const* hf, const float* hprev,
+                         int32_t nxf, int32_t nyf, int32_t nzf, float* const* coarse,
+                         int32_t nxc, int32_t nyc, int32_t nzc, int32_t x0, int32_t y0,
+                         int32_t z0, int32_t side, float* corr, void* stream) {
+  if (!ef || !hf || !hprev || !coarse || !corr) return fail(-1, "bad argument");
+  for (int c = 0; c < 3; ++c)
+    if (!ef[c] || !hf[c]) return fail(-1, "null fine field");
+  for (int c = 0; c < 6; ++c)
+    if (!coarse[c]) return fail(-1, "null coarse field");
+  if (nxf % 2 || nyf % 2 || nzf % 2) return fail(-1, "fine grid sides must be even");
+  if (!box_inside(x0, y0, z0, side, nxf / 2, nyf / 2, nzf / 2) ||
+      !box_inside(x0, y0, z0, side, nxc, nyc, nzc))
+    return fail(-1, "box outside the coarse grid");
+  const int n = side * side * side;
+  hipLaunchKernelGGL(patch_correct_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, ef[0], ef[1], ef[2], hf[0], hf[1], hf[2], hprev, nxf,
+                     nyf, coarse[0], coarse[1], coarse[2], coarse[3], coarse[4], coarse[5], nxc,
+                     nyc, x0, y0, z0, side, corr);
   CONV_CHECK(hipGetLastError());
   return 0;
 }

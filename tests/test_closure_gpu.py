@@ -127,3 +127,40 @@ def test_near_source_patch_beats_plain_coarse_far_from_source(gpu):
         print(f"[closure] seed {seed} far-field error: patch {errs['patch']:.4f}, "
               f"plain {errs['plain']:.4f}", flush=True)
         assert errs["patch"] < 0.5 * errs["plain"]
+
+
+def test_patch_kernels_match_torch_restriction(gpu):
+    """conv3d_patch_restrict_h / conv3d_patch_correct give the same bits as
+    the torch restriction (restrict_e, restrict_h, the mean of two H
+    restrictions) on the box, and touch no coarse cell outside it."""
+    from jaxphys_amd import _conv
+
+    g = torch.Generator(device="cpu").manual_seed(4)
+    nzf, nyf, nxf = 40, 36, 32
+    side, org = 7, (5, 4, 3)                                  # origin (x, y, z)
+
+    def rnd(*shape):
+        return torch.randn(*shape, generator=g).to(gpu)
+
+    e = [rnd(nzf, nyf, nxf) for _ in range(3)]
+    h0 = [rnd(nzf, nyf, nxf) for _ in range(3)]
+    h1 = [rnd(nzf, nyf, nxf) for _ in range(3)]
+    coarse = [rnd(nzf // 2, nyf // 2, nxf // 2) for _ in range(6)]
+    before = [c.clone() for c in coarse]
+    hprev = torch.empty(3, side, side, side, device=gpu)
+    corr = torch.empty(6, side, side, side, device=gpu)
+    fshape, cshape = (nxf, nyf, nzf), (nxf // 2, nyf // 2, nzf // 2)
+    _conv.patch_restrict_h(h0, fshape, org, side, hprev)
+    box = (slice(org[2], org[2] + side), slice(org[1], org[1] + side),
+           slice(org[0], org[0] + side))
+    rh0 = CL.restrict_h(h0)
+    for c in range(3):
+        assert torch.equal(hprev[c], rh0[c][box])
+    _conv.patch_correct(e, h1, hprev, fshape, coarse, cshape, org, side, corr)
+    st = CL.restrict_e(e) + [0.5 * (a + b) for a, b in zip(rh0, CL.restrict_h(h1))]
+    for c in range(6):
+        d = st[c][box] - before[c][box]
+        assert torch.equal(corr[c], d)
+        want = before[c].clone()
+        want[box] = before[c][box] + d
+        assert torch.equal(coarse[c], want)

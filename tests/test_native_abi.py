@@ -177,3 +177,23 @@ def test_conv3d_rejects_bad_channel_configs_host_only():
     rc = _conv.lib().conv3d_stencil_f32(4, 4, 4, (ctypes.c_void_p * 6)(*[0x1000] * 6), 0x1000,
                                         0x2000, 0x3000, 0x4000, 5, None)
     assert rc == -1
+
+
+def test_patch_kernels_reject_bad_boxes():
+    """The near-source patch entry points check their arguments before any
+    GPU work: an odd fine grid or a box outside the grid is -1 with a
+    message (pointer values are never dereferenced on these paths)."""
+    import ctypes
+
+    from jaxphys_amd import _conv
+
+    lib = _conv.lib()
+    fake = (ctypes.c_void_p * 6)(*([16] * 6))
+    out = ctypes.c_void_p(16)
+    assert lib.conv3d_patch_restrict_h(fake, 33, 32, 32, 0, 0, 0, 4, out, None) == -1
+    assert b"even" in lib.conv3d_last_error()
+    assert lib.conv3d_patch_restrict_h(fake, 32, 32, 32, 14, 0, 0, 4, out, None) == -1
+    assert b"outside" in lib.conv3d_last_error()
+    assert lib.conv3d_patch_correct(fake, fake, out, 32, 32, 32, fake, 16, 16, 8, 0, 0, 6, 4,
+                                    out, None) == -1
+    assert b"outside" in lib.conv3d_last_error()
