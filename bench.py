@@ -573,6 +573,7 @@ def closure_bench(args, dev):
                                  lr_final=args.closure_lr_final,
                                  onpolicy_from=(None if args.closure_onpolicy_from < 0
                                                 else args.closure_onpolicy_from),
+                                 onpolicy_with_net=args.closure_onpolicy_with_net,
                                  forcing=not args.closure_no_forcing,
                                  seed=args.closure_seed,
                                  far_field=not args.closure_no_far_field,
@@ -619,6 +620,7 @@ def closure_bench(args, dev):
                    "linear source model", "patch_half": net.patch_half,
                    "patch_tail": net.patch_tail,
                    "onpolicy_from_epoch": args.closure_onpolicy_from,
+                   "onpolicy_with_net": args.closure_onpolicy_with_net,
                    "lr": [args.closure_lr, args.closure_lr_final]},
         "err_coarse_no_closure": round(float(np.mean(err_0)), 5),
         "err_far_closure": round(float(np.mean([r["err_far_closure"] for r in res])), 5),
@@ -682,6 +684,9 @@ def main() -> None:
     ap.add_argument("--closure-lr", type=float, default=1e-3)
     ap.add_argument("--closure-lr-final", type=float, default=1e-4,
                     help="learning rate at the end of the cosine schedule")
+    ap.add_argument("--closure-onpolicy-with-net", action="store_true",
+                    help="on-policy samples come from rollouts corrected by the whole closure, "
+                    "the net included")
     ap.add_argument("--closure-onpolicy-from", type=int, default=-1,
                     help="epoch from which training uses the corrected rollout's own states "
                          "(-1: teacher-forced only)")
