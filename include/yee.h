@@ -164,7 +164,13 @@ int yee_step_e(YeePlan* plan, float* const* fields, float* psi, int32_t n,
  * Ca) to the adjoint state: lam[6] holds dJ/d(E^{n+1}, H^{n+1/2}) on entry
  * and dJ/d(E^n, H^{n-1/2}) on exit; lam_psi (same size as psi) holds the
  * psi adjoints, updated with one step of deferral for the H part (the
- * caller runs steps n = N-1 .. 0 in order and does nothing else with it).
+ * caller runs steps n = N-1 .. 0 in order and does nothing else with it):
+ * each step first applies the psi_H-adjoint update left over from step n+1,
+ * computed from the H part of lam on entry, so a sweep starts at step N
+ * with the H part of lam zero (as it is for objectives over E) and lam_psi
+ * zero.  With that start the sweep is the exact transpose of the forward
+ * run: <lam_N, S^N x> = <lam_0, x> for states x with zero psi
+ * (tests/test_adjoint_gpu.py).
  * Objective terms are added between steps with yee_probe_scatter_add.
  * Optional design box: cells where dJ/d(eps_r) is wanted, all with damping
  * 1 (outside sponge/CPML slabs) and every E component updated; the caller
