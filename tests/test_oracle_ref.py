@@ -207,3 +207,104 @@ def test_fp32_restatement_close_to_fp64():
                       save_every=20, dtype=np.float32)
     rel = np.linalg.norm(a["ez"][-1] - b["ez"][-1]) / np.linalg.norm(a["ez"][-1])
     assert rel < 1e-4
+
+
+def _step_old_h(f, eps, ch, ce):
+    """A planted mistake (test only): the E update of a periodic step
+    reads the old H instead of the new one."""
+    ex, ey, ez, hx, hy, hz = f
+    cx, cy, cz = R._curl_e_periodic(ex, ey, ez)
+    ax, ay, az = R._curl_h_periodic(hx, hy, hz)
+    cb = ce / eps
+    return [ex + cb * ax, ey + cb * ay, ez + cb * az, hx - ch * cx, hy - ch * cy, hz - ch * cz]
+
+
+def _reversal_error(step, size, n, seed=2):
+    """Run n steps from random (E^0, H^-1/2), then n steps from
+    (E^n, -H^{n+1/2}): a leapfrog scheme (H from the old E, then E from the
+    new H) returns exactly to (E^0, -H^{1/2})."""
+    rng = np.random.default_rng(seed)
+    eps = 1.0 + 3.0 * rng.random(size)
+    f0 = _rand_fields(size, rng)
+    f = f0
+    for _ in range(n):
+        f = step(f, eps)
+    h_next = step(f, eps)[3:]
+    r = f[:3] + [-h for h in h_next]
+    for _ in range(n):
+        r = step(r, eps)
+    want = f0[:3] + [-h for h in step(f0, eps)[3:]]
+    scale = max(np.abs(a).max() for a in want)
+    return max(np.abs(a - b).max() for a, b in zip(r, want)) / scale
+
+
+def test_periodic_time_reversal_fp64():
+    """Time reversal of the Yee leapfrog (fdtd3d.py:290-301: H from E^n,
+    then E from the new H): 60 steps forward and 60 back from the
+    H-reversed state recover the start to rounding; a step whose E update
+    reads the old H does not."""
+    size = (10, 12, 9)
+    dt = R.default_dt(DX, np.float64)
+    ch, ce = dt / (R.MU0 * DX), dt / (R.EPS0 * DX)
+    good = _reversal_error(lambda f, eps: R.step(f, eps, None, "periodic", ch, ce, np.float64),
+                           size, 60)
+    assert good < 1e-11
+    bad = _reversal_error(lambda f, eps: _step_old_h(f, eps, ch, ce), size, 60)
+    assert bad > 1e-3
+
+
+def _mirror_pairs(n, c):
+    """Index maps of the x (or y) mirror about node c: integer-position
+    nodes i -> 2c - i, half-integer ones i -> 2c - 1 - i (periodic)."""
+    i = np.arange(n)
+    return (2 * c - i) % n, (2 * c - 1 - i) % n
+
+
+def _mirror_error(step, n=16, nz=10, steps=25):
+    """A blob of Ez centred on node (c, c), in a medium layered along z,
+    stays symmetric: under x -> 2c - x, Ey, Ez, Hx are even and Ex, Hy, Hz
+    odd (E polar, H axial); under y -> 2c - y, Ex, Ez, Hy even and Ey, Hx,
+    Hz odd.  (The medium varies along z only: eps_r of a cell index serves
+    all three E components, which sit at different positions, so an eps
+    that varies along x or y is not mirror-symmetric for all of them.)"""
+    c = n // 2
+    rng = np.random.default_rng(7)
+    ii, jj = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    dx, dy = np.minimum(abs(ii - c), n - abs(ii - c)), np.minimum(abs(jj - c), n - abs(jj - c))
+    eps = np.broadcast_to((1.0 + 3.0 * rng.random(nz))[None, None, :], (n, n, nz)).copy()
+    f = [np.zeros((n, n, nz)) for _ in range(6)]
+    f[2] = np.exp(-0.3 * (dx ** 2 + dy ** 2))[:, :, None] * rng.standard_normal(nz)[None, None, :]
+    for _ in range(steps):
+        f = step(f, eps)
+    ie, ih = _mirror_pairs(n, c)
+    x_even = {1: ie, 2: ie, 3: ie}
+    y_even = {0: ie, 2: ie, 4: ie}
+    err = 0.0
+    scale = max(np.abs(a).max() for a in f)
+    for comp, a in enumerate(f):
+        mx = a[x_even[comp]] if comp in x_even else -a[ih]
+        my = a[:, y_even[comp]] if comp in y_even else -a[:, ih]
+        err = max(err, np.abs(a - mx).max(), np.abs(a - my).max())
+    return err / scale
+
+
+def test_periodic_mirror_symmetry_fp64():
+    """Mirror symmetry of a centred source (the build's oracle pin for the
+    update's index offsets): 25 steps keep every component even or odd
+    under the x and y mirrors to rounding; moving one difference of the E
+    update by a cell (Hy(i+1) - Hy(i) instead of Hy(i) - Hy(i-1)) breaks
+    it."""
+    dt = R.default_dt(DX, np.float64)
+    ch, ce = dt / (R.MU0 * DX), dt / (R.EPS0 * DX)
+    good = _mirror_error(lambda f, eps: R.step(f, eps, None, "periodic", ch, ce, np.float64))
+    assert good < 1e-12
+
+    def shifted(f, eps):
+        g = R.step(f, eps, None, "periodic", ch, ce, np.float64)
+        # redo Ez with the Hy difference one cell off
+        hx, hy = g[3], g[4]
+        cz = (np.roll(hy, -1, 0) - hy) - (hx - np.roll(hx, 1, 1))
+        g[2] = f[2] + (ce / eps) * cz
+        return g
+
+    assert _mirror_error(shifted) > 1e-3
