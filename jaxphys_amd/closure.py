@@ -665,6 +665,12 @@ def _fine_twin(s: PointSource3D, pos_c) -> list:
     return out
 
 
+def _refine(t: torch.Tensor) -> torch.Tensor:
+    """A cell-centred (x, y, z) array on the 2x-refined grid: each value
+    repeated 2x2x2."""
+    return t.repeat_interleave(2, 0).repeat_interleave(2, 1).repeat_interleave(2, 2)
+
+
 class NearSourcePatch:
     """Corrections of the (2R+1)^3 box around each point-source cell of a
     coarse grid, one per correction interval, computed on a local
@@ -725,17 +731,13 @@ class NearSourcePatch:
             sl = (slice(o[2], o[2] + n_loc), slice(o[1], o[1] + n_loc), slice(o[0], o[0] + n_loc))
             eps_c = eps[sl].permute(2, 1, 0).contiguous()      # (x, y, z)
             sig_c = None if sigma is None else sigma[sl].permute(2, 1, 0).contiguous()
-
-            def rep(t):
-                return t.repeat_interleave(2, 0).repeat_interleave(2, 1).repeat_interleave(2, 2)
-
             gc = EMGrid3D((n_loc,) * 3, grid._dx, "cpml", npml, device=dev, **cpml)
             gf = EMGrid3D((2 * n_loc,) * 3, 0.5 * grid._dx, "cpml", 2 * npml, device=dev, **cpml)
             ones = np.ones((n_loc,) * 3, bool)
             gc.add_material(DielectricRegion(mask=ones, epsilon_r=eps_c, sigma=sig_c))
             gf.add_material(DielectricRegion(
-                mask=np.ones((2 * n_loc,) * 3, bool), epsilon_r=rep(eps_c),
-                sigma=None if sig_c is None else rep(sig_c)))
+                mask=np.ones((2 * n_loc,) * 3, bool), epsilon_r=_refine(eps_c),
+                sigma=None if sig_c is None else _refine(sig_c)))
             for s in srcs:
                 gc.add_source(dataclasses.replace(s, position=tuple(loc)))
                 for t in _fine_twin(s, loc):
