@@ -586,14 +586,36 @@ def closure_bench(args, dev):
                                              else args.closure_patch_tail),
                                  log=lambda m: print("[closure]", m, file=sys.stderr, flush=True),
                                  **pkw)
+    val_seeds = list(range(900, 900 + args.closure_val))
+    rollout = {}
+    if args.closure_rollout_epochs > 0:
+        # the net trained through the corrected rollouts (backpropagation
+        # through the coarse steps), its gain chosen on its own validation seeds
+        val_seeds = list(range(900, 900 + args.closure_rollout_val))
+        rollout = {"rollout_epochs": args.closure_rollout_epochs,
+                   "rollout_lr": args.closure_rollout_lr,
+                   "rollout_window": args.closure_rollout_window}
+        CL.train_rollout(net, dev, train_seeds, n_fine=nf, n_coarse=nf * 400 // 256,
+                         every=args.closure_every, epochs=args.closure_rollout_epochs,
+                         lr=args.closure_rollout_lr, lr_final=0.1 * args.closure_rollout_lr,
+                         window=args.closure_rollout_window, seed=args.closure_seed,
+                         val_seeds=val_seeds,
+                         log=lambda m: print("[closure]", m, file=sys.stderr, flush=True), **pkw)
     t_train = time.time() - t0
     if args.closure_save:
         os.makedirs(os.path.dirname(os.path.abspath(args.closure_save)), exist_ok=True)
-        net.save(args.closure_save, info={
-            "workload": "closure_c5", "n_fine": nf, "train_seeds": train_seeds,
-            "val_seeds": list(range(900, 900 + args.closure_val)),
-            "epochs": args.closure_epochs, "every": args.closure_every, "seed": args.closure_seed,
-            "train_s": round(t_train, 1), **pkw})
+        info = {"workload": "closure_c5", "n_fine": nf, "train_seeds": train_seeds,
+                "epochs": args.closure_epochs, "every": args.closure_every,
+                "seed": args.closure_seed, "train_s": round(t_train, 1), **pkw}
+        if args.closure_load:
+            # the linear parts come from the loaded file: keep its record,
+            # with what was trained here added
+            info = dict(CL.ClosureNet.info(args.closure_load))
+            if rollout:
+                info["rollout_train_s"] = round(t_train, 1)
+        info.update(rollout)
+        info["val_seeds"] = val_seeds
+        net.save(args.closure_save, info=info)
     summ = closure_summary(net, dev, test_seeds, nf, args.closure_every, **pkw)
     res = summ.pop("_res")
     err_c = [r["err_closure"] for r in res]
@@ -696,6 +718,15 @@ def main() -> None:
                     help="teacher-forced samples start one corrected interval earlier")
     ap.add_argument("--closure-val", type=int, default=2,
                     help="validation seeds (900, 901, ...) that choose the net's gain")
+    ap.add_argument("--closure-rollout-epochs", type=int, default=0,
+                    help="closure_c5: epochs of training the net through the corrected "
+                    "rollouts (backpropagation through the coarse steps; fresh net weights)")
+    ap.add_argument("--closure-rollout-lr", type=float, default=6e-5,
+                    help="rollout training: Adam learning rate (cosine to a tenth of it)")
+    ap.add_argument("--closure-rollout-window", type=int, default=10,
+                    help="rollout training: correction intervals per update")
+    ap.add_argument("--closure-rollout-val", type=int, default=8,
+                    help="rollout training: validation seeds (900, ...) that choose the gain")
     ap.add_argument("--closure-net-in-box", action="store_true",
                     help="the net also corrects the source boxes (on top of the source model)")
     ap.add_argument("--closure-source", default="patch", choices=["patch", "linear"],

@@ -9,10 +9,15 @@ source; or, with ``source_patch=False``, the linear source model below,
 fitted to the solver's own fine runs); outside those boxes a float32
 linear far-field operator (a 27-tap stencil of the fields, fitted by
 least squares, which corrects the coarse grid's dispersion away from the
-source; ``fit_far_field``); and a BF16 3D conv net trained on what the
-far-field operator leaves, whose gain validation seeds choose
-(``select_net_gain``; zero when it does not lower their error by more
-than 1 %).
+source; ``fit_far_field``); and a BF16 3D conv net, whose gain validation
+seeds choose (``select_net_gain``; zero when it does not lower their error
+by more than 1 %).  The net is trained through the corrected rollouts
+themselves (``train_rollout``): the trajectory error of a whole corrected
+coarse run is back-propagated through the coarse steps with the transposed
+Yee step of the adjoint kernels and through the far-field operator's
+transpose into every correction the net made.  (Trained instead on
+one-interval residuals from restricted fine states, ``train_closure``, it
+learns the transient of those restarts and validation switches it off.)
 
 Network (``ClosureNet``): channels 7 -> 32 -> 32 -> 6 (E, H normalised and
 eps_r in; a field correction out), 3x3x3 zero-padded convolutions, GELU
@@ -1061,17 +1066,22 @@ def fit_far_field(net: ClosureNet, device, seeds, n_fine: int, n_coarse: int, ev
 
 def select_net_gain(net: ClosureNet, device, seeds, n_fine: int, n_coarse: int, every: int,
                     gains=(0.0, 0.25, 0.5, 1.0), log=None, tol: float = 0.01,
-                    **problem_kw) -> dict:
+                    near_radius: float | None = None, **problem_kw) -> dict:
     """Scale of the net's correction, chosen on validation seeds (neither
     training nor test seeds) by the trajectory error of E of the corrected
     coarse runs against the restricted fine states, summed over the seeds:
     the smallest gain within ``tol`` (relative) of the lowest error, so that
     a difference at the noise level of two seeds does not switch the net
-    on.  Sets ``net.net_gain``; returns the errors per gain."""
+    on.  With ``near_radius`` the error leaves out the cells within that
+    many coarse cells of the source (the far-field error, where the net
+    corrects; the source's own cells otherwise weigh most).  Sets
+    ``net.net_gain``; returns the errors per gain."""
     errs = {g: 0.0 for g in gains}
     for sd in seeds:
         prob = ClosureProblem(sd, n_fine, device=device, **problem_kw)
         states, _ = prob.fine_states(n_coarse, every)
+        mask = (None if near_radius is None else
+                far_mask(prob.nc, prob.npml_c, prob.src_c, near_radius, net.device))
         patch = None
         for g in gains:
             net.net_gain = g
@@ -1082,7 +1092,7 @@ def select_net_gain(net: ClosureNet, device, seeds, n_fine: int, n_coarse: int, 
             for q, m in enumerate(range(0, n_coarse, every)):
                 csol.run(m, every)
                 net.correct(csol, 1.0, m, every)
-                a_, b_ = _err_terms(csol.fields, states[q + 1], prob.npml_c)
+                a_, b_ = _err_terms(csol.fields, states[q + 1], prob.npml_c, mask)
                 num, den = num + a_, den + b_
             errs[g] += math.sqrt(num / den) if den > 0 else 0.0
         del states
@@ -1093,6 +1103,239 @@ def select_net_gain(net: ClosureNet, device, seeds, n_fine: int, n_coarse: int, 
                                                         for g, e in errs.items())
             + f" -> {net.net_gain:g}")
     return errs
+
+
+def _reinit_net_weights(net: ClosureNet, seed: int) -> None:
+    """Fresh network weights (the initialisation of ``ClosureNet``, with the
+    output layer at zero so that the net starts with no correction), Adam
+    state cleared; the linear parts and the normalisation are kept."""
+    fresh = ClosureNet(net.device, seed=seed, forcing=net.forcing)
+    fresh.params["w3"].zero_()
+    net.params = fresh.params
+    net.adam = fresh.adam
+    net.step_count = 0
+    net._packed = None
+    net.version += 1
+
+
+def far_field_transpose_weights(net: ClosureNet) -> tuple:
+    """Weights of the far-field operator's transpose for ``conv3d_stencil_f32``.
+
+    The operator (gain folded in) maps the fields f to
+    y_co(i) = sum_{t,ci} (W0[t,ci,co] + e(i) W1[t,ci,co]) f_ci(i + o_t) / s_ci
+    with e = eps - 1.  For u = dL/dy its transpose is
+    dL/df_ci(j) = sum_{t,co} W0[t,ci,co] / s_ci u_co(j - o_t)
+                + sum_{t,co} W1[t,ci,co] / s_ci (e u_co)(j - o_t),
+    two stencils of the same kernel with the taps mirrored (t -> 26 - t),
+    field and output channels swapped, 1 / s_ci folded into the weights
+    and unit scales: applied to u and to e u."""
+    w = net.far_w.view(FAR_TAPS, 6, 2, COUT)
+    inv = (1.0 / net.scale).view(1, 1, 6)
+    out = []
+    for pe in (0, 1):
+        v = torch.zeros(FAR_TAPS, COUT, 2, 6, device=net.device)
+        # v[t', co, 0, ci] = W[t, ci, pe, co] / s_ci,  t' = 26 - t
+        v[:, :, 0, :] = (w[:, :, pe, :].permute(0, 2, 1) * inv).flip(0)
+        out.append(v.reshape(-1).contiguous())
+    return tuple(out)
+
+
+def _rollout_window(net: ClosureNet, csol, states: list, q0: int, q1: int, every: int,
+                    wm: torch.Tensor, cm: torch.Tensor, box, den: float, wc: list,
+                    far_t: tuple | None, grads: bool = True):
+    """Run intervals q0 .. q1-1 of a corrected coarse rollout (the solver
+    holds the state at step q0 * every) and return the loss of those
+    intervals (trajectory error against ``states`` after each correction,
+    weights ``wm`` per cell and ``wc`` per field, over ``den``, or over the
+    window's own reference sum when ``den`` is None) and, with ``grads``,
+    its gradient with respect to the net's weights (see
+    ``train_rollout``)."""
+    from jaxphys_amd import _native
+
+    nc = csol.nx
+    shape = (nc, nc, nc)
+    npos = nc ** 3
+    sc = [float(v) for v in net.scale]
+    patch = getattr(csol, "_closure_patch", None)
+    net.pack()
+    x0 = net.pack_zero(csol)
+    y0 = net.forward(x0, shape, train=grads, tag="z").clone()
+    grad_state = []
+    loss = 0.0
+    if den is None:
+        # normalised by the window's own reference sum
+        den = sum(wc[c] * float((wm * (states[q + 1][c] / sc[c]) ** 2).sum())
+                  for q in range(q0, q1) for c in range(6))
+    for q in range(q0, q1):
+        m = q * every
+        csol.run(m, every)
+        x = net.pack_fields(csol, m, every)
+        tag = f"r{q - q0}"
+        xq = net._buf("x" + tag, tuple(x.shape), x.dtype)
+        xq.copy_(x)
+        y = net.forward(xq, shape, train=grads, tag=tag)
+        y.sub_(y0)
+        if net.far_gain > 0:
+            net.add_far_field(y, csol)
+        if box is not None:
+            y.view(-1, OUT_STRIDE)[box] = 0.0
+        _conv.apply_correction(shape, csol.fields, y, OUT_STRIDE, net.scale, 1.0, net.margin)
+        if patch is not None:
+            patch.apply(csol, q, 1.0)
+        gs = []
+        for c in range(6):
+            r = csol.fields[c] - states[q + 1][c]
+            loss += wc[c] * float((wm * r * r).sum()) / (sc[c] ** 2)
+            if grads:
+                gs.append((2.0 * wc[c] / (den * sc[c] ** 2)) * wm * r)
+        grad_state.append(gs)
+    loss /= den
+    if not grads:
+        return loss, None
+    # adjoint of the state after each correction, from the window's last
+    # interval to its first; the psi adjoints start at zero (every loss and
+    # correction term lies outside the absorbing slabs, so the deferred
+    # psi_H-adjoint update of the first step has nothing to carry)
+    lam = [torch.zeros(shape, device=net.device) for _ in range(6)]
+    lam_psi = None if csol.psi is None else torch.zeros_like(csol.psi)
+    dy = torch.zeros(npos, OUT_STRIDE, device=net.device)
+    dy_sum = torch.zeros(npos, OUT_STRIDE, device=net.device)
+    tmp = torch.zeros(npos, OUT_STRIDE, device=net.device) if far_t is not None else None
+    ones = torch.ones(6, device=net.device)
+    e = (csol.eps_r - 1.0).view(-1)
+    out = None
+    for q in range(q1 - 1, q0 - 1, -1):
+        for c in range(6):
+            lam[c].add_(grad_state[q - q0][c])
+            dy[:, c] = (sc[c] * cm) * lam[c].view(-1)
+        g = net.backward(dy, tag=f"r{q - q0}")
+        out = g if out is None else {k: out[k] + g[k] for k in g}
+        dy_sum.add_(dy)
+        if far_t is not None:
+            # the far-field operator's contribution to the state's adjoint
+            u = [dy[:, c].reshape(shape).contiguous() for c in range(6)]
+            ue = [(dy[:, c] * e).reshape(shape).contiguous() for c in range(6)]
+            tmp.zero_()
+            _conv.stencil_f32(shape, u, csol.eps_r, ones, far_t[0], tmp, OUT_STRIDE)
+            _conv.stencil_f32(shape, ue, csol.eps_r, ones, far_t[1], tmp, OUT_STRIDE)
+            for c in range(6):
+                lam[c].view(-1).add_(tmp[:, c])
+        if q > q0:
+            for _ in range(every):
+                _native.adjoint_step(csol.plan, lam, lam_psi)
+    g0 = net.backward(dy_sum, tag="z")
+    return loss, {k: out[k] - g0[k] for k in out}
+
+
+def rollout_setup(net: ClosureNet, prob: "ClosureProblem", csol, states: list, every: int,
+                  near_radius: float = 3.0, h_weight: float = 1.0):
+    """Per-problem tensors of ``_rollout_window``: the loss weights (the
+    far-field evaluation cells: ``npml`` cells or more inside the faces
+    minus ``near_radius`` around the source), the correction mask (where
+    the correction is applied: ``net.margin`` inside the faces, outside the
+    source boxes), the source-box cells, the loss normaliser and the field
+    weights."""
+    nc = prob.nc
+    shape = (nc, nc, nc)
+    sc = [float(v) for v in net.scale]
+    wc = [1.0, 1.0, 1.0, h_weight, h_weight, h_weight]
+    wm = torch.zeros(shape, device=net.device)
+    a_, b_ = prob.npml_c, nc - prob.npml_c
+    wm[a_:b_, a_:b_, a_:b_] = far_mask(nc, prob.npml_c, prob.src_c, near_radius,
+                                        net.device).to(torch.float32)
+    n_int = len(states) - 1
+    den = sum(wc[c] * float((wm * (states[q + 1][c] / sc[c]) ** 2).sum())
+              for q in range(n_int) for c in range(6))
+    cm = torch.zeros(shape, device=net.device)
+    mg = net.margin
+    cm[mg:nc - mg, mg:nc - mg, mg:nc - mg] = 1.0
+    cm = cm.view(-1)
+    box = net._box_terms(csol, shape, (0, 0, 0), every)[0] if net.forcing else None
+    if box is not None:
+        cm[box] = 0.0
+    return wm, cm, box, den, wc
+
+
+def train_rollout(net: ClosureNet, device, seeds, n_fine: int = 256, n_coarse: int = 400,
+                  every: int = 10, epochs: int = 8, lr: float = 6e-5,
+                  lr_final: float | None = 6e-6, window: int | None = 10,
+                  h_weight: float = 0.0, near_radius: float = 3.0, reinit: bool = True,
+                  relative: bool = True, seed: int = 0, val_seeds=None, log=None,
+                  **problem_kw) -> list:
+    """Train the net through the corrected coarse rollouts themselves
+    (backpropagation through time) instead of on one-interval residuals.
+
+    For each training seed the closure (near-source patch, far-field
+    operator and the net at gain 1) runs the whole ``n_coarse``-step
+    rollout from the zero state, keeping the net's activations of every
+    correction.  The loss is the trajectory error of that rollout against
+    the restricted fine states after each correction, over the cells the
+    far-field error is measured on (the interior ``npml`` cells or more
+    from the faces, minus ``near_radius`` cells around the source), of the
+    normalised fields (H weighted by ``h_weight``), divided by the
+    reference's own sum.  Its gradient flows backwards through the coarse
+    steps with the transposed Yee step (``yee_adjoint_step``, the kernels
+    of the adjoint), through the far-field operator's transpose (the same
+    stencil kernel with mirrored weights) and into the net at every
+    correction.  The net's output enters the gradient as an additive
+    forcing: its dependence on the state it is computed from is not
+    differentiated (exact while that dependence is weak; the net starts
+    with a zero output layer).  With ``window`` the rollout is cut into
+    windows of that many intervals, each followed by its own update (the
+    next window continues from the state the previous one reached).  With
+    ``relative`` each window's loss is its relative error (the square root
+    of its squared error over its reference sum), so that every seed and
+    window weighs alike whatever its error; otherwise the squared relative
+    error over the whole rollout.  The linear parts (normalisation, far-field operator, correction margin,
+    patch settings) must already be set, e.g. by ``train_closure`` or
+    ``ClosureNet.load``; ``reinit`` starts from fresh network weights.
+    With ``val_seeds`` the net's gain is chosen on them afterwards
+    (``select_net_gain``).  Returns the loss of every update."""
+    rng = np.random.default_rng(seed)
+    if reinit:
+        _reinit_net_weights(net, seed)
+    net.net_gain = 1.0
+    n_int = n_coarse // every
+    window = n_int if not window else int(window)
+    per_seed = (n_int + window - 1) // window
+    lr_final = lr if lr_final is None else lr_final
+    n_updates = epochs * len(seeds) * per_seed
+    upd, history = 0, []
+    far_t = far_field_transpose_weights(net) if net.far_gain > 0 else None
+    t_start = time.time()
+    for ep in range(epochs):
+        for sd in [seeds[i] for i in rng.permutation(len(seeds))]:
+            prob = ClosureProblem(sd, n_fine, device=device, **problem_kw)
+            states, _ = prob.fine_states(n_coarse, every)
+            csol = coarse_solver(prob, net, n_coarse, every)
+            wm, cm, box, den, wc = rollout_setup(net, prob, csol, states, every, near_radius,
+                                                 h_weight)
+            for q0 in range(0, n_int, window):
+                loss, g = _rollout_window(net, csol, states, q0, min(n_int, q0 + window), every,
+                                          wm, cm, box, None if relative else den, wc, far_t)
+                if relative and loss > 0:
+                    # d sqrt(L) = dL / (2 sqrt(L))
+                    g = {k: v * (0.5 / math.sqrt(loss)) for k, v in g.items()}
+                    loss = math.sqrt(loss)
+                frac = upd / max(1, n_updates - 1)
+                lr_now = lr_final + 0.5 * (lr - lr_final) * (1.0 + math.cos(math.pi * frac))
+                upd += 1
+                if math.isfinite(loss) and all(bool(torch.isfinite(v).all()) for v in g.values()):
+                    net.adam_step(g, lr=lr_now)
+                history.append(loss)
+                if not all(bool(torch.isfinite(f).all()) for f in csol.fields[:3]):
+                    break
+            if log:
+                log(f"rollout epoch {ep} seed {sd}: loss {np.mean(history[-per_seed:]):.4e} "
+                    f"({time.time() - t_start:.1f} s)")
+            del states, csol
+    # the per-correction activations are training buffers only
+    net._saved = {}
+    net._ws = {}
+    if val_seeds:
+        select_net_gain(net, device, list(val_seeds), n_fine, n_coarse, every, log=log,
+                        near_radius=near_radius, **problem_kw)
+    return history
 
 
 def train_closure(device, seeds, n_fine: int = 256, n_coarse: int = 400, every: int = 10,

@@ -169,3 +169,45 @@ def test_saved_closure_keeps_the_source_correction_mode(tmp_path):
     net.save(path)
     back = ClosureNet.load(path, "cpu")
     assert back.source_patch is False and back.patch_half == 20
+
+
+def test_far_field_transpose_is_the_adjoint_of_the_operator():
+    """The weights that make conv3d_stencil_f32 apply the far-field
+    operator's transpose (rollout training back-propagates through it):
+    <T f, u> = <f, T^T u> for random fields f, eps and u, with T and T^T
+    both evaluated by the float64 oracle of the kernel.  Planted mistakes
+    (taps not mirrored, 1 / scale at the wrong channel) fail it."""
+    from oracle import closure_ops as O
+
+    from jaxphys_amd.closure import ClosureNet, far_field_transpose_weights
+
+    rng = np.random.default_rng(11)
+    net = ClosureNet("cpu")
+    net.far_w = torch.from_numpy(rng.standard_normal(27 * 6 * 2 * 6).astype(np.float32))
+    net.scale = torch.tensor([0.5, 1.5, 2.0, 3e-3, 4e-3, 1e-3])
+    shape = (5, 6, 7)
+    f = [rng.standard_normal(shape) * float(net.scale[c]) for c in range(6)]
+    eps = 1.0 + 3.0 * rng.random(shape)
+    u = rng.standard_normal(shape + (6,))
+    w = net.far_w.view(27, 6, 2, 6).double().numpy()
+    lhs = float((O.stencil(f, eps, net.scale.numpy(), w) * u).sum())
+    ones = np.ones(6)
+
+    def adj(v0, v1):
+        ug = [u[..., c] for c in range(6)]
+        ue = [u[..., c] * (eps - 1.0) for c in range(6)]
+        g = (O.stencil(ug, eps, ones, v0.view(27, 6, 2, 6).double().numpy())
+             + O.stencil(ue, eps, ones, v1.view(27, 6, 2, 6).double().numpy()))
+        return float(sum((g[..., c] * f[c]).sum() for c in range(6)))
+
+    v0, v1 = far_field_transpose_weights(net)
+    assert abs(adj(v0, v1) - lhs) <= 1e-6 * abs(lhs)
+    # the second weight set has no (eps - 1) part of its own
+    assert float(v0.view(27, 6, 2, 6)[:, :, 1].abs().max()) == 0.0
+    # planted: taps not mirrored
+    un = lambda v: v.view(27, 6, 2, 6).flip(0).reshape(-1)  # noqa: E731
+    assert abs(adj(un(v0), un(v1)) - lhs) > 1e-3 * abs(lhs)
+    # planted: 1 / scale applied per output channel instead of per field
+    s = net.scale.view(1, 6, 1, 1)
+    sw = lambda v: (v.view(27, 6, 2, 6) * s / s.view(1, 1, 1, 6)).reshape(-1)  # noqa: E731
+    assert abs(adj(sw(v0), sw(v1)) - lhs) > 1e-3 * abs(lhs)

@@ -212,3 +212,79 @@ def test_near_source_patch_general_source(gpu):
             worst = max(worst, float((sol.fields[c][box] - ref).abs().max()) / scale)
     print(f"[closure] general source: patch box vs restricted fine {worst:.2e}", flush=True)
     assert worst <= 1e-5
+
+
+def test_far_field_transpose_on_the_kernel(gpu):
+    """conv3d_stencil_f32 with the transpose weights is the adjoint of the
+    far-field operator on the GPU: <T f, u> = <f, T^T u> over the six
+    channels, to float32 rounding."""
+    torch.manual_seed(4)
+    n = 24
+    shape = (n, n, n)
+    net = CL.ClosureNet(gpu, seed=0)
+    net.far_w = torch.randn(27 * 6 * 2 * 6, device=gpu)
+    net.scale = torch.tensor([0.5, 1.5, 2.0, 3e-3, 4e-3, 1e-3], device=gpu)
+    f = [torch.randn(shape, device=gpu) * float(net.scale[c]) for c in range(6)]
+    eps = 1.0 + 3.0 * torch.rand(shape, device=gpu)
+    u = torch.randn(n ** 3, CL.OUT_STRIDE, device=gpu)
+    y = torch.zeros(n ** 3, CL.OUT_STRIDE, device=gpu)
+    from jaxphys_amd import _conv
+    _conv.stencil_f32(shape, f, eps, net.scale, net.far_w, y, CL.OUT_STRIDE)
+    lhs = float((y[:, :6].double() * u[:, :6].double()).sum())
+    v0, v1 = CL.far_field_transpose_weights(net)
+    ones = torch.ones(6, device=gpu)
+    g = torch.zeros(n ** 3, CL.OUT_STRIDE, device=gpu)
+    e = (eps - 1.0).reshape(-1)
+    _conv.stencil_f32(shape, [u[:, c].reshape(shape).contiguous() for c in range(6)], eps, ones,
+                      v0, g, CL.OUT_STRIDE)
+    _conv.stencil_f32(shape, [(u[:, c] * e).reshape(shape).contiguous() for c in range(6)], eps,
+                      ones, v1, g, CL.OUT_STRIDE)
+    rhs = sum(float((g[:, c].double() * f[c].reshape(-1).double()).sum()) for c in range(6))
+    print(f"[closure] transpose identity: {lhs:.6e} vs {rhs:.6e}", flush=True)
+    assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
+
+
+def test_rollout_gradient_matches_finite_differences(gpu):
+    """The gradient of the rollout loss (back-propagated through the coarse
+    Yee steps with the transposed step, through the far-field operator's
+    transpose and into the net) against central differences of the loss
+    along the gradient's direction in the output-layer weights, on a
+    64^3 / 32^3 problem with the committed closure's linear parts and a
+    small random output layer (where the net's own dependence on the state,
+    which the gradient leaves out, is negligible).  Without the far-field
+    transpose the gradient is off by about 1 %; with it, 0.1 %."""
+    net = CL.ClosureNet.load(CL.__file__.replace("closure.py", "data/closure_c5.safetensors"),
+                             gpu)
+    CL._reinit_net_weights(net, 5)
+    g_ = torch.Generator(device="cpu").manual_seed(7)
+    w3 = torch.randn(net.params["w3"].shape, generator=g_) * 2e-6
+    w3[:, :, 32:] = 0
+    net.params["w3"] = w3.to(gpu)
+    net.net_gain = 1.0
+    every, n_coarse = 10, 200
+    prob = CL.ClosureProblem(3, 128, npml_fine=16, device=gpu)
+    states, _ = prob.fine_states(n_coarse, every)
+    csol = CL.coarse_solver(prob, net, n_coarse, every)
+    wm, cm, box, den, wc = CL.rollout_setup(net, prob, csol, states, every)
+    assert float(cm.sum()) > 0
+    far_t = CL.far_field_transpose_weights(net)
+    Q = n_coarse // every
+    csol.reset()
+    loss, g = CL._rollout_window(net, csol, states, 0, Q, every, wm, cm, box, den, wc, far_t)
+    base = {k: v.clone() for k, v in net.params.items()}
+    D = g["w3"] / g["w3"].norm()
+    h = 0.1 * float(base["w3"].norm())
+    ls = []
+    for s in (1, -1):
+        net.params = {k: v.clone() for k, v in base.items()}
+        net.params["w3"] = base["w3"] + s * h * D
+        net._packed = None
+        net.version += 1
+        csol.reset()
+        ls.append(CL._rollout_window(net, csol, states, 0, Q, every, wm, cm, box, den, wc, None,
+                                     grads=False)[0])
+    fd = (ls[0] - ls[1]) / (2 * h)
+    an = float((g["w3"] * D).sum())
+    print(f"[closure] rollout loss {loss:.4e}: directional derivative {an:.6e}, central "
+          f"difference {fd:.6e}", flush=True)
+    assert abs(fd - an) <= 3e-3 * abs(an)
