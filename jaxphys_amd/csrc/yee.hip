@@ -695,6 +695,7 @@ int yee_plan_set_fused_rows(YeePlan* p, int32_t rows) {
 void yee_plan_destroy(YeePlan* p) {
   if (!p) return;
   free_sources(p);
+  if (p->d_adjsets) (void)hipFree(p->d_adjsets);
   delete p;
 }
 

@@ -451,32 +451,77 @@ __device__ __forceinline__ void adj_e_body(
 
 // ---------------------------------------------------------------------------
 // Software-pipelined bodies of the two adjoint kernels: the same terms, sums
-// and order as adj_h_body / adj_e_body (so the same bits), but every
-// full-width load of plane k+1 is issued before the arithmetic of plane k,
-// as the forward two-pass kernels do, instead of each plane waiting for its
-// own loads.  The E (H) terms are evaluated from loaded values by eterm_v
-// (hterm_v), which perform eterm's (hterm's) arithmetic.
+// and order as adj_h_body / adj_e_body (so the same bits), but every load of
+// plane k+1 -- the fields, and under CPML every psi value and z table row
+// the plane's terms use -- is issued before the arithmetic of plane k, as
+// the forward two-pass kernels do, instead of each plane (and each psi term
+// inside it) waiting for its own loads.  The x and y table rows are loaded
+// once per march.  eterm_q / hterm_q / psi_*_q perform the arithmetic of
+// eterm / hterm / psi_h_defer / psi_e_update on the preloaded values.
 // ---------------------------------------------------------------------------
 
+// CPML data of one coordinate of one axis, loaded once and reused: whether
+// it lies in the axis's slabs, its slab index (0 outside them, so that
+// addresses built from it stay inside the psi arrays) and the three table
+// values of the E (rows 0-2: b, c, 1/kappa) or H (rows 3-5) recursion.
+struct AxAt {
+  bool in;
+  int s;
+  float b, c, k;
+};
+
+template <int AX, int ROW>
+__device__ __forceinline__ AxAt ax_at(const AdjArgs& a, int v) {
+  const int n = extent<AX>(a);
+  const float* t = adj_tab<AX>(a);
+  AxAt r;
+  r.in = v < a.p || v >= n - a.p;
+  r.s = r.in ? slab_index(v, n, a.p) : 0;
+  r.b = t[ROW * n + v];
+  r.c = t[(ROW + 1) * n + v];
+  r.k = t[(ROW + 2) * n + v];
+  return r;
+}
+
+// adjoint psi of slot S at (i, j, k) from the slab index s of the slot's
+// axis coordinate (adj_psi with the slab index given)
+template <int S>
+__device__ __forceinline__ float* adj_psi_s(const AdjArgs& a, int i, int j, int k, int s) {
+  unsigned o;
+  if (S < 4) o = ((unsigned)(k * 2 * a.p + s)) * a.nx + i;
+  else if (S < 8) o = ((unsigned)(s * a.ny + j)) * a.nx + i;
+  else o = ((unsigned)k * a.ny + j) * 2 * a.p + s;
+  return a.lpsi + adj_slot(a, S) + o;
+}
+
+// the psi value of slot S (axis AX) a term at (i, j, k) needs: loaded when
+// the axis is compiled in; y and z coordinates are uniform across the
+// wavefront, so outside their slabs the load is skipped by a uniform
+// branch, while an x coordinate outside its slab loads from the clamped
+// (valid) slab index 0 and the value is not selected
+template <int CPX, int AX, int S>
+__device__ __forceinline__ float psi_pre(const AdjArgs& a, int i, int j, int k, const AxAt& x) {
+  if (!axis_on<CPX, AX>()) return 0.f;
+  if (AX != 0 && !x.in) return 0.f;
+  return *adj_psi_s<S>(a, i, j, k, x.s);
+}
+
+// eterm with the psi values and table rows given: the same arithmetic
 template <int MODE, int CPX, int C>
-__device__ __forceinline__ void eterm_v(const AdjArgs& a, float lam, float cb, int i, int j,
-                                        int k, bool valid, float& ga, float& gb) {
+__device__ __forceinline__ void eterm_q(const AdjArgs& a, float lam, float cb, int i, int j,
+                                        int k, bool valid, const AxAt& xa_, float psa,
+                                        const AxAt& xb_, float psb, float& ga, float& gb) {
   using T = Terms<C>;
   const float gU = cb * (adj_damp<MODE>(a, i, j, k) * lam);
   ga = gU;
   gb = -gU;
-  const int p = a.p;
-  const int va = coord<T::axa>(i, j, k), na = extent<T::axa>(a);
-  const int vb = coord<T::axb>(i, j, k), nb = extent<T::axb>(a);
-  if (axis_on<CPX, T::axa>() && (va < p || va >= na - p)) {
-    const float* t = adj_tab<T::axa>(a);
-    const float L = *adj_psi<T::ea>(a, i, j, k) + gU;
-    ga = t[2 * na + va] * gU + t[1 * na + va] * L;
+  if (axis_on<CPX, T::axa>() && xa_.in) {
+    const float L = psa + gU;
+    ga = xa_.k * gU + xa_.c * L;
   }
-  if (axis_on<CPX, T::axb>() && (vb < p || vb >= nb - p)) {
-    const float* t = adj_tab<T::axb>(a);
-    const float L = *adj_psi<T::eb>(a, i, j, k) - gU;
-    gb = -t[2 * nb + vb] * gU + t[1 * nb + vb] * L;
+  if (axis_on<CPX, T::axb>() && xb_.in) {
+    const float L = psb - gU;
+    gb = -xb_.k * gU + xb_.c * L;
   }
   if (!(valid && e_upd<MODE, C>(a, i, j, k))) {
     ga = 0.f;
@@ -484,25 +529,22 @@ __device__ __forceinline__ void eterm_v(const AdjArgs& a, float lam, float cb, i
   }
 }
 
+// hterm with the psi values and table rows given: the same arithmetic
 template <int MODE, int CPX, int C>
-__device__ __forceinline__ void hterm_v(const AdjArgs& a, float lamh, int i, int j, int k,
-                                        bool valid, float& ga, float& gb) {
+__device__ __forceinline__ void hterm_q(const AdjArgs& a, float lamh, int i, int j, int k,
+                                        bool valid, const AxAt& xa_, float psa, const AxAt& xb_,
+                                        float psb, float& ga, float& gb) {
   using T = Terms<C>;
   const float gU = -a.ch * lamh;
   ga = gU;
   gb = -gU;
-  const int p = a.p;
-  const int va = coord<T::axa>(i, j, k), na = extent<T::axa>(a);
-  const int vb = coord<T::axb>(i, j, k), nb = extent<T::axb>(a);
-  if (axis_on<CPX, T::axa>() && (va < p || va >= na - p)) {
-    const float* t = adj_tab<T::axa>(a);
-    const float L = *adj_psi<T::ha>(a, i, j, k) + gU;
-    ga = t[5 * na + va] * gU + t[4 * na + va] * L;
+  if (axis_on<CPX, T::axa>() && xa_.in) {
+    const float L = psa + gU;
+    ga = xa_.k * gU + xa_.c * L;
   }
-  if (axis_on<CPX, T::axb>() && (vb < p || vb >= nb - p)) {
-    const float* t = adj_tab<T::axb>(a);
-    const float L = *adj_psi<T::hb>(a, i, j, k) - gU;
-    gb = -t[5 * nb + vb] * gU + t[4 * nb + vb] * L;
+  if (axis_on<CPX, T::axb>() && xb_.in) {
+    const float L = psb - gU;
+    gb = -xb_.k * gU + xb_.c * L;
   }
   if (!(valid && h_upd<MODE, C>(a, i, j, k))) {
     ga = 0.f;
@@ -510,11 +552,39 @@ __device__ __forceinline__ void hterm_v(const AdjArgs& a, float lamh, int i, int
   }
 }
 
+// psi_h_defer with the old psi values and table rows given
+template <int MODE, int CPX, int C>
+__device__ __forceinline__ void psi_h_defer_q(const AdjArgs& a, float lamh, int i, int j, int k,
+                                              const AxAt& xa_, float psa, const AxAt& xb_,
+                                              float psb) {
+  using T = Terms<C>;
+  if (!h_upd<MODE, C>(a, i, j, k)) return;
+  const float gU = -a.ch * lamh;
+  if (axis_on<CPX, T::axa>() && xa_.in) *adj_psi_s<T::ha>(a, i, j, k, xa_.s) = xa_.b * (psa + gU);
+  if (axis_on<CPX, T::axb>() && xb_.in) *adj_psi_s<T::hb>(a, i, j, k, xb_.s) = xb_.b * (psb - gU);
+}
+
+// psi_e_update with the old psi values and table rows given
+template <int CPX, int C>
+__device__ __forceinline__ void psi_e_update_q(const AdjArgs& a, float gU, int i, int j, int k,
+                                               const AxAt& xa_, float psa, const AxAt& xb_,
+                                               float psb) {
+  using T = Terms<C>;
+  if (axis_on<CPX, T::axa>() && xa_.in) *adj_psi_s<T::ea>(a, i, j, k, xa_.s) = xa_.b * (psa + gU);
+  if (axis_on<CPX, T::axb>() && xb_.in) *adj_psi_s<T::eb>(a, i, j, k, xb_.s) = xb_.b * (psb - gU);
+}
+
 // loads of one plane of kernel A: E adjoint (and Cb) of the cell's z+1, y+1
 // and (edge lanes) x+1 neighbours and of the cell itself, H adjoint of the
-// cell
+// cell, and under CPML every psi value the plane's terms use, with the z
+// table rows of that plane
 struct AdjHRaw {
   float ez, ex1, ey1, cb1, ezy, exy, cby, hx, hy, hz, ezx, eyx, cbx;
+  // E psi: Ez at the cell (x, y terms), Ex / Ey at z+1, Ez / Ey at x+1
+  // (edge lanes), Ez / Ex at y+1; H psi of the cell (deferred update)
+  float pEzx, pEzy, pExy1, pExz1, pEyz1, pEyx1, pEzxX, pEyxX, pEzyY, pExyY;
+  float pHxy, pHxz, pHyz, pHyx, pHzx, pHzy;
+  AxAt zE1, zH;  // z rows: E at k+1, H at k
 };
 
 template <int MODE, int CPX>
@@ -536,7 +606,16 @@ __device__ __forceinline__ void adj_h_body_p(
   const int lane = (int)(threadIdx.x % kBX);
   const int seg = tp.pair ? 32 : kBX;
   const bool xp_lane = (lane % seg) < seg - 1 && ip == i + 1 && i + 1 < xl;
-  const unsigned P = (unsigned)nx * ny;
+  // x and y table rows of the march (the column and the row are fixed)
+  AxAt xE{}, xEp{}, yE{}, yEp{}, xH{}, yH{};
+  if (CPX > 0) {
+    xE = ax_at<0, 0>(a, i);
+    xEp = ax_at<0, 0>(a, ip);
+    yE = ax_at<1, 0>(a, j);
+    yEp = ax_at<1, 0>(a, jp);
+    xH = ax_at<0, 3>(a, i);
+    yH = ax_at<1, 3>(a, j);
+  }
   auto load = [&](int k) __attribute__((always_inline)) {
     AdjHRaw r;
     bool vz;
@@ -555,11 +634,34 @@ __device__ __forceinline__ void adj_h_body_p(
     r.hy = lhy[c];
     r.hz = lhz[c];
     r.ezx = r.eyx = r.cbx = 0.f;
+    r.pEzxX = r.pEyxX = 0.f;
     if (!xp_lane) {
       const unsigned cx = ((unsigned)k * ny + j) * nx + ip;
       r.ezx = lez[cx];
       r.eyx = ley[cx];
       r.cbx = a.cb[cx];
+      if (CPX > 0) {
+        r.pEzxX = psi_pre<CPX, 0, kEzx>(a, ip, j, k, xEp);
+        r.pEyxX = psi_pre<CPX, 0, kEyx>(a, ip, j, k, xEp);
+      }
+    }
+    if (CPX > 0) {
+      r.zE1 = ax_at<2, 0>(a, kp);
+      r.zH = ax_at<2, 3>(a, k);
+      r.pEzx = psi_pre<CPX, 0, kEzx>(a, i, j, k, xE);
+      r.pEzy = psi_pre<CPX, 1, kEzy>(a, i, j, k, yE);
+      r.pExy1 = psi_pre<CPX, 1, kExy>(a, i, j, kp, yE);
+      r.pExz1 = psi_pre<CPX, 2, kExz>(a, i, j, kp, r.zE1);
+      r.pEyz1 = psi_pre<CPX, 2, kEyz>(a, i, j, kp, r.zE1);
+      r.pEyx1 = psi_pre<CPX, 0, kEyx>(a, i, j, kp, xE);
+      r.pEzyY = psi_pre<CPX, 1, kEzy>(a, i, jp, k, yEp);
+      r.pExyY = psi_pre<CPX, 1, kExy>(a, i, jp, k, yEp);
+      r.pHxy = psi_pre<CPX, 1, kHxy>(a, i, j, k, yH);
+      r.pHxz = psi_pre<CPX, 2, kHxz>(a, i, j, k, r.zH);
+      r.pHyz = psi_pre<CPX, 2, kHyz>(a, i, j, k, r.zH);
+      r.pHyx = psi_pre<CPX, 0, kHyx>(a, i, j, k, xH);
+      r.pHzx = psi_pre<CPX, 0, kHzx>(a, i, j, k, xH);
+      r.pHzy = psi_pre<CPX, 1, kHzy>(a, i, j, k, yH);
     }
     return r;
   };
@@ -568,8 +670,8 @@ __device__ __forceinline__ void adj_h_body_p(
   {
     const unsigned c0 = ((unsigned)k0 * ny + j) * nx + i;
     cbc = a.cb[c0];
-    eterm_v<MODE, CPX, 0>(a, lex[c0], cbc, i, j, k0, true, cxa, cxb);
-    eterm_v<MODE, CPX, 1>(a, ley[c0], cbc, i, j, k0, true, cya, cyb);
+    eterm<MODE, CPX, 0>(a, lex, i, j, k0, true, cxa, cxb);
+    eterm<MODE, CPX, 1>(a, ley, i, j, k0, true, cya, cyb);
   }
   AdjHRaw cur = load(k0);
   for (int k = k0; k < k1; ++k) {
@@ -579,15 +681,19 @@ __device__ __forceinline__ void adj_h_body_p(
     const unsigned c = ((unsigned)k * ny + j) * nx + i;
     const float hx0 = cur.hx, hy0 = cur.hy, hz0 = cur.hz;
     if (CPX > 0) {
-      psi_h_defer<MODE, CPX, 0>(a, hx0, i, j, k);
-      psi_h_defer<MODE, CPX, 1>(a, hy0, i, j, k);
-      psi_h_defer<MODE, CPX, 2>(a, hz0, i, j, k);
+      psi_h_defer_q<MODE, CPX, 0>(a, hx0, i, j, k, yH, cur.pHxy, cur.zH, cur.pHxz);
+      psi_h_defer_q<MODE, CPX, 1>(a, hy0, i, j, k, cur.zH, cur.pHyz, xH, cur.pHyx);
+      psi_h_defer_q<MODE, CPX, 2>(a, hz0, i, j, k, xH, cur.pHzx, yH, cur.pHzy);
     }
     const float xa = cxa, xb = cxb, ya = cya, yb = cyb;
     float za, zb, t0, t1;
-    eterm_v<MODE, CPX, 2>(a, cur.ez, cbc, i, j, k, true, za, zb);
-    eterm_v<MODE, CPX, 0>(a, cur.ex1, cur.cb1, i, j, kp, vz, cxa, cxb);
-    eterm_v<MODE, CPX, 1>(a, cur.ey1, cur.cb1, i, j, kp, vz, cya, cyb);
+    // Ez: a along x, b along y; Ex: a along y, b along z; Ey: a along z,
+    // b along x
+    eterm_q<MODE, CPX, 2>(a, cur.ez, cbc, i, j, k, true, xE, cur.pEzx, yE, cur.pEzy, za, zb);
+    eterm_q<MODE, CPX, 0>(a, cur.ex1, cur.cb1, i, j, kp, vz, yE, cur.pExy1, cur.zE1, cur.pExz1,
+                          cxa, cxb);
+    eterm_q<MODE, CPX, 1>(a, cur.ey1, cur.cb1, i, j, kp, vz, cur.zE1, cur.pEyz1, xE, cur.pEyx1,
+                          cya, cyb);
     const float sh_za = __shfl_down(za, 1, kBX);
     const float sh_yb = __shfl_down(yb, 1, kBX);
     float zpa, ypb;
@@ -595,12 +701,13 @@ __device__ __forceinline__ void adj_h_body_p(
       zpa = sh_za;
       ypb = sh_yb;
     } else {
-      eterm_v<MODE, CPX, 2>(a, cur.ezx, cur.cbx, ip, j, k, vx, zpa, t1);
-      eterm_v<MODE, CPX, 1>(a, cur.eyx, cur.cbx, ip, j, k, vx, t0, ypb);
+      eterm_q<MODE, CPX, 2>(a, cur.ezx, cur.cbx, ip, j, k, vx, xEp, cur.pEzxX, yE, 0.f, zpa, t1);
+      eterm_q<MODE, CPX, 1>(a, cur.eyx, cur.cbx, ip, j, k, vx, cur.zE1, 0.f, xEp, cur.pEyxX, t0,
+                            ypb);
     }
     float s = hx0;
     s += ya - cya;
-    eterm_v<MODE, CPX, 2>(a, cur.ezy, cur.cby, i, jp, k, vy, t0, t1);
+    eterm_q<MODE, CPX, 2>(a, cur.ezy, cur.cby, i, jp, k, vy, xE, 0.f, yEp, cur.pEzyY, t0, t1);
     s += zb - t1;
     lhx[c] = s;
     s = hy0;
@@ -608,7 +715,8 @@ __device__ __forceinline__ void adj_h_body_p(
     s += za - zpa;
     lhy[c] = s;
     s = hz0;
-    eterm_v<MODE, CPX, 0>(a, cur.exy, cur.cby, i, jp, k, vy, t0, t1);
+    eterm_q<MODE, CPX, 0>(a, cur.exy, cur.cby, i, jp, k, vy, yEp, cur.pExyY, cur.zE1, 0.f, t0,
+                          t1);
     s += xa - t0;
     s += yb - ypb;
     lhz[c] = s;
@@ -619,9 +727,15 @@ __device__ __forceinline__ void adj_h_body_p(
 
 // loads of one plane of kernel B: E adjoint, Cb (Ca) and H adjoint of the
 // cell, H adjoint of the y-1 neighbour and (edge lanes) of the x-1
-// neighbour
+// neighbour, and under CPML every psi value the plane's terms use, with the
+// z table rows of that plane
 struct AdjERaw {
   float ex, ey, ez, cb, ca, hx, hy, hz, hzm, hxm, hzx, hyx;
+  // H psi: the cell's Hx, Hy, Hz (both terms), Hz / Hy at x-1 (edge
+  // lanes), Hz / Hx at y-1; E psi of the cell (its update)
+  float pHxy, pHxz, pHyz, pHyx, pHzx, pHzy, pHzxX, pHyxX, pHzyY, pHxyY;
+  float pExy, pExz, pEyz, pEyx, pEzx, pEzy;
+  AxAt zH, zE;  // z rows at k: H, E
 };
 
 template <int MODE, bool LOSSY, int CPX>
@@ -643,6 +757,15 @@ __device__ __forceinline__ void adj_e_body_p(
   const int lane = (int)(threadIdx.x % kBX);
   const int seg = tp.pair ? 32 : kBX;
   const bool xm_lane = (lane % seg) > 0 && im == i - 1;
+  AxAt xH{}, xHm{}, yH{}, yHm{}, xE{}, yE{};
+  if (CPX > 0) {
+    xH = ax_at<0, 3>(a, i);
+    xHm = ax_at<0, 3>(a, im);
+    yH = ax_at<1, 3>(a, j);
+    yHm = ax_at<1, 3>(a, jm);
+    xE = ax_at<0, 0>(a, i);
+    yE = ax_at<1, 0>(a, j);
+  }
   auto load = [&](int k) __attribute__((always_inline)) {
     AdjERaw r;
     const unsigned c = ((unsigned)k * ny + j) * nx + i;
@@ -658,10 +781,33 @@ __device__ __forceinline__ void adj_e_body_p(
     r.hzm = lhz[cm];
     r.hxm = lhx[cm];
     r.hzx = r.hyx = 0.f;
+    r.pHzxX = r.pHyxX = 0.f;
     if (!xm_lane) {
       const unsigned cx = ((unsigned)k * ny + j) * nx + im;
       r.hzx = lhz[cx];
       r.hyx = lhy[cx];
+      if (CPX > 0) {
+        r.pHzxX = psi_pre<CPX, 0, kHzx>(a, im, j, k, xHm);
+        r.pHyxX = psi_pre<CPX, 0, kHyx>(a, im, j, k, xHm);
+      }
+    }
+    if (CPX > 0) {
+      r.zH = ax_at<2, 3>(a, k);
+      r.zE = ax_at<2, 0>(a, k);
+      r.pHxy = psi_pre<CPX, 1, kHxy>(a, i, j, k, yH);
+      r.pHxz = psi_pre<CPX, 2, kHxz>(a, i, j, k, r.zH);
+      r.pHyz = psi_pre<CPX, 2, kHyz>(a, i, j, k, r.zH);
+      r.pHyx = psi_pre<CPX, 0, kHyx>(a, i, j, k, xH);
+      r.pHzx = psi_pre<CPX, 0, kHzx>(a, i, j, k, xH);
+      r.pHzy = psi_pre<CPX, 1, kHzy>(a, i, j, k, yH);
+      r.pHzyY = psi_pre<CPX, 1, kHzy>(a, i, jm, k, yHm);
+      r.pHxyY = psi_pre<CPX, 1, kHxy>(a, i, jm, k, yHm);
+      r.pExy = psi_pre<CPX, 1, kExy>(a, i, j, k, yE);
+      r.pExz = psi_pre<CPX, 2, kExz>(a, i, j, k, r.zE);
+      r.pEyz = psi_pre<CPX, 2, kEyz>(a, i, j, k, r.zE);
+      r.pEyx = psi_pre<CPX, 0, kEyx>(a, i, j, k, xE);
+      r.pEzx = psi_pre<CPX, 0, kEzx>(a, i, j, k, xE);
+      r.pEzy = psi_pre<CPX, 1, kEzy>(a, i, j, k, yE);
     }
     return r;
   };
@@ -681,22 +827,26 @@ __device__ __forceinline__ void adj_e_body_p(
     const float ca = cur.ca;
     const float g0 = d * cur.ex, g1 = d * cur.ey, g2 = d * cur.ez;
     float e0 = g0, e1 = g1, e2 = g2;
+    // Ex: a along y, b along z; Ey: a along z, b along x; Ez: a along x,
+    // b along y
     if (e_upd<MODE, 0>(a, i, j, k)) {
-      if (CPX > 0) psi_e_update<CPX, 0>(a, cb * g0, i, j, k);
+      if (CPX > 0) psi_e_update_q<CPX, 0>(a, cb * g0, i, j, k, yE, cur.pExy, cur.zE, cur.pExz);
       e0 = LOSSY ? ca * g0 : g0;
     }
     if (e_upd<MODE, 1>(a, i, j, k)) {
-      if (CPX > 0) psi_e_update<CPX, 1>(a, cb * g1, i, j, k);
+      if (CPX > 0) psi_e_update_q<CPX, 1>(a, cb * g1, i, j, k, cur.zE, cur.pEyz, xE, cur.pEyx);
       e1 = LOSSY ? ca * g1 : g1;
     }
     if (e_upd<MODE, 2>(a, i, j, k)) {
-      if (CPX > 0) psi_e_update<CPX, 2>(a, cb * g2, i, j, k);
+      if (CPX > 0) psi_e_update_q<CPX, 2>(a, cb * g2, i, j, k, xE, cur.pEzx, yE, cur.pEzy);
       e2 = LOSSY ? ca * g2 : g2;
     }
+    // Hx: a along y, b along z; Hy: a along z, b along x; Hz: a along x,
+    // b along y
     float xa, xb, ya, yb, za, zb, t0, t1;
-    hterm_v<MODE, CPX, 0>(a, cur.hx, i, j, k, true, xa, xb);
-    hterm_v<MODE, CPX, 1>(a, cur.hy, i, j, k, true, ya, yb);
-    hterm_v<MODE, CPX, 2>(a, cur.hz, i, j, k, true, za, zb);
+    hterm_q<MODE, CPX, 0>(a, cur.hx, i, j, k, true, yH, cur.pHxy, cur.zH, cur.pHxz, xa, xb);
+    hterm_q<MODE, CPX, 1>(a, cur.hy, i, j, k, true, cur.zH, cur.pHyz, xH, cur.pHyx, ya, yb);
+    hterm_q<MODE, CPX, 2>(a, cur.hz, i, j, k, true, xH, cur.pHzx, yH, cur.pHzy, za, zb);
     const float sh_za = __shfl_up(za, 1, kBX);
     const float sh_yb = __shfl_up(yb, 1, kBX);
     float zma, ymb;
@@ -704,15 +854,15 @@ __device__ __forceinline__ void adj_e_body_p(
       zma = sh_za;
       ymb = sh_yb;
     } else {
-      hterm_v<MODE, CPX, 2>(a, cur.hzx, im, j, k, vx, zma, t1);
-      hterm_v<MODE, CPX, 1>(a, cur.hyx, im, j, k, vx, t0, ymb);
+      hterm_q<MODE, CPX, 2>(a, cur.hzx, im, j, k, vx, xHm, cur.pHzxX, yH, 0.f, zma, t1);
+      hterm_q<MODE, CPX, 1>(a, cur.hyx, im, j, k, vx, cur.zH, 0.f, xHm, cur.pHyxX, t0, ymb);
     }
     e0 += pya - ya;
-    hterm_v<MODE, CPX, 2>(a, cur.hzm, i, jm, k, vy, t0, t1);
+    hterm_q<MODE, CPX, 2>(a, cur.hzm, i, jm, k, vy, xH, 0.f, yHm, cur.pHzyY, t0, t1);
     e0 += t1 - zb;
     e1 += pxb - xb;
     e1 += zma - za;
-    hterm_v<MODE, CPX, 0>(a, cur.hxm, i, jm, k, vy, t0, t1);
+    hterm_q<MODE, CPX, 0>(a, cur.hxm, i, jm, k, vy, yHm, cur.pHxyY, cur.zH, 0.f, t0, t1);
     e2 += t0 - xa;
     e2 += ymb - yb;
     pxa = xa;
@@ -763,9 +913,10 @@ __device__ __forceinline__ void adj_e_body_p(
 // neighbour rows): they are built for 8 waves per SIMD.  Of their uniform
 // values the ones beyond that SGPR budget live in VGPR lanes.
 #define YEE_ADJ_OCCUPANCY __attribute__((amdgpu_waves_per_eu(8)))
-// the pipelined bodies hold two planes' loads: at least 6 waves per SIMD,
-// which their registers allow without spills
-#define YEE_ADJ_OCCUPANCY_P __attribute__((amdgpu_waves_per_eu(6)))
+// the pipelined bodies hold two planes' loads, psi values included (82-89
+// VGPRs): at least 5 waves per SIMD, which their registers allow without
+// spills (at 6 they spill to scratch)
+#define YEE_ADJ_OCCUPANCY_P __attribute__((amdgpu_waves_per_eu(5)))
 
 // One launch per kernel and tile set (no CPML: the whole grid is one set).
 template <int MODE, int CPX, bool PIPE>
@@ -817,9 +968,10 @@ struct AdjSets {
 
 template <int MODE, bool PIPE>
 __global__ __launch_bounds__(kThreads) YEE_ADJ_OCCUPANCY void yee_adj_h3_kernel(
-    AdjArgs a, AdjSets sets, const float* __restrict__ lex, const float* __restrict__ ley,
+    AdjArgs a, const AdjSets* __restrict__ setsp, const float* __restrict__ lex, const float* __restrict__ ley,
     const float* __restrict__ lez, float* __restrict__ lhx, float* __restrict__ lhy,
     float* __restrict__ lhz) {
+  const AdjSets& sets = *setsp;
   int L = (int)blockIdx.x;
   if (L < sets.nb_in) {
     if (PIPE) adj_h_body_p<MODE, 0>(a, sets.in, L, lex, ley, lez, lhx, lhy, lhz);
@@ -838,9 +990,10 @@ __global__ __launch_bounds__(kThreads) YEE_ADJ_OCCUPANCY void yee_adj_h3_kernel(
 
 template <int MODE, bool LOSSY, bool PIPE>
 __global__ __launch_bounds__(kThreads) YEE_ADJ_OCCUPANCY void yee_adj_e3_kernel(
-    AdjArgs a, AdjSets sets, float* __restrict__ lex, float* __restrict__ ley,
+    AdjArgs a, const AdjSets* __restrict__ setsp, float* __restrict__ lex, float* __restrict__ ley,
     float* __restrict__ lez, const float* __restrict__ lhx, const float* __restrict__ lhy,
     const float* __restrict__ lhz) {
+  const AdjSets& sets = *setsp;
   int L = (int)blockIdx.x;
   if (L < sets.nb_in) {
     if (PIPE) adj_e_body_p<MODE, LOSSY, 0>(a, sets.in, L, lex, ley, lez, lhx, lhy, lhz);
@@ -859,9 +1012,10 @@ __global__ __launch_bounds__(kThreads) YEE_ADJ_OCCUPANCY void yee_adj_e3_kernel(
 
 template <int MODE, bool PIPE>
 __global__ __launch_bounds__(kThreads) YEE_ADJ_OCCUPANCY_P void yee_adj_h3p_kernel(
-    AdjArgs a, AdjSets sets, const float* __restrict__ lex, const float* __restrict__ ley,
+    AdjArgs a, const AdjSets* __restrict__ setsp, const float* __restrict__ lex, const float* __restrict__ ley,
     const float* __restrict__ lez, float* __restrict__ lhx, float* __restrict__ lhy,
     float* __restrict__ lhz) {
+  const AdjSets& sets = *setsp;
   int L = (int)blockIdx.x;
   if (L < sets.nb_in) {
     if (PIPE) adj_h_body_p<MODE, 0>(a, sets.in, L, lex, ley, lez, lhx, lhy, lhz);
@@ -880,9 +1034,10 @@ __global__ __launch_bounds__(kThreads) YEE_ADJ_OCCUPANCY_P void yee_adj_h3p_kern
 
 template <int MODE, bool LOSSY, bool PIPE>
 __global__ __launch_bounds__(kThreads) YEE_ADJ_OCCUPANCY_P void yee_adj_e3p_kernel(
-    AdjArgs a, AdjSets sets, float* __restrict__ lex, float* __restrict__ ley,
+    AdjArgs a, const AdjSets* __restrict__ setsp, float* __restrict__ lex, float* __restrict__ ley,
     float* __restrict__ lez, const float* __restrict__ lhx, const float* __restrict__ lhy,
     const float* __restrict__ lhz) {
+  const AdjSets& sets = *setsp;
   int L = (int)blockIdx.x;
   if (L < sets.nb_in) {
     if (PIPE) adj_e_body_p<MODE, LOSSY, 0>(a, sets.in, L, lex, ley, lez, lhx, lhy, lhz);
@@ -1030,8 +1185,35 @@ int adjoint_args(const YeePlan* p, float* const* lam, float* lam_psi, const YeeD
 
 // Both kernels of one adjoint step, each as two launches under CPML: tiles
 // with no slab within one cell (no psi code) and the slab shell.
+// Device copy of the CPML launches' tile sets, kept on the plan and copied
+// again only when they change (grid, tile order, planes per tile).  The
+// copy is synchronous, so the launches that follow read the new sets.
+const AdjSets* adj_sets_device(const YeePlan* p, const AdjSets& sets) {
+  if (!p->d_adjsets) {
+    void* d = nullptr;
+    if (hipMalloc(&d, sizeof(AdjSets)) != hipSuccess) {
+      fail(YEE_EHIP, "adjoint tile sets: device allocation failed");
+      return nullptr;
+    }
+    p->d_adjsets = d;
+    p->h_adjsets.clear();
+  }
+  const unsigned char* raw = reinterpret_cast<const unsigned char*>(&sets);
+  if (p->h_adjsets.size() != sizeof(AdjSets) ||
+      std::memcmp(p->h_adjsets.data(), raw, sizeof(AdjSets)) != 0) {
+    p->h_adjsets.clear();
+    const hipError_t e = hipMemcpy(p->d_adjsets, raw, sizeof(AdjSets), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      fail(YEE_EHIP, std::string("adjoint tile sets: ") + hipGetErrorString(e));
+      return nullptr;
+    }
+    p->h_adjsets.assign(raw, raw + sizeof(AdjSets));
+  }
+  return static_cast<const AdjSets*>(p->d_adjsets);
+}
+
 template <int MODE, bool LOSSY, bool PIPE>
-void adjoint_launch_t(const YeePlan* p, const AdjArgs& a, float* const* lam, hipStream_t st) {
+int adjoint_launch_t(const YeePlan* p, const AdjArgs& a, float* const* lam, hipStream_t st) {
   TileSet in, shn, shx;
   const int order = p->xcd < 0 ? 1 : p->xcd;
   if (a.p > 0) {
@@ -1059,17 +1241,23 @@ void adjoint_launch_t(const YeePlan* p, const AdjArgs& a, float* const* lam, hip
     sets.nb_in = nblocks(in);
     sets.nb_shn = nblocks(shn);
     const int total = sets.nb_in + sets.nb_shn + nblocks(shx);
+    // the tile sets reach the kernels through a pointer: as kernel
+    // arguments (about 0.9 KB) they filled the scalar registers, and the
+    // kernels spent their issue slots moving spilled values in and out of
+    // vector-register lanes
+    const AdjSets* dsets = adj_sets_device(p, sets);
+    if (!dsets) return YEE_EHIP;
     if (total > 0) {
       if constexpr (PIPE) {
-        yee_adj_h3p_kernel<MODE, true><<<total, kThreads, 0, st>>>(a, sets, lam[0], lam[1],
+        yee_adj_h3p_kernel<MODE, true><<<total, kThreads, 0, st>>>(a, dsets, lam[0], lam[1],
                                                                    lam[2], lam[3], lam[4], lam[5]);
         yee_adj_e3p_kernel<MODE, LOSSY, true><<<total, kThreads, 0, st>>>(
-            a, sets, lam[0], lam[1], lam[2], lam[3], lam[4], lam[5]);
+            a, dsets, lam[0], lam[1], lam[2], lam[3], lam[4], lam[5]);
       } else {
-        yee_adj_h3_kernel<MODE, false><<<total, kThreads, 0, st>>>(a, sets, lam[0], lam[1],
+        yee_adj_h3_kernel<MODE, false><<<total, kThreads, 0, st>>>(a, dsets, lam[0], lam[1],
                                                                    lam[2], lam[3], lam[4], lam[5]);
         yee_adj_e3_kernel<MODE, LOSSY, false><<<total, kThreads, 0, st>>>(
-            a, sets, lam[0], lam[1], lam[2], lam[3], lam[4], lam[5]);
+            a, dsets, lam[0], lam[1], lam[2], lam[3], lam[4], lam[5]);
       }
     }
   } else {
@@ -1087,17 +1275,19 @@ void adjoint_launch_t(const YeePlan* p, const AdjArgs& a, float* const* lam, hip
       }
     }
   }
+  return YEE_OK;
 }
 
 int adjoint_launch(const YeePlan* p, const AdjArgs& a, float* const* lam, hipStream_t st) {
   const bool lossy = p->d.ca != nullptr;
+  int rc = YEE_OK;
 #define YEE_ADJ(M)                                                       \
   if (p->adj_pipe) {                                                     \
-    if (lossy) adjoint_launch_t<M, true, true>(p, a, lam, st);           \
-    else adjoint_launch_t<M, false, true>(p, a, lam, st);                \
+    if (lossy) rc = adjoint_launch_t<M, true, true>(p, a, lam, st);      \
+    else rc = adjoint_launch_t<M, false, true>(p, a, lam, st);           \
   } else {                                                               \
-    if (lossy) adjoint_launch_t<M, true, false>(p, a, lam, st);          \
-    else adjoint_launch_t<M, false, false>(p, a, lam, st);               \
+    if (lossy) rc = adjoint_launch_t<M, true, false>(p, a, lam, st);     \
+    else rc = adjoint_launch_t<M, false, false>(p, a, lam, st);          \
   }
   switch (p->d.boundary) {
     case YEE_BOUNDARY_PERIODIC: YEE_ADJ(kPeriodic) break;
@@ -1106,6 +1296,7 @@ int adjoint_launch(const YeePlan* p, const AdjArgs& a, float* const* lam, hipStr
     default: YEE_ADJ(kWalls) break;
   }
 #undef YEE_ADJ
+  if (rc != YEE_OK) return rc;
   YEE_HIP_CHECK(hipGetLastError());
   return YEE_OK;
 }

@@ -375,6 +375,10 @@ struct YeePlan {
   int* d_comp = nullptr;     // device copies for the unfused tail
   long long* d_idx = nullptr;
   float* d_table = nullptr;
+  // device copy of the adjoint's CPML tile sets and the host bytes it holds
+  // (a cache of what the launches derive from the fields above)
+  mutable void* d_adjsets = nullptr;
+  mutable std::vector<unsigned char> h_adjsets;
 };
 
 
