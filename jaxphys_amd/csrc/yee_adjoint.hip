@@ -959,99 +959,104 @@ __global__ __launch_bounds__(kThreads) YEE_ADJ_OCCUPANCY_P void yee_adj_ep_kerne
 // columns) of one kernel in a single launch.  The workgroup index picks the
 // set (each set's block count is a multiple of 8, so the XCD-aware order
 // within every set is kept); the small, latency-bound shell tiles then run
-// beside the interior tiles instead of in launches of their own.
+// beside the interior tiles instead of in launches of their own.  The
+// x-slab columns, the slowest tiles, take the first workgroups and the
+// interior the last, so the interior's short tiles fill the launch's end.
 struct AdjSets {
   TileSet in, shn, shx;
-  int nb_in, nb_shn;
+  int nb_shx, nb_shn;
 };
-
 
 template <int MODE, bool PIPE>
 __global__ __launch_bounds__(kThreads) YEE_ADJ_OCCUPANCY void yee_adj_h3_kernel(
-    AdjArgs a, const AdjSets* __restrict__ setsp, const float* __restrict__ lex, const float* __restrict__ ley,
-    const float* __restrict__ lez, float* __restrict__ lhx, float* __restrict__ lhy,
-    float* __restrict__ lhz) {
+    AdjArgs a, const AdjSets* __restrict__ setsp, const float* __restrict__ lex,
+    const float* __restrict__ ley, const float* __restrict__ lez, float* __restrict__ lhx,
+    float* __restrict__ lhy, float* __restrict__ lhz) {
   const AdjSets& sets = *setsp;
   int L = (int)blockIdx.x;
-  if (L < sets.nb_in) {
-    if (PIPE) adj_h_body_p<MODE, 0>(a, sets.in, L, lex, ley, lez, lhx, lhy, lhz);
-    else adj_h_body<MODE, 0>(a, sets.in, L, lex, ley, lez, lhx, lhy, lhz);
+  if (L < sets.nb_shx) {
+    if (PIPE) adj_h_body_p<MODE, 2>(a, sets.shx, L, lex, ley, lez, lhx, lhy, lhz);
+    else adj_h_body<MODE, 2>(a, sets.shx, L, lex, ley, lez, lhx, lhy, lhz);
     return;
   }
-  L -= sets.nb_in;
+  L -= sets.nb_shx;
   if (L < sets.nb_shn) {
     if (PIPE) adj_h_body_p<MODE, 1>(a, sets.shn, L, lex, ley, lez, lhx, lhy, lhz);
     else adj_h_body<MODE, 1>(a, sets.shn, L, lex, ley, lez, lhx, lhy, lhz);
     return;
   }
-  if (PIPE) adj_h_body_p<MODE, 2>(a, sets.shx, L - sets.nb_shn, lex, ley, lez, lhx, lhy, lhz);
-  else adj_h_body<MODE, 2>(a, sets.shx, L - sets.nb_shn, lex, ley, lez, lhx, lhy, lhz);
+  L -= sets.nb_shn;
+  if (PIPE) adj_h_body_p<MODE, 0>(a, sets.in, L, lex, ley, lez, lhx, lhy, lhz);
+  else adj_h_body<MODE, 0>(a, sets.in, L, lex, ley, lez, lhx, lhy, lhz);
 }
 
 template <int MODE, bool LOSSY, bool PIPE>
 __global__ __launch_bounds__(kThreads) YEE_ADJ_OCCUPANCY void yee_adj_e3_kernel(
-    AdjArgs a, const AdjSets* __restrict__ setsp, float* __restrict__ lex, float* __restrict__ ley,
-    float* __restrict__ lez, const float* __restrict__ lhx, const float* __restrict__ lhy,
-    const float* __restrict__ lhz) {
+    AdjArgs a, const AdjSets* __restrict__ setsp, float* __restrict__ lex,
+    float* __restrict__ ley, float* __restrict__ lez, const float* __restrict__ lhx,
+    const float* __restrict__ lhy, const float* __restrict__ lhz) {
   const AdjSets& sets = *setsp;
   int L = (int)blockIdx.x;
-  if (L < sets.nb_in) {
-    if (PIPE) adj_e_body_p<MODE, LOSSY, 0>(a, sets.in, L, lex, ley, lez, lhx, lhy, lhz);
-    else adj_e_body<MODE, LOSSY, 0>(a, sets.in, L, lex, ley, lez, lhx, lhy, lhz);
+  if (L < sets.nb_shx) {
+    if (PIPE) adj_e_body_p<MODE, LOSSY, 2>(a, sets.shx, L, lex, ley, lez, lhx, lhy, lhz);
+    else adj_e_body<MODE, LOSSY, 2>(a, sets.shx, L, lex, ley, lez, lhx, lhy, lhz);
     return;
   }
-  L -= sets.nb_in;
+  L -= sets.nb_shx;
   if (L < sets.nb_shn) {
     if (PIPE) adj_e_body_p<MODE, LOSSY, 1>(a, sets.shn, L, lex, ley, lez, lhx, lhy, lhz);
     else adj_e_body<MODE, LOSSY, 1>(a, sets.shn, L, lex, ley, lez, lhx, lhy, lhz);
     return;
   }
-  if (PIPE) adj_e_body_p<MODE, LOSSY, 2>(a, sets.shx, L - sets.nb_shn, lex, ley, lez, lhx, lhy, lhz);
-  else adj_e_body<MODE, LOSSY, 2>(a, sets.shx, L - sets.nb_shn, lex, ley, lez, lhx, lhy, lhz);
+  L -= sets.nb_shn;
+  if (PIPE) adj_e_body_p<MODE, LOSSY, 0>(a, sets.in, L, lex, ley, lez, lhx, lhy, lhz);
+  else adj_e_body<MODE, LOSSY, 0>(a, sets.in, L, lex, ley, lez, lhx, lhy, lhz);
 }
 
 template <int MODE, bool PIPE>
 __global__ __launch_bounds__(kThreads) YEE_ADJ_OCCUPANCY_P void yee_adj_h3p_kernel(
-    AdjArgs a, const AdjSets* __restrict__ setsp, const float* __restrict__ lex, const float* __restrict__ ley,
-    const float* __restrict__ lez, float* __restrict__ lhx, float* __restrict__ lhy,
-    float* __restrict__ lhz) {
+    AdjArgs a, const AdjSets* __restrict__ setsp, const float* __restrict__ lex,
+    const float* __restrict__ ley, const float* __restrict__ lez, float* __restrict__ lhx,
+    float* __restrict__ lhy, float* __restrict__ lhz) {
   const AdjSets& sets = *setsp;
   int L = (int)blockIdx.x;
-  if (L < sets.nb_in) {
-    if (PIPE) adj_h_body_p<MODE, 0>(a, sets.in, L, lex, ley, lez, lhx, lhy, lhz);
-    else adj_h_body<MODE, 0>(a, sets.in, L, lex, ley, lez, lhx, lhy, lhz);
+  if (L < sets.nb_shx) {
+    if (PIPE) adj_h_body_p<MODE, 2>(a, sets.shx, L, lex, ley, lez, lhx, lhy, lhz);
+    else adj_h_body<MODE, 2>(a, sets.shx, L, lex, ley, lez, lhx, lhy, lhz);
     return;
   }
-  L -= sets.nb_in;
+  L -= sets.nb_shx;
   if (L < sets.nb_shn) {
     if (PIPE) adj_h_body_p<MODE, 1>(a, sets.shn, L, lex, ley, lez, lhx, lhy, lhz);
     else adj_h_body<MODE, 1>(a, sets.shn, L, lex, ley, lez, lhx, lhy, lhz);
     return;
   }
-  if (PIPE) adj_h_body_p<MODE, 2>(a, sets.shx, L - sets.nb_shn, lex, ley, lez, lhx, lhy, lhz);
-  else adj_h_body<MODE, 2>(a, sets.shx, L - sets.nb_shn, lex, ley, lez, lhx, lhy, lhz);
+  L -= sets.nb_shn;
+  if (PIPE) adj_h_body_p<MODE, 0>(a, sets.in, L, lex, ley, lez, lhx, lhy, lhz);
+  else adj_h_body<MODE, 0>(a, sets.in, L, lex, ley, lez, lhx, lhy, lhz);
 }
 
 template <int MODE, bool LOSSY, bool PIPE>
 __global__ __launch_bounds__(kThreads) YEE_ADJ_OCCUPANCY_P void yee_adj_e3p_kernel(
-    AdjArgs a, const AdjSets* __restrict__ setsp, float* __restrict__ lex, float* __restrict__ ley,
-    float* __restrict__ lez, const float* __restrict__ lhx, const float* __restrict__ lhy,
-    const float* __restrict__ lhz) {
+    AdjArgs a, const AdjSets* __restrict__ setsp, float* __restrict__ lex,
+    float* __restrict__ ley, float* __restrict__ lez, const float* __restrict__ lhx,
+    const float* __restrict__ lhy, const float* __restrict__ lhz) {
   const AdjSets& sets = *setsp;
   int L = (int)blockIdx.x;
-  if (L < sets.nb_in) {
-    if (PIPE) adj_e_body_p<MODE, LOSSY, 0>(a, sets.in, L, lex, ley, lez, lhx, lhy, lhz);
-    else adj_e_body<MODE, LOSSY, 0>(a, sets.in, L, lex, ley, lez, lhx, lhy, lhz);
+  if (L < sets.nb_shx) {
+    if (PIPE) adj_e_body_p<MODE, LOSSY, 2>(a, sets.shx, L, lex, ley, lez, lhx, lhy, lhz);
+    else adj_e_body<MODE, LOSSY, 2>(a, sets.shx, L, lex, ley, lez, lhx, lhy, lhz);
     return;
   }
-  L -= sets.nb_in;
+  L -= sets.nb_shx;
   if (L < sets.nb_shn) {
     if (PIPE) adj_e_body_p<MODE, LOSSY, 1>(a, sets.shn, L, lex, ley, lez, lhx, lhy, lhz);
     else adj_e_body<MODE, LOSSY, 1>(a, sets.shn, L, lex, ley, lez, lhx, lhy, lhz);
     return;
   }
-  if (PIPE) adj_e_body_p<MODE, LOSSY, 2>(a, sets.shx, L - sets.nb_shn, lex, ley, lez, lhx, lhy, lhz);
-  else adj_e_body<MODE, LOSSY, 2>(a, sets.shx, L - sets.nb_shn, lex, ley, lez, lhx, lhy, lhz);
+  L -= sets.nb_shn;
+  if (PIPE) adj_e_body_p<MODE, LOSSY, 0>(a, sets.in, L, lex, ley, lez, lhx, lhy, lhz);
+  else adj_e_body<MODE, LOSSY, 0>(a, sets.in, L, lex, ley, lez, lhx, lhy, lhz);
 }
 
 __global__ void probe_gather_kernel(GridArgs g, const int* comp, const long long* idx, int n,
@@ -1238,9 +1243,9 @@ int adjoint_launch_t(const YeePlan* p, const AdjArgs& a, float* const* lam, hipS
     sets.in = in;
     sets.shn = shn;
     sets.shx = shx;
-    sets.nb_in = nblocks(in);
+    sets.nb_shx = nblocks(shx);
     sets.nb_shn = nblocks(shn);
-    const int total = sets.nb_in + sets.nb_shn + nblocks(shx);
+    const int total = sets.nb_shx + sets.nb_shn + nblocks(in);
     // the tile sets reach the kernels through a pointer: as kernel
     // arguments (about 0.9 KB) they filled the scalar registers, and the
     // kernels spent their issue slots moving spilled values in and out of

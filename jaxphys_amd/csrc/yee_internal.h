@@ -355,10 +355,11 @@ __device__ __forceinline__ bool tile_of(const Tiling& tl, int& bx, int& by, int&
 struct YeePlan {
   YeeGridDesc d;
   int kz;            // z planes per block of the two-pass kernels
-  int adj_kz = 4;    // z planes per block of the adjoint kernels (160^3
-                     // design workload, software-pipelined bodies: 2, 3,
-                     // 4, 6 planes give 3.67, 3.62, 3.60, 3.72x one
-                     // forward run in the checkpointed mode)
+  int adj_kz = 6;    // z planes per block of the adjoint kernels (160^3
+                     // design workload, software-pipelined bodies with
+                     // preloaded psi: 2, 3, 4, 6, 8 planes give 3.42,
+                     // 3.25, 3.22, 3.19, 3.22x one forward run in the
+                     // checkpointed mode)
   int fused_kz = 0;  // z planes per block of the fused kernel (0: automatic)
   int adj_pipe = 1;  // adjoint kernels: software-pipelined bodies (1) or
                      // loads issued per plane (0); same results
