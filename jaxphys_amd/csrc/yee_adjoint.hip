@@ -736,6 +736,9 @@ struct AdjERaw {
   float pHxy, pHxz, pHyz, pHyx, pHzx, pHzy, pHzxX, pHyxX, pHzyY, pHxyY;
   float pExy, pExz, pEyz, pEyx, pEzx, pEzy;
   AxAt zH, zE;  // z rows at k: H, E
+  // design box: the cell's record d
+  bool inbox;
+  float d0, d1, d2;
 };
 
 template <int MODE, bool LOSSY, int CPX>
@@ -809,6 +812,18 @@ __device__ __forceinline__ void adj_e_body_p(
       r.pEzx = psi_pre<CPX, 0, kEzx>(a, i, j, k, xE);
       r.pEzy = psi_pre<CPX, 1, kEzy>(a, i, j, k, yE);
     }
+    r.inbox = false;
+    if (a.bnx > 0) {
+      const int li = i - a.bx0, lj = j - a.by0, lk = k - a.bz0;
+      r.inbox = li >= 0 && li < a.bnx && lj >= 0 && lj < a.bny && lk >= 0 && lk < a.bnz;
+      if (r.inbox) {
+        const unsigned bc = (unsigned)a.bnx * a.bny * a.bnz;
+        const unsigned l = ((unsigned)lk * a.bny + lj) * a.bnx + li;
+        r.d0 = a.hist_d[l];
+        r.d1 = a.hist_d[bc + l];
+        r.d2 = a.hist_d[2 * bc + l];
+      }
+    }
     return r;
   };
   float pxa, pxb, pya, pyb;
@@ -869,22 +884,23 @@ __device__ __forceinline__ void adj_e_body_p(
     pxb = xb;
     pya = ya;
     pyb = yb;
-    if (a.bnx > 0) {
+    // design box accumulation (cells with damp == 1, all components
+    // updated): the sums of the plain body, the record d preloaded
+    if (cur.inbox) {
       const int li = i - a.bx0, lj = j - a.by0, lk = k - a.bz0;
-      if (li >= 0 && li < a.bnx && lj >= 0 && lj < a.bny && lk >= 0 && lk < a.bnz) {
+      const unsigned l = ((unsigned)lk * a.bny + lj) * a.bnx + li;
+      const float gc[3] = {g0, g1, g2};
+      const float dv[3] = {cur.d0, cur.d1, cur.d2};
+      float s1 = 0.f;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) s1 += gc[q] * dv[q];
+      a.acc_cb[l] += s1;
+      if (LOSSY && a.acc_ca) {
         const unsigned bc = (unsigned)a.bnx * a.bny * a.bnz;
-        const unsigned l = ((unsigned)lk * a.bny + lj) * a.bnx + li;
-        const float gc[3] = {g0, g1, g2};
-        float s1 = 0.f;
+        float s2 = 0.f;
 #pragma unroll
-        for (int q = 0; q < 3; ++q) s1 += gc[q] * a.hist_d[q * bc + l];
-        a.acc_cb[l] += s1;
-        if (LOSSY && a.acc_ca) {
-          float s2 = 0.f;
-#pragma unroll
-          for (int q = 0; q < 3; ++q) s2 += gc[q] * a.hist_e[q * bc + l];
-          a.acc_ca[l] += s2;
-        }
+        for (int q = 0; q < 3; ++q) s2 += gc[q] * a.hist_e[q * bc + l];
+        a.acc_ca[l] += s2;
       }
     }
     lex[c] = e0;
