@@ -300,14 +300,11 @@ class ClosureNet:
         / ``use_far=False`` leave the net / the far-field operator out (for
         measuring what each part adds)."""
         if use_net and self.net_gain > 0:
-            y0 = self.zero_response(solver)
-            x = self.pack_fields(solver, m, every)
-            y = self.forward(x, (solver.nx, solver.ny, solver.nz))
-            y.sub_(y0)
+            y = self.net_correction(solver, m, every)
             if self.net_gain != 1.0:
                 y.mul_(self.net_gain)
         else:
-            y = self._buf("y", (solver.nx * solver.ny * solver.nz, OUT_STRIDE), torch.float32)
+            y = self._buf("ycorr", (solver.nx * solver.ny * solver.nz, OUT_STRIDE), torch.float32)
             y.zero_()
         shape = (solver.nx, solver.ny, solver.nz)
         box = self._box_terms(solver, shape, (0, 0, 0), every)[0] if self.forcing else None
@@ -339,6 +336,41 @@ class ClosureNet:
         if patch is not None:
             patch.apply(solver, m // every, gain)
 
+    # the net's receptive field: three 3x3x3 layers
+    HALO = 3
+
+    def _crop(self, solver):
+        """(z, y, x) slices of the box the net is evaluated on: the cells the
+        correction is applied to (``margin`` or more inside the faces) and
+        the net's receptive field around them, clamped to the grid."""
+        lo = max(0, self.margin - self.HALO)
+        return tuple(slice(lo, max(lo, n - lo)) for n in (solver.nz, solver.ny, solver.nx))
+
+    def net_correction(self, solver, m: int | None = None, every: int | None = None):
+        """net(state) - net(zero state) on the whole grid, float32 (npos, 8)
+        with channels 0-5 real, zero outside the cells ``margin`` or more
+        inside the faces (which is all the correction uses).  The net runs on the box of those cells
+        plus its receptive field only: every output inside that region then
+        depends on inputs inside the box alone, and the convolutions compute
+        each output with the same operations whatever the grid's size, so
+        the values are those of the net on the whole grid, bit for bit."""
+        x = self.pack_fields(solver, m, every)
+        sl = self._crop(solver)
+        nz, ny, nx = solver.nz, solver.ny, solver.nx
+        shape_c = (sl[2].stop - sl[2].start, sl[1].stop - sl[1].start, sl[0].stop - sl[0].start)
+        xc = self._buf("xc", (shape_c[0] * shape_c[1] * shape_c[2], CIN), torch.bfloat16)
+        xc.view(shape_c[2], shape_c[1], shape_c[0], CIN).copy_(x.view(nz, ny, nx, CIN)[sl])
+        yc = self.forward(xc, shape_c).view(shape_c[2], shape_c[1], shape_c[0], OUT_STRIDE)
+        y0 = self.zero_response(solver)
+        y = self._buf("ycorr", (nx * ny * nz, OUT_STRIDE), torch.float32)
+        y.zero_()
+        mg = self.margin
+        inner = tuple(slice(mg, max(mg, n - mg)) for n in (nz, ny, nx))
+        rel = tuple(slice(a.start - c.start, a.stop - c.start) for a, c in zip(inner, sl))
+        ch = (slice(0, COUT),)  # the output layer writes the real channels only
+        y.view(nz, ny, nx, OUT_STRIDE)[inner + ch] = yc[rel + ch] - y0[rel + ch]
+        return y
+
     def pack_zero(self, solver) -> torch.Tensor:
         """Network input of the zero state of the solver's grid (eps_r only):
         a bf16 (npos, 8) tensor of its own."""
@@ -351,17 +383,21 @@ class ClosureNet:
         return x0
 
     def zero_response(self, solver) -> torch.Tensor:
-        """The network's output for zero fields and zero forcing on this
-        grid (it depends on eps_r only), cached on the solver until the
-        weights change.  Every correction subtracts it, so the correction
-        of a zero state is exactly zero: no constant is added to the
-        fields at every correction."""
-        key = (id(self), self.version)
+        """The network's output for zero fields and zero forcing on the box
+        ``net_correction`` evaluates it on ((z, y, x, 8) float32; it depends
+        on eps_r only), cached on the solver until the weights change.
+        Every correction subtracts it, so the correction of a zero state is
+        exactly zero: no constant is added to the fields at every
+        correction."""
+        key = (id(self), self.version, self.margin)
         cache = getattr(solver, "_closure_y0", None)
         if cache is not None and cache[0] == key:
             return cache[1]
-        x0 = self.pack_zero(solver)
-        y0 = self.forward(x0, (solver.nx, solver.ny, solver.nz), tag="0").clone()
+        sl = self._crop(solver)
+        shape_c = (sl[2].stop - sl[2].start, sl[1].stop - sl[1].start, sl[0].stop - sl[0].start)
+        x0 = self.pack_zero(solver).view(solver.nz, solver.ny, solver.nx, CIN)[sl].contiguous()
+        y0 = self.forward(x0.view(-1, CIN), shape_c, tag="0").clone()
+        y0 = y0.view(shape_c[2], shape_c[1], shape_c[0], OUT_STRIDE)
         solver._closure_y0 = (key, y0)
         return y0
 

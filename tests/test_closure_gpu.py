@@ -288,3 +288,33 @@ def test_rollout_gradient_matches_finite_differences(gpu):
     print(f"[closure] rollout loss {loss:.4e}: directional derivative {an:.6e}, central "
           f"difference {fd:.6e}", flush=True)
     assert abs(fd - an) <= 3e-3 * abs(an)
+
+
+def test_net_on_the_corrected_box_equals_the_whole_grid(gpu):
+    """The closure evaluates its net only on the cells it corrects plus the
+    net's receptive field; there the correction equals the net evaluated
+    on the whole grid bit for bit, and it is zero elsewhere."""
+    net = CL.ClosureNet.load(CL.__file__.replace("closure.py", "data/closure_c5.safetensors"),
+                             gpu)
+    prob = CL.ClosureProblem(7, 96, npml_fine=16, device=gpu)
+    sol = prob.coarse_grid().build_solver(prob.dt_coarse(), 40)
+    g_ = torch.Generator(device="cpu").manual_seed(3)
+    for c, f in enumerate(sol.fields):
+        f.copy_(torch.randn(f.shape, generator=g_) * float(net.scale[c]))
+    got = net.net_correction(sol, 0, 10).clone()
+    n = sol.nx
+    x = net.pack_fields(sol, 0, 10).clone()
+    full = net.forward(x, (n, n, n)).clone()
+    zero = net.forward(net.pack_zero(sol), (n, n, n)).clone()
+    want = (full - zero).view(n, n, n, CL.OUT_STRIDE)
+    mg = net.margin
+    inner = (slice(mg, n - mg),) * 3
+    # channels 0-5 (the output layer does not write its padding channels)
+    g4 = got.view(n, n, n, CL.OUT_STRIDE)[..., :CL.COUT]
+    want = want[..., :CL.COUT]
+    assert mg > 0 and n - 2 * mg > 4
+    assert torch.equal(g4[inner], want[inner])
+    assert float(g4[inner].abs().max()) > 0.0
+    mask = torch.ones(n, n, n, dtype=torch.bool, device=gpu)
+    mask[inner] = False
+    assert float(g4[mask].abs().max()) == 0.0
