@@ -318,3 +318,20 @@ def test_net_on_the_corrected_box_equals_the_whole_grid(gpu):
     mask = torch.ones(n, n, n, dtype=torch.bool, device=gpu)
     mask[inner] = False
     assert float(g4[mask].abs().max()) == 0.0
+
+
+def test_train_rollout_runs_and_updates_the_net(gpu):
+    """train_rollout on a 128^3 / 64^3 problem: finite losses, the output
+    layer leaves its zero start, the linear parts stay as loaded, and the
+    validation step sets a gain from its candidates."""
+    net = CL.ClosureNet.load(CL.__file__.replace("closure.py", "data/closure_c5.safetensors"),
+                             gpu)
+    far_w = net.far_w.clone()
+    hist = CL.train_rollout(net, gpu, seeds=[3, 4], n_fine=128, n_coarse=100, every=10,
+                            epochs=1, window=5, lr=6e-5, val_seeds=[905], npml_fine=16)
+    print(f"[closure] rollout losses {[round(h, 5) for h in hist]}, gain {net.net_gain}",
+          flush=True)
+    assert len(hist) == 4 and all(np.isfinite(hist))
+    assert float(net.params["w3"].abs().max()) > 0.0
+    assert torch.equal(net.far_w, far_w)
+    assert net.net_gain in (0.0, 0.25, 0.5, 1.0)
