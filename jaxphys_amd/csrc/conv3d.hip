@@ -50,6 +50,7 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned short u16;
 typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
+typedef int i32x2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ float bf2f(u16 h) { return __uint_as_float((unsigned)h << 16); }
 // float -> bf16, round to nearest even (v_cvt_pk_bf16_f32 on gfx950)
@@ -120,6 +121,33 @@ __device__ __forceinline__ bf16x8 shift_left(bf16x8 cur, bf16x8 next) {
     const int e = __builtin_amdgcn_update_dpp(0, n[d], 0x12F /* row_ror:15 */, 0xF, 0xF, false);
     o[d] = __builtin_amdgcn_update_dpp(e, c[d], 0x101 /* row_shl:1 */, 0xF, 0xF, false);
   }
+  bf16x8 r;
+  __builtin_memcpy(&r, o, 16);
+  return r;
+}
+
+// One DPP move per dword: shift_in_right gives lane r the value of lane r-1
+// of `cur` and lane 0 its own value of `edge`; shift_in_left gives lane r the
+// value of lane r+1 of `cur` and lane 15 its own value of `edge` (DPP with
+// bound_ctrl off keeps the old value where the source lane leaves the row).
+__device__ __forceinline__ bf16x8 shift_in_right(bf16x8 cur, bf16x8 edge) {
+  int c[4], e[4], o[4];
+  __builtin_memcpy(c, &cur, 16);
+  __builtin_memcpy(e, &edge, 16);
+#pragma unroll
+  for (int d = 0; d < 4; ++d)
+    o[d] = __builtin_amdgcn_update_dpp(e[d], c[d], 0x111 /* row_shr:1 */, 0xF, 0xF, false);
+  bf16x8 r;
+  __builtin_memcpy(&r, o, 16);
+  return r;
+}
+__device__ __forceinline__ bf16x8 shift_in_left(bf16x8 cur, bf16x8 edge) {
+  int c[4], e[4], o[4];
+  __builtin_memcpy(c, &cur, 16);
+  __builtin_memcpy(e, &edge, 16);
+#pragma unroll
+  for (int d = 0; d < 4; ++d)
+    o[d] = __builtin_amdgcn_update_dpp(e[d], c[d], 0x101 /* row_shl:1 */, 0xF, 0xF, false);
   bf16x8 r;
   __builtin_memcpy(&r, o, 16);
   return r;
@@ -329,6 +357,288 @@ __global__ __launch_bounds__(64 * WAVES) void conv_fwd_kernel(ConvArgs a) {
         }
       }
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Row kernel for 8 or 32 input channels (every layer of the closure net and
+// its input gradients).  Same GEMM as conv_fwd_kernel, arranged so that the
+// K loop issues little besides loads, LDS reads and MFMAs:
+//  * input rows are read through buffer resources whose size is the row's
+//    bytes (zero for a row outside the grid): the hardware returns zero for
+//    an offset past the end, so x-1 < 0, x >= nx and missing rows need no
+//    per-lane test, and every address is a wave-uniform base plus a lane
+//    offset fixed for the whole launch;
+//  * the 9 (dz, dy) pairs run two at a time, the next pair's fragments (after
+//    a plane's last pair, the next plane's first) in flight during the
+//    current pair's MFMAs and the epilogue;
+//  * (32 channels) one edge fragment per pair holds column x0 - 1 in lane 0
+//    and x0 + 64 in lane 15 of each row group, so the outer tiles' shifts
+//    take one DPP move per dword.
+// The column tiles are 16 consecutive x each: a tile's load is one 1 KiB
+// contiguous span.  (Interleaving the tiles, x = x0 + 4 r + n, would make the
+// x-1 / x+1 fragments of inner tiles plain register copies, but spreads each
+// load over 4 KiB: layer 3 of the net took 171 instead of 115 us.)
+// 32 channels: one K step per tap, K order tap * 32 + channel as in the
+// global weights.  8 channels: one K step per (dz, dy) pair, k group g of
+// the B operand holding tap dx = g - 1 (g = 3 a zero column of the LDS
+// weights), so each lane loads one fragment per tile and nothing is shifted.
+// The bf16 GELU epilogue reads gelu of the bf16 pre-activation from a table
+// in LDS for |z| in [2^-14, 8) (bf16 exponents 113..129, both signs), built
+// by each block with the same gelu() as the direct path, and applies
+// gelu(z) = z (z >= 8) and -0 (z <= -8) beyond it (there erf_as(z / sqrt 2)
+// is +-1.0f exactly); the rare rest (tiny, inf, nan) takes the direct path.
+// Results are those of conv_fwd_kernel bit for bit with 32 channels; with 8
+// channels the K steps group the taps differently (float32 accumulation
+// order).
+// ---------------------------------------------------------------------------
+constexpr int kRowE0 = 113;            // smallest bf16 exponent in the table
+constexpr int kRowLutN = 17 * 128;     // exponents 113..129, one sign
+constexpr unsigned kRowWord3 = 0x00020000u;  // raw buffer, 32-bit format
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(const void* base, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes,
+                                           (int)kRowWord3);
+}
+__device__ __forceinline__ bf16x8 row_ld(__amdgpu_buffer_rsrc_t rs, unsigned off) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)off, 0, 0);
+  return __builtin_bit_cast(bf16x8, v);
+}
+// gelu of K bf16 pre-activations: the K table reads are issued together;
+// values outside the table's range take the rules above or, rarely, the
+// direct path (one branch for all K)
+template <int K>
+__device__ __forceinline__ void gelu_lut(const u16* lut, const u16* zb, u16* y) {
+  unsigned idx[K];
+  bool any_slow = false;
+#pragma unroll
+  for (int v = 0; v < K; ++v) {
+    const unsigned mag = zb[v] & 0x7FFFu;
+    const unsigned i = mag - kRowE0 * 128u;
+    idx[v] = (i < (unsigned)kRowLutN ? i : 0u) + (zb[v] >> 15) * kRowLutN;
+    any_slow |= mag < kRowE0 * 128u || mag >= 255u * 128u;
+  }
+  u16 tv[K];
+#pragma unroll
+  for (int v = 0; v < K; ++v) tv[v] = lut[idx[v]];
+#pragma unroll
+  for (int v = 0; v < K; ++v) {
+    const unsigned i = (zb[v] & 0x7FFFu) - kRowE0 * 128u;
+    y[v] = i < (unsigned)kRowLutN ? tv[v] : ((zb[v] >> 15) ? (u16)0x8000 : zb[v]);  // |z| >= 8
+  }
+  if (any_slow) {
+#pragma unroll
+    for (int v = 0; v < K; ++v) {
+      const unsigned mag = zb[v] & 0x7FFFu;
+      if (mag < kRowE0 * 128u || mag >= 255u * 128u) y[v] = f2bf(gelu(bf2f(zb[v])));
+    }
+  }
+}
+
+template <int EPI, int MT, int CIN>
+__global__ __launch_bounds__(1024) void conv_row_kernel(ConvArgs a) {
+  constexpr int NT = 4;
+  constexpr int KP = CIN == 32 ? 27 * 32 : 9 * 32;  // K per output channel in LDS
+  constexpr int LDW = KP + 8;
+  extern __shared__ u16 sh[];
+  u16* wsh = sh;                          // [MT * 16][LDW]
+  u16* lut = sh + MT * 16 * LDW;          // [2][kRowLutN] (GELU only)
+  {
+    constexpr int C16 = KP / 8;           // 16-byte chunks per output channel
+    for (int q = threadIdx.x; q < MT * 16 * C16; q += 1024) {
+      const int row = q / C16, c = q - row * C16;
+      uint4 v = {0u, 0u, 0u, 0u};
+      if (CIN == 32) {
+        v = *reinterpret_cast<const uint4*>(a.w + (long long)row * a.kpad + c * 8);
+      } else {
+        const int p = c >> 2, gg = c & 3;
+        if (gg < 3) v = *reinterpret_cast<const uint4*>(a.w + (long long)row * a.kpad + (3 * p + gg) * 8);
+      }
+      *reinterpret_cast<uint4*>(wsh + row * LDW + c * 8) = v;
+    }
+    if (EPI == CONV_EPI_GELU) {
+      for (int q = threadIdx.x; q < 2 * kRowLutN; q += 1024) {
+        const unsigned neg = q >= kRowLutN;
+        const unsigned zb = (neg << 15) | (kRowE0 * 128u + (q - neg * kRowLutN));
+        lut[q] = f2bf(gelu(bf2f((u16)zb)));
+      }
+    }
+  }
+  __syncthreads();
+  const int t = threadIdx.x & 63;
+  const int g = t >> 4, r = t & 15;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const int yy = blockIdx.y * 16 + wv;
+  if (yy >= ny) return;
+  const int x0 = blockIdx.x * 16 * NT;
+  const int z0 = blockIdx.z * a.zchunk;
+  const int z1 = min(z0 + a.zchunk, nz);
+  const unsigned rowbytes = (unsigned)nx * CIN * 2u;
+  // lane offsets within an input row (bytes; negative wraps past the end)
+  // (32 channels) the edge fragment: column x0 - 1 in lane 0 of each row
+  // group (the x-1 tap of tile 0), x0 + 64 in lane 15 (x+1 of tile 3)
+  unsigned off[NT], offe = 0;
+  if (CIN == 32) {
+#pragma unroll
+    for (int n = 0; n < NT; ++n) off[n] = (unsigned)((x0 + 16 * n + r) * 64 + 16 * g);
+    offe = (unsigned)((r == 15 ? x0 + 16 * NT : x0 - 1) * 64 + 16 * g);
+  } else {
+    const int dx = g < 3 ? g - 1 : 0;     // g = 3 meets zero weights
+#pragma unroll
+    for (int n = 0; n < NT; ++n) off[n] = (unsigned)((x0 + 16 * n + r + dx) * 16);
+  }
+  const int ycp = a.ycp;
+  float bv[MT][4];
+#pragma unroll
+  for (int m = 0; m < MT; ++m)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int co = m * 16 + 4 * g + v;
+      bv[m][v] = a.bias && co < a.cout ? a.bias[co] : 0.f;
+    }
+  const unsigned esz = EPI == CONV_EPI_LINEAR_F32 ? 4u : 2u;
+  const unsigned obytes = (unsigned)nx * ycp * esz;
+  auto rsrc_of = [&](int zz, int p) {
+    const int sz = zz + p / 3 - 1, sy = yy + p % 3 - 1;
+    const bool pl = sz >= 0 && sz < nz && sy >= 0 && sy < ny;
+    const u16* base = a.x + ((long long)(pl ? sz : 0) * ny + (pl ? sy : 0)) * nx * CIN;
+    return row_rsrc(base, pl ? rowbytes : 0u);
+  };
+  // fragments of one (dz, dy) pair: the NT x+0 tiles and the edge fragment
+  struct Pair {
+    bf16x8 c[NT], e;
+  };
+  auto load_pair = [&](int zz, int p, Pair& f) {
+    const auto rs = rsrc_of(zz, p);
+#pragma unroll
+    for (int n = 0; n < NT; ++n) f.c[n] = row_ld(rs, off[n]);
+    if (CIN == 32) f.e = row_ld(rs, offe);
+  };
+  // two fragment sets: the next pair's loads (after the last pair of a
+  // plane, the next plane's first) are in flight during the current pair's
+  // MFMAs and the epilogue
+  Pair fa, fb;
+  load_pair(z0, 0, fa);
+  for (int zz = z0; zz < z1; ++zz) {
+    f32x4 acc[MT][NT];
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+      for (int n = 0; n < NT; ++n) acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+    auto mma_pair = [&](int p, const Pair& f) {
+      if (CIN == 32) {
+#pragma unroll
+        for (int dx = 0; dx < 3; ++dx) {
+          bf16x8 b[NT];
+#pragma unroll
+          for (int n = 0; n < NT; ++n) {
+            if (dx == 0) b[n] = n == 0 ? shift_in_right(f.c[0], f.e) : shift_right(f.c[n], f.c[n - 1]);
+            else if (dx == 1) b[n] = f.c[n];
+            else b[n] = n == NT - 1 ? shift_in_left(f.c[NT - 1], f.e) : shift_left(f.c[n], f.c[n + 1]);
+          }
+          bf16x8 af[MT];
+#pragma unroll
+          for (int m = 0; m < MT; ++m) af[m] = load8(wsh + (m * 16 + r) * LDW + (3 * p + dx) * 32 + 8 * g);
+#pragma unroll
+          for (int n = 0; n < NT; ++n)
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+              acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[m], b[n], acc[m][n], 0, 0, 0);
+        }
+      } else {
+        bf16x8 af[MT];
+#pragma unroll
+        for (int m = 0; m < MT; ++m) af[m] = load8(wsh + (m * 16 + r) * LDW + p * 32 + 8 * g);
+#pragma unroll
+        for (int n = 0; n < NT; ++n)
+#pragma unroll
+          for (int m = 0; m < MT; ++m)
+            acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[m], f.c[n], acc[m][n], 0, 0, 0);
+      }
+    };
+    // fa holds pair p on entry to each iteration
+#pragma unroll 1
+    for (int p = 0; p < 9; p += 2) {
+      if (p + 1 < 9) load_pair(zz, p + 1, fb);
+      else if (zz + 1 < z1) load_pair(zz + 1, 0, fb);
+      mma_pair(p, fa);
+      if (p + 1 < 9) {
+        if (p + 2 < 9) load_pair(zz, p + 2, fa);
+        mma_pair(p + 1, fb);
+      }
+    }
+    // epilogue: lane holds D[co = m*16 + 4g + v][x = x0 + 16n + r]; stores past
+    // the row's end (x >= nx) fall outside the buffer and are dropped
+    const long long orow = ((long long)zz * ny + yy) * nx * ycp;
+    char* ybase = reinterpret_cast<char*>(a.y) + orow * esz;
+    const auto ors = row_rsrc(ybase, obytes);
+#pragma unroll
+    for (int n = 0; n < NT; ++n) {
+      const int px = x0 + 16 * n + r;
+#pragma unroll
+      for (int m = 0; m < MT; ++m) {
+        const int cb = m * 16 + 4 * g;
+        if (EPI == CONV_EPI_LINEAR_F32) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int co = cb + v;
+            if (co < a.cout && co < ycp)
+              __builtin_amdgcn_raw_buffer_store_b32(
+                  __builtin_bit_cast(int, acc[m][n][v] + bv[m][v]), ors,
+                  (int)((unsigned)(px * ycp + co) * 4u), 0, 0);
+          }
+          continue;
+        }
+      }
+      if (EPI == CONV_EPI_LINEAR_F32) continue;
+      // bf16 outputs: the lane's 4 consecutive channels per M tile in one
+      // 8-byte store; channels past cout are written as zero
+      u16 zv[MT * 4] = {}, yv[MT * 4] = {};
+#pragma unroll
+      for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const float val = acc[m][n][v];
+          if (EPI == CONV_EPI_GELU) zv[m * 4 + v] = f2bf(val + bv[m][v]);
+          else if (EPI == CONV_EPI_LINEAR_BF16) yv[m * 4 + v] = f2bf(val + bv[m][v]);
+        }
+      if (EPI == CONV_EPI_GELU) gelu_lut<MT * 4>(lut, zv, yv);
+      if (EPI == CONV_EPI_GELU_BWD) {
+#pragma unroll
+        for (int m = 0; m < MT; ++m) {
+          const int cb = m * 16 + 4 * g;
+          const unsigned o = (unsigned)(px * ycp + cb) * 2u;
+          const auto av = __builtin_amdgcn_raw_buffer_load_b64(row_rsrc(a.aux + orow, obytes),
+                                                               (int)o, 0, 0);
+          const u16x4 aux4 = __builtin_bit_cast(u16x4, av);
+#pragma unroll
+          for (int v = 0; v < 4; ++v)
+            yv[m * 4 + v] = f2bf(acc[m][n][v] * gelu_grad(bf2f(aux4[v])));
+        }
+      }
+#pragma unroll
+      for (int m = 0; m < MT; ++m) {
+        const int cb = m * 16 + 4 * g;
+        if (cb >= ycp) continue;
+        const unsigned o = (unsigned)(px * ycp + cb) * 2u;
+        u16 y4[4], z4[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const bool real = cb + v < a.cout;
+          y4[v] = real ? yv[m * 4 + v] : (u16)0;
+          z4[v] = real ? zv[m * 4 + v] : (u16)0;
+        }
+        const u16x4 yy4 = {y4[0], y4[1], y4[2], y4[3]};
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(i32x2, yy4), ors, (int)o, 0, 0);
+        if (EPI == CONV_EPI_GELU && a.zout) {
+          const u16x4 zz4 = {z4[0], z4[1], z4[2], z4[3]};
+          __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(i32x2, zz4),
+                                                row_rsrc(a.zout + orow, obytes), (int)o, 0, 0);
+        }
+      }
+    }
+    fa = fb;
   }
 }
 
@@ -696,6 +1006,22 @@ bool shift_enabled() {
   return on;
 }
 
+// JAXPHYS_AMD_CONV_ROWS=0 turns the row kernel off (the general kernel then
+// runs every layer; for comparison).
+bool rows_enabled() {
+  static const bool on = [] {
+    const char* e = std::getenv("JAXPHYS_AMD_CONV_ROWS");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+
+size_t row_shm(int cin, int mt, int epi) {
+  const size_t kp = cin == 32 ? 27 * 32 : 9 * 32;
+  return (size_t)mt * 16 * (kp + 8) * sizeof(u16) +
+         (epi == CONV_EPI_GELU ? 2 * (size_t)kRowLutN * sizeof(u16) : 0);
+}
+
 }  // namespace conv
 
 using namespace conv;
@@ -748,9 +1074,19 @@ int conv3d_fwd(const ConvDesc* d, const void* x, const void* w, const float* bia
   if (shm > 160 * 1024) return fail(-1, "weights exceed LDS");
   hipStream_t st = (hipStream_t)stream;
   const bool shift = d->cin_pad % 32 == 0 && shift_enabled();
+  auto al8 = [](const void* p) { return ((uintptr_t)p & 7) == 0; };
+  const bool rows = rows_enabled() && (d->cin_pad == 32 || d->cin_pad == 8) &&
+                    (epi == CONV_EPI_LINEAR_F32 ||
+                     (d->y_cpad % 4 == 0 && al8(y) && al8(z_out) && al8(aux)));
 #define CONV_LAUNCH(E, MT)                                                                   \
   do {                                                                                       \
-    if (shift)                                                                               \
+    if (rows && d->cin_pad == 32)                                                            \
+      hipLaunchKernelGGL((conv_row_kernel<E, MT, 32>), grid, dim3(1024),                     \
+                         row_shm(32, MT, E), st, a);                                         \
+    else if (rows)                                                                           \
+      hipLaunchKernelGGL((conv_row_kernel<E, MT, 8>), grid, dim3(1024),                      \
+                         row_shm(8, MT, E), st, a);                                          \
+    else if (shift)                                                                          \
       hipLaunchKernelGGL((conv_fwd_kernel<E, NT, MT, WAVES, true>), grid, dim3(64 * WAVES),  \
                          shm, st, a);                                                        \
     else if (d->cin_pad == 8)                                                                \

@@ -222,3 +222,45 @@ def test_far_field_stencil_matches_oracle(gpu):
     got = y.cpu().numpy()
     np.testing.assert_allclose(got[:, :6] - y0[:, :6], ref, rtol=0, atol=2e-5 * np.abs(ref).max())
     np.testing.assert_array_equal(got[:, 6:], y0[:, 6:])
+
+
+def test_gelu_table_matches_direct_gelu_on_every_bf16_value(gpu):
+    """The row kernel (8 input channels) reads the GELU of a bf16
+    pre-activation from its LDS table, or applies z / -0 beyond |z| = 8; the
+    general kernel (16 input channels) evaluates erf directly.  With one unit
+    weight (channel 0, centre tap) the pre-activation is the input value
+    exactly, so both kernels must give the same outputs for every finite
+    bf16 value, and both must agree with the float64 GELU to bf16 rounding."""
+    vals = torch.arange(65536, dtype=torch.int32).to(torch.int16).view(torch.bfloat16)
+    vals = vals[torch.isfinite(vals.float())]
+    nx, ny = 64, 32
+    nz = (vals.numel() + nx * ny - 1) // (nx * ny)
+    n = nx * ny * nz
+    flat = torch.zeros(n, dtype=torch.bfloat16)
+    flat[: vals.numel()] = vals
+    outs = {}
+    for cin_pad in (8, 16):
+        x = torch.zeros(nz, ny, nx, cin_pad, dtype=torch.bfloat16)
+        x[..., 0] = flat.view(nz, ny, nx)
+        kpad = _conv.kpad_of(cin_pad)
+        wp = torch.zeros(32, kpad, dtype=torch.bfloat16)
+        wp[:, 13 * cin_pad] = 1.0  # centre tap (13 of 27), channel 0
+        d = _conv.desc((nx, ny, nz), cin_pad, 32, 32, 32)
+        y = torch.zeros(n, 32, dtype=torch.bfloat16, device=gpu)
+        z = torch.zeros(n, 32, dtype=torch.bfloat16, device=gpu)
+        _conv.fwd(d, x.to(gpu).contiguous(), wp.to(gpu), torch.zeros(32, device=gpu),
+                  _conv.EPI_GELU, y, z_out=z)
+        outs[cin_pad] = (y.cpu(), z.cpu())
+    (y8, z8), (y16, z16) = outs[8], outs[16]
+    assert torch.equal(z8.float(), z16.float())
+    assert torch.equal(z8[:, 0].float(), flat.float())
+    assert torch.equal(y8.float(), y16.float())
+    # against the float64 GELU: the erf approximation (|error| <= 1.5e-7,
+    # so at most 8 * 1.5e-7 / 2 in GELU for |z| < 8) may move a value across
+    # a bf16 rounding boundary
+    zr = flat.double()
+    ref = _gelu(zr).to(torch.bfloat16).double()
+    got = y8[:, 0].double()
+    step = ref.abs() * 2.0 ** -7
+    assert bool(((got - ref).abs() <= step + 1e-6).all())
+    assert float((got != ref).double().mean()) < 1e-2
