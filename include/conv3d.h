@@ -31,7 +31,8 @@ extern "C" {
 
 /* epilogue of conv3d_fwd */
 #define CONV_EPI_GELU 0      /* z = acc + bias (bf16 to z_out if given); y = gelu(z) bf16 */
-#define CONV_EPI_LINEAR_F32 1 /* y = acc + bias as float32 [..][cout_real] */
+#define CONV_EPI_LINEAR_F32 1 /* y = acc + bias as float32, channel stride y_cpad (channels
+                                 cout..y_cpad-1 written as zero when y_cpad is a multiple of 4) */
 #define CONV_EPI_GELU_BWD 2  /* y = acc * gelu'(aux) bf16 (aux: bf16 pre-activation) */
 #define CONV_EPI_LINEAR_BF16 3 /* y = acc + bias as bf16 */
 

@@ -404,34 +404,68 @@ __device__ __forceinline__ bf16x8 row_ld(__amdgpu_buffer_rsrc_t rs, unsigned off
   const auto v = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)off, 0, 0);
   return __builtin_bit_cast(bf16x8, v);
 }
-// gelu of K bf16 pre-activations: the K table reads are issued together;
-// values outside the table's range take the rules above or, rarely, the
-// direct path (one branch for all K)
+// gelu of K (even) bf16 pre-activations, two per register in packed 16-bit
+// arithmetic: t = (|z| bits) - 113 * 128 (mod 2^16) indexes the table when
+// below kRowLutN (one unused entry per sign follows each half, so the index
+// min(t, kRowLutN) needs no select); 2176 <= t < 18176 is 8 <= |z| < inf,
+// whose gelu is z or -0 (min(z bits, 0x8000)); t >= 18176 (tiny, inf, nan)
+// takes the direct path, one branch for all K values
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+constexpr int kRowLutS = kRowLutN + 1;  // table entries per sign
 template <int K>
 __device__ __forceinline__ void gelu_lut(const u16* lut, const u16* zb, u16* y) {
+  u16x2 tt[K / 2], big[K / 2];
   unsigned idx[K];
-  bool any_slow = false;
+  u16x2 tmax = {0, 0};
+#pragma unroll
+  for (int v = 0; v < K; v += 2) {
+    const u16x2 z = {zb[v], zb[v + 1]};
+    const u16x2 tv = (z & (u16)0x7FFF) - (u16)(kRowE0 * 128);
+    const u16x2 id = __builtin_elementwise_min(tv, (u16x2)(u16)kRowLutN) + (z >> 15) * (u16)kRowLutS;
+    tt[v / 2] = tv;
+    big[v / 2] = __builtin_elementwise_min(z, (u16x2)(u16)0x8000);
+    tmax = __builtin_elementwise_max(tmax, tv);
+    idx[v] = id.x;
+    idx[v + 1] = id.y;
+  }
+  u16 lv[K];
+#pragma unroll
+  for (int v = 0; v < K; ++v) lv[v] = lut[idx[v]];
 #pragma unroll
   for (int v = 0; v < K; ++v) {
-    const unsigned mag = zb[v] & 0x7FFFu;
-    const unsigned i = mag - kRowE0 * 128u;
-    idx[v] = (i < (unsigned)kRowLutN ? i : 0u) + (zb[v] >> 15) * kRowLutN;
-    any_slow |= mag < kRowE0 * 128u || mag >= 255u * 128u;
+    const u16 t = (v & 1) ? tt[v / 2].y : tt[v / 2].x;
+    y[v] = t < (u16)kRowLutN ? lv[v] : ((v & 1) ? big[v / 2].y : big[v / 2].x);
   }
-  u16 tv[K];
-#pragma unroll
-  for (int v = 0; v < K; ++v) tv[v] = lut[idx[v]];
-#pragma unroll
-  for (int v = 0; v < K; ++v) {
-    const unsigned i = (zb[v] & 0x7FFFu) - kRowE0 * 128u;
-    y[v] = i < (unsigned)kRowLutN ? tv[v] : ((zb[v] >> 15) ? (u16)0x8000 : zb[v]);  // |z| >= 8
-  }
-  if (any_slow) {
+  if (__builtin_elementwise_max(tmax.x, tmax.y) >= (u16)((255 - kRowE0) * 128)) {
 #pragma unroll
     for (int v = 0; v < K; ++v) {
-      const unsigned mag = zb[v] & 0x7FFFu;
-      if (mag < kRowE0 * 128u || mag >= 255u * 128u) y[v] = f2bf(gelu(bf2f(zb[v])));
+      const u16 t = (v & 1) ? tt[v / 2].y : tt[v / 2].x;
+      if (t >= (u16)((255 - kRowE0) * 128)) y[v] = f2bf(gelu(bf2f(zb[v])));
     }
+  }
+}
+
+// float32 output of one lane's 4 channels cb..cb+3 at position px: one
+// 16-byte store when the channel stride is a multiple of 4 (channels past
+// cout written as zero), else one 4-byte store per real channel
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void f32_out(const f32x4& acc, const float* bv, int cb, int cout, int ycp,
+                                        __amdgpu_buffer_rsrc_t ors, int px) {
+  if ((ycp & 3) == 0) {
+    if (cb >= ycp) return;
+    f32x4 o;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) o[v] = cb + v < cout ? acc[v] + bv[v] : 0.f;
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(i32x4, o), ors,
+                                           (int)((unsigned)(px * ycp + cb) * 4u), 0, 0);
+    return;
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int co = cb + v;
+    if (co < cout && co < ycp)
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, acc[v] + bv[v]), ors,
+                                            (int)((unsigned)(px * ycp + co) * 4u), 0, 0);
   }
 }
 
@@ -442,7 +476,7 @@ __global__ __launch_bounds__(1024) void conv_row_kernel(ConvArgs a) {
   constexpr int LDW = KP + 8;
   extern __shared__ u16 sh[];
   u16* wsh = sh;                          // [MT * 16][LDW]
-  u16* lut = sh + MT * 16 * LDW;          // [2][kRowLutN] (GELU only)
+  u16* lut = sh + MT * 16 * LDW;          // [2][kRowLutS] (GELU only)
   {
     constexpr int C16 = KP / 8;           // 16-byte chunks per output channel
     for (int q = threadIdx.x; q < MT * 16 * C16; q += 1024) {
@@ -457,10 +491,10 @@ __global__ __launch_bounds__(1024) void conv_row_kernel(ConvArgs a) {
       *reinterpret_cast<uint4*>(wsh + row * LDW + c * 8) = v;
     }
     if (EPI == CONV_EPI_GELU) {
-      for (int q = threadIdx.x; q < 2 * kRowLutN; q += 1024) {
-        const unsigned neg = q >= kRowLutN;
-        const unsigned zb = (neg << 15) | (kRowE0 * 128u + (q - neg * kRowLutN));
-        lut[q] = f2bf(gelu(bf2f((u16)zb)));
+      for (int q = threadIdx.x; q < 2 * kRowLutS; q += 1024) {
+        const unsigned neg = q >= kRowLutS, j = q - neg * kRowLutS;
+        const unsigned zb = (neg << 15) | (kRowE0 * 128u + j);
+        lut[q] = j < (unsigned)kRowLutN ? f2bf(gelu(bf2f((u16)zb))) : (u16)0;
       }
     }
   }
@@ -580,14 +614,7 @@ __global__ __launch_bounds__(1024) void conv_row_kernel(ConvArgs a) {
       for (int m = 0; m < MT; ++m) {
         const int cb = m * 16 + 4 * g;
         if (EPI == CONV_EPI_LINEAR_F32) {
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const int co = cb + v;
-            if (co < a.cout && co < ycp)
-              __builtin_amdgcn_raw_buffer_store_b32(
-                  __builtin_bit_cast(int, acc[m][n][v] + bv[m][v]), ors,
-                  (int)((unsigned)(px * ycp + co) * 4u), 0, 0);
-          }
+          f32_out(acc[m][n], bv[m], cb, a.cout, ycp, ors, px);
           continue;
         }
       }
@@ -640,6 +667,240 @@ __global__ __launch_bounds__(1024) void conv_row_kernel(ConvArgs a) {
     }
     fa = fb;
   }
+}
+
+// ---------------------------------------------------------------------------
+// Slab kernel for 32 input channels: the input planes a block needs are
+// staged in LDS by direct-to-LDS buffer loads and every wave reads its
+// fragments there, so each input row is fetched from the caches about 2.6
+// times per output plane instead of 9.
+//  * a block (16 wavefronts) computes 64 x by 8 rows by 2 planes per step;
+//    wave w owns row y0 + (w & 7) of plane zz + (w >> 3); lane column r of
+//    column tile n is x = x0 + 4 r + n, so the x-1 / x+1 fragments of tile n
+//    are the x+0 fragments of tiles n-1 / n+1 and only tiles 0 and 3 need a
+//    DPP shift (with an edge fragment: x0 - 1 in lane 0, x0 + 64 in lane 15);
+//  * a slab is one input plane's rows y0-1 .. y0+8 and columns x0-1 .. x0+64;
+//    within a row, position p's channel chunk g (16 bytes) sits at slot
+//    17 (p >> 2) + 4 (p & 3) + g (one unused slot per 4 positions), which
+//    puts the 16 lanes that read positions 4 r + c on distinct LDS banks;
+//  * each lane of a direct-to-LDS load writes 16 bytes at consecutive LDS
+//    addresses, so every slot's global source is chosen per lane; positions
+//    and rows outside the grid (and the unused slots) get an offset past the
+//    buffer's end and arrive as zeros, a plane outside the grid a buffer of
+//    size zero;
+//  * the four slabs iz = zz-1 .. zz+2 of a step stream through two LDS
+//    buffers, the next slab's loads in flight during the current slab's
+//    MFMAs; plane zz's waves have nothing to add from slab zz+2 and plane
+//    zz+1's nothing from slab zz-1: each wave writes its finished row in
+//    that idle slab step (plane zz+1's at the next step's first slab).
+// Taps accumulate in the order (dz, dy, dx) of conv_fwd_kernel, so the
+// results are the same bits.
+// ---------------------------------------------------------------------------
+constexpr int kSlabRows = 10, kSlabRowSlots = 280;          // 66 positions + 16 unused slots
+constexpr int kSlabSlots = kSlabRows * kSlabRowSlots;        // 16-byte slots
+constexpr int kSlabBytes = kSlabSlots * 16;                  // 44800
+constexpr int kSlabLoads = (kSlabSlots + 63) / 64;           // 44 wave loads per slab
+
+__device__ __forceinline__ int slab_slot(int pos, int g) { return 17 * (pos >> 2) + 4 * (pos & 3) + g; }
+
+template <int EPI, int MT>
+__global__ __launch_bounds__(1024) void conv_slab_kernel(ConvArgs a) {
+  constexpr int NT = 4, CIN = 32, LDW = 27 * 32 + 8;
+  // separate LDS objects (one per weight M tile, one per slab buffer), so
+  // that the compiler can tell the slab being read and the weights from the
+  // slab being loaded and does not wait for those loads
+  __shared__ __attribute__((aligned(16))) u16 wsh0[16 * LDW];
+  __shared__ __attribute__((aligned(16))) u16 wsh1[MT == 2 ? 16 * LDW : 8];
+  __shared__ u16 lut[EPI == CONV_EPI_GELU ? 2 * kRowLutS : 2];
+  __shared__ __attribute__((aligned(16))) unsigned char slab0[kSlabBytes];
+  __shared__ __attribute__((aligned(16))) unsigned char slab1[kSlabBytes];
+  {
+    constexpr int C16 = 27 * 32 / 8;
+    for (int q = threadIdx.x; q < MT * 16 * C16; q += 1024) {
+      const int row = q / C16, c = q - row * C16;
+      *reinterpret_cast<uint4*>((row < 16 ? wsh0 : wsh1) + (row & 15) * LDW + c * 8) =
+          *reinterpret_cast<const uint4*>(a.w + (long long)row * a.kpad + c * 8);
+    }
+    if (EPI == CONV_EPI_GELU) {
+      for (int q = threadIdx.x; q < 2 * kRowLutS; q += 1024) {
+        const unsigned neg = q >= kRowLutS, j = q - neg * kRowLutS;
+        const unsigned zb = (neg << 15) | (kRowE0 * 128u + j);
+        lut[q] = j < (unsigned)kRowLutN ? f2bf(gelu(bf2f((u16)zb))) : (u16)0;
+      }
+    }
+  }
+  const int t = threadIdx.x & 63;
+  const int g = t >> 4, r = t & 15;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const int x0 = blockIdx.x * 64, y0 = blockIdx.y * 8;
+  const int z0 = blockIdx.z * a.zchunk, z1 = min(z0 + a.zchunk, nz);
+  const int jw = wv >> 3, iw = wv & 7;
+  const int yy = y0 + iw;
+  const bool rowok = yy < ny;
+  // this lane's slots of the block's slab loads (instruction k = wv + 16 u)
+  unsigned soff[3];
+#pragma unroll
+  for (int u = 0; u < 3; ++u) {
+    const int q = (wv + 16 * u) * 64 + t;
+    const int row = q / kSlabRowSlots, rs = q - row * kSlabRowSlots;
+    const int k4 = rs / 17, j = rs - 17 * k4;
+    const int pos = 4 * k4 + (j >> 2), gg = j & 3;
+    const int sy = y0 - 1 + row, sx = x0 - 1 + pos;
+    const bool ok = q < kSlabSlots && j < 16 && pos < 66 && sy >= 0 && sy < ny && sx >= 0 && sx < nx;
+    soff[u] = ok ? (unsigned)(((long long)sy * nx + sx) * 64 + 16 * gg) : 0x80000000u;
+  }
+  const unsigned planebytes = (unsigned)((long long)nx * ny * CIN * 2);
+  auto stage = [&](int iz, unsigned char* dst) __attribute__((always_inline)) {
+    const bool pl = iz >= 0 && iz < nz;
+    const auto rs = row_rsrc(a.x + (long long)(pl ? iz : 0) * ny * nx * CIN, pl ? planebytes : 0u);
+#pragma unroll
+    for (int u = 0; u < 3; ++u) {
+      const int k = wv + 16 * u;
+      if (k < kSlabLoads && (k * 64 + t) < kSlabSlots)
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(
+            rs, (__attribute__((address_space(3))) void*)(dst + k * 1024), 16, soff[u], 0, 0, 0);
+    }
+  };
+  float bv[MT][4];
+#pragma unroll
+  for (int m = 0; m < MT; ++m)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int co = m * 16 + 4 * g + v;
+      bv[m][v] = a.bias && co < a.cout ? a.bias[co] : 0.f;
+    }
+  const int ycp = a.ycp;
+  const unsigned esz = EPI == CONV_EPI_LINEAR_F32 ? 4u : 2u;
+  const unsigned obytes = (unsigned)nx * ycp * esz;
+  // LDS byte offsets of this lane's fragments within a slab row
+  unsigned loff[NT], eoff;
+#pragma unroll
+  for (int n = 0; n < NT; ++n) loff[n] = slab_slot(4 * r + n + 1, g) * 16;
+  eoff = slab_slot(r == 15 ? 65 : 0, g) * 16;
+  const int nsteps = (z1 - z0 + 1) / 2;
+  const int nslab = 4 * nsteps;
+  f32x4 acc[MT][NT];
+#pragma unroll
+  for (int m = 0; m < MT; ++m)
+#pragma unroll
+    for (int n = 0; n < NT; ++n) acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+  // write this wave's finished row of plane zw, then clear the sums
+  auto epilogue = [&](int zw) __attribute__((always_inline)) {
+    if (rowok && zw < z1) {
+      const long long orow = ((long long)zw * ny + yy) * nx * ycp;
+      char* ybase = reinterpret_cast<char*>(a.y) + orow * esz;
+      const auto ors = row_rsrc(ybase, obytes);
+#pragma unroll
+      for (int n = 0; n < NT; ++n) {
+        const int px = x0 + 4 * r + n;
+        if (EPI == CONV_EPI_LINEAR_F32) {
+#pragma unroll
+          for (int m = 0; m < MT; ++m) f32_out(acc[m][n], bv[m], m * 16 + 4 * g, a.cout, ycp, ors, px);
+          continue;
+        }
+        u16 zv[MT * 4] = {}, yv[MT * 4] = {};
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const float val = acc[m][n][v];
+            if (EPI == CONV_EPI_GELU) zv[m * 4 + v] = f2bf(val + bv[m][v]);
+            else if (EPI == CONV_EPI_LINEAR_BF16) yv[m * 4 + v] = f2bf(val + bv[m][v]);
+          }
+        if (EPI == CONV_EPI_GELU) gelu_lut<MT * 4>(lut, zv, yv);
+        if (EPI == CONV_EPI_GELU_BWD) {
+#pragma unroll
+          for (int m = 0; m < MT; ++m) {
+            const unsigned o = (unsigned)(px * ycp + m * 16 + 4 * g) * 2u;
+            const auto av = __builtin_amdgcn_raw_buffer_load_b64(row_rsrc(a.aux + orow, obytes),
+                                                                 (int)o, 0, 0);
+            const u16x4 aux4 = __builtin_bit_cast(u16x4, av);
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+              yv[m * 4 + v] = f2bf(acc[m][n][v] * gelu_grad(bf2f(aux4[v])));
+          }
+        }
+#pragma unroll
+        for (int m = 0; m < MT; ++m) {
+          const int cb = m * 16 + 4 * g;
+          if (cb >= ycp) continue;
+          const unsigned o = (unsigned)(px * ycp + cb) * 2u;
+          u16 y4[4], z4[4];
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const bool real = cb + v < a.cout;
+            y4[v] = real ? yv[m * 4 + v] : (u16)0;
+            z4[v] = real ? zv[m * 4 + v] : (u16)0;
+          }
+          const u16x4 yy4 = {y4[0], y4[1], y4[2], y4[3]};
+          __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(i32x2, yy4), ors, (int)o, 0, 0);
+          if (EPI == CONV_EPI_GELU && a.zout) {
+            const u16x4 zz4 = {z4[0], z4[1], z4[2], z4[3]};
+            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(i32x2, zz4),
+                                                  row_rsrc(a.zout + orow, obytes), (int)o, 0, 0);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+      for (int n = 0; n < NT; ++n) acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+  };
+  // slab ts is read from sb while slab ts + 1 loads into nxt
+  auto slab_step = [&](int ts, const unsigned char* sb, unsigned char* nxt)
+                       __attribute__((always_inline)) {
+    const int zz = z0 + 2 * (ts >> 2);
+    const int q = ts & 3;
+    const int iz = zz - 1 + q;
+    if (ts + 1 < nslab) stage(z0 + 2 * ((ts + 1) >> 2) - 1 + ((ts + 1) & 3), nxt);
+    if (jw == 0 && q == 3) epilogue(zz);
+    if (jw == 1 && q == 0 && ts > 0) epilogue(zz - 1);
+    const int zw = zz + jw;
+    const int dz = iz - zw;
+    if (rowok && zw < z1 && dz >= -1 && dz <= 1) {
+#pragma unroll
+      for (int dy = 0; dy < 3; ++dy) {
+        const unsigned char* rowp = sb + (iw + dy) * (kSlabRowSlots * 16);
+        bf16x8 c[NT], e;
+#pragma unroll
+        for (int n = 0; n < NT; ++n) c[n] = *reinterpret_cast<const bf16x8*>(rowp + loff[n]);
+        e = *reinterpret_cast<const bf16x8*>(rowp + eoff);
+        const int tap0 = ((dz + 1) * 3 + dy) * 3;
+        const bf16x8 lft = shift_in_right(c[NT - 1], e), rgt = shift_in_left(c[0], e);
+#pragma unroll
+        for (int dx = 0; dx < 3; ++dx) {
+          bf16x8 b[NT];
+#pragma unroll
+          for (int n = 0; n < NT; ++n) {
+            if (dx == 0) b[n] = n == 0 ? lft : c[n - 1];
+            else if (dx == 1) b[n] = c[n];
+            else b[n] = n == NT - 1 ? rgt : c[n + 1];
+          }
+          bf16x8 af[MT];
+#pragma unroll
+          for (int m = 0; m < MT; ++m)
+            af[m] = load8((m == 0 ? wsh0 : wsh1) + r * LDW + (tap0 + dx) * 32 + 8 * g);
+#pragma unroll
+          for (int n = 0; n < NT; ++n)
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+              acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[m], b[n], acc[m][n], 0, 0, 0);
+        }
+      }
+    }
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's slab loads
+    __syncthreads();
+  };
+  stage(z0 - 1, slab0);
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+  __syncthreads();
+  for (int ts = 0; ts < nslab; ts += 2) {  // nslab is even
+    slab_step(ts, slab0, slab1);
+    slab_step(ts + 1, slab1, slab0);
+  }
+  if (jw == 1) epilogue(z0 + 2 * (nsteps - 1) + 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -1016,10 +1277,26 @@ bool rows_enabled() {
   return on;
 }
 
+// JAXPHYS_AMD_CONV_SLAB=0 turns the slab kernel off (32-channel layers then
+// take the row kernel).
+bool slab_enabled() {
+  static const bool on = [] {
+    const char* e = std::getenv("JAXPHYS_AMD_CONV_SLAB");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+
+template <int EPI, int MT>
+int slab_launch(const ConvArgs& a, dim3 grid, hipStream_t st) {
+  hipLaunchKernelGGL((conv_slab_kernel<EPI, MT>), grid, dim3(1024), 0, st, a);
+  return 0;
+}
+
 size_t row_shm(int cin, int mt, int epi) {
   const size_t kp = cin == 32 ? 27 * 32 : 9 * 32;
   return (size_t)mt * 16 * (kp + 8) * sizeof(u16) +
-         (epi == CONV_EPI_GELU ? 2 * (size_t)kRowLutN * sizeof(u16) : 0);
+         (epi == CONV_EPI_GELU ? 2 * (size_t)kRowLutS * sizeof(u16) : 0);
 }
 
 }  // namespace conv
@@ -1078,9 +1355,21 @@ int conv3d_fwd(const ConvDesc* d, const void* x, const void* w, const float* bia
   const bool rows = rows_enabled() && (d->cin_pad == 32 || d->cin_pad == 8) &&
                     (epi == CONV_EPI_LINEAR_F32 ||
                      (d->y_cpad % 4 == 0 && al8(y) && al8(z_out) && al8(aux)));
+  // slab kernel: 64 x 8 x (2 z) per block step, about one block per CU
+  const bool slab = rows && d->cin_pad == 32 && slab_enabled() &&
+                    (long long)d->nx * d->ny * 64 < (1LL << 31);
+  const int sgx = (d->nx + 63) / 64, sgy = (d->ny + 7) / 8;
+  int szc = (int)(((long long)sgx * sgy * d->nz + 255) / 256);
+  szc = szc < 2 ? 2 : szc + (szc & 1);
+  ConvArgs as = a;
+  as.zchunk = szc;
+  const dim3 sgrid(sgx, sgy, (d->nz + szc - 1) / szc);
 #define CONV_LAUNCH(E, MT)                                                                   \
   do {                                                                                       \
-    if (rows && d->cin_pad == 32)                                                            \
+    if (slab) {                                                                              \
+      const int rc_ = slab_launch<E, MT>(as, sgrid, st);                                     \
+      if (rc_) return rc_;                                                                   \
+    } else if (rows && d->cin_pad == 32)                                                     \
       hipLaunchKernelGGL((conv_row_kernel<E, MT, 32>), grid, dim3(1024),                     \
                          row_shm(32, MT, E), st, a);                                         \
     else if (rows)                                                                           \
