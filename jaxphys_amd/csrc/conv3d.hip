@@ -385,9 +385,8 @@ __global__ __launch_bounds__(64 * WAVES) void conv_fwd_kernel(ConvArgs a) {
 // weights), so each lane loads one fragment per tile and nothing is shifted.
 // The bf16 GELU epilogue reads gelu of the bf16 pre-activation from a table
 // in LDS for |z| in [2^-14, 8) (bf16 exponents 113..129, both signs), built
-// by each block with the same gelu() as the direct path, and applies
-// gelu(z) = z (z >= 8) and -0 (z <= -8) beyond it (there erf_as(z / sqrt 2)
-// is +-1.0f exactly); the rare rest (tiny, inf, nan) takes the direct path.
+// by each block with the same gelu() as the direct path; the rare rest takes
+// the direct path.
 // Results are those of conv_fwd_kernel bit for bit with 32 channels; with 8
 // channels the K steps group the taps differently (float32 accumulation
 // order).
@@ -407,14 +406,14 @@ __device__ __forceinline__ bf16x8 row_ld(__amdgpu_buffer_rsrc_t rs, unsigned off
 // gelu of K (even) bf16 pre-activations, two per register in packed 16-bit
 // arithmetic: t = (|z| bits) - 113 * 128 (mod 2^16) indexes the table when
 // below kRowLutN (one unused entry per sign follows each half, so the index
-// min(t, kRowLutN) needs no select); 2176 <= t < 18176 is 8 <= |z| < inf,
-// whose gelu is z or -0 (min(z bits, 0x8000)); t >= 18176 (tiny, inf, nan)
-// takes the direct path, one branch for all K values
+// min(t, kRowLutN) needs no select); every other value (|z| >= 8, tiny,
+// inf, nan: t >= kRowLutN) takes the direct path, one branch for all K
+// values, rarely taken
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 constexpr int kRowLutS = kRowLutN + 1;  // table entries per sign
 template <int K>
 __device__ __forceinline__ void gelu_lut(const u16* lut, const u16* zb, u16* y) {
-  u16x2 tt[K / 2], big[K / 2];
+  u16x2 tt[K / 2];
   unsigned idx[K];
   u16x2 tmax = {0, 0};
 #pragma unroll
@@ -423,25 +422,40 @@ __device__ __forceinline__ void gelu_lut(const u16* lut, const u16* zb, u16* y) 
     const u16x2 tv = (z & (u16)0x7FFF) - (u16)(kRowE0 * 128);
     const u16x2 id = __builtin_elementwise_min(tv, (u16x2)(u16)kRowLutN) + (z >> 15) * (u16)kRowLutS;
     tt[v / 2] = tv;
-    big[v / 2] = __builtin_elementwise_min(z, (u16x2)(u16)0x8000);
     tmax = __builtin_elementwise_max(tmax, tv);
     idx[v] = id.x;
     idx[v + 1] = id.y;
   }
-  u16 lv[K];
 #pragma unroll
-  for (int v = 0; v < K; ++v) lv[v] = lut[idx[v]];
-#pragma unroll
-  for (int v = 0; v < K; ++v) {
-    const u16 t = (v & 1) ? tt[v / 2].y : tt[v / 2].x;
-    y[v] = t < (u16)kRowLutN ? lv[v] : ((v & 1) ? big[v / 2].y : big[v / 2].x);
-  }
-  if (__builtin_elementwise_max(tmax.x, tmax.y) >= (u16)((255 - kRowE0) * 128)) {
+  for (int v = 0; v < K; ++v) y[v] = lut[idx[v]];
+  if (__builtin_elementwise_max(tmax.x, tmax.y) >= (u16)kRowLutN) {
 #pragma unroll
     for (int v = 0; v < K; ++v) {
       const u16 t = (v & 1) ? tt[v / 2].y : tt[v / 2].x;
-      if (t >= (u16)((255 - kRowE0) * 128)) y[v] = f2bf(gelu(bf2f(zb[v])));
+      if (t >= (u16)kRowLutN) y[v] = f2bf(gelu(bf2f(zb[v])));
     }
+  }
+}
+
+// bf16 outputs of one lane's 4 channels cb..cb+3 (one 8-byte store each
+// for y and, when given, the pre-activation z); channels past cout are
+// written as zero (no test when every channel of the layer is real)
+__device__ __forceinline__ void bf16_out4(const u16* yv, const u16* zv, int cb, int cout, bool allreal,
+                                          __amdgpu_buffer_rsrc_t ors, const u16* zout, long long orow,
+                                          unsigned obytes, unsigned o) {
+  u16 y4[4], z4[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const bool real = allreal || cb + v < cout;
+    y4[v] = real ? yv[v] : (u16)0;
+    z4[v] = real ? zv[v] : (u16)0;
+  }
+  const u16x4 yy4 = {y4[0], y4[1], y4[2], y4[3]};
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(i32x2, yy4), ors, (int)o, 0, 0);
+  if (zout) {
+    const u16x4 zz4 = {z4[0], z4[1], z4[2], z4[3]};
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(i32x2, zz4), row_rsrc(zout + orow, obytes),
+                                          (int)o, 0, 0);
   }
 }
 
@@ -523,6 +537,7 @@ __global__ __launch_bounds__(1024) void conv_row_kernel(ConvArgs a) {
     for (int n = 0; n < NT; ++n) off[n] = (unsigned)((x0 + 16 * n + r + dx) * 16);
   }
   const int ycp = a.ycp;
+  const bool allreal = a.cout >= MT * 16;
   float bv[MT][4];
 #pragma unroll
   for (int m = 0; m < MT; ++m)
@@ -649,20 +664,8 @@ __global__ __launch_bounds__(1024) void conv_row_kernel(ConvArgs a) {
         const int cb = m * 16 + 4 * g;
         if (cb >= ycp) continue;
         const unsigned o = (unsigned)(px * ycp + cb) * 2u;
-        u16 y4[4], z4[4];
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const bool real = cb + v < a.cout;
-          y4[v] = real ? yv[m * 4 + v] : (u16)0;
-          z4[v] = real ? zv[m * 4 + v] : (u16)0;
-        }
-        const u16x4 yy4 = {y4[0], y4[1], y4[2], y4[3]};
-        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(i32x2, yy4), ors, (int)o, 0, 0);
-        if (EPI == CONV_EPI_GELU && a.zout) {
-          const u16x4 zz4 = {z4[0], z4[1], z4[2], z4[3]};
-          __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(i32x2, zz4),
-                                                row_rsrc(a.zout + orow, obytes), (int)o, 0, 0);
-        }
+        bf16_out4(yv + m * 4, zv + m * 4, cb, a.cout, allreal, ors,
+                  EPI == CONV_EPI_GELU ? a.zout : nullptr, orow, obytes, o);
       }
     }
     fa = fb;
@@ -681,8 +684,12 @@ __global__ __launch_bounds__(1024) void conv_row_kernel(ConvArgs a) {
 //    DPP shift (with an edge fragment: x0 - 1 in lane 0, x0 + 64 in lane 15);
 //  * a slab is one input plane's rows y0-1 .. y0+8 and columns x0-1 .. x0+64;
 //    within a row, position p's channel chunk g (16 bytes) sits at slot
-//    17 (p >> 2) + 4 (p & 3) + g (one unused slot per 4 positions), which
-//    puts the 16 lanes that read positions 4 r + c on distinct LDS banks;
+//    17 (4 (p & 3) + g) + (p >> 2): the 16 lanes of a row group, which read
+//    positions 4 r + c, read 16 consecutive slots, the one pattern in which
+//    ds_read_b128 runs at the full LDS rate on gfx950 (a 16-lane group
+//    spread over more than 256 bytes runs at half of it or less);
+//  * the weights sit in LDS as [tap][k chunk g][output channel r] 16-byte
+//    slots for the same reason;
 //  * each lane of a direct-to-LDS load writes 16 bytes at consecutive LDS
 //    addresses, so every slot's global source is chosen per lane; positions
 //    and rows outside the grid (and the unused slots) get an offset past the
@@ -696,29 +703,30 @@ __global__ __launch_bounds__(1024) void conv_row_kernel(ConvArgs a) {
 // Taps accumulate in the order (dz, dy, dx) of conv_fwd_kernel, so the
 // results are the same bits.
 // ---------------------------------------------------------------------------
-constexpr int kSlabRows = 10, kSlabRowSlots = 280;          // 66 positions + 16 unused slots
+constexpr int kSlabRows = 10, kSlabRowSlots = 272;          // 66 positions (+ 16 unused slots)
 constexpr int kSlabSlots = kSlabRows * kSlabRowSlots;        // 16-byte slots
-constexpr int kSlabBytes = kSlabSlots * 16;                  // 44800
-constexpr int kSlabLoads = (kSlabSlots + 63) / 64;           // 44 wave loads per slab
+constexpr int kSlabBytes = kSlabSlots * 16;                  // 43520
+constexpr int kSlabLoads = (kSlabSlots + 63) / 64;           // 43 wave loads per slab
 
-__device__ __forceinline__ int slab_slot(int pos, int g) { return 17 * (pos >> 2) + 4 * (pos & 3) + g; }
+__device__ __forceinline__ int slab_slot(int pos, int g) { return 17 * (4 * (pos & 3) + g) + (pos >> 2); }
 
 template <int EPI, int MT>
 __global__ __launch_bounds__(1024) void conv_slab_kernel(ConvArgs a) {
-  constexpr int NT = 4, CIN = 32, LDW = 27 * 32 + 8;
+  constexpr int NT = 4, CIN = 32;
   // separate LDS objects (one per weight M tile, one per slab buffer), so
   // that the compiler can tell the slab being read and the weights from the
   // slab being loaded and does not wait for those loads
-  __shared__ __attribute__((aligned(16))) u16 wsh0[16 * LDW];
-  __shared__ __attribute__((aligned(16))) u16 wsh1[MT == 2 ? 16 * LDW : 8];
+  __shared__ __attribute__((aligned(16))) uint4 wsh0[27 * 4 * 16];
+  __shared__ __attribute__((aligned(16))) uint4 wsh1[MT == 2 ? 27 * 4 * 16 : 1];
   __shared__ u16 lut[EPI == CONV_EPI_GELU ? 2 * kRowLutS : 2];
   __shared__ __attribute__((aligned(16))) unsigned char slab0[kSlabBytes];
   __shared__ __attribute__((aligned(16))) unsigned char slab1[kSlabBytes];
   {
+    // k chunk c = 4 tap + g (8 channels) of output channel row -> slot (c, row)
     constexpr int C16 = 27 * 32 / 8;
     for (int q = threadIdx.x; q < MT * 16 * C16; q += 1024) {
       const int row = q / C16, c = q - row * C16;
-      *reinterpret_cast<uint4*>((row < 16 ? wsh0 : wsh1) + (row & 15) * LDW + c * 8) =
+      (row < 16 ? wsh0 : wsh1)[c * 16 + (row & 15)] =
           *reinterpret_cast<const uint4*>(a.w + (long long)row * a.kpad + c * 8);
     }
     if (EPI == CONV_EPI_GELU) {
@@ -744,10 +752,10 @@ __global__ __launch_bounds__(1024) void conv_slab_kernel(ConvArgs a) {
   for (int u = 0; u < 3; ++u) {
     const int q = (wv + 16 * u) * 64 + t;
     const int row = q / kSlabRowSlots, rs = q - row * kSlabRowSlots;
-    const int k4 = rs / 17, j = rs - 17 * k4;
-    const int pos = 4 * k4 + (j >> 2), gg = j & 3;
+    const int cg = rs / 17, qd = rs - 17 * cg;
+    const int pos = 4 * qd + (cg >> 2), gg = cg & 3;
     const int sy = y0 - 1 + row, sx = x0 - 1 + pos;
-    const bool ok = q < kSlabSlots && j < 16 && pos < 66 && sy >= 0 && sy < ny && sx >= 0 && sx < nx;
+    const bool ok = q < kSlabSlots && pos < 66 && sy >= 0 && sy < ny && sx >= 0 && sx < nx;
     soff[u] = ok ? (unsigned)(((long long)sy * nx + sx) * 64 + 16 * gg) : 0x80000000u;
   }
   const unsigned planebytes = (unsigned)((long long)nx * ny * CIN * 2);
@@ -771,6 +779,7 @@ __global__ __launch_bounds__(1024) void conv_slab_kernel(ConvArgs a) {
       bv[m][v] = a.bias && co < a.cout ? a.bias[co] : 0.f;
     }
   const int ycp = a.ycp;
+  const bool allreal = a.cout >= MT * 16;
   const unsigned esz = EPI == CONV_EPI_LINEAR_F32 ? 4u : 2u;
   const unsigned obytes = (unsigned)nx * ycp * esz;
   // LDS byte offsets of this lane's fragments within a slab row
@@ -826,20 +835,8 @@ __global__ __launch_bounds__(1024) void conv_slab_kernel(ConvArgs a) {
           const int cb = m * 16 + 4 * g;
           if (cb >= ycp) continue;
           const unsigned o = (unsigned)(px * ycp + cb) * 2u;
-          u16 y4[4], z4[4];
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const bool real = cb + v < a.cout;
-            y4[v] = real ? yv[m * 4 + v] : (u16)0;
-            z4[v] = real ? zv[m * 4 + v] : (u16)0;
-          }
-          const u16x4 yy4 = {y4[0], y4[1], y4[2], y4[3]};
-          __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(i32x2, yy4), ors, (int)o, 0, 0);
-          if (EPI == CONV_EPI_GELU && a.zout) {
-            const u16x4 zz4 = {z4[0], z4[1], z4[2], z4[3]};
-            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(i32x2, zz4),
-                                                  row_rsrc(a.zout + orow, obytes), (int)o, 0, 0);
-          }
+          bf16_out4(yv + m * 4, zv + m * 4, cb, a.cout, allreal, ors,
+                    EPI == CONV_EPI_GELU ? a.zout : nullptr, orow, obytes, o);
         }
       }
     }
@@ -881,7 +878,7 @@ __global__ __launch_bounds__(1024) void conv_slab_kernel(ConvArgs a) {
           bf16x8 af[MT];
 #pragma unroll
           for (int m = 0; m < MT; ++m)
-            af[m] = load8((m == 0 ? wsh0 : wsh1) + r * LDW + (tap0 + dx) * 32 + 8 * g);
+            af[m] = __builtin_bit_cast(bf16x8, (m == 0 ? wsh0 : wsh1)[((tap0 + dx) * 4 + g) * 16 + r]);
 #pragma unroll
           for (int n = 0; n < NT; ++n)
 #pragma unroll
@@ -901,6 +898,190 @@ __global__ __launch_bounds__(1024) void conv_slab_kernel(ConvArgs a) {
     slab_step(ts + 1, slab1, slab0);
   }
   if (jw == 1) epilogue(z0 + 2 * (nsteps - 1) + 1);
+}
+
+// ---------------------------------------------------------------------------
+// Rolling-slab kernel for 8 input channels (the closure net's first layer,
+// and input gradients into 8-channel tensors).  A block (16 wavefronts)
+// computes 64 x by 16 rows, one plane per step; wave w owns row y0 + w.  An
+// input plane's rows y0-1 .. y0+16 and columns x0-1 .. x0+64 (18 x 66
+// positions of 16 bytes, 19 KiB) fill one of four LDS buffers: planes
+// zz-1, zz, zz+1 are read while plane zz+2 loads (direct-to-LDS), so every
+// input plane is fetched once per block.  K step (dz, dy) as in the row
+// kernel: k group g of the B operand is tap dx = g - 1, read straight from
+// the slab at x + dx (no shifts); group 3 meets zero weights.  Same K order
+// as the row kernel, so the same bits.
+// ---------------------------------------------------------------------------
+constexpr int kRollRows = 18, kRollPos = 66;
+constexpr int kRollSlots = kRollRows * kRollPos;             // 16-byte slots
+constexpr int kRollBytes = kRollSlots * 16;                  // 19008
+constexpr int kRollLoads = (kRollSlots + 63) / 64;           // 19 wave loads per plane
+
+template <int EPI, int MT>
+__global__ __launch_bounds__(1024) void conv_roll8_kernel(ConvArgs a) {
+  constexpr int NT = 4, CIN = 8;
+  // separate LDS objects, so that the compiler can tell each buffer being
+  // read from the one being loaded; weights as [pair][k group g][output
+  // channel r] 16-byte slots (a row group reads 16 consecutive slots)
+  __shared__ __attribute__((aligned(16))) uint4 wsh0[9 * 4 * 16];
+  __shared__ __attribute__((aligned(16))) uint4 wsh1[MT == 2 ? 9 * 4 * 16 : 1];
+  __shared__ u16 lut[EPI == CONV_EPI_GELU ? 2 * kRowLutS : 2];
+  __shared__ __attribute__((aligned(16))) unsigned char sl0[kRollBytes];
+  __shared__ __attribute__((aligned(16))) unsigned char sl1[kRollBytes];
+  __shared__ __attribute__((aligned(16))) unsigned char sl2[kRollBytes];
+  __shared__ __attribute__((aligned(16))) unsigned char sl3[kRollBytes];
+  {
+    // the 27 taps x 8 channels of each (dz, dy) pair over 32 K slots, the
+    // last 8 zero
+    constexpr int C16 = 9 * 4;
+    for (int q = threadIdx.x; q < MT * 16 * C16; q += 1024) {
+      const int row = q / C16, c = q - row * C16;
+      const int p = c >> 2, gg = c & 3;
+      uint4 v = {0u, 0u, 0u, 0u};
+      if (gg < 3) v = *reinterpret_cast<const uint4*>(a.w + (long long)row * a.kpad + (3 * p + gg) * 8);
+      (row < 16 ? wsh0 : wsh1)[c * 16 + (row & 15)] = v;
+    }
+    if (EPI == CONV_EPI_GELU) {
+      for (int q = threadIdx.x; q < 2 * kRowLutS; q += 1024) {
+        const unsigned neg = q >= kRowLutS, j = q - neg * kRowLutS;
+        const unsigned zb = (neg << 15) | (kRowE0 * 128u + j);
+        lut[q] = j < (unsigned)kRowLutN ? f2bf(gelu(bf2f((u16)zb))) : (u16)0;
+      }
+    }
+  }
+  const int t = threadIdx.x & 63;
+  const int g = t >> 4, r = t & 15;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const int x0 = blockIdx.x * 64, y0 = blockIdx.y * 16;
+  const int z0 = blockIdx.z * a.zchunk, z1 = min(z0 + a.zchunk, nz);
+  const int yy = y0 + wv;
+  const bool rowok = yy < ny;
+  // this lane's slots of the block's plane loads (instruction k = wv + 16 u)
+  unsigned soff[2];
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int q = (wv + 16 * u) * 64 + t;
+    const int row = q / kRollPos, pos = q - row * kRollPos;
+    const int sy = y0 - 1 + row, sx = x0 - 1 + pos;
+    const bool ok = q < kRollSlots && sy >= 0 && sy < ny && sx >= 0 && sx < nx;
+    soff[u] = ok ? (unsigned)(((long long)sy * nx + sx) * 16) : 0x80000000u;
+  }
+  const unsigned planebytes = (unsigned)((long long)nx * ny * CIN * 2);
+  auto stage = [&](int iz, unsigned char* dst) __attribute__((always_inline)) {
+    const bool pl = iz >= 0 && iz < nz;
+    const auto rs = row_rsrc(a.x + (long long)(pl ? iz : 0) * ny * nx * CIN, pl ? planebytes : 0u);
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int k = wv + 16 * u;
+      if (k < kRollLoads && (k * 64 + t) < kRollSlots)
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(
+            rs, (__attribute__((address_space(3))) void*)(dst + k * 1024), 16, soff[u], 0, 0, 0);
+    }
+  };
+  float bv[MT][4];
+#pragma unroll
+  for (int m = 0; m < MT; ++m)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int co = m * 16 + 4 * g + v;
+      bv[m][v] = a.bias && co < a.cout ? a.bias[co] : 0.f;
+    }
+  const int ycp = a.ycp;
+  const bool allreal = a.cout >= MT * 16;
+  const unsigned esz = EPI == CONV_EPI_LINEAR_F32 ? 4u : 2u;
+  const unsigned obytes = (unsigned)nx * ycp * esz;
+  // this lane's B fragment of tile n, tap dx = g - 1 (group 3: dx = 0), in
+  // slab row wv + 1 + dy: position 16 n + r + 1 + dx
+  unsigned boff[NT];
+#pragma unroll
+  for (int n = 0; n < NT; ++n) boff[n] = (unsigned)((16 * n + r + (g < 3 ? g : 1)) * 16);
+  const int nplanes = z1 - z0;
+  auto plane_step = [&](int s, const unsigned char* pm, const unsigned char* p0, const unsigned char* pp,
+                        unsigned char* nxt) __attribute__((always_inline)) {
+    const int zz = z0 + s;
+    if (s + 1 < nplanes) stage(zz + 2, nxt);
+    if (rowok) {
+      f32x4 acc[MT][NT];
+#pragma unroll
+      for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int n = 0; n < NT; ++n) acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int p = 0; p < 9; ++p) {
+        const unsigned char* sb = p < 3 ? pm : (p < 6 ? p0 : pp);
+        const unsigned char* rowp = sb + (wv + p % 3) * (kRollPos * 16);
+        bf16x8 b[NT];
+#pragma unroll
+        for (int n = 0; n < NT; ++n) b[n] = *reinterpret_cast<const bf16x8*>(rowp + boff[n]);
+        bf16x8 af[MT];
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+          af[m] = __builtin_bit_cast(bf16x8, (m == 0 ? wsh0 : wsh1)[(p * 4 + g) * 16 + r]);
+#pragma unroll
+        for (int n = 0; n < NT; ++n)
+#pragma unroll
+          for (int m = 0; m < MT; ++m)
+            acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[m], b[n], acc[m][n], 0, 0, 0);
+      }
+      const long long orow = ((long long)zz * ny + yy) * nx * ycp;
+      char* ybase = reinterpret_cast<char*>(a.y) + orow * esz;
+      const auto ors = row_rsrc(ybase, obytes);
+#pragma unroll
+      for (int n = 0; n < NT; ++n) {
+        const int px = x0 + 16 * n + r;
+        if (EPI == CONV_EPI_LINEAR_F32) {
+#pragma unroll
+          for (int m = 0; m < MT; ++m) f32_out(acc[m][n], bv[m], m * 16 + 4 * g, a.cout, ycp, ors, px);
+          continue;
+        }
+        u16 zv[MT * 4] = {}, yv[MT * 4] = {};
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const float val = acc[m][n][v];
+            if (EPI == CONV_EPI_GELU) zv[m * 4 + v] = f2bf(val + bv[m][v]);
+            else if (EPI == CONV_EPI_LINEAR_BF16) yv[m * 4 + v] = f2bf(val + bv[m][v]);
+          }
+        if (EPI == CONV_EPI_GELU) gelu_lut<MT * 4>(lut, zv, yv);
+        if (EPI == CONV_EPI_GELU_BWD) {
+#pragma unroll
+          for (int m = 0; m < MT; ++m) {
+            const unsigned o = (unsigned)(px * ycp + m * 16 + 4 * g) * 2u;
+            const auto av = __builtin_amdgcn_raw_buffer_load_b64(row_rsrc(a.aux + orow, obytes),
+                                                                 (int)o, 0, 0);
+            const u16x4 aux4 = __builtin_bit_cast(u16x4, av);
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+              yv[m * 4 + v] = f2bf(acc[m][n][v] * gelu_grad(bf2f(aux4[v])));
+          }
+        }
+#pragma unroll
+        for (int m = 0; m < MT; ++m) {
+          const int cb = m * 16 + 4 * g;
+          if (cb >= ycp) continue;
+          const unsigned o = (unsigned)(px * ycp + cb) * 2u;
+          bf16_out4(yv + m * 4, zv + m * 4, cb, a.cout, allreal, ors,
+                    EPI == CONV_EPI_GELU ? a.zout : nullptr, orow, obytes, o);
+        }
+      }
+    }
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's plane loads
+    __syncthreads();
+  };
+  stage(z0 - 1, sl0);
+  stage(z0, sl1);
+  stage(z0 + 1, sl2);
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+  __syncthreads();
+  // plane z0 + s reads buffers s, s+1, s+2 (mod 4) and loads into s+3
+  for (int s = 0; s < nplanes; s += 4) {
+    plane_step(s, sl0, sl1, sl2, sl3);
+    if (s + 1 < nplanes) plane_step(s + 1, sl1, sl2, sl3, sl0);
+    if (s + 2 < nplanes) plane_step(s + 2, sl2, sl3, sl0, sl1);
+    if (s + 3 < nplanes) plane_step(s + 3, sl3, sl0, sl1, sl2);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1364,11 +1545,23 @@ int conv3d_fwd(const ConvDesc* d, const void* x, const void* w, const float* bia
   ConvArgs as = a;
   as.zchunk = szc;
   const dim3 sgrid(sgx, sgy, (d->nz + szc - 1) / szc);
+  // rolling-slab kernel for 8 channels: 64 x 16 rows per block, about one
+  // block per CU
+  const bool roll = rows && d->cin_pad == 8 && slab_enabled() &&
+                    (long long)d->nx * d->ny * 16 < (1LL << 31);
+  const int rgx = (d->nx + 63) / 64, rgy = (d->ny + 15) / 16;
+  int rzc = (int)(((long long)rgx * rgy * d->nz + 255) / 256);
+  rzc = rzc < 1 ? 1 : rzc;
+  ConvArgs ar = a;
+  ar.zchunk = rzc;
+  const dim3 rgrid(rgx, rgy, (d->nz + rzc - 1) / rzc);
 #define CONV_LAUNCH(E, MT)                                                                   \
   do {                                                                                       \
     if (slab) {                                                                              \
       const int rc_ = slab_launch<E, MT>(as, sgrid, st);                                     \
       if (rc_) return rc_;                                                                   \
+    } else if (roll) {                                                                       \
+      hipLaunchKernelGGL((conv_roll8_kernel<E, MT>), rgrid, dim3(1024), 0, st, ar);          \
     } else if (rows && d->cin_pad == 32)                                                     \
       hipLaunchKernelGGL((conv_row_kernel<E, MT, 32>), grid, dim3(1024),                     \
                          row_shm(32, MT, E), st, a);                                         \
