@@ -493,7 +493,7 @@ def closure_net_bench(args, dev):
         "data": "synthetic", "scaling": "weak", "vs_baseline": None,
         "config": {"workload": "closure_net", "grid": [n] * 3, "channels": [7, 32, 32, 6],
                    "kernel": "3x3x3"},
-        "roofline": {"bound": "mfma", "kernel": "conv_fwd_kernel (3 launches per inference)",
+        "roofline": {"bound": "mfma", "kernel": "conv_roll8_kernel + 2 conv_slab_kernel launches per inference",
                      "achieved": round(tflops, 2), "peak": 2500.0, "unit": "TFLOP/s",
                      "frac": round(tflops / 2500.0, 4), "traffic": None},
         "training_step": train,
