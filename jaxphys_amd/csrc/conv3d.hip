@@ -403,59 +403,116 @@ __device__ __forceinline__ bf16x8 row_ld(__amdgpu_buffer_rsrc_t rs, unsigned off
   const auto v = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)off, 0, 0);
   return __builtin_bit_cast(bf16x8, v);
 }
-// gelu of K (even) bf16 pre-activations, two per register in packed 16-bit
-// arithmetic: t = (|z| bits) - 113 * 128 (mod 2^16) indexes the table when
-// below kRowLutN (one unused entry per sign follows each half, so the index
-// min(t, kRowLutN) needs no select); every other value (|z| >= 8, tiny,
-// inf, nan: t >= kRowLutN) takes the direct path, one branch for all K
-// values, rarely taken
+// two floats -> two bf16 (round to nearest even, v_cvt_pk_bf16_f32), the
+// first in the low half.  The empty asm keeps the packed word opaque, so
+// that the table index below is computed from it (two values per packed
+// 16-bit instruction) rather than rebuilt from the floats' sign bits.
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ unsigned cvt2(float lo, float hi) {
+  const f32x2 v = {lo, hi};
+  unsigned w = __builtin_bit_cast(unsigned, __builtin_convertvector(v, bf16x2));
+  asm volatile("" : "+v"(w));
+  return w;
+}
+
+// gelu of W words of two bf16 pre-activations each: t = (|z| bits) -
+// 113 * 128 (mod 2^16) indexes the table when below kRowLutN (one unused
+// entry per sign follows each half, so the index min(t, kRowLutN) needs no
+// select); every other value (|z| >= 8, tiny, inf, nan: t >= kRowLutN)
+// takes the direct path, one branch for all W words, rarely taken
 constexpr int kRowLutS = kRowLutN + 1;  // table entries per sign
-template <int K>
-__device__ __forceinline__ void gelu_lut(const u16* lut, const u16* zb, u16* y) {
-  u16x2 tt[K / 2];
-  unsigned idx[K];
+template <int W>
+__device__ __forceinline__ void gelu_words(const u16* lut, const unsigned* zw, unsigned* yw) {
+  u16x2 tt[W];
+  unsigned lo[W], hi[W];
   u16x2 tmax = {0, 0};
 #pragma unroll
-  for (int v = 0; v < K; v += 2) {
-    const u16x2 z = {zb[v], zb[v + 1]};
+  for (int w = 0; w < W; ++w) {
+    const u16x2 z = __builtin_bit_cast(u16x2, zw[w]);
     const u16x2 tv = (z & (u16)0x7FFF) - (u16)(kRowE0 * 128);
-    const u16x2 id = __builtin_elementwise_min(tv, (u16x2)(u16)kRowLutN) + (z >> 15) * (u16)kRowLutS;
-    tt[v / 2] = tv;
+    const u16x2 id =
+        (__builtin_elementwise_min(tv, (u16x2)(u16)kRowLutN) + (z >> 15) * (u16)kRowLutS) << 1;
+    tt[w] = tv;
     tmax = __builtin_elementwise_max(tmax, tv);
-    idx[v] = id.x;
-    idx[v + 1] = id.y;
+    const unsigned idw = __builtin_bit_cast(unsigned, id);
+    lo[w] = idw & 0xFFFFu;
+    hi[w] = idw >> 16;
   }
+  const unsigned char* lb = reinterpret_cast<const unsigned char*>(lut);
 #pragma unroll
-  for (int v = 0; v < K; ++v) y[v] = lut[idx[v]];
+  for (int w = 0; w < W; ++w)
+    yw[w] = (unsigned)*reinterpret_cast<const u16*>(lb + lo[w]) |
+            ((unsigned)*reinterpret_cast<const u16*>(lb + hi[w]) << 16);
   if (__builtin_elementwise_max(tmax.x, tmax.y) >= (u16)kRowLutN) {
 #pragma unroll
-    for (int v = 0; v < K; ++v) {
-      const u16 t = (v & 1) ? tt[v / 2].y : tt[v / 2].x;
-      if (t >= (u16)kRowLutN) y[v] = f2bf(gelu(bf2f(zb[v])));
+    for (int w = 0; w < W; ++w) {
+      if (tt[w].x >= (u16)kRowLutN)
+        yw[w] = (yw[w] & 0xFFFF0000u) | f2bf(gelu(bf2f((u16)(zw[w] & 0xFFFFu))));
+      if (tt[w].y >= (u16)kRowLutN)
+        yw[w] = (yw[w] & 0xFFFFu) | ((unsigned)f2bf(gelu(bf2f((u16)(zw[w] >> 16)))) << 16);
     }
   }
 }
 
-// bf16 outputs of one lane's 4 channels cb..cb+3 (one 8-byte store each
-// for y and, when given, the pre-activation z); channels past cout are
-// written as zero (no test when every channel of the layer is real)
-__device__ __forceinline__ void bf16_out4(const u16* yv, const u16* zv, int cb, int cout, bool allreal,
-                                          __amdgpu_buffer_rsrc_t ors, const u16* zout, long long orow,
-                                          unsigned obytes, unsigned o) {
-  u16 y4[4], z4[4];
-#pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    const bool real = allreal || cb + v < cout;
-    y4[v] = real ? yv[v] : (u16)0;
-    z4[v] = real ? zv[v] : (u16)0;
+// bf16 outputs of one lane's 4 channels cb..cb+3, two per word (one 8-byte
+// store each for y and, when given, the pre-activation z); channels past
+// cout are written as zero (no test when every channel of the layer is real)
+__device__ __forceinline__ void bf16_store4(const unsigned* yw, const unsigned* zw, int cb, int cout,
+                                            bool allreal, __amdgpu_buffer_rsrc_t ors, const u16* zout,
+                                            long long orow, unsigned obytes, unsigned o) {
+  unsigned y0 = yw[0], y1 = yw[1], z0 = zw[0], z1 = zw[1];
+  if (!allreal) {
+    const unsigned m0 = (cb < cout ? 0xFFFFu : 0u) | (cb + 1 < cout ? 0xFFFF0000u : 0u);
+    const unsigned m1 = (cb + 2 < cout ? 0xFFFFu : 0u) | (cb + 3 < cout ? 0xFFFF0000u : 0u);
+    y0 &= m0; z0 &= m0; y1 &= m1; z1 &= m1;
   }
-  const u16x4 yy4 = {y4[0], y4[1], y4[2], y4[3]};
-  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(i32x2, yy4), ors, (int)o, 0, 0);
+  const i32x2 yy = {(int)y0, (int)y1};
+  __builtin_amdgcn_raw_buffer_store_b64(yy, ors, (int)o, 0, 0);
   if (zout) {
-    const u16x4 zz4 = {z4[0], z4[1], z4[2], z4[3]};
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(i32x2, zz4), row_rsrc(zout + orow, obytes),
-                                          (int)o, 0, 0);
+    const i32x2 zz = {(int)z0, (int)z1};
+    __builtin_amdgcn_raw_buffer_store_b64(zz, row_rsrc(zout + orow, obytes), (int)o, 0, 0);
+  }
+}
+
+// bf16 epilogue of one column tile: an[m] holds the sums of channels
+// m * 16 + 4 g .. + 3 at position px
+template <int EPI, int MT>
+__device__ __forceinline__ void bf16_epi(const f32x4* an, const float (*bv)[4], const u16* lut,
+                                         const ConvArgs& a, int g, int px, int ycp, bool allreal,
+                                         __amdgpu_buffer_rsrc_t ors, long long orow, unsigned obytes) {
+  unsigned zw[MT * 2] = {}, yw[MT * 2] = {};
+  if (EPI == CONV_EPI_GELU || EPI == CONV_EPI_LINEAR_BF16) {
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const unsigned w = cvt2(an[m][2 * h] + bv[m][2 * h], an[m][2 * h + 1] + bv[m][2 * h + 1]);
+        if (EPI == CONV_EPI_GELU) zw[m * 2 + h] = w;
+        else yw[m * 2 + h] = w;
+      }
+  }
+  if (EPI == CONV_EPI_GELU) gelu_words<MT * 2>(lut, zw, yw);
+  if (EPI == CONV_EPI_GELU_BWD) {
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+      const unsigned o = (unsigned)(px * ycp + m * 16 + 4 * g) * 2u;
+      const auto av = __builtin_amdgcn_raw_buffer_load_b64(row_rsrc(a.aux + orow, obytes), (int)o, 0, 0);
+      const u16x4 aux4 = __builtin_bit_cast(u16x4, av);
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+        yw[m * 2 + h] = (unsigned)f2bf(an[m][2 * h] * gelu_grad(bf2f(aux4[2 * h]))) |
+                        ((unsigned)f2bf(an[m][2 * h + 1] * gelu_grad(bf2f(aux4[2 * h + 1]))) << 16);
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < MT; ++m) {
+    const int cb = m * 16 + 4 * g;
+    if (cb >= ycp) continue;
+    const unsigned o = (unsigned)(px * ycp + cb) * 2u;
+    bf16_store4(yw + m * 2, zw + m * 2, cb, a.cout, allreal, ors,
+                EPI == CONV_EPI_GELU ? a.zout : nullptr, orow, obytes, o);
   }
 }
 
@@ -636,36 +693,11 @@ __global__ __launch_bounds__(1024) void conv_row_kernel(ConvArgs a) {
       if (EPI == CONV_EPI_LINEAR_F32) continue;
       // bf16 outputs: the lane's 4 consecutive channels per M tile in one
       // 8-byte store; channels past cout are written as zero
-      u16 zv[MT * 4] = {}, yv[MT * 4] = {};
+      {
+        f32x4 an[MT];
 #pragma unroll
-      for (int m = 0; m < MT; ++m)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const float val = acc[m][n][v];
-          if (EPI == CONV_EPI_GELU) zv[m * 4 + v] = f2bf(val + bv[m][v]);
-          else if (EPI == CONV_EPI_LINEAR_BF16) yv[m * 4 + v] = f2bf(val + bv[m][v]);
-        }
-      if (EPI == CONV_EPI_GELU) gelu_lut<MT * 4>(lut, zv, yv);
-      if (EPI == CONV_EPI_GELU_BWD) {
-#pragma unroll
-        for (int m = 0; m < MT; ++m) {
-          const int cb = m * 16 + 4 * g;
-          const unsigned o = (unsigned)(px * ycp + cb) * 2u;
-          const auto av = __builtin_amdgcn_raw_buffer_load_b64(row_rsrc(a.aux + orow, obytes),
-                                                               (int)o, 0, 0);
-          const u16x4 aux4 = __builtin_bit_cast(u16x4, av);
-#pragma unroll
-          for (int v = 0; v < 4; ++v)
-            yv[m * 4 + v] = f2bf(acc[m][n][v] * gelu_grad(bf2f(aux4[v])));
-        }
-      }
-#pragma unroll
-      for (int m = 0; m < MT; ++m) {
-        const int cb = m * 16 + 4 * g;
-        if (cb >= ycp) continue;
-        const unsigned o = (unsigned)(px * ycp + cb) * 2u;
-        bf16_out4(yv + m * 4, zv + m * 4, cb, a.cout, allreal, ors,
-                  EPI == CONV_EPI_GELU ? a.zout : nullptr, orow, obytes, o);
+        for (int m = 0; m < MT; ++m) an[m] = acc[m][n];
+        bf16_epi<EPI, MT>(an, bv, lut, a, g, px, ycp, allreal, ors, orow, obytes);
       }
     }
     fa = fb;
@@ -770,6 +802,8 @@ __global__ __launch_bounds__(1024) void conv_slab_kernel(ConvArgs a) {
             rs, (__attribute__((address_space(3))) void*)(dst + k * 1024), 16, soff[u], 0, 0, 0);
     }
   };
+  const int ycp = a.ycp;
+  const bool allreal = a.cout >= MT * 16;
   float bv[MT][4];
 #pragma unroll
   for (int m = 0; m < MT; ++m)
@@ -778,8 +812,6 @@ __global__ __launch_bounds__(1024) void conv_slab_kernel(ConvArgs a) {
       const int co = m * 16 + 4 * g + v;
       bv[m][v] = a.bias && co < a.cout ? a.bias[co] : 0.f;
     }
-  const int ycp = a.ycp;
-  const bool allreal = a.cout >= MT * 16;
   const unsigned esz = EPI == CONV_EPI_LINEAR_F32 ? 4u : 2u;
   const unsigned obytes = (unsigned)nx * ycp * esz;
   // LDS byte offsets of this lane's fragments within a slab row
@@ -808,35 +840,11 @@ __global__ __launch_bounds__(1024) void conv_slab_kernel(ConvArgs a) {
           for (int m = 0; m < MT; ++m) f32_out(acc[m][n], bv[m], m * 16 + 4 * g, a.cout, ycp, ors, px);
           continue;
         }
-        u16 zv[MT * 4] = {}, yv[MT * 4] = {};
+        {
+          f32x4 an[MT];
 #pragma unroll
-        for (int m = 0; m < MT; ++m)
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const float val = acc[m][n][v];
-            if (EPI == CONV_EPI_GELU) zv[m * 4 + v] = f2bf(val + bv[m][v]);
-            else if (EPI == CONV_EPI_LINEAR_BF16) yv[m * 4 + v] = f2bf(val + bv[m][v]);
-          }
-        if (EPI == CONV_EPI_GELU) gelu_lut<MT * 4>(lut, zv, yv);
-        if (EPI == CONV_EPI_GELU_BWD) {
-#pragma unroll
-          for (int m = 0; m < MT; ++m) {
-            const unsigned o = (unsigned)(px * ycp + m * 16 + 4 * g) * 2u;
-            const auto av = __builtin_amdgcn_raw_buffer_load_b64(row_rsrc(a.aux + orow, obytes),
-                                                                 (int)o, 0, 0);
-            const u16x4 aux4 = __builtin_bit_cast(u16x4, av);
-#pragma unroll
-            for (int v = 0; v < 4; ++v)
-              yv[m * 4 + v] = f2bf(acc[m][n][v] * gelu_grad(bf2f(aux4[v])));
-          }
-        }
-#pragma unroll
-        for (int m = 0; m < MT; ++m) {
-          const int cb = m * 16 + 4 * g;
-          if (cb >= ycp) continue;
-          const unsigned o = (unsigned)(px * ycp + cb) * 2u;
-          bf16_out4(yv + m * 4, zv + m * 4, cb, a.cout, allreal, ors,
-                    EPI == CONV_EPI_GELU ? a.zout : nullptr, orow, obytes, o);
+          for (int m = 0; m < MT; ++m) an[m] = acc[m][n];
+          bf16_epi<EPI, MT>(an, bv, lut, a, g, px, ycp, allreal, ors, orow, obytes);
         }
       }
     }
@@ -997,76 +1005,73 @@ __global__ __launch_bounds__(1024) void conv_roll8_kernel(ConvArgs a) {
 #pragma unroll
   for (int n = 0; n < NT; ++n) boff[n] = (unsigned)((16 * n + r + (g < 3 ? g : 1)) * 16);
   const int nplanes = z1 - z0;
+  f32x4 acc[MT][NT];
+  // fragments of one (dz, dy) pair; the next pair's six LDS reads are issued
+  // before the current pair's MFMAs (the scheduling barrier keeps them there:
+  // left alone, the compiler reads each fragment just before its two MFMAs
+  // and waits for it)
+  struct Frag {
+    bf16x8 b[NT], af[MT];
+  };
+  auto compute = [&](const unsigned char* pm, const unsigned char* p0, const unsigned char* pp)
+                     __attribute__((always_inline)) {
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+      for (int n = 0; n < NT; ++n) acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+    auto load = [&](int p, Frag& f) __attribute__((always_inline)) {
+      const unsigned char* sb = p < 3 ? pm : (p < 6 ? p0 : pp);
+      const unsigned char* rowp = sb + (wv + p % 3) * (kRollPos * 16);
+#pragma unroll
+      for (int n = 0; n < NT; ++n) f.b[n] = *reinterpret_cast<const bf16x8*>(rowp + boff[n]);
+#pragma unroll
+      for (int m = 0; m < MT; ++m)
+        f.af[m] = __builtin_bit_cast(bf16x8, (m == 0 ? wsh0 : wsh1)[(p * 4 + g) * 16 + r]);
+    };
+    Frag f[2];
+    load(0, f[0]);
+#pragma unroll
+    for (int p = 0; p < 9; ++p) {
+      if (p + 1 < 9) load(p + 1, f[(p + 1) & 1]);
+      __builtin_amdgcn_sched_barrier(0);
+      const Frag& c = f[p & 1];
+#pragma unroll
+      for (int n = 0; n < NT; ++n)
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+          acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(c.af[m], c.b[n], acc[m][n], 0, 0, 0);
+    }
+  };
+  auto epilogue = [&](int zz) __attribute__((always_inline)) {
+    if (!rowok) return;
+    const long long orow = ((long long)zz * ny + yy) * nx * ycp;
+    char* ybase = reinterpret_cast<char*>(a.y) + orow * esz;
+    const auto ors = row_rsrc(ybase, obytes);
+#pragma unroll
+    for (int n = 0; n < NT; ++n) {
+      const int px = x0 + 16 * n + r;
+      if (EPI == CONV_EPI_LINEAR_F32) {
+#pragma unroll
+        for (int m = 0; m < MT; ++m) f32_out(acc[m][n], bv[m], m * 16 + 4 * g, a.cout, ycp, ors, px);
+        continue;
+      }
+      {
+        f32x4 an[MT];
+#pragma unroll
+        for (int m = 0; m < MT; ++m) an[m] = acc[m][n];
+        bf16_epi<EPI, MT>(an, bv, lut, a, g, px, ycp, allreal, ors, orow, obytes);
+      }
+    }
+  };
+  // plane zz - 1 is written at the start of step zz: its stores then drain
+  // during plane zz's MFMAs, and the wait for this wave's plane loads at the
+  // end of the step (the same counter counts stores) finds them done
   auto plane_step = [&](int s, const unsigned char* pm, const unsigned char* p0, const unsigned char* pp,
                         unsigned char* nxt) __attribute__((always_inline)) {
     const int zz = z0 + s;
+    if (s > 0) epilogue(zz - 1);
     if (s + 1 < nplanes) stage(zz + 2, nxt);
-    if (rowok) {
-      f32x4 acc[MT][NT];
-#pragma unroll
-      for (int m = 0; m < MT; ++m)
-#pragma unroll
-        for (int n = 0; n < NT; ++n) acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int p = 0; p < 9; ++p) {
-        const unsigned char* sb = p < 3 ? pm : (p < 6 ? p0 : pp);
-        const unsigned char* rowp = sb + (wv + p % 3) * (kRollPos * 16);
-        bf16x8 b[NT];
-#pragma unroll
-        for (int n = 0; n < NT; ++n) b[n] = *reinterpret_cast<const bf16x8*>(rowp + boff[n]);
-        bf16x8 af[MT];
-#pragma unroll
-        for (int m = 0; m < MT; ++m)
-          af[m] = __builtin_bit_cast(bf16x8, (m == 0 ? wsh0 : wsh1)[(p * 4 + g) * 16 + r]);
-#pragma unroll
-        for (int n = 0; n < NT; ++n)
-#pragma unroll
-          for (int m = 0; m < MT; ++m)
-            acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[m], b[n], acc[m][n], 0, 0, 0);
-      }
-      const long long orow = ((long long)zz * ny + yy) * nx * ycp;
-      char* ybase = reinterpret_cast<char*>(a.y) + orow * esz;
-      const auto ors = row_rsrc(ybase, obytes);
-#pragma unroll
-      for (int n = 0; n < NT; ++n) {
-        const int px = x0 + 16 * n + r;
-        if (EPI == CONV_EPI_LINEAR_F32) {
-#pragma unroll
-          for (int m = 0; m < MT; ++m) f32_out(acc[m][n], bv[m], m * 16 + 4 * g, a.cout, ycp, ors, px);
-          continue;
-        }
-        u16 zv[MT * 4] = {}, yv[MT * 4] = {};
-#pragma unroll
-        for (int m = 0; m < MT; ++m)
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const float val = acc[m][n][v];
-            if (EPI == CONV_EPI_GELU) zv[m * 4 + v] = f2bf(val + bv[m][v]);
-            else if (EPI == CONV_EPI_LINEAR_BF16) yv[m * 4 + v] = f2bf(val + bv[m][v]);
-          }
-        if (EPI == CONV_EPI_GELU) gelu_lut<MT * 4>(lut, zv, yv);
-        if (EPI == CONV_EPI_GELU_BWD) {
-#pragma unroll
-          for (int m = 0; m < MT; ++m) {
-            const unsigned o = (unsigned)(px * ycp + m * 16 + 4 * g) * 2u;
-            const auto av = __builtin_amdgcn_raw_buffer_load_b64(row_rsrc(a.aux + orow, obytes),
-                                                                 (int)o, 0, 0);
-            const u16x4 aux4 = __builtin_bit_cast(u16x4, av);
-#pragma unroll
-            for (int v = 0; v < 4; ++v)
-              yv[m * 4 + v] = f2bf(acc[m][n][v] * gelu_grad(bf2f(aux4[v])));
-          }
-        }
-#pragma unroll
-        for (int m = 0; m < MT; ++m) {
-          const int cb = m * 16 + 4 * g;
-          if (cb >= ycp) continue;
-          const unsigned o = (unsigned)(px * ycp + cb) * 2u;
-          bf16_out4(yv + m * 4, zv + m * 4, cb, a.cout, allreal, ors,
-                    EPI == CONV_EPI_GELU ? a.zout : nullptr, orow, obytes, o);
-        }
-      }
-    }
+    if (rowok) compute(pm, p0, pp);
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's plane loads
     __syncthreads();
   };
@@ -1082,6 +1087,7 @@ __global__ __launch_bounds__(1024) void conv_roll8_kernel(ConvArgs a) {
     if (s + 2 < nplanes) plane_step(s + 2, sl2, sl3, sl0, sl1);
     if (s + 3 < nplanes) plane_step(s + 3, sl3, sl0, sl1, sl2);
   }
+  epilogue(z0 + nplanes - 1);
 }
 
 // ---------------------------------------------------------------------------
