@@ -74,6 +74,13 @@ int conv3d_flip_weights(const ConvDesc* d, const void* w, int32_t dy_cpad, void*
 int conv3d_pack_fields(int32_t nx, int32_t ny, int32_t nz, float* const* fields, const float* eps,
                        const float* scale, float eps_shift, float eps_scale, int32_t flags,
                        void* out, void* stream);
+/* conv3d_pack_fields over the box (x0, y0, z0) + [0, bnx) x [0, bny) x
+ * [0, bnz) of the grid, out in box order (bnz, bny, bnx, 8): the closure
+ * net's input on the cells it is evaluated on.                            */
+int conv3d_pack_fields_box(int32_t nx, int32_t ny, int32_t nz, float* const* fields,
+                           const float* eps, const float* scale, float eps_shift, float eps_scale,
+                           int32_t flags, int32_t x0, int32_t y0, int32_t z0, int32_t bnx,
+                           int32_t bny, int32_t bnz, void* out, void* stream);
 /* fields[c] += gain * scale[c] * y[..][c], y float32 channels last [..][ystride],
  * for the cells at least `margin` cells from every face; the others (the
  * absorbing layers, where the closure is not trained) are left unchanged. */
@@ -91,6 +98,35 @@ int conv3d_apply_correction(int32_t nx, int32_t ny, int32_t nz, float* const* fi
 int conv3d_stencil_f32(int32_t nx, int32_t ny, int32_t nz, float* const* fields,
                        const float* eps, const float* scale, const float* w, float* y,
                        int32_t ystride, void* stream);
+/* conv3d_stencil_f32 with y[i][co] = (not +=) the operator's value.    */
+int conv3d_stencil_f32_set(int32_t nx, int32_t ny, int32_t nz, float* const* fields,
+                           const float* eps, const float* scale, const float* w, float* y,
+                           int32_t ystride, void* stream);
+
+/* One closure correction in one pass (no reference counterpart; the same
+ * float operations as y = net_gain * (yb - y0b), y += s, y[mask] = 0 and
+ * conv3d_apply_correction): for the cells at least `margin` from every
+ * face,
+ *   v[c] = net_gain * (yb[j][c] - y0b[j][c])   (yb, y0b: the net's output
+ *          and its zero-state response, float32 [..][ybstride], on the box
+ *          box[0..2] + [0, box[3]) x [0, box[4]) x [0, box[5]) (x, y, z),
+ *          which must hold those cells; 0 when yb is NULL)
+ *        + s[i][c]                              (the far-field operator's
+ *          value, float32 [..][sstride]; 0 when s is NULL),
+ *   where mask[i] != 0 (uint8 per cell, may be NULL): v = 0 (mask_all) or
+ *   the s term left out (!mask_all);
+ *   fields[c][i] += gain * scale[c] * v[c].                               */
+int conv3d_apply_closure(int32_t nx, int32_t ny, int32_t nz, float* const* fields, const float* s,
+                         int32_t sstride, const float* yb, const float* y0b, int32_t ybstride,
+                         const int32_t* box, const unsigned char* mask, int32_t mask_all,
+                         float net_gain, const float* scale, float gain, int32_t margin,
+                         void* stream);
+/* The near-source patch's correction of one interval: fields[c] += gain *
+ * corr[c] (fused multiply-add) over the box (x0, y0, z0) + [0, bnx) x
+ * [0, bny) x [0, bnz); corr float32 [6][bnz][bny][bnx].                  */
+int conv3d_patch_add(int32_t nx, int32_t ny, int32_t nz, float* const* fields, const float* corr,
+                     int32_t x0, int32_t y0, int32_t z0, int32_t bnx, int32_t bny, int32_t bnz,
+                     float gain, void* stream);
 
 /* The closure's near-source patch (no reference counterpart): a local grid
  * and its 2x-refined twin run side by side, and the coarse box of side^3

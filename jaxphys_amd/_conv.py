@@ -43,6 +43,23 @@ EXPORTS = {
                                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
                                           ctypes.c_void_p]),
+    "conv3d_pack_fields_box": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                              ctypes.c_float, ctypes.c_float, ctypes.c_int32]
+                               + [ctypes.c_int32] * 6 + [ctypes.c_void_p, ctypes.c_void_p]),
+    "conv3d_stencil_f32_set": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                              ctypes.c_void_p]),
+    "conv3d_apply_closure": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                            ctypes.c_float, ctypes.c_void_p, ctypes.c_float,
+                                            ctypes.c_int32, ctypes.c_void_p]),
+    "conv3d_patch_add": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                        ctypes.c_void_p, ctypes.c_void_p]
+                         + [ctypes.c_int32] * 6 + [ctypes.c_float, ctypes.c_void_p]),
     "conv3d_patch_restrict_h": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_int32] * 7
                                 + [ctypes.c_void_p, ctypes.c_void_p]),
     "conv3d_patch_correct": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
@@ -118,6 +135,43 @@ def pack_fields(shape_xyz, fields, eps, scale, eps_shift, eps_scale, out, flags=
                                    stream_handle(stream)), "conv3d_pack_fields")
 
 
+def pack_fields_box(shape_xyz, fields, eps, scale, eps_shift, eps_scale, box, out, flags=0,
+                    stream=None) -> None:
+    """``conv3d_pack_fields_box``: box = (x0, y0, z0, bnx, bny, bnz)."""
+    nx, ny, nz = shape_xyz
+    x0, y0, z0, bx, by, bz = (int(v) for v in box)
+    check(lib().conv3d_pack_fields_box(nx, ny, nz, field_array(fields), ptr(eps), ptr(scale),
+                                       float(eps_shift), float(eps_scale), int(flags), x0, y0,
+                                       z0, bx, by, bz, ptr(out), stream_handle(stream)),
+          "conv3d_pack_fields_box")
+
+
+def apply_closure(shape_xyz, fields, s, sstride, yb, y0b, ybstride, box, mask, mask_all,
+                  net_gain, scale, gain=1.0, margin=0, stream=None) -> None:
+    """``conv3d_apply_closure`` (s, yb/y0b, mask may be None; box = (x0,
+    y0, z0, bnx, bny, bnz) of yb)."""
+    nx, ny, nz = shape_xyz
+    b = (ctypes.c_int32 * 6)(*(int(v) for v in (box if box is not None else (0,) * 6)))
+    check(lib().conv3d_apply_closure(nx, ny, nz, field_array(fields), ptr(s) or None,
+                                     int(sstride), ptr(yb) or None, ptr(y0b) or None,
+                                     int(ybstride), b if yb is not None else None,
+                                     ptr(mask) or None, int(bool(mask_all)), float(net_gain),
+                                     ptr(scale), float(gain), int(margin),
+                                     stream_handle(stream)),
+          "conv3d_apply_closure")
+
+
+def patch_add(shape_xyz, fields, corr, origin_xyz, box_xyz, gain=1.0, stream=None) -> None:
+    """``conv3d_patch_add``: fields += gain * corr (6, bz, by, bx) over the
+    box at ``origin_xyz``."""
+    nx, ny, nz = shape_xyz
+    x0, y0, z0 = origin_xyz
+    bx, by, bz = box_xyz
+    check(lib().conv3d_patch_add(nx, ny, nz, field_array(fields), ptr(corr), int(x0), int(y0),
+                                 int(z0), int(bx), int(by), int(bz), float(gain),
+                                 stream_handle(stream)), "conv3d_patch_add")
+
+
 def apply_correction(shape_xyz, fields, y, ystride, scale, gain=1.0, margin=0,
                      stream=None) -> None:
     nx, ny, nz = shape_xyz
@@ -134,6 +188,14 @@ def stencil_f32(shape_xyz, fields, eps, scale, w, y, ystride, stream=None) -> No
     check(lib().conv3d_stencil_f32(nx, ny, nz, field_array(fields), ptr(eps), ptr(scale), ptr(w),
                                    ptr(y), int(ystride), stream_handle(stream)),
           "conv3d_stencil_f32")
+
+
+def stencil_f32_set(shape_xyz, fields, eps, scale, w, y, ystride, stream=None) -> None:
+    """``conv3d_stencil_f32_set``: y[..][co] = the far-field operator's value."""
+    nx, ny, nz = shape_xyz
+    check(lib().conv3d_stencil_f32_set(nx, ny, nz, field_array(fields), ptr(eps), ptr(scale),
+                                       ptr(w), ptr(y), int(ystride), stream_handle(stream)),
+          "conv3d_stencil_f32_set")
 
 
 def patch_restrict_h(fine_h, fine_shape_xyz, origin_xyz, side, out, stream=None) -> None:

@@ -296,9 +296,108 @@ class ClosureNet:
                 every: int | None = None, use_net: bool = True, use_far: bool = True) -> None:
         """fields += net(fields, eps[, forcing of steps m .. m+every-1])
         plus the linear far-field operator (denormalised) outside the source
-        boxes, plus the linear source model inside them.  ``use_net=False``
-        / ``use_far=False`` leave the net / the far-field operator out (for
-        measuring what each part adds)."""
+        boxes, plus the near-source patch (or the linear source model)
+        inside them.  ``use_net=False`` / ``use_far=False`` leave the net /
+        the far-field operator out (for measuring what each part adds).
+
+        The net runs on its box of the grid only (``conv3d_pack_fields_box``
+        packs that box), the far-field operator writes its own buffer, and
+        one kernel (``conv3d_apply_closure``) forms net - net(0) + far field,
+        clears the source boxes and adds the result to the fields: the same
+        float operations as ``_correct_unfused``, without the full-grid
+        correction tensor and the passes that build it."""
+        if self.forcing and not self.source_patch:
+            self._correct_unfused(solver, gain, m, every, use_net, use_far)
+            return
+        shape = (solver.nx, solver.ny, solver.nz)
+        yb = y0b = box6 = None
+        if use_net and self.net_gain > 0:
+            yb, y0b, box6 = self._net_box(solver, m, every)
+        s = None
+        if use_far and self.far_gain > 0:
+            s = self._buf("far_s", (solver.nx * solver.ny * solver.nz, OUT_STRIDE), torch.float32)
+            _conv.stencil_f32_set(shape, solver.fields, solver.eps_r, self.scale, self.far_w, s,
+                                  OUT_STRIDE)
+        mask, mask_all, patch = None, True, None
+        if self.forcing:
+            box = self._box_terms(solver, shape, (0, 0, 0), every)[0]
+            if not self.net_in_box or s is not None:
+                mask = self._box_mask(solver, shape, every)
+                mask_all = not self.net_in_box
+            patch = getattr(solver, "_closure_patch", None)
+            if patch is None and box.numel() > 0:
+                raise RuntimeError("this closure corrects the source boxes with the "
+                                   "near-source patch: call closure.attach_patch(solver, "
+                                   "grid, n_steps, every) first")
+        _conv.apply_closure(shape, solver.fields, s, OUT_STRIDE, yb, y0b, OUT_STRIDE, box6,
+                            mask, mask_all, self.net_gain if yb is not None else 1.0,
+                            self.scale, gain, self.margin)
+        if patch is not None:
+            patch.apply(solver, m // every, gain)
+
+    def _box_mask(self, solver, shape_xyz, every) -> torch.Tensor:
+        """uint8 per cell: 1 in the source boxes (cached on the solver)."""
+        key = (tuple(shape_xyz), every)
+        cache = getattr(solver, "_closure_box_mask", None)
+        if cache is not None and cache[0] == key:
+            return cache[1]
+        nx, ny, nz = shape_xyz
+        mask = torch.zeros(nx * ny * nz, dtype=torch.uint8, device=self.device)
+        mask[self._box_terms(solver, shape_xyz, (0, 0, 0), every)[0]] = 1
+        solver._closure_box_mask = (key, mask)
+        return mask
+
+    def _net_box(self, solver, m: int | None, every: int | None):
+        """The net's output and zero-state response on its box of the grid
+        (float32 (npos, 8) each, box order) and the box (x0, y0, z0, bnx,
+        bny, bnz)."""
+        sl = self._crop(solver)
+        box6 = (sl[2].start, sl[1].start, sl[0].start, sl[2].stop - sl[2].start,
+                sl[1].stop - sl[1].start, sl[0].stop - sl[0].start)
+        shape_c = box6[3:]
+        nb = shape_c[0] * shape_c[1] * shape_c[2]
+        xc = self._buf("xc", (nb, CIN), torch.bfloat16)
+        _conv.pack_fields_box((solver.nx, solver.ny, solver.nz), solver.fields, solver.eps_r,
+                              self.scale, self.eps_shift, self.eps_scale, box6, xc,
+                              self.pack_flags)
+        if self.forcing:
+            if m is None or every is None:
+                raise ValueError("a closure net with the forcing channel needs the interval")
+            fc = interval_forcing(solver, every)
+            if fc is not None:
+                q = m // every
+                if m % every or q >= fc.feats.shape[0]:
+                    raise ValueError(f"interval start {m} is not a multiple of {every} "
+                                     "inside the source table")
+                sel, cflat = self._forcing_in_box(solver, fc, box6)
+                xc[cflat, 7] = (fc.feats[q, sel, 0] * (1.0 / self.force_scale)).to(torch.bfloat16)
+        yc = self.forward(xc, shape_c)
+        y0 = self.zero_response(solver).view(nb, OUT_STRIDE)
+        return yc, y0, box6
+
+    def _forcing_in_box(self, solver, fc, box6):
+        """(entries of the forcing table inside the box, their box-order
+        flat indices), cached on the solver."""
+        key = (id(fc), tuple(box6))
+        cache = getattr(solver, "_closure_fc_box", None)
+        if cache is not None and cache[0] == key:
+            return cache[1]
+        nx, ny = solver.nx, solver.ny
+        x0, y0, z0, bx, by, bz = box6
+        f = fc.flat
+        x, y, z = f % nx - x0, (f // nx) % ny - y0, f // (nx * ny) - z0
+        inside = (x >= 0) & (x < bx) & (y >= 0) & (y < by) & (z >= 0) & (z < bz)
+        sel = torch.nonzero(inside).flatten()
+        cflat = ((z * by + y) * bx + x)[sel]
+        solver._closure_fc_box = (key, (sel, cflat))
+        return sel, cflat
+
+    def _correct_unfused(self, solver, gain: float = 1.0, m: int | None = None,
+                         every: int | None = None, use_net: bool = True,
+                         use_far: bool = True) -> None:
+        """``correct`` built from the full-grid correction tensor: the net's
+        correction, the far-field operator added into it, the source boxes
+        cleared or given the source model, ``conv3d_apply_correction``."""
         if use_net and self.net_gain > 0:
             y = self.net_correction(solver, m, every)
             if self.net_gain != 1.0:
@@ -819,11 +918,15 @@ class NearSourcePatch:
                                     for v in (pos[2], pos[1], pos[0])))
 
     def apply(self, solver, q: int, gain: float = 1.0) -> None:
-        """fields += gain * correction of interval q in every source box."""
+        """fields += gain * correction of interval q in every source box (one
+        launch per box: ``conv3d_patch_add``)."""
+        shape = (solver.nx, solver.ny, solver.nz)
         for box, c in zip(self.boxes, self.corr):
             if q < c.shape[0]:
-                for k in range(6):
-                    solver.fields[k][box].add_(c[q, k], alpha=gain)
+                _conv.patch_add(shape, solver.fields, c[q],
+                                (box[2].start, box[1].start, box[0].start),
+                                (box[2].stop - box[2].start, box[1].stop - box[1].start,
+                                 box[0].stop - box[0].start), gain)
 
 
 def attach_patch(solver, grid: EMGrid3D, n_coarse: int, every: int,

@@ -1250,13 +1250,18 @@ __global__ void flip_weights_kernel(const u16* w, int cout, int cout_pad, int ci
   wt[i] = v;
 }
 
-__global__ void pack_fields_kernel(int nx, int ny, int nz, const float* f0, const float* f1,
+// the cells of the box (bx0, by0, bz0) + [0, bnx) x [0, bny) x [0, bnz) of
+// the grid, out in box order (the whole grid: the box (0, 0, 0, nx, ny, nz))
+__global__ void pack_fields_kernel(int nx, int ny, int bx0, int by0, int bz0, int bnx, int bny,
+                                   int bnz, const float* f0, const float* f1,
                                    const float* f2, const float* f3, const float* f4,
                                    const float* f5, const float* eps, const float* scale,
                                    float eshift, float escale, int flags, u16* out) {
-  const long long n = (long long)nx * ny * nz;
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  const long long nb = (long long)bnx * bny * bnz;
+  const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nb) return;
+  const int lx = (int)(j % bnx), ly = (int)((j / bnx) % bny), lz = (int)(j / ((long long)bnx * bny));
+  const long long i = ((long long)(bz0 + lz) * ny + (by0 + ly)) * nx + (bx0 + lx);
   const float* f[6] = {f0, f1, f2, f3, f4, f5};
   u16 v[8];
 #pragma unroll
@@ -1268,7 +1273,7 @@ __global__ void pack_fields_kernel(int nx, int ny, int nz, const float* f0, cons
   v[7] = 0;
   uint4 pk;
   __builtin_memcpy(&pk, v, 16);
-  *reinterpret_cast<uint4*>(out + i * 8) = pk;
+  *reinterpret_cast<uint4*>(out + j * 8) = pk;
 }
 
 __global__ void apply_correction_kernel(int nx, int ny, int nz, float* f0, float* f1, float* f2,
@@ -1286,6 +1291,55 @@ __global__ void apply_correction_kernel(int nx, int ny, int nz, float* f0, float
   for (int c = 0; c < 6; ++c) f[c][i] += gain * scale[c] * y[i * ys + c];
 }
 
+// One closure correction in one pass (the same float operations, in the
+// same order, as building y = net_gain (net - net(0)), y += far field,
+// y[source boxes] = 0 and apply_correction): for the cells at least
+// `margin` from every face,
+//   v = net_gain * (yb - y0b) (the net's output and zero-state response on
+//       the box (bx0, by0, bz0, bnx, bny, bnz), which holds those cells;
+//       0 without yb) + s (the far-field operator's value; 0 without s),
+//   with s dropped (mask_all == 0) or v = 0 (mask_all == 1) where mask != 0,
+//   fields[c] += gain * scale[c] * v.
+__global__ void apply_closure_kernel(int nx, int ny, int nz, float* f0, float* f1, float* f2,
+                                     float* f3, float* f4, float* f5, const float* s, int ss,
+                                     const float* yb, const float* y0b, int ybs, int bx0,
+                                     int by0, int bz0, int bnx, int bny,
+                                     const unsigned char* mask, int mask_all, float ng,
+                                     const float* scale, float gain, int margin) {
+  const long long n = (long long)nx * ny * nz;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int x = (int)(i % nx), yy = (int)((i / nx) % ny), z = (int)(i / ((long long)nx * ny));
+  if (x < margin || x >= nx - margin || yy < margin || yy >= ny - margin || z < margin ||
+      z >= nz - margin)
+    return;
+  const bool mk = mask && mask[i];
+  const long long jb = yb ? ((long long)(z - bz0) * bny + (yy - by0)) * bnx + (x - bx0) : 0;
+  float* f[6] = {f0, f1, f2, f3, f4, f5};
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    float v = yb ? (yb[jb * ybs + c] - y0b[jb * ybs + c]) * ng : 0.f;
+    if (s) v += (mk && !mask_all) ? 0.f : s[i * ss + c];
+    if (mk && mask_all) v = 0.f;
+    f[c][i] += gain * scale[c] * v;
+  }
+}
+
+// fields[c] += gain * corr[c] over the box (x0, y0, z0) + [0, bnx) x [0, bny)
+// x [0, bnz); corr is [6][bnz][bny][bnx]
+__global__ void patch_add_kernel(int nx, int ny, float* f0, float* f1, float* f2, float* f3,
+                                 float* f4, float* f5, const float* corr, int x0, int y0, int z0,
+                                 int bnx, int bny, int bnz, float gain) {
+  const int nb = bnx * bny * bnz;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nb) return;
+  const int lx = j % bnx, ly = (j / bnx) % bny, lz = j / (bnx * bny);
+  const long long i = ((long long)(z0 + lz) * ny + (y0 + ly)) * nx + (x0 + lx);
+  float* f[6] = {f0, f1, f2, f3, f4, f5};
+#pragma unroll
+  for (int c = 0; c < 6; ++c) f[c][i] = __builtin_fmaf(gain, corr[(long long)c * nb + j], f[c][i]);
+}
+
 // The closure's linear far-field operator, float32:
 // y[i][co] += sum_{t, ci} (w[t][ci][0][co] + (eps[i] - 1) w[t][ci][1][co])
 //             * f_ci(i + off(t)) / scale[ci]
@@ -1296,6 +1350,7 @@ __global__ void apply_correction_kernel(int nx, int ny, int nz, float* f0, float
 // weight set (12 floats, contiguous, uniform across the workgroup: scalar
 // loads) serves two cells.  Per field the taps are summed first and scaled
 // once by 1 / scale[ci].
+template <bool SET>
 __global__ __launch_bounds__(256) void stencil_f32_kernel(
     int nx, int ny, int nz, const float* __restrict__ f0, const float* __restrict__ f1,
     const float* __restrict__ f2, const float* __restrict__ f3, const float* __restrict__ f4,
@@ -1357,12 +1412,20 @@ __global__ __launch_bounds__(256) void stencil_f32_kernel(
   }
   const float e0 = eps[i0] - 1.0f;
 #pragma unroll
-  for (int co = 0; co < 6; ++co) y[i0 * ys + co] += a0[co] + e0 * b0[co];
+  for (int co = 0; co < 6; ++co) {
+    const float v = a0[co] + e0 * b0[co];
+    if (SET) y[i0 * ys + co] = v;
+    else y[i0 * ys + co] += v;
+  }
   if (two) {
     const long long i1 = i0 + plane;
     const float e1 = eps[i1] - 1.0f;
 #pragma unroll
-    for (int co = 0; co < 6; ++co) y[i1 * ys + co] += a1[co] + e1 * b1[co];
+    for (int co = 0; co < 6; ++co) {
+      const float v = a1[co] + e1 * b1[co];
+      if (SET) y[i1 * ys + co] = v;
+      else y[i1 * ys + co] += v;
+    }
   }
 }
 
@@ -1685,11 +1748,70 @@ int conv3d_flip_weights(const ConvDesc* d, const void* w, int32_t dy_cpad, void*
 int conv3d_pack_fields(int32_t nx, int32_t ny, int32_t nz, float* const* fields, const float* eps,
                        const float* scale, float eps_shift, float eps_scale, int32_t flags,
                        void* out, void* stream) {
+  return conv3d_pack_fields_box(nx, ny, nz, fields, eps, scale, eps_shift, eps_scale, flags, 0, 0,
+                                0, nx, ny, nz, out, stream);
+}
+
+int conv3d_pack_fields_box(int32_t nx, int32_t ny, int32_t nz, float* const* fields,
+                           const float* eps, const float* scale, float eps_shift, float eps_scale,
+                           int32_t flags, int32_t x0, int32_t y0, int32_t z0, int32_t bnx,
+                           int32_t bny, int32_t bnz, void* out, void* stream) {
   if (!fields || !eps || !scale || !out) return fail(-1, "null argument");
+  for (int c = 0; c < 6; ++c)
+    if (!fields[c]) return fail(-1, "null field");
+  if (bnx <= 0 || bny <= 0 || bnz <= 0 || x0 < 0 || y0 < 0 || z0 < 0 || x0 + bnx > nx ||
+      y0 + bny > ny || z0 + bnz > nz)
+    return fail(-1, "box outside the grid");
+  const long long nb = (long long)bnx * bny * bnz;
+  hipLaunchKernelGGL(pack_fields_kernel, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, nx, ny, x0, y0, z0, bnx, bny, bnz, fields[0], fields[1],
+                     fields[2], fields[3], fields[4], fields[5], eps, scale, eps_shift, eps_scale,
+                     flags, (u16*)out);
+  CONV_CHECK(hipGetLastError());
+  return 0;
+}
+
+int conv3d_apply_closure(int32_t nx, int32_t ny, int32_t nz, float* const* fields, const float* s,
+                         int32_t sstride, const float* yb, const float* y0b, int32_t ybstride,
+                         const int32_t* box, const unsigned char* mask, int32_t mask_all,
+                         float net_gain, const float* scale, float gain, int32_t margin,
+                         void* stream) {
+  if (!fields || !scale || margin < 0) return fail(-1, "bad argument");
+  for (int c = 0; c < 6; ++c)
+    if (!fields[c]) return fail(-1, "null field");
+  if (s && sstride < 6) return fail(-1, "far-field stride below 6");
+  int b[6] = {0, 0, 0, 0, 0, 0};
+  if (yb) {
+    if (!y0b || !box || ybstride < 6) return fail(-1, "net output without its box");
+    for (int k = 0; k < 6; ++k) b[k] = box[k];
+    // the box must hold every cell `margin` or more from the faces
+    const int lo[3] = {margin, margin, margin}, hi[3] = {nx - margin, ny - margin, nz - margin};
+    for (int k = 0; k < 3; ++k)
+      if (hi[k] > lo[k] && (b[k] > lo[k] || b[k] + b[k + 3] < hi[k]))
+        return fail(-1, "net box does not cover the corrected cells");
+  }
   const long long n = (long long)nx * ny * nz;
-  hipLaunchKernelGGL(pack_fields_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+  hipLaunchKernelGGL(apply_closure_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                      (hipStream_t)stream, nx, ny, nz, fields[0], fields[1], fields[2], fields[3],
-                     fields[4], fields[5], eps, scale, eps_shift, eps_scale, flags, (u16*)out);
+                     fields[4], fields[5], s, sstride, yb, y0b, ybstride, b[0], b[1], b[2], b[3],
+                     b[4], mask, mask_all, net_gain, scale, gain, margin);
+  CONV_CHECK(hipGetLastError());
+  return 0;
+}
+
+int conv3d_patch_add(int32_t nx, int32_t ny, int32_t nz, float* const* fields, const float* corr,
+                     int32_t x0, int32_t y0, int32_t z0, int32_t bnx, int32_t bny, int32_t bnz,
+                     float gain, void* stream) {
+  if (!fields || !corr) return fail(-1, "bad argument");
+  for (int c = 0; c < 6; ++c)
+    if (!fields[c]) return fail(-1, "null field");
+  if (bnx <= 0 || bny <= 0 || bnz <= 0 || x0 < 0 || y0 < 0 || z0 < 0 || x0 + bnx > nx ||
+      y0 + bny > ny || z0 + bnz > nz)
+    return fail(-1, "box outside the grid");
+  const int nb = bnx * bny * bnz;
+  hipLaunchKernelGGL(patch_add_kernel, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, nx, ny, fields[0], fields[1], fields[2], fields[3],
+                     fields[4], fields[5], corr, x0, y0, z0, bnx, bny, bnz, gain);
   CONV_CHECK(hipGetLastError());
   return 0;
 }
@@ -1714,8 +1836,23 @@ int conv3d_stencil_f32(int32_t nx, int32_t ny, int32_t nz, float* const* fields,
   for (int c = 0; c < 6; ++c)
     if (!fields[c]) return fail(-1, "null field");
   const long long npairs = (long long)nx * ny * ((nz + 1) / 2);
-  hipLaunchKernelGGL(stencil_f32_kernel, dim3((unsigned)((npairs + 255) / 256)), dim3(256), 0,
-                     (hipStream_t)stream, nx, ny, nz, fields[0], fields[1], fields[2], fields[3],
+  hipLaunchKernelGGL(stencil_f32_kernel<false>, dim3((unsigned)((npairs + 255) / 256)), dim3(256),
+                     0, (hipStream_t)stream, nx, ny, nz, fields[0], fields[1], fields[2], fields[3],
+                     fields[4], fields[5], eps, scale, w, y, ystride);
+  CONV_CHECK(hipGetLastError());
+  return 0;
+}
+
+int conv3d_stencil_f32_set(int32_t nx, int32_t ny, int32_t nz, float* const* fields,
+                           const float* eps, const float* scale, const float* w, float* y,
+                           int32_t ystride, void* stream) {
+  if (!fields || !eps || !scale || !w || !y || ystride < 6 || nx <= 0 || ny <= 0 || nz <= 0)
+    return fail(-1, "bad argument");
+  for (int c = 0; c < 6; ++c)
+    if (!fields[c]) return fail(-1, "null field");
+  const long long npairs = (long long)nx * ny * ((nz + 1) / 2);
+  hipLaunchKernelGGL(stencil_f32_kernel<true>, dim3((unsigned)((npairs + 255) / 256)), dim3(256),
+                     0, (hipStream_t)stream, nx, ny, nz, fields[0], fields[1], fields[2], fields[3],
                      fields[4], fields[5], eps, scale, w, y, ystride);
   CONV_CHECK(hipGetLastError());
   return 0;

@@ -335,3 +335,47 @@ def test_train_rollout_runs_and_updates_the_net(gpu):
     assert float(net.params["w3"].abs().max()) > 0.0
     assert torch.equal(net.far_w, far_w)
     assert net.net_gain in (0.0, 0.25, 0.5, 1.0)
+
+
+@pytest.mark.parametrize("net_in_box,net_gain,use_far", [(False, 1.0, True), (True, 0.5, True),
+                                                         (False, 1.0, False)])
+def test_fused_correction_equals_the_unfused_one(gpu, net_in_box, net_gain, use_far):
+    """``correct`` (the net on its box, the far-field operator into its own
+    buffer, one apply kernel) changes the fields exactly as
+    ``_correct_unfused`` (the full-grid correction tensor built by separate
+    passes), interval after interval of a corrected rollout with the
+    near-source patch."""
+    net = CL.ClosureNet.load(CL.__file__.replace("closure.py", "data/closure_c5.safetensors"),
+                             gpu)
+    assert net.forcing and net.source_patch
+    net.net_in_box, net.net_gain = net_in_box, net_gain
+    prob = CL.ClosureProblem(7, 96, npml_fine=16, device=gpu)
+    every, n_coarse = 10, 40
+    a = CL.coarse_solver(prob, net, n_coarse, every)
+    b = CL.coarse_solver(prob, net, n_coarse, every, patch=a._closure_patch)
+    for m in range(0, n_coarse, every):
+        a.run(m, every)
+        b.run(m, every)
+        net.correct(a, 1.0, m, every, True, use_far)
+        net._correct_unfused(b, 1.0, m, every, True, use_far)
+        for fa, fb in zip(a.fields, b.fields):
+            assert torch.equal(fa, fb), m
+    assert float(a.fields[2].abs().max()) > 0.0
+
+
+def test_patch_add_equals_torch_add(gpu):
+    """``conv3d_patch_add`` (one launch for the six fields of a box) gives
+    the bits of torch's ``add_(corr, alpha=gain)`` on each field's box."""
+    from jaxphys_amd import _conv
+    g_ = torch.Generator(device="cpu").manual_seed(11)
+    n = (20, 18, 16)
+    fields = [torch.randn(n[2], n[1], n[0], generator=g_).to(gpu) for _ in range(6)]
+    ref = [f.clone() for f in fields]
+    corr = torch.randn(6, 5, 6, 7, generator=g_).to(gpu)
+    org, gain = (3, 4, 5), 0.7
+    _conv.patch_add(n, fields, corr, org, (7, 6, 5), gain)
+    box = (slice(5, 10), slice(4, 10), slice(3, 10))
+    for k in range(6):
+        ref[k][box].add_(corr[k], alpha=gain)
+    for f, r in zip(fields, ref):
+        assert torch.equal(f, r)
