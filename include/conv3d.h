@@ -98,10 +98,12 @@ int conv3d_apply_correction(int32_t nx, int32_t ny, int32_t nz, float* const* fi
 int conv3d_stencil_f32(int32_t nx, int32_t ny, int32_t nz, float* const* fields,
                        const float* eps, const float* scale, const float* w, float* y,
                        int32_t ystride, void* stream);
-/* conv3d_stencil_f32 with y[i][co] = (not +=) the operator's value.    */
+/* conv3d_stencil_f32 with y[i][co] = (not +=) the operator's value, for
+ * the cells of box (x0, y0, z0, bnx, bny, bnz; NULL: the whole grid) only
+ * (y indexed on the whole grid; the other cells untouched).               */
 int conv3d_stencil_f32_set(int32_t nx, int32_t ny, int32_t nz, float* const* fields,
                            const float* eps, const float* scale, const float* w, float* y,
-                           int32_t ystride, void* stream);
+                           int32_t ystride, const int32_t* box, void* stream);
 
 /* One closure correction in one pass (no reference counterpart; the same
  * float operations as y = net_gain * (yb - y0b), y += s, y[mask] = 0 and

@@ -50,7 +50,7 @@ EXPORTS = {
     "conv3d_stencil_f32_set": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                               ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                               ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
-                                              ctypes.c_void_p]),
+                                              ctypes.c_void_p, ctypes.c_void_p]),
     "conv3d_apply_closure": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
@@ -190,11 +190,14 @@ def stencil_f32(shape_xyz, fields, eps, scale, w, y, ystride, stream=None) -> No
           "conv3d_stencil_f32")
 
 
-def stencil_f32_set(shape_xyz, fields, eps, scale, w, y, ystride, stream=None) -> None:
-    """``conv3d_stencil_f32_set``: y[..][co] = the far-field operator's value."""
+def stencil_f32_set(shape_xyz, fields, eps, scale, w, y, ystride, box=None,
+                    stream=None) -> None:
+    """``conv3d_stencil_f32_set``: y[..][co] = the far-field operator's value
+    on the cells of ``box`` (x0, y0, z0, bnx, bny, bnz; None: all)."""
     nx, ny, nz = shape_xyz
+    b = (ctypes.c_int32 * 6)(*(int(v) for v in box)) if box is not None else None
     check(lib().conv3d_stencil_f32_set(nx, ny, nz, field_array(fields), ptr(eps), ptr(scale),
-                                       ptr(w), ptr(y), int(ystride), stream_handle(stream)),
+                                       ptr(w), ptr(y), int(ystride), b, stream_handle(stream)),
           "conv3d_stencil_f32_set")
 
 

@@ -315,9 +315,12 @@ class ClosureNet:
             yb, y0b, box6 = self._net_box(solver, m, every)
         s = None
         if use_far and self.far_gain > 0:
+            # on the corrected cells only (``margin`` or more from the faces)
+            mg = self.margin
+            inner = (mg, mg, mg) + tuple(max(1, n - 2 * mg) for n in shape)
             s = self._buf("far_s", (solver.nx * solver.ny * solver.nz, OUT_STRIDE), torch.float32)
             _conv.stencil_f32_set(shape, solver.fields, solver.eps_r, self.scale, self.far_w, s,
-                                  OUT_STRIDE)
+                                  OUT_STRIDE, inner if min(n - 2 * mg for n in shape) > 0 else None)
         mask, mask_all, patch = None, True, None
         if self.forcing:
             box = self._box_terms(solver, shape, (0, 0, 0), every)[0]

@@ -1350,19 +1350,22 @@ __global__ void patch_add_kernel(int nx, int ny, float* f0, float* f1, float* f2
 // weight set (12 floats, contiguous, uniform across the workgroup: scalar
 // loads) serves two cells.  Per field the taps are summed first and scaled
 // once by 1 / scale[ci].
+// (SET: y = the value instead of y += it; the cells of the box (bx0, by0,
+// bz0) + [0, bnx) x [0, bny) x [0, bnz) only, y indexed on the whole grid)
 template <bool SET>
 __global__ __launch_bounds__(256) void stencil_f32_kernel(
     int nx, int ny, int nz, const float* __restrict__ f0, const float* __restrict__ f1,
     const float* __restrict__ f2, const float* __restrict__ f3, const float* __restrict__ f4,
     const float* __restrict__ f5, const float* __restrict__ eps,
     const float* __restrict__ scale, const float* __restrict__ w, float* __restrict__ y,
-    int ys) {
-  const int nzp = (nz + 1) / 2;
-  const long long npairs = (long long)nx * ny * nzp;
+    int ys, int bx0, int by0, int bz0, int bnx, int bny, int bnz) {
+  const int nzp = (bnz + 1) / 2;
+  const long long npairs = (long long)bnx * bny * nzp;
   const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= npairs) return;
-  const int x = (int)(p % nx), yy = (int)((p / nx) % ny), z0 = 2 * (int)(p / ((long long)nx * ny));
-  const bool two = z0 + 1 < nz;
+  const int x = bx0 + (int)(p % bnx), yy = by0 + (int)((p / bnx) % bny);
+  const int z0 = bz0 + 2 * (int)(p / ((long long)bnx * bny));
+  const bool two = z0 + 1 < bz0 + bnz;
   const long long plane = (long long)nx * ny;
   const long long i0 = (long long)z0 * plane + (long long)yy * nx + x;
   const float* f[6] = {f0, f1, f2, f3, f4, f5};
@@ -1838,22 +1841,30 @@ int conv3d_stencil_f32(int32_t nx, int32_t ny, int32_t nz, float* const* fields,
   const long long npairs = (long long)nx * ny * ((nz + 1) / 2);
   hipLaunchKernelGGL(stencil_f32_kernel<false>, dim3((unsigned)((npairs + 255) / 256)), dim3(256),
                      0, (hipStream_t)stream, nx, ny, nz, fields[0], fields[1], fields[2], fields[3],
-                     fields[4], fields[5], eps, scale, w, y, ystride);
+                     fields[4], fields[5], eps, scale, w, y, ystride, 0, 0, 0, nx, ny, nz);
   CONV_CHECK(hipGetLastError());
   return 0;
 }
 
 int conv3d_stencil_f32_set(int32_t nx, int32_t ny, int32_t nz, float* const* fields,
                            const float* eps, const float* scale, const float* w, float* y,
-                           int32_t ystride, void* stream) {
+                           int32_t ystride, const int32_t* box, void* stream) {
   if (!fields || !eps || !scale || !w || !y || ystride < 6 || nx <= 0 || ny <= 0 || nz <= 0)
     return fail(-1, "bad argument");
   for (int c = 0; c < 6; ++c)
     if (!fields[c]) return fail(-1, "null field");
-  const long long npairs = (long long)nx * ny * ((nz + 1) / 2);
+  int b[6] = {0, 0, 0, nx, ny, nz};
+  if (box) {
+    for (int k = 0; k < 6; ++k) b[k] = box[k];
+    if (b[3] <= 0 || b[4] <= 0 || b[5] <= 0 || b[0] < 0 || b[1] < 0 || b[2] < 0 ||
+        b[0] + b[3] > nx || b[1] + b[4] > ny || b[2] + b[5] > nz)
+      return fail(-1, "box outside the grid");
+  }
+  const long long npairs = (long long)b[3] * b[4] * ((b[5] + 1) / 2);
   hipLaunchKernelGGL(stencil_f32_kernel<true>, dim3((unsigned)((npairs + 255) / 256)), dim3(256),
                      0, (hipStream_t)stream, nx, ny, nz, fields[0], fields[1], fields[2], fields[3],
-                     fields[4], fields[5], eps, scale, w, y, ystride);
+                     fields[4], fields[5], eps, scale, w, y, ystride, b[0], b[1], b[2], b[3], b[4],
+                     b[5]);
   CONV_CHECK(hipGetLastError());
   return 0;
 }
