@@ -242,9 +242,11 @@ class YeeSolver:
     # Fused step only when the slab-free interior (the box of the fused
     # interior launch) holds at least this share of the cells: its CPML shell
     # launches cost about twice the two-pass kernels per cell (on MI355X:
-    # 512^3 with 16-cell slabs, 77 % interior: fused 1.94 ms vs 2.31 ms;
-    # 160^3 with 12-cell slabs, 42 %: fused 0.123 ms vs 0.075 ms).
-    FUSED_MIN_INTERIOR = 0.6
+    # 512^3 with 16-cell slabs, 77 % interior: fused 1.78 ms vs 2.31 ms;
+    # 256^3 with 16-cell slabs, 57 %: fused 0.349 ms vs 0.369 ms;
+    # 160^3 with 12-cell slabs, 42 %: fused 0.123 ms vs 0.075 ms;
+    # 128^3 with 8-cell slabs, 38 %: fused 0.086 ms vs 0.043 ms).
+    FUSED_MIN_INTERIOR = 0.55
 
     def interior_fraction(self) -> float:
         """Share of the cells in the fused step's slab-free interior box."""
