@@ -915,15 +915,25 @@ __global__ __launch_bounds__(1024) void conv_slab_kernel(ConvArgs a) {
 // input plane's rows y0-1 .. y0+16 and columns x0-1 .. x0+64 (18 x 66
 // positions of 16 bytes, 19 KiB) fill one of four LDS buffers: planes
 // zz-1, zz, zz+1 are read while plane zz+2 loads (direct-to-LDS), so every
-// input plane is fetched once per block.  K step (dz, dy) as in the row
-// kernel: k group g of the B operand is tap dx = g - 1, read straight from
-// the slab at x + dx (no shifts); group 3 meets zero weights.  Same K order
-// as the row kernel, so the same bits.
+// input plane is fetched once per block.  K steps of four taps (k group g
+// of the B operand one tap, read straight from the slab at the tap's row
+// and x offset, no shifts): two per input plane for its in-plane taps 0-7,
+// and one for the last in-plane tap of the three planes (lane group g reads
+// plane g; group 3 meets zero weights) -- 7 steps where one per (dz, dy)
+// pair with a zero fourth group would take 9.  (The row kernel groups the
+// taps by (dz, dy): the float32 sums may differ in the last bit.)
 // ---------------------------------------------------------------------------
 constexpr int kRollRows = 18, kRollPos = 66;
 constexpr int kRollSlots = kRollRows * kRollPos;             // 16-byte slots
 constexpr int kRollBytes = kRollSlots * 16;                  // 19008
 constexpr int kRollLoads = (kRollSlots + 63) / 64;           // 19 wave loads per plane
+
+// tap of K step s, k group g in the 8-channel kernel: steps 0-5 take the 9
+// taps of plane s / 2 four at a time (in-plane taps 0-3, 4-7), step 6 the
+// last in-plane tap of the three planes; 27 (no tap) for the spare group
+__host__ __device__ constexpr int roll8_tap(int s, int g) {
+  return s < 6 ? (s >> 1) * 9 + 4 * (s & 1) + g : (g < 3 ? g * 9 + 8 : 27);
+}
 
 template <int EPI, int MT>
 __global__ __launch_bounds__(1024) void conv_roll8_kernel(ConvArgs a) {
@@ -931,22 +941,22 @@ __global__ __launch_bounds__(1024) void conv_roll8_kernel(ConvArgs a) {
   // separate LDS objects, so that the compiler can tell each buffer being
   // read from the one being loaded; weights as [pair][k group g][output
   // channel r] 16-byte slots (a row group reads 16 consecutive slots)
-  __shared__ __attribute__((aligned(16))) uint4 wsh0[9 * 4 * 16];
-  __shared__ __attribute__((aligned(16))) uint4 wsh1[MT == 2 ? 9 * 4 * 16 : 1];
+  __shared__ __attribute__((aligned(16))) uint4 wsh0[7 * 4 * 16];
+  __shared__ __attribute__((aligned(16))) uint4 wsh1[MT == 2 ? 7 * 4 * 16 : 1];
   __shared__ u16 lut[EPI == CONV_EPI_GELU ? 2 * kRowLutS : 2];
   __shared__ __attribute__((aligned(16))) unsigned char sl0[kRollBytes];
   __shared__ __attribute__((aligned(16))) unsigned char sl1[kRollBytes];
   __shared__ __attribute__((aligned(16))) unsigned char sl2[kRollBytes];
   __shared__ __attribute__((aligned(16))) unsigned char sl3[kRollBytes];
   {
-    // the 27 taps x 8 channels of each (dz, dy) pair over 32 K slots, the
-    // last 8 zero
-    constexpr int C16 = 9 * 4;
+    // K step s, k group g -> tap roll8_tap(s, g) (8 channels); 27 taps in
+    // 7 steps, the last slot zero
+    constexpr int C16 = 7 * 4;
     for (int q = threadIdx.x; q < MT * 16 * C16; q += 1024) {
       const int row = q / C16, c = q - row * C16;
-      const int p = c >> 2, gg = c & 3;
+      const int tap = roll8_tap(c >> 2, c & 3);
       uint4 v = {0u, 0u, 0u, 0u};
-      if (gg < 3) v = *reinterpret_cast<const uint4*>(a.w + (long long)row * a.kpad + (3 * p + gg) * 8);
+      if (tap < 27) v = *reinterpret_cast<const uint4*>(a.w + (long long)row * a.kpad + tap * 8);
       (row < 16 ? wsh0 : wsh1)[c * 16 + (row & 15)] = v;
     }
     if (EPI == CONV_EPI_GELU) {
@@ -999,11 +1009,16 @@ __global__ __launch_bounds__(1024) void conv_roll8_kernel(ConvArgs a) {
   const bool allreal = a.cout >= MT * 16;
   const unsigned esz = EPI == CONV_EPI_LINEAR_F32 ? 4u : 2u;
   const unsigned obytes = (unsigned)nx * ycp * esz;
-  // this lane's B fragment of tile n, tap dx = g - 1 (group 3: dx = 0), in
-  // slab row wv + 1 + dy: position 16 n + r + 1 + dx
-  unsigned boff[NT];
+  // this lane's byte offset in a plane buffer for tile 0 of the K steps
+  // that read one plane (k = s % 2: in-plane tap 4 k + g, (dy, dx) =
+  // (j / 3 - 1, j % 3 - 1)) and of the step that reads tap (1, 1) of all
+  // three; tile n adds 256 n
+  unsigned boff[3];
 #pragma unroll
-  for (int n = 0; n < NT; ++n) boff[n] = (unsigned)((16 * n + r + (g < 3 ? g : 1)) * 16);
+  for (int k = 0; k < 3; ++k) {
+    const int j = k < 2 ? 4 * k + g : 8;
+    boff[k] = (unsigned)((wv + j / 3) * (kRollPos * 16) + (r + j % 3) * 16);
+  }
   const int nplanes = z1 - z0;
   f32x4 acc[MT][NT];
   // fragments of one (dz, dy) pair; the next pair's six LDS reads are issued
@@ -1019,11 +1034,22 @@ __global__ __launch_bounds__(1024) void conv_roll8_kernel(ConvArgs a) {
     for (int m = 0; m < MT; ++m)
 #pragma unroll
       for (int n = 0; n < NT; ++n) acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // steps 0-5 read plane s / 2 (zz - 1, zz, zz + 1); step 6 reads plane g
+    // in lane group g (group 3 meets zero weights: any valid position)
     auto load = [&](int p, Frag& f) __attribute__((always_inline)) {
-      const unsigned char* sb = p < 3 ? pm : (p < 6 ? p0 : pp);
-      const unsigned char* rowp = sb + (wv + p % 3) * (kRollPos * 16);
+      if (p < 6) {
+        const unsigned char* sb = (p >> 1) == 0 ? pm : ((p >> 1) == 1 ? p0 : pp);
 #pragma unroll
-      for (int n = 0; n < NT; ++n) f.b[n] = *reinterpret_cast<const bf16x8*>(rowp + boff[n]);
+        for (int n = 0; n < NT; ++n)
+          f.b[n] = *reinterpret_cast<const bf16x8*>(sb + boff[p & 1] + 256 * n);
+      } else {
+#pragma unroll
+        for (int n = 0; n < NT; ++n) {
+          if (g == 0) f.b[n] = *reinterpret_cast<const bf16x8*>(pm + boff[2] + 256 * n);
+          else if (g == 1) f.b[n] = *reinterpret_cast<const bf16x8*>(p0 + boff[2] + 256 * n);
+          else f.b[n] = *reinterpret_cast<const bf16x8*>(pp + boff[2] + 256 * n);
+        }
+      }
 #pragma unroll
       for (int m = 0; m < MT; ++m)
         f.af[m] = __builtin_bit_cast(bf16x8, (m == 0 ? wsh0 : wsh1)[(p * 4 + g) * 16 + r]);
@@ -1031,8 +1057,8 @@ __global__ __launch_bounds__(1024) void conv_roll8_kernel(ConvArgs a) {
     Frag f[2];
     load(0, f[0]);
 #pragma unroll
-    for (int p = 0; p < 9; ++p) {
-      if (p + 1 < 9) load(p + 1, f[(p + 1) & 1]);
+    for (int p = 0; p < 7; ++p) {
+      if (p + 1 < 7) load(p + 1, f[(p + 1) & 1]);
       __builtin_amdgcn_sched_barrier(0);
       const Frag& c = f[p & 1];
 #pragma unroll
