@@ -734,6 +734,17 @@ __global__ __launch_bounds__(1024) void conv_row_kernel(ConvArgs a) {
 //    that idle slab step (plane zz+1's at the next step's first slab).
 // Taps accumulate in the order (dz, dy, dx) of conv_fwd_kernel, so the
 // results are the same bits.
+// PX (float32 outputs, at most 8 channels: the closure net's last layer)
+// stacks two taps in the 16 output rows of one MFMA instead of padding 6
+// channels to 16: per (dz, dy) the fragment of input column x meets rows
+// 0-7 = tap dx = -1 (a term of output x+1) and rows 8-15 = tap dx = +1 (of
+// output x-1) in one MFMA, tap dx = 0 in a second, and the edge fragment
+// (x0 - 1, x0 + 64) the first once more: 9 MFMAs per (dz, dy) and wave
+// instead of 12, and no shifted fragments.  The epilogue moves the x-1
+// parts from lane groups 2-3 to 0-1 (ds_bpermute), shifts the x+1 / x-1
+// parts one column (DPP, the edge columns filling lanes 0 and 15) and adds
+// the three sums: float32 results within rounding of the taps' sum, not the
+// bits of the (dz, dy, dx) order.
 // ---------------------------------------------------------------------------
 constexpr int kSlabRows = 10, kSlabRowSlots = 272;          // 66 positions (+ 16 unused slots)
 constexpr int kSlabSlots = kSlabRows * kSlabRowSlots;        // 16-byte slots
@@ -742,18 +753,33 @@ constexpr int kSlabLoads = (kSlabSlots + 63) / 64;           // 43 wave loads pe
 
 __device__ __forceinline__ int slab_slot(int pos, int g) { return 17 * (4 * (pos & 3) + g) + (pos >> 2); }
 
-template <int EPI, int MT>
+template <int EPI, int MT, bool PX = false>
 __global__ __launch_bounds__(1024) void conv_slab_kernel(ConvArgs a) {
   constexpr int NT = 4, CIN = 32;
+  static_assert(!PX || (MT == 1 && EPI == CONV_EPI_LINEAR_F32), "paired taps: f32, one M tile");
   // separate LDS objects (one per weight M tile, one per slab buffer), so
   // that the compiler can tell the slab being read and the weights from the
   // slab being loaded and does not wait for those loads
-  __shared__ __attribute__((aligned(16))) uint4 wsh0[27 * 4 * 16];
+  __shared__ __attribute__((aligned(16))) uint4 wsh0[PX ? 9 * 2 * 4 * 16 : 27 * 4 * 16];
   __shared__ __attribute__((aligned(16))) uint4 wsh1[MT == 2 ? 27 * 4 * 16 : 1];
   __shared__ u16 lut[EPI == CONV_EPI_GELU ? 2 * kRowLutS : 2];
   __shared__ __attribute__((aligned(16))) unsigned char slab0[kSlabBytes];
   __shared__ __attribute__((aligned(16))) unsigned char slab1[kSlabBytes];
-  {
+  if constexpr (PX) {
+    // slot ((2 p + h) 4 + g) 16 + row for (dz, dy) pair p = 3 (dz + 1) +
+    // dy: h = 0 holds tap dx = -1 in rows 0-7 (output channel row) and
+    // dx = +1 in rows 8-15 (channel row - 8), h = 1 tap dx = 0 in rows 0-7
+    // and zeros; k chunk g = channels 8 g .. 8 g + 7
+    for (int q = threadIdx.x; q < 9 * 2 * 4 * 16; q += 1024) {
+      const int row = q & 15, c = q >> 4;
+      const int gc = c & 3, h = (c >> 2) & 1, p = c >> 3;
+      const int co = row & 7, dx = h ? 1 : (row < 8 ? 0 : 2);
+      const bool ok = co < a.cout && !(h && row >= 8);
+      wsh0[q] = ok ? *reinterpret_cast<const uint4*>(a.w + (long long)co * a.kpad +
+                                                     (p * 3 + dx) * 32 + gc * 8)
+                   : make_uint4(0u, 0u, 0u, 0u);
+    }
+  } else {
     // k chunk c = 4 tap + g (8 channels) of output channel row -> slot (c, row)
     constexpr int C16 = 27 * 32 / 8;
     for (int q = threadIdx.x; q < MT * 16 * C16; q += 1024) {
@@ -761,6 +787,8 @@ __global__ __launch_bounds__(1024) void conv_slab_kernel(ConvArgs a) {
       (row < 16 ? wsh0 : wsh1)[c * 16 + (row & 15)] =
           *reinterpret_cast<const uint4*>(a.w + (long long)row * a.kpad + c * 8);
     }
+  }
+  {
     if (EPI == CONV_EPI_GELU) {
       for (int q = threadIdx.x; q < 2 * kRowLutS; q += 1024) {
         const unsigned neg = q >= kRowLutS, j = q - neg * kRowLutS;
@@ -822,13 +850,49 @@ __global__ __launch_bounds__(1024) void conv_slab_kernel(ConvArgs a) {
   const int nsteps = (z1 - z0 + 1) / 2;
   const int nslab = 4 * nsteps;
   f32x4 acc[MT][NT];
+  // PX: acc holds the dx = -1 / +1 pair, acc2 the dx = 0 taps, acce the
+  // edge fragment's pair
+  f32x4 acc2[PX ? NT : 1], acce;
+  const f32x4 zero4 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int m = 0; m < MT; ++m)
 #pragma unroll
-    for (int n = 0; n < NT; ++n) acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int n = 0; n < NT; ++n) acc[m][n] = zero4;
+#pragma unroll
+  for (int n = 0; n < (PX ? NT : 1); ++n) acc2[n] = zero4;
+  acce = zero4;
   // write this wave's finished row of plane zw, then clear the sums
   auto epilogue = [&](int zw) __attribute__((always_inline)) {
-    if (rowok && zw < z1) {
+    if (PX && rowok && zw < z1) {
+      const long long orow = ((long long)zw * ny + yy) * nx * ycp;
+      const auto ors = row_rsrc(reinterpret_cast<char*>(a.y) + orow * 4, obytes);
+      // the x-1 parts (rows 8-15, lane groups 2-3) to lane groups 0-1
+      f32x4 mv[NT], me;
+      const int src = (t ^ 32) << 2;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+#pragma unroll
+        for (int n = 0; n < NT; ++n)
+          mv[n][v] = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(acc[0][n][v])));
+        me[v] = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(acce[v])));
+      }
+      // output x0 + 4 r + n: acc2 of tile n, the x+1 part of column x - 1
+      // (tile n - 1; for n = 0 tile 3 of lane r - 1, the edge x0 - 1 in
+      // lane 0) and the x-1 part of column x + 1 (tile n + 1; for n = 3
+      // tile 0 of lane r + 1, the edge x0 + 64 in lane 15)
+      const f32x4 pl = __builtin_bit_cast(
+          f32x4, shift_in_right(__builtin_bit_cast(bf16x8, acc[0][NT - 1]), __builtin_bit_cast(bf16x8, acce)));
+      const f32x4 mr = __builtin_bit_cast(
+          f32x4, shift_in_left(__builtin_bit_cast(bf16x8, mv[0]), __builtin_bit_cast(bf16x8, me)));
+      if (g < 2) {
+#pragma unroll
+        for (int n = 0; n < NT; ++n) {
+          const f32x4 o = acc2[n] + (n == 0 ? pl : acc[0][n - 1]) + (n == NT - 1 ? mr : mv[n + 1]);
+          f32_out(o, bv[0], 4 * g, a.cout, ycp, ors, x0 + 4 * r + n);
+        }
+      }
+    }
+    if (!PX && rowok && zw < z1) {
       const long long orow = ((long long)zw * ny + yy) * nx * ycp;
       char* ybase = reinterpret_cast<char*>(a.y) + orow * esz;
       const auto ors = row_rsrc(ybase, obytes);
@@ -851,7 +915,10 @@ __global__ __launch_bounds__(1024) void conv_slab_kernel(ConvArgs a) {
 #pragma unroll
     for (int m = 0; m < MT; ++m)
 #pragma unroll
-      for (int n = 0; n < NT; ++n) acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+      for (int n = 0; n < NT; ++n) acc[m][n] = zero4;
+#pragma unroll
+    for (int n = 0; n < (PX ? NT : 1); ++n) acc2[n] = zero4;
+    acce = zero4;
   };
   // slab ts is read from sb while slab ts + 1 loads into nxt
   auto slab_step = [&](int ts, const unsigned char* sb, unsigned char* nxt)
@@ -864,7 +931,26 @@ __global__ __launch_bounds__(1024) void conv_slab_kernel(ConvArgs a) {
     if (jw == 1 && q == 0 && ts > 0) epilogue(zz - 1);
     const int zw = zz + jw;
     const int dz = iz - zw;
-    if (rowok && zw < z1 && dz >= -1 && dz <= 1) {
+    if (PX && rowok && zw < z1 && dz >= -1 && dz <= 1) {
+#pragma unroll
+      for (int dy = 0; dy < 3; ++dy) {
+        const unsigned char* rowp = sb + (iw + dy) * (kSlabRowSlots * 16);
+        bf16x8 c[NT], e;
+#pragma unroll
+        for (int n = 0; n < NT; ++n) c[n] = *reinterpret_cast<const bf16x8*>(rowp + loff[n]);
+        e = *reinterpret_cast<const bf16x8*>(rowp + eoff);
+        const int p = (dz + 1) * 3 + dy;
+        const bf16x8 a1 = __builtin_bit_cast(bf16x8, wsh0[((2 * p) * 4 + g) * 16 + r]);
+        const bf16x8 a2 = __builtin_bit_cast(bf16x8, wsh0[((2 * p + 1) * 4 + g) * 16 + r]);
+#pragma unroll
+        for (int n = 0; n < NT; ++n) {
+          acc[0][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, c[n], acc[0][n], 0, 0, 0);
+          acc2[n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, c[n], acc2[n], 0, 0, 0);
+        }
+        acce = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, e, acce, 0, 0, 0);
+      }
+    }
+    if (!PX && rowok && zw < z1 && dz >= -1 && dz <= 1) {
 #pragma unroll
       for (int dy = 0; dy < 3; ++dy) {
         const unsigned char* rowp = sb + (iw + dy) * (kSlabRowSlots * 16);
@@ -1566,8 +1652,24 @@ bool slab_enabled() {
   return on;
 }
 
+// JAXPHYS_AMD_CONV_PAIRX=0 turns the paired-tap slab kernel off (float32
+// layers of at most 8 channels then pad them to 16; for comparison).
+bool pairx_enabled() {
+  static const bool on = [] {
+    const char* e = std::getenv("JAXPHYS_AMD_CONV_PAIRX");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+
 template <int EPI, int MT>
 int slab_launch(const ConvArgs& a, dim3 grid, hipStream_t st) {
+  if constexpr (EPI == CONV_EPI_LINEAR_F32 && MT == 1) {
+    if (a.cout <= 8 && a.ycp <= 8 && pairx_enabled()) {
+      hipLaunchKernelGGL((conv_slab_kernel<EPI, MT, true>), grid, dim3(1024), 0, st, a);
+      return 0;
+    }
+  }
   hipLaunchKernelGGL((conv_slab_kernel<EPI, MT>), grid, dim3(1024), 0, st, a);
   return 0;
 }

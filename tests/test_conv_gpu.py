@@ -78,6 +78,35 @@ def test_conv_fwd_matches_torch(gpu, cin_pad, cin_real, cout, cout_pad, ycp, epi
         assert float(y.double().cpu()[:, cout:].abs().max()) == 0.0
 
 
+@pytest.mark.parametrize("shape,cout,ycp", [
+    ((130, 19, 9), 6, 8),   # three 64-column blocks: both block edges inside the grid
+    ((130, 19, 9), 6, 6),   # channel stride not a multiple of 4 (4-byte stores)
+    ((64, 9, 5), 3, 4),     # x exactly one block; fewer channels
+    ((200, 3, 4), 8, 8),    # eight channels, a partial last block
+])
+def test_conv_fwd_f32_few_channels(gpu, shape, cout, ycp):
+    """Float32 outputs of at most 8 channels (the closure net's last layer)
+    take the slab kernel with two taps stacked per MFMA and the x+1 / x-1
+    partial sums moved across lanes in the epilogue; the sums must equal the
+    float64 convolution to float32 rounding, at block edges too."""
+    g = torch.Generator().manual_seed(cout * 1000 + shape[0])
+    nx, ny, nz = shape
+    x = _rand_bf16((nz, ny, nx, 32), g)
+    w = _rand_bf16((cout, 27, 32), g, 1.0 / math.sqrt(27 * 32))
+    b = torch.randn(cout, generator=g) * 0.1
+    wp = torch.zeros(16, _conv.kpad_of(32), dtype=torch.bfloat16)
+    wp[:cout, :27 * 32] = w.reshape(cout, -1)
+    d = _conv.desc(shape, 32, cout, 16, ycp)
+    y = torch.full((nz * ny * nx, ycp), float("nan"), device=gpu)
+    _conv.fwd(d, x.to(gpu).contiguous(), wp.to(gpu), b.to(gpu), _conv.EPI_LINEAR_F32, y)
+    ref = _ref_conv(x, w, b, 32).reshape(-1, cout)
+    got = y.double().cpu()
+    err = (got[:, :cout] - ref).abs() / (ref.abs().max() + 1e-30)
+    assert float(err.max()) < 1e-5, float(err.max())
+    if ycp > cout:
+        assert float(got[:, cout:].abs().max()) == 0.0
+
+
 def test_conv_wgrad_and_dgrad_match_autograd(gpu):
     g = torch.Generator().manual_seed(5)
     shape = (40, 13, 9)
