@@ -1364,15 +1364,19 @@ __global__ void flip_weights_kernel(const u16* w, int cout, int cout_pad, int ci
 
 // the cells of the box (bx0, by0, bz0) + [0, bnx) x [0, bny) x [0, bnz) of
 // the grid, out in box order (the whole grid: the box (0, 0, 0, nx, ny, nz))
-__global__ void pack_fields_kernel(int nx, int ny, int bx0, int by0, int bz0, int bnx, int bny,
-                                   int bnz, const float* f0, const float* f1,
-                                   const float* f2, const float* f3, const float* f4,
-                                   const float* f5, const float* eps, const float* scale,
-                                   float eshift, float escale, int flags, u16* out) {
-  const long long nb = (long long)bnx * bny * bnz;
-  const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= nb) return;
-  const int lx = (int)(j % bnx), ly = (int)((j / bnx) % bny), lz = (int)(j / ((long long)bnx * bny));
+// one thread per box cell: x across the workgroup, blockIdx.y / .z the
+// box row and plane (no 64-bit index division)
+__global__ __launch_bounds__(128) void pack_fields_kernel(int nx, int ny, int bx0, int by0, int bz0,
+                                                          int bnx, int bny, const float* f0,
+                                                          const float* f1, const float* f2,
+                                                          const float* f3, const float* f4,
+                                                          const float* f5, const float* eps,
+                                                          const float* scale, float eshift,
+                                                          float escale, int flags, u16* out) {
+  const int lx = blockIdx.x * 128 + threadIdx.x;
+  if (lx >= bnx) return;
+  const int ly = blockIdx.y, lz = blockIdx.z;
+  const long long j = ((long long)lz * bny + ly) * bnx + lx;
   const long long i = ((long long)(bz0 + lz) * ny + (by0 + ly)) * nx + (bx0 + lx);
   const float* f[6] = {f0, f1, f2, f3, f4, f5};
   u16 v[8];
@@ -1412,26 +1416,48 @@ __global__ void apply_correction_kernel(int nx, int ny, int nz, float* f0, float
 //       0 without yb) + s (the far-field operator's value; 0 without s),
 //   with s dropped (mask_all == 0) or v = 0 (mask_all == 1) where mask != 0,
 //   fields[c] += gain * scale[c] * v.
-__global__ void apply_closure_kernel(int nx, int ny, int nz, float* f0, float* f1, float* f2,
-                                     float* f3, float* f4, float* f5, const float* s, int ss,
-                                     const float* yb, const float* y0b, int ybs, int bx0,
-                                     int by0, int bz0, int bnx, int bny,
-                                     const unsigned char* mask, int mask_all, float ng,
-                                     const float* scale, float gain, int margin) {
-  const long long n = (long long)nx * ny * nz;
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int x = (int)(i % nx), yy = (int)((i / nx) % ny), z = (int)(i / ((long long)nx * ny));
+// (one thread per cell, x across the workgroup, blockIdx.y / .z the row and
+// plane; VEC: yb, y0b and s with a channel stride of 8, read as two 16-byte
+// loads per cell instead of six 4-byte ones)
+template <bool VEC>
+__global__ __launch_bounds__(128) void apply_closure_kernel(
+    int nx, int ny, int nz, float* f0, float* f1, float* f2, float* f3, float* f4, float* f5,
+    const float* s, int ss, const float* yb, const float* y0b, int ybs, int bx0, int by0, int bz0,
+    int bnx, int bny, const unsigned char* mask, int mask_all, float ng, const float* scale,
+    float gain, int margin) {
+  const int x = blockIdx.x * 128 + threadIdx.x, yy = blockIdx.y, z = blockIdx.z;
   if (x < margin || x >= nx - margin || yy < margin || yy >= ny - margin || z < margin ||
       z >= nz - margin)
     return;
+  const long long i = ((long long)z * ny + yy) * nx + x;
   const bool mk = mask && mask[i];
   const long long jb = yb ? ((long long)(z - bz0) * bny + (yy - by0)) * bnx + (x - bx0) : 0;
+  float yv[8], y0v[8], sv[8];
+  if (VEC) {
+    if (yb) {
+      const float4* p = reinterpret_cast<const float4*>(yb + jb * 8);
+      const float4* q = reinterpret_cast<const float4*>(y0b + jb * 8);
+      const float4 a = p[0], b = p[1], c = q[0], d = q[1];
+      yv[0] = a.x, yv[1] = a.y, yv[2] = a.z, yv[3] = a.w, yv[4] = b.x, yv[5] = b.y;
+      y0v[0] = c.x, y0v[1] = c.y, y0v[2] = c.z, y0v[3] = c.w, y0v[4] = d.x, y0v[5] = d.y;
+    }
+    if (s) {
+      const float4* p = reinterpret_cast<const float4*>(s + i * 8);
+      const float4 a = p[0], b = p[1];
+      sv[0] = a.x, sv[1] = a.y, sv[2] = a.z, sv[3] = a.w, sv[4] = b.x, sv[5] = b.y;
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      if (yb) yv[c] = yb[jb * ybs + c], y0v[c] = y0b[jb * ybs + c];
+      if (s) sv[c] = s[i * ss + c];
+    }
+  }
   float* f[6] = {f0, f1, f2, f3, f4, f5};
 #pragma unroll
   for (int c = 0; c < 6; ++c) {
-    float v = yb ? (yb[jb * ybs + c] - y0b[jb * ybs + c]) * ng : 0.f;
-    if (s) v += (mk && !mask_all) ? 0.f : s[i * ss + c];
+    float v = yb ? (yv[c] - y0v[c]) * ng : 0.f;
+    if (s) v += (mk && !mask_all) ? 0.f : sv[c];
     if (mk && mask_all) v = 0.f;
     f[c][i] += gain * scale[c] * v;
   }
@@ -1460,8 +1486,9 @@ __global__ void patch_add_kernel(int nx, int ny, float* f0, float* f1, float* f2
 // (x, y, z0) and (x, y, z0 + 1): per field and (dx, dy) it loads the four
 // planes z0 - 1 .. z0 + 2 once and feeds both cells, and each (tap, field)
 // weight set (12 floats, contiguous, uniform across the workgroup: scalar
-// loads) serves two cells.  Per field the taps are summed first and scaled
-// once by 1 / scale[ci].
+// loads) serves two cells.  Per field the taps are summed first (fused
+// multiply-adds: one VALU instruction per term instead of a multiply and an
+// add) and scaled once by 1 / scale[ci].
 // (SET: y = the value instead of y += it; the cells of the box (bx0, by0,
 // bz0) + [0, bnx) x [0, bny) x [0, bnz) only, y indexed on the whole grid)
 template <bool SET>
@@ -1509,10 +1536,10 @@ __global__ __launch_bounds__(256) void stencil_f32_kernel(
         const float* wt = w + ((dz * 9 + q) * 6 + ci) * 12;
 #pragma unroll
         for (int co = 0; co < 6; ++co) {
-          pa0[co] += wt[co] * v[dz];
-          pb0[co] += wt[6 + co] * v[dz];
-          pa1[co] += wt[co] * v[dz + 1];
-          pb1[co] += wt[6 + co] * v[dz + 1];
+          pa0[co] = __builtin_fmaf(wt[co], v[dz], pa0[co]);
+          pb0[co] = __builtin_fmaf(wt[6 + co], v[dz], pb0[co]);
+          pa1[co] = __builtin_fmaf(wt[co], v[dz + 1], pa1[co]);
+          pb1[co] = __builtin_fmaf(wt[6 + co], v[dz + 1], pb1[co]);
         }
       }
     }
@@ -1893,9 +1920,9 @@ int conv3d_pack_fields_box(int32_t nx, int32_t ny, int32_t nz, float* const* fie
   if (bnx <= 0 || bny <= 0 || bnz <= 0 || x0 < 0 || y0 < 0 || z0 < 0 || x0 + bnx > nx ||
       y0 + bny > ny || z0 + bnz > nz)
     return fail(-1, "box outside the grid");
-  const long long nb = (long long)bnx * bny * bnz;
-  hipLaunchKernelGGL(pack_fields_kernel, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0,
-                     (hipStream_t)stream, nx, ny, x0, y0, z0, bnx, bny, bnz, fields[0], fields[1],
+  if (bny > 65535 || bnz > 65535) return fail(-1, "box too large");
+  hipLaunchKernelGGL(pack_fields_kernel, dim3((unsigned)((bnx + 127) / 128), bny, bnz), dim3(128),
+                     0, (hipStream_t)stream, nx, ny, x0, y0, z0, bnx, bny, fields[0], fields[1],
                      fields[2], fields[3], fields[4], fields[5], eps, scale, eps_shift, eps_scale,
                      flags, (u16*)out);
   CONV_CHECK(hipGetLastError());
@@ -1921,11 +1948,21 @@ int conv3d_apply_closure(int32_t nx, int32_t ny, int32_t nz, float* const* field
       if (hi[k] > lo[k] && (b[k] > lo[k] || b[k] + b[k + 3] < hi[k]))
         return fail(-1, "net box does not cover the corrected cells");
   }
-  const long long n = (long long)nx * ny * nz;
-  hipLaunchKernelGGL(apply_closure_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                     (hipStream_t)stream, nx, ny, nz, fields[0], fields[1], fields[2], fields[3],
-                     fields[4], fields[5], s, sstride, yb, y0b, ybstride, b[0], b[1], b[2], b[3],
-                     b[4], mask, mask_all, net_gain, scale, gain, margin);
+  if (ny > 65535 || nz > 65535) return fail(-1, "grid too large");
+  auto al16 = [](const void* p) { return ((uintptr_t)p & 15) == 0; };
+  const bool vec = (!yb || (ybstride == 8 && al16(yb) && al16(y0b))) &&
+                   (!s || (sstride == 8 && al16(s)));
+  const dim3 grid((unsigned)((nx + 127) / 128), ny, nz);
+  if (vec)
+    hipLaunchKernelGGL(apply_closure_kernel<true>, grid, dim3(128), 0, (hipStream_t)stream, nx, ny,
+                       nz, fields[0], fields[1], fields[2], fields[3], fields[4], fields[5], s,
+                       sstride, yb, y0b, ybstride, b[0], b[1], b[2], b[3], b[4], mask, mask_all,
+                       net_gain, scale, gain, margin);
+  else
+    hipLaunchKernelGGL(apply_closure_kernel<false>, grid, dim3(128), 0, (hipStream_t)stream, nx,
+                       ny, nz, fields[0], fields[1], fields[2], fields[3], fields[4], fields[5], s,
+                       sstride, yb, y0b, ybstride, b[0], b[1], b[2], b[3], b[4], mask, mask_all,
+                       net_gain, scale, gain, margin);
   CONV_CHECK(hipGetLastError());
   return 0;
 }
