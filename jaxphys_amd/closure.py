@@ -841,7 +841,7 @@ class NearSourcePatch:
 
     def __init__(self, grid: EMGrid3D, dt: float, n_coarse: int, every: int,
                  half: int = PATCH_HALF, radius: int = STENCIL // 2, device=None,
-                 tail: int | None = None):
+                 tail: int | None = None, overlap: bool = True):
         from jaxphys_amd import _native
         if grid._boundary != "cpml":
             raise ConfigurationError("the near-source patch needs a cpml grid")
@@ -904,14 +904,28 @@ class NearSourcePatch:
             # the fine run advanced interval by interval: H^{2m-3/2} restricted
             # into hprev, then one more step to (E^{2m}, H^{2m-1/2}); the
             # kernels restrict, average and correct the coarse box in place
-            # (the steps of restricted_states, on the box only)
+            # (the steps of restricted_states, on the box only).  The local
+            # coarse run of an interval goes on a second stream beside the
+            # fine run: both grids are too small to fill the GPU, and they
+            # meet only in patch_correct (which then waits for the coarse
+            # run, and which the next coarse run waits for: it corrects the
+            # coarse box in place)
+            # (overlap=False: both on the current stream, for comparison)
+            main = torch.cuda.current_stream(dev)
+            aside = torch.cuda.Stream(device=dev) if overlap else main
             done = 0
             for q in range(n_int):
                 m = (q + 1) * every
+                ev = torch.cuda.Event()
+                ev.record(main)
+                aside.wait_event(ev)
+                csol.run(q * every, every, stream=aside)
                 fsol.run(done, 2 * m - 1 - done)
                 _conv.patch_restrict_h(fsol.fields[3:], fshape, org, side, hprev)
                 fsol.run(2 * m - 1, 1)
-                csol.run(q * every, every)
+                ev = torch.cuda.Event()
+                ev.record(aside)
+                main.wait_event(ev)
                 _conv.patch_correct(fsol.fields[:3], fsol.fields[3:], hprev, fshape,
                                     csol.fields, cshape, org, side, corr[q])
                 done = 2 * m

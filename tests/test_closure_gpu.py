@@ -107,6 +107,20 @@ def test_near_source_patch_reproduces_fine_run_in_source_box(gpu):
     assert worst_plain > 0.1
 
 
+def test_near_source_patch_overlapped_equals_sequential(gpu):
+    """The patch's local coarse runs go on a second stream beside the fine
+    runs; the corrections must be those of running everything in order on
+    one stream, bit for bit, for every interval."""
+    prob = CL.ClosureProblem(5, 96, npml_fine=16, device=gpu)
+    g = prob.coarse_grid()
+    dt = prob.dt_coarse()
+    a = CL.NearSourcePatch(g, dt, 120, 10, half=16, device=gpu, tail=2)
+    b = CL.NearSourcePatch(g, dt, 120, 10, half=16, device=gpu, tail=2, overlap=False)
+    assert len(a.corr) == len(b.corr) == 1 and a.corr[0].shape[0] >= 2
+    assert torch.equal(a.corr[0], b.corr[0])
+    assert float(a.corr[0].abs().max()) > 0.0
+
+
 def test_near_source_patch_beats_plain_coarse_far_from_source(gpu):
     """With the near-source patch (and nothing else) the coarse run's field
     error away from the source (more than 3 cells) on held-out seeds is
