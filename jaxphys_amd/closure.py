@@ -372,16 +372,17 @@ class ClosureNet:
                 if m % every or q >= fc.feats.shape[0]:
                     raise ValueError(f"interval start {m} is not a multiple of {every} "
                                      "inside the source table")
-                sel, cflat = self._forcing_in_box(solver, fc, box6)
-                xc[cflat, 7] = (fc.feats[q, sel, 0] * (1.0 / self.force_scale)).to(torch.bfloat16)
+                cflat, vals = self._forcing_in_box(solver, fc, box6)
+                xc[cflat, 7] = vals[q]
         yc = self.forward(xc, shape_c)
         y0 = self.zero_response(solver).view(nb, OUT_STRIDE)
         return yc, y0, box6
 
     def _forcing_in_box(self, solver, fc, box6):
-        """(entries of the forcing table inside the box, their box-order
-        flat indices), cached on the solver."""
-        key = (id(fc), tuple(box6))
+        """(box-order flat indices of the forcing cells inside the box, the
+        forcing channel's bf16 values there for every interval: (intervals,
+        cells)), cached on the solver: one scatter per correction."""
+        key = (id(fc), tuple(box6), self.force_scale)
         cache = getattr(solver, "_closure_fc_box", None)
         if cache is not None and cache[0] == key:
             return cache[1]
@@ -392,8 +393,9 @@ class ClosureNet:
         inside = (x >= 0) & (x < bx) & (y >= 0) & (y < by) & (z >= 0) & (z < bz)
         sel = torch.nonzero(inside).flatten()
         cflat = ((z * by + y) * bx + x)[sel]
-        solver._closure_fc_box = (key, (sel, cflat))
-        return sel, cflat
+        vals = (fc.feats[:, sel, 0] * (1.0 / self.force_scale)).to(torch.bfloat16)
+        solver._closure_fc_box = (key, (cflat, vals))
+        return cflat, vals
 
     def _correct_unfused(self, solver, gain: float = 1.0, m: int | None = None,
                          every: int | None = None, use_net: bool = True,
@@ -912,7 +914,7 @@ class NearSourcePatch:
             # coarse box in place)
             # (overlap=False: both on the current stream, for comparison)
             main = torch.cuda.current_stream(dev)
-            aside = torch.cuda.Stream(device=dev) if overlap else main
+            aside = _side_stream(dev) if overlap else main
             done = 0
             for q in range(n_int):
                 m = (q + 1) * every
@@ -944,6 +946,20 @@ class NearSourcePatch:
                                 (box[2].start, box[1].start, box[0].start),
                                 (box[2].stop - box[2].start, box[1].stop - box[1].start,
                                  box[0].stop - box[0].start), gain)
+
+
+_SIDE_STREAMS: dict = {}
+
+
+def _side_stream(dev) -> "torch.cuda.Stream":
+    """One second stream per device, kept: a stream's first use creates its
+    queue, which costs milliseconds (torch hands out a different pooled
+    stream at every ``torch.cuda.Stream()``)."""
+    key = torch.device(dev).index or 0
+    st = _SIDE_STREAMS.get(key)
+    if st is None:
+        st = _SIDE_STREAMS[key] = torch.cuda.Stream(device=dev)
+    return st
 
 
 def attach_patch(solver, grid: EMGrid3D, n_coarse: int, every: int,
