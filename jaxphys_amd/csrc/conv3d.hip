@@ -1571,6 +1571,85 @@ __global__ __launch_bounds__(256) void stencil_f32_kernel(
   }
 }
 
+// The same operator, four cells (x, y, z0 .. z0 + 3) per thread: the
+// workgroup first stages the weights divided by scale[ci] in LDS (read back
+// as wave-uniform 16-byte broadcasts), so the taps of all six fields sum
+// straight into the outputs (no per-field partial sums), and each loaded
+// input value feeds up to three of the four cells.  The float32 sums are
+// those of the two-cell kernel up to rounding (w / scale[ci] in place of
+// (sum w f) / scale[ci]).
+template <bool SET>
+__global__ __launch_bounds__(256) void stencil4_f32_kernel(
+    int nx, int ny, int nz, const float* __restrict__ f0, const float* __restrict__ f1,
+    const float* __restrict__ f2, const float* __restrict__ f3, const float* __restrict__ f4,
+    const float* __restrict__ f5, const float* __restrict__ eps,
+    const float* __restrict__ scale, const float* __restrict__ w, float* __restrict__ y,
+    int ys, int bx0, int by0, int bz0, int bnx, int bny, int bnz) {
+  __shared__ __attribute__((aligned(16))) float ws[27 * 6 * 12];
+  for (int e = threadIdx.x; e < 27 * 6 * 12; e += 256) ws[e] = w[e] * (1.0f / scale[(e / 12) % 6]);
+  __syncthreads();
+  const int nzq = (bnz + 3) / 4;
+  const long long nq = (long long)bnx * bny * nzq;
+  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nq) return;
+  const int lx = (int)(p % bnx);
+  const int rest = (int)(p / bnx);
+  const int x = bx0 + lx, yy = by0 + rest % bny;
+  const int z0 = bz0 + 4 * (rest / bny);
+  const int ncell = min(4, bz0 + bnz - z0);
+  const long long plane = (long long)nx * ny;
+  const long long i0 = (long long)z0 * plane + (long long)yy * nx + x;
+  const float* f[6] = {f0, f1, f2, f3, f4, f5};
+  float a[4][6], b[4][6];
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+#pragma unroll
+    for (int co = 0; co < 6; ++co) a[c][co] = b[c][co] = 0.f;
+#pragma unroll 1
+  for (int ci = 0; ci < 6; ++ci) {
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+      const int dx = q % 3 - 1, dy = q / 3 - 1;
+      const int sx = x + dx, sy = yy + dy;
+      if (sx < 0 || sx >= nx || sy < 0 || sy >= ny) continue;
+      // planes z0 - 1 .. z0 + 4 of column (sx, sy); zero outside the grid
+      float v[6];
+#pragma unroll
+      for (int pz = 0; pz < 6; ++pz) {
+        const int sz = z0 - 1 + pz;
+        v[pz] = (sz >= 0 && sz < nz) ? f[ci][i0 + (long long)(pz - 1) * plane + dy * nx + dx]
+                                     : 0.f;
+      }
+#pragma unroll
+      for (int dz = 0; dz < 3; ++dz) {
+        const float4* wt = reinterpret_cast<const float4*>(ws + ((dz * 9 + q) * 6 + ci) * 12);
+        const float4 w0 = wt[0], w1 = wt[1], w2 = wt[2];
+        const float wa[6] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y};
+        const float wb[6] = {w1.z, w1.w, w2.x, w2.y, w2.z, w2.w};
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+#pragma unroll
+          for (int co = 0; co < 6; ++co) {
+            a[c][co] = __builtin_fmaf(wa[co], v[c + dz], a[c][co]);
+            b[c][co] = __builtin_fmaf(wb[co], v[c + dz], b[c][co]);
+          }
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    if (c >= ncell) break;
+    const long long i = i0 + c * plane;
+    const float e = eps[i] - 1.0f;
+#pragma unroll
+    for (int co = 0; co < 6; ++co) {
+      const float v = a[c][co] + e * b[c][co];
+      if (SET) y[i * ys + co] = v;
+      else y[i * ys + co] += v;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Near-source patch of the closure: restriction of a 2x-refined local grid
 // to the coarse box around a source, and the coarse box's correction.  The
@@ -1674,6 +1753,16 @@ bool rows_enabled() {
 bool slab_enabled() {
   static const bool on = [] {
     const char* e = std::getenv("JAXPHYS_AMD_CONV_SLAB");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+
+// JAXPHYS_AMD_STENCIL4=0 takes the two-cell far-field stencil kernel
+// instead of the four-cell one (for comparison).
+bool stencil4_enabled() {
+  static const bool on = [] {
+    const char* e = std::getenv("JAXPHYS_AMD_STENCIL4");
     return !(e && e[0] == '0');
   }();
   return on;
@@ -2003,10 +2092,19 @@ int conv3d_stencil_f32(int32_t nx, int32_t ny, int32_t nz, float* const* fields,
     return fail(-1, "bad argument");
   for (int c = 0; c < 6; ++c)
     if (!fields[c]) return fail(-1, "null field");
-  const long long npairs = (long long)nx * ny * ((nz + 1) / 2);
-  hipLaunchKernelGGL(stencil_f32_kernel<false>, dim3((unsigned)((npairs + 255) / 256)), dim3(256),
-                     0, (hipStream_t)stream, nx, ny, nz, fields[0], fields[1], fields[2], fields[3],
-                     fields[4], fields[5], eps, scale, w, y, ystride, 0, 0, 0, nx, ny, nz);
+  if (stencil4_enabled()) {
+    const long long nq = (long long)nx * ny * ((nz + 3) / 4);
+    hipLaunchKernelGGL(stencil4_f32_kernel<false>, dim3((unsigned)((nq + 255) / 256)), dim3(256),
+                       0, (hipStream_t)stream, nx, ny, nz, fields[0], fields[1], fields[2],
+                       fields[3], fields[4], fields[5], eps, scale, w, y, ystride, 0, 0, 0, nx, ny,
+                       nz);
+  } else {
+    const long long npairs = (long long)nx * ny * ((nz + 1) / 2);
+    hipLaunchKernelGGL(stencil_f32_kernel<false>, dim3((unsigned)((npairs + 255) / 256)),
+                       dim3(256), 0, (hipStream_t)stream, nx, ny, nz, fields[0], fields[1],
+                       fields[2], fields[3], fields[4], fields[5], eps, scale, w, y, ystride, 0, 0,
+                       0, nx, ny, nz);
+  }
   CONV_CHECK(hipGetLastError());
   return 0;
 }
@@ -2025,11 +2123,19 @@ int conv3d_stencil_f32_set(int32_t nx, int32_t ny, int32_t nz, float* const* fie
         b[0] + b[3] > nx || b[1] + b[4] > ny || b[2] + b[5] > nz)
       return fail(-1, "box outside the grid");
   }
-  const long long npairs = (long long)b[3] * b[4] * ((b[5] + 1) / 2);
-  hipLaunchKernelGGL(stencil_f32_kernel<true>, dim3((unsigned)((npairs + 255) / 256)), dim3(256),
-                     0, (hipStream_t)stream, nx, ny, nz, fields[0], fields[1], fields[2], fields[3],
-                     fields[4], fields[5], eps, scale, w, y, ystride, b[0], b[1], b[2], b[3], b[4],
-                     b[5]);
+  if (stencil4_enabled()) {
+    const long long nq = (long long)b[3] * b[4] * ((b[5] + 3) / 4);
+    hipLaunchKernelGGL(stencil4_f32_kernel<true>, dim3((unsigned)((nq + 255) / 256)), dim3(256),
+                       0, (hipStream_t)stream, nx, ny, nz, fields[0], fields[1], fields[2],
+                       fields[3], fields[4], fields[5], eps, scale, w, y, ystride, b[0], b[1], b[2],
+                       b[3], b[4], b[5]);
+  } else {
+    const long long npairs = (long long)b[3] * b[4] * ((b[5] + 1) / 2);
+    hipLaunchKernelGGL(stencil_f32_kernel<true>, dim3((unsigned)((npairs + 255) / 256)),
+                       dim3(256), 0, (hipStream_t)stream, nx, ny, nz, fields[0], fields[1],
+                       fields[2], fields[3], fields[4], fields[5], eps, scale, w, y, ystride, b[0],
+                       b[1], b[2], b[3], b[4], b[5]);
+  }
   CONV_CHECK(hipGetLastError());
   return 0;
 }
