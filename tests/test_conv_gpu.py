@@ -253,6 +253,38 @@ def test_far_field_stencil_matches_oracle(gpu):
     np.testing.assert_array_equal(got[:, 6:], y0[:, 6:])
 
 
+def test_far_field_stencil_set_on_a_box_matches_oracle(gpu):
+    """conv3d_stencil_f32_set on a box (odd sizes and offsets, so that the
+    four-cell groups along z end inside the box) writes the operator's value
+    on the box's cells, equal to the float64 oracle to float32 rounding, and
+    leaves every other entry untouched."""
+    from jaxphys_amd import _conv
+    from oracle import closure_ops as O
+
+    rng = np.random.default_rng(12)
+    nx, ny, nz = 23, 17, 14
+    fields_np = [rng.standard_normal((nz, ny, nx)).astype(np.float32) for _ in range(6)]
+    eps_np = (1.0 + 5.0 * rng.random((nz, ny, nx))).astype(np.float32)
+    scale_np = (0.5 + rng.random(6)).astype(np.float32)
+    w_np = (0.1 * rng.standard_normal((27, 6, 2, 6))).astype(np.float32)
+    box = (3, 2, 1, 15, 13, 11)  # x0, y0, z0, bnx, bny, bnz
+    y = torch.full((nz * ny * nx, 8), float("nan"), device=gpu)
+    _conv.stencil_f32_set((nx, ny, nz), [torch.from_numpy(f).to(gpu) for f in fields_np],
+                          torch.from_numpy(eps_np).to(gpu), torch.from_numpy(scale_np).to(gpu),
+                          torch.from_numpy(w_np).to(gpu), y, 8, box)
+    torch.cuda.synchronize()
+    ref = O.stencil(fields_np, eps_np, scale_np, w_np).reshape(nz, ny, nx, 6)
+    got = y.cpu().numpy().reshape(nz, ny, nx, 8)
+    x0, y0, z0, bx, by, bz = box
+    sl = (slice(z0, z0 + bz), slice(y0, y0 + by), slice(x0, x0 + bx))
+    np.testing.assert_allclose(got[sl][..., :6], ref[sl], rtol=0,
+                               atol=2e-5 * np.abs(ref[sl]).max())
+    outside = np.ones((nz, ny, nx), bool)
+    outside[sl] = False
+    assert np.isnan(got[outside]).all()
+    assert np.isnan(got[sl][..., 6:]).all()
+
+
 def test_gelu_table_matches_direct_gelu_on_every_bf16_value(gpu):
     """The row kernel (8 input channels) reads the GELU of a bf16
     pre-activation from its LDS table, or applies z / -0 beyond |z| = 8; the
